@@ -1,0 +1,254 @@
+// C ABI entry points: context, scene and ray-batch queries.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bgpu_internal.h"
+
+namespace bgpu {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& m) { g_err = m; }
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? BGPU_E_NOMEM : BGPU_E_HIP;
+}
+
+DevBuf::~DevBuf() { release(); }
+
+int DevBuf::alloc(Ctx* c, size_t n) {
+  release();
+  if (n == 0) n = 16;
+  size_t prev = c->used.fetch_add(n);
+  if (c->budget && prev + n > c->budget) {
+    c->used.fetch_sub(n);
+    return fail(BGPU_E_BUDGET, "HBM budget exceeded: need " + std::to_string(prev + n) +
+                                   " bytes, budget " + std::to_string(c->budget));
+  }
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    c->used.fetch_sub(n);
+    ptr = nullptr;
+    return hip_fail(e, "hipMalloc");
+  }
+  ctx = c;
+  bytes = n;
+  return BGPU_OK;
+}
+
+int DevBuf::upload(Ctx* c, const void* src, size_t n, hipStream_t s) {
+  int rc = alloc(c, n);
+  if (rc) return rc;
+  if (n) BGPU_HIP(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, s));
+  return BGPU_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) {
+    (void)hipFree(ptr);
+    ctx->used.fetch_sub(bytes);
+  }
+  ptr = nullptr;
+  bytes = 0;
+}
+
+size_t Scene::device_bytes() const {
+  size_t b = 0;
+  const DevBuf* bufs[] = {&nodes, &leaf, &shapes, &tri_idx, &verts, &normals, &uvs, &tri_mat,
+                          &tri_flags, &shape_x, &materials, &lights, &light_maps, &light_dist,
+                          &filter_tbl};
+  for (const DevBuf* d : bufs) b += d->bytes;
+  return b;
+}
+
+int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc);
+
+}  // namespace bgpu
+
+using namespace bgpu;
+
+// ---- entry points ----------------------------------------------------------
+
+extern "C" {
+
+const char* bgpu_last_error(void) { return bgpu::g_err.c_str(); }
+int bgpu_version(void) { return BGPU_ABI_VERSION; }
+
+int bgpu_init(int device, size_t hbm_budget_bytes, bgpu_ctx** out) {
+  if (!out) return fail(BGPU_E_INVALID, "bgpu_init: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(BGPU_E_NODEV, "bgpu_init: no HIP device available");
+  if (device < 0 || device >= ndev) return fail(BGPU_E_INVALID, "bgpu_init: bad device index");
+  BGPU_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  BGPU_HIP(hipGetDeviceProperties(&prop, device));
+  std::unique_ptr<bgpu_ctx> c(new bgpu_ctx());
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  size_t free_b = 0, total_b = 0;
+  BGPU_HIP(hipMemGetInfo(&free_b, &total_b));
+  c->budget = hbm_budget_bytes ? std::min(hbm_budget_bytes, total_b) : (size_t)(0.9 * (double)free_b);
+  BGPU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  BGPU_HIP(hipEventCreate(&c->ev0));
+  BGPU_HIP(hipEventCreate(&c->ev1));
+  *out = c.release();
+  return BGPU_OK;
+}
+
+int bgpu_shutdown(bgpu_ctx* ctx) {
+  if (!ctx) return fail(BGPU_E_INVALID, "bgpu_shutdown: NULL context");
+  (void)hipSetDevice(ctx->device);
+  delete ctx;
+  return BGPU_OK;
+}
+
+int bgpu_scene_create(bgpu_ctx* ctx, const bgpu_scene_desc* desc, bgpu_scene** out) {
+  if (!ctx || !desc || !out) return fail(BGPU_E_INVALID, "bgpu_scene_create: NULL argument");
+  *out = nullptr;
+  BGPU_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<bgpu_scene> sc(new bgpu_scene());
+  sc->ctx = ctx;
+  int rc = flatten_and_upload(ctx, desc, sc.get());
+  if (rc) return rc;
+  *out = sc.release();
+  return BGPU_OK;
+}
+
+int bgpu_scene_destroy(bgpu_scene* scene) {
+  if (!scene) return fail(BGPU_E_INVALID, "bgpu_scene_destroy: NULL scene");
+  (void)hipSetDevice(scene->ctx->device);
+  if (--scene->refs == 0) delete scene;
+  return BGPU_OK;
+}
+
+int bgpu_scene_bytes(const bgpu_scene* scene, size_t* bytes) {
+  if (!scene || !bytes) return fail(BGPU_E_INVALID, "bgpu_scene_bytes: NULL argument");
+  *bytes = scene->device_bytes();
+  return BGPU_OK;
+}
+
+static int check_rays(const bgpu_rays* r) {
+  if (!r || !r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->tmin || !r->tmax)
+    return fail(BGPU_E_INVALID, "ray batch has a NULL array");
+  return BGPU_OK;
+}
+
+// Stages host rays into the context's buffer; returns device views.
+static int stage_rays(Ctx* c, const bgpu_rays* r, size_t n, DevRays* dr) {
+  size_t need = 8 * n * sizeof(float);
+  if (c->stage_rays.bytes < need) {
+    int rc = c->stage_rays.alloc(c, need);
+    if (rc) return rc;
+  }
+  float* base = c->stage_rays.as<float>();
+  const float* src[8] = {r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+  for (int k = 0; k < 8; ++k)
+    BGPU_HIP(hipMemcpyAsync(base + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  *dr = DevRays{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n, base + 6 * n, base + 7 * n};
+  return BGPU_OK;
+}
+
+static int stage_out(Ctx* c, size_t need) {
+  if (c->stage_hits.bytes < need) return c->stage_hits.alloc(c, need);
+  return BGPU_OK;
+}
+
+int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, bgpu_hits* hits, int flags) {
+  if (!scene || !hits || !hits->t || !hits->b1 || !hits->b2 || !hits->prim)
+    return fail(BGPU_E_INVALID, "bgpu_intersect_closest: NULL argument");
+  if (int rc = check_rays(rays)) return rc;
+  if (n > (size_t)0x7fffffff) return fail(BGPU_E_INVALID, "ray batch larger than 2^31-1");
+  if (n == 0) return BGPU_OK;
+  Ctx* c = scene->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  BGPU_HIP(hipSetDevice(c->device));
+  DevRays dr;
+  DevHits dh;
+  bool dev = flags & BGPU_DEVICE_PTRS;
+  if (dev) {
+    dr = DevRays{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    dh = DevHits{hits->t, hits->b1, hits->b2, hits->prim};
+  } else {
+    if (int rc = stage_rays(c, rays, n, &dr)) return rc;
+    if (int rc = stage_out(c, 16 * n)) return rc;
+    float* o = c->stage_hits.as<float>();
+    dh = DevHits{o, o + n, o + 2 * n, reinterpret_cast<int32_t*>(o + 3 * n)};
+  }
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
+  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, nullptr));
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
+  if (!dev) {
+    BGPU_HIP(hipMemcpyAsync(hits->t, dh.t, n * 4, hipMemcpyDeviceToHost, c->stream));
+    BGPU_HIP(hipMemcpyAsync(hits->b1, dh.b1, n * 4, hipMemcpyDeviceToHost, c->stream));
+    BGPU_HIP(hipMemcpyAsync(hits->b2, dh.b2, n * 4, hipMemcpyDeviceToHost, c->stream));
+    BGPU_HIP(hipMemcpyAsync(hits->prim, dh.prim, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  BGPU_HIP(hipStreamSynchronize(c->stream));
+  if (flags & BGPU_TIMED) {
+    float ms = 0.0f;
+    BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+  }
+  return BGPU_OK;
+}
+
+int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n, uint8_t* occluded, int flags) {
+  if (!scene || !occluded) return fail(BGPU_E_INVALID, "bgpu_intersect_any: NULL argument");
+  if (int rc = check_rays(rays)) return rc;
+  if (n > (size_t)0x7fffffff) return fail(BGPU_E_INVALID, "ray batch larger than 2^31-1");
+  if (n == 0) return BGPU_OK;
+  Ctx* c = scene->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  BGPU_HIP(hipSetDevice(c->device));
+  DevRays dr;
+  uint8_t* docc;
+  bool dev = flags & BGPU_DEVICE_PTRS;
+  if (dev) {
+    dr = DevRays{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    docc = occluded;
+  } else {
+    if (int rc = stage_rays(c, rays, n, &dr)) return rc;
+    if (int rc = stage_out(c, n)) return rc;
+    docc = c->stage_hits.as<uint8_t>();
+  }
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
+  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, nullptr));
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
+  if (!dev) BGPU_HIP(hipMemcpyAsync(occluded, docc, n, hipMemcpyDeviceToHost, c->stream));
+  BGPU_HIP(hipStreamSynchronize(c->stream));
+  if (flags & BGPU_TIMED) {
+    float ms = 0.0f;
+    BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+  }
+  return BGPU_OK;
+}
+
+int bgpu_last_kernel_ms(bgpu_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return fail(BGPU_E_INVALID, "bgpu_last_kernel_ms: NULL argument");
+  *ms = ctx->last_ms;
+  return BGPU_OK;
+}
+
+int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out) {
+  if (!ctx || !out) return fail(BGPU_E_INVALID, "bgpu_stats_get: NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->stats_mu);
+  *out = ctx->stats;
+  return BGPU_OK;
+}
+
+int bgpu_stats_reset(bgpu_ctx* ctx) {
+  if (!ctx) return fail(BGPU_E_INVALID, "bgpu_stats_reset: NULL context");
+  std::lock_guard<std::mutex> lk(ctx->stats_mu);
+  ctx->stats = bgpu_stats{};
+  return BGPU_OK;
+}
+
+}  // extern "C"

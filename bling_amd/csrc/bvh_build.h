@@ -1,0 +1,53 @@
+// Host binned-SAH BVH builder (replaces the SAH kd-tree of
+// src/lib/Graphics/Bling/Primitive/KdTree.hs:107-203; the result of a query
+// does not depend on the structure, only the work does).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace bgpu {
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+// Binary BVH node as built on the host.  Children are inner-node indices
+// (>= 0) or leaves encoded as ~((first << kLeafCountBits) | count).
+struct BNode {
+  Box child[2];
+  int32_t link[2];
+};
+
+constexpr int kLeafCountBits = 4;
+constexpr int kMaxLeaf = (1 << kLeafCountBits) - 1;  // 15
+constexpr int kMaxDepth = 60;                         // bounds the traversal stack
+
+struct BuildParams {
+  int bins = 12;          // binned SAH, 12 bins
+  int max_leaf = 4;       // SAH leaf threshold
+  float cost_node = 1.0f; // traversal cost relative to one primitive test
+  int threads = 0;        // 0 = hardware concurrency
+};
+
+struct BuildStats {
+  int nodes = 0, leaves = 0, max_depth = 0, max_leaf_size = 0;
+  double sah = 0.0, build_ms = 0.0;
+};
+
+// Builds over per-primitive bounds; `order` receives primitive ids in leaf
+// order.  Node boxes are padded outward so that the float slab test on the
+// device is conservative for every primitive hit.
+void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
+                std::vector<BNode>* nodes, std::vector<int32_t>* order, BuildStats* stats);
+
+// Collapses a BVH2 into a BVH of width w (4 or 8) by pulling grandchildren up
+// in SAH order; same leaf encoding.
+struct WNode {
+  std::vector<Box> child;
+  std::vector<int32_t> link;
+};
+void collapse_bvh(const std::vector<BNode>& bin, int width, std::vector<WNode>* out);
+
+Box pad_box(const Box& b);
+
+}  // namespace bgpu

@@ -1,0 +1,57 @@
+// Device-side scene layout and the traversal kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dev_geom.h"
+
+namespace bgpu {
+
+// BVH2 node, 64 B (one fetch tests both children):
+//   a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   b = (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x)
+//   c = (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z)
+//   d = (link0, link1, 0, 0)  link >= 0: inner node; link < 0: leaf ~((first<<4)|count)
+// Leaf records, 48 B each, in leaf order:
+//   r0 = (p1.xyz, prim id bits), r1 = (e1.xyz, kind bits), r2 = (e2.xyz, 0)
+//   kind >= 0: triangle (e1 = p2-p1, e2 = p3-p1); kind < 0: shape index ~kind.
+//
+// Triangles and analytic shapes live in two BVHs sharing these arrays: the
+// triangle tree is rooted at node tri_root, the shape tree at shape_root (-1
+// when absent).  Shape-tree boxes are tested inflated by
+// shape_pad_k * (distance from the ray origin to the box + box diagonal):
+// the reference's binary32 sphere/cylinder quadratic (Shape.hs:113-187)
+// evaluates c = |o|^2 - r^2 with cancellation and so reports hits up to about
+// 3*sqrt(2^-24)*|o| outside the true surface; a box padded only by rounding
+// would cull those hits, which the reference does report.
+struct DevScene {
+  const float4* nodes;
+  const float4* leaf;
+  const DShape* shapes;
+  float root_lo[3], root_hi[3];
+  int32_t tri_root;    // -1 if no triangles
+  int32_t shape_root;  // -1 if no shapes
+  float shape_pad_k;
+  int32_t num_nodes;
+};
+
+struct DevRays {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+struct DevHits {
+  float *t, *b1, *b2;
+  int32_t* prim;
+};
+
+// Launch-shape knobs chosen by the autotuner.
+struct TraceConfig {
+  int block = 128;
+};
+
+hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,
+                          const TraceConfig& cfg, hipStream_t s, uint32_t* counters);
+hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
+                      const TraceConfig& cfg, hipStream_t s, uint32_t* counters);
+
+}  // namespace bgpu

@@ -1,0 +1,265 @@
+// Scene flattening: the description (world-space vertices after the host's
+// transPoint, shapes with their matrices, materials, lights, camera, film) is
+// validated like mkTriangleMesh (Primitive/TriangleMesh.hs:127-136), bounded
+// like mkKdTree (Primitive/KdTree.hs:107-112), organised into a BVH and copied
+// once into device-resident arrays.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bgpu_internal.h"
+
+namespace bgpu {
+
+Ctx::~Ctx() {
+  if (sched) destroy_scheduler(sched);
+  sched = nullptr;
+  stage_rays.release();
+  stage_hits.release();
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+inline float hmax_h(float x, float y) { return (x <= y) ? y : x; }
+inline float hmin_h(float x, float y) { return (x <= y) ? x : y; }
+
+inline void xp_host(const float* m, const float* p, float* o) {  // Transform.hs:341-350
+  float xp = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+  float yp = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
+  float zp = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
+  float wp = m[12] * p[0] + m[13] * p[1] + m[14] * p[2] + m[15];
+  if (wp == 1.0f) {
+    o[0] = xp; o[1] = yp; o[2] = zp;
+  } else {
+    o[0] = xp / wp; o[1] = yp / wp; o[2] = zp / wp;
+  }
+}
+
+Box empty_box() {
+  Box b;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = INFINITY;
+    b.hi[k] = -INFINITY;
+  }
+  return b;
+}
+void extend_p(Box& b, const float* p) {  // extendAABBP (AABB.hs:43-46)
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = hmin_h(b.lo[k], p[k]);
+    b.hi[k] = hmax_h(b.hi[k], p[k]);
+  }
+}
+void extend_b(Box& a, const Box& b) {  // extendAABB (AABB.hs:34-41)
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = hmin_h(a.lo[k], b.lo[k]);
+    a.hi[k] = hmax_h(a.hi[k], b.hi[k]);
+  }
+}
+
+// transBox o2w (objectBounds s)  (Shape.hs:286-311, Transform.hs:375-386)
+Box shape_world_bounds(const bgpu_shape& s) {
+  float lo[3], hi[3];
+  const float* p = s.params;
+  switch (s.type) {
+    case BGPU_SHAPE_BOX:
+      lo[0] = p[0]; lo[1] = p[1]; lo[2] = p[2]; hi[0] = p[3]; hi[1] = p[4]; hi[2] = p[5];
+      break;
+    case BGPU_SHAPE_CYLINDER:
+      lo[0] = -p[0]; lo[1] = -p[0]; lo[2] = p[1]; hi[0] = p[0]; hi[1] = p[0]; hi[2] = p[2];
+      break;
+    case BGPU_SHAPE_DISK:
+      lo[0] = -p[1]; lo[1] = -p[1]; lo[2] = p[0]; hi[0] = p[1]; hi[1] = p[1]; hi[2] = p[0];
+      break;
+    case BGPU_SHAPE_QUAD:
+      lo[0] = -p[0]; lo[1] = -p[1]; lo[2] = 0.0f; hi[0] = p[0]; hi[1] = p[1]; hi[2] = 0.0f;
+      break;
+    default:
+      lo[0] = lo[1] = lo[2] = -p[0];
+      hi[0] = hi[1] = hi[2] = p[0];
+      break;
+  }
+  float c[8][3] = {{lo[0], lo[1], lo[2]}, {lo[0], lo[1], hi[2]}, {lo[0], hi[1], lo[2]},
+                   {lo[0], hi[1], hi[2]}, {hi[0], lo[1], lo[2]}, {hi[0], lo[1], hi[2]},
+                   {hi[0], hi[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+  Box b = empty_box();
+  for (int i = 0; i < 8; ++i) {
+    float o[3];
+    xp_host(s.o2w, c[i], o);
+    extend_p(b, o);
+  }
+  return b;
+}
+
+inline uint32_t fbits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+inline float bitsf(int32_t i) {
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
+}
+
+}  // namespace
+
+int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc);
+
+int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
+  // -- validation (TriangleMesh.hs:127-136 raise `error`; here: BGPU_E_INVALID)
+  if (d->num_triangles && (!d->indices || !d->vertices))
+    return fail(BGPU_E_INVALID, "triangles given without vertices/indices");
+  if (d->num_shapes && !d->shapes) return fail(BGPU_E_INVALID, "shapes pointer is NULL");
+  if ((uint64_t)d->num_triangles + d->num_shapes > (1u << 27) - 1)
+    return fail(BGPU_E_INVALID, "too many primitives (limit 2^27-1)");
+  for (uint64_t i = 0; i < 3ull * d->num_triangles; ++i) {
+    int v = d->indices[i];
+    if (v < 0) return fail(BGPU_E_INVALID, "mkTriangleMesh: contains negative indices");
+    if ((uint32_t)v >= d->num_vertices)
+      return fail(BGPU_E_INVALID, "mkTriangleMesh: contains out of bounds indices");
+  }
+  uint32_t nmesh = d->num_meshes ? d->num_meshes : 1;
+  if (d->tri_mesh)
+    for (uint32_t i = 0; i < d->num_triangles; ++i)
+      if (d->tri_mesh[i] < 0 || (uint32_t)d->tri_mesh[i] >= nmesh)
+        return fail(BGPU_E_INVALID, "tri_mesh index out of range");
+  for (uint32_t i = 0; i < d->num_shapes; ++i) {
+    const bgpu_shape& s = d->shapes[i];
+    if (s.type < BGPU_SHAPE_BOX || s.type > BGPU_SHAPE_SPHERE)
+      return fail(BGPU_E_INVALID, "unknown shape type");
+  }
+
+  sc->ntri = d->num_triangles;
+  sc->nshape = d->num_shapes;
+  const uint32_t np = d->num_triangles + d->num_shapes;
+
+  // -- primitive bounds and the root box fold (KdTree.hs:111)
+  std::vector<Box> pb(np);
+  Box root = empty_box();
+  for (uint32_t t = 0; t < d->num_triangles; ++t) {
+    Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
+    for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * (size_t)t + k]);
+    pb[t] = b;
+    extend_b(root, b);
+  }
+  for (uint32_t s = 0; s < d->num_shapes; ++s) {
+    pb[d->num_triangles + s] = shape_world_bounds(d->shapes[s]);
+    extend_b(root, pb[d->num_triangles + s]);
+  }
+
+  // -- BVHs: one over the triangles, one over the shapes, in one node array
+  std::vector<BNode> nodes;
+  std::vector<int32_t> order;
+  BuildParams bp;
+  int32_t tri_root = -1, shape_root = -1;
+  float aniso = 1.0f;
+  if (d->num_triangles) {
+    std::vector<Box> tb(pb.begin(), pb.begin() + d->num_triangles);
+    build_bvh2(tb, bp, &nodes, &order, &sc->build);
+    tri_root = 0;
+  }
+  if (d->num_shapes) {
+    std::vector<Box> sb(pb.begin() + d->num_triangles, pb.end());
+    std::vector<BNode> sn;
+    std::vector<int32_t> so;
+    BuildStats sst;
+    build_bvh2(sb, bp, &sn, &so, &sst);
+    int32_t noff = (int32_t)nodes.size(), loff = (int32_t)order.size();
+    for (BNode& n : sn)
+      for (int s = 0; s < 2; ++s) {
+        if (n.link[s] >= 0) {
+          n.link[s] += noff;
+        } else {
+          int enc = ~n.link[s];
+          n.link[s] = ~((((enc >> kLeafCountBits) + loff) << kLeafCountBits) | (enc & kMaxLeaf));
+        }
+      }
+    shape_root = noff;
+    nodes.insert(nodes.end(), sn.begin(), sn.end());
+    for (int32_t id : so) order.push_back(id + (int32_t)d->num_triangles);
+    sc->build.nodes += sst.nodes;
+    sc->build.leaves += sst.leaves;
+    // anisotropy of each o2w's linear part scales the false-hit radius
+    for (uint32_t s = 0; s < d->num_shapes; ++s) {
+      const float* m = d->shapes[s].o2w;
+      float cn[3];
+      for (int c = 0; c < 3; ++c)
+        cn[c] = std::sqrt(m[c] * m[c] + m[4 + c] * m[4 + c] + m[8 + c] * m[8 + c]);
+      float mx = std::max(cn[0], std::max(cn[1], cn[2]));
+      float mn = std::min(cn[0], std::min(cn[1], cn[2]));
+      float a = (mn > 0.0f && std::isfinite(mx / mn)) ? mx / mn : 1000.0f;
+      aniso = std::max(aniso, std::min(a, 1000.0f));
+    }
+  }
+
+  std::vector<float> hn(16 * nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const BNode& n = nodes[i];
+    float* o = &hn[16 * i];
+    o[0] = n.child[0].lo[0]; o[1] = n.child[0].hi[0]; o[2] = n.child[0].lo[1]; o[3] = n.child[0].hi[1];
+    o[4] = n.child[0].lo[2]; o[5] = n.child[0].hi[2]; o[6] = n.child[1].lo[0]; o[7] = n.child[1].hi[0];
+    o[8] = n.child[1].lo[1]; o[9] = n.child[1].hi[1]; o[10] = n.child[1].lo[2]; o[11] = n.child[1].hi[2];
+    o[12] = bitsf(n.link[0]); o[13] = bitsf(n.link[1]); o[14] = 0.0f; o[15] = 0.0f;
+  }
+  std::vector<float> hl(12 * std::max<size_t>(order.size(), 1), 0.0f);
+  for (size_t k = 0; k < order.size(); ++k) {
+    int id = order[k];
+    float* o = &hl[12 * k];
+    if ((uint32_t)id < d->num_triangles) {
+      const float* p1 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id];
+      const float* p2 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 1];
+      const float* p3 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 2];
+      o[0] = p1[0]; o[1] = p1[1]; o[2] = p1[2]; o[3] = bitsf(id);
+      // e1 = p2 - p1, e2 = p3 - p1: the same IEEE subtractions as TriangleMesh.hs:249
+      o[4] = p2[0] - p1[0]; o[5] = p2[1] - p1[1]; o[6] = p2[2] - p1[2]; o[7] = bitsf(0);
+      o[8] = p3[0] - p1[0]; o[9] = p3[1] - p1[1]; o[10] = p3[2] - p1[2]; o[11] = 0.0f;
+    } else {
+      int si = id - (int)d->num_triangles;
+      o[3] = bitsf(id);
+      o[7] = bitsf(~si);
+    }
+  }
+  std::vector<DShape> hs(std::max<uint32_t>(d->num_shapes, 1));
+  for (uint32_t s = 0; s < d->num_shapes; ++s) {
+    const bgpu_shape& in = d->shapes[s];
+    DShape& o = hs[s];
+    std::memset(&o, 0, sizeof(o));
+    o.type = in.type;
+    o.material = in.material;
+    o.light = -1;
+    std::memcpy(o.p, in.params, sizeof(o.p));
+    std::memcpy(o.w2o, in.w2o, sizeof(o.w2o));
+  }
+
+  hipStream_t st = ctx->stream;
+  if (int rc = sc->nodes.upload(ctx, hn.data(), hn.size() * 4, st)) return rc;
+  if (int rc = sc->leaf.upload(ctx, hl.data(), hl.size() * 4, st)) return rc;
+  if (int rc = build_pt_data(ctx, d, sc)) return rc;
+  // shape light indices are assigned by build_pt_data (light list order)
+  for (uint32_t s = 0; s < d->num_shapes; ++s) hs[s].light = sc->pt.shape_light_host.empty() ? -1 : sc->pt.shape_light_host[s];
+  if (int rc = sc->shapes.upload(ctx, hs.data(), hs.size() * sizeof(DShape), st)) return rc;
+  BGPU_HIP(hipStreamSynchronize(st));
+
+  DevScene& ds = sc->dev;
+  ds.nodes = sc->nodes.as<float4>();
+  ds.leaf = sc->leaf.as<float4>();
+  ds.shapes = sc->shapes.as<DShape>();
+  for (int k = 0; k < 3; ++k) {
+    ds.root_lo[k] = root.lo[k];
+    ds.root_hi[k] = root.hi[k];
+  }
+  ds.num_nodes = (int32_t)nodes.size();
+  ds.tri_root = tri_root;
+  ds.shape_root = shape_root;
+  // 3*sqrt(2^-24) = 7.3e-4 bounds the quadratic's false-hit radius per unit
+  // of origin distance; 2e-3 leaves a margin for the transformed origin.
+  ds.shape_pad_k = 2e-3f * aniso;
+  sc->film = d->film;
+  sc->camera = d->camera;
+  return BGPU_OK;
+}
+
+}  // namespace bgpu

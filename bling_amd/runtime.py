@@ -1,0 +1,236 @@
+"""Python host mirror of bling's accelerator and renderer interfaces over the
+C ABI (include/bgpu.h).
+
+  Accelerator  -- `GpuScene.intersect` / `GpuScene.intersects` batch the
+                  `Primitive{intersect, intersects}` closures of the kd-tree
+                  primitive (src/lib/Graphics/Bling/Primitive/KdTree.hs:236-246);
+                  a miss is prim id -1, as `Nothing`.
+  Renderer     -- `GpuRenderer.render(job, report)` follows
+                  `class Renderer a where render :: a -> RenderJob ->
+                  ProgressReporter -> IO ()` (Rendering.hs:173-174): it calls
+                  report(PassDone(pass, image, 1)) after every pass and stops
+                  when report returns False (Rendering.hs:223-236).
+
+Everything runs on the GPU through libbgpu.so; there is no CPU fallback.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+
+
+def _ck(code):
+    return abi.check(code)
+
+
+class Context:
+    def __init__(self, device=0, hbm_budget=0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        _ck(self.lib.bgpu_init(device, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _ck(self.lib.bgpu_shutdown(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_kernel_ms(self):
+        v = C.c_double()
+        _ck(self.lib.bgpu_last_kernel_ms(self.h, C.byref(v)))
+        return v.value
+
+    def stats(self):
+        s = abi.Stats()
+        _ck(self.lib.bgpu_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def reset_stats(self):
+        _ck(self.lib.bgpu_stats_reset(self.h))
+
+    def scene(self, flat):
+        return GpuScene(self, flat)
+
+
+def _rays_struct(arrs):
+    r = abi.Rays()
+    for name, a in zip(("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax"), arrs):
+        setattr(r, name, a)
+    return r
+
+
+def _host_f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class GpuScene:
+    def __init__(self, ctx, flat):
+        self.ctx = ctx
+        self.flat = flat
+        h = C.c_void_p()
+        _ck(ctx.lib.bgpu_scene_create(ctx.h, C.byref(flat.desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _ck(self.ctx.lib.bgpu_scene_destroy(self.h))
+            self.h = None
+
+    def device_bytes(self):
+        v = C.c_size_t()
+        _ck(self.ctx.lib.bgpu_scene_bytes(self.h, C.byref(v)))
+        return v.value
+
+    # --- host arrays ---------------------------------------------------
+    def intersect(self, rays):
+        """rays: (n, 8) float32 [ox oy oz dx dy dz tmin tmax]; returns
+        (t, b1, b2, prim) numpy arrays."""
+        rays = _host_f32(rays)
+        n = len(rays)
+        cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+        t = np.empty(n, np.float32)
+        b1 = np.empty(n, np.float32)
+        b2 = np.empty(n, np.float32)
+        prim = np.empty(n, np.int32)
+        r = _rays_struct([c.ctypes.data for c in cols])
+        h = abi.Hits(t.ctypes.data, b1.ctypes.data, b2.ctypes.data, prim.ctypes.data)
+        _ck(self.ctx.lib.bgpu_intersect_closest(self.h, C.byref(r), n, C.byref(h), 0))
+        return t, b1, b2, prim
+
+    def intersects(self, rays):
+        rays = _host_f32(rays)
+        n = len(rays)
+        cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+        occ = np.empty(n, np.uint8)
+        r = _rays_struct([c.ctypes.data for c in cols])
+        _ck(self.ctx.lib.bgpu_intersect_any(self.h, C.byref(r), n, occ.ctypes.data, 0))
+        return occ.astype(bool)
+
+    # --- device-resident arrays (integer device addresses) -------------
+    def intersect_dev(self, ray_ptrs, n, hit_ptrs, timed=False):
+        r = _rays_struct(ray_ptrs)
+        h = abi.Hits(*hit_ptrs)
+        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0)
+        _ck(self.ctx.lib.bgpu_intersect_closest(self.h, C.byref(r), n, C.byref(h), flags))
+        return self.ctx.last_kernel_ms() if timed else None
+
+    def intersects_dev(self, ray_ptrs, n, occ_ptr, timed=False):
+        r = _rays_struct(ray_ptrs)
+        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0)
+        _ck(self.ctx.lib.bgpu_intersect_any(self.h, C.byref(r), n, occ_ptr, flags))
+        return self.ctx.last_kernel_ms() if timed else None
+
+
+@dataclass
+class JobSpec:
+    """A render job: `sampler { stratified nu nv }` and
+    `integrator { path maxDepth md sampleDepth sd }`, `passes` passes."""
+    spp_nu: int = 2
+    spp_nv: int = 2
+    max_depth: int = 7
+    sample_depth: int = 3
+    rr_start_depth: int = 7
+    passes: int = 1
+    seed: int = 1
+    variance: bool = False
+
+    def desc(self):
+        d = abi.JobDesc()
+        d.spp_nu, d.spp_nv = self.spp_nu, self.spp_nv
+        d.max_depth, d.sample_depth = self.max_depth, self.sample_depth
+        d.rr_start_depth, d.passes = self.rr_start_depth, self.passes
+        d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        d.flags = abi.JOB_VARIANCE if self.variance else 0
+        return d
+
+
+@dataclass
+class PassDone:
+    """Progress event after a pass (Rendering.hs:165-169)."""
+    pass_num: int
+    final_img: np.ndarray  # (h, w, 4) float32 = (weight, X, Y, Z)
+    splat_weight: float = 1.0
+
+
+class Job:
+    def __init__(self, scene, spec):
+        self.scene = scene
+        self.spec = spec
+        lib = scene.ctx.lib
+        h = C.c_void_p()
+        _ck(lib.bgpu_job_submit(scene.h, C.byref(spec.desc()), C.byref(h)))
+        self.h = h
+        w, hh = scene.flat.film[0], scene.flat.film[1]
+        self.shape = (hh, w)
+
+    def poll(self, want_image=True):
+        lib = self.scene.ctx.lib
+        h, w = self.shape
+        img = np.empty((h, w, 4), np.float32) if want_image else None
+        sq = np.empty((h, w, 3), np.float32) if (want_image and self.spec.variance) else None
+        done = C.c_int32()
+        state = C.c_int32()
+        _ck(lib.bgpu_job_poll(self.h, img.ctypes.data if img is not None else None,
+                              sq.ctypes.data if sq is not None else None,
+                              C.byref(done), C.byref(state)))
+        return img, sq, done.value, state.value
+
+    def wait(self, timeout=600.0):
+        _ck(self.scene.ctx.lib.bgpu_job_wait(self.h, float(timeout)))
+
+    def cancel(self):
+        _ck(self.scene.ctx.lib.bgpu_job_cancel(self.h))
+
+    def latency_ms(self):
+        v = C.c_double()
+        _ck(self.scene.ctx.lib.bgpu_job_latency_ms(self.h, C.byref(v)))
+        return v.value
+
+    def release(self):
+        if self.h:
+            _ck(self.scene.ctx.lib.bgpu_job_release(self.h))
+            self.h = None
+
+
+class GpuRenderer:
+    """The `gpu` renderer: one job, progressive passes (Rendering.hs:207-236)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+
+    def render_plain(self, scene):
+        """Blocking one-job path: returns ((h, w, 4) wxyz, (h, w, 3) sum of squares or None)."""
+        lib = scene.ctx.lib
+        w, h = scene.flat.film[0], scene.flat.film[1]
+        img = np.zeros((h, w, 4), np.float32)
+        sq = np.zeros((h, w, 3), np.float32) if self.spec.variance else None
+        _ck(lib.bgpu_render(scene.h, C.byref(self.spec.desc()), img.ctypes.data,
+                            sq.ctypes.data if sq is not None else None))
+        return img, sq
+
+    def render(self, scene, report):
+        job = Job(scene, self.spec)
+        try:
+            seen = 0
+            while True:
+                job.wait(timeout=0.05) if False else None
+                img, _, done, state = job.poll()
+                if done > seen:
+                    seen = done
+                    if not report(PassDone(done, img, 1.0)):
+                        job.cancel()
+                        break
+                if state in (abi.JOB_DONE, abi.JOB_CANCELLED, abi.JOB_FAILED):
+                    break
+                import time
+                time.sleep(0.002)
+        finally:
+            job.release()

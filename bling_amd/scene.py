@@ -1,0 +1,333 @@
+"""Scene description on the host side: the flattened arrays handed to
+`bgpu_scene_create`, built the way bling's `.bling` front end builds its
+primitives (src/lib/Graphics/Bling/IO/PrimitiveParser.hs:28-139):
+
+  * meshes are pre-transformed to world space with transPoint/transNormal
+    (Primitive/TriangleMesh.hs:119-140);
+  * shapes keep their object-to-world transform and its inverse and get an
+    AreaLight with a fresh id when an emission is active (Primitive/Geometry.hs:14-37,
+    IO/PrimitiveParser.hs:63-67);
+  * spectra come from rgbR / rgbI / spd / constant exactly as Spectrum.hs
+    converts them (rgbToSpectrum Spectrum.hs:146-159, fromSpd Spectrum.hs:291-335).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .spectral_tables import TABLES
+from .transform import Transform
+
+F = np.float32
+BANDS = 16
+
+_T = {k: np.array(v, dtype=np.float32) for k, v in TABLES.items()}
+
+
+def const(c):
+    """sConst (Spectrum.hs:375-377)."""
+    return np.full(BANDS, F(c), dtype=np.float32)
+
+
+def _rgb_to_spectrum(kind, r, g, b):
+    """rgbToSpectrum (Spectrum.hs:146-159), elementwise binary32."""
+    r, g, b = F(r), F(g), F(b)
+    t = lambda n: _T["rgb_%s_%s" % (kind, n)]
+    wb, cb, mb, yb = t("white"), t("cyan"), t("magenta"), t("yellow")
+    rb, gb, bb = t("red"), t("green"), t("blue")
+    if r <= g and r <= b:
+        if g <= b:
+            rest = (cb * F(g - r)).astype(F) + (bb * F(b - g)).astype(F)
+        else:
+            rest = (cb * F(b - r)).astype(F) + (gb * F(g - b)).astype(F)
+        return ((wb * r).astype(F) + rest.astype(F)).astype(F)
+    if g <= r and g <= b:
+        if r <= b:
+            rest = (mb * F(r - g)).astype(F) + (bb * F(b - r)).astype(F)
+        else:
+            rest = (mb * F(b - g)).astype(F) + (rb * F(r - b)).astype(F)
+        return ((wb * g).astype(F) + rest.astype(F)).astype(F)
+    if r <= b:
+        rest = (yb * F(r - b)).astype(F) + (gb * F(g - r)).astype(F)
+    else:
+        rest = (yb * F(g - b)).astype(F) + (rb * F(r - g)).astype(F)
+    return ((wb * b).astype(F) + rest.astype(F)).astype(F)
+
+
+def rgb_refl(r, g, b):
+    return _rgb_to_spectrum("refl", r, g, b)
+
+
+def rgb_illum(r, g, b):
+    return _rgb_to_spectrum("illum", r, g, b)
+
+
+def _lerp(t, a, b):
+    return F(F(F(F(1) - t) * a) + F(t * b))
+
+
+def spd(pairs):
+    """fromSpd (mkSpd pairs) for an irregular SPD (Spectrum.hs:199-207,307-314)."""
+    pairs = sorted(((F(l), F(v)) for l, v in pairs), key=lambda p: p[0])
+    ls = [p[0] for p in pairs]
+    vs = [p[1] for p in pairs]
+    out = []
+    for i in range(BANDS):
+        l0 = _lerp(F(F(i) / F(BANDS)), F(400), F(700))
+        l1 = _lerp(F(F(i + 1) / F(BANDS)), F(400), F(700))
+        if l1 <= ls[0]:
+            out.append(vs[0])
+            continue
+        if l0 >= ls[-1]:
+            out.append(vs[-1])
+            continue
+        i0 = next((k for k, l in enumerate(ls) if l >= l0), 0)
+        i1 = next((k for k, l in enumerate(ls) if l >= l1), len(vs) - 1)
+        sl = vs[i0:i1 + 1]
+        acc = F(0)
+        for v in sl:
+            acc = F(acc + v)
+        out.append(F(acc / F(len(sl))))
+    return np.array(out, dtype=np.float32)
+
+
+def spectrum_to_xyz(s):
+    """spectrumToXYZ (Spectrum.hs:349-355), binary32, left fold."""
+    x = y = z = F(0)
+    for v, a, b, c in zip(np.asarray(s, F), _T["cie_x"], _T["cie_y"], _T["cie_z"]):
+        x = F(x + F(a * v))
+        y = F(y + F(b * v))
+        z = F(z + F(c * v))
+    from .spectral_tables import CIE_Y_SUM
+    ys = F(CIE_Y_SUM)
+    return F(x / ys), F(y / ys), F(z / ys)
+
+
+def xyz_to_rgb(xyz):
+    """xyzToRgb (Spectrum.hs:162-168), vectorised over (..., 3)."""
+    x, y, z = xyz[..., 0], xyz[..., 1], xyz[..., 2]
+    r = 3.240479 * x - 1.537150 * y - 0.498535 * z
+    g = -0.969256 * x + 1.875991 * y + 0.041556 * z
+    b = 0.055648 * x - 0.204043 * y + 1.057311 * z
+    return np.stack([r, g, b], axis=-1)
+
+
+class Material:
+    """Material parameters in the descriptor's layout (bgpu_material)."""
+
+    def __init__(self, kind, spec0=None, spec1=None, s0=0.0, s1=0.0):
+        self.kind = kind
+        self.spec0 = const(0) if spec0 is None else np.asarray(spec0, F)
+        self.spec1 = const(0) if spec1 is None else np.asarray(spec1, F)
+        self.s0, self.s1 = F(s0), F(s1)
+
+    @staticmethod
+    def matte(kd, sigma=0.0):
+        return Material(abi.MAT_MATTE, kd, None, sigma)
+
+    @staticmethod
+    def plastic(kd, ks, rough):
+        return Material(abi.MAT_PLASTIC, kd, ks, rough)
+
+    @staticmethod
+    def metal(eta, k, rough):
+        return Material(abi.MAT_METAL, eta, k, rough)
+
+    @staticmethod
+    def glass(ior, kr, kt):
+        return Material(abi.MAT_GLASS, kr, kt, ior)
+
+    @staticmethod
+    def mirror(kr):
+        return Material(abi.MAT_MIRROR, kr)
+
+    @staticmethod
+    def black():
+        return Material(abi.MAT_BLACK)
+
+
+class SceneBuilder:
+    """Accumulates primitives like bling's parser state (PState,
+    IO/ParserCore.hs:45-58) and flattens them into a bgpu_scene_desc."""
+
+    def __init__(self):
+        self.verts = []      # world-space (n, 3) blocks
+        self.norms = []
+        self.uvs = []
+        self.idx = []
+        self.tri_mesh = []
+        self.meshes = []     # (material, flags)
+        self.nvert = 0
+        self.ntri = 0
+        self.shapes = []     # abi.Shape
+        self.materials = []
+        self.lights = []     # (abi.Light, radiance array)
+        self.next_id = 0
+        self.camera = None
+        self.film = (256, 256, abi.FILTER_BOX, (0.0, 0.0, 0.0, 0.0))
+
+    def material(self, m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def mesh(self, verts, faces, o2w=None, material=0, normals=None, uvs=None):
+        """mkTriangleMesh o2w mat p i n uv: faces are index triples (0-based)."""
+        o2w = o2w or Transform()
+        v = np.asarray(verts, F).reshape(-1, 3)
+        i = np.asarray(faces, np.int64).reshape(-1, 3)
+        if (i < 0).any():
+            raise ValueError("mkTriangleMesh: contains negative indices")
+        if (i >= len(v)).any():
+            raise ValueError("mkTriangleMesh: contains out of bounds indices")
+        flags = 0
+        wv = o2w.points(v)
+        self.verts.append(wv)
+        if normals is not None:
+            flags |= abi.MESH_NORMALS
+            self.norms.append(o2w.normals(np.asarray(normals, F).reshape(-1, 3)))
+        else:
+            self.norms.append(None)
+        if uvs is not None:
+            flags |= abi.MESH_UVS
+            self.uvs.append(np.asarray(uvs, F).reshape(-1, 2))
+        else:
+            self.uvs.append(None)
+        self.idx.append((i + self.nvert).astype(np.int32))
+        mid = len(self.meshes)
+        self.meshes.append((material, flags))
+        self.tri_mesh.append(np.full(len(i), mid, np.int32))
+        self.nvert += len(v)
+        self.ntri += len(i)
+        return mid
+
+    def shape(self, kind, params, o2w=None, material=0, emission=None):
+        """mkGeom o2w False mat emission shape id (Primitive/Geometry.hs:14-37)."""
+        o2w = o2w or Transform()
+        s = abi.Shape()
+        s.type = kind
+        s.material = material
+        p = list(params) + [0.0] * (6 - len(params))
+        if kind == abi.SHAPE_BOX:  # mkBox orders the corners (Shape.hs:41-44)
+            lo = [min(F(p[k]), F(p[3 + k])) for k in range(3)]
+            hi = [max(F(p[k]), F(p[3 + k])) for k in range(3)]
+            p = lo + hi
+        for k in range(6):
+            s.params[k] = F(p[k])
+        for k in range(16):
+            s.o2w[k] = o2w.m[k]
+            s.w2o[k] = o2w.inv[k]
+        s.light_id = self.next_id
+        self.next_id += 1
+        if emission is not None:
+            s.emissive = 1
+            em = np.asarray(emission, F)
+            for k in range(16):
+                s.emission[k] = em[k]
+        self.shapes.append(s)
+        return len(self.shapes) - 1
+
+    def infinite_light(self, radiance_map, w2l=None):
+        """mkInfiniteAreaLight rmap t (Light.hs:72-82); radiance_map is
+        (height, width, 16) spectral pixels."""
+        w2l = w2l or Transform()
+        m = np.ascontiguousarray(np.asarray(radiance_map, F).reshape(-1, 16))
+        h, w = np.asarray(radiance_map).shape[:2]
+        l = abi.Light()
+        l.type = abi.LIGHT_INFINITE
+        l.width, l.height = int(w), int(h)
+        for k in range(16):
+            l.w2l[k] = w2l.m[k]
+            l.l2w[k] = w2l.inv[k]
+        self.lights.append((l, m))
+
+    def perspective_camera(self, c2w, r2c, lens_radius=0.0, focal_distance=1.0):
+        cam = abi.Camera()
+        cam.type = abi.CAMERA_PERSPECTIVE
+        for k in range(16):
+            cam.c2w[k] = c2w.m[k]
+            cam.r2c[k] = r2c.m[k]
+        cam.lens_radius = F(lens_radius)
+        cam.focal_distance = F(focal_distance)
+        self.camera = cam
+        self.c2w, self.r2c = c2w, r2c
+
+    def set_film(self, width, height, filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
+        self.film = (int(width), int(height), int(filt), tuple(fparams))
+
+    def build(self):
+        return FlatScene(self)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class FlatScene:
+    """Owns the flattened arrays and the ctypes descriptor pointing at them."""
+
+    def __init__(self, b):
+        self.vertices = (np.concatenate(b.verts) if b.verts else np.zeros((0, 3), F)).astype(F)
+        self.indices = (np.concatenate(b.idx) if b.idx else np.zeros((0, 3), np.int32)).astype(np.int32)
+        self.tri_mesh = (np.concatenate(b.tri_mesh) if b.tri_mesh else np.zeros(0, np.int32)).astype(np.int32)
+        nv = len(self.vertices)
+        self.normals = None
+        if any(n is not None for n in b.norms):
+            parts = [n if n is not None else np.zeros((len(v), 3), F) for n, v in zip(b.norms, b.verts)]
+            self.normals = np.ascontiguousarray(np.concatenate(parts).astype(F))
+        self.uvs = None
+        if any(u is not None for u in b.uvs):
+            parts = [u if u is not None else np.zeros((len(v), 2), F) for u, v in zip(b.uvs, b.verts)]
+            self.uvs = np.ascontiguousarray(np.concatenate(parts).astype(F))
+        self.vertices = np.ascontiguousarray(self.vertices)
+        self.indices = np.ascontiguousarray(self.indices)
+        mats = b.materials or [Material.matte(const(0.5))]
+        self.materials = (abi.Material * len(mats))()
+        for i, m in enumerate(mats):
+            self.materials[i].type = m.kind
+            self.materials[i].scalar[0] = m.s0
+            self.materials[i].scalar[1] = m.s1
+            for k in range(16):
+                self.materials[i].spec[0][k] = m.spec0[k]
+                self.materials[i].spec[1][k] = m.spec1[k]
+        meshes = b.meshes or [(0, 0)]
+        self.meshes = (abi.Mesh * len(meshes))()
+        for i, (mat, fl) in enumerate(meshes):
+            self.meshes[i].material = mat
+            self.meshes[i].flags = fl
+        self.shapes = (abi.Shape * max(1, len(b.shapes)))()
+        for i, s in enumerate(b.shapes):
+            self.shapes[i] = s
+        self.light_maps = [m for _, m in b.lights]
+        self.lights = (abi.Light * max(1, len(b.lights)))()
+        for i, (l, m) in enumerate(b.lights):
+            l.radiance = m.ctypes.data_as(abi.FP)
+            self.lights[i] = l
+        d = abi.SceneDesc()
+        d.num_vertices = nv
+        d.num_triangles = len(self.indices)
+        d.vertices = _ptr(self.vertices, C.c_float)
+        d.normals = _ptr(self.normals, C.c_float)
+        d.uvs = _ptr(self.uvs, C.c_float)
+        d.indices = _ptr(self.indices, C.c_int32)
+        d.tri_mesh = _ptr(self.tri_mesh, C.c_int32)
+        d.num_meshes = len(meshes)
+        d.meshes = C.cast(self.meshes, C.POINTER(abi.Mesh))
+        d.num_shapes = len(b.shapes)
+        d.shapes = C.cast(self.shapes, C.POINTER(abi.Shape))
+        d.num_materials = len(mats)
+        d.materials = C.cast(self.materials, C.POINTER(abi.Material))
+        d.num_lights = len(b.lights)
+        d.lights = C.cast(self.lights, C.POINTER(abi.Light))
+        if b.camera is not None:
+            d.camera = b.camera
+        fw, fh, ft, fp = b.film
+        d.film.width, d.film.height, d.film.filter = fw, fh, ft
+        for k in range(4):
+            d.film.fparams[k] = fp[k]
+        self.desc = d
+        self.num_prims = len(self.indices) + len(b.shapes)
+        self.num_triangles = len(self.indices)
+        self.num_shapes = len(b.shapes)
+        self.c2w = getattr(b, "c2w", None)
+        self.r2c = getattr(b, "r2c", None)
+        self.film = b.film

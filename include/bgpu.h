@@ -1,0 +1,262 @@
+/*
+ * bgpu.h -- C ABI of the MI355X (gfx950) ray-tracing hot path for bling.
+ *
+ * The library replaces three things in the Haskell renderer bling:
+ *   - the accelerator behind `Primitive{intersect, intersects}`
+ *     (src/lib/Graphics/Bling/Primitive.hs:21-27), built by `kdTreePrimitive`
+ *     (Primitive/KdTree.hs:236-246) and consumed through `scIntersect` /
+ *     `occluded` (Scene.hs:45-51)  -> bgpu_intersect_closest / bgpu_intersect_any;
+ *   - the path integrator `mkPathIntegrator md sd` (Integrator/Path.hs:30-87)
+ *     driven per camera sample by the sampler renderer `prender`
+ *     (Rendering.hs:207-246)  -> bgpu_job_submit / bgpu_job_poll / bgpu_render;
+ *   - the film accumulation `addSample` (Image.hs:250-299), returned as the
+ *     (w, X, Y, Z) float quadruple per pixel of `Image` (Image.hs:123-129).
+ *
+ * Every entry point returns 0 (BGPU_OK) or a negative BGPU_E_* code; the text
+ * of the last error on the calling thread is returned by bgpu_last_error().
+ * All handles are opaque.  Host arrays passed in are copied before the call
+ * returns unless BGPU_DEVICE_PTRS is set, in which case they must be device
+ * pointers valid on the context's device until the call returns.
+ * A context serialises submissions internally and may be used from any thread.
+ */
+#ifndef BGPU_H
+#define BGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BGPU_ABI_VERSION 1
+#define BGPU_NUM_BANDS 16 /* Spectrum.hs:34-36 */
+
+/* ---- error codes ------------------------------------------------------ */
+enum {
+  BGPU_OK = 0,
+  BGPU_E_INVALID = -1,   /* bad argument or malformed scene (cf. TriangleMesh.hs:127-136) */
+  BGPU_E_NOMEM = -2,     /* host or device allocation failed */
+  BGPU_E_HIP = -3,       /* HIP runtime error */
+  BGPU_E_BUDGET = -4,    /* job or scene does not fit the context's HBM budget */
+  BGPU_E_STATE = -5,     /* call not valid in the handle's current state */
+  BGPU_E_CANCELLED = -6, /* job was cancelled */
+  BGPU_E_NODEV = -7,     /* no usable GPU */
+  BGPU_E_TIMEOUT = -8    /* bgpu_job_wait timed out */
+};
+
+typedef struct bgpu_ctx bgpu_ctx;
+typedef struct bgpu_scene bgpu_scene;
+typedef struct bgpu_job bgpu_job;
+
+/* ---- scene description (copied by bgpu_scene_create) ----------------- */
+
+/* Analytic shapes, Shape.hs:22-38.  params:
+ *   BOX      pmin.xyz pmax.xyz                 (mkBox orders them, Shape.hs:41-44)
+ *   CYLINDER radius zmin zmax phimax[rad]      (Shape.hs:47-56)
+ *   DISK     height radius inner phimax[rad]   (Shape.hs:59-68)
+ *   QUAD     sx sy                             (spans [-sx,sx]x[-sy,sy], Shape.hs:157-171)
+ *   SPHERE   radius                            (Shape.hs:75-78) */
+enum {
+  BGPU_SHAPE_BOX = 0,
+  BGPU_SHAPE_CYLINDER = 1,
+  BGPU_SHAPE_DISK = 2,
+  BGPU_SHAPE_QUAD = 3,
+  BGPU_SHAPE_SPHERE = 4
+};
+
+/* One `mkGeom` primitive (Primitive/Geometry.hs:14-37): a shape intersected
+ * in object space through w2o, with an AreaLight when emissive. */
+typedef struct {
+  int32_t type;
+  int32_t material;   /* index into materials[] */
+  int32_t emissive;   /* nonzero: carries an AreaLight with radiance emission[] */
+  int32_t light_id;   /* the AreaLight id used by light equality (Light.hs:47-50) */
+  float params[6];
+  float o2w[16];      /* object-to-world matrix, row-major 4x4 */
+  float w2o[16];      /* its stored inverse (Transform.hs:218-221) */
+  float emission[16]; /* spectral radiance when emissive */
+} bgpu_shape;
+
+/* Materials, Material.hs:32-96 (constant textures only). */
+enum {
+  BGPU_MAT_BLACK = 0,   /* blackBodyMaterial: no BxDF (Reflection.hs:337-338) */
+  BGPU_MAT_MATTE = 1,   /* spec[0]=kd, scalar[0]=sigma (Lambertian if 0, else Oren-Nayar) */
+  BGPU_MAT_PLASTIC = 2, /* spec[0]=kd, spec[1]=ks, scalar[0]=roughness */
+  BGPU_MAT_METAL = 3,   /* spec[0]=eta, spec[1]=k, scalar[0]=roughness */
+  BGPU_MAT_GLASS = 4,   /* spec[0]=reflect, spec[1]=transmit (both clamped to [0,1]), scalar[0]=ior */
+  BGPU_MAT_MIRROR = 5   /* spec[0]=reflect (clamped to [0,1]) */
+};
+typedef struct {
+  int32_t type;
+  int32_t pad;
+  float scalar[2];
+  float spec[2][16];
+} bgpu_material;
+
+/* A triangle mesh made by mkTriangleMesh (Primitive/TriangleMesh.hs:119-140). */
+enum { BGPU_MESH_NORMALS = 1, BGPU_MESH_UVS = 2 };
+typedef struct {
+  int32_t material;
+  int32_t flags;
+} bgpu_mesh;
+
+/* Explicit (non-geometric) lights.  Only the infinite area light with a
+ * discrete spectral map is on the path (Light.hs:72-82,104-106,130-141,220-225). */
+enum { BGPU_LIGHT_INFINITE = 1 };
+typedef struct {
+  int32_t type;
+  int32_t width, height; /* texSize of the map */
+  int32_t pad;
+  const float* radiance; /* width*height*16 floats, pixel (x,y) at ((y*width)+x)*16 */
+  float w2l[16];         /* world-to-light matrix, row-major */
+  float l2w[16];         /* its inverse */
+} bgpu_light;
+
+enum { BGPU_CAMERA_PERSPECTIVE = 0, BGPU_CAMERA_ENVIRONMENT = 1 };
+/* ProjectiveCamera / Environment (Camera.hs:24-33) after mkProjective. */
+typedef struct {
+  int32_t type;
+  int32_t pad;
+  float c2w[16];           /* camera-to-world matrix, row-major */
+  float r2c[16];           /* raster-to-camera matrix, row-major */
+  float lens_radius;
+  float focal_distance;
+  float xres, yres;        /* environment camera only */
+} bgpu_camera;
+
+/* Pixel filters, Filter.hs:12-96. fparams:
+ *   BOX -, GAUSS w h alpha, MITCHELL w h b c, SINC w h tau, TRIANGLE w h */
+enum {
+  BGPU_FILTER_BOX = 0,
+  BGPU_FILTER_GAUSS = 1,
+  BGPU_FILTER_MITCHELL = 2,
+  BGPU_FILTER_SINC = 3,
+  BGPU_FILTER_TRIANGLE = 4
+};
+typedef struct {
+  int32_t width, height;
+  int32_t filter;
+  int32_t pad;
+  float fparams[4];
+} bgpu_film;
+
+/* The flattened scene.  Primitive ids: triangles are 0..num_triangles-1 in
+ * index-buffer order, shapes follow as num_triangles + shape index.  Lights:
+ * lights[] (explicit) first, then the emissive shapes in shape order, the
+ * order of `mkScene` (Scene.hs:37-43). */
+typedef struct {
+  uint32_t num_vertices;
+  uint32_t num_triangles;
+  const float* vertices;   /* 3*num_vertices, world space (transPoint applied) */
+  const float* normals;    /* 3*num_vertices world-space shading normals, or NULL */
+  const float* uvs;        /* 2*num_vertices, or NULL */
+  const int32_t* indices;  /* 3*num_triangles */
+  const int32_t* tri_mesh; /* mesh index per triangle, or NULL for mesh 0 */
+  uint32_t num_meshes;
+  const bgpu_mesh* meshes;
+  uint32_t num_shapes;
+  const bgpu_shape* shapes;
+  uint32_t num_materials;
+  const bgpu_material* materials;
+  uint32_t num_lights;
+  const bgpu_light* lights;
+  bgpu_camera camera;
+  bgpu_film film;
+} bgpu_scene_desc;
+
+/* ---- ray batches ------------------------------------------------------ */
+/* Ray = origin, direction, tmin, tmax (Math.hs:375-380), structure of arrays. */
+typedef struct {
+  const float *ox, *oy, *oz;
+  const float *dx, *dy, *dz;
+  const float *tmin, *tmax;
+} bgpu_rays;
+
+/* Closest-hit result: t, the triangle barycentrics (b1, b2) of
+ * TriangleMesh.hs:254-261 (0 for shapes) and the primitive id, -1 on a miss. */
+typedef struct {
+  float *t, *b1, *b2;
+  int32_t* prim;
+} bgpu_hits;
+
+enum {
+  BGPU_DEVICE_PTRS = 1, /* ray/hit arrays are device pointers */
+  BGPU_TIMED = 2        /* record the kernel time, read by bgpu_last_kernel_ms */
+};
+
+/* ---- jobs --------------------------------------------------------------- */
+/* One render job: `renderer { gpu sampled { sampler { stratified nu nv }
+ * integrator { path maxDepth md sampleDepth sd } } }` (RendererParser.hs:56-73,
+ * IntegratorParser.hs:41-44), `passes` progressive passes of nu*nv samples per
+ * pixel over the film's sample extent (Image.hs:162-168, Rendering.hs:207-236). */
+enum { BGPU_JOB_VARIANCE = 1 /* also accumulate per-pixel sum of squared XYZ */ };
+typedef struct {
+  int32_t spp_nu, spp_nv;
+  int32_t max_depth, sample_depth;
+  int32_t rr_start_depth; /* Russian roulette when depth > this (Path.hs:70 uses 7) */
+  int32_t passes;         /* number of progressive passes, >= 1 */
+  uint64_t seed;
+  int32_t flags;
+  int32_t pad;
+} bgpu_job_desc;
+
+enum {
+  BGPU_JOB_QUEUED = 0,
+  BGPU_JOB_RUNNING = 1,
+  BGPU_JOB_DONE = 2,
+  BGPU_JOB_CANCELLED = 3,
+  BGPU_JOB_FAILED = 4
+};
+
+typedef struct {
+  uint64_t rays_camera, rays_extend, rays_shadow, rays_mis;
+  uint64_t samples;          /* camera samples completed */
+  uint64_t jobs_done;
+  double gpu_ms;             /* device time of the render launches */
+} bgpu_stats;
+
+/* ---- entry points ------------------------------------------------------ */
+const char* bgpu_last_error(void);
+int bgpu_version(void);
+
+int bgpu_init(int device, size_t hbm_budget_bytes, bgpu_ctx** out);
+int bgpu_shutdown(bgpu_ctx* ctx);
+
+/* kdTreePrimitive (Primitive/KdTree.hs:236-246) over `mkScene`'s primitives:
+ * flattens the description into device-resident SoA arrays and builds the BVH. */
+int bgpu_scene_create(bgpu_ctx* ctx, const bgpu_scene_desc* desc, bgpu_scene** out);
+int bgpu_scene_destroy(bgpu_scene* scene);
+/* Device bytes held by a scene. */
+int bgpu_scene_bytes(const bgpu_scene* scene, size_t* bytes);
+
+/* `intersect` of the accelerator primitive (Primitive/KdTree.hs:240-242). */
+int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
+                           bgpu_hits* hits, int flags);
+/* `intersects` of the accelerator primitive (Primitive/KdTree.hs:244-246). */
+int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
+                       uint8_t* occluded, int flags);
+/* Duration in ms of the last BGPU_TIMED intersect launch on this context. */
+int bgpu_last_kernel_ms(bgpu_ctx* ctx, double* ms);
+
+/* Plain path: render one job to completion, blocking (prender, Rendering.hs:207-236).
+ * wxyz: width*height*4 floats (w, X, Y, Z); sq: width*height*3 or NULL. */
+int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float* sq);
+
+/* Served path: asynchronous jobs that share the device (continuous batching). */
+int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out);
+/* Progressive result (PassDone, Rendering.hs:165-169): copies the film so far. */
+int bgpu_job_poll(bgpu_job* job, float* wxyz, float* sq, int32_t* passes_done, int32_t* state);
+int bgpu_job_wait(bgpu_job* job, double timeout_s);
+int bgpu_job_cancel(bgpu_job* job);
+int bgpu_job_release(bgpu_job* job);
+/* Arrival-to-completion latency of a finished job, in ms. */
+int bgpu_job_latency_ms(bgpu_job* job, double* ms);
+
+int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out);
+int bgpu_stats_reset(bgpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BGPU_H */
