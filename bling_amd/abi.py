@@ -21,7 +21,7 @@ MESH_NORMALS, MESH_UVS = 1, 2
 LIGHT_INFINITE = 1
 CAMERA_PERSPECTIVE, CAMERA_ENVIRONMENT = 0, 1
 FILTER_BOX, FILTER_GAUSS, FILTER_MITCHELL, FILTER_SINC, FILTER_TRIANGLE = range(5)
-DEVICE_PTRS, TIMED = 1, 2
+DEVICE_PTRS, TIMED, COUNT = 1, 2, 4
 JOB_VARIANCE = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = range(5)
 
@@ -90,7 +90,8 @@ class JobDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extend", C.c_uint64),
                 ("rays_shadow", C.c_uint64), ("rays_mis", C.c_uint64),
-                ("samples", C.c_uint64), ("jobs_done", C.c_uint64), ("gpu_ms", C.c_double)]
+                ("samples", C.c_uint64), ("jobs_done", C.c_uint64), ("gpu_ms", C.c_double),
+                ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64)]
 
 
 # every symbol declared in include/bgpu.h

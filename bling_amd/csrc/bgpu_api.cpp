@@ -160,6 +160,27 @@ static int stage_out(Ctx* c, size_t need) {
   return BGPU_OK;
 }
 
+// BGPU_COUNT: zeroed device counters for the counting kernel variant.
+static int count_begin(Ctx* c, int flags, unsigned long long** cnt) {
+  *cnt = nullptr;
+  if (!(flags & BGPU_COUNT)) return BGPU_OK;
+  if (!c->count_buf.ptr)
+    if (int rc = c->count_buf.alloc(c, 2 * sizeof(unsigned long long))) return rc;
+  BGPU_HIP(hipMemsetAsync(c->count_buf.ptr, 0, 2 * sizeof(unsigned long long), c->stream));
+  *cnt = c->count_buf.as<unsigned long long>();
+  return BGPU_OK;
+}
+
+static int count_end(Ctx* c, unsigned long long* cnt) {
+  if (!cnt) return BGPU_OK;
+  unsigned long long h[2];
+  BGPU_HIP(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
+  std::lock_guard<std::mutex> lk(c->stats_mu);
+  c->stats.nodes_visited += h[0];
+  c->stats.prims_tested += h[1];
+  return BGPU_OK;
+}
+
 int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, bgpu_hits* hits, int flags) {
   if (!scene || !hits || !hits->t || !hits->b1 || !hits->b2 || !hits->prim)
     return fail(BGPU_E_INVALID, "bgpu_intersect_closest: NULL argument");
@@ -181,8 +202,10 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, b
     float* o = c->stage_hits.as<float>();
     dh = DevHits{o, o + n, o + 2 * n, reinterpret_cast<int32_t*>(o + 3 * n)};
   }
+  unsigned long long* cnt;
+  if (int rc = count_begin(c, flags, &cnt)) return rc;
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
-  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, nullptr));
+  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, cnt));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) {
     BGPU_HIP(hipMemcpyAsync(hits->t, dh.t, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -191,6 +214,7 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, b
     BGPU_HIP(hipMemcpyAsync(hits->prim, dh.prim, n * 4, hipMemcpyDeviceToHost, c->stream));
   }
   BGPU_HIP(hipStreamSynchronize(c->stream));
+  if (int rc = count_end(c, cnt)) return rc;
   if (flags & BGPU_TIMED) {
     float ms = 0.0f;
     BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -218,11 +242,14 @@ int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n, uint8
     if (int rc = stage_out(c, n)) return rc;
     docc = c->stage_hits.as<uint8_t>();
   }
+  unsigned long long* cnt;
+  if (int rc = count_begin(c, flags, &cnt)) return rc;
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
-  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, nullptr));
+  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, cnt));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) BGPU_HIP(hipMemcpyAsync(occluded, docc, n, hipMemcpyDeviceToHost, c->stream));
   BGPU_HIP(hipStreamSynchronize(c->stream));
+  if (int rc = count_end(c, cnt)) return rc;
   if (flags & BGPU_TIMED) {
     float ms = 0.0f;
     BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));

@@ -63,6 +63,7 @@ struct Ctx {
   std::mutex mu;                  // serialises submissions on `stream`
   // staging for host-pointer ray batches
   DevBuf stage_rays, stage_hits;
+  DevBuf count_buf;               // BGPU_COUNT traversal counters
   std::mutex stats_mu;
   bgpu_stats stats{};
   Scheduler* sched = nullptr;     // served path, created on first job_submit

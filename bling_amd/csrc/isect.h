@@ -50,8 +50,8 @@ struct TraceConfig {
 };
 
 hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,
-                          const TraceConfig& cfg, hipStream_t s, uint32_t* counters);
+                          const TraceConfig& cfg, hipStream_t s, unsigned long long* counters);
 hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
-                      const TraceConfig& cfg, hipStream_t s, uint32_t* counters);
+                      const TraceConfig& cfg, hipStream_t s, unsigned long long* counters);
 
 }  // namespace bgpu

@@ -81,8 +81,8 @@ struct Best {
 // Traverses one tree.  CLOSEST: shrinks best.t; ANY: returns at the first hit.
 template <bool ANY, bool FAT, bool STATS>
 __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRay& r, f3 inv,
-                                         f3 oi, Stack& st, Best& best, uint32_t& nvisit,
-                                         uint32_t& ntest) {
+                                         f3 oi, Stack& st, Best& best, unsigned long long& nvisit,
+                                         unsigned long long& ntest) {
   int cur = root;
   st.sp = 0;
   for (;;) {
@@ -159,14 +159,14 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
 
 template <int BLOCK, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, int n, DevHits out,
-                                                   uint32_t* counters) {
+                                                   unsigned long long* counters) {
   __shared__ int lds[kShort * BLOCK];
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
   Best best{r.tmax, 0.0f, 0.0f, -1};
-  uint32_t nvisit = 0, ntest = 0;
+  unsigned long long nvisit = 0, ntest = 0;
   if (slab_haskell(sc.root_lo, sc.root_hi, r)) {  // KdTree.hs:240
     f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
     f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
@@ -197,14 +197,14 @@ __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, in
 
 template <int BLOCK, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_any(DevScene sc, DevRays rays, int n, uint8_t* occ,
-                                               uint32_t* counters) {
+                                               unsigned long long* counters) {
   __shared__ int lds[kShort * BLOCK];
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
   bool found = false;
-  uint32_t nvisit = 0, ntest = 0;
+  unsigned long long nvisit = 0, ntest = 0;
   if (slab_haskell(sc.root_lo, sc.root_hi, r)) {  // KdTree.hs:244
     f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
     f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(DevScene sc, DevRays rays, int n,
 
 template <int BLOCK>
 hipError_t run_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h, hipStream_t s,
-                       uint32_t* counters) {
+                       unsigned long long* counters) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
   if (counters)
     hipLaunchKernelGGL((k_closest<BLOCK, true>), grid, dim3(BLOCK), 0, s, sc, r, n, h, counters);
@@ -237,7 +237,7 @@ hipError_t run_closest(const DevScene& sc, const DevRays& r, int n, const DevHit
 
 template <int BLOCK>
 hipError_t run_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hipStream_t s,
-                   uint32_t* counters) {
+                   unsigned long long* counters) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
   if (counters)
     hipLaunchKernelGGL((k_any<BLOCK, true>), grid, dim3(BLOCK), 0, s, sc, r, n, occ, counters);
@@ -249,7 +249,7 @@ hipError_t run_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hi
 }  // namespace
 
 hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,
-                          const TraceConfig& cfg, hipStream_t s, uint32_t* counters) {
+                          const TraceConfig& cfg, hipStream_t s, unsigned long long* counters) {
   if (n <= 0) return hipSuccess;
   switch (cfg.block) {
     case 64: return run_closest<64>(sc, r, n, h, s, counters);
@@ -259,7 +259,7 @@ hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const Dev
 }
 
 hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
-                      const TraceConfig& cfg, hipStream_t s, uint32_t* counters) {
+                      const TraceConfig& cfg, hipStream_t s, unsigned long long* counters) {
   if (n <= 0) return hipSuccess;
   switch (cfg.block) {
     case 64: return run_any<64>(sc, r, n, occ, s, counters);

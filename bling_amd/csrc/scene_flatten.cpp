@@ -16,6 +16,7 @@ Ctx::~Ctx() {
   sched = nullptr;
   stage_rays.release();
   stage_hits.release();
+  count_buf.release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);

@@ -182,7 +182,9 @@ typedef struct {
 
 enum {
   BGPU_DEVICE_PTRS = 1, /* ray/hit arrays are device pointers */
-  BGPU_TIMED = 2        /* record the kernel time, read by bgpu_last_kernel_ms */
+  BGPU_TIMED = 2,       /* record the kernel time, read by bgpu_last_kernel_ms */
+  BGPU_COUNT = 4        /* run the counting variant: adds node visits and
+                           primitive tests to bgpu_stats (not for timing) */
 };
 
 /* ---- jobs --------------------------------------------------------------- */
@@ -214,6 +216,8 @@ typedef struct {
   uint64_t samples;          /* camera samples completed */
   uint64_t jobs_done;
   double gpu_ms;             /* device time of the render launches */
+  uint64_t nodes_visited;    /* BGPU_COUNT queries: BVH node fetches */
+  uint64_t prims_tested;     /* BGPU_COUNT queries: primitive tests */
 } bgpu_stats;
 
 /* ---- entry points ------------------------------------------------------ */
