@@ -84,8 +84,9 @@ struct Scene {
       light_maps, light_dist, filter_tbl;
   bgpu_film film{};
   bgpu_camera camera{};
-  std::atomic<int> refs{1};
+  std::atomic<int> refs{1};  // the user's handle plus engines referencing it
   size_t device_bytes() const;
+  virtual ~Scene() = default;
 };
 
 // Host-side flattening of a scene description (shared by scene creation).

@@ -1,4 +1,5 @@
-// Wavefront path tracer: device scene data and kernel launchers.
+// Wavefront path tracer: device scene data, job table, path pool and the
+// bounce-loop kernels (Integrator/Path.hs:41-87 restated as queues).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -9,12 +10,134 @@
 
 namespace bgpu {
 
-struct PtConfig {
-  int block = 128;
+constexpr int kBands = 16;
+constexpr int kMaxJobs = 1024;   // job table slots (reused)
+constexpr int kMaxScenes = 64;   // scenes referenced by the engine
+
+// BxDF flags and kinds (Reflection.hs:94-108, Reflection/*.hs)
+enum { BX_R = 1, BX_T = 2, BX_D = 4, BX_G = 8, BX_S = 16 };
+enum { K_LAMBERT = 0, K_OREN = 1, K_MF = 2, K_SPECR = 3, K_SPECT = 4 };
+enum { FR_NOOP = 0, FR_DIEL = 1, FR_COND = 2 };
+
+struct DBxdf {
+  int32_t kind, type, fr, pad;
+  float A, B, e, ei, et, pad2[3];
+  float r[16], eta[16], k[16];
+};
+struct DMaterial {
+  int32_t n;
+  int32_t pad[3];
+  DBxdf c[2];
 };
 
+enum { LT_AREA = 0, LT_INF = 1 };
+struct DLight {
+  int32_t type, shape, id, w, h, map_off, dist_off, pad;
+  float L[16];
+  float w2l[16], l2w[16];
+};
+
+// Everything the bounce kernels need about one scene (device resident).
+struct DevPtScene {
+  DevScene geo;
+  const int32_t* tri_idx;
+  const float* verts;
+  const float* normals;
+  const float* uvs;
+  const int2* tri_info;  // (material, mesh flags)
+  const DShapeX* shapex;
+  const DMaterial* mats;
+  const DLight* lights;
+  const float* maps;
+  const float* dist;
+  int32_t ntri, nlights, nmats, cam_type;
+  float c2w[16], r2c[16];
+  float lens_radius, focal_distance, xres, yres;
+  int32_t film_w, film_h, ext_x0, ext_y0, ext_w, ext_h;
+  float fw, fh;
+  float filter[256];
+};
+
+// One job in the device job table.  Its samples occupy the work indices
+// [begin, end) of a monotonic 64-bit work space; a work index maps to
+// (pass, pixel of the sample extent, sample n) pass-major.
+struct DevJob {
+  unsigned long long begin, end;
+  unsigned long long per_pass;  // extent pixels * nu * nv
+  uint64_t seed;
+  int32_t scene, nu, nv, N, md, sd, rr, passes;
+  float* wxyz;
+  float* sq;
+  unsigned int* pass_done;      // finished samples per pass
+  unsigned long long* skipped;  // samples dropped after cancellation
+  int32_t cancelled, pad;
+};
+
+// Device counters of the engine.
+struct PtCounters {
+  unsigned long long work_next, work_total;
+  unsigned int n_act, n_shadow, n_mis, last_act;
+  unsigned long long tot_camera, tot_extend, tot_shadow, tot_mis, tot_samples;
+  unsigned int ring_head, ring_tail;  // open-job ring [head, tail)
+  unsigned int iter, pad;
+};
+
+// Admission request consumed by the admit kernel.
+struct AdmitReq {
+  int32_t slot, valid;
+  unsigned long long size;
+};
+
+// Path pool, structure of arrays of P slots.
+struct PathPool {
+  int32_t P;
+  int32_t* state;   // 0 free, 1 active, 2 done
+  int32_t* job;
+  int32_t* pass;
+  int32_t* pix;
+  int32_t* samp;
+  int32_t* dspec;   // depth | spec << 16
+  float* sxy;       // 2P: image position of the camera sample
+  float* ray;       // 7P: ox oy oz dx dy dz tmin
+  float* hit;       // 3P: t b1 b2
+  int32_t* hprim;
+  float* thr;       // 16P throughput
+  float* rad;       // 16P radiance
+  float* sray;      // 8P shadow ray
+  float* pend_s;    // 16P contribution if the shadow ray is unoccluded
+  float* mray;      // 7P MIS ray
+  float* pend_m;    // 16P MIS factor, times the light's emission on a hit
+  int32_t* mlight;  // light index of the MIS ray
+  uint32_t* q_act;
+  uint32_t* q_shadow;
+  uint32_t* q_mis;
+};
+
+struct PtConfig {
+  int block = 128;
+  int pool = 1 << 20;
+};
+
+struct PtKernelArgs {
+  PathPool pool;
+  const DevPtScene* scenes;
+  DevJob* jobs;
+  uint32_t* ring;  // job slots in admission order (kMaxJobs entries, modulo)
+  PtCounters* cnt;
+  AdmitReq* admit;
+};
+
+// Host-side shading data owned by a Scene.
 struct PtScene {
   std::vector<int32_t> shape_light_host;  // device light index per shape, -1 if none
+  DevPtScene host{};                      // filled by build_pt_data, pointers are device
 };
+
+// Enqueues one bounce iteration (reset, generate, extend, shade, shadow,
+// MIS, finish) on `s`.  Capturable into a hipGraph.
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, int block, hipStream_t s);
+// Enqueues the admission of jobs listed in `admit` (count n) on `s`.
+hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s);
+hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s);
 
 }  // namespace bgpu

@@ -1,0 +1,549 @@
+// Wavefront path tracer kernels for gfx950 (Integrator/Path.hs:41-87 as
+// queues).  One bounce iteration:
+//   reset     (1 thread)  totals += queue counts, queues := 0, retire consumed jobs
+//   generate  (P slots)   free slots take the next work item (job, pass, pixel,
+//                         sample) and fire a camera ray; every active slot is
+//                         appended to the active queue (wave ballot + prefix)
+//   extend    (active)    closest hit of the path ray
+//   shade     (active)    emission, next-event estimation with MIS (one shadow
+//                         ray + one BSDF-sampled MIS ray per vertex, Scene.hs:61-118),
+//                         Russian roulette and the continuation sample
+//   shadow    (queue)     any hit; unoccluded adds the light-sample contribution
+//   mis       (queue)     closest hit; hitting the sampled light (or escaping to
+//                         an infinite light) adds the BSDF-sample contribution
+//   finish    (active)    terminated paths splat into their job's film
+// Kernel grids cover the pool; queue lengths are read on the device, so the
+// sequence replays from a captured hipGraph without host round trips.
+#include "pt_device.h"
+#include "trav.h"
+
+namespace bgpu {
+
+using namespace sh;
+
+namespace {
+
+// Wave-aggregated append: one atomic per wave, lanes get consecutive slots.
+__device__ __forceinline__ unsigned int wave_append(unsigned int* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0) return 0;
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0) return;
+  int lane = __lane_id();
+  if (lane == __ffsll((long long)mask) - 1) atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+__global__ void k_init(PtKernelArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    PtCounters z = {};
+    *a.cnt = z;
+  }
+  if (i < kMaxJobs) a.ring[i] = 0;
+  if (i < a.pool.P) a.pool.state[i] = 0;
+}
+
+__global__ void k_admit(PtKernelArgs a, int n) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  PtCounters* c = a.cnt;
+  for (int k = 0; k < n; ++k) {
+    AdmitReq r = a.admit[k];
+    if (!r.valid) continue;
+    DevJob& J = a.jobs[r.slot];
+    unsigned long long b = c->work_total > c->work_next ? c->work_total : c->work_next;
+    J.begin = b;
+    J.end = b + r.size;
+    c->work_total = J.end;
+    a.ring[c->ring_tail % kMaxJobs] = (uint32_t)r.slot;
+    c->ring_tail++;
+  }
+}
+
+__global__ void k_reset(PtKernelArgs a) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  PtCounters* c = a.cnt;
+  c->tot_extend += c->n_act;
+  c->tot_shadow += c->n_shadow;
+  c->tot_mis += c->n_mis;
+  c->last_act = c->n_act;
+  c->n_act = 0;
+  c->n_shadow = 0;
+  c->n_mis = 0;
+  c->iter++;
+  while (c->ring_head != c->ring_tail) {
+    const DevJob& J = a.jobs[a.ring[c->ring_head % kMaxJobs]];
+    if (J.end <= c->work_next) c->ring_head++;
+    else break;
+  }
+}
+
+// Job holding work index w: the last ring entry whose begin <= w.
+__device__ __forceinline__ int find_job(const PtKernelArgs& a, unsigned long long w, unsigned int head,
+                                        unsigned int tail) {
+  unsigned int lo = 0, hi = tail - head;  // search [lo, hi)
+  while (hi - lo > 1) {
+    unsigned int mid = (lo + hi) >> 1;
+    if (a.jobs[a.ring[(head + mid) % kMaxJobs]].begin <= w) lo = mid;
+    else hi = mid;
+  }
+  return (int)a.ring[(head + lo) % kMaxJobs];
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= a.pool.P) return;
+  const PathPool& P = a.pool;
+  int st = P.state[i];
+  bool want = st == 0;
+  unsigned long long mask = __ballot(want);
+  unsigned long long w = ~0ull;
+  if (mask) {
+    int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&a.cnt->work_next, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    w = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+  }
+  bool fresh = false;
+  const unsigned int head = a.cnt->ring_head, tail = a.cnt->ring_tail;
+  if (want && w < a.cnt->work_total && tail != head && tail - head <= (unsigned)kMaxJobs) {
+    int j = find_job(a, w, head, tail);
+    if (j < 0 || j >= kMaxJobs) j = 0;
+    const DevJob& J = a.jobs[j];
+    if (w >= J.begin && w < J.end) {
+      if (J.cancelled) {
+        atomicAdd(J.skipped, 1ull);
+      } else {
+        unsigned long long local = w - J.begin;
+        int pass = (int)(local / J.per_pass);
+        unsigned long long rem = local % J.per_pass;
+        int pix = (int)(rem / (unsigned long long)J.N);
+        int n = (int)(rem % (unsigned long long)J.N);
+        const DevPtScene& S = a.scenes[J.scene];
+        int ix = S.ext_x0 + pix % S.ext_w, iy = S.ext_y0 + pix / S.ext_w;
+        Sampler sm{pixel_key(J.seed, (uint32_t)pass, ix, iy), (uint32_t)J.nu, (uint32_t)J.nv,
+                   (uint32_t)J.N, 4u * (uint32_t)J.sd, 3u * (uint32_t)J.sd};
+        float ox, oy, lu, lv;
+        sm.camera((uint32_t)n, &ox, &oy);
+        sm.lens((uint32_t)n, &lu, &lv);
+        float sx = (float)ix + ox, sy = (float)iy + oy;
+        DRay r = fire_ray(S, sx, sy, lu, lv);
+        const int Pn = P.P;
+        P.job[i] = j;
+        P.pass[i] = pass;
+        P.pix[i] = pix;
+        P.samp[i] = n;
+        P.dspec[i] = 1 << 16;  // depth 0, specular (camera ray)
+        P.sxy[i] = sx;
+        P.sxy[Pn + i] = sy;
+        P.ray[i] = r.o.x; P.ray[Pn + i] = r.o.y; P.ray[2 * Pn + i] = r.o.z;
+        P.ray[3 * Pn + i] = r.d.x; P.ray[4 * Pn + i] = r.d.y; P.ray[5 * Pn + i] = r.d.z;
+        P.ray[6 * Pn + i] = r.tmin;
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          P.thr[b * Pn + i] = 1.0f;
+          P.rad[b * Pn + i] = 0.0f;
+        }
+        P.state[i] = 1;
+        fresh = true;
+      }
+    }
+  }
+  wave_count(&a.cnt->tot_camera, fresh);
+  bool active = st == 1 || fresh;
+  unsigned int q = wave_append(&a.cnt->n_act, active);
+  if (active) P.q_act[q] = (uint32_t)i;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Runs body(scene) for every valid lane, one distinct scene at a time, with
+// the scene index made wave-uniform so the scene descriptor loads are scalar.
+template <class F>
+__device__ __forceinline__ void for_uniform_scene(bool valid, int my, F&& body) {
+  bool done = !valid;
+  while (__ballot(!done)) {
+    int s = wave_min(done ? 0x7fffffff : my);
+    if (!done && my == s) {
+      body(__builtin_amdgcn_readfirstlane(s));
+      done = true;
+    }
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
+  __shared__ int lds[trav::kShort * BLOCK];
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < n;
+  int i = valid ? (int)P.q_act[q] : 0;
+  int scene = valid ? a.jobs[P.job[i]].scene : 0;
+  for_uniform_scene(valid, scene, [&](int scene_u) {
+    {
+      const DevScene sc = a.scenes[scene_u].geo;
+      DRay r;
+      r.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
+      r.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
+      r.tmin = P.ray[6 * Pn + i];
+      r.tmax = INFINITY;
+      trav::Best best;
+      unsigned long long nv = 0, nt = 0;
+      trav::closest_hit<false>(sc, r, lds + threadIdx.x, BLOCK, best, nv, nt);
+      P.hit[i] = best.t;
+      P.hit[Pn + i] = best.b1;
+      P.hit[2 * Pn + i] = best.b2;
+      P.hprim[i] = best.prim;
+    }
+  });
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int nact = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= nact) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < nact;
+  int i = valid ? (int)P.q_act[q] : 0;
+  int jslot = valid ? P.job[i] : 0;
+  bool want_shadow = false, want_mis = false;
+  for_uniform_scene(valid, valid ? a.jobs[jslot].scene : 0, [&](int scene_u) {
+    {
+      const DevPtScene& S = a.scenes[scene_u];
+      const DevJob& J = a.jobs[jslot];
+      DRay ray;
+      ray.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
+      ray.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
+      ray.tmin = P.ray[6 * Pn + i];
+      ray.tmax = INFINITY;
+      int prim = P.hprim[i];
+      int ds = P.dspec[i];
+      int depth = ds & 0xffff;
+      bool spec = (ds >> 16) != 0;
+      if (prim < 0) {  // Path.hs:44-47
+        if (spec) {
+          for (int li = 0; li < S.nlights; ++li) {
+            const float* le = light_le(S, li, ray.d);
+            if (le) {
+#pragma unroll
+              for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.thr[b * Pn + i] * le[b];
+            }
+          }
+        }
+        P.state[i] = 2;
+      } else if (depth == J.md) {  // Path.hs:51
+        P.state[i] = 2;
+      } else {
+        float th = P.hit[i], hb1 = P.hit[Pn + i], hb2 = P.hit[2 * Pn + i];
+        int pix = P.pix[i], n = P.samp[i];
+        int ix = S.ext_x0 + pix % S.ext_w, iy = S.ext_y0 + pix / S.ext_w;
+        Sampler sm{pixel_key(J.seed, (uint32_t)P.pass[i], ix, iy), (uint32_t)J.nu, (uint32_t)J.nv,
+                   (uint32_t)J.N, 4u * (uint32_t)J.sd, 3u * (uint32_t)J.sd};
+        DG dgg, dgs;
+        float eps;
+        int mat, light;
+        hit_dg(S, ray, th, hb1, hb2, prim, &dgg, &dgs, &eps, &mat, &light);
+        Bsdf bsdf = make_bsdf(S, mat, dgg, dgs);
+        uint32_t d = (uint32_t)depth, un = (uint32_t)n;
+        float lNumU = sm.get1(un, 1 + 4 * d);
+        float ld1, ld2, bd1, bd2;
+        sm.get2(un, 1 + 3 * d, &ld1, &ld2);
+        float bComp = sm.get1(un, 2 + 4 * d);
+        sm.get2(un, 2 + 3 * d, &bd1, &bd2);
+        f3 wo = neg3(ray.d);
+        f3 p = bsdf.p;
+        Sp thr;
+#pragma unroll
+        for (int b = 0; b < 16; ++b) thr.v[b] = P.thr[b * Pn + i];
+        // emission seen through a specular chain (Path.hs:53, intLe with rd)
+        if (spec && light >= 0 && dot3(dgg.n, ray.d) > 0.0f) {
+          const float* L = S.lights[light].L;
+#pragma unroll
+          for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += thr.v[b] * L[b];
+        }
+        int lc = S.nlights;
+        if (lc > 0) {  // sampleOneLight (Scene.hs:110-118)
+          int ln = 0;
+          float lscale = 1.0f;
+          if (lc > 1) {
+            ln = (int)floorf(lNumU * (float)lc);
+            ln = ln < lc - 1 ? ln : lc - 1;
+            lscale = (float)lc;
+          }
+          {  // sampleLightMis (Scene.hs:61-69)
+            LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
+            Sp f;
+            eval_bsdf(bsdf, wo, ls.wi, f);
+            bool liBlack = true;
+            if (ls.li) {
+#pragma unroll
+              for (int b = 0; b < 16; ++b) liBlack = liBlack && ls.li[b] == 0.0f;
+            }
+            if (!(ls.pdf == 0.0f || liBlack || sp_black(f))) {
+              float w = power_heuristic(ls.pdf, bsdf_pdf(bsdf, wo, ls.wi));
+              float k = w / ls.pdf;
+#pragma unroll
+              for (int b = 0; b < 16; ++b) P.pend_s[b * Pn + i] = thr.v[b] * (((f.v[b] * ls.li[b]) * k) * lscale);
+              P.sray[i] = ls.ray.o.x; P.sray[Pn + i] = ls.ray.o.y; P.sray[2 * Pn + i] = ls.ray.o.z;
+              P.sray[3 * Pn + i] = ls.ray.d.x; P.sray[4 * Pn + i] = ls.ray.d.y; P.sray[5 * Pn + i] = ls.ray.d.z;
+              P.sray[6 * Pn + i] = ls.ray.tmin; P.sray[7 * Pn + i] = ls.ray.tmax;
+              want_shadow = true;
+            }
+          }
+          {  // sampleBsdfMis (Scene.hs:71-82)
+            Sp fb;
+            f3 wib;
+            int tb;
+            float bpdf = sample_bsdf(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
+            if (!(bpdf == 0.0f || sp_black(fb))) {
+              float lpdf = light_pdf(S, ln, p, wib);
+              float w = power_heuristic(bpdf, lpdf);
+#pragma unroll
+              for (int b = 0; b < 16; ++b) P.pend_m[b * Pn + i] = thr.v[b] * ((fb.v[b] * w) * lscale);
+              P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
+              P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
+              P.mray[6 * Pn + i] = eps;
+              P.mlight[i] = ln;
+              want_mis = true;
+            }
+          }
+        }
+        // Russian roulette (Path.hs:70-72)
+        float pc = depth <= J.rr ? 1.0f : hmin(0.75f, sp_y(thr));
+        float x = sm.get1(un, 3 + 4 * d);
+        bool cont = !(x > pc);
+        if (cont) {
+          float uc = sm.get1(un, 0 + 4 * d);
+          float u1, u2;
+          sm.get2(un, 0 + 3 * d, &u1, &u2);
+          Sp f;
+          f3 wi;
+          int tp;
+          float spdf = sample_bsdf(bsdf, wo, uc, u1, u2, f, &wi, &tp);
+          if (spdf == 0.0f || sp_black(f)) {
+            cont = false;
+          } else {
+            float ipc = 1.0f / pc;
+#pragma unroll
+            for (int b = 0; b < 16; ++b) P.thr[b * Pn + i] = (f.v[b] * thr.v[b]) * ipc;
+            P.ray[i] = p.x; P.ray[Pn + i] = p.y; P.ray[2 * Pn + i] = p.z;
+            P.ray[3 * Pn + i] = wi.x; P.ray[4 * Pn + i] = wi.y; P.ray[5 * Pn + i] = wi.z;
+            P.ray[6 * Pn + i] = eps;
+            P.dspec[i] = (depth + 1) | ((tp & BX_S) ? (1 << 16) : 0);
+          }
+        }
+        P.state[i] = cont ? 1 : 2;
+      }
+    }
+  });
+  unsigned int qs = wave_append(&a.cnt->n_shadow, want_shadow);
+  if (want_shadow) P.q_shadow[qs] = (uint32_t)i;
+  unsigned int qm = wave_append(&a.cnt->n_mis, want_mis);
+  if (want_mis) P.q_mis[qm] = (uint32_t)i;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
+  __shared__ int lds[trav::kShort * BLOCK];
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_shadow;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < n;
+  int i = valid ? (int)P.q_shadow[q] : 0;
+  for_uniform_scene(valid, valid ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
+    {
+      const DevScene sc = a.scenes[scene_u].geo;
+      DRay r;
+      r.o = mk3(P.sray[i], P.sray[Pn + i], P.sray[2 * Pn + i]);
+      r.d = mk3(P.sray[3 * Pn + i], P.sray[4 * Pn + i], P.sray[5 * Pn + i]);
+      r.tmin = P.sray[6 * Pn + i];
+      r.tmax = P.sray[7 * Pn + i];
+      unsigned long long nv = 0, nt = 0;
+      if (!trav::any_hit<false>(sc, r, lds + threadIdx.x, BLOCK, nv, nt)) {  // occluded (Scene.hs:45-47)
+#pragma unroll
+        for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.pend_s[b * Pn + i];
+      }
+    }
+  });
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
+  __shared__ int lds[trav::kShort * BLOCK];
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_mis;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < n;
+  int i = valid ? (int)P.q_mis[q] : 0;
+  for_uniform_scene(valid, valid ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
+    {
+      const DevPtScene& S = a.scenes[scene_u];
+      const DevScene sc = S.geo;
+      DRay r;
+      r.o = mk3(P.mray[i], P.mray[Pn + i], P.mray[2 * Pn + i]);
+      r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
+      r.tmin = P.mray[6 * Pn + i];
+      r.tmax = INFINITY;
+      trav::Best best;
+      unsigned long long nv = 0, nt = 0;
+      trav::closest_hit<false>(sc, r, lds + threadIdx.x, BLOCK, best, nv, nt);
+      int ln = P.mlight[i];
+      const DLight& L = S.lights[ln];
+      const float* e = nullptr;
+      if (best.prim >= 0) {
+        // the hit primitive's light must be the sampled AreaLight (Light.hs:47-50)
+        if (best.prim >= S.ntri && L.type == LT_AREA) {
+          int hl = sc.shapes[best.prim - S.ntri].light;
+          if (hl >= 0 && S.lights[hl].id == L.id) {
+            DG dgg, dgs;
+            float eps;
+            int mat, light;
+            hit_dg(S, r, best.t, best.b1, best.b2, best.prim, &dgg, &dgs, &eps, &mat, &light);
+            if (dot3(dgg.n, neg3(r.d)) > 0.0f) e = S.lights[hl].L;  // lEmit with -wi
+          }
+        }
+      } else {
+        e = light_le(S, ln, r.d);
+      }
+      if (e) {
+#pragma unroll
+        for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.pend_m[b * Pn + i] * e[b];
+      }
+    }
+  });
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_act;
+  if ((unsigned)q >= n) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  int i = (int)P.q_act[q];
+  if (P.state[i] != 2) return;
+  const DevJob& J = a.jobs[P.job[i]];
+  if (!J.cancelled) {
+    const DevPtScene& S = a.scenes[J.scene];
+    Sp L;
+    bool bad = false;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      L.v[b] = P.rad[b * Pn + i];
+      bad = bad || isnan(L.v[b]) || isinf(L.v[b]);
+    }
+    if (!bad) {  // addSample (Image.hs:250-299) on the full image
+      float X = 0.0f, Y = 0.0f, Z = 0.0f;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        X = X + cCieX[b] * L.v[b];
+        Y = Y + cCieY[b] * L.v[b];
+        Z = Z + cCieZ[b] * L.v[b];
+      }
+      X = X / BGPU_CIE_Y_SUM;
+      Y = Y / BGPU_CIE_Y_SUM;
+      Z = Z / BGPU_CIE_Y_SUM;
+      float sx = P.sxy[i], sy = P.sxy[Pn + i];
+      float fw = S.fw, fh = S.fh, ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
+      float dx = sx - 0.5f, dy = sy - 0.5f;
+      int x0 = (int)ceilf(dx - fw), x1 = (int)floorf(dx + fw);
+      int y0 = (int)ceilf(dy - fh), y1 = (int)floorf(dy + fh);
+      x0 = x0 < 0 ? 0 : x0;
+      y0 = y0 < 0 ? 0 : y0;
+      x1 = x1 > S.film_w - 1 ? S.film_w - 1 : x1;
+      y1 = y1 > S.film_h - 1 ? S.film_h - 1 : y1;
+      for (int y = y0; y <= y1; ++y) {
+        int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
+        fy = fy < 15 ? fy : 15;
+        for (int x = x0; x <= x1; ++x) {
+          int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
+          fx = fx < 15 ? fx : 15;
+          float w = S.filter[fy * 16 + fx];
+          float* o = J.wxyz + 4 * ((size_t)y * S.film_w + x);
+          atomicAdd(o, w);
+          atomicAdd(o + 1, X * w);
+          atomicAdd(o + 2, Y * w);
+          atomicAdd(o + 3, Z * w);
+          if (J.sq) {
+            float* s2 = J.sq + 3 * ((size_t)y * S.film_w + x);
+            atomicAdd(s2, X * X * w);
+            atomicAdd(s2 + 1, Y * Y * w);
+            atomicAdd(s2 + 2, Z * Z * w);
+          }
+        }
+      }
+    }
+  }
+  atomicAdd(J.pass_done + P.pass[i], 1u);
+  P.state[i] = 0;
+}
+
+}  // namespace
+
+hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s) {
+  static bool tables = false;
+  if (!tables) {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cCieX), BGPU_CIE_X, sizeof(BGPU_CIE_X));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cCieY), BGPU_CIE_Y, sizeof(BGPU_CIE_Y));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cCieZ), BGPU_CIE_Z, sizeof(BGPU_CIE_Z));
+    if (e != hipSuccess) return e;
+    tables = true;
+  }
+  int P = a.pool.P;
+  hipLaunchKernelGGL(k_init, dim3((P + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, s, a, n);
+  return hipGetLastError();
+}
+
+template <int BLOCK>
+static hipError_t enqueue_iter(const PtKernelArgs& a, hipStream_t s) {
+  dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_generate<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_extend<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_shade<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_shadow<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_mis<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_finish<BLOCK>, g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, int block, hipStream_t s) {
+  switch (block) {
+    case 64: return enqueue_iter<64>(a, s);
+    case 256: return enqueue_iter<256>(a, s);
+    default: return enqueue_iter<128>(a, s);
+  }
+}
+
+}  // namespace bgpu

@@ -1,0 +1,873 @@
+// Device shading for the path tracer (gfx950).  Restates, per path vertex:
+//   Integrator/Path.hs:41-87, Scene.hs:61-118 (NEE with MIS), Reflection.hs:201-335,
+//   Reflection/{Diffuse,Specular,Microfacet}.hs, Fresnel.hs:31-69, Material.hs:32-96,
+//   Light.hs:72-229, Shape.hs:314-410, Montecarlo.hs:40-186, Camera.hs:49-68,
+//   DifferentialGeometry.hs:41-81, Primitive/TriangleMesh.hs:202-287.
+// Spectra are 16 binary32 bands in registers; material and light spectra are
+// read from HBM where they are used.
+#pragma once
+#include "../../include/bgpu.h"
+#include "pt.h"
+#include "spectral_tables.h"
+
+namespace bgpu {
+namespace sh {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 2.0f * kPi;
+constexpr float kInvPi = 1.0f / kPi;
+constexpr float kInvTwoPi = 1.0f / (2.0f * kPi);
+
+__constant__ float cCieX[16], cCieY[16], cCieZ[16];
+
+// ------------------------------------------------------------- sampler
+// Counter-based stratified sampler (spec shared with the oracle): splitmix64
+// finaliser, keyed cycle-walking permutation standing in for `shuffle`.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+__device__ __forceinline__ uint64_t pixel_key(uint64_t seed, uint32_t pass, int ix, int iy) {
+  uint64_t c = ((uint64_t)pass << 42) | ((uint64_t)(uint32_t)(iy + 1024) << 21) |
+               (uint64_t)(uint32_t)(ix + 1024);
+  return mix64(mix64(seed) ^ mix64(c + 0x9e3779b97f4a7c15ULL));
+}
+__device__ __forceinline__ uint64_t rng_bits(uint64_t key, uint32_t dim, uint32_t idx) {
+  return mix64(key + 0x9e3779b97f4a7c15ULL * (((uint64_t)dim << 24) + (uint64_t)idx + 1ULL));
+}
+__device__ __forceinline__ float uni(uint64_t b) { return (float)(uint32_t)(b >> 40) * 0x1p-24f; }
+__device__ __forceinline__ uint32_t permute(uint32_t i, uint32_t n, uint32_t p) {
+  if (n <= 1) return 0;
+  uint32_t w = n - 1;
+  w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+  do {
+    i = (i ^ p) & w;
+    i = (i * 0x9e3779b1u) & w;
+    i ^= i >> 5;
+    i ^= (i << 7) & w;
+    i = (i * 0x85ebca77u) & w;
+    i ^= (p >> 16) & w;
+    i ^= i >> 9;
+    i = (i * 0xc2b2ae3du) & w;
+    i ^= i >> 11;
+  } while (i >= n);
+  return i;
+}
+enum : uint32_t {
+  D_CAM_X = 1, D_CAM_Y = 2, D_LENS_X = 3, D_LENS_Y = 4, D_LENS_P = 5,
+  D_T1_J = 0x1000, D_T1_P = 0x2000, D_T2_JX = 0x3000, D_T2_JY = 0x4000, D_T2_P = 0x5000,
+  D_R1 = 0x6000, D_R2X = 0x7000, D_R2Y = 0x8000
+};
+constexpr float kAlmostOne = 0x1.fffffep-1f;
+
+struct Sampler {
+  uint64_t key;
+  uint32_t nu, nv, N, n1d, n2d;
+  __device__ __forceinline__ void strat2(uint32_t i, float ju, float jv, float* u, float* v) const {
+    float du = 1.0f / (float)nu, dv = 1.0f / (float)nv;
+    float a = ((float)(i / nu) + ju) * du, b = ((float)(i % nu) + jv) * dv;
+    *u = a < kAlmostOne ? a : kAlmostOne;
+    *v = b < kAlmostOne ? b : kAlmostOne;
+  }
+  __device__ __forceinline__ void camera(uint32_t n, float* ox, float* oy) const {
+    strat2(n, uni(rng_bits(key, D_CAM_X, n)), uni(rng_bits(key, D_CAM_Y, n)), ox, oy);
+  }
+  __device__ __forceinline__ void lens(uint32_t n, float* u, float* v) const {
+    uint32_t i = permute(n, N, (uint32_t)rng_bits(key, D_LENS_P, 0));
+    strat2(i, uni(rng_bits(key, D_LENS_X, i)), uni(rng_bits(key, D_LENS_Y, i)), u, v);
+  }
+  __device__ __forceinline__ float get1(uint32_t n, uint32_t k) const {
+    if (k < n1d) {
+      uint32_t i = permute(n, N, (uint32_t)rng_bits(key, D_T1_P + k, 0));
+      float du = 1.0f / (float)N;
+      float a = ((float)i + uni(rng_bits(key, D_T1_J + k, i))) * du;
+      return a < kAlmostOne ? a : kAlmostOne;
+    }
+    return uni(rng_bits(key, D_R1 + k, n));
+  }
+  __device__ __forceinline__ void get2(uint32_t n, uint32_t k, float* u, float* v) const {
+    if (k < n2d) {
+      uint32_t i = permute(n, N, (uint32_t)rng_bits(key, D_T2_P + k, 0));
+      strat2(i, uni(rng_bits(key, D_T2_JX + k, i)), uni(rng_bits(key, D_T2_JY + k, i)), u, v);
+      return;
+    }
+    *u = uni(rng_bits(key, D_R2X + k, n));
+    *v = uni(rng_bits(key, D_R2Y + k, n));
+  }
+};
+
+// ------------------------------------------------------------- spectra
+struct Sp {
+  float v[16];
+};
+__device__ __forceinline__ Sp sp_const(float c) {
+  Sp s;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s.v[i] = c;
+  return s;
+}
+__device__ __forceinline__ Sp sp_load(const float* p) {
+  Sp s;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s.v[i] = p[i];
+  return s;
+}
+__device__ __forceinline__ bool sp_black(const Sp& s) {
+  bool b = true;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) b = b && (s.v[i] == 0.0f);
+  return b;
+}
+__device__ __forceinline__ float sp_y(const Sp& s) {  // sY (Spectrum.hs:371-373)
+  float a = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a = a + s.v[i] * cCieY[i];
+  return a / BGPU_CIE_Y_SUM;
+}
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ float clampf_(float v, float lo, float hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+__device__ __forceinline__ float lerpf_(float t, float a, float b) { return (1.0f - t) * a + t * b; }
+__device__ __forceinline__ f3 sscale3(float f, f3 v) { return mk3(f * v.x, f * v.y, f * v.z); }
+struct Frame {
+  f3 s, t, n;
+};
+__device__ __forceinline__ Frame coord_system(f3 v) {  // Math.hs:413-425
+  if (fabsf(v.x) > fabsf(v.y)) {
+    float il = 1.0f / __builtin_sqrtf(v.x * v.x + v.z * v.z);
+    f3 v2 = mk3(-v.z * il, 0.0f, v.x * il);
+    return Frame{v2, cross3(v, v2), v};
+  }
+  float il = 1.0f / __builtin_sqrtf(v.y * v.y + v.z * v.z);
+  f3 v2 = mk3(0.0f, v.z * il, -v.y * il);
+  return Frame{v2, cross3(v, v2), v};
+}
+__device__ __forceinline__ f3 to_local(const Frame& f, f3 v) {
+  return mk3(dot3(v, f.s), dot3(v, f.t), dot3(v, f.n));
+}
+__device__ __forceinline__ f3 to_world(const Frame& f, f3 v) {
+  return mk3(f.s.x * v.x + f.t.x * v.y + f.n.x * v.z, f.s.y * v.x + f.t.y * v.y + f.n.y * v.z,
+             f.s.z * v.x + f.t.z * v.y + f.n.z * v.z);
+}
+__device__ __forceinline__ f3 set_comp(int d, float t, f3 v) {
+  if (d == 0) return mk3(t, v.y, v.z);
+  if (d == 1) return mk3(v.x, t, v.z);
+  return mk3(v.x, v.y, t);
+}
+
+// ------------------------------------------------------- differential geom
+struct DG {
+  f3 p, n, dpdu, dpdv;
+  float b1, b2;
+};
+__device__ __forceinline__ DG mk_dg(f3 p, f3 dpdu, f3 dpdv) {
+  return DG{p, norm3(cross3(dpdu, dpdv)), dpdu, dpdv, 0.0f, 0.0f};
+}
+__device__ __forceinline__ DG mk_dg1(f3 p, f3 n) {
+  Frame f = coord_system(n);
+  return DG{p, n, f.s, f.t, 0.0f, 0.0f};
+}
+
+// Object-space DG of a shape hit (Shape.hs:86-229), without u/v.
+__device__ __forceinline__ DG shape_dg(const DShape& s, const DRay& ray, float t, int axis) {
+  f3 p = ray_at(ray, t);
+  switch (s.type) {
+    case SH_BOX: {
+      float half = (s.p[axis] + s.p[3 + axis]) / 2.0f;
+      float dir = comp3(p, axis) > half ? 1.0f : -1.0f;
+      return mk_dg1(p, norm3(set_comp(axis, dir, mk3(0, 0, 0))));
+    }
+    case SH_CYL: {
+      float phimax = s.p[3];
+      f3 dpdu = mk3(-(phimax * p.y), phimax * p.x, 0.0f);
+      f3 dpdv = mk3(0.0f, 0.0f, s.p[2] - s.p[1]);
+      return mk_dg1(p, norm3(cross3(dpdu, dpdv)));
+    }
+    case SH_DISK:
+      return mk_dg1(p, mk3(0.0f, 0.0f, -1.0f));
+    case SH_QUAD:
+      return mk_dg(p, mk3(s.p[0], 0.0f, 0.0f), mk3(0.0f, s.p[1], 0.0f));
+    default: {
+      float r = s.p[0];
+      const float thetaMin = kPi, thetaMax = 0.0f, phiMax = kTwoPi;
+      float theta = acosf(clampf_(p.z / r, -1.0f, 1.0f));
+      float zr = __builtin_sqrtf(p.x * p.x + p.y * p.y);
+      float iz = 1.0f / zr;
+      float cosphi = p.x * iz, sinphi = p.y * iz;
+      f3 dpdu = mk3(-(phiMax * p.y), phiMax * p.x, 0.0f);
+      f3 dpdv = scl3(mk3(p.z * cosphi, p.z * sinphi, -(r * sinf(theta))), thetaMax - thetaMin);
+      return mk_dg(p, dpdu, dpdv);
+    }
+  }
+}
+
+// Geometric (dgg) and shading (dgs) DG at a hit, world space.
+__device__ __forceinline__ void hit_dg(const DevPtScene& S, const DRay& ray, float t, float b1, float b2,
+                                       int prim, DG* dgg, DG* dgs, float* eps, int* material,
+                                       int* light) {
+  if (prim < S.ntri) {
+    int i0 = S.tri_idx[3 * prim], i1 = S.tri_idx[3 * prim + 1], i2 = S.tri_idx[3 * prim + 2];
+    f3 p1 = mk3(S.verts[3 * i0], S.verts[3 * i0 + 1], S.verts[3 * i0 + 2]);
+    f3 p2 = mk3(S.verts[3 * i1], S.verts[3 * i1 + 1], S.verts[3 * i1 + 2]);
+    f3 p3 = mk3(S.verts[3 * i2], S.verts[3 * i2 + 1], S.verts[3 * i2 + 2]);
+    int2 info = S.tri_info[prim];
+    float uv00 = 0, uv01 = 0, uv10 = 1, uv11 = 0, uv20 = 1, uv21 = 1;
+    if ((info.y & BGPU_MESH_UVS) && S.uvs) {
+      uv00 = S.uvs[2 * i0]; uv01 = S.uvs[2 * i0 + 1];
+      uv10 = S.uvs[2 * i1]; uv11 = S.uvs[2 * i1 + 1];
+      uv20 = S.uvs[2 * i2]; uv21 = S.uvs[2 * i2 + 1];
+    }
+    f3 e1 = sub3(p2, p1), e2 = sub3(p3, p1);
+    float du1 = uv00 - uv20, du2 = uv10 - uv20, dv1 = uv01 - uv21, dv2 = uv11 - uv21;
+    f3 dp1 = sub3(p1, p3), dp2 = sub3(p2, p3);
+    float det = du1 * dv2 - dv1 * du2;
+    f3 dpdu, dpdv;
+    if (det == 0.0f) {
+      Frame f = coord_system(norm3(cross3(e1, e2)));
+      dpdu = f.s;
+      dpdv = f.t;
+    } else {
+      float id = 1.0f / det;
+      dpdu = sscale3(id, sub3(sscale3(dv2, dp1), sscale3(dv1, dp2)));
+      dpdv = sscale3(id, add3(sscale3(-du2, dp1), sscale3(du1, dp2)));
+    }
+    *dgg = mk_dg(ray_at(ray, t), dpdu, dpdv);
+    dgg->b1 = b1;
+    dgg->b2 = b2;
+    *dgs = *dgg;
+    if ((info.y & BGPU_MESH_NORMALS) && S.normals) {  // TriangleMesh.hs:202-214
+      f3 n0 = mk3(S.normals[3 * i0], S.normals[3 * i0 + 1], S.normals[3 * i0 + 2]);
+      f3 n1 = mk3(S.normals[3 * i1], S.normals[3 * i1 + 1], S.normals[3 * i1 + 2]);
+      f3 n2 = mk3(S.normals[3 * i2], S.normals[3 * i2 + 1], S.normals[3 * i2 + 2]);
+      float b0 = 1.0f - b1 - b2;
+      f3 nsp = add3(add3(sscale3(b0, n0), sscale3(b1, n1)), sscale3(b2, n2));
+      // transNormal by the identity: (1*x + 0*y) + 0*z per component
+      nsp = mk3(1.0f * nsp.x + 0.0f * nsp.y + 0.0f * nsp.z, 0.0f * nsp.x + 1.0f * nsp.y + 0.0f * nsp.z,
+                0.0f * nsp.x + 0.0f * nsp.y + 1.0f * nsp.z);
+      f3 ns = norm3(nsp);
+      f3 ss0 = norm3(dgg->dpdu);
+      f3 ts0 = cross3(ss0, ns);
+      dgs->n = ns;
+      if (sqlen3(ts0) > 0.0f) {
+        dgs->dpdu = cross3(norm3(ts0), ns);
+        dgs->dpdv = norm3(ts0);
+      } else {
+        Frame f = coord_system(ns);
+        dgs->dpdu = f.s;
+        dgs->dpdv = f.t;
+      }
+    }
+    *eps = 1e-3f * t;
+    *material = info.x;
+    *light = -1;
+    return;
+  }
+  int si = prim - S.ntri;
+  const DShape& s = S.geo.shapes[si];
+  const float* o2w = S.shapex[si].o2w;
+  DRay ro;
+  ro.o = xform_point(s.w2o, ray.o);
+  ro.d = xform_vector(s.w2o, ray.d);
+  ro.tmin = ray.tmin;
+  ro.tmax = ray.tmax;
+  int axis = 0;
+  if (s.type == SH_BOX) {  // recover the entry face axis (Shape.hs:100-111)
+    float nearT = -INFINITY, farT = INFINITY;
+    int dd = 0;
+#pragma unroll
+    for (int dim = 0; dim < 3; ++dim) {
+      if (!(nearT > farT)) {
+        float dInv = 1.0f / comp3(ro.d, dim);
+        float oc = comp3(ro.o, dim);
+        float t1p = (s.p[3 + dim] - oc) * dInv, t2p = (s.p[dim] - oc) * dInv;
+        float ta = t1p > t2p ? t2p : t1p, tb = t1p > t2p ? t1p : t2p;
+        int ndd = (nearT < ta) ? dim : dd;
+        nearT = hmax(nearT, ta);
+        farT = hmin(farT, tb);
+        dd = ndd;
+      }
+    }
+    axis = dd;
+  }
+  DG d = shape_dg(s, ro, t, axis);
+  // transDg o2w (DifferentialGeometry.hs:72-81); the normal uses the inverse (w2o)
+  dgg->p = xform_point(o2w, d.p);
+  dgg->n = norm3(xform_normal(s.w2o, d.n));
+  dgg->dpdu = xform_vector(o2w, d.dpdu);
+  dgg->dpdv = xform_vector(o2w, d.dpdv);
+  dgg->b1 = dgg->b2 = 0.0f;
+  *dgs = *dgg;
+  *eps = 5e-4f * t;
+  *material = s.material;
+  *light = s.light;
+}
+
+// ------------------------------------------------------------- warps
+__device__ __forceinline__ float power_heuristic(float fp, float gp) {
+  float f = 1.0f * fp, g = 1.0f * gp;
+  return (f * f) / (f * f + g * g);
+}
+__device__ __forceinline__ void concentric_disk(float u1, float u2, float* ox, float* oy) {
+  float sx = u1 * 2.0f - 1.0f, sy = u2 * 2.0f - 1.0f;
+  if (sx == 0.0f && sy == 0.0f) {
+    *ox = 0.0f;
+    *oy = 0.0f;
+    return;
+  }
+  float r, th;
+  if (sx >= -sy) {
+    if (sx > sy) {
+      r = sx;
+      th = sy > 0.0f ? sy / sx : 8.0f + sy / sx;
+    } else {
+      r = sy;
+      th = 2.0f - sx / sy;
+    }
+  } else if (sx <= sy) {
+    r = -sx;
+    th = 4.0f - sy / (-sx);
+  } else {
+    r = -sy;
+    th = 6.0f + sx / (-sy);
+  }
+  float theta = th * kPi / 4.0f;
+  *ox = r * cosf(theta);
+  *oy = r * sinf(theta);
+}
+__device__ __forceinline__ f3 cos_hemisphere(float u1, float u2) {
+  float x, y;
+  concentric_disk(u1, u2, &x, &y);
+  return mk3(x, y, __builtin_sqrtf(hmax(0.0f, 1.0f - x * x - y * y)));
+}
+__device__ __forceinline__ f3 uniform_sphere(float u1, float u2) {
+  float u = u1 * 2.0f - 1.0f;
+  float s = __builtin_sqrtf(1.0f - u * u);
+  float om = u2 * 2.0f * kPi;
+  return mk3(s * cosf(om), s * sinf(om), u);
+}
+
+// ---------------------------------------------------------------- BSDF
+struct Bsdf {
+  const DMaterial* m;
+  Frame cs;
+  f3 p, ng;
+};
+__device__ __forceinline__ float cos_t(f3 v) { return v.z; }
+__device__ __forceinline__ float acos_t(f3 v) { return fabsf(v.z); }
+__device__ __forceinline__ float sin_t2(f3 v) { return hmax(0.0f, 1.0f - v.z * v.z); }
+__device__ __forceinline__ float sin_t(f3 v) { return __builtin_sqrtf(sin_t2(v)); }
+__device__ __forceinline__ bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
+
+__device__ __forceinline__ float fr_diel_scalar(float etai, float etat, float cosi) {
+  float c = hmax(0.0f, 1.0f - cosi * cosi);
+  float costp = cosi > 0.0f ? c / (etat * etat) : c * (etat * etat);
+  float cost = __builtin_sqrtf(1.0f - clampf_(costp, 0.0f, 1.0f));
+  float ac = fabsf(cosi);
+  float eta = etat / etai;
+  float rpl = eta * ac;
+  float rParl = (cost - rpl) / (cost + rpl);
+  float rpp = eta * cost;
+  float rPerp = (ac - rpp) / (ac + rpp);
+  return (rParl * rParl + rPerp * rPerp) * 0.5f;
+}
+// f := r * fresnel(cosi)  (per band)
+__device__ __forceinline__ void r_times_fresnel(const DBxdf& b, float cosi, Sp& out) {
+  if (b.fr == FR_DIEL) {
+    float f = fr_diel_scalar(b.ei, b.et, cosi);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) out.v[i] = b.r[i] * f;
+  } else if (b.fr == FR_COND) {  // Fresnel.hs:254-266
+    float ac = fabsf(cosi);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      float eta = b.eta[i], k = b.k[i];
+      float tmpF = eta * eta + k * k;
+      float tmp = tmpF * (ac * ac);
+      float ec2 = eta * (2.0f * ac);
+      float c2 = ac * ac;
+      float rPer2 = (tmpF - ec2 + c2) / (tmpF + ec2 + c2);
+      float rPar2 = (tmp - ec2 + 1.0f) / (tmp + ec2 + 1.0f);
+      out.v[i] = b.r[i] * ((rPer2 + rPar2) / 2.0f);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) out.v[i] = b.r[i] * 1.0f;
+  }
+}
+
+__device__ __forceinline__ float mf_g(f3 wo, f3 wi, f3 wh) {
+  float nh = acos_t(wh), no = acos_t(wo), ni = acos_t(wi), oh = fabsf(dot3(wo, wh));
+  return hmin(1.0f, hmin(2.0f * nh * no / oh, 2.0f * nh * ni / oh));
+}
+__device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // Diffuse.hs:431-443
+  float sinti = sin_t(wi), sinto = sin_t(wo);
+  float maxcos = 0.0f;
+  if (sinti > 1e-4f && sinto > 1e-4f) {
+    float cpi = sinti == 0.0f ? 1.0f : clampf_(wi.x / sinti, -1.0f, 1.0f);
+    float spi = sinti == 0.0f ? 0.0f : clampf_(wi.y / sinti, -1.0f, 1.0f);
+    float cpo = sinto == 0.0f ? 1.0f : clampf_(wo.x / sinto, -1.0f, 1.0f);
+    float spo = sinto == 0.0f ? 0.0f : clampf_(wo.y / sinto, -1.0f, 1.0f);
+    maxcos = hmax(0.0f, cpi * cpo + spi * spo);
+  }
+  float sina, tanb;
+  if (acos_t(wi) > acos_t(wo)) {
+    sina = sinto;
+    tanb = sinti / acos_t(wi);
+  } else {
+    sina = sinti;
+    tanb = sinto / acos_t(wo);
+  }
+  return b.A + b.B * maxcos * sina * tanb;
+}
+
+// bxdfEval b wo wi, accumulated: acc += value
+__device__ __forceinline__ void bx_eval_acc(const DBxdf& b, f3 wo, f3 wi, Sp& acc) {
+  if (b.kind == K_LAMBERT) {
+    float s = kInvPi * acos_t(wo);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + b.r[i] * s;
+  } else if (b.kind == K_OREN) {
+    float o = oren_scale(b, wo, wi), s = kInvPi * acos_t(wo);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + (b.r[i] * o) * s;
+  } else if (b.kind == K_MF) {  // Microfacet.hs:21-32
+    float costo = acos_t(wo), costi = acos_t(wi);
+    if (costi == 0.0f || costo == 0.0f) return;
+    f3 whp = add3(wi, wo);
+    if (whp.x == 0.0f && whp.y == 0.0f && whp.z == 0.0f) return;
+    f3 wh = norm3(whp);
+    if (cos_t(wh) < 0.0f) return;
+    float costh = dot3(wi, wh);
+    float D = (b.e + 2.0f) * kInvTwoPi * powf(acos_t(wh), b.e);
+    float x = D * mf_g(wo, wi, wh) / (4.0f * costi);
+    Sp rf;
+    r_times_fresnel(b, costh, rf);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + rf.v[i] * x;
+  }
+}
+__device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
+  if (b.kind == K_LAMBERT || b.kind == K_OREN) return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
+  if (b.kind == K_MF) {
+    f3 whp = add3(wo, wi);
+    if (sqlen3(whp) == 0.0f) return 0.0f;
+    f3 wh = norm3(whp);
+    if (cos_t(wh) < 0.0f) return 0.0f;
+    return (b.e + 1.0f) * powf(acos_t(wh), b.e) * kInvTwoPi / (4.0f * fabsf(dot3(wo, wh)));
+  }
+  return 0.0f;
+}
+// bxdfSample with adj = False
+__device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float u2, Sp& f, f3* wi,
+                                          float* pdf) {
+  if (b.kind == K_LAMBERT || b.kind == K_OREN) {
+    f3 w = cos_hemisphere(u1, u2);
+    if (wo.z < 0.0f) w.z = -w.z;  // toSameHemisphere
+    if (same_hemi(wo, w)) {
+      float o = b.kind == K_OREN ? oren_scale(b, wo, w) : 1.0f;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) f.v[i] = b.kind == K_OREN ? b.r[i] * o : b.r[i];
+      *wi = w;
+      *pdf = kInvPi * acos_t(w);
+    } else {
+      f = sp_const(0.0f);
+      *wi = b.kind == K_LAMBERT ? wo : w;
+      *pdf = 0.0f;
+    }
+    return;
+  }
+  if (b.kind == K_MF) {  // Microfacet.hs:42-54,175-182
+    float cost = powf(u1, 1.0f / (b.e + 1.0f));
+    float sint = __builtin_sqrtf(hmax(0.0f, 1.0f - cost * cost));
+    float phi = u2 * 2.0f * kPi;
+    f3 whp = mk3(sint * cosf(phi), sint * sinf(phi), cost);
+    float ff = powf(cost, b.e) * kInvTwoPi;
+    float d = (b.e + 2.0f) * ff, pd = (b.e + 1.0f) * ff;
+    f3 wh = cos_t(whp) < 0.0f ? neg3(whp) : whp;
+    f3 w = sub3(sscale3(2.0f * dot3(wo, wh), wh), wo);
+    float costH = dot3(wo, wh);
+    if (same_hemi(wo, w)) {
+      float fact = d * fabsf(costH) / pd * mf_g(wo, w, wh);
+      Sp rf;
+      r_times_fresnel(b, costH, rf);
+      float sc = fact / acos_t(w);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) f.v[i] = rf.v[i] * sc;
+      *wi = w;
+      *pdf = pd / (4.0f * fabsf(costH));
+    } else {
+      f = sp_const(0.0f);
+      *wi = wo;
+      *pdf = 0.0f;
+    }
+    return;
+  }
+  if (b.kind == K_SPECR) {  // Specular.hs:460-465
+    r_times_fresnel(b, cos_t(wo), f);
+    *wi = mk3(-wo.x, -wo.y, wo.z);
+    *pdf = 1.0f;
+    return;
+  }
+  // specTrans (Specular.hs:478-502)
+  bool entering = cos_t(wo) > 0.0f;
+  float ei = entering ? b.ei : b.et, et = entering ? b.et : b.ei;
+  float eta = ei / et, eta2 = eta * eta;
+  float sint2 = eta2 * sin_t2(wo);
+  if (sint2 >= 1.0f) {
+    f = sp_const(0.0f);
+    *wi = wo;
+    *pdf = 0.0f;
+    return;
+  }
+  float c = __builtin_sqrtf(hmax(0.0f, 1.0f - sint2));
+  float cost = entering ? -c : c;
+  *wi = mk3(eta * (-wo.x), eta * (-wo.y), cost);
+  float fr = fr_diel_scalar(ei, et, cost);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) f.v[i] = ((1.0f - fr) * b.r[i]) * eta2;
+  *pdf = 1.0f;
+}
+
+__device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, const DG& dgg, const DG& dgs) {
+  Bsdf b;
+  b.m = (material >= 0 && material < S.nmats) ? &S.mats[material] : nullptr;
+  f3 nn = dgs.n, sn = norm3(dgs.dpdu);
+  b.cs = Frame{sn, cross3(nn, sn), nn};
+  b.p = dgs.p;
+  b.ng = dgg.n;
+  return b;
+}
+__device__ __forceinline__ int bsdf_n(const Bsdf& b) { return b.m ? b.m->n : 0; }
+
+__device__ __forceinline__ void eval_bsdf(const Bsdf& b, f3 woW, f3 wiW, Sp& f) {  // Reflection.hs:318-332
+  f = sp_const(0.0f);
+  float cosWo = dot3(woW, b.ng);
+  float side = dot3(wiW, b.ng) / cosWo;
+  if (side == 0.0f || fabsf(cosWo) < 1e-5f) return;
+  int need = side < 0.0f ? BX_T : BX_R;
+  f3 wo = to_local(b.cs, woW), wi = to_local(b.cs, wiW);
+  int n = bsdf_n(b);
+  for (int i = 0; i < n; ++i)
+    if (b.m->c[i].type & need) bx_eval_acc(b.m->c[i], wi, wo, f);
+}
+__device__ __forceinline__ float bsdf_pdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:251-257
+  int n = bsdf_n(b);
+  if (n == 0) return 0.0f;
+  f3 wo = to_local(b.cs, woW), wi = to_local(b.cs, wiW);
+  float s = 0.0f;
+  for (int i = 0; i < n; ++i) s = s + bx_pdf(b.m->c[i], wo, wi);
+  return s / (float)n;
+}
+// sampleBsdf (Reflection.hs:278-316), adj = False, all flags; returns pdf.
+__device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp, float u1, float u2,
+                                             Sp& f, f3* wiW, int* type) {
+  f = sp_const(0.0f);
+  *wiW = mk3(0.0f, 1.0f, 0.0f);
+  *type = BX_R | BX_D;
+  int cnt = bsdf_n(b);
+  if (cnt == 0) return 0.0f;
+  f3 wo = to_local(b.cs, woW);
+  float cntf = (float)cnt, invCnt = 1.0f / cntf;
+  int sNum = (int)floorf(uComp * cntf);
+  sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
+  sNum = sNum < 0 ? 0 : sNum;
+  const DBxdf& bx = b.m->c[sNum];
+  Sp fs;
+  f3 wi;
+  float pdfp;
+  bx_sample(bx, wo, u1, u2, fs, &wi, &pdfp);
+  f3 ww = to_world(b.cs, wi);
+  float side = dot3(ww, b.ng) / dot3(woW, b.ng);
+  if (pdfp == 0.0f || side == 0.0f) return 0.0f;
+  int need = side < 0.0f ? BX_T : BX_R;
+  if (!(bx.type & need)) return 0.0f;
+  *wiW = ww;
+  *type = bx.type;
+  if (bx.type & BX_S) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) f.v[i] = fs.v[i] * cntf;
+    return pdfp * invCnt;
+  }
+  if (cnt == 1) {
+    f = fs;
+    return pdfp;
+  }
+  float other = 0.0f;
+  Sp fo = sp_const(0.0f);
+  for (int i = 0; i < cnt; ++i) {
+    if (i == sNum) continue;
+    other = other + bx_pdf(b.m->c[i], wo, wi);
+    if (b.m->c[i].type & need) bx_eval_acc(b.m->c[i], wi, wo, fo);
+  }
+  float pdf = (pdfp + other) * invCnt;
+  float ip = 1.0f / pdf;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) f.v[i] = (fs.v[i] * pdfp + fo.v[i]) * ip;
+  return pdf;
+}
+
+// -------------------------------------------------------------- lights
+__device__ __forceinline__ float shape_area(const DShape& s) {  // Shape.hs:314-328
+  switch (s.type) {
+    case SH_BOX: {
+      float h = s.p[3] - s.p[0], w = s.p[4] - s.p[1], l = s.p[5] - s.p[2];
+      return 2.0f * (h * w + h * l + w * l);
+    }
+    case SH_CYL: return 2.0f * kPi * s.p[0] * (s.p[2] - s.p[1]);
+    case SH_DISK: return kPi * (s.p[1] * s.p[1] - s.p[2] * s.p[2]);
+    case SH_QUAD: return 4.0f * s.p[0] * s.p[1];
+    default: return s.p[0] * s.p[0] * 4.0f * kPi;
+  }
+}
+__device__ __forceinline__ float cone_pdf(float c) { return c >= 1.0f ? 0.0f : 1.0f / (kTwoPi * (1.0f - c)); }
+__device__ __forceinline__ float general_pdf(const DShape& s, f3 p, f3 wi) {  // Shape.hs:346-350
+  DRay r;
+  r.o = p;
+  r.d = wi;
+  r.tmin = 1e-3f;
+  r.tmax = INFINITY;
+  float t;
+  int ax;
+  if (!shape_intersect(s, r, &t, &ax)) return 0.0f;
+  DG d = shape_dg(s, r, t, ax);
+  float pd = sqlen3(sub3(p, ray_at(r, t))) / (fabsf(dot3(d.n, neg3(wi))) * shape_area(s));
+  return isinf(pd) ? 0.0f : pd;
+}
+__device__ __forceinline__ float shape_pdf(const DShape& s, f3 p, f3 wi) {  // Shape.hs:333-344
+  if (s.type == SH_SPHERE && !(sqlen3(p) - s.p[0] * s.p[0] < 1e-4f)) {
+    float r = s.p[0];
+    float st2 = r * r / sqlen3(p);
+    return cone_pdf(__builtin_sqrtf(hmax(0.0f, 1.0f - st2)));
+  }
+  return general_pdf(s, p, wi);
+}
+__device__ __forceinline__ void remap_rand(int segs, float u, int* seg, float* u2) {
+  float fs = (float)segs;
+  int sg = (int)floorf(u * fs);
+  sg = segs - 1 < sg ? segs - 1 : sg;
+  *seg = sg;
+  *u2 = (u - (float)sg / fs) * fs;
+}
+__device__ __forceinline__ void sample_shape(const DShape& s, f3 p, float u1, float u2, f3* ps, f3* ns) {
+  switch (s.type) {
+    case SH_SPHERE: {
+      float r = s.p[0];
+      if (!(sqlen3(p) - r * r < 1e-4f)) {  // Shape.hs:364-375
+        f3 dn = norm3(neg3(p));
+        Frame cs = coord_system(dn);
+        float ctm = __builtin_sqrtf(hmax(0.0f, 1.0f - (r * r) / sqlen3(p)));
+        float ct = lerpf_(u1, ctm, 1.0f);
+        float st = __builtin_sqrtf(1.0f - ct * ct);
+        float phi = u2 * kTwoPi;
+        f3 d = add3(add3(scl3(cs.s, cosf(phi) * st), scl3(cs.t, sinf(phi) * st)), scl3(cs.n, ct));
+        DRay ray;
+        ray.o = p;
+        ray.d = d;
+        ray.tmin = 0.0f;
+        ray.tmax = INFINITY;
+        float t;
+        int ax;
+        f3 q = shape_intersect(s, ray, &t, &ax) ? ray_at(ray, t) : scl3(dn, r);
+        *ps = q;
+        *ns = norm3(q);
+        return;
+      }
+      f3 q = uniform_sphere(u1, u2);
+      *ps = scl3(q, r);
+      *ns = q;
+      return;
+    }
+    case SH_BOX: {
+      int axis, nf;
+      float a, c;
+      remap_rand(3, u1, &axis, &a);
+      remap_rand(2, u2, &nf, &c);
+      *ns = set_comp(axis, (float)nf * 2.0f - 1.0f, mk3(0, 0, 0));
+      int oa0 = (axis + 1) % 3, oa1 = (axis + 2) % 3;
+      f3 pmin = mk3(s.p[0], s.p[1], s.p[2]), pmax = mk3(s.p[3], s.p[4], s.p[5]);
+      f3 base = nf == 0 ? pmin : pmax;
+      base = set_comp(oa1, lerpf_(c, comp3(pmin, oa1), comp3(pmax, oa1)), base);
+      *ps = set_comp(oa0, lerpf_(a, comp3(pmin, oa0), comp3(pmax, oa0)), base);
+      return;
+    }
+    case SH_CYL: {
+      float z = lerpf_(u1, s.p[1], s.p[2]), phi = lerpf_(u2, 0.0f, kTwoPi);
+      *ps = mk3(s.p[0] * cosf(phi), s.p[0] * sinf(phi), z);
+      *ns = norm3(mk3(ps->x, ps->y, 0.0f));
+      return;
+    }
+    case SH_DISK: {
+      float r = lerpf_(u1, s.p[2], s.p[1]), phi = lerpf_(u2, 0.0f, s.p[3]);
+      *ps = mk3(r * cosf(phi), r * sinf(phi), s.p[0]);
+      *ns = mk3(0.0f, 0.0f, -1.0f);
+      return;
+    }
+    default:
+      *ps = mk3(lerpf_(u1, -s.p[0], s.p[0]), lerpf_(u2, -s.p[1], s.p[1]), 0.0f);
+      *ns = mk3(0.0f, 0.0f, -1.0f);
+      return;
+  }
+}
+
+// Dist1D arrays at `d`: n, func[n], cdf[n+1], fi ; Dist2D: nu, nv, marginal, then nv conditionals.
+__device__ __forceinline__ int d1_upper(const float* cdf, int len, float u) {
+  int lo = 0, hi = len;  // first index with cdf >= u (cdf is non-decreasing)
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cdf[mid] >= u) hi = mid;
+    else lo = mid + 1;
+  }
+  int r = lo == len ? len - 1 : lo - 1;
+  r = r < 0 ? 0 : r;
+  return r < len - 2 ? r : len - 2;
+}
+__device__ __forceinline__ void d1_sample(const float* d, float u, float* x, float* pdf, int* off) {
+  int n = __float_as_int(d[0]);
+  const float* func = d + 1;
+  const float* cdf = d + 1 + n;
+  float fi = d[2 + 2 * n];
+  int o = d1_upper(cdf, n + 1, u);
+  *pdf = fi == 0.0f ? 0.0f : func[o] / fi;
+  float du = (u - cdf[o]) / (cdf[o + 1] - cdf[o]);
+  *x = ((float)o + du) / (float)n;
+  *off = o;
+}
+__device__ __forceinline__ const float* d2_cond(const float* d, int v) {
+  int nu = __float_as_int(d[0]), nv = __float_as_int(d[1]);
+  const float* marg = d + 2;
+  const float* first = marg + 3 + 2 * nv;
+  return first + (size_t)v * (3 + 2 * nu);
+}
+__device__ __forceinline__ const float* tex_px(const DevPtScene& S, const DLight& l, float u, float v) {
+  int px = (int)floorf(u * (float)l.w) % l.w;  // mod' (Texture.hs:89-92)
+  px = px < 0 ? px + l.w : px;
+  int py = (int)floorf((-v) * (float)l.h) % l.h;
+  py = py < 0 ? py + l.h : py;
+  px = px < 0 ? 0 : (px >= l.w ? l.w - 1 : px);  // non-finite coordinates
+  py = py < 0 ? 0 : (py >= l.h ? l.h - 1 : py);
+  return S.maps + l.map_off + 16 * ((size_t)py * l.w + px);
+}
+
+struct LSample {
+  const float* li;  // radiance bands in HBM, nullptr when black
+  f3 wi;
+  DRay ray;
+  float pdf;
+};
+__device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 pW, float e, float u1,
+                                                float u2) {
+  const DLight& l = S.lights[li];
+  LSample ls;
+  ls.li = nullptr;
+  ls.wi = mk3(0, 1, 0);
+  ls.ray.o = mk3(0, 0, 0);
+  ls.ray.d = mk3(0, 1, 0);
+  ls.ray.tmin = 0.0f;
+  ls.ray.tmax = 1.0f;
+  ls.pdf = 0.0f;
+  if (l.type == LT_INF) {  // Light.hs:130-141
+    const float* d = S.dist + l.dist_off;
+    float v, pdf1, u, pdf0;
+    int off, o2;
+    d1_sample(d + 2, u2, &v, &pdf1, &off);
+    d1_sample(d2_cond(d, off), u1, &u, &pdf0, &o2);
+    float mapPdf = pdf0 * pdf1;
+    if (mapPdf == 0.0f) return ls;
+    float phi = u * 2.0f * kPi, theta = v * kPi;
+    float sint = sinf(theta);
+    if (sint == 0.0f) return ls;
+    f3 dl = mk3(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
+    f3 wi = xform_vector(l.l2w, dl);
+    ls.li = tex_px(S, l, u, v);
+    ls.wi = wi;
+    ls.ray.o = pW;
+    ls.ray.d = wi;
+    ls.ray.tmin = e;
+    ls.ray.tmax = INFINITY;
+    ls.pdf = mapPdf / (2.0f * kPi * kPi * sint);
+    return ls;
+  }
+  const DShape& s = S.geo.shapes[l.shape];  // Light.hs:152-160
+  const float* l2w = S.shapex[l.shape].o2w;
+  f3 p = xform_point(s.w2o, pW);
+  f3 ps, ns;
+  sample_shape(s, p, u1, u2, &ps, &ns);
+  f3 wi = norm3(sub3(ps, p));
+  ls.li = dot3(ns, wi) < 0.0f ? l.L : nullptr;
+  ls.wi = xform_vector(l2w, wi);
+  ls.pdf = shape_pdf(s, p, wi);
+  ls.ray.o = xform_point(l2w, p);
+  ls.ray.d = xform_vector(l2w, wi);
+  ls.ray.tmin = e;
+  ls.ray.tmax = __builtin_sqrtf(sqlen3(sub3(ps, p))) - e;
+  return ls;
+}
+__device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3 wi) {  // Light.hs:215-229
+  const DLight& l = S.lights[li];
+  if (l.type == LT_INF) {
+    f3 v = xform_vector(l.w2l, wi);
+    float theta = acosf(hmax(-1.0f, hmin(1.0f, v.z)));
+    float pp = atan2f(v.y, v.x);
+    float phi = pp < 0.0f ? pp + 2.0f * kPi : pp;
+    float sint = sinf(theta);
+    if (sint == 0.0f) return 0.0f;
+    float u = phi / (2.0f * kPi), vv = theta / kPi;
+    const float* d = S.dist + l.dist_off;
+    int nu = __float_as_int(d[0]), nv = __float_as_int(d[1]);
+    const float* marg = d + 2;
+    int iu = (int)floorf(u * (float)nu);
+    iu = iu < 0 ? 0 : (iu > nu - 1 ? nu - 1 : iu);
+    int iv = (int)floorf(vv * (float)nv);
+    iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
+    const float* c = d2_cond(d, iv);
+    float mfi = marg[2 + 2 * nv], cfi = c[2 + 2 * nu];
+    if (mfi * cfi == 0.0f) return 0.0f;
+    float pd = (c[1 + iu] * marg[1 + iv]) / (cfi * mfi);
+    return pd / (2.0f * kPi * kPi * sint);
+  }
+  const DShape& s = S.geo.shapes[l.shape];
+  return shape_pdf(s, xform_point(s.w2o, p), xform_vector(s.w2o, wi));
+}
+// le light ray (Light.hs:98-106): nullptr for area lights
+__device__ __forceinline__ const float* light_le(const DevPtScene& S, int li, f3 dir) {
+  const DLight& l = S.lights[li];
+  if (l.type != LT_INF) return nullptr;
+  f3 wh = norm3(xform_vector(l.w2l, dir));
+  float theta = acosf(hmax(-1.0f, hmin(1.0f, wh.z)));
+  float pp = atan2f(wh.y, wh.x);
+  float phi = pp < 0.0f ? pp + 2.0f * kPi : pp;
+  return tex_px(S, l, phi / (2.0f * kPi), theta / kPi);
+}
+
+// ------------------------------------------------------------- camera
+__device__ __forceinline__ DRay fire_ray(const DevPtScene& S, float ix, float iy, float lu, float lv) {
+  f3 pc = xform_point(S.r2c, mk3(ix, iy, 0.0f));  // Camera.hs:49-68
+  DRay r;
+  r.o = mk3(0.0f, 0.0f, 0.0f);
+  r.d = norm3(pc);
+  r.tmin = 0.0f;
+  r.tmax = INFINITY;
+  if (S.lens_radius > 0.0f) {
+    float a, b;
+    concentric_disk(lu, lv, &a, &b);
+    f3 ro = mk3(a * S.lens_radius, b * S.lens_radius, 0.0f);
+    f3 pf = ray_at(r, S.focal_distance / r.d.z);
+    r.o = ro;
+    r.d = norm3(sub3(pf, ro));
+  }
+  DRay w;
+  w.o = xform_point(S.c2w, r.o);
+  w.d = xform_vector(S.c2w, r.d);
+  w.tmin = r.tmin;
+  w.tmax = r.tmax;
+  return w;
+}
+
+}  // namespace sh
+}  // namespace bgpu

@@ -1,0 +1,334 @@
+// Host preparation of the path tracer's per-scene data:
+//   materials  -> BxDF records (Material.hs:32-96, Microfacet.hs:127-134 fixExponent)
+//   lights     -> explicit lights, then geometric (emissive shape) lights in
+//                 shape order, the list of mkScene (Scene.hs:37-43)
+//   env maps   -> the Dist2D of mkInfiniteAreaLight (Light.hs:76-77,
+//                 Montecarlo.hs:263-317), built in binary32 in the same order
+//   film       -> 16x16 filter table (Image.hs:48-61) and sample extent (Image.hs:162-168)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bgpu_internal.h"
+#include "spectral_tables.h"
+
+namespace bgpu {
+
+namespace {
+
+inline float hmaxf(float x, float y) { return (x <= y) ? y : x; }
+inline float hminf(float x, float y) { return (x <= y) ? x : y; }
+inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+float sY(const float* s) {  // Spectrum.hs:371-373
+  float a = 0.0f;
+  for (int i = 0; i < 16; ++i) a = a + s[i] * BGPU_CIE_Y[i];
+  return a / BGPU_CIE_Y_SUM;
+}
+
+const float kPiF = 3.14159265358979323846f;
+
+float eval_filter(const bgpu_film& f, float x, float y) {  // Filter.hs:68-96
+  switch (f.filter) {
+    case BGPU_FILTER_GAUSS: {
+      float w = f.fparams[0], h = f.fparams[1], a = f.fparams[2];
+      float ex = std::exp(-a * w * w), ey = std::exp(-a * h * h);
+      return hmaxf(0.0f, std::exp(-a * x * x) - ex) * hmaxf(0.0f, std::exp(-a * y * y) - ey);
+    }
+    case BGPU_FILTER_MITCHELL: {
+      float w = f.fparams[0], h = f.fparams[1], B = f.fparams[2], C = f.fparams[3];
+      auto m1d = [&](float xp) {
+        float xx = std::fabs(2.0f * xp);
+        if (xx > 1.0f)
+          return ((-B - 6 * C) * xx * xx * xx + (6 * B + 30 * C) * xx * xx + (-12 * B - 48 * C) * xx +
+                  (8 * B + 24 * C)) * (1.0f / 6.0f);
+        return ((12 - 9 * B - 6 * C) * xx * xx * xx + (-18 + 12 * B + 6 * C) * xx * xx + (6 - 2 * B)) *
+               (1.0f / 6.0f);
+      };
+      return m1d(x * (1.0f / w)) * m1d(y * (1.0f / h));
+    }
+    case BGPU_FILTER_SINC: {
+      float ix = 1.0f / f.fparams[0], iy = 1.0f / f.fparams[1], tau = f.fparams[2];
+      auto s1 = [&](float v) {
+        if (std::fabs(v) > 1.0f) return 0.0f;
+        if (std::fabs(v) < 1e-5f) return 1.0f;
+        float xp = std::fabs(v) * kPiF;
+        return (std::sin(xp) / xp) * (std::sin(xp * tau) / (xp * tau));
+      };
+      return s1(x * ix) * s1(y * iy);
+    }
+    case BGPU_FILTER_TRIANGLE: {
+      float w = f.fparams[0], h = f.fparams[1];
+      return (hmaxf(0.0f, w - std::fabs(x * 2.0f)) * hmaxf(0.0f, h - std::fabs(y * 2.0f))) / (w * h);
+    }
+    default:
+      return (std::fabs(x) < 0.5f && std::fabs(y) < 0.5f) ? 1.0f : 0.0f;
+  }
+}
+
+// Dist1D [n, func[n], cdf[n+1], fi] (Montecarlo.hs:269-277)
+void push_dist1d(std::vector<float>& out, const std::vector<float>& f) {
+  int n = (int)f.size();
+  float nb;
+  std::memcpy(&nb, &n, 4);
+  out.push_back(nb);
+  for (float v : f) out.push_back(v);
+  std::vector<float> i(n + 1);
+  i[0] = 0.0f;
+  for (int k = 0; k < n; ++k) i[k + 1] = i[k] + f[k] / (float)n;
+  float fi = i[n];
+  for (int k = 0; k <= n; ++k) out.push_back(fi != 0.0f ? i[k] / fi : (float)k / (float)n);
+  out.push_back(fi);
+}
+
+float dist1d_fi(const std::vector<float>& f) {
+  int n = (int)f.size();
+  float acc = 0.0f;
+  for (int k = 0; k < n; ++k) acc = acc + f[k] / (float)n;
+  return acc;
+}
+
+const float* tex_px(const bgpu_light& l, float u, float v) {  // Texture.hs:94-99
+  int px = (int)std::floor(u * (float)l.width) % l.width;
+  px = px < 0 ? px + l.width : px;
+  int py = (int)std::floor((-v) * (float)l.height) % l.height;
+  py = py < 0 ? py + l.height : py;
+  return l.radiance + 16 * ((size_t)py * l.width + px);
+}
+
+DBxdf bx_zero() {
+  DBxdf b;
+  std::memset(&b, 0, sizeof(b));
+  return b;
+}
+
+}  // namespace
+
+int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
+  hipStream_t st = ctx->stream;
+  PtScene& pt = sc->pt;
+  DevPtScene& H = pt.host;
+  std::memset(&H, 0, sizeof(H));
+  const uint32_t nt = d->num_triangles, nv = d->num_vertices, ns = d->num_shapes;
+
+  // geometry for shading
+  if (nt) {
+    if (int rc = sc->tri_idx.upload(ctx, d->indices, 12ull * nt, st)) return rc;
+    if (int rc = sc->verts.upload(ctx, d->vertices, 12ull * nv, st)) return rc;
+    if (d->normals)
+      if (int rc = sc->normals.upload(ctx, d->normals, 12ull * nv, st)) return rc;
+    if (d->uvs)
+      if (int rc = sc->uvs.upload(ctx, d->uvs, 8ull * nv, st)) return rc;
+    std::vector<int32_t> info(2ull * nt);
+    uint32_t nmesh = d->num_meshes;
+    for (uint32_t t = 0; t < nt; ++t) {
+      int m = d->tri_mesh ? d->tri_mesh[t] : 0;
+      bgpu_mesh mesh = (nmesh && d->meshes) ? d->meshes[m] : bgpu_mesh{0, 0};
+      info[2 * t] = mesh.material;
+      info[2 * t + 1] = mesh.flags;
+    }
+    if (int rc = sc->tri_mat.upload(ctx, info.data(), info.size() * 4, st)) return rc;
+  }
+  std::vector<DShapeX> sx(std::max<uint32_t>(ns, 1));
+  for (uint32_t s = 0; s < ns; ++s) std::memcpy(sx[s].o2w, d->shapes[s].o2w, sizeof(sx[s].o2w));
+  if (int rc = sc->shape_x.upload(ctx, sx.data(), sx.size() * sizeof(DShapeX), st)) return rc;
+
+  // materials (Material.hs:32-96)
+  std::vector<DMaterial> mats(std::max<uint32_t>(d->num_materials, 1));
+  auto blinn_e = [](float rough) {
+    float e = 1.0f / rough;
+    return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
+  };
+  for (uint32_t i = 0; i < d->num_materials; ++i) {
+    const bgpu_material& m = d->materials[i];
+    DMaterial& o = mats[i];
+    std::memset(&o, 0, sizeof(o));
+    switch (m.type) {
+      case BGPU_MAT_MATTE: {
+        DBxdf b = bx_zero();
+        std::memcpy(b.r, m.spec[0], 64);
+        float sig = m.scalar[0];
+        if (sig == 0.0f) {
+          b.kind = K_LAMBERT;
+        } else {
+          b.kind = K_OREN;
+          float s = clampf(sig, 0.0f, 1.0f);
+          float s2 = s * s;
+          b.A = 1.0f - (s2 / (2.0f * (s2 + 0.33f)));
+          b.B = 0.45f * s2 / (s2 + 0.09f);
+        }
+        b.type = BX_R | BX_D;
+        o.c[o.n++] = b;
+        break;
+      }
+      case BGPU_MAT_PLASTIC: {
+        DBxdf dd = bx_zero();
+        dd.kind = K_LAMBERT;
+        dd.type = BX_R | BX_D;
+        std::memcpy(dd.r, m.spec[0], 64);
+        DBxdf g = bx_zero();
+        g.kind = K_MF;
+        g.type = BX_R | BX_G;
+        std::memcpy(g.r, m.spec[1], 64);
+        g.e = blinn_e(m.scalar[0]);
+        g.fr = FR_DIEL;
+        g.ei = 1.0f;
+        g.et = 1.5f;
+        o.c[o.n++] = dd;
+        o.c[o.n++] = g;
+        break;
+      }
+      case BGPU_MAT_METAL: {
+        DBxdf g = bx_zero();
+        g.kind = K_MF;
+        g.type = BX_R | BX_G;
+        for (int k = 0; k < 16; ++k) g.r[k] = 1.0f;
+        g.e = blinn_e(m.scalar[0]);
+        g.fr = FR_COND;
+        std::memcpy(g.eta, m.spec[0], 64);
+        std::memcpy(g.k, m.spec[1], 64);
+        o.c[o.n++] = g;
+        break;
+      }
+      case BGPU_MAT_GLASS: {
+        DBxdf r = bx_zero();
+        r.kind = K_SPECR;
+        r.type = BX_R | BX_S;
+        r.fr = FR_DIEL;
+        r.ei = 1.0f;
+        r.et = m.scalar[0];
+        DBxdf t = bx_zero();
+        t.kind = K_SPECT;
+        t.type = BX_T | BX_S;
+        t.ei = 1.0f;
+        t.et = m.scalar[0];
+        for (int k = 0; k < 16; ++k) {  // sClamp' (Spectrum.hs:459-461)
+          r.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[0][k]));
+          t.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[1][k]));
+        }
+        o.c[o.n++] = r;
+        o.c[o.n++] = t;
+        break;
+      }
+      case BGPU_MAT_MIRROR: {
+        DBxdf r = bx_zero();
+        r.kind = K_SPECR;
+        r.type = BX_R | BX_S;
+        r.fr = FR_NOOP;
+        for (int k = 0; k < 16; ++k) r.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[0][k]));
+        o.c[o.n++] = r;
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  if (int rc = sc->materials.upload(ctx, mats.data(), mats.size() * sizeof(DMaterial), st)) return rc;
+
+  // lights: explicit first, then emissive shapes (Scene.hs:38-42)
+  std::vector<DLight> lights;
+  std::vector<float> maps, dist;
+  for (uint32_t i = 0; i < d->num_lights; ++i) {
+    const bgpu_light& l = d->lights[i];
+    if (l.type != BGPU_LIGHT_INFINITE || l.width <= 0 || l.height <= 0 || !l.radiance)
+      return fail(BGPU_E_INVALID, "unsupported or empty explicit light");
+    DLight o;
+    std::memset(&o, 0, sizeof(o));
+    o.type = LT_INF;
+    o.shape = -1;
+    o.id = -1;
+    o.w = l.width;
+    o.h = l.height;
+    o.map_off = (int32_t)maps.size();
+    maps.insert(maps.end(), l.radiance, l.radiance + 16ull * l.width * l.height);
+    std::memcpy(o.w2l, l.w2l, 64);
+    std::memcpy(o.l2w, l.l2w, 64);
+    // mkDist2D (texSize rmap) (sY . eval), eval (x, y) at Cartesian (x/sx, y/sy)
+    o.dist_off = (int32_t)dist.size();
+    int nu = l.width, nvv = l.height;
+    float b;
+    std::memcpy(&b, &nu, 4);
+    dist.push_back(b);
+    std::memcpy(&b, &nvv, 4);
+    dist.push_back(b);
+    float fsx = (float)nu, fsy = (float)nvv;
+    std::vector<std::vector<float>> rows(nvv);
+    std::vector<float> fis(nvv);
+    for (int y = 0; y < nvv; ++y) {
+      rows[y].resize(nu);
+      for (int x = 0; x < nu; ++x) rows[y][x] = sY(tex_px(l, (float)x / fsx, (float)y / fsy));
+      fis[y] = dist1d_fi(rows[y]);
+    }
+    push_dist1d(dist, fis);
+    for (int y = 0; y < nvv; ++y) push_dist1d(dist, rows[y]);
+    lights.push_back(o);
+  }
+  pt.shape_light_host.assign(ns, -1);
+  for (uint32_t s = 0; s < ns; ++s) {
+    if (!d->shapes[s].emissive) continue;
+    DLight o;
+    std::memset(&o, 0, sizeof(o));
+    o.type = LT_AREA;
+    o.shape = (int32_t)s;
+    o.id = d->shapes[s].light_id;
+    std::memcpy(o.L, d->shapes[s].emission, 64);
+    pt.shape_light_host[s] = (int32_t)lights.size();
+    lights.push_back(o);
+  }
+  if (lights.empty()) lights.push_back(DLight{});
+  if (maps.empty()) maps.push_back(0.0f);
+  if (dist.empty()) dist.push_back(0.0f);
+  if (int rc = sc->lights.upload(ctx, lights.data(), lights.size() * sizeof(DLight), st)) return rc;
+  if (int rc = sc->light_maps.upload(ctx, maps.data(), maps.size() * 4, st)) return rc;
+  if (int rc = sc->light_dist.upload(ctx, dist.data(), dist.size() * 4, st)) return rc;
+
+  H.tri_idx = sc->tri_idx.as<int32_t>();
+  H.verts = sc->verts.as<float>();
+  H.normals = d->normals ? sc->normals.as<float>() : nullptr;
+  H.uvs = d->uvs ? sc->uvs.as<float>() : nullptr;
+  H.tri_info = sc->tri_mat.as<int2>();
+  H.shapex = sc->shape_x.as<DShapeX>();
+  H.mats = sc->materials.as<DMaterial>();
+  H.lights = sc->lights.as<DLight>();
+  H.maps = sc->light_maps.as<float>();
+  H.dist = sc->light_dist.as<float>();
+  H.ntri = (int32_t)nt;
+  H.nlights = (int32_t)(d->num_lights + std::count_if(pt.shape_light_host.begin(),
+                                                      pt.shape_light_host.end(),
+                                                      [](int32_t v) { return v >= 0; }));
+  H.nmats = (int32_t)d->num_materials;
+  H.cam_type = d->camera.type;
+  std::memcpy(H.c2w, d->camera.c2w, 64);
+  std::memcpy(H.r2c, d->camera.r2c, 64);
+  H.lens_radius = d->camera.lens_radius;
+  H.focal_distance = d->camera.focal_distance;
+  H.xres = d->camera.xres;
+  H.yres = d->camera.yres;
+  // film: filter table and the sample extent
+  const bgpu_film& f = d->film;
+  H.film_w = f.width;
+  H.film_h = f.height;
+  if (f.filter == BGPU_FILTER_BOX) {
+    H.fw = 0.5f;
+    H.fh = 0.5f;
+  } else {
+    H.fw = f.fparams[0];
+    H.fh = f.fparams[1];
+  }
+  for (int y = 0; y < 16; ++y) {
+    float fy = ((float)y + 0.5f) * H.fh / 16.0f;
+    for (int x = 0; x < 16; ++x) {
+      float fx = ((float)x + 0.5f) * H.fw / 16.0f;
+      H.filter[y * 16 + x] = eval_filter(f, fx, fy);
+    }
+  }
+  int x0 = (int)std::floor(0.5f - H.fw), x1 = (int)std::floor(0.5f + (float)f.width + H.fw);
+  int y0 = (int)std::floor(0.5f - H.fh), y1 = (int)std::floor(0.5f + (float)f.height + H.fh);
+  H.ext_x0 = x0;
+  H.ext_y0 = y0;
+  H.ext_w = x1 - x0 + 1;
+  H.ext_h = y1 - y0 + 1;
+  return BGPU_OK;
+}
+
+}  // namespace bgpu

@@ -216,21 +216,23 @@ class GpuRenderer:
                             sq.ctypes.data if sq is not None else None))
         return img, sq
 
-    def render(self, scene, report):
+    def render(self, scene, report, poll_interval=0.005):
+        """Progressive render; report(PassDone) after each completed pass,
+        stopping early when it returns False."""
+        import time
         job = Job(scene, self.spec)
         try:
             seen = 0
             while True:
-                job.wait(timeout=0.05) if False else None
-                img, _, done, state = job.poll()
+                _, _, done, state = job.poll(want_image=False)
                 if done > seen:
                     seen = done
+                    img, _, _, _ = job.poll()
                     if not report(PassDone(done, img, 1.0)):
                         job.cancel()
                         break
                 if state in (abi.JOB_DONE, abi.JOB_CANCELLED, abi.JOB_FAILED):
                     break
-                import time
-                time.sleep(0.002)
+                time.sleep(poll_interval)
         finally:
             job.release()

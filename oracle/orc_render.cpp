@@ -1,0 +1,1180 @@
+// ORACLE -- test infrastructure only (see orc_math.h).
+// CPU restatement of bling's path integrator and everything it calls, per
+// camera sample, in the reference's order of operations:
+//   Integrator/Path.hs:41-87 (nextVertex)    Scene.hs:61-118 (direct lighting, MIS)
+//   Reflection.hs:201-335 (BSDF)             Reflection/{Diffuse,Specular,Microfacet}.hs
+//   Fresnel.hs:31-69   Material.hs:32-96     Light.hs:72-229   Shape.hs:314-410
+//   Montecarlo.hs:40-186                     Camera.hs:49-68   Image.hs:40-61,162-168,250-299
+//   DifferentialGeometry.hs:41-81            Primitive/TriangleMesh.hs:202-287
+//   Rendering.hs:207-246 (passes over the sample extent)
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../bling_amd/csrc/spectral_tables.h"
+#include "orc_sampler.h"
+#include "orc_scene.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------- spectra
+using Spec = std::array<float, 16>;
+static inline Spec sconst(float c) {
+  Spec s;
+  s.fill(c);
+  return s;
+}
+static inline Spec operator+(const Spec& a, const Spec& b) {
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = a[i] + b[i];
+  return r;
+}
+static inline Spec operator-(const Spec& a, const Spec& b) {
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = a[i] - b[i];
+  return r;
+}
+static inline Spec operator*(const Spec& a, const Spec& b) {
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = a[i] * b[i];
+  return r;
+}
+static inline Spec operator/(const Spec& a, const Spec& b) {
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = a[i] / b[i];
+  return r;
+}
+static inline Spec sscale(const Spec& a, float f) {  // sScale (Spectrum.hs:448-450)
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = a[i] * f;
+  return r;
+}
+static inline bool isBlack(const Spec& s) {
+  for (float v : s)
+    if (v != 0.0f) return false;
+  return true;
+}
+static inline float sY(const Spec& s) {  // Spectrum.hs:371-373
+  float acc = 0.0f;
+  for (int i = 0; i < 16; ++i) acc = acc + s[i] * BGPU_CIE_Y[i];
+  return acc / BGPU_CIE_Y_SUM;
+}
+static inline void toXYZ(const Spec& s, float* x, float* y, float* z) {  // Spectrum.hs:349-355
+  float a = 0, b = 0, c = 0;
+  for (int i = 0; i < 16; ++i) {
+    a = a + BGPU_CIE_X[i] * s[i];
+    b = b + BGPU_CIE_Y[i] * s[i];
+    c = c + BGPU_CIE_Z[i] * s[i];
+  }
+  *x = a / BGPU_CIE_Y_SUM;
+  *y = b / BGPU_CIE_Y_SUM;
+  *z = c / BGPU_CIE_Y_SUM;
+}
+static inline Spec specOf(const float* v) {
+  Spec s;
+  for (int i = 0; i < 16; ++i) s[i] = v[i];
+  return s;
+}
+static inline Spec sclamp(const Spec& s, float lo, float hi) {  // sClamp (Spectrum.hs:453-456)
+  Spec r;
+  for (int i = 0; i < 16; ++i) r[i] = hmax(lo, hmin(hi, s[i]));
+  return r;
+}
+
+// ------------------------------------------------------- differential geom
+struct DG {
+  V3 p, n;
+  float u, v;
+  V3 dpdu, dpdv;
+  bool tri;
+  float b1, b2;
+};
+
+static DG mkDg(V3 p, float u, float v, V3 dpdu, V3 dpdv) {  // DifferentialGeometry.hs:41-51
+  return DG{p, normalize(cross(dpdu, dpdv)), u, v, dpdu, dpdv, false, 0, 0};
+}
+static DG mkDg1(V3 p, V3 n) {  // mkDg' (DifferentialGeometry.hs:53-56)
+  Frame f = coordinateSystem(n);
+  return DG{p, n, 0, 0, f.s, f.t, false, 0, 0};
+}
+static DG transDg(const M4& o2w, const M4& w2o, const DG& d) {  // DifferentialGeometry.hs:72-81
+  DG r = d;
+  r.p = transPoint(o2w, d.p);
+  r.n = normalize(transNormal(w2o, d.n));
+  r.dpdu = transVector(o2w, d.dpdu);
+  r.dpdv = transVector(o2w, d.dpdv);
+  return r;
+}
+
+// Object-space DG of a shape hit (Shape.hs:86-229).
+static DG shapeDG(const Shape& s, const Ray& ray, const ShapeHit& h) {
+  float t = h.t;
+  V3 p = rayAt(ray, t);
+  switch (s.type) {
+    case SH_BOX: {
+      int axis = h.axis;
+      float half = (s.p[axis] + s.p[3 + axis]) / 2.0f;
+      float dir = comp(p, axis) > half ? 1.0f : -1.0f;
+      V3 n = normalize(setComp(axis, dir, mk(0, 0, 0)));
+      return mkDg1(p, n);
+    }
+    case SH_CYL: {
+      float phimax = s.p[3];
+      V3 dpdu = mk(-(phimax * p.y), phimax * p.x, 0.0f);
+      V3 dpdv = mk(0.0f, 0.0f, s.p[2] - s.p[1]);
+      return mkDg1(p, normalize(cross(dpdu, dpdv)));
+    }
+    case SH_DISK:
+      return mkDg1(p, mk(0.0f, 0.0f, -1.0f));
+    case SH_QUAD: {
+      float sx = s.p[0], sy = s.p[1];
+      float u = (sx + p.x) / (2.0f * sx), v = (sy + p.y) / (2.0f * sy);
+      return mkDg(p, u, v, mk(sx, 0, 0), mk(0, sy, 0));
+    }
+    default: {  // sphere
+      float r = s.p[0];
+      const float thetaMin = kPi, thetaMax = 0.0f, phiMax = kTwoPi;
+      float phi = atan2p(p.y, p.x);
+      float u = phi / phiMax;
+      float theta = std::acos(clampf(p.z / r, -1.0f, 1.0f));
+      float v = (theta - thetaMin) / (thetaMax - thetaMin);
+      float zr = std::sqrt(p.x * p.x + p.y * p.y);
+      float iz = 1.0f / zr;
+      float cosphi = p.x * iz, sinphi = p.y * iz;
+      V3 dpdu = mk(-(phiMax * p.y), phiMax * p.x, 0.0f);
+      V3 dpdv = scale(mk(p.z * cosphi, p.z * sinphi, -(r * std::sin(theta))), thetaMax - thetaMin);
+      return mkDg(p, u, v, dpdu, dpdv);
+    }
+  }
+}
+
+// ------------------------------------------------------------ intersection
+struct Isect {
+  float t, eps;
+  int prim;
+  DG dgg, dgs;  // geometric and shading DG (world space)
+};
+
+static void triUVs(const Scene& s, int t, float* uv) {
+  const bgpu_mesh& m = s.meshes[s.tri_mesh[t]];
+  if ((m.flags & BGPU_MESH_UVS) && !s.uvs.empty()) {
+    for (int k = 0; k < 3; ++k) {
+      int o = s.idx[3 * t + k];
+      uv[2 * k] = s.uvs[2 * o];
+      uv[2 * k + 1] = s.uvs[2 * o + 1];
+    }
+  } else {  // triangleDefaultUVs (TriangleMesh.hs:199-200)
+    uv[0] = 0; uv[1] = 0; uv[2] = 1; uv[3] = 0; uv[4] = 1; uv[5] = 1;
+  }
+}
+
+static DG triDG(const Scene& s, int t, const Ray& r, float th, float b1, float b2) {
+  V3 p1, p2, p3;
+  s.tri(t, &p1, &p2, &p3);
+  float uv[6];
+  triUVs(s, t, uv);
+  float uv00 = uv[0], uv01 = uv[1], uv10 = uv[2], uv11 = uv[3], uv20 = uv[4], uv21 = uv[5];
+  V3 e1 = p2 - p1, e2 = p3 - p1;
+  V3 n = normalize(cross(e1, e2));
+  float du1 = uv00 - uv20, du2 = uv10 - uv20, dv1 = uv01 - uv21, dv2 = uv11 - uv21;
+  V3 dp1 = p1 - p3, dp2 = p2 - p3;
+  float determinant = du1 * dv2 - dv1 * du2;
+  V3 dpdu, dpdv;
+  if (determinant == 0.0f) {
+    Frame f = coordinateSystem(n);
+    dpdu = f.s;
+    dpdv = f.t;
+  } else {
+    float invDet = 1.0f / determinant;
+    dpdu = sscale(invDet, sscale(dv2, dp1) - sscale(dv1, dp2));
+    dpdv = sscale(invDet, sscale(-du2, dp1) + sscale(du1, dp2));
+  }
+  float b0 = 1.0f - b1 - b2;
+  float tu = b0 * uv00 + b1 * uv10 + b2 * uv20;
+  float tv = b0 * uv01 + b1 * uv11 + b2 * uv21;
+  DG d = mkDg(rayAt(r, th), tu, tv, dpdu, dpdv);
+  d.tri = true;
+  d.b1 = b1;
+  d.b2 = b2;
+  return d;
+}
+
+// triangleShadingGeometry with o2w = identity (TriangleMesh.hs:202-214, Primitive.hs:65)
+static DG triShading(const Scene& s, int t, const DG& dgg) {
+  V3 n0 = s.normals[s.idx[3 * t]], n1 = s.normals[s.idx[3 * t + 1]], n2 = s.normals[s.idx[3 * t + 2]];
+  float b0 = 1.0f - dgg.b1 - dgg.b2;
+  V3 nsp = sscale(b0, n0) + sscale(dgg.b1, n1) + sscale(dgg.b2, n2);
+  M4 id{{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
+  V3 ns = normalize(transNormal(id, nsp));
+  V3 ss0 = normalize(dgg.dpdu);
+  V3 ts0 = cross(ss0, ns);
+  DG r = dgg;
+  r.n = ns;
+  if (sqLen(ts0) > 0.0f) {
+    r.dpdu = cross(normalize(ts0), ns);
+    r.dpdv = normalize(ts0);
+  } else {
+    Frame f = coordinateSystem(ns);
+    r.dpdu = f.s;
+    r.dpdv = f.t;
+  }
+  return r;
+}
+
+static bool intersectFull(const Scene& s, const Ray& r, Isect* is) {
+  Hit h;
+  if (!sceneIntersect(s, r, &h, false, nullptr)) return false;
+  is->t = h.t;
+  is->eps = h.eps;
+  is->prim = h.prim;
+  if (h.prim < s.ntri) {
+    is->dgg = triDG(s, h.prim, r, h.t, h.b1, h.b2);
+    const bgpu_mesh& m = s.meshes[s.tri_mesh[h.prim]];
+    is->dgs = ((m.flags & BGPU_MESH_NORMALS) && !s.normals.empty()) ? triShading(s, h.prim, is->dgg)
+                                                                     : is->dgg;
+  } else {
+    const ShapePrim& sp = s.shapes[h.prim - s.ntri];
+    Ray ro = transRay(sp.w2o, r);
+    ShapeHit sh{h.t, h.eps, h.axis};
+    is->dgg = transDg(sp.o2w, sp.w2o, shapeDG(sp.shape, ro, sh));
+    is->dgs = is->dgg;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------- monte carlo
+static inline float powerHeuristic(float fPdf, float gPdf) {  // Montecarlo.hs:342-345
+  float f = 1.0f * fPdf, g = 1.0f * gPdf;
+  return (f * f) / (f * f + g * g);
+}
+static void concentricSampleDisk(float u1, float u2, float* ox, float* oy) {  // Montecarlo.hs:389-406
+  float sx = u1 * 2.0f - 1.0f, sy = u2 * 2.0f - 1.0f;
+  if (sx == 0.0f && sy == 0.0f) {
+    *ox = 0.0f;
+    *oy = 0.0f;
+    return;
+  }
+  float r, th;
+  if (sx >= -sy) {
+    if (sx > sy) {
+      if (sy > 0.0f) { r = sx; th = sy / sx; }
+      else { r = sx; th = 8.0f + sy / sx; }
+    } else {
+      r = sy;
+      th = 2.0f - sx / sy;
+    }
+  } else if (sx <= sy) {
+    r = -sx;
+    th = 4.0f - sy / (-sx);
+  } else {
+    r = -sy;
+    th = 6.0f + sx / (-sy);
+  }
+  float theta = th * kPi / 4.0f;
+  *ox = r * std::cos(theta);
+  *oy = r * std::sin(theta);
+}
+static V3 cosineSampleHemisphere(float u1, float u2) {  // Montecarlo.hs:375-378
+  float x, y;
+  concentricSampleDisk(u1, u2, &x, &y);
+  return mk(x, y, std::sqrt(hmax(0.0f, 1.0f - x * x - y * y)));
+}
+static V3 uniformSampleCone(const Frame& f, float cosThetaMax, float u1, float u2) {  // :362-373
+  float cosTheta = lerpf(u1, cosThetaMax, 1.0f);
+  float sinTheta = std::sqrt(1.0f - cosTheta * cosTheta);
+  float phi = u2 * kTwoPi;
+  return scale(f.s, std::cos(phi) * sinTheta) + scale(f.t, std::sin(phi) * sinTheta) +
+         scale(f.n, cosTheta);
+}
+static float uniformConePdf(float c) { return c >= 1.0f ? 0.0f : 1.0f / (kTwoPi * (1.0f - c)); }
+static V3 uniformSampleSphere(float u1, float u2) {  // Montecarlo.hs:410-415
+  float u = u1 * 2.0f - 1.0f;
+  float s = std::sqrt(1.0f - u * u);
+  float omega = u2 * 2.0f * kPi;
+  return mk(s * std::cos(omega), s * std::sin(omega), u);
+}
+
+// Dist1D / Dist2D (Montecarlo.hs:263-333)
+struct Dist1D {
+  std::vector<float> func, cdf;
+  float fi;
+};
+static Dist1D mkDist1D(const std::vector<float>& f) {
+  Dist1D d;
+  d.func = f;
+  int n = (int)f.size();
+  std::vector<float> i(n + 1);
+  i[0] = 0.0f;
+  for (int k = 0; k < n; ++k) i[k + 1] = i[k] + f[k] / (float)n;
+  d.fi = i[n];
+  d.cdf.resize(n + 1);
+  for (int k = 0; k <= n; ++k) d.cdf[k] = d.fi != 0.0f ? i[k] / d.fi : (float)k / (float)n;
+  return d;
+}
+static int upperBound(const std::vector<float>& v, float u) {
+  int len = (int)v.size(), idx = -1;
+  for (int k = 0; k < len; ++k)
+    if (v[k] >= u) { idx = k; break; }
+  int r = idx < 0 ? len - 1 : idx - 1;
+  r = r < 0 ? 0 : r;
+  return r < len - 2 ? r : len - 2;
+}
+static void sampleC1D(const Dist1D& d, float u, float* x, float* pdf, int* off) {
+  int o = upperBound(d.cdf, u);
+  *pdf = d.fi == 0.0f ? 0.0f : d.func[o] / d.fi;
+  float du = (u - d.cdf[o]) / (d.cdf[o + 1] - d.cdf[o]);
+  *x = ((float)o + du) / (float)d.func.size();
+  *off = o;
+}
+struct Dist2D {
+  std::vector<Dist1D> cond;
+  Dist1D marg;
+};
+
+// ------------------------------------------------------------------ lights
+struct LightRec {
+  int type;  // 0 area (shape index), 1 infinite
+  int shape;
+  int id;
+  Spec L;
+  // infinite
+  int w, h;
+  const float* map;
+  M4 w2l, l2w;
+  Dist2D dist;
+};
+
+static Spec texEval(const LightRec& l, float u, float v) {  // getPixel (Texture.hs:94-99)
+  auto md = [](int a, int b) { int r = a % b; return r < 0 ? r + b : r; };
+  int px = md((int)std::floor(u * (float)l.w), l.w);
+  int py = md((int)std::floor((-v) * (float)l.h), l.h);
+  return specOf(l.map + 16 * ((size_t)py * l.w + px));
+}
+
+static Dist2D buildDist(const LightRec& l) {  // mkDist2D (texSize rmap) (sY . eval)  (Light.hs:76-77)
+  Dist2D d;
+  float sx = (float)l.w, sy = (float)l.h;
+  std::vector<float> fi;
+  for (int v = 0; v < l.h; ++v) {
+    std::vector<float> row(l.w);
+    for (int u = 0; u < l.w; ++u) row[u] = sY(texEval(l, (float)u / sx, (float)v / sy));
+    d.cond.push_back(mkDist1D(row));
+    fi.push_back(d.cond.back().fi);
+  }
+  d.marg = mkDist1D(fi);
+  return d;
+}
+
+struct LightSample {
+  Spec li;
+  V3 wi;
+  Ray ray;
+  float pdf;
+};
+
+static float shapeArea(const Shape& s) {  // Shape.hs:314-328
+  switch (s.type) {
+    case SH_BOX: {
+      float h = s.p[3] - s.p[0], w = s.p[4] - s.p[1], l = s.p[5] - s.p[2];
+      return 2.0f * (h * w + h * l + w * l);
+    }
+    case SH_CYL: return 2.0f * kPi * s.p[0] * (s.p[2] - s.p[1]);
+    case SH_DISK: return kPi * (s.p[1] * s.p[1] - s.p[2] * s.p[2]);
+    case SH_QUAD: return 4.0f * s.p[0] * s.p[1];
+    default: return s.p[0] * s.p[0] * 4.0f * kPi;
+  }
+}
+static bool insideSphere(float r, V3 p) { return sqLen(p) - r * r < 1e-4f; }  // Shape.hs:330-331
+
+static float generalPdf(const Shape& s, V3 p, V3 wi) {  // Shape.hs:346-350
+  Ray r{p, wi, 1e-3f, kInf};
+  ShapeHit h;
+  if (!shapeIntersect(s, r, &h)) return 0.0f;
+  DG d = shapeDG(s, r, h);
+  float pd = sqLen(p - rayAt(r, h.t)) / (absDot(d.n, -wi) * shapeArea(s));
+  return std::isinf(pd) ? 0.0f : pd;
+}
+static float shapePdf(const Shape& s, V3 p, V3 wi) {  // Shape.hs:333-344
+  if (s.type == SH_SPHERE && !insideSphere(s.p[0], p)) {
+    float r = s.p[0];
+    float sinThetaMax2 = r * r / sqLen(p);
+    float cosThetaMax = std::sqrt(hmax(0.0f, 1.0f - sinThetaMax2));
+    return uniformConePdf(cosThetaMax);
+  }
+  return generalPdf(s, p, wi);
+}
+static void remapRand(int segs, float u, int* seg, float* u2) {  // Math.hs:113-121
+  float fs = (float)segs;
+  int sg = (int)std::floor(u * fs);
+  sg = segs - 1 < sg ? segs - 1 : sg;
+  *seg = sg;
+  *u2 = (u - (float)sg / fs) * fs;
+}
+static void sampleShape1(const Shape& s, float u1, float u2, V3* p, V3* n) {  // Shape.hs:379-409
+  switch (s.type) {
+    case SH_BOX: {
+      int axis, nf;
+      float u1p, u2p;
+      remapRand(3, u1, &axis, &u1p);
+      remapRand(2, u2, &nf, &u2p);
+      *n = setComp(axis, (float)nf * 2.0f - 1.0f, mk(0, 0, 0));
+      int oa0 = (axis + 1) % 3, oa1 = (axis + 2) % 3;
+      V3 pmin = mk(s.p[0], s.p[1], s.p[2]), pmax = mk(s.p[3], s.p[4], s.p[5]);
+      V3 base = nf == 0 ? pmin : pmax;
+      base = setComp(oa1, lerpf(u2p, comp(pmin, oa1), comp(pmax, oa1)), base);
+      *p = setComp(oa0, lerpf(u1p, comp(pmin, oa0), comp(pmax, oa0)), base);
+      return;
+    }
+    case SH_CYL: {
+      float r = s.p[0];
+      float z = lerpf(u1, s.p[1], s.p[2]);
+      float phi = lerpf(u2, 0.0f, kTwoPi);
+      *p = mk(r * std::cos(phi), r * std::sin(phi), z);
+      *n = normalize(mk(p->x, p->y, 0.0f));
+      return;
+    }
+    case SH_DISK: {
+      float r = lerpf(u1, s.p[2], s.p[1]);
+      float phi = lerpf(u2, 0.0f, s.p[3]);
+      *p = mk(r * std::cos(phi), r * std::sin(phi), s.p[0]);
+      *n = mk(0, 0, -1);
+      return;
+    }
+    case SH_QUAD:
+      *p = mk(lerpf(u1, -s.p[0], s.p[0]), lerpf(u2, -s.p[1], s.p[1]), 0.0f);
+      *n = mk(0, 0, -1);
+      return;
+    default: {
+      V3 q = uniformSampleSphere(u1, u2);
+      *p = scale(q, s.p[0]);
+      *n = q;
+      return;
+    }
+  }
+}
+static void sampleShape(const Shape& s, V3 p, float u1, float u2, V3* ps, V3* ns) {  // :362-376
+  if (s.type == SH_SPHERE && !insideSphere(s.p[0], p)) {
+    float r = s.p[0];
+    V3 dn = normalize(-p);
+    Frame cs = coordinateSystem(dn);
+    float cosThetaMax = std::sqrt(hmax(0.0f, 1.0f - (r * r) / sqLen(p)));
+    V3 d = uniformSampleCone(cs, cosThetaMax, u1, u2);
+    Ray ray{p, d, 0.0f, kInf};
+    ShapeHit h;
+    V3 q = shapeIntersect(s, ray, &h) ? rayAt(ray, h.t) : scale(dn, r);
+    *ps = q;
+    *ns = normalize(q);
+    return;
+  }
+  sampleShape1(s, u1, u2, ps, ns);
+}
+
+struct Ctx {
+  const Scene* s;
+  std::vector<LightRec> lights;
+  std::vector<int> shapeLight;  // shape index -> light index or -1
+  float filter[256];
+  float fw, fh;
+  std::atomic<uint64_t> rays{0};
+};
+
+static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, float u1, float u2) {
+  LightSample ls;
+  if (l.type == 1) {  // Light.hs:130-141
+    float uv[2], mapPdf;
+    float pdf1, pdf0;
+    int off;
+    sampleC1D(l.dist.marg, u2, &uv[1], &pdf1, &off);
+    int dummy;
+    sampleC1D(l.dist.cond[off], u1, &uv[0], &pdf0, &dummy);
+    mapPdf = pdf0 * pdf1;
+    Ray r0{mk(0, 0, 0), mk(0, 1, 0), 0.0f, 1.0f};
+    if (mapPdf == 0.0f) return LightSample{sconst(0), mk(0, 1, 0), r0, 0.0f};
+    float phi = uv[0] * 2.0f * kPi, theta = uv[1] * kPi;  // cartToSph (Types.hs:512-514)
+    float sint = std::sin(theta);
+    if (sint == 0.0f) return LightSample{sconst(0), mk(0, 1, 0), r0, 0.0f};
+    V3 dl = mk(std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta));
+    V3 wi = transVector(l.l2w, dl);
+    ls.li = texEval(l, uv[0], uv[1]);
+    ls.wi = wi;
+    ls.ray = Ray{pW, wi, e, kInf};
+    ls.pdf = mapPdf / (2.0f * kPi * kPi * sint);
+    return ls;
+  }
+  const ShapePrim& sp = c.s->shapes[l.shape];  // Light.hs:152-160
+  V3 p = transPoint(sp.w2o, pW);
+  V3 ps, ns;
+  sampleShape(sp.shape, p, u1, u2, &ps, &ns);
+  V3 wi = normalize(ps - p);
+  ls.li = dot(ns, wi) < 0.0f ? l.L : sconst(0);
+  ls.wi = transVector(sp.o2w, wi);
+  ls.pdf = shapePdf(sp.shape, p, wi);
+  Ray ray{p, wi, e, len(ps - p) - e};
+  ls.ray = transRay(sp.o2w, ray);
+  return ls;
+}
+
+static float lightPdf(const Ctx& c, const LightRec& l, V3 p, V3 wi) {  // Light.hs:215-229
+  if (l.type == 1) {
+    V3 v = transVector(l.w2l, wi);
+    float theta = std::acos(hmax(-1.0f, hmin(1.0f, v.z)));  // sphericalTheta (Math.hs:159-161)
+    float pp = std::atan2(v.y, v.x);                        // sphericalPhi (Math.hs:163-169)
+    float phi = pp < 0.0f ? pp + 2.0f * kPi : pp;
+    float sint = std::sin(theta);
+    if (sint == 0.0f) return 0.0f;
+    float u = phi / (2.0f * kPi), vv = theta / kPi;
+    const Dist2D& d = l.dist;
+    int nu = (int)d.cond[0].func.size(), nv = (int)d.marg.func.size();
+    int iu = (int)std::floor(u * (float)nu);
+    iu = iu < 0 ? 0 : (iu > nu - 1 ? nu - 1 : iu);
+    int iv = (int)std::floor(vv * (float)nv);
+    iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
+    float pd;
+    if (d.marg.fi * d.cond[iv].fi == 0.0f) pd = 0.0f;
+    else pd = (d.cond[iv].func[iu] * d.marg.func[iv]) / (d.cond[iv].fi * d.marg.fi);
+    return pd / (2.0f * kPi * kPi * sint);
+  }
+  const ShapePrim& sp = c.s->shapes[l.shape];
+  return shapePdf(sp.shape, transPoint(sp.w2o, p), transVector(sp.w2o, wi));
+}
+
+static Spec lightLe(const LightRec& l, const Ray& r) {  // Light.hs:98-106
+  if (l.type != 1) return sconst(0);
+  V3 wh = normalize(transVector(l.w2l, r.d));
+  float theta = std::acos(hmax(-1.0f, hmin(1.0f, wh.z)));
+  float pp = std::atan2(wh.y, wh.x);
+  float phi = pp < 0.0f ? pp + 2.0f * kPi : pp;
+  return texEval(l, phi / (2.0f * kPi), theta / kPi);
+}
+
+// -------------------------------------------------------------------- BSDF
+enum { BX_R = 1, BX_T = 2, BX_D = 4, BX_G = 8, BX_S = 16 };
+enum { K_LAMBERT, K_OREN, K_MF, K_SPECR, K_SPECT };
+enum { FR_NOOP, FR_DIEL, FR_COND };
+
+struct Bxdf {
+  int kind, type;
+  Spec r;
+  float A, B;   // Oren-Nayar
+  float e;      // Blinn exponent
+  int fr;
+  float ei, et; // dielectric Fresnel / transmission etas
+  Spec eta, k;  // conductor
+};
+
+struct Bsdf {
+  int n;
+  Bxdf c[2];
+  Frame cs;
+  V3 p, ng;
+};
+
+static inline float cosT(V3 v) { return v.z; }
+static inline float absCosT(V3 v) { return std::fabs(v.z); }
+static inline float sinT2(V3 v) { return hmax(0.0f, 1.0f - v.z * v.z); }
+static inline float sinT(V3 v) { return std::sqrt(sinT2(v)); }
+static inline float cosPhi(V3 v) {
+  float st = sinT(v);
+  return st == 0.0f ? 1.0f : clampf(v.x / st, -1.0f, 1.0f);
+}
+static inline float sinPhi(V3 v) {
+  float st = sinT(v);
+  return st == 0.0f ? 0.0f : clampf(v.y / st, -1.0f, 1.0f);
+}
+static inline bool sameHemi(V3 a, V3 b) { return a.z * b.z > 0.0f; }
+static inline V3 toSameHemi(V3 wo, V3 wi) { return wo.z < 0.0f ? mk(wi.x, wi.y, -wi.z) : wi; }
+
+static Spec frDielectric(float etai, float etat, float cosi) {  // Fresnel.hs:227-251
+  float c = hmax(0.0f, 1.0f - cosi * cosi);
+  float costp = cosi > 0.0f ? c / (etat * etat) : c * (etat * etat);
+  float cost = std::sqrt(1.0f - clampf(costp, 0.0f, 1.0f));
+  float ac = std::fabs(cosi);
+  float eta = etat / etai;
+  float rParlp = eta * ac;
+  float rParl = (cost - rParlp) / (cost + rParlp);
+  float rPerpp = eta * cost;
+  float rPerp = (ac - rPerpp) / (ac + rPerpp);
+  return sconst((rParl * rParl + rPerp * rPerp) * 0.5f);
+}
+static Spec frConductor(const Spec& eta, const Spec& k, float cosi) {  // Fresnel.hs:254-266
+  float ac = std::fabs(cosi);
+  Spec tmpF = eta * eta + k * k;
+  Spec tmp = sscale(tmpF, ac * ac);
+  Spec ec2 = sscale(eta, 2.0f * ac);
+  Spec c2 = sconst(ac * ac);
+  Spec one = sconst(1.0f);
+  Spec rPer2 = (tmpF - ec2 + c2) / (tmpF + ec2 + c2);
+  Spec rPar2 = (tmp - ec2 + one) / (tmp + ec2 + one);
+  return (rPer2 + rPar2) / sconst(2.0f);
+}
+static Spec fresnel(const Bxdf& b, float cosi) {
+  if (b.fr == FR_DIEL) return frDielectric(b.ei, b.et, cosi);
+  if (b.fr == FR_COND) return frConductor(b.eta, b.k, cosi);
+  return sconst(1.0f);
+}
+
+static float cosPdf(V3 wo, V3 wi) { return sameHemi(wo, wi) ? kInvPi * absCosT(wi) : 0.0f; }
+
+static float mfG(V3 wo, V3 wi, V3 wh) {  // Microfacet.hs:113-120
+  float nDotWh = absCosT(wh), nDotWo = absCosT(wo), nDotWi = absCosT(wi), woDotWh = absDot(wo, wh);
+  return hmin(1.0f, hmin(2.0f * nDotWh * nDotWo / woDotWh, 2.0f * nDotWh * nDotWi / woDotWh));
+}
+static float blinnD(float e, V3 wh) { return (e + 2.0f) * kInvTwoPi * std::pow(absCosT(wh), e); }
+static float blinnPdf(float e, V3 wh) { return (e + 1.0f) * std::pow(absCosT(wh), e) * kInvTwoPi; }
+
+static Spec orenNayar(const Bxdf& b, V3 wo, V3 wi) {  // Diffuse.hs:431-443
+  float sinti = sinT(wi), sinto = sinT(wo);
+  float maxcos = 0.0f;
+  if (sinti > 1e-4f && sinto > 1e-4f)
+    maxcos = hmax(0.0f, cosPhi(wi) * cosPhi(wo) + sinPhi(wi) * sinPhi(wo));
+  float sina, tanb;
+  if (absCosT(wi) > absCosT(wo)) { sina = sinto; tanb = sinti / absCosT(wi); }
+  else { sina = sinti; tanb = sinto / absCosT(wo); }
+  return sscale(b.r, b.A + b.B * maxcos * sina * tanb);
+}
+
+static Spec bxEval(const Bxdf& b, V3 wo, V3 wi) {
+  switch (b.kind) {
+    case K_LAMBERT: return sscale(b.r, kInvPi * absCosT(wo));
+    case K_OREN: return sscale(orenNayar(b, wo, wi), kInvPi * absCosT(wo));
+    case K_MF: {  // Microfacet.hs:21-32
+      float costo = absCosT(wo), costi = absCosT(wi);
+      if (costi == 0.0f || costo == 0.0f) return sconst(0);
+      V3 whp = wi + wo;
+      if (whp.x == 0.0f && whp.y == 0.0f && whp.z == 0.0f) return sconst(0);
+      V3 wh = normalize(whp);
+      if (cosT(wh) < 0.0f) return sconst(0);
+      float costh = dot(wi, wh);
+      float x = blinnD(b.e, wh) * mfG(wo, wi, wh) / (4.0f * costi);
+      return sscale(b.r * fresnel(b, costh), x);
+    }
+    default: return sconst(0);
+  }
+}
+static float bxPdf(const Bxdf& b, V3 wo, V3 wi) {
+  switch (b.kind) {
+    case K_LAMBERT:
+    case K_OREN: return cosPdf(wo, wi);
+    case K_MF: {  // Microfacet.hs:34-40
+      V3 whp = wo + wi;
+      if (sqLen(whp) == 0.0f) return 0.0f;
+      V3 wh = normalize(whp);
+      if (cosT(wh) < 0.0f) return 0.0f;
+      return blinnPdf(b.e, wh) / (4.0f * absDot(wo, wh));
+    }
+    default: return 0.0f;
+  }
+}
+// bxdfSample with adj = False
+static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, float* pdf) {
+  switch (b.kind) {
+    case K_LAMBERT:
+    case K_OREN: {  // Diffuse.hs:392-400,416-420
+      V3 w = toSameHemi(wo, cosineSampleHemisphere(u1, u2));
+      if (sameHemi(wo, w)) {
+        *f = b.kind == K_LAMBERT ? b.r : orenNayar(b, wo, w);
+        *wi = w;
+        *pdf = cosPdf(wo, w);
+      } else {
+        *f = sconst(0);
+        *wi = b.kind == K_LAMBERT ? wo : w;
+        *pdf = 0.0f;
+      }
+      return;
+    }
+    case K_MF: {  // Microfacet.hs:42-54,175-182
+      float cost = std::pow(u1, 1.0f / (b.e + 1.0f));
+      float sint = std::sqrt(hmax(0.0f, 1.0f - cost * cost));
+      float phi = u2 * 2.0f * kPi;
+      V3 whp = mk(sint * std::cos(phi), sint * std::sin(phi), cost);
+      float ff = std::pow(cost, b.e) * kInvTwoPi;
+      float d = (b.e + 2.0f) * ff;
+      float pd = (b.e + 1.0f) * ff;
+      V3 wh = cosT(whp) < 0.0f ? -whp : whp;
+      V3 w = sscale(2.0f * dot(wo, wh), wh) - wo;
+      float costH = dot(wo, wh);
+      float fact = d * std::fabs(costH) / pd * mfG(wo, w, wh);
+      Spec fp = b.r * fresnel(b, costH);
+      if (sameHemi(wo, w)) {
+        *f = sscale(fp, fact / absCosT(w));
+        *wi = w;
+        *pdf = pd / (4.0f * std::fabs(costH));
+      } else {
+        *f = sconst(0);
+        *wi = wo;
+        *pdf = 0.0f;
+      }
+      return;
+    }
+    case K_SPECR:  // Specular.hs:460-465
+      *f = b.r * fresnel(b, cosT(wo));
+      *wi = mk(-wo.x, -wo.y, wo.z);
+      *pdf = 1.0f;
+      return;
+    default: {  // specTrans (Specular.hs:478-502), adj = False
+      bool entering = cosT(wo) > 0.0f;
+      float ei = entering ? b.ei : b.et, et = entering ? b.et : b.ei;
+      float sini2 = sinT2(wo);
+      float eta = ei / et;
+      float eta2 = eta * eta;
+      float sint2 = eta2 * sini2;
+      if (sint2 >= 1.0f) {
+        *f = sconst(0);
+        *wi = wo;
+        *pdf = 0.0f;
+        return;
+      }
+      float c = std::sqrt(hmax(0.0f, 1.0f - sint2));
+      float cost = entering ? -c : c;
+      *wi = mk(eta * (-wo.x), eta * (-wo.y), cost);
+      Spec fr = frDielectric(ei, et, cost);
+      *f = sscale((sconst(1.0f) - fr) * b.r, eta2);
+      *pdf = 1.0f;
+      return;
+    }
+  }
+}
+
+static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
+  Bsdf b;
+  b.n = 0;
+  // mkBsdf (Reflection.hs:209-218)
+  V3 nn = dgs.n, sn = normalize(dgs.dpdu);
+  b.cs = Frame{sn, cross(nn, sn), nn};
+  b.p = dgs.p;
+  b.ng = dgg.n;
+  if (material < 0 || material >= (int)c.s->materials.size()) return b;
+  const bgpu_material& m = c.s->materials[material];
+  Spec s0 = specOf(m.spec[0]), s1 = specOf(m.spec[1]);
+  auto blinnE = [](float rough) {  // mkBlinn (1 / rough), fixExponent (Microfacet.hs:127-134)
+    float e = 1.0f / rough;
+    return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
+  };
+  switch (m.type) {
+    case BGPU_MAT_MATTE: {  // Material.hs:307-316
+      float sig = m.scalar[0];
+      Bxdf x{};
+      x.r = s0;
+      if (sig == 0.0f) {
+        x.kind = K_LAMBERT;
+      } else {
+        x.kind = K_OREN;
+        float sc = clampf(sig, 0.0f, 1.0f);
+        float sig2 = sc * sc;
+        x.A = 1.0f - (sig2 / (2.0f * (sig2 + 0.33f)));
+        x.B = 0.45f * sig2 / (sig2 + 0.09f);
+      }
+      x.type = BX_R | BX_D;
+      b.c[b.n++] = x;
+      break;
+    }
+    case BGPU_MAT_PLASTIC: {  // Material.hs:351-361
+      Bxdf d{};
+      d.kind = K_LAMBERT;
+      d.type = BX_R | BX_D;
+      d.r = s0;
+      Bxdf g{};
+      g.kind = K_MF;
+      g.type = BX_R | BX_G;
+      g.r = s1;
+      g.e = blinnE(m.scalar[0]);
+      g.fr = FR_DIEL;
+      g.ei = 1.0f;
+      g.et = 1.5f;
+      b.c[b.n++] = d;
+      b.c[b.n++] = g;
+      break;
+    }
+    case BGPU_MAT_METAL: {  // Material.hs:364-371
+      Bxdf g{};
+      g.kind = K_MF;
+      g.type = BX_R | BX_G;
+      g.r = sconst(1.0f);
+      g.e = blinnE(m.scalar[0]);
+      g.fr = FR_COND;
+      g.eta = s0;
+      g.k = s1;
+      b.c[b.n++] = g;
+      break;
+    }
+    case BGPU_MAT_GLASS: {  // Material.hs:331-341
+      float ior = m.scalar[0];
+      Bxdf r{};
+      r.kind = K_SPECR;
+      r.type = BX_R | BX_S;
+      r.r = sclamp(s0, 0.0f, 1.0f);
+      r.fr = FR_DIEL;
+      r.ei = 1.0f;
+      r.et = ior;
+      Bxdf t{};
+      t.kind = K_SPECT;
+      t.type = BX_T | BX_S;
+      t.r = sclamp(s1, 0.0f, 1.0f);
+      t.ei = 1.0f;
+      t.et = ior;
+      b.c[b.n++] = r;
+      b.c[b.n++] = t;
+      break;
+    }
+    case BGPU_MAT_MIRROR: {  // Material.hs:343-347
+      Bxdf r{};
+      r.kind = K_SPECR;
+      r.type = BX_R | BX_S;
+      r.r = sclamp(s0, 0.0f, 1.0f);
+      r.fr = FR_NOOP;
+      b.c[b.n++] = r;
+      break;
+    }
+    default:
+      break;
+  }
+  return b;
+}
+
+static Spec evalBsdf(const Bsdf& b, V3 woW, V3 wiW) {  // Reflection.hs:318-332, adj = False
+  float cosWo = dot(woW, b.ng);
+  float sideTest = dot(wiW, b.ng) / cosWo;
+  if (sideTest == 0.0f) return sconst(0);
+  if (std::fabs(cosWo) < 1e-5f) return sconst(0);
+  int need = sideTest < 0.0f ? BX_T : BX_R;
+  V3 wo = worldToLocal(b.cs, woW), wi = worldToLocal(b.cs, wiW);
+  Spec f = sconst(0);
+  for (int i = 0; i < b.n; ++i)
+    if (b.c[i].type & need) f = f + bxEval(b.c[i], wi, wo);
+  return f;
+}
+static float bsdfPdf(const Bsdf& b, V3 woW, V3 wiW) {  // Reflection.hs:251-257
+  if (b.n == 0) return 0.0f;
+  V3 wo = worldToLocal(b.cs, woW), wi = worldToLocal(b.cs, wiW);
+  float s = 0.0f;
+  for (int i = 0; i < b.n; ++i) s = s + bxPdf(b.c[i], wo, wi);
+  return s / (float)b.n;
+}
+struct BsdfSample {
+  int type;
+  float pdf;
+  Spec f;
+  V3 wi;
+};
+static BsdfSample sampleBsdf(const Bsdf& b, V3 woW, float uComp, float u1, float u2) {  // :278-316
+  BsdfSample empty{BX_R | BX_D, 0.0f, sconst(0), mk(0, 1, 0)};
+  if (b.n == 0) return empty;
+  V3 wo = worldToLocal(b.cs, woW);
+  int cnt = b.n;
+  float cntf = (float)cnt, invCnt = 1.0f / cntf;
+  int sNum = (int)std::floor(uComp * cntf);
+  sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
+  sNum = sNum < 0 ? 0 : sNum;
+  const Bxdf& bx = b.c[sNum];
+  Spec fS;
+  V3 wi;
+  float pdfp;
+  bxSample(bx, wo, u1, u2, &fS, &wi, &pdfp);
+  V3 wiW = localToWorld(b.cs, wi);
+  float sideTest = dot(wiW, b.ng) / dot(woW, b.ng);
+  if (pdfp == 0.0f || sideTest == 0.0f) return empty;
+  int need = sideTest < 0.0f ? BX_T : BX_R;
+  if (!(bx.type & need)) return empty;
+  if (bx.type & BX_S) return BsdfSample{bx.type, pdfp * invCnt, sscale(fS, cntf), wiW};
+  if (cnt == 1) return BsdfSample{bx.type, pdfp, fS, wiW};
+  float other = 0.0f;
+  Spec fOthers = sconst(0);
+  for (int i = 0; i < cnt; ++i) {
+    if (i == sNum) continue;
+    other = other + bxPdf(b.c[i], wo, wi);
+    if (b.c[i].type & need) fOthers = fOthers + bxEval(b.c[i], wi, wo);
+  }
+  float pdf = (pdfp + other) * invCnt;
+  Spec fSum = sscale(sscale(fS, pdfp) + fOthers, 1.0f / pdf);
+  return BsdfSample{bx.type, pdf, fSum, wiW};
+}
+
+// --------------------------------------------------------------- integrator
+struct SampleCtx {
+  const PixelSampler* ps;
+  uint32_t n;
+};
+
+static Spec lEmitAt(const Ctx& c, const Isect& is, V3 w) {  // intLe (Primitive.hs:68-76), lEmit (Light.hs:85-93)
+  if (is.prim < c.s->ntri) return sconst(0);
+  int si = is.prim - c.s->ntri;
+  int li = c.shapeLight[si];
+  if (li < 0) return sconst(0);
+  return dot(is.dgg.n, w) > 0.0f ? c.lights[li].L : sconst(0);
+}
+
+static Spec estimateDirect(Ctx& c, int li, V3 p, float eps, V3 n, V3 wo, const Bsdf& bsdf,
+                           float ulDir1, float ulDir2, float uBC, float uBD1, float uBD2) {
+  const LightRec& l = c.lights[li];
+  Spec ls = sconst(0);
+  {  // sampleLightMis (Scene.hs:61-69)
+    LightSample s = sampleLight(c, l, p, eps, ulDir1, ulDir2);
+    Spec f = evalBsdf(bsdf, wo, s.wi);
+    if (!(s.pdf == 0.0f || isBlack(s.li) || isBlack(f))) {
+      c.rays++;
+      if (!sceneOccluded(*c.s, s.ray, false)) {
+        float w = powerHeuristic(s.pdf, bsdfPdf(bsdf, wo, s.wi));
+        ls = sscale(f * s.li, w / s.pdf);
+      }
+    }
+  }
+  Spec bs = sconst(0);
+  {  // sampleBsdfMis (Scene.hs:71-82)
+    BsdfSample smp = sampleBsdf(bsdf, wo, uBC, uBD1, uBD2);
+    if (!(smp.pdf == 0.0f || isBlack(smp.f))) {
+      Ray ray{p, smp.wi, eps, kInf};
+      float lPdf = lightPdf(c, l, p, smp.wi);
+      float w = powerHeuristic(smp.pdf, lPdf);
+      Isect is;
+      c.rays++;
+      if (intersectFull(*c.s, ray, &is)) {
+        int si = is.prim - c.s->ntri;
+        int hitLight = (is.prim >= c.s->ntri) ? c.shapeLight[si] : -1;
+        // AreaLight equality by id (Light.hs:47-50); other lights never compare equal
+        if (hitLight >= 0 && l.type == 0 && c.lights[hitLight].id == l.id)
+          bs = sscale(smp.f * lEmitAt(c, is, -smp.wi), w);
+      } else {
+        bs = sscale(smp.f * lightLe(l, ray), w);
+      }
+    }
+  }
+  (void)n;
+  return ls + bs;
+}
+
+static Spec li(Ctx& c, const Ray& camRay, const PixelSampler& ps, uint32_t n, int md, int rrDepth) {
+  Spec L = sconst(0), t = sconst(1.0f);
+  bool spec = true;
+  int depth = 0;
+  Ray ray = camRay;
+  Isect is;
+  c.rays++;
+  bool hit = intersectFull(*c.s, ray, &is);
+  for (;;) {
+    if (!hit) {  // Path.hs:44-47
+      if (spec) {
+        Spec sum = sconst(0);
+        for (const LightRec& l : c.lights) sum = sum + lightLe(l, ray);
+        L = L + t * sum;
+      }
+      return L;
+    }
+    if (depth == md) return L;  // Path.hs:51
+    uint32_t d = (uint32_t)depth;
+    float lNumU = ps.get1(n, 1 + 4 * d);
+    float lDir1, lDir2, bDir1, bDir2, ud1, ud2;
+    ps.get2(n, 1 + 3 * d, &lDir1, &lDir2);
+    float lBsdfCompU = ps.get1(n, 2 + 4 * d);
+    ps.get2(n, 2 + 3 * d, &bDir1, &bDir2);
+    Spec intl = spec ? lEmitAt(c, is, ray.d) : sconst(0);
+    V3 wo = -ray.d;
+    const Scene& s = *c.s;
+    int mat = is.prim < s.ntri ? s.meshes[s.tri_mesh[is.prim]].material
+                               : s.shapes[is.prim - s.ntri].material;
+    Bsdf bsdf = makeBsdf(c, mat, is.dgg, is.dgs);
+    V3 nrm = bsdf.cs.n, p = bsdf.p;
+    float eps = is.eps;
+    // sampleOneLight (Scene.hs:110-118)
+    Spec ld = sconst(0);
+    int lc = (int)c.lights.size();
+    if (lc == 1) {
+      ld = estimateDirect(c, 0, p, eps, nrm, wo, bsdf, lDir1, lDir2, lBsdfCompU, bDir1, bDir2);
+    } else if (lc > 1) {
+      int ln = (int)std::floor(lNumU * (float)lc);
+      ln = ln < lc - 1 ? ln : lc - 1;
+      ld = sscale(estimateDirect(c, ln, p, eps, nrm, wo, bsdf, lDir1, lDir2, lBsdfCompU, bDir1, bDir2),
+                  (float)lc);
+    }
+    Spec lHere = intl + ld;
+    L = L + t * lHere;
+    float pc = depth <= rrDepth ? 1.0f : hmin(0.75f, sY(t));  // Path.hs:70
+    float x = ps.get1(n, 3 + 4 * d);
+    if (x > pc) return L;
+    float uc = ps.get1(n, 0 + 4 * d);
+    ps.get2(n, 0 + 3 * d, &ud1, &ud2);
+    BsdfSample smp = sampleBsdf(bsdf, wo, uc, ud1, ud2);
+    if (smp.pdf == 0.0f || isBlack(smp.f)) return L;
+    ray = Ray{p, smp.wi, eps, kInf};
+    spec = (smp.type & BX_S) != 0;
+    t = sscale(smp.f * t, 1.0f / pc);
+    depth++;
+    c.rays++;
+    hit = intersectFull(*c.s, ray, &is);
+  }
+}
+
+// --------------------------------------------------------------------- film
+static float evalFilter(const bgpu_film& f, float x, float y) {  // Filter.hs:68-96
+  switch (f.filter) {
+    case BGPU_FILTER_GAUSS: {
+      float w = f.fparams[0], h = f.fparams[1], a = f.fparams[2];
+      float ex = std::exp(-a * w * w), ey = std::exp(-a * h * h);
+      float gx = hmax(0.0f, std::exp(-a * x * x) - ex), gy = hmax(0.0f, std::exp(-a * y * y) - ey);
+      return gx * gy;
+    }
+    case BGPU_FILTER_MITCHELL: {
+      float w = f.fparams[0], h = f.fparams[1], B = f.fparams[2], C = f.fparams[3];
+      auto m1d = [&](float xp) {
+        float xx = std::fabs(2.0f * xp);
+        if (xx > 1.0f)
+          return ((-B - 6 * C) * xx * xx * xx + (6 * B + 30 * C) * xx * xx + (-12 * B - 48 * C) * xx +
+                  (8 * B + 24 * C)) * (1.0f / 6.0f);
+        return ((12 - 9 * B - 6 * C) * xx * xx * xx + (-18 + 12 * B + 6 * C) * xx * xx + (6 - 2 * B)) *
+               (1.0f / 6.0f);
+      };
+      return m1d(x * (1.0f / w)) * m1d(y * (1.0f / h));
+    }
+    case BGPU_FILTER_SINC: {
+      float ix = 1.0f / f.fparams[0], iy = 1.0f / f.fparams[1], tau = f.fparams[2];
+      auto s1 = [&](float v) {
+        if (std::fabs(v) > 1.0f) return 0.0f;
+        if (std::fabs(v) < 1e-5f) return 1.0f;
+        float xp = std::fabs(v) * kPi;
+        return (std::sin(xp) / xp) * (std::sin(xp * tau) / (xp * tau));
+      };
+      return s1(x * ix) * s1(y * iy);
+    }
+    case BGPU_FILTER_TRIANGLE: {
+      float w = f.fparams[0], h = f.fparams[1];
+      return (hmax(0.0f, w - std::fabs(x * 2.0f)) * hmax(0.0f, h - std::fabs(y * 2.0f))) / (w * h);
+    }
+    default:
+      return (std::fabs(x) < 0.5f && std::fabs(y) < 0.5f) ? 1.0f : 0.0f;
+  }
+}
+static void filterSize(const bgpu_film& f, float* fw, float* fh) {
+  if (f.filter == BGPU_FILTER_BOX) { *fw = 0.5f; *fh = 0.5f; }
+  else { *fw = f.fparams[0]; *fh = f.fparams[1]; }
+}
+
+static void addSample(const Ctx& c, float* wxyz, float* sq, int W, int H, float sx, float sy,
+                      const Spec& L) {  // Image.hs:250-299 on the full image
+  for (float v : L)
+    if (std::isnan(v) || std::isinf(v)) return;
+  float X, Y, Z;
+  toXYZ(L, &X, &Y, &Z);
+  float fw = c.fw, fh = c.fh;
+  float ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:263 (sic)
+  float dx = sx - 0.5f, dy = sy - 0.5f;
+  int x0 = std::max(0, (int)std::ceil(dx - fw)), x1 = std::min(W - 1, (int)std::floor(dx + fw));
+  int y0 = std::max(0, (int)std::ceil(dy - fh)), y1 = std::min(H - 1, (int)std::floor(dy + fh));
+  if (x1 - x0 < 0 || y1 - y0 < 0) return;
+  for (int y = y0; y <= y1; ++y) {
+    int iy = std::min((int)std::floor(std::fabs(((float)y - dy) * ifh * 16.0f)), 15);
+    for (int x = x0; x <= x1; ++x) {
+      int ix = std::min((int)std::floor(std::fabs(((float)x - dx) * ifw * 16.0f)), 15);
+      float w = c.filter[iy * 16 + ix];
+      float* o = wxyz + 4 * ((size_t)y * W + x);
+#pragma omp atomic
+      o[0] += w;
+#pragma omp atomic
+      o[1] += X * w;
+#pragma omp atomic
+      o[2] += Y * w;
+#pragma omp atomic
+      o[3] += Z * w;
+      if (sq) {
+        float* q = sq + 3 * ((size_t)y * W + x);
+#pragma omp atomic
+        q[0] += X * X * w;
+#pragma omp atomic
+        q[1] += Y * Y * w;
+#pragma omp atomic
+        q[2] += Z * Z * w;
+      }
+    }
+  }
+}
+
+static Ray fireRay(const bgpu_camera& cam, float ix, float iy, float lu, float lv) {  // Camera.hs:49-68
+  M4 r2c, c2w;
+  std::memcpy(r2c.m, cam.r2c, sizeof(r2c.m));
+  std::memcpy(c2w.m, cam.c2w, sizeof(c2w.m));
+  V3 pCamera = transPoint(r2c, mk(ix, iy, 0.0f));
+  Ray ray{mk(0, 0, 0), normalize(pCamera), 0.0f, kInf};
+  if (cam.lens_radius > 0.0f) {
+    float a, b;
+    concentricSampleDisk(lu, lv, &a, &b);
+    V3 ro = mk(a * cam.lens_radius, b * cam.lens_radius, 0.0f);
+    V3 pFocus = rayAt(ray, cam.focal_distance / ray.d.z);
+    ray = Ray{ro, normalize(pFocus - ro), 0.0f, kInf};
+  }
+  return transRay(c2w, ray);
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Renders a whole job (all passes) on the CPU.  Returns the number of ray
+// queries traced via *rays.  wxyz: w*h*4, sq: w*h*3 or NULL.
+int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int nthreads,
+               uint64_t* rays, int pass_begin, int pass_end) {
+  using namespace orc;
+  const Scene& s = *(const Scene*)sp;
+  Ctx c;
+  c.s = &s;
+  // light list: explicit lights then geometric lights in primitive order (Scene.hs:37-43)
+  for (size_t i = 0; i < s.lights.size(); ++i) {
+    LightRec l{};
+    l.type = 1;
+    l.shape = -1;
+    l.id = -1;
+    l.w = s.lights[i].width;
+    l.h = s.lights[i].height;
+    l.map = s.light_maps[i].data();
+    std::memcpy(l.w2l.m, s.lights[i].w2l, sizeof(l.w2l.m));
+    std::memcpy(l.l2w.m, s.lights[i].l2w, sizeof(l.l2w.m));
+    c.lights.push_back(l);
+  }
+  for (auto& l : c.lights) l.dist = buildDist(l);
+  c.shapeLight.assign(s.shapes.size(), -1);
+  for (size_t i = 0; i < s.shapes.size(); ++i) {
+    if (!s.shapes[i].emissive) continue;
+    LightRec l{};
+    l.type = 0;
+    l.shape = (int)i;
+    l.id = s.shapes[i].light_id;
+    l.L = specOf(s.shapes[i].emission);
+    c.shapeLight[i] = (int)c.lights.size();
+    c.lights.push_back(l);
+  }
+  filterSize(s.film, &c.fw, &c.fh);
+  for (int y = 0; y < 16; ++y) {  // mkTableFilter (Image.hs:48-61)
+    float fy = ((float)y + 0.5f) * c.fh / 16.0f;
+    for (int x = 0; x < 16; ++x) {
+      float fx = ((float)x + 0.5f) * c.fw / 16.0f;
+      c.filter[y * 16 + x] = evalFilter(s.film, fx, fy);
+    }
+  }
+  const int W = s.film.width, H = s.film.height;
+  // sampleExtent (Image.hs:162-168)
+  int x0 = (int)std::floor(0.5f - c.fw), x1 = (int)std::floor(0.5f + (float)W + c.fw);
+  int y0 = (int)std::floor(0.5f - c.fh), y1 = (int)std::floor(0.5f + (float)H + c.fh);
+  int ew = x1 - x0 + 1, eh = y1 - y0 + 1;
+  uint32_t nu = job->spp_nu, nv = job->spp_nv, N = nu * nv;
+  int md = job->max_depth, sd = job->sample_depth, rr = job->rr_start_depth;
+  for (int pass = pass_begin; pass < pass_end; ++pass) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (long pix = 0; pix < (long)ew * eh; ++pix) {
+      int ix = x0 + (int)(pix % ew), iy = y0 + (int)(pix / ew);
+      PixelSampler ps{pixelKey(job->seed, (uint32_t)pass, ix, iy), nu, nv, N, 4u * (uint32_t)sd,
+                      3u * (uint32_t)sd};
+      for (uint32_t n = 0; n < N; ++n) {
+        float ox, oy, lu, lv;
+        ps.camera(n, &ox, &oy);
+        ps.lens(n, &lu, &lv);
+        float sx = (float)ix + ox, sy = (float)iy + oy;
+        Ray r = fireRay(s.camera, sx, sy, lu, lv);
+        Spec L = li(c, r, ps, n, md, rr);
+        addSample(c, wxyz, sq, W, H, sx, sy, L);
+      }
+    }
+  }
+  if (rays) *rays = c.rays.load();
+  return 0;
+}
+
+}  // extern "C"

@@ -93,3 +93,22 @@ class OracleScene:
         load().orc_intersect_any(self.h, C.addressof(r), n, occ.ctypes.data, int(brute),
                                  threads or os.cpu_count())
         return occ.astype(bool)
+
+
+def render(flat, spec, threads=None, passes=None):
+    """Oracle render of a whole job: returns ((h, w, 4) wxyz, (h, w, 3) sum of
+    squared XYZ, ray queries traced)."""
+    lib = load()
+    o = OracleScene(flat)
+    w, h = flat.film[0], flat.film[1]
+    img = np.zeros((h, w, 4), np.float32)
+    sq = np.zeros((h, w, 3), np.float32)
+    rays = C.c_uint64()
+    d = spec.desc()
+    p0, p1 = (0, spec.passes) if passes is None else passes
+    lib.orc_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                               C.POINTER(C.c_uint64), C.c_int, C.c_int]
+    lib.orc_render(o.h, C.addressof(d), img.ctypes.data, sq.ctypes.data,
+                   threads or os.cpu_count(), C.byref(rays), p0, p1)
+    o.close()
+    return img, sq, rays.value
