@@ -260,7 +260,8 @@ struct Engine {
 
   // Reads a job's per-pass completion; true when every sample is accounted for.
   int progress(HostJob* j, bool* complete) {
-    size_t need = (size_t)j->desc.passes * 4 + 8;
+    // per-pass counts, padded to 8 bytes, then the 8-byte skipped counter
+    size_t need = (size_t)(j->desc.passes + (j->desc.passes & 1)) * 4 + 8;
     if (h_pass_cap < need) {
       if (h_pass) (void)hipHostFree(h_pass);
       h_pass = nullptr;

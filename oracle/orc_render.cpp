@@ -1108,6 +1108,23 @@ static Ray fireRay(const bgpu_camera& cam, float ix, float iy, float lu, float l
 
 extern "C" {
 
+// Sampler values of every sample n of pixel (pix, pix) in `pass`, for the
+// stratification tests: per row [camera x, camera y, 1D table dim 0,
+// 2D table dim 0 (u, v), 1D dim 100 (past the tables), lens (u, v)].
+void orc_sampler_table(uint64_t seed, int pass, int pix, int nu, int nv, float* out) {
+  using namespace orc;
+  uint32_t N = (uint32_t)(nu * nv);
+  PixelSampler ps{pixelKey(seed, (uint32_t)pass, pix, pix), (uint32_t)nu, (uint32_t)nv, N, 4u, 3u};
+  for (uint32_t n = 0; n < N; ++n) {
+    float* o = out + 8 * n;
+    ps.camera(n, &o[0], &o[1]);
+    o[2] = ps.get1(n, 0);
+    ps.get2(n, 0, &o[3], &o[4]);
+    o[5] = ps.get1(n, 100);
+    ps.lens(n, &o[6], &o[7]);
+  }
+}
+
 // Renders a whole job (all passes) on the CPU.  Returns the number of ray
 // queries traced via *rays.  wxyz: w*h*4, sq: w*h*3 or NULL.
 int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int nthreads,

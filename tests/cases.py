@@ -1,0 +1,107 @@
+"""Shared scene builders for the closed-form fixtures and analytic renders."""
+import json
+import os
+
+import numpy as np
+
+from bling_amd import abi, scenes
+from bling_amd import transform as T
+from bling_amd.scene import Material, SceneBuilder, const
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SHAPES = {"sphere": abi.SHAPE_SPHERE, "quad": abi.SHAPE_QUAD, "disk": abi.SHAPE_DISK,
+          "box": abi.SHAPE_BOX, "cylinder": abi.SHAPE_CYLINDER}
+
+
+def intersection_cases():
+    with open(os.path.join(GOLDEN, "intersection_cases.json")) as f:
+        return json.load(f)
+
+
+def case_scene(c):
+    sb = SceneBuilder()
+    sb.material(Material.matte(const(0.5)))
+    if c["kind"] == "triangle":
+        sb.mesh(np.array(c["verts"], np.float32), np.array([[0, 1, 2]]))
+    else:
+        sb.shape(SHAPES[c["shape"]], c["params"], o2w=T.translate(c["translate"]))
+    return sb.build()
+
+
+def case_ray(c):
+    o, d, tmin, tmax = c["ray"]
+    return np.array([list(o) + list(d) + [tmin, np.inf if tmax == "inf" else tmax]], np.float32)
+
+
+def sky_map(spec, w=1, h=1):
+    return np.tile(np.asarray(spec, np.float32), (h, w, 1))
+
+
+def plane_under_sky(rho=0.5, sky=1.0, res=8, fov=2.0):
+    """A large diffuse quad at z = 0 under a uniform infinite light; the camera
+    looks down at the origin.  Direct reflected radiance is rho * L."""
+    sb = SceneBuilder()
+    m = sb.material(Material.matte(const(rho)))
+    sb.shape(abi.SHAPE_QUAD, (1000.0, 1000.0), material=m)
+    sb.infinite_light(sky_map(const(sky)))
+    c2w = T.look_at((0.0, -3.0, 4.0), (0.0, 0.0, 0.0), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
+    sb.set_film(res, res)
+    return sb.build()
+
+
+def sphere_light_over_plane(rho=0.5, Le=4.0, R=0.25, D=2.0, res=8, fov=0.1):
+    """A spherical area light of radius R at height D over a diffuse plane:
+    at the point below it, E = pi Le (R/D)^2, so Lo = rho Le (R/D)^2.  The
+    narrow field of view keeps the pixels' footprint within +-0.009 of that
+    point, where E falls off by under 2e-5."""
+    sb = SceneBuilder()
+    m = sb.material(Material.matte(const(rho)))
+    sb.shape(abi.SHAPE_QUAD, (1000.0, 1000.0), material=m)
+    sb.shape(abi.SHAPE_SPHERE, (R,), o2w=T.translate((0.0, 0.0, D)), material=m,
+             emission=const(Le))
+    c2w = T.look_at((3.0, 0.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
+    sb.set_film(res, res)
+    return sb.build()
+
+
+def sky_only(sky=2.0, res=8):
+    sb = SceneBuilder()
+    sb.infinite_light(sky_map(const(sky), 4, 2))
+    c2w = T.look_at((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0, 1, 0))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 60.0, res, res))
+    sb.set_film(res, res)
+    return sb.build()
+
+
+def materials_scene(res=48, env=True, seed=3):
+    """Spheres of every material over a matte floor, lit by a quad light, a
+    sphere light and (optionally) a gradient sky."""
+    from bling_amd.scene import rgb_refl, rgb_illum
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.55)))
+    rough = sb.material(Material.matte(rgb_refl(0.7, 0.3, 0.2), sigma=0.4))
+    plastic = sb.material(Material.plastic(rgb_refl(0.2, 0.3, 0.7), const(0.4), 0.02))
+    metal = sb.material(Material.metal(const(0.2), const(3.9), 0.05))
+    glass = sb.material(Material.glass(1.5, const(1.0), const(1.0)))
+    mirror = sb.material(Material.mirror(const(0.9)))
+    sb.mesh(np.array([[-4, -4, 0], [4, -4, 0], [4, 4, 0], [-4, 4, 0]], np.float32),
+            np.array([[0, 1, 2], [0, 2, 3]]), material=floor)
+    for k, m in enumerate((rough, plastic, metal, glass, mirror)):
+        x = -2.0 + k * 1.0
+        sb.shape(abi.SHAPE_SPHERE, (0.4,), o2w=T.translate((x, 0.0, 0.4)), material=m)
+    lt = T.rotate_x(180) @ T.translate((0.0, 0.0, 3.0))
+    sb.shape(abi.SHAPE_QUAD, (0.5, 0.5), o2w=lt, material=floor, emission=rgb_illum(6, 6, 6))
+    sb.shape(abi.SHAPE_SPHERE, (0.15,), o2w=T.translate((2.0, -1.0, 1.5)), material=floor,
+             emission=const(20.0))
+    if env:
+        h, w = 16, 32
+        v = (np.arange(h, dtype=np.float32) + 0.5) / h
+        sky = np.stack([np.outer(0.2 + 0.8 * v, np.ones(w)) * c for c in np.linspace(0.6, 1.0, 16)], -1)
+        sb.infinite_light(sky.astype(np.float32))
+    c2w = T.look_at((0.0, -5.0, 2.0), (0.0, 0.0, 0.4), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 45.0, res, res))
+    sb.set_film(res, res)
+    return sb.build()

@@ -146,3 +146,24 @@ def test_displaced_icosphere_medium(gpu_ctx):
     finally:
         gs.close()
         o.close()
+
+
+@pytest.mark.parametrize("case", __import__("tests.cases", fromlist=["x"]).intersection_cases(),
+                         ids=lambda c: c["name"])
+def test_closed_form_cases_on_gpu(gpu_ctx, case):
+    from tests import cases
+    gs = gpu_ctx.scene(cases.case_scene(case))
+    try:
+        t, b1, b2, prim = gs.intersect(cases.case_ray(case))
+        occ = gs.intersects(cases.case_ray(case))
+    finally:
+        gs.close()
+    exp = case["expect"]
+    if exp is None:
+        assert prim[0] == -1
+    else:
+        assert prim[0] == 0 and t[0] == np.float32(exp["t"])
+        if "b1" in exp:
+            assert (b1[0], b2[0]) == (np.float32(exp["b1"]), np.float32(exp["b2"]))
+    if "any" in case:
+        assert occ[0] == case["any"]

@@ -123,3 +123,43 @@ def test_invalid_job_rejected(gpu_ctx):
             assert e.value.code == abi.E_INVALID
     finally:
         gs.close()
+
+
+def _y_mean(img, sq):
+    m, se = mean_se(img, sq)
+    return m[..., 1].mean(), np.sqrt((se[..., 1] ** 2).sum()) / m[..., 1].size
+
+
+def test_analytic_radiance_on_gpu(gpu_ctx):
+    from tests import cases
+    spec = JobSpec(spp_nu=8, spp_nv=8, max_depth=1, sample_depth=1, passes=8, seed=5, variance=True)
+    y, s = _y_mean(*render_gpu(gpu_ctx, cases.plane_under_sky(rho=0.5, sky=1.0), spec))
+    assert abs(y - 0.5) < 4 * s + 5e-5, (y, s)
+    R, D, Le, rho = 0.25, 2.0, 4.0, 0.5
+    y, s = _y_mean(*render_gpu(gpu_ctx, cases.sphere_light_over_plane(rho=rho, Le=Le, R=R, D=D), spec))
+    want = rho * Le * (R / D) ** 2
+    assert abs(y - want) < 4 * s + 1e-4 * want, (y, want, s)
+    img, _ = render_gpu(gpu_ctx, cases.sky_only(sky=2.0), JobSpec(spp_nu=2, spp_nv=2, passes=1))
+    from bling_amd.scene import const, spectrum_to_xyz
+    np.testing.assert_allclose(img[..., 1:] / img[..., :1],
+                               np.broadcast_to(np.array(spectrum_to_xyz(const(2.0))), img[..., 1:].shape),
+                               rtol=1e-6)
+
+
+@pytest.mark.parametrize("env", [False, True])
+def test_materials_scene_matches_oracle(gpu_ctx, env):
+    from tests import cases
+    flat = cases.materials_scene(res=48, env=env)
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=8, sample_depth=3, rr_start_depth=3, passes=4,
+                   seed=21, variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    spec2 = JobSpec(spp_nu=4, spp_nv=4, max_depth=8, sample_depth=3, rr_start_depth=3, passes=4,
+                    seed=22, variance=True)
+    o2, osq2, _ = pyoracle.render(flat, spec2)
+    frac, zmean, zmax = compare(g, gsq, o2, osq2)
+    assert frac < 0.015, (frac, zmean, zmax)
+    assert abs(zmean) < 0.08, zmean
