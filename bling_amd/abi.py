@@ -84,7 +84,7 @@ class JobDesc(C.Structure):
     _fields_ = [("spp_nu", C.c_int32), ("spp_nv", C.c_int32), ("max_depth", C.c_int32),
                 ("sample_depth", C.c_int32), ("rr_start_depth", C.c_int32),
                 ("passes", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
-                ("pad", C.c_int32)]
+                ("pad", C.c_int32), ("camera", C.POINTER(Camera)), ("film", C.POINTER(Film))]
 
 
 class Stats(C.Structure):

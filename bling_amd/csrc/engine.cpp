@@ -25,7 +25,9 @@ static double now_ms() {
 
 struct HostJob {
   bgpu_job_desc desc{};
-  Scene* scene = nullptr;
+  Scene* scene = nullptr;  // holds a reference while the job exists
+  DevView view{};          // the job's camera and film
+  size_t px = 0;           // film pixels
   int slot = -1;
   unsigned long long total = 0, per_pass = 0;
   DevBuf film, sq, pass_done, skipped;
@@ -37,7 +39,33 @@ struct HostJob {
   bool cancel_req = false;
   bool released = false;  // user released the handle
   size_t bytes = 0;
-  virtual ~HostJob() = default;
+
+  // Copies a validated description; the camera and film default to the
+  // scene's (the caller's descriptors are not referenced afterwards).
+  int prepare(Scene* s, const bgpu_job_desc& d) {
+    if (int rc = make_view(d.camera ? *d.camera : s->camera, d.film ? *d.film : s->film, &view)) return rc;
+    desc = d;
+    desc.camera = nullptr;
+    desc.film = nullptr;
+    px = (size_t)view.film_w * view.film_h;
+    per_pass = (unsigned long long)view.ext_w * view.ext_h * (unsigned long long)(d.spp_nu * d.spp_nv);
+    if (per_pass > (1ull << 62) / (unsigned long long)d.passes)
+      return fail(BGPU_E_INVALID, "job has too many samples");
+    total = per_pass * (unsigned long long)d.passes;
+    scene = s;
+    s->refs++;
+    return BGPU_OK;
+  }
+  virtual ~HostJob() {
+    film.release();
+    sq.release();
+    pass_done.release();
+    skipped.release();
+    if (scene && --scene->refs == 0) {
+      (void)hipSetDevice(scene->ctx->device);
+      delete scene;
+    }
+  }
 };
 
 struct Engine {
@@ -182,10 +210,9 @@ struct Engine {
     }
   }
 
-  static size_t job_bytes(const Scene* s, const bgpu_job_desc& d) {
-    size_t px = (size_t)s->film.width * s->film.height;
-    size_t b = px * 16 + (size_t)d.passes * 4 + 8;
-    if (d.flags & BGPU_JOB_VARIANCE) b += px * 12;
+  static size_t job_bytes(const HostJob* j) {
+    size_t b = j->px * 16 + (size_t)j->desc.passes * 4 + 8;
+    if (j->desc.flags & BGPU_JOB_VARIANCE) b += j->px * 12;
     return b;
   }
 
@@ -194,9 +221,8 @@ struct Engine {
     if (free_slots.empty()) return fail(BGPU_E_BUDGET, "job table full");
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
-    const DevPtScene& S = j->scene->pt.host;
     const bgpu_job_desc& d = j->desc;
-    size_t px = (size_t)S.film_w * S.film_h;
+    const size_t px = j->px;
     if (int rc = j->film.alloc(ctx, px * 16)) return rc;
     if (d.flags & BGPU_JOB_VARIANCE)
       if (int rc = j->sq.alloc(ctx, px * 12)) return rc;
@@ -210,8 +236,6 @@ struct Engine {
     free_slots.pop_back();
     j->slot = slot;
     slots[slot] = j;
-    j->per_pass = (unsigned long long)S.ext_w * S.ext_h * d.spp_nu * d.spp_nv;
-    j->total = j->per_pass * (unsigned long long)d.passes;
     j->h_pass.assign(d.passes, 0);
     DevJob& J = *h_job;
     std::memset(&J, 0, sizeof(J));
@@ -230,6 +254,7 @@ struct Engine {
     J.pass_done = j->pass_done.as<unsigned int>();
     J.skipped = j->skipped.as<unsigned long long>();
     J.cancelled = 0;
+    J.view = j->view;
     BGPU_HIP(hipMemcpyAsync(args.jobs + slot, h_job, sizeof(DevJob), hipMemcpyHostToDevice, stream));
     h_admit[0] = AdmitReq{slot, 1, j->total};
     BGPU_HIP(hipMemcpyAsync(args.admit, h_admit, sizeof(AdmitReq), hipMemcpyHostToDevice, stream));
@@ -405,7 +430,7 @@ struct Scheduler {
         // admit by HBM budget, in arrival order
         while (!pending.empty()) {
           HostJob* j = pending.front();
-          size_t need = Engine::job_bytes(j->scene, j->desc);
+          size_t need = Engine::job_bytes(j);
           if (ctx->used.load() + need > ctx->budget && !running.empty()) break;
           if (eng.free_slots.empty()) break;
           pending.pop_front();
@@ -509,15 +534,11 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   if (int rc = validate_job(job)) return rc;
   Ctx* c = scene->ctx;
   BGPU_HIP(hipSetDevice(c->device));
-  const DevPtScene& S = scene->pt.host;
-  unsigned long long total =
-      (unsigned long long)S.ext_w * S.ext_h * job->spp_nu * job->spp_nv * (unsigned long long)job->passes;
-  int pool = (int)std::min<unsigned long long>((unsigned long long)c->pt_cfg.pool, total);
+  HostJob j;
+  if (int rc = j.prepare(scene, *job)) return rc;
+  int pool = (int)std::min<unsigned long long>((unsigned long long)c->pt_cfg.pool, j.total);
   Engine eng;
   if (int rc = eng.init(c, pool, c->pt_cfg.block)) return rc;
-  HostJob j;
-  j.desc = *job;
-  j.scene = scene;
   if ((job->flags & BGPU_JOB_VARIANCE) == 0 && sq) j.desc.flags |= BGPU_JOB_VARIANCE;
   double t0 = now_ms();
   if (int rc = eng.admit(&j)) return rc;
@@ -539,9 +560,8 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   if (!complete) return fail(BGPU_E_STATE, "render finished with samples unaccounted for");
   double t1 = now_ms();
   accumulate_stats(c, before, *eng.h_cnt, t1 - t0, j.total);
-  size_t px = (size_t)S.film_w * S.film_h;
-  BGPU_HIP(hipMemcpy(wxyz, j.film.ptr, px * 16, hipMemcpyDeviceToHost));
-  if (sq && j.sq.ptr) BGPU_HIP(hipMemcpy(sq, j.sq.ptr, px * 12, hipMemcpyDeviceToHost));
+  BGPU_HIP(hipMemcpy(wxyz, j.film.ptr, j.px * 16, hipMemcpyDeviceToHost));
+  if (sq && j.sq.ptr) BGPU_HIP(hipMemcpy(sq, j.sq.ptr, j.px * 12, hipMemcpyDeviceToHost));
   eng.retire(&j);
   return BGPU_OK;
 }
@@ -551,8 +571,15 @@ int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out)
   *out = nullptr;
   if (int rc = validate_job(job)) return rc;
   Ctx* c = scene->ctx;
-  size_t need = Engine::job_bytes(scene, *job);
-  if (need > c->budget) return fail(BGPU_E_BUDGET, "job film does not fit the HBM budget");
+  bgpu_job* j = new bgpu_job();
+  if (int rc = j->prepare(scene, *job)) {
+    delete j;
+    return rc;
+  }
+  if (Engine::job_bytes(j) > c->budget) {
+    delete j;
+    return fail(BGPU_E_BUDGET, "job film does not fit the HBM budget");
+  }
   {
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->sched) {
@@ -561,14 +588,12 @@ int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out)
       int rc = s->start(c);
       if (rc) {
         delete s;
+        delete j;
         return rc;
       }
       c->sched = s;
     }
   }
-  bgpu_job* j = new bgpu_job();
-  j->desc = *job;
-  j->scene = scene;
   j->t_submit = now_ms();
   j->state = BGPU_JOB_QUEUED;
   Scheduler* s = c->sched;
@@ -588,7 +613,7 @@ int bgpu_job_poll(bgpu_job* job, float* wxyz, float* sq, int32_t* passes_done, i
   if (passes_done) *passes_done = job->passes_done;
   if (state) *state = job->state;
   if (wxyz || sq) {
-    size_t px = (size_t)job->scene->film.width * job->scene->film.height;
+    const size_t px = job->px;
     if (job->film.ptr && wxyz) BGPU_HIP(hipMemcpy(wxyz, job->film.ptr, px * 16, hipMemcpyDeviceToHost));
     else if (wxyz) std::memset(wxyz, 0, px * 16);
     if (sq && job->sq.ptr) BGPU_HIP(hipMemcpy(sq, job->sq.ptr, px * 12, hipMemcpyDeviceToHost));

@@ -50,9 +50,15 @@ struct DevPtScene {
   const DLight* lights;
   const float* maps;
   const float* dist;
-  int32_t ntri, nlights, nmats, cam_type;
+  int32_t ntri, nlights, nmats, pad;
+};
+
+// Camera and film of a job (ProjectiveCamera, Camera.hs:24-31; the film's
+// 16x16 filter table, Image.hs:48-61; the sample extent, Image.hs:162-168).
+struct DevView {
   float c2w[16], r2c[16];
-  float lens_radius, focal_distance, xres, yres;
+  float lens_radius, focal_distance;
+  int32_t cam_type, pad;
   int32_t film_w, film_h, ext_x0, ext_y0, ext_w, ext_h;
   float fw, fh;
   float filter[256];
@@ -71,6 +77,7 @@ struct DevJob {
   unsigned int* pass_done;      // finished samples per pass
   unsigned long long* skipped;  // samples dropped after cancellation
   int32_t cancelled, pad;
+  DevView view;
 };
 
 // Device counters of the engine.
@@ -132,6 +139,9 @@ struct PtScene {
   std::vector<int32_t> shape_light_host;  // device light index per shape, -1 if none
   DevPtScene host{};                      // filled by build_pt_data, pointers are device
 };
+
+// Fills a job view from a camera and film description.
+int make_view(const bgpu_camera& cam, const bgpu_film& film, DevView* v);  // 0 or BGPU_E_INVALID
 
 // Enqueues one bounce iteration (reset, generate, extend, shade, shadow,
 // MIS, finish) on `s`.  Capturable into a hipGraph.

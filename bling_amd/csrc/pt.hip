@@ -129,15 +129,14 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
         unsigned long long rem = local % J.per_pass;
         int pix = (int)(rem / (unsigned long long)J.N);
         int n = (int)(rem % (unsigned long long)J.N);
-        const DevPtScene& S = a.scenes[J.scene];
-        int ix = S.ext_x0 + pix % S.ext_w, iy = S.ext_y0 + pix / S.ext_w;
+        int ix = J.view.ext_x0 + pix % J.view.ext_w, iy = J.view.ext_y0 + pix / J.view.ext_w;
         Sampler sm{pixel_key(J.seed, (uint32_t)pass, ix, iy), (uint32_t)J.nu, (uint32_t)J.nv,
                    (uint32_t)J.N, 4u * (uint32_t)J.sd, 3u * (uint32_t)J.sd};
         float ox, oy, lu, lv;
         sm.camera((uint32_t)n, &ox, &oy);
         sm.lens((uint32_t)n, &lu, &lv);
         float sx = (float)ix + ox, sy = (float)iy + oy;
-        DRay r = fire_ray(S, sx, sy, lu, lv);
+        DRay r = fire_ray(J.view, sx, sy, lu, lv);
         const int Pn = P.P;
         P.job[i] = j;
         P.pass[i] = pass;
@@ -258,7 +257,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
       } else {
         float th = P.hit[i], hb1 = P.hit[Pn + i], hb2 = P.hit[2 * Pn + i];
         int pix = P.pix[i], n = P.samp[i];
-        int ix = S.ext_x0 + pix % S.ext_w, iy = S.ext_y0 + pix / S.ext_w;
+        int ix = J.view.ext_x0 + pix % J.view.ext_w, iy = J.view.ext_y0 + pix / J.view.ext_w;
         Sampler sm{pixel_key(J.seed, (uint32_t)P.pass[i], ix, iy), (uint32_t)J.nu, (uint32_t)J.nv,
                    (uint32_t)J.N, 4u * (uint32_t)J.sd, 3u * (uint32_t)J.sd};
         DG dgg, dgs;
@@ -450,7 +449,6 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
   if (P.state[i] != 2) return;
   const DevJob& J = a.jobs[P.job[i]];
   if (!J.cancelled) {
-    const DevPtScene& S = a.scenes[J.scene];
     Sp L;
     bool bad = false;
 #pragma unroll
@@ -470,28 +468,28 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
       Y = Y / BGPU_CIE_Y_SUM;
       Z = Z / BGPU_CIE_Y_SUM;
       float sx = P.sxy[i], sy = P.sxy[Pn + i];
-      float fw = S.fw, fh = S.fh, ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
+      float fw = J.view.fw, fh = J.view.fh, ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
       float dx = sx - 0.5f, dy = sy - 0.5f;
       int x0 = (int)ceilf(dx - fw), x1 = (int)floorf(dx + fw);
       int y0 = (int)ceilf(dy - fh), y1 = (int)floorf(dy + fh);
       x0 = x0 < 0 ? 0 : x0;
       y0 = y0 < 0 ? 0 : y0;
-      x1 = x1 > S.film_w - 1 ? S.film_w - 1 : x1;
-      y1 = y1 > S.film_h - 1 ? S.film_h - 1 : y1;
+      x1 = x1 > J.view.film_w - 1 ? J.view.film_w - 1 : x1;
+      y1 = y1 > J.view.film_h - 1 ? J.view.film_h - 1 : y1;
       for (int y = y0; y <= y1; ++y) {
         int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
         fy = fy < 15 ? fy : 15;
         for (int x = x0; x <= x1; ++x) {
           int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
           fx = fx < 15 ? fx : 15;
-          float w = S.filter[fy * 16 + fx];
-          float* o = J.wxyz + 4 * ((size_t)y * S.film_w + x);
+          float w = J.view.filter[fy * 16 + fx];
+          float* o = J.wxyz + 4 * ((size_t)y * J.view.film_w + x);
           atomicAdd(o, w);
           atomicAdd(o + 1, X * w);
           atomicAdd(o + 2, Y * w);
           atomicAdd(o + 3, Z * w);
           if (J.sq) {
-            float* s2 = J.sq + 3 * ((size_t)y * S.film_w + x);
+            float* s2 = J.sq + 3 * ((size_t)y * J.view.film_w + x);
             atomicAdd(s2, X * X * w);
             atomicAdd(s2 + 1, Y * Y * w);
             atomicAdd(s2 + 2, Z * Z * w);

@@ -846,24 +846,24 @@ __device__ __forceinline__ const float* light_le(const DevPtScene& S, int li, f3
 }
 
 // ------------------------------------------------------------- camera
-__device__ __forceinline__ DRay fire_ray(const DevPtScene& S, float ix, float iy, float lu, float lv) {
-  f3 pc = xform_point(S.r2c, mk3(ix, iy, 0.0f));  // Camera.hs:49-68
+__device__ __forceinline__ DRay fire_ray(const DevView& V, float ix, float iy, float lu, float lv) {
+  f3 pc = xform_point(V.r2c, mk3(ix, iy, 0.0f));  // Camera.hs:49-68
   DRay r;
   r.o = mk3(0.0f, 0.0f, 0.0f);
   r.d = norm3(pc);
   r.tmin = 0.0f;
   r.tmax = INFINITY;
-  if (S.lens_radius > 0.0f) {
+  if (V.lens_radius > 0.0f) {
     float a, b;
     concentric_disk(lu, lv, &a, &b);
-    f3 ro = mk3(a * S.lens_radius, b * S.lens_radius, 0.0f);
-    f3 pf = ray_at(r, S.focal_distance / r.d.z);
+    f3 ro = mk3(a * V.lens_radius, b * V.lens_radius, 0.0f);
+    f3 pf = ray_at(r, V.focal_distance / r.d.z);
     r.o = ro;
     r.d = norm3(sub3(pf, ro));
   }
   DRay w;
-  w.o = xform_point(S.c2w, r.o);
-  w.d = xform_vector(S.c2w, r.d);
+  w.o = xform_point(V.c2w, r.o);
+  w.d = xform_vector(V.c2w, r.d);
   w.tmin = r.tmin;
   w.tmax = r.tmax;
   return w;

@@ -297,37 +297,46 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
                                                       pt.shape_light_host.end(),
                                                       [](int32_t v) { return v >= 0; }));
   H.nmats = (int32_t)d->num_materials;
-  H.cam_type = d->camera.type;
-  std::memcpy(H.c2w, d->camera.c2w, 64);
-  std::memcpy(H.r2c, d->camera.r2c, 64);
-  H.lens_radius = d->camera.lens_radius;
-  H.focal_distance = d->camera.focal_distance;
-  H.xres = d->camera.xres;
-  H.yres = d->camera.yres;
-  // film: filter table and the sample extent
-  const bgpu_film& f = d->film;
-  H.film_w = f.width;
-  H.film_h = f.height;
+  return BGPU_OK;
+}
+
+int make_view(const bgpu_camera& cam, const bgpu_film& f, DevView* v) {
+  if (cam.type != BGPU_CAMERA_PERSPECTIVE) return fail(BGPU_E_INVALID, "unsupported camera type");
+  if (f.width <= 0 || f.height <= 0 || f.width > 65536 || f.height > 65536)
+    return fail(BGPU_E_INVALID, "film size must be in [1, 65536]");
+  if (f.filter < BGPU_FILTER_BOX || f.filter > BGPU_FILTER_TRIANGLE)
+    return fail(BGPU_E_INVALID, "unknown filter type");
+  std::memset(v, 0, sizeof(*v));
+  v->cam_type = cam.type;
+  std::memcpy(v->c2w, cam.c2w, 64);
+  std::memcpy(v->r2c, cam.r2c, 64);
+  v->lens_radius = cam.lens_radius;
+  v->focal_distance = cam.focal_distance;
+  v->film_w = f.width;
+  v->film_h = f.height;
   if (f.filter == BGPU_FILTER_BOX) {
-    H.fw = 0.5f;
-    H.fh = 0.5f;
+    v->fw = 0.5f;
+    v->fh = 0.5f;
   } else {
-    H.fw = f.fparams[0];
-    H.fh = f.fparams[1];
+    v->fw = f.fparams[0];
+    v->fh = f.fparams[1];
   }
+  if (!(v->fw > 0.0f && v->fw <= 64.0f && v->fh > 0.0f && v->fh <= 64.0f))
+    return fail(BGPU_E_INVALID, "filter extent must be in (0, 64]");
   for (int y = 0; y < 16; ++y) {
-    float fy = ((float)y + 0.5f) * H.fh / 16.0f;
+    float fy = ((float)y + 0.5f) * v->fh / 16.0f;
     for (int x = 0; x < 16; ++x) {
-      float fx = ((float)x + 0.5f) * H.fw / 16.0f;
-      H.filter[y * 16 + x] = eval_filter(f, fx, fy);
+      float fx = ((float)x + 0.5f) * v->fw / 16.0f;
+      v->filter[y * 16 + x] = eval_filter(f, fx, fy);
     }
   }
-  int x0 = (int)std::floor(0.5f - H.fw), x1 = (int)std::floor(0.5f + (float)f.width + H.fw);
-  int y0 = (int)std::floor(0.5f - H.fh), y1 = (int)std::floor(0.5f + (float)f.height + H.fh);
-  H.ext_x0 = x0;
-  H.ext_y0 = y0;
-  H.ext_w = x1 - x0 + 1;
-  H.ext_h = y1 - y0 + 1;
+  int x0 = (int)std::floor(0.5f - v->fw), x1 = (int)std::floor(0.5f + (float)f.width + v->fw);
+  int y0 = (int)std::floor(0.5f - v->fh), y1 = (int)std::floor(0.5f + (float)f.height + v->fh);
+  v->ext_x0 = x0;
+  v->ext_y0 = y0;
+  v->ext_w = x1 - x0 + 1;
+  v->ext_h = y1 - y0 + 1;
+  if ((long long)v->ext_w * v->ext_h > (1ll << 30)) return fail(BGPU_E_INVALID, "film too large");
   return BGPU_OK;
 }
 

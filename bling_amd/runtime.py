@@ -141,6 +141,7 @@ class JobSpec:
     passes: int = 1
     seed: int = 1
     variance: bool = False
+    view: object = None  # scene.View; None renders the scene's own camera and film
 
     def desc(self):
         d = abi.JobDesc()
@@ -149,7 +150,16 @@ class JobSpec:
         d.rr_start_depth, d.passes = self.rr_start_depth, self.passes
         d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
         d.flags = abi.JOB_VARIANCE if self.variance else 0
+        if self.view is not None:
+            d.camera = C.pointer(self.view.camera)
+            d.film = C.pointer(self.view.film)
         return d
+
+    def image_shape(self, flat):
+        """(height, width) of the job's film on the FlatScene `flat`."""
+        if self.view is not None:
+            return self.view.film.height, self.view.film.width
+        return flat.film[1], flat.film[0]
 
 
 @dataclass
@@ -168,8 +178,7 @@ class Job:
         h = C.c_void_p()
         _ck(lib.bgpu_job_submit(scene.h, C.byref(spec.desc()), C.byref(h)))
         self.h = h
-        w, hh = scene.flat.film[0], scene.flat.film[1]
-        self.shape = (hh, w)
+        self.shape = spec.image_shape(scene.flat)
 
     def poll(self, want_image=True):
         lib = self.scene.ctx.lib
@@ -209,7 +218,7 @@ class GpuRenderer:
     def render_plain(self, scene):
         """Blocking one-job path: returns ((h, w, 4) wxyz, (h, w, 3) sum of squares or None)."""
         lib = scene.ctx.lib
-        w, h = scene.flat.film[0], scene.flat.film[1]
+        h, w = self.spec.image_shape(scene.flat)
         img = np.zeros((h, w, 4), np.float32)
         sq = np.zeros((h, w, 3), np.float32) if self.spec.variance else None
         _ck(lib.bgpu_render(scene.h, C.byref(self.spec.desc()), img.ctypes.data,

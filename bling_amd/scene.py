@@ -254,8 +254,47 @@ class SceneBuilder:
     def set_film(self, width, height, filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
         self.film = (int(width), int(height), int(filt), tuple(fparams))
 
+    def look(self, c2w, fov, width, height, lens_radius=0.0, focal_distance=1.0,
+             filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
+        """`camera { perspective ... }` plus `film`: sets both and remembers
+        the lens so that views at other image sizes can be made later."""
+        from . import transform as T
+        self.perspective_camera(*T.perspective_camera(c2w, lens_radius, focal_distance, fov, width, height),
+                                lens_radius=lens_radius, focal_distance=focal_distance)
+        self.set_film(width, height, filt, fparams)
+        self.lens = (c2w, float(fov), float(lens_radius), float(focal_distance), int(filt), tuple(fparams))
+
     def build(self):
         return FlatScene(self)
+
+
+class View:
+    """A job's camera and film (bgpu_camera, bgpu_film)."""
+
+    def __init__(self, camera, film):
+        self.camera, self.film = camera, film
+
+    @property
+    def size(self):
+        return self.film.width, self.film.height
+
+    @staticmethod
+    def perspective(c2w, fov, width, height, lens_radius=0.0, focal_distance=1.0,
+                    filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
+        from . import transform as T
+        c2w, r2c = T.perspective_camera(c2w, lens_radius, focal_distance, fov, width, height)
+        cam = abi.Camera()
+        cam.type = abi.CAMERA_PERSPECTIVE
+        for k in range(16):
+            cam.c2w[k] = c2w.m[k]
+            cam.r2c[k] = r2c.m[k]
+        cam.lens_radius = F(lens_radius)
+        cam.focal_distance = F(focal_distance)
+        film = abi.Film()
+        film.width, film.height, film.filter = int(width), int(height), int(filt)
+        for k in range(4):
+            film.fparams[k] = fparams[k]
+        return View(cam, film)
 
 
 def _ptr(a, t):
@@ -331,3 +370,12 @@ class FlatScene:
         self.c2w = getattr(b, "c2w", None)
         self.r2c = getattr(b, "r2c", None)
         self.film = b.film
+        self.lens = getattr(b, "lens", None)
+
+    def view(self, width, height):
+        """The scene's camera re-made for a `width` x `height` image (a
+        RenderJob's camera follows its imageSize, Rendering.hs:133-137)."""
+        if self.lens is None:
+            raise ValueError("scene was not built with SceneBuilder.look")
+        c2w, fov, lr, fd, ft, fp = self.lens
+        return View.perspective(c2w, fov, width, height, lr, fd, ft, fp)

@@ -110,7 +110,9 @@ def unit_box():
 
 
 # --------------------------------------------------------------- scenes
-def cornell(res=256, light_radiance=15.0):
+def cornell(res=256, light_radiance=15.0, height=None):
+    """Cornell box (unit cube, quad light under the ceiling); a `res` x
+    `height` (default `res`) image."""
     sb = SceneBuilder()
     red = sb.material(Material.matte(rgb_refl(0.63, 0.065, 0.05)))
     green = sb.material(Material.matte(rgb_refl(0.14, 0.45, 0.091)))
@@ -136,8 +138,7 @@ def cornell(res=256, light_radiance=15.0):
     sb.shape(abi.SHAPE_QUAD, (0.125, 0.125), o2w=lt, material=lightm,
              emission=rgb_illum(light_radiance, light_radiance, light_radiance))
     c2w = T.look_at((0.5, 0.5, -1.4), (0.5, 0.5, 0.5), (0, 1, 0))
-    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 40.0, res, res))
-    sb.set_film(res, res, abi.FILTER_BOX)
+    sb.look(c2w, 40.0, res, height or res)
     return sb.build()
 
 
@@ -154,8 +155,7 @@ def icosphere_scene(levels=8, num_spheres=10000, seed=1, amplitude=0.05, res=204
     for c, rr in zip(cen, rad):
         sb.shape(abi.SHAPE_SPHERE, (float(rr),), o2w=T.translate(tuple(c)), material=m)
     c2w = T.look_at((0.6, 0.9, -3.4), (0.0, 0.0, 0.0), (0, 1, 0))
-    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 50.0, res, res))
-    sb.set_film(res, res, abi.FILTER_BOX)
+    sb.look(c2w, 50.0, res, res)
     return sb.build()
 
 
@@ -185,6 +185,100 @@ def small_scene(num_tris=2000, num_spheres=200, seed=5, shapes_all=True):
         else:
             p = (s * 0.5, -s, s, float(T.radians(360.0 if i % 3 else 300.0)))
         sb.shape(k, p, o2w=xf, material=m)
+    return sb.build()
+
+
+def heightmap_mesh(elev, ns, nt):
+    """heightMap elev (ns, nt) (Primitive/Heightmap.hs:15-50): an ns x nt
+    vertex grid over [0,1]^2 with y = elev(x, z), two triangles per cell,
+    shading normals from central differences and uv = (x, z) / (ns - 1)."""
+    fns, fnt = float(ns), float(nt)
+    xs = np.arange(ns) / (fns - 1)
+    zs = np.arange(nt) / (fnt - 1)
+    X, Z = np.meshgrid(xs, zs)  # row z, column x: vertex index x + z * ns
+    X, Z = X.reshape(-1), Z.reshape(-1)
+    Y = elev(X, Z)
+    verts = np.stack([X, Y, Z], 1).astype(F)
+    ex, ez = 1.0 / fns, 1.0 / fnt
+    dx = elev(X - ex, Z) - elev(X + ex, Z)
+    dz = elev(X, Z - ez) - elev(X, Z + ez)
+    n = np.stack([dx, np.full_like(X, ex + ez), dz], 1)
+    n = -(n / np.linalg.norm(n, axis=1, keepdims=True))
+    uvs = np.stack([X / (fns - 1), Z / (fns - 1)], 1)
+    x, y = np.meshgrid(np.arange(ns - 1), np.arange(nt - 1))
+    x, y = x.reshape(-1), y.reshape(-1)
+    v = lambda a, b: a + b * ns
+    first = np.stack([v(x, y), v(x + 1, y), v(x + 1, y + 1)], 1)
+    second = np.stack([v(x, y), v(x + 1, y + 1), v(x, y + 1)], 1)
+    faces = np.stack([first, second], 1).reshape(-1, 3).astype(np.int32)
+    return verts, faces, n.astype(F), uvs.astype(F)
+
+
+def _lit_room(sb, size, light_radiance=12.0, height=None):
+    """Floor, a quad light above, a small sphere light; Lambertian."""
+    floor = sb.material(Material.matte(rgb_refl(0.7, 0.7, 0.7)))
+    h = height if height is not None else size
+    sb.mesh(np.array([[-size, 0, -size], [size, 0, -size], [size, 0, size], [-size, 0, size]], F),
+            np.array([[0, 1, 2], [0, 2, 3]], np.int32), material=floor)
+    # light sampling emits towards the quad's local -z (Shape.hs:405, Light.hs:153)
+    lt = T.rotate_x(-90) @ T.translate((0.0, h * 1.5, 0.0))
+    sb.shape(abi.SHAPE_QUAD, (size * 0.3, size * 0.3), o2w=lt, material=floor,
+             emission=rgb_illum(light_radiance, light_radiance, light_radiance))
+    sb.shape(abi.SHAPE_SPHERE, (size * 0.05,), o2w=T.translate((size * 0.6, h * 0.8, -size * 0.4)),
+             material=floor, emission=const(light_radiance * 4))
+    return floor
+
+
+def stack_scene(num_tris, res=256, seed=0):
+    """Random axis-aligned box stacks (12 triangles per box) on a lit floor."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    _lit_room(sb, 2.0)
+    mats = [sb.material(Material.matte(rgb_refl(*rng.uniform(0.2, 0.9, 3)))) for _ in range(4)]
+    nbox = max(1, num_tris // 12)
+    bv, bf = unit_box()
+    cols = int(np.ceil(np.sqrt(max(1, nbox // 8))))
+    centres, sizes = [], []
+    for k in range(nbox):
+        c = k % max(1, cols * cols)
+        i, j = divmod(c, cols)
+        level = k // max(1, cols * cols)
+        s = rng.uniform(0.3, 0.9) * 3.2 / cols
+        centres.append((-1.6 + (i + 0.5) * 3.2 / cols, (level + 0.5) * s, -1.6 + (j + 0.5) * 3.2 / cols))
+        sizes.append(s)
+    centres = np.array(centres, F)
+    sizes = np.array(sizes, F)
+    verts = (bv[None, :, :] * sizes[:, None, None] + centres[:, None, :]).reshape(-1, 3)
+    faces = (bf[None, :, :] + 8 * np.arange(nbox)[:, None, None]).reshape(-1, 3)
+    m = np.array(mats)[np.arange(nbox) % len(mats)]
+    # one mesh per material keeps mkTriangleMesh's one-material-per-mesh rule
+    for mi in mats:
+        sel = np.nonzero(m == mi)[0]
+        if len(sel):
+            sb.mesh(verts.reshape(nbox, 8, 3)[sel].reshape(-1, 3),
+                    (bf[None] + 8 * np.arange(len(sel))[:, None, None]).reshape(-1, 3), material=mi)
+    c2w = T.look_at((3.5, 3.0, -4.5), (0.0, 0.6, 0.0), (0, 1, 0))
+    sb.look(c2w, 45.0, res, res)
+    return sb.build()
+
+
+def grid_scene(num_tris, res=256, seed=0):
+    """A displaced heightmap grid (fBm heights) with shading normals, lit."""
+    n = max(2, int(np.sqrt(num_tris / 2.0)) + 1)
+    noise_seed = seed + 1
+    sb = SceneBuilder()
+    _lit_room(sb, 2.0, height=1.0)
+    m = sb.material(Material.matte(rgb_refl(0.5, 0.6, 0.4)))
+
+    def elev(x, z):
+        p = np.stack([x * 4.0, z * 4.0, np.zeros_like(x)], 1)
+        return 0.25 * fbm(p, octaves=6, omega=0.5, seed=noise_seed)
+
+    v, f, nrm, uv = heightmap_mesh(elev, n, n)
+    o2w = T.scale((3.0, 1.0, 3.0)) @ T.translate((-1.5, 0.1, -1.5))
+    sb.mesh(v, f, o2w=o2w, material=m, normals=nrm, uvs=uv)
+    c2w = T.look_at((0.0, 2.5, -3.5), (0.0, 0.2, 0.0), (0, 1, 0))
+    sb.look(c2w, 50.0, res, res)
     return sb.build()
 
 

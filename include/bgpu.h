@@ -201,6 +201,10 @@ typedef struct {
   uint64_t seed;
   int32_t flags;
   int32_t pad;
+  /* Per-job camera and film (a RenderJob's camera and imageSize,
+   * Rendering.hs:133-137); NULL selects the scene description's. */
+  const bgpu_camera* camera;
+  const bgpu_film* film;
 } bgpu_job_desc;
 
 enum {

@@ -1158,15 +1158,18 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
     c.shapeLight[i] = (int)c.lights.size();
     c.lights.push_back(l);
   }
-  filterSize(s.film, &c.fw, &c.fh);
+  // a job's camera and film follow its imageSize (Rendering.hs:133-137)
+  const bgpu_film& film = job->film ? *job->film : s.film;
+  const bgpu_camera& camera = job->camera ? *job->camera : s.camera;
+  filterSize(film, &c.fw, &c.fh);
   for (int y = 0; y < 16; ++y) {  // mkTableFilter (Image.hs:48-61)
     float fy = ((float)y + 0.5f) * c.fh / 16.0f;
     for (int x = 0; x < 16; ++x) {
       float fx = ((float)x + 0.5f) * c.fw / 16.0f;
-      c.filter[y * 16 + x] = evalFilter(s.film, fx, fy);
+      c.filter[y * 16 + x] = evalFilter(film, fx, fy);
     }
   }
-  const int W = s.film.width, H = s.film.height;
+  const int W = film.width, H = film.height;
   // sampleExtent (Image.hs:162-168)
   int x0 = (int)std::floor(0.5f - c.fw), x1 = (int)std::floor(0.5f + (float)W + c.fw);
   int y0 = (int)std::floor(0.5f - c.fh), y1 = (int)std::floor(0.5f + (float)H + c.fh);
@@ -1184,7 +1187,7 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
         ps.camera(n, &ox, &oy);
         ps.lens(n, &lu, &lv);
         float sx = (float)ix + ox, sy = (float)iy + oy;
-        Ray r = fireRay(s.camera, sx, sy, lu, lv);
+        Ray r = fireRay(camera, sx, sy, lu, lv);
         Spec L = li(c, r, ps, n, md, rr);
         addSample(c, wxyz, sq, W, H, sx, sy, L);
       }

@@ -100,7 +100,7 @@ def render(flat, spec, threads=None, passes=None):
     squared XYZ, ray queries traced)."""
     lib = load()
     o = OracleScene(flat)
-    w, h = flat.film[0], flat.film[1]
+    h, w = spec.image_shape(flat)
     img = np.zeros((h, w, 4), np.float32)
     sq = np.zeros((h, w, 3), np.float32)
     rays = C.c_uint64()

@@ -163,3 +163,38 @@ def test_materials_scene_matches_oracle(gpu_ctx, env):
     frac, zmean, zmax = compare(g, gsq, o2, osq2)
     assert frac < 0.015, (frac, zmean, zmax)
     assert abs(zmean) < 0.08, zmean
+
+
+def test_job_view_tracks_oracle_and_scene_film(gpu_ctx):
+    """A job's own camera and film: same image as a scene described at that
+    size, on the plain and the served path, and tracking the oracle."""
+    import dataclasses
+    other = scenes.cornell(48)
+    direct = scenes.cornell(40, height=24)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=2,
+                   seed=13, variance=True, view=other.view(40, 24))
+    g, gsq = render_gpu(gpu_ctx, other, spec)
+    assert g.shape == (24, 40, 4)
+    o, osq, _ = pyoracle.render(other, spec)
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    d, _ = render_gpu(gpu_ctx, direct, dataclasses.replace(spec, view=None))
+    np.testing.assert_allclose(g, d, rtol=1e-4, atol=1e-5)
+    gs = gpu_ctx.scene(other)
+    try:
+        j = Job(gs, spec)
+        j.wait(timeout=120)
+        img, _, done, state = j.poll()
+        assert state == abi.JOB_DONE and img.shape == (24, 40, 4)
+        np.testing.assert_allclose(img, g, rtol=1e-4, atol=1e-5)
+        j.release()
+        bad = other.view(16, 16)
+        bad.film.width = 0
+        with pytest.raises(abi.BgpuError) as e:
+            GpuRenderer(dataclasses.replace(spec, view=bad)).render_plain(gs)
+        assert e.value.code == abi.E_INVALID
+    finally:
+        gs.close()

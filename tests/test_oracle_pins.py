@@ -179,3 +179,20 @@ def test_mesh_validation_matches_reference_errors():
     flat.desc.vertices = flat.vertices.ctypes.data_as(C.POINTER(C.c_float))
     with pytest.raises(ValueError, match="out of bounds"):
         OracleScene(flat)
+
+
+def test_job_view_equals_scene_built_at_that_size():
+    """A job's own camera and film (the RenderJob's imageSize) give exactly
+    the image of a scene described at that size (one thread: same order)."""
+    from bling_amd import scenes
+    from bling_amd.runtime import JobSpec
+    direct = scenes.cornell(24, height=16)
+    other = scenes.cornell(48)
+    spec = JobSpec(spp_nu=2, spp_nv=1, max_depth=3, sample_depth=2, rr_start_depth=2, seed=3)
+    a, asq, _ = pyoracle.render(direct, spec, threads=1)
+    spec_v = JobSpec(spp_nu=2, spp_nv=1, max_depth=3, sample_depth=2, rr_start_depth=2, seed=3,
+                     view=other.view(24, 16))
+    b, bsq, _ = pyoracle.render(other, spec_v, threads=1)
+    assert a.shape == (16, 24, 4)
+    assert np.array_equal(a, b) and np.array_equal(asq, bsq)
+    assert a[..., 2].sum() > 0
