@@ -9,6 +9,19 @@ HBM: closest hit and any hit on 2048x2048 coherent pinhole primary rays and
 on 4,194,304 incoherent rays (origin uniform in the bounds, direction uniform
 on the sphere).  `value` is ray queries per second over the whole step.
 
+Two more legs are reported in the same line (skip them with --legs):
+  render  the Cornell box workload (256x256, 4x4 stratified samples per pixel,
+          max depth 5, Russian roulette after depth 3) on the plain path:
+          samples/s, device-counted Mrays/s, and the image RMSE (luminance)
+          against a 4096-spp oracle render committed under tests/golden;
+  serve   a Poisson stream of render jobs over 8 resident scenes (the Cornell
+          box and 7 procedural box-stack / displaced-grid / icosphere scenes of
+          0.1M-8M triangles), each job with its own resolution and spp, run
+          once through the served path (continuous batching) and once through
+          the plain path (one job at a time, in arrival order): Mrays/s,
+          samples/s and p50/p99 job latency per arrival rate, and whether
+          every job's image is the same on both paths.
+
 Prints one JSON line.  Only N=1 is supported: this build targets one GPU.
 """
 import argparse
@@ -25,13 +38,21 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s and p50/p99 job latency, served vs plain GPU path at equal RMSE"
 PEAK_HBM_GBS = 8000.0
 # algorithmic bytes per unit of work (BVH2 node = 2 child boxes of 24 B + 2 links
-# of 4 B; triangle = 9 floats; shape = type/param index 4 B + w2o 48 B + params 16 B)
+# of 4 B; triangle = 9 floats)
 B_RAY_IN, B_HIT_OUT, B_OCC_OUT = 32, 16, 1
-B_NODE, B_TRI, B_SHAPE = 56, 36, 68
+B_NODE, B_TRI = 56, 36
+CORNELL_REF = os.path.join(ROOT, "tests", "golden", "cornell256_4096spp.npz")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    """CPU threads for the baseline: the host's share (OMP_NUM_THREADS when
+    set), never more than the cores visible."""
+    n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
 
 
 def main():
@@ -39,7 +60,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--res", type=int, default=2048)
@@ -48,15 +69,54 @@ def main():
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy, the reference for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--legs", default="intersect,render,serve",
+                    help="comma-separated subset of intersect,render,serve")
+    ap.add_argument("--render-steps", type=int, default=10)
+    ap.add_argument("--serve-jobs", type=int, default=32, help="jobs per arrival rate")
+    ap.add_argument("--serve-rates", default="5,50,200", help="Poisson arrival rates, jobs/s")
+    ap.add_argument("--serve-max-tris", type=float, default=8e6)
+    ap.add_argument("--serve-res", default="128,256,512")
+    ap.add_argument("--serve-spp", default="1,4,16")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported (this build runs on one GPU; "
               "the multi-GPU sharding is a design note, not a code path)", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
+
+    from bling_amd.runtime import Context
+    ctx = Context(0)
+    out = {
+        "metric": METRIC,
+        "value": None,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {},
+    }
+    if "intersect" in legs:
+        intersect_leg(ctx, args, out)
+    if "render" in legs:
+        out["config"]["render"] = render_leg(ctx, args)
+    if "serve" in legs:
+        out["config"]["serve"] = serve_leg(ctx, args)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------ intersection
+def intersect_leg(ctx, args, out):
+    import ctypes as C
 
     import torch
-    from bling_amd import scenes
-    from bling_amd.runtime import Context
+    from bling_amd import abi, scenes
 
     t0 = time.time()
     flat = scenes.icosphere_scene(levels=args.levels, num_spheres=10000, res=args.res)
@@ -69,7 +129,6 @@ def main():
     log("workload built in %.1fs: %d tris, %d shapes, 2x%d rays" % (
         time.time() - t0, flat.num_triangles, flat.num_shapes, nr))
 
-    ctx = Context(0)
     t0 = time.time()
     gs = ctx.scene(flat)
     build_s = time.time() - t0
@@ -93,15 +152,13 @@ def main():
         launches.append(("any/" + name, lambda rp=rp, op=occ.data_ptr(), **kw: gs.intersects_dev(rp, nr, op, **kw)))
 
     # counting pass: node visits and primitive tests per launch (not timed)
-    from bling_amd import abi
-    import ctypes as C
     work = {}
     for label, fn in launches:
         ctx.reset_stats()
-        rp_kind = label.split("/")[0]
-        cols, outs, occ = dbat[label.split("/")[1]]
+        kind, batch = label.split("/")
+        cols, outs, occ = dbat[batch]
         r = abi.Rays(*[c.data_ptr() for c in cols])
-        if rp_kind == "closest":
+        if kind == "closest":
             h = abi.Hits(*[o.data_ptr() for o in outs])
             abi.check(ctx.lib.bgpu_intersect_closest(gs.h, C.byref(r), nr, C.byref(h),
                                                      abi.DEVICE_PTRS | abi.COUNT))
@@ -110,15 +167,13 @@ def main():
                                                  abi.DEVICE_PTRS | abi.COUNT))
         st = ctx.stats()
         work[label] = (st["nodes_visited"], st["prims_tested"])
-    # primitive tests split into triangle and shape tests is not counted; price
-    # every test as a triangle except the shape share estimated from hit counts
-    t_c, b1_c, b2_c, p_c = [o.cpu().numpy() for o in dbat["incoherent"][1]]
-    shape_frac = float(((p_c >= flat.num_triangles) & (p_c >= 0)).mean())
+    p_c = dbat["incoherent"][1][3].cpu().numpy()
+    shape_frac = float((p_c >= flat.num_triangles).mean())
 
     def alg_bytes(label):
         nodes, prims = work[label]
-        out = B_HIT_OUT if label.startswith("closest") else B_OCC_OUT
-        return nr * (B_RAY_IN + out) + nodes * B_NODE + prims * B_TRI
+        o = B_HIT_OUT if label.startswith("closest") else B_OCC_OUT
+        return nr * (B_RAY_IN + o) + nodes * B_NODE + prims * B_TRI
 
     if args.copy_ref:
         # an elementwise kernel that reads exactly 1 GiB and writes exactly 1 GiB
@@ -144,13 +199,13 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     queries = len(launches) * nr * args.steps
-    value = queries / wall / 1e6
     avg = {k: float(np.mean(v)) for k, v in kms.items()}
     dom = max(avg, key=lambda k: avg[k])
     achieved = alg_bytes(dom) / (avg[dom] * 1e-3) / 1e9
     per_launch = {k: {"ms": round(avg[k], 4), "mrays_s": round(nr / avg[k] / 1e3, 1),
                       "nodes_per_ray": round(work[k][0] / nr, 2),
                       "prims_per_ray": round(work[k][1] / nr, 2),
+                      "alg_bytes": int(alg_bytes(k)),
                       "alg_GBps": round(alg_bytes(k) / (avg[k] * 1e-3) / 1e9, 1)} for k in avg}
     log("per launch:", json.dumps(per_launch))
 
@@ -158,57 +213,38 @@ def main():
     pmc_path = os.path.join(ROOT, "profiles", "closest_incoherent_c2_pmc.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as f:
-            pmc = json.load(f)
-        traffic = pmc.get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(flat, batches, args.cpu_seconds)
-
-    out = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "intersection microbench: displaced icosphere %d tris + %d spheres, "
-                        "binned-SAH BVH2 (12 bins); closest+any hit on %d coherent (%dx%d pinhole) "
-                        "and %d incoherent rays per step" % (flat.num_triangles, flat.num_shapes,
-                                                             nr, args.res, args.res, nr),
-            "rays_per_step": 4 * nr,
-            "bvh_build_s": round(build_s, 2),
-            "per_launch": per_launch,
-            "shape_hit_fraction_incoherent": round(shape_frac, 4),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": traffic,
-        },
-        "cpu_baseline": cpu,
+    out["value"] = round(queries / wall / 1e6, 2)
+    out["ms_per_step"] = round(wall / args.steps * 1e3, 3)
+    out["config"].update({
+        "workload": "intersection microbench: displaced icosphere %d tris + %d spheres, "
+                    "binned-SAH BVH2 (12 bins); closest+any hit on %d coherent (%dx%d pinhole) "
+                    "and %d incoherent rays per step" % (flat.num_triangles, flat.num_shapes,
+                                                         nr, args.res, args.res, nr),
+        "rays_per_step": len(launches) * nr,
+        "bvh_build_s": round(build_s, 2),
+        "per_launch": per_launch,
+        "shape_hit_fraction_incoherent": round(shape_frac, 4),
+    })
+    out["roofline"] = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": traffic,
     }
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(flat, batches, args.cpu_seconds)
     gs.close()
-    ctx.close()
-    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(flat, batches, seconds):
     """The oracle (C++ restatement, OpenMP over rays) on a bounded sample of
     the same four query batches, on this host's cores."""
     from oracle.pyoracle import OracleScene
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     o = OracleScene(flat)
     n = 4096
     rng = np.random.default_rng(0)
@@ -231,6 +267,210 @@ def cpu_baseline(flat, batches, seconds):
             "sample": "%d rays from each of the 4 query batches (closest+any, coherent+incoherent), "
                       "oracle C++ restatement with its own median BVH, %d OpenMP threads"
                       % (total_q // 4, threads)}
+
+
+# ------------------------------------------------------------------ render
+def rays_of(st):
+    return st["rays_camera"] + st["rays_extend"] + st["rays_shadow"] + st["rays_mis"]
+
+
+def luminance(img):
+    w = np.maximum(img[..., 0], 1e-30)
+    return img[..., 2] / w
+
+
+def render_leg(ctx, args):
+    """Cornell box 256x256, 16 spp (4x4), max depth 5, RR after depth 3."""
+    from bling_amd import scenes
+    from bling_amd.runtime import GpuRenderer, JobSpec
+    flat = scenes.cornell(256)
+    gs = ctx.scene(flat)
+    mk = lambda seed: JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=3, rr_start_depth=3,
+                              passes=1, seed=seed)
+    for k in range(2):
+        GpuRenderer(mk(1000 + k)).render_plain(gs)
+    ctx.reset_stats()
+    imgs = []
+    t0 = time.perf_counter()
+    for k in range(args.render_steps):
+        img, _ = GpuRenderer(mk(1 + k)).render_plain(gs)
+        imgs.append(img)
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    gs.close()
+    res = {
+        "workload": "Cornell box 256x256, 4x4 stratified spp, max depth 5, RR after depth 3, "
+                    "plain path, one render per step",
+        "steps": args.render_steps,
+        "ms_per_render": round(wall / args.render_steps * 1e3, 3),
+        "msamples_s": round(st["samples"] / wall / 1e6, 2),
+        "mrays_s": round(rays_of(st) / wall / 1e6, 2),
+        "rays_per_sample": round(rays_of(st) / max(1, st["samples"]), 3),
+        "ray_mix": {k: st[k] for k in ("rays_camera", "rays_extend", "rays_shadow", "rays_mis")},
+    }
+    if os.path.exists(CORNELL_REF):
+        ref = np.load(CORNELL_REF)["y"]
+        rm = [float(np.sqrt(np.mean((luminance(i) - ref) ** 2))) for i in imgs]
+        res["rmse_y_vs_4096spp"] = round(float(np.mean(rm)), 6)
+        res["reference_image"] = "oracle CPU render, 4096 spp (tests/golden/cornell256_4096spp.npz)"
+        if not args.no_cpu:
+            res["cpu"] = render_cpu(flat, mk, ref)
+    log("render:", json.dumps(res))
+    return res
+
+
+def render_cpu(flat, mk, ref):
+    """The oracle integrator on the same job (16 spp, all 66049 extent
+    pixels): samples/s on this host, and its image RMSE at equal spp."""
+    from oracle import pyoracle
+    threads = host_threads()
+    t0 = time.perf_counter()
+    img, _, rays = pyoracle.render(flat, mk(1), threads=threads)
+    dt = time.perf_counter() - t0
+    n = (flat.film[0] + 1) * (flat.film[1] + 1) * 16
+    return {"msamples_s": round(n / dt / 1e6, 4), "mrays_s": round(rays / dt / 1e6, 3),
+            "cores": threads, "kind": "port", "seconds": round(dt, 2),
+            "rmse_y_vs_4096spp": round(float(np.sqrt(np.mean((luminance(img) - ref) ** 2))), 6)}
+
+
+# ------------------------------------------------------------------- serve
+def serve_scenes(max_tris):
+    """The Cornell box plus 7 procedural scenes, 0.1M..max_tris triangles
+    (log-spaced), alternating box stacks, displaced grids and an icosphere."""
+    from bling_amd import scenes
+    sizes = np.exp(np.linspace(np.log(1e5), np.log(max_tris), 7))
+    out = [("cornell", scenes.cornell(256))]
+    kinds = ["stack", "grid", "icosphere", "stack", "grid", "stack", "grid"]
+    for k, (kind, n) in enumerate(zip(kinds, sizes)):
+        if kind == "stack":
+            out.append(("stack", scenes.stack_scene(int(n), seed=k)))
+        elif kind == "grid":
+            out.append(("grid", scenes.grid_scene(int(n), seed=k)))
+        else:
+            lv = int(np.clip(np.round(np.log(n / 20.0) / np.log(4.0)), 3, 9))
+            out.append(("icosphere", scenes.icosphere_scene(levels=lv, num_spheres=1000, seed=k, res=256)))
+    return out
+
+
+def job_stream(n, rate, nscenes, res_set, spp_set, seed):
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    jobs = []
+    for k in range(n):
+        jobs.append({"arrival": float(arrivals[k]), "scene": int(rng.integers(nscenes)),
+                     "res": int(rng.choice(res_set)), "spp": int(rng.choice(spp_set)),
+                     "seed": int(seed * 100003 + k)})
+    return jobs
+
+
+def spec_for(job, flat):
+    from bling_amd.runtime import JobSpec
+    s = int(round(np.sqrt(job["spp"])))
+    nu, nv = (s, job["spp"] // s) if s * (job["spp"] // s) == job["spp"] else (job["spp"], 1)
+    return JobSpec(spp_nu=nu, spp_nv=nv, max_depth=5, sample_depth=3, rr_start_depth=3, passes=1,
+                   seed=job["seed"], view=flat.view(job["res"], job["res"]))
+
+
+def lat_stats(lat):
+    a = np.array(lat) * 1e3
+    return {"p50_ms": round(float(np.percentile(a, 50)), 2), "p99_ms": round(float(np.percentile(a, 99)), 2),
+            "mean_ms": round(float(a.mean()), 2)}
+
+
+def run_plain(ctx, gss, flats, jobs):
+    from bling_amd.runtime import GpuRenderer
+    ctx.reset_stats()
+    t_start = time.perf_counter()
+    lat, imgs, t_end = [], [], t_start
+    for job in jobs:
+        now = time.perf_counter() - t_start
+        if now < job["arrival"]:
+            time.sleep(job["arrival"] - now)
+        img, _ = GpuRenderer(spec_for(job, flats[job["scene"]])).render_plain(gss[job["scene"]])
+        t_end = time.perf_counter() - t_start
+        lat.append(t_end - job["arrival"])
+        imgs.append(img)
+    st = ctx.stats()
+    return lat, imgs, t_end, st
+
+
+def run_served(ctx, gss, flats, jobs):
+    from bling_amd.runtime import Job
+    ctx.reset_stats()
+    t_start = time.perf_counter()
+    handles, lag = [], []
+    for job in jobs:
+        now = time.perf_counter() - t_start
+        if now < job["arrival"]:
+            time.sleep(job["arrival"] - now)
+        lag.append(time.perf_counter() - t_start - job["arrival"])
+        handles.append(Job(gss[job["scene"]], spec_for(job, flats[job["scene"]])))
+    lat, imgs = [], []
+    for h, lg in zip(handles, lag):
+        h.wait(timeout=600)
+    t_end = time.perf_counter() - t_start
+    for h, lg in zip(handles, lag):
+        lat.append(lg + h.latency_ms() * 1e-3)
+        imgs.append(h.poll()[0])
+        h.release()
+    st = ctx.stats()
+    makespan = max(j["arrival"] + l for j, l in zip(jobs, lat))
+    return lat, imgs, min(t_end, makespan), st
+
+
+def serve_leg(ctx, args):
+    t0 = time.time()
+    named = serve_scenes(args.serve_max_tris)
+    flats = [f for _, f in named]
+    log("serve: %d scenes generated in %.1fs" % (len(flats), time.time() - t0))
+    t0 = time.time()
+    gss = [ctx.scene(f) for f in flats]
+    log("serve: uploaded + BVHs in %.1fs, %.1f MB" % (time.time() - t0,
+                                                       sum(g.device_bytes() for g in gss) / 1e6))
+    res_set = [int(v) for v in args.serve_res.split(",")]
+    spp_set = [int(v) for v in args.serve_spp.split(",")]
+    # warm both paths (engine creation, graph capture)
+    warm = job_stream(2, 1000.0, len(flats), [128], [1], 999)
+    run_plain(ctx, gss, flats, warm)
+    run_served(ctx, gss, flats, warm)
+    rows = []
+    for ri, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
+        jobs = job_stream(args.serve_jobs, rate, len(flats), res_set, spp_set, seed=ri + 1)
+        p_lat, p_img, p_span, p_st = run_plain(ctx, gss, flats, jobs)
+        s_lat, s_img, s_span, s_st = run_served(ctx, gss, flats, jobs)
+        worst = 0.0
+        for a, b in zip(s_img, p_img):
+            d = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+            worst = max(worst, float(d.max()))
+        row = {
+            "rate_jobs_s": rate,
+            "jobs": len(jobs),
+            "samples": int(s_st["samples"]),
+            "served": dict(mrays_s=round(rays_of(s_st) / s_span / 1e6, 2),
+                           msamples_s=round(s_st["samples"] / s_span / 1e6, 2),
+                           makespan_s=round(s_span, 3), **lat_stats(s_lat)),
+            "plain": dict(mrays_s=round(rays_of(p_st) / p_span / 1e6, 2),
+                          msamples_s=round(p_st["samples"] / p_span / 1e6, 2),
+                          makespan_s=round(p_span, 3), **lat_stats(p_lat)),
+            "rays_served": int(rays_of(s_st)),
+            "rays_plain": int(rays_of(p_st)),
+            "image_max_rel_diff": worst,
+        }
+        row["speedup_mrays"] = round(row["served"]["mrays_s"] / max(1e-9, row["plain"]["mrays_s"]), 3)
+        log("serve:", json.dumps(row))
+        rows.append(row)
+    for g in gss:
+        g.close()
+    return {
+        "workload": "Poisson job stream over 8 resident scenes (%s), resolution uniform in %s, "
+                    "spp uniform in %s, max depth 5, RR after depth 3; served = continuous batching "
+                    "through bgpu_job_submit, plain = bgpu_render one job at a time in arrival order; "
+                    "same seeds on both paths (equal RMSE: the images agree to film-atomic rounding)"
+                    % (", ".join("%s %d tris" % (k, f.num_triangles) for k, f in named),
+                       res_set, spp_set),
+        "rates": rows,
+    }
 
 
 if __name__ == "__main__":

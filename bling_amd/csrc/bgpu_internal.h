@@ -50,7 +50,9 @@ struct DevBuf {
 struct Scene;
 struct Job;
 struct Scheduler;
+struct Engine;
 void destroy_scheduler(Scheduler* s);
+void destroy_engine(Engine* e);
 
 struct Ctx {
   int device = 0;
@@ -67,6 +69,8 @@ struct Ctx {
   std::mutex stats_mu;
   bgpu_stats stats{};
   Scheduler* sched = nullptr;     // served path, created on first job_submit
+  Engine* plain = nullptr;        // plain path engine, reused by bgpu_render calls
+  std::mutex plain_mu;            // one plain render at a time
   TraceConfig trace_cfg;
   PtConfig pt_cfg;
   ~Ctx();

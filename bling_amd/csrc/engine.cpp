@@ -30,9 +30,14 @@ struct HostJob {
   size_t px = 0;           // film pixels
   int slot = -1;
   unsigned long long total = 0, per_pass = 0;
-  DevBuf film, sq, pass_done, skipped;
+  DevBuf film, sq;
   std::vector<unsigned int> h_pass;  // host copy of per-pass counts
   unsigned long long h_skipped = 0;
+  uint32_t prog_off = 0;  // the job's range in the engine's progress arena
+  bool has_prog = false;
+  bool cancel_sent = false;
+  unsigned long long samples_seen = 0;  // finished + skipped at the last poll
+  int passes_seen = 0;                  // complete passes at the last poll
   int state = BGPU_JOB_QUEUED;
   int passes_done = 0;
   double t_submit = 0.0, t_done = 0.0;
@@ -59,8 +64,6 @@ struct HostJob {
   virtual ~HostJob() {
     film.release();
     sq.release();
-    pass_done.release();
-    skipped.release();
     if (scene && --scene->refs == 0) {
       (void)hipSetDevice(scene->ctx->device);
       delete scene;
@@ -68,23 +71,31 @@ struct HostJob {
   }
 };
 
+// Per-pass sample counters of every job in an engine live in one device
+// arena, so that the progress of all jobs in flight is read with one copy:
+// words [0, kSkipWords) hold each slot's 64-bit count of samples skipped after
+// cancellation, the rest is handed out in per-job ranges of `passes` words.
+constexpr uint32_t kProgWords = 1u << 20;
+constexpr uint32_t kSkipWords = 2u * kMaxJobs;
+constexpr int kMaxPasses = 1 << 16;
+
 struct Engine {
   Ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
   int P = 0, block = 128;
-  DevBuf pool_buf, jobs_buf, ring_buf, scenes_buf, cnt_buf, admit_buf;
+  DevBuf pool_buf, jobs_buf, ring_buf, scenes_buf, cnt_buf, admit_buf, prog_buf;
   PtKernelArgs args{};
   AdmitReq* h_admit = nullptr;
   PtCounters* h_cnt = nullptr;
   DevJob* h_job = nullptr;
   DevPtScene* h_scene = nullptr;
-  unsigned int* h_pass = nullptr;  // pinned staging for progress reads
-  size_t h_pass_cap = 0;
+  uint32_t* h_prog = nullptr;  // pinned copy of the progress arena
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<Scene*> scenes;             // table index -> scene
   std::vector<HostJob*> slots;            // job slot -> job
   std::vector<int> free_slots;
+  std::vector<std::pair<uint32_t, uint32_t>> arena_used;  // sorted (offset, words)
   // Retired slots wait until a bounce iteration has run after retirement:
   // its reset step drops the finished job from the device's job ring, after
   // which the slot's table entry may be overwritten.
@@ -98,7 +109,7 @@ struct Engine {
     if (h_cnt) (void)hipHostFree(h_cnt);
     if (h_job) (void)hipHostFree(h_job);
     if (h_scene) (void)hipHostFree(h_scene);
-    if (h_pass) (void)hipHostFree(h_pass);
+    if (h_prog) (void)hipHostFree(h_prog);
     for (Scene* s : scenes)
       if (s && --s->refs == 0) delete s;
     pool_buf.release();
@@ -107,6 +118,7 @@ struct Engine {
     scenes_buf.release();
     cnt_buf.release();
     admit_buf.release();
+    prog_buf.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -115,7 +127,7 @@ struct Engine {
     P = std::max(1024, (pool + 255) / 256 * 256);
     block = blk;
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 11 int arrays + 91 float arrays of P, plus 3 queues
+    // pool layout: 8 int arrays + 91 float arrays of P, plus 3 queues
     size_t per = (size_t)P * 4;
     size_t total = per * (8 + 91 + 3);
     if (int rc = pool_buf.alloc(c, total)) return rc;
@@ -152,6 +164,7 @@ struct Engine {
     if (int rc = scenes_buf.alloc(c, sizeof(DevPtScene) * kMaxScenes)) return rc;
     if (int rc = cnt_buf.alloc(c, sizeof(PtCounters))) return rc;
     if (int rc = admit_buf.alloc(c, sizeof(AdmitReq) * kMaxJobs)) return rc;
+    if (int rc = prog_buf.alloc(c, sizeof(uint32_t) * kProgWords)) return rc;
     args.scenes = scenes_buf.as<DevPtScene>();
     args.jobs = jobs_buf.as<DevJob>();
     args.ring = ring_buf.as<uint32_t>();
@@ -161,8 +174,11 @@ struct Engine {
     BGPU_HIP(hipHostMalloc((void**)&h_cnt, sizeof(PtCounters), hipHostMallocDefault));
     BGPU_HIP(hipHostMalloc((void**)&h_job, sizeof(DevJob), hipHostMallocDefault));
     BGPU_HIP(hipHostMalloc((void**)&h_scene, sizeof(DevPtScene), hipHostMallocDefault));
+    BGPU_HIP(hipHostMalloc((void**)&h_prog, sizeof(uint32_t) * kProgWords, hipHostMallocDefault));
+    std::memset(h_cnt, 0, sizeof(PtCounters));
     slots.assign(kMaxJobs, nullptr);
     for (int i = kMaxJobs - 1; i >= 0; --i) free_slots.push_back(i);
+    BGPU_HIP(hipMemsetAsync(prog_buf.ptr, 0, sizeof(uint32_t) * kSkipWords, stream));
     BGPU_HIP(pt_enqueue_init(args, stream));
     // capture one bounce iteration
     BGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -174,6 +190,38 @@ struct Engine {
     BGPU_HIP(hipStreamSynchronize(stream));
     return BGPU_OK;
   }
+
+  // First fit in the progress arena; false when no gap of `n` words is left.
+  bool arena_alloc(uint32_t n, uint32_t* off) {
+    uint32_t cur = kSkipWords;
+    auto it = arena_used.begin();
+    for (; it != arena_used.end(); ++it) {
+      if (it->first - cur >= n) break;
+      cur = it->first + it->second;
+    }
+    if (it == arena_used.end() && kProgWords - cur < n) return false;
+    arena_used.insert(it, {cur, n});
+    *off = cur;
+    return true;
+  }
+  bool arena_fits(uint32_t n) const {
+    uint32_t cur = kSkipWords;
+    for (const auto& u : arena_used) {
+      if (u.first - cur >= n) return true;
+      cur = u.first + u.second;
+    }
+    return kProgWords - cur >= n;
+  }
+  void arena_free(uint32_t off) {
+    for (auto it = arena_used.begin(); it != arena_used.end(); ++it)
+      if (it->first == off) {
+        arena_used.erase(it);
+        return;
+      }
+  }
+  uint32_t arena_end() const { return arena_used.empty() ? kSkipWords : arena_used.back().first + arena_used.back().second; }
+
+  bool can_admit(const HostJob* j) const { return !free_slots.empty() && arena_fits((uint32_t)j->desc.passes); }
 
   int scene_index(Scene* s) {
     for (size_t i = 0; i < scenes.size(); ++i)
@@ -211,14 +259,15 @@ struct Engine {
   }
 
   static size_t job_bytes(const HostJob* j) {
-    size_t b = j->px * 16 + (size_t)j->desc.passes * 4 + 8;
+    size_t b = j->px * 16;
     if (j->desc.flags & BGPU_JOB_VARIANCE) b += j->px * 12;
     return b;
   }
 
-  // Allocates the job's device state and queues its admission.
+  // Allocates the job's device state and queues its admission.  On failure
+  // the caller retires the job, which returns what was taken.
   int admit(HostJob* j) {
-    if (free_slots.empty()) return fail(BGPU_E_BUDGET, "job table full");
+    if (!can_admit(j)) return fail(BGPU_E_BUDGET, "job table or progress counters full");
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
     const bgpu_job_desc& d = j->desc;
@@ -226,17 +275,21 @@ struct Engine {
     if (int rc = j->film.alloc(ctx, px * 16)) return rc;
     if (d.flags & BGPU_JOB_VARIANCE)
       if (int rc = j->sq.alloc(ctx, px * 12)) return rc;
-    if (int rc = j->pass_done.alloc(ctx, (size_t)d.passes * 4)) return rc;
-    if (int rc = j->skipped.alloc(ctx, 8)) return rc;
-    BGPU_HIP(hipMemsetAsync(j->film.ptr, 0, px * 16, stream));
-    if (j->sq.ptr) BGPU_HIP(hipMemsetAsync(j->sq.ptr, 0, px * 12, stream));
-    BGPU_HIP(hipMemsetAsync(j->pass_done.ptr, 0, (size_t)d.passes * 4, stream));
-    BGPU_HIP(hipMemsetAsync(j->skipped.ptr, 0, 8, stream));
     int slot = free_slots.back();
     free_slots.pop_back();
     j->slot = slot;
     slots[slot] = j;
+    uint32_t off = 0;
+    (void)arena_alloc((uint32_t)d.passes, &off);  // can_admit checked the space
+    j->prog_off = off;
+    j->has_prog = true;
+    uint32_t* prog = prog_buf.as<uint32_t>();
+    BGPU_HIP(hipMemsetAsync(j->film.ptr, 0, px * 16, stream));
+    if (j->sq.ptr) BGPU_HIP(hipMemsetAsync(j->sq.ptr, 0, px * 12, stream));
+    BGPU_HIP(hipMemsetAsync(prog + off, 0, (size_t)d.passes * 4, stream));
+    BGPU_HIP(hipMemsetAsync(prog + 2 * slot, 0, 8, stream));
     j->h_pass.assign(d.passes, 0);
+    j->h_skipped = 0;
     DevJob& J = *h_job;
     std::memset(&J, 0, sizeof(J));
     J.per_pass = j->per_pass;
@@ -251,8 +304,8 @@ struct Engine {
     J.passes = d.passes;
     J.wxyz = j->film.as<float>();
     J.sq = j->sq.ptr ? j->sq.as<float>() : nullptr;
-    J.pass_done = j->pass_done.as<unsigned int>();
-    J.skipped = j->skipped.as<unsigned long long>();
+    J.pass_done = prog + off;
+    J.skipped = reinterpret_cast<unsigned long long*>(prog + 2 * slot);
     J.cancelled = 0;
     J.view = j->view;
     BGPU_HIP(hipMemcpyAsync(args.jobs + slot, h_job, sizeof(DevJob), hipMemcpyHostToDevice, stream));
@@ -265,10 +318,12 @@ struct Engine {
   }
 
   int cancel(HostJob* j) {
-    if (j->slot < 0) return BGPU_OK;
-    int one = 1;
-    BGPU_HIP(hipMemcpyAsync(&args.jobs[j->slot].cancelled, &one, sizeof(int), hipMemcpyHostToDevice, stream));
+    if (j->slot < 0 || j->cancel_sent) return BGPU_OK;
+    h_admit[0].slot = 1;  // staging: the flag value
+    BGPU_HIP(hipMemcpyAsync(&args.jobs[j->slot].cancelled, &h_admit[0].slot, sizeof(int32_t),
+                            hipMemcpyHostToDevice, stream));
     BGPU_HIP(hipStreamSynchronize(stream));
+    j->cancel_sent = true;
     return BGPU_OK;
   }
 
@@ -277,42 +332,38 @@ struct Engine {
     return BGPU_OK;
   }
 
-  int read_counters() {
+  // Reads the engine counters and the progress of `js` with one copy each
+  // and one synchronisation; `done` receives the jobs whose every sample is
+  // accounted for (finished, or skipped after cancellation).
+  int poll(const std::vector<HostJob*>& js, std::vector<HostJob*>* done) {
     BGPU_HIP(hipMemcpyAsync(h_cnt, args.cnt, sizeof(PtCounters), hipMemcpyDeviceToHost, stream));
+    if (!js.empty())
+      BGPU_HIP(hipMemcpyAsync(h_prog, prog_buf.ptr, (size_t)arena_end() * 4, hipMemcpyDeviceToHost, stream));
     BGPU_HIP(hipStreamSynchronize(stream));
-    return BGPU_OK;
-  }
-
-  // Reads a job's per-pass completion; true when every sample is accounted for.
-  int progress(HostJob* j, bool* complete) {
-    // per-pass counts, padded to 8 bytes, then the 8-byte skipped counter
-    size_t need = (size_t)(j->desc.passes + (j->desc.passes & 1)) * 4 + 8;
-    if (h_pass_cap < need) {
-      if (h_pass) (void)hipHostFree(h_pass);
-      h_pass = nullptr;
-      BGPU_HIP(hipHostMalloc((void**)&h_pass, need, hipHostMallocDefault));
-      h_pass_cap = need;
+    for (HostJob* j : js) {
+      if (j->slot < 0 || !j->has_prog) continue;
+      std::memcpy(j->h_pass.data(), h_prog + j->prog_off, (size_t)j->desc.passes * 4);
+      std::memcpy(&j->h_skipped, h_prog + 2 * j->slot, 8);
+      unsigned long long n = j->h_skipped;
+      int pd = 0;
+      bool prefix = true;
+      for (int p = 0; p < j->desc.passes; ++p) {
+        n += j->h_pass[p];
+        if (prefix && j->h_pass[p] == j->per_pass) pd++;
+        else prefix = false;
+      }
+      j->passes_seen = pd;
+      j->samples_seen = n;
+      if (done && n >= j->total) done->push_back(j);
     }
-    BGPU_HIP(hipMemcpyAsync(h_pass, j->pass_done.ptr, (size_t)j->desc.passes * 4, hipMemcpyDeviceToHost, stream));
-    BGPU_HIP(hipMemcpyAsync(h_pass + j->desc.passes + (j->desc.passes & 1), j->skipped.ptr, 8,
-                            hipMemcpyDeviceToHost, stream));
-    BGPU_HIP(hipStreamSynchronize(stream));
-    std::memcpy(j->h_pass.data(), h_pass, (size_t)j->desc.passes * 4);
-    std::memcpy(&j->h_skipped, h_pass + j->desc.passes + (j->desc.passes & 1), 8);
-    unsigned long long done = j->h_skipped;
-    int pd = 0;
-    bool prefix = true;
-    for (int p = 0; p < j->desc.passes; ++p) {
-      done += j->h_pass[p];
-      if (prefix && j->h_pass[p] == j->per_pass) pd++;
-      else prefix = false;
-    }
-    j->passes_done = pd;
-    *complete = done >= j->total;
     return BGPU_OK;
   }
 
   void retire(HostJob* j) {
+    if (j->has_prog) {
+      arena_free(j->prog_off);
+      j->has_prog = false;
+    }
     if (j->slot >= 0) {
       slots[j->slot] = nullptr;
       quarantine.emplace_back(j->slot, h_cnt->iter);
@@ -334,6 +385,8 @@ struct Engine {
   }
 };
 
+void destroy_engine(Engine* e) { delete e; }
+
 static int validate_job(const bgpu_job_desc* d) {
   if (!d) return fail(BGPU_E_INVALID, "job description is NULL");
   if (d->spp_nu <= 0 || d->spp_nv <= 0)
@@ -341,7 +394,7 @@ static int validate_job(const bgpu_job_desc* d) {
   if ((long long)d->spp_nu * d->spp_nv > (1 << 20)) return fail(BGPU_E_INVALID, "too many samples per pixel");
   if (d->max_depth < 0 || d->max_depth > 1000) return fail(BGPU_E_INVALID, "max_depth must be in [0, 1000]");
   if (d->sample_depth < 0 || d->sample_depth > 1000) return fail(BGPU_E_INVALID, "sample_depth must be in [0, 1000]");
-  if (d->passes < 1 || d->passes > (1 << 20)) return fail(BGPU_E_INVALID, "passes must be >= 1");
+  if (d->passes < 1 || d->passes > kMaxPasses) return fail(BGPU_E_INVALID, "passes must be in [1, 65536]");
   return BGPU_OK;
 }
 
@@ -407,7 +460,6 @@ struct Scheduler {
   void loop() {
     (void)hipSetDevice(ctx->device);
     PtCounters prev{};
-    bool have_prev = false;
     while (true) {
       std::vector<HostJob*> to_admit;
       {
@@ -428,16 +480,25 @@ struct Scheduler {
           }
         }
         // admit by HBM budget, in arrival order
+        size_t admitting = 0;
         while (!pending.empty()) {
           HostJob* j = pending.front();
           size_t need = Engine::job_bytes(j);
-          if (ctx->used.load() + need > ctx->budget && !running.empty()) break;
-          if (eng.free_slots.empty()) break;
+          if (ctx->used.load() + admitting + need > ctx->budget && !(running.empty() && to_admit.empty()))
+            break;
+          if (!eng.can_admit(j) || to_admit.size() >= eng.free_slots.size()) break;
           pending.pop_front();
           to_admit.push_back(j);
+          admitting += need;
         }
       }
-      for (HostJob* j : to_admit) {
+      for (size_t k = 0; k < to_admit.size(); ++k) {
+        HostJob* j = to_admit[k];
+        if (!eng.can_admit(j)) {  // several jobs taken at once may not all fit
+          std::lock_guard<std::mutex> lk(mu);
+          pending.insert(pending.begin(), to_admit.begin() + k, to_admit.end());
+          break;
+        }
         int rc = eng.admit(j);
         std::lock_guard<std::mutex> lk(mu);
         if (rc) {
@@ -445,14 +506,29 @@ struct Scheduler {
           j->t_done = now_ms();
           set_error(rc, bgpu_last_error());
           eng.retire(j);
+          if (j->released) delete j;
           done_cv.notify_all();
         } else {
           running.push_back(j);
         }
       }
-      if (running.empty()) continue;
+      std::vector<HostJob*> run_now;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (running.empty()) continue;
+        for (HostJob* j : running)
+          if (j->cancel_req) eng.cancel(j);
+        run_now = running;
+      }
       double t0 = now_ms();
-      if (eng.run(4) || eng.read_counters()) {
+      uint64_t seen_before = 0;
+      for (HostJob* j : run_now) seen_before += j->samples_seen;
+      std::vector<HostJob*> finished;
+      // jobs in `run_now` are deleted only by this thread, and poll writes
+      // fields no other thread reads; results are published under `mu`
+      int rc = eng.run(4);
+      if (!rc) rc = eng.poll(run_now, &finished);
+      if (rc) {
         std::lock_guard<std::mutex> lk(mu);
         for (HostJob* j : running) {
           j->state = BGPU_JOB_FAILED;
@@ -465,35 +541,14 @@ struct Scheduler {
       }
       double t1 = now_ms();
       PtCounters cur = *eng.h_cnt;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        eng.release_quarantine();
-      }
-      if (!have_prev) {
-        prev = PtCounters{};
-        have_prev = true;
-      }
-      uint64_t samples_before = 0, samples_after = 0;
-      std::vector<HostJob*> finished;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        for (HostJob* j : running) {
-          if (j->cancel_req) eng.cancel(j);
-          unsigned long long before = j->h_skipped;
-          for (unsigned v : j->h_pass) before += v;
-          bool complete = false;
-          eng.progress(j, &complete);
-          unsigned long long after = j->h_skipped;
-          for (unsigned v : j->h_pass) after += v;
-          samples_before += before;
-          samples_after += after;
-          if (complete) finished.push_back(j);
-        }
-      }
-      accumulate_stats(ctx, prev, cur, t1 - t0, samples_after - samples_before);
+      uint64_t seen_after = 0;
+      for (HostJob* j : run_now) seen_after += j->samples_seen;
+      accumulate_stats(ctx, prev, cur, t1 - t0, seen_after - seen_before);
       prev = cur;
+      std::lock_guard<std::mutex> lk(mu);
+      for (HostJob* j : run_now) j->passes_done = j->passes_seen;
+      eng.release_quarantine();
       if (!finished.empty()) {
-        std::lock_guard<std::mutex> lk(mu);
         for (HostJob* j : finished) {
           j->state = j->cancel_req ? BGPU_JOB_CANCELLED : BGPU_JOB_DONE;
           j->t_done = now_ms();
@@ -536,33 +591,53 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   BGPU_HIP(hipSetDevice(c->device));
   HostJob j;
   if (int rc = j.prepare(scene, *job)) return rc;
-  int pool = (int)std::min<unsigned long long>((unsigned long long)c->pt_cfg.pool, j.total);
-  Engine eng;
-  if (int rc = eng.init(c, pool, c->pt_cfg.block)) return rc;
   if ((job->flags & BGPU_JOB_VARIANCE) == 0 && sq) j.desc.flags |= BGPU_JOB_VARIANCE;
+  std::lock_guard<std::mutex> lk(c->plain_mu);
+  if (!c->plain) {
+    Engine* e = new Engine();
+    if (int rc = e->init(c, c->pt_cfg.pool, c->pt_cfg.block)) {
+      delete e;
+      return rc;
+    }
+    c->plain = e;
+  }
+  Engine& eng = *c->plain;
+  // a failure below leaves the engine in an unknown state: drop it
+  auto bail = [&](int rc) {
+    eng.retire(&j);
+    destroy_engine(c->plain);
+    c->plain = nullptr;
+    return rc;
+  };
   double t0 = now_ms();
-  if (int rc = eng.admit(&j)) return rc;
-  PtCounters before{};
-  // every path ends within max_depth + 1 bounces, so the work drains in at
+  if (int rc = eng.poll({}, nullptr)) return bail(rc);
+  eng.release_quarantine();
+  const PtCounters before = *eng.h_cnt;
+  if (int rc = eng.admit(&j)) return bail(rc);
+  // every path ends within max_depth + 1 bounces, so the job drains in at
   // most (total / P + 1) * (max_depth + 2) iterations
   unsigned long long cap = (j.total / (unsigned long long)eng.P + 2) * (unsigned long long)(job->max_depth + 2) + 64;
   unsigned long long iters = 0;
-  while (true) {
-    if (int rc = eng.run(4)) return rc;
-    if (int rc = eng.read_counters()) return rc;
+  std::vector<HostJob*> one{&j}, done;
+  while (done.empty()) {
+    if (int rc = eng.run(4)) return bail(rc);
+    if (int rc = eng.poll(one, &done)) return bail(rc);
     iters += 4;
-    const PtCounters& cn = *eng.h_cnt;
-    if (cn.work_next >= cn.work_total && cn.n_act == 0) break;
-    if (iters > cap) return fail(BGPU_E_STATE, "render did not drain within the iteration bound");
+    if (iters > cap && done.empty()) return bail(fail(BGPU_E_STATE, "render did not drain within the iteration bound"));
   }
-  bool complete = false;
-  if (int rc = eng.progress(&j, &complete)) return rc;
-  if (!complete) return fail(BGPU_E_STATE, "render finished with samples unaccounted for");
   double t1 = now_ms();
+  j.passes_done = j.passes_seen;
   accumulate_stats(c, before, *eng.h_cnt, t1 - t0, j.total);
-  BGPU_HIP(hipMemcpy(wxyz, j.film.ptr, j.px * 16, hipMemcpyDeviceToHost));
-  if (sq && j.sq.ptr) BGPU_HIP(hipMemcpy(sq, j.sq.ptr, j.px * 12, hipMemcpyDeviceToHost));
+  {
+    std::lock_guard<std::mutex> lk2(c->stats_mu);
+    c->stats.jobs_done++;
+  }
+  hipError_t e1 = hipMemcpy(wxyz, j.film.ptr, j.px * 16, hipMemcpyDeviceToHost);
+  hipError_t e2 = (sq && j.sq.ptr) ? hipMemcpy(sq, j.sq.ptr, j.px * 12, hipMemcpyDeviceToHost) : hipSuccess;
   eng.retire(&j);
+  eng.gc_scenes();
+  if (e1 != hipSuccess) return hip_fail(e1, "hipMemcpy (film)");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipMemcpy (sum of squares)");
   return BGPU_OK;
 }
 

@@ -14,6 +14,8 @@ namespace bgpu {
 Ctx::~Ctx() {
   if (sched) destroy_scheduler(sched);
   sched = nullptr;
+  if (plain) destroy_engine(plain);
+  plain = nullptr;
   stage_rays.release();
   stage_hits.release();
   count_buf.release();

@@ -269,10 +269,21 @@ def grid_scene(num_tris, res=256, seed=0):
     sb = SceneBuilder()
     _lit_room(sb, 2.0, height=1.0)
     m = sb.material(Material.matte(rgb_refl(0.5, 0.6, 0.4)))
+    # fBm sampled once on a lattice of at most 513^2 points, interpolated
+    # bilinearly (the heightmap evaluates elev five times per vertex)
+    L = min(n, 512) + 1
+    lx, lz = np.meshgrid(np.arange(L) / (L - 1.0), np.arange(L) / (L - 1.0))
+    lat = 0.25 * fbm(np.stack([lx.reshape(-1) * 4.0, lz.reshape(-1) * 4.0, np.zeros(L * L)], 1),
+                     octaves=6, omega=0.5, seed=noise_seed).reshape(L, L)
 
     def elev(x, z):
-        p = np.stack([x * 4.0, z * 4.0, np.zeros_like(x)], 1)
-        return 0.25 * fbm(p, octaves=6, omega=0.5, seed=noise_seed)
+        u = np.clip(x, 0.0, 1.0) * (L - 1)
+        w = np.clip(z, 0.0, 1.0) * (L - 1)
+        i = np.minimum(u.astype(np.int64), L - 2)
+        k = np.minimum(w.astype(np.int64), L - 2)
+        fu, fw = u - i, w - k
+        return ((1 - fu) * (1 - fw) * lat[k, i] + fu * (1 - fw) * lat[k, i + 1]
+                + (1 - fu) * fw * lat[k + 1, i] + fu * fw * lat[k + 1, i + 1])
 
     v, f, nrm, uv = heightmap_mesh(elev, n, n)
     o2w = T.scale((3.0, 1.0, 3.0)) @ T.translate((-1.5, 0.1, -1.5))
