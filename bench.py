@@ -84,6 +84,9 @@ def main():
         sys.exit(2)
     legs = set(args.legs.split(","))
 
+    # torch's bundled HIP runtime has the same soname as the system one that
+    # libbgpu.so links: loading torch first makes the process use one runtime
+    import torch  # noqa: F401
     from bling_amd.runtime import Context
     ctx = Context(0)
     out = {

@@ -23,23 +23,103 @@ using namespace sh;
 
 namespace {
 
-// Wave-aggregated append: one atomic per wave, lanes get consecutive slots.
-__device__ __forceinline__ unsigned int wave_append(unsigned int* counter, bool pred) {
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0) return 0;
-  int lane = __lane_id();
-  int leader = __ffsll((long long)mask) - 1;
-  unsigned int base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+// Same-address atomics serialise at the memory side (device-scope atomics on
+// a chip of several L2 slices), so queue appends and counters are aggregated
+// per block: one atomic per block and counter, positions from an LDS prefix.
+
+// Block-wide exclusive prefix of `pred` plus one atomicAdd on `counter` for
+// the block total; returns the lane's position.  Every thread of the block
+// calls it (it synchronises the block); `s` holds BLOCK/64 + 1 entries.
+template <int BLOCK, class T>
+__device__ __forceinline__ T block_append(T* counter, bool pred, T* s) {
+  constexpr int NW = BLOCK / 64;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(pred);
+  if (lane == 0) s[wave] = (T)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (int w = 0; w < NW; ++w) {
+      T c = s[w];
+      s[w] = run;
+      run += c;
+    }
+    s[NW] = run ? atomicAdd(counter, run) : (T)0;
+  }
+  __syncthreads();
+  T r = s[NW] + s[wave] + (T)__popcll(m & ((1ull << lane) - 1ull));
+  __syncthreads();  // `s` may be reused by the caller
+  return r;
 }
 
-__device__ __forceinline__ void wave_count(unsigned long long* counter, bool pred) {
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0) return;
-  int lane = __lane_id();
-  if (lane == __ffsll((long long)mask) - 1) atomicAdd(counter, (unsigned long long)__popcll(mask));
+// Block-wide count of `pred` added to `counter` with one atomic.
+template <int BLOCK, class T>
+__device__ __forceinline__ void block_count(T* counter, bool pred, T* s) {
+  constexpr int NW = BLOCK / 64;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(pred);
+  if (lane == 0) s[wave] = (T)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T run = 0;
+    for (int w = 0; w < NW; ++w) run += s[w];
+    if (run) atomicAdd(counter, run);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ unsigned int wave_sum(unsigned int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Adds 1 to *ctr for every lane with `pred`, merging equal counters: each
+// wave groups its lanes by counter address (up to kGroups groups go to LDS,
+// further ones straight to memory), then wave 0 merges the block's groups and
+// issues one atomic per distinct counter.  Every thread of the block calls it.
+constexpr int kGroups = 4;
+template <int BLOCK>
+__device__ __forceinline__ void block_increment(unsigned int* ctr, bool pred, unsigned long long* s_key,
+                                                unsigned int* s_cnt, int* s_n) {
+  constexpr int NW = BLOCK / 64;
+  static_assert(NW * kGroups <= 64, "one merge entry per lane of wave 0");
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const unsigned long long key = pred ? (unsigned long long)(uintptr_t)ctr : 0ull;
+  unsigned long long pending = __ballot(pred);
+  int ng = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const unsigned long long lk = __shfl(key, leader);
+    const unsigned long long m = __ballot(pred && key == lk) & pending;
+    if (lane == leader) {
+      if (ng < kGroups) {
+        s_key[wave * kGroups + ng] = lk;
+        s_cnt[wave * kGroups + ng] = (unsigned int)__popcll(m);
+      } else {
+        atomicAdd((unsigned int*)(uintptr_t)lk, (unsigned int)__popcll(m));
+      }
+    }
+    ++ng;
+    pending &= ~m;
+  }
+  if (lane == 0) s_n[wave] = ng < kGroups ? ng : kGroups;
+  __syncthreads();
+  if (wave == 0) {
+    const int e = lane;
+    const bool has = e < NW * kGroups && (e % kGroups) < s_n[e / kGroups];
+    const unsigned long long k2 = has ? s_key[e] : 0ull;
+    const unsigned int c2 = has ? s_cnt[e] : 0u;
+    unsigned long long pend = __ballot(has);
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const unsigned long long lk = __shfl(k2, leader);
+      const unsigned long long m = __ballot(has && k2 == lk) & pend;
+      const unsigned int sum = wave_sum(((m >> lane) & 1ull) ? c2 : 0u);
+      if (lane == leader) atomicAdd((unsigned int*)(uintptr_t)lk, sum);
+      pend &= ~m;
+    }
+  }
 }
 
 __global__ void k_init(PtKernelArgs a) {
@@ -100,19 +180,35 @@ __device__ __forceinline__ int find_job(const PtKernelArgs& a, unsigned long lon
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
+  __shared__ unsigned long long s64[BLOCK / 64 + 1];
+  __shared__ unsigned int s32[BLOCK / 64 + 1];
   const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= a.pool.P) return;
   const PathPool& P = a.pool;
-  int st = P.state[i];
-  bool want = st == 0;
-  unsigned long long mask = __ballot(want);
+  const bool inpool = i < P.P;
+  const int st = inpool ? P.state[i] : 1;
+  const bool want = inpool && st == 0;
+  // work indices for the block's free slots, taken with one atomic while
+  // work remains (a stale read only costs an idle slot this iteration)
   unsigned long long w = ~0ull;
-  if (mask) {
-    int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(&a.cnt->work_next, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    w = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+  {
+    constexpr int NW = BLOCK / 64;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(want);
+    if (lane == 0) s64[wave] = (unsigned long long)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long run = 0;
+      for (int k = 0; k < NW; ++k) {
+        unsigned long long c = s64[k];
+        s64[k] = run;
+        run += c;
+      }
+      const volatile PtCounters* vc = a.cnt;
+      s64[NW] = (run && vc->work_next < vc->work_total) ? atomicAdd(&a.cnt->work_next, run) : ~0ull;
+    }
+    __syncthreads();
+    if (want && s64[NW] != ~0ull) w = s64[NW] + s64[wave] + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();
   }
   bool fresh = false;
   const unsigned int head = a.cnt->ring_head, tail = a.cnt->ring_tail;
@@ -158,9 +254,9 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
       }
     }
   }
-  wave_count(&a.cnt->tot_camera, fresh);
-  bool active = st == 1 || fresh;
-  unsigned int q = wave_append(&a.cnt->n_act, active);
+  block_count<BLOCK>(&a.cnt->tot_camera, fresh, s64);
+  const bool active = inpool && (st == 1 || fresh);
+  const unsigned int q = block_append<BLOCK>(&a.cnt->n_act, active, s32);
   if (active) P.q_act[q] = (uint32_t)i;
 }
 
@@ -357,9 +453,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
       }
     }
   });
-  unsigned int qs = wave_append(&a.cnt->n_shadow, want_shadow);
+  __shared__ unsigned int s32[BLOCK / 64 + 1];
+  const unsigned int qs = block_append<BLOCK>(&a.cnt->n_shadow, want_shadow, s32);
   if (want_shadow) P.q_shadow[qs] = (uint32_t)i;
-  unsigned int qm = wave_append(&a.cnt->n_mis, want_mis);
+  const unsigned int qm = block_append<BLOCK>(&a.cnt->n_mis, want_mis, s32);
   if (want_mis) P.q_mis[qm] = (uint32_t)i;
 }
 
@@ -440,66 +537,73 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
+  __shared__ unsigned long long s_key[BLOCK / 64 * kGroups];
+  __shared__ unsigned int s_cnt[BLOCK / 64 * kGroups];
+  __shared__ int s_n[BLOCK / 64];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_act;
-  if ((unsigned)q >= n) return;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
   const PathPool& P = a.pool;
   const int Pn = P.P;
-  int i = (int)P.q_act[q];
-  if (P.state[i] != 2) return;
-  const DevJob& J = a.jobs[P.job[i]];
-  if (!J.cancelled) {
-    Sp L;
-    bool bad = false;
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      L.v[b] = P.rad[b * Pn + i];
-      bad = bad || isnan(L.v[b]) || isinf(L.v[b]);
-    }
-    if (!bad) {  // addSample (Image.hs:250-299) on the full image
-      float X = 0.0f, Y = 0.0f, Z = 0.0f;
+  const int i = (unsigned)q < n ? (int)P.q_act[q] : 0;
+  const bool term = (unsigned)q < n && P.state[i] == 2;
+  unsigned int* ctr = nullptr;
+  if (term) {
+    const DevJob& J = a.jobs[P.job[i]];
+    ctr = J.pass_done + P.pass[i];
+    if (!J.cancelled) {
+      Sp L;
+      bool bad = false;
 #pragma unroll
       for (int b = 0; b < 16; ++b) {
-        X = X + cCieX[b] * L.v[b];
-        Y = Y + cCieY[b] * L.v[b];
-        Z = Z + cCieZ[b] * L.v[b];
+        L.v[b] = P.rad[b * Pn + i];
+        bad = bad || isnan(L.v[b]) || isinf(L.v[b]);
       }
-      X = X / BGPU_CIE_Y_SUM;
-      Y = Y / BGPU_CIE_Y_SUM;
-      Z = Z / BGPU_CIE_Y_SUM;
-      float sx = P.sxy[i], sy = P.sxy[Pn + i];
-      float fw = J.view.fw, fh = J.view.fh, ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
-      float dx = sx - 0.5f, dy = sy - 0.5f;
-      int x0 = (int)ceilf(dx - fw), x1 = (int)floorf(dx + fw);
-      int y0 = (int)ceilf(dy - fh), y1 = (int)floorf(dy + fh);
-      x0 = x0 < 0 ? 0 : x0;
-      y0 = y0 < 0 ? 0 : y0;
-      x1 = x1 > J.view.film_w - 1 ? J.view.film_w - 1 : x1;
-      y1 = y1 > J.view.film_h - 1 ? J.view.film_h - 1 : y1;
-      for (int y = y0; y <= y1; ++y) {
-        int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
-        fy = fy < 15 ? fy : 15;
-        for (int x = x0; x <= x1; ++x) {
-          int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
-          fx = fx < 15 ? fx : 15;
-          float w = J.view.filter[fy * 16 + fx];
-          float* o = J.wxyz + 4 * ((size_t)y * J.view.film_w + x);
-          atomicAdd(o, w);
-          atomicAdd(o + 1, X * w);
-          atomicAdd(o + 2, Y * w);
-          atomicAdd(o + 3, Z * w);
-          if (J.sq) {
-            float* s2 = J.sq + 3 * ((size_t)y * J.view.film_w + x);
-            atomicAdd(s2, X * X * w);
-            atomicAdd(s2 + 1, Y * Y * w);
-            atomicAdd(s2 + 2, Z * Z * w);
+      if (!bad) {  // addSample (Image.hs:250-299) on the full image
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          X = X + cCieX[b] * L.v[b];
+          Y = Y + cCieY[b] * L.v[b];
+          Z = Z + cCieZ[b] * L.v[b];
+        }
+        X = X / BGPU_CIE_Y_SUM;
+        Y = Y / BGPU_CIE_Y_SUM;
+        Z = Z / BGPU_CIE_Y_SUM;
+        float sx = P.sxy[i], sy = P.sxy[Pn + i];
+        float fw = J.view.fw, fh = J.view.fh, ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
+        float dx = sx - 0.5f, dy = sy - 0.5f;
+        int x0 = (int)ceilf(dx - fw), x1 = (int)floorf(dx + fw);
+        int y0 = (int)ceilf(dy - fh), y1 = (int)floorf(dy + fh);
+        x0 = x0 < 0 ? 0 : x0;
+        y0 = y0 < 0 ? 0 : y0;
+        x1 = x1 > J.view.film_w - 1 ? J.view.film_w - 1 : x1;
+        y1 = y1 > J.view.film_h - 1 ? J.view.film_h - 1 : y1;
+        for (int y = y0; y <= y1; ++y) {
+          int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
+          fy = fy < 15 ? fy : 15;
+          for (int x = x0; x <= x1; ++x) {
+            int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
+            fx = fx < 15 ? fx : 15;
+            float w = J.view.filter[fy * 16 + fx];
+            float* o = J.wxyz + 4 * ((size_t)y * J.view.film_w + x);
+            atomicAdd(o, w);
+            atomicAdd(o + 1, X * w);
+            atomicAdd(o + 2, Y * w);
+            atomicAdd(o + 3, Z * w);
+            if (J.sq) {
+              float* s2 = J.sq + 3 * ((size_t)y * J.view.film_w + x);
+              atomicAdd(s2, X * X * w);
+              atomicAdd(s2 + 1, Y * Y * w);
+              atomicAdd(s2 + 2, Z * Z * w);
+            }
           }
         }
       }
     }
+    P.state[i] = 0;
   }
-  atomicAdd(J.pass_done + P.pass[i], 1u);
-  P.state[i] = 0;
+  block_increment<BLOCK>(ctr, term, s_key, s_cnt, s_n);
 }
 
 }  // namespace
@@ -523,16 +627,21 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// generate and finish are light per thread and aggregate their counters per
+// block, so they run in the largest blocks
+constexpr int kWideBlock = 1024;
+
 template <int BLOCK>
 static hipError_t enqueue_iter(const PtKernelArgs& a, hipStream_t s) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
+  dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
-  hipLaunchKernelGGL(k_generate<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
   hipLaunchKernelGGL(k_extend<BLOCK>, g, b, 0, s, a);
   hipLaunchKernelGGL(k_shade<BLOCK>, g, b, 0, s, a);
   hipLaunchKernelGGL(k_shadow<BLOCK>, g, b, 0, s, a);
   hipLaunchKernelGGL(k_mis<BLOCK>, g, b, 0, s, a);
-  hipLaunchKernelGGL(k_finish<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
   return hipGetLastError();
 }
 
