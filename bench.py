@@ -75,8 +75,10 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs per arrival rate")
     ap.add_argument("--serve-rates", default="5,50,200", help="Poisson arrival rates, jobs/s")
     ap.add_argument("--serve-max-tris", type=float, default=8e6)
-    ap.add_argument("--serve-res", default="128,256,512")
-    ap.add_argument("--serve-spp", default="1,4,16")
+    ap.add_argument("--serve-res", default="128,256,512,1024")
+    ap.add_argument("--serve-spp", default="4,16,64")
+    ap.add_argument("--config", default="",
+                    help="tuning knobs, key=value[,key=value] (bgpu_config_set keys)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported (this build runs on one GPU; "
@@ -89,6 +91,12 @@ def main():
     import torch  # noqa: F401
     from bling_amd.runtime import Context
     ctx = Context(0)
+    knobs = {}
+    for kv in filter(None, args.config.split(",")):
+        k, v = kv.split("=")
+        ctx.config_set(k, int(v))
+    for k in ("trace.block", "pt.block", "pt.pool", "pt.shade_waves"):
+        knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,
         "value": None,
@@ -102,7 +110,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {},
+        "config": {"knobs": knobs},
     }
     if "intersect" in legs:
         intersect_leg(ctx, args, out)
@@ -385,7 +393,7 @@ def run_plain(ctx, gss, flats, jobs):
     from bling_amd.runtime import GpuRenderer
     ctx.reset_stats()
     t_start = time.perf_counter()
-    lat, imgs, t_end = [], [], t_start
+    lat, imgs, t_end = [], [], 0.0
     for job in jobs:
         now = time.perf_counter() - t_start
         if now < job["arrival"]:

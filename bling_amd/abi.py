@@ -100,7 +100,7 @@ EXPORTS = [
     "bgpu_scene_destroy", "bgpu_scene_bytes", "bgpu_intersect_closest", "bgpu_intersect_any",
     "bgpu_last_kernel_ms", "bgpu_render", "bgpu_job_submit", "bgpu_job_poll", "bgpu_job_wait",
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
-    "bgpu_stats_reset",
+    "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get",
 ]
 
 _lib = None
@@ -140,6 +140,8 @@ def load(path=LIB_PATH):
     lib.bgpu_job_latency_ms.argtypes = [vp, C.POINTER(C.c_double)]
     lib.bgpu_stats_get.argtypes = [vp, C.POINTER(Stats)]
     lib.bgpu_stats_reset.argtypes = [vp]
+    lib.bgpu_config_set.argtypes = [vp, C.c_char_p, C.c_int64]
+    lib.bgpu_config_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     _lib = lib
     return lib
 

@@ -71,6 +71,7 @@ struct Ctx {
   Scheduler* sched = nullptr;     // served path, created on first job_submit
   Engine* plain = nullptr;        // plain path engine, reused by bgpu_render calls
   std::mutex plain_mu;            // one plain render at a time
+  std::atomic<int> live_jobs{0};  // served job handles not yet deleted
   TraceConfig trace_cfg;
   PtConfig pt_cfg;
   ~Ctx();

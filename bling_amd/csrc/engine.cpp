@@ -44,6 +44,7 @@ struct HostJob {
   bool cancel_req = false;
   bool released = false;  // user released the handle
   size_t bytes = 0;
+  Ctx* counted = nullptr;  // served jobs count themselves in Ctx::live_jobs
 
   // Copies a validated description; the camera and film default to the
   // scene's (the caller's descriptors are not referenced afterwards).
@@ -62,6 +63,7 @@ struct HostJob {
     return BGPU_OK;
   }
   virtual ~HostJob() {
+    if (counted) counted->live_jobs--;
     film.release();
     sq.release();
     if (scene && --scene->refs == 0) {
@@ -82,7 +84,8 @@ constexpr int kMaxPasses = 1 << 16;
 struct Engine {
   Ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
-  int P = 0, block = 128;
+  int P = 0;
+  PtConfig cfg;
   DevBuf pool_buf, jobs_buf, ring_buf, scenes_buf, cnt_buf, admit_buf, prog_buf;
   PtKernelArgs args{};
   AdmitReq* h_admit = nullptr;
@@ -122,14 +125,14 @@ struct Engine {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  int init(Ctx* c, int pool, int blk) {
+  int init(Ctx* c, const PtConfig& config) {
     ctx = c;
-    P = std::max(1024, (pool + 255) / 256 * 256);
-    block = blk;
+    cfg = config;
+    P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 8 int arrays + 91 float arrays of P, plus 3 queues
+    // pool layout: 8 int arrays + 52 float arrays of P, plus 3 queues
     size_t per = (size_t)P * 4;
-    size_t total = per * (8 + 91 + 3);
+    size_t total = per * (8 + 52 + 3);
     if (int rc = pool_buf.alloc(c, total)) return rc;
     char* b = pool_buf.as<char>();
     auto take = [&](size_t n) {
@@ -151,11 +154,11 @@ struct Engine {
     pp.ray = (float*)take(7);
     pp.hit = (float*)take(3);
     pp.thr = (float*)take(16);
-    pp.rad = (float*)take(16);
+    pp.rad = (float*)take(3);
     pp.sray = (float*)take(8);
-    pp.pend_s = (float*)take(16);
+    pp.pend_s = (float*)take(3);
     pp.mray = (float*)take(7);
-    pp.pend_m = (float*)take(16);
+    pp.pend_m = (float*)take(3);
     pp.q_act = (uint32_t*)take(1);
     pp.q_shadow = (uint32_t*)take(1);
     pp.q_mis = (uint32_t*)take(1);
@@ -182,7 +185,7 @@ struct Engine {
     BGPU_HIP(pt_enqueue_init(args, stream));
     // capture one bounce iteration
     BGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = pt_enqueue_iteration(args, block, stream);
+    hipError_t e = pt_enqueue_iteration(args, cfg, stream);
     hipError_t e2 = hipStreamEndCapture(stream, &graph);
     if (e != hipSuccess) return hip_fail(e, "pt_enqueue_iteration (capture)");
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
@@ -425,7 +428,7 @@ struct Scheduler {
 
   int start(Ctx* c) {
     ctx = c;
-    if (int rc = eng.init(c, c->pt_cfg.pool, c->pt_cfg.block)) return rc;
+    if (int rc = eng.init(c, c->pt_cfg)) return rc;
     th = std::thread([this] { loop(); });
     return BGPU_OK;
   }
@@ -595,7 +598,7 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   std::lock_guard<std::mutex> lk(c->plain_mu);
   if (!c->plain) {
     Engine* e = new Engine();
-    if (int rc = e->init(c, c->pt_cfg.pool, c->pt_cfg.block)) {
+    if (int rc = e->init(c, c->pt_cfg)) {
       delete e;
       return rc;
     }
@@ -655,25 +658,25 @@ int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out)
     delete j;
     return fail(BGPU_E_BUDGET, "job film does not fit the HBM budget");
   }
-  {
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (!c->sched) {
-      BGPU_HIP(hipSetDevice(c->device));
-      Scheduler* s = new Scheduler();
-      int rc = s->start(c);
-      if (rc) {
-        delete s;
-        delete j;
-        return rc;
-      }
-      c->sched = s;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->sched) {
+    BGPU_HIP(hipSetDevice(c->device));
+    Scheduler* s = new Scheduler();
+    int rc = s->start(c);
+    if (rc) {
+      delete s;
+      delete j;
+      return rc;
     }
+    c->sched = s;
   }
   j->t_submit = now_ms();
   j->state = BGPU_JOB_QUEUED;
+  j->counted = c;
+  c->live_jobs++;
   Scheduler* s = c->sched;
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::lock_guard<std::mutex> lk2(s->mu);
     s->pending.push_back(j);
   }
   s->cv.notify_all();
@@ -742,6 +745,56 @@ int bgpu_job_latency_ms(bgpu_job* job, double* ms) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (job->state < BGPU_JOB_DONE) return fail(BGPU_E_STATE, "job has not finished");
   *ms = job->t_done - job->t_submit;
+  return BGPU_OK;
+}
+
+int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
+  if (!ctx || !key) return fail(BGPU_E_INVALID, "bgpu_config_set: NULL argument");
+  const std::string k(key);
+  auto block_ok = [](int64_t v) { return v == 64 || v == 128 || v == 256; };
+  if (k == "trace.block") {
+    if (!block_ok(value)) return fail(BGPU_E_INVALID, "trace.block must be 64, 128 or 256");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->trace_cfg.block = (int)value;
+    return BGPU_OK;
+  }
+  PtConfig next = ctx->pt_cfg;
+  if (k == "pt.block") {
+    if (!block_ok(value)) return fail(BGPU_E_INVALID, "pt.block must be 64, 128 or 256");
+    next.block = (int)value;
+  } else if (k == "pt.pool") {
+    if (value < 1024 || value > (1 << 26)) return fail(BGPU_E_INVALID, "pt.pool must be in [1024, 2^26]");
+    next.pool = (int)value;
+  } else if (k == "pt.shade_waves") {
+    if (value != 0 && value != 2) return fail(BGPU_E_INVALID, "pt.shade_waves must be 0 or 2");
+    next.shade_waves = (int)value;
+  } else {
+    return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
+  }
+  // engines capture their kernels and pool at creation: drop the idle ones
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->live_jobs.load() > 0) return fail(BGPU_E_STATE, "served job handles are still live");
+  if (ctx->sched) {
+    destroy_scheduler(ctx->sched);
+    ctx->sched = nullptr;
+  }
+  std::lock_guard<std::mutex> lk3(ctx->plain_mu);
+  if (ctx->plain) {
+    destroy_engine(ctx->plain);
+    ctx->plain = nullptr;
+  }
+  ctx->pt_cfg = next;
+  return BGPU_OK;
+}
+
+int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
+  if (!ctx || !key || !value) return fail(BGPU_E_INVALID, "bgpu_config_get: NULL argument");
+  const std::string k(key);
+  if (k == "trace.block") *value = ctx->trace_cfg.block;
+  else if (k == "pt.block") *value = ctx->pt_cfg.block;
+  else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
+  else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
+  else return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   return BGPU_OK;
 }
 

@@ -109,11 +109,11 @@ struct PathPool {
   float* hit;       // 3P: t b1 b2
   int32_t* hprim;
   float* thr;       // 16P throughput
-  float* rad;       // 16P radiance
+  float* rad;       // 3P radiance as unnormalised XYZ sums
   float* sray;      // 8P shadow ray
-  float* pend_s;    // 16P contribution if the shadow ray is unoccluded
+  float* pend_s;    // 3P XYZ of the light sample if the shadow ray is unoccluded
   float* mray;      // 7P MIS ray
-  float* pend_m;    // 16P MIS factor, times the light's emission on a hit
+  float* pend_m;    // 3P XYZ of the BSDF sample if the MIS ray reaches the light
   int32_t* mlight;  // light index of the MIS ray
   uint32_t* q_act;
   uint32_t* q_shadow;
@@ -123,6 +123,7 @@ struct PathPool {
 struct PtConfig {
   int block = 128;
   int pool = 1 << 20;
+  int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2)
 };
 
 struct PtKernelArgs {
@@ -145,7 +146,7 @@ int make_view(const bgpu_camera& cam, const bgpu_film& film, DevView* v);  // 0 
 
 // Enqueues one bounce iteration (reset, generate, extend, shade, shadow,
 // MIS, finish) on `s`.  Capturable into a hipGraph.
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, int block, hipStream_t s);
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, hipStream_t s);
 // Enqueues the admission of jobs listed in `admit` (count n) on `s`.
 hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s);
 hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s);

@@ -68,6 +68,34 @@ __device__ __forceinline__ void block_count(T* counter, bool pred, T* s) {
   __syncthreads();
 }
 
+// A path's radiance is carried as unnormalised CIE XYZ sums
+// (sum_b cie[b] * L[b]); the film divides by the Y integral.  XYZ is linear in
+// the spectrum, so summing the XYZ of each contribution equals the XYZ of the
+// summed spectrum up to float rounding.
+struct Xyz {
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  __device__ __forceinline__ void add(int b, float c) {
+    x = x + cCieX[b] * c;
+    y = y + cCieY[b] * c;
+    z = z + cCieZ[b] * c;
+  }
+};
+__device__ __forceinline__ void xyz_store(float* a, int Pn, int i, const Xyz& v) {
+  a[i] = v.x;
+  a[Pn + i] = v.y;
+  a[2 * Pn + i] = v.z;
+}
+__device__ __forceinline__ void xyz_accum(float* a, int Pn, int i, const Xyz& v) {
+  a[i] += v.x;
+  a[Pn + i] += v.y;
+  a[2 * Pn + i] += v.z;
+}
+__device__ __forceinline__ void xyz_accum(float* a, int Pn, int i, const float* src) {
+  a[i] += src[i];
+  a[Pn + i] += src[Pn + i];
+  a[2 * Pn + i] += src[2 * Pn + i];
+}
+
 __device__ __forceinline__ unsigned int wave_sum(unsigned int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -245,10 +273,8 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
         P.ray[3 * Pn + i] = r.d.x; P.ray[4 * Pn + i] = r.d.y; P.ray[5 * Pn + i] = r.d.z;
         P.ray[6 * Pn + i] = r.tmin;
 #pragma unroll
-        for (int b = 0; b < 16; ++b) {
-          P.thr[b * Pn + i] = 1.0f;
-          P.rad[b * Pn + i] = 0.0f;
-        }
+        for (int b = 0; b < 16; ++b) P.thr[b * Pn + i] = 1.0f;
+        xyz_store(P.rad, Pn, i, Xyz{});
         P.state[i] = 1;
         fresh = true;
       }
@@ -313,8 +339,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
   });
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
+template <int BLOCK, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int nact = a.cnt->n_act;
   if ((unsigned)(blockIdx.x * BLOCK) >= nact) return;
@@ -342,8 +368,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
           for (int li = 0; li < S.nlights; ++li) {
             const float* le = light_le(S, li, ray.d);
             if (le) {
+              Xyz c;
 #pragma unroll
-              for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.thr[b * Pn + i] * le[b];
+              for (int b = 0; b < 16; ++b) c.add(b, P.thr[b * Pn + i] * le[b]);
+              xyz_accum(P.rad, Pn, i, c);
             }
           }
         }
@@ -369,14 +397,14 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
         sm.get2(un, 2 + 3 * d, &bd1, &bd2);
         f3 wo = neg3(ray.d);
         f3 p = bsdf.p;
-        Sp thr;
-#pragma unroll
-        for (int b = 0; b < 16; ++b) thr.v[b] = P.thr[b * Pn + i];
+        const float* thr = P.thr + i;  // band b at thr[b * Pn]
         // emission seen through a specular chain (Path.hs:53, intLe with rd)
         if (spec && light >= 0 && dot3(dgg.n, ray.d) > 0.0f) {
           const float* L = S.lights[light].L;
+          Xyz c;
 #pragma unroll
-          for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += thr.v[b] * L[b];
+          for (int b = 0; b < 16; ++b) c.add(b, thr[b * Pn] * L[b]);
+          xyz_accum(P.rad, Pn, i, c);
         }
         int lc = S.nlights;
         if (lc > 0) {  // sampleOneLight (Scene.hs:110-118)
@@ -389,64 +417,98 @@ __global__ __launch_bounds__(BLOCK) void k_shade(PtKernelArgs a) {
           }
           {  // sampleLightMis (Scene.hs:61-69)
             LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
-            Sp f;
-            eval_bsdf(bsdf, wo, ls.wi, f);
-            bool liBlack = true;
-            if (ls.li) {
-#pragma unroll
-              for (int b = 0; b < 16; ++b) liBlack = liBlack && ls.li[b] == 0.0f;
-            }
-            if (!(ls.pdf == 0.0f || liBlack || sp_black(f))) {
+            if (ls.pdf != 0.0f && ls.li) {
+              SpecVal f = eval_bsdf(bsdf, wo, ls.wi);
               float w = power_heuristic(ls.pdf, bsdf_pdf(bsdf, wo, ls.wi));
               float k = w / ls.pdf;
-#pragma unroll
-              for (int b = 0; b < 16; ++b) P.pend_s[b * Pn + i] = thr.v[b] * (((f.v[b] * ls.li[b]) * k) * lscale);
-              P.sray[i] = ls.ray.o.x; P.sray[Pn + i] = ls.ray.o.y; P.sray[2 * Pn + i] = ls.ray.o.z;
-              P.sray[3 * Pn + i] = ls.ray.d.x; P.sray[4 * Pn + i] = ls.ray.d.y; P.sray[5 * Pn + i] = ls.ray.d.z;
-              P.sray[6 * Pn + i] = ls.ray.tmin; P.sray[7 * Pn + i] = ls.ray.tmax;
-              want_shadow = true;
+              Xyz c;
+              bool liBlack = true, fBlack = true;
+#pragma unroll 4
+              for (int b = 0; b < 16; ++b) {
+                float fv = spec_at(f, b), li = ls.li[b];
+                liBlack = liBlack && li == 0.0f;
+                fBlack = fBlack && fv == 0.0f;
+                c.add(b, thr[b * Pn] * (((fv * li) * k) * lscale));
+              }
+              if (!(liBlack || fBlack)) {
+                xyz_store(P.pend_s, Pn, i, c);
+                P.sray[i] = ls.ray.o.x; P.sray[Pn + i] = ls.ray.o.y; P.sray[2 * Pn + i] = ls.ray.o.z;
+                P.sray[3 * Pn + i] = ls.ray.d.x; P.sray[4 * Pn + i] = ls.ray.d.y; P.sray[5 * Pn + i] = ls.ray.d.z;
+                P.sray[6 * Pn + i] = ls.ray.tmin; P.sray[7 * Pn + i] = ls.ray.tmax;
+                want_shadow = true;
+              }
             }
           }
           {  // sampleBsdfMis (Scene.hs:71-82)
-            Sp fb;
+            SpecVal fb;
             f3 wib;
             int tb;
             float bpdf = sample_bsdf(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
-            if (!(bpdf == 0.0f || sp_black(fb))) {
+            if (bpdf != 0.0f) {
               float lpdf = light_pdf(S, ln, p, wib);
               float w = power_heuristic(bpdf, lpdf);
-#pragma unroll
-              for (int b = 0; b < 16; ++b) P.pend_m[b * Pn + i] = thr.v[b] * ((fb.v[b] * w) * lscale);
-              P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
-              P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
-              P.mray[6 * Pn + i] = eps;
-              P.mlight[i] = ln;
-              want_mis = true;
+              // the emission a hit on the sampled light returns is known here:
+              // the area light's radiance, or the infinite light's along wib
+              const float* e = S.lights[ln].type == LT_INF ? light_le(S, ln, wib) : S.lights[ln].L;
+              Xyz c;
+              bool fBlack = true;
+#pragma unroll 4
+              for (int b = 0; b < 16; ++b) {
+                float fv = spec_at(fb, b);
+                fBlack = fBlack && fv == 0.0f;
+                c.add(b, (thr[b * Pn] * ((fv * w) * lscale)) * e[b]);
+              }
+              if (!fBlack) {
+                xyz_store(P.pend_m, Pn, i, c);
+                P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
+                P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
+                P.mray[6 * Pn + i] = eps;
+                P.mlight[i] = ln;
+                want_mis = true;
+              }
             }
           }
         }
         // Russian roulette (Path.hs:70-72)
-        float pc = depth <= J.rr ? 1.0f : hmin(0.75f, sp_y(thr));
+        float pc = 1.0f;
+        if (depth > J.rr) {
+          float y = 0.0f;  // sY (Spectrum.hs:371-373)
+#pragma unroll
+          for (int b = 0; b < 16; ++b) y = y + thr[b * Pn] * cCieY[b];
+          pc = hmin(0.75f, y / BGPU_CIE_Y_SUM);
+        }
         float x = sm.get1(un, 3 + 4 * d);
         bool cont = !(x > pc);
         if (cont) {
           float uc = sm.get1(un, 0 + 4 * d);
           float u1, u2;
           sm.get2(un, 0 + 3 * d, &u1, &u2);
-          Sp f;
+          SpecVal f;
           f3 wi;
           int tp;
           float spdf = sample_bsdf(bsdf, wo, uc, u1, u2, f, &wi, &tp);
-          if (spdf == 0.0f || sp_black(f)) {
+          if (spdf == 0.0f) {
             cont = false;
           } else {
+            // the new throughput is written band by band; a black sample ends
+            // the path, and then the values written are never read
             float ipc = 1.0f / pc;
-#pragma unroll
-            for (int b = 0; b < 16; ++b) P.thr[b * Pn + i] = (f.v[b] * thr.v[b]) * ipc;
-            P.ray[i] = p.x; P.ray[Pn + i] = p.y; P.ray[2 * Pn + i] = p.z;
-            P.ray[3 * Pn + i] = wi.x; P.ray[4 * Pn + i] = wi.y; P.ray[5 * Pn + i] = wi.z;
-            P.ray[6 * Pn + i] = eps;
-            P.dspec[i] = (depth + 1) | ((tp & BX_S) ? (1 << 16) : 0);
+            bool fBlack = true;
+            float* thw = P.thr + i;
+#pragma unroll 4
+            for (int b = 0; b < 16; ++b) {
+              float fv = spec_at(f, b);
+              fBlack = fBlack && fv == 0.0f;
+              thw[b * Pn] = (fv * thw[b * Pn]) * ipc;
+            }
+            if (fBlack) {
+              cont = false;
+            } else {
+              P.ray[i] = p.x; P.ray[Pn + i] = p.y; P.ray[2 * Pn + i] = p.z;
+              P.ray[3 * Pn + i] = wi.x; P.ray[4 * Pn + i] = wi.y; P.ray[5 * Pn + i] = wi.z;
+              P.ray[6 * Pn + i] = eps;
+              P.dspec[i] = (depth + 1) | ((tp & BX_S) ? (1 << 16) : 0);
+            }
           }
         }
         P.state[i] = cont ? 1 : 2;
@@ -480,8 +542,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
       r.tmax = P.sray[7 * Pn + i];
       unsigned long long nv = 0, nt = 0;
       if (!trav::any_hit<false>(sc, r, lds + threadIdx.x, BLOCK, nv, nt)) {  // occluded (Scene.hs:45-47)
-#pragma unroll
-        for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.pend_s[b * Pn + i];
+        xyz_accum(P.rad, Pn, i, P.pend_s);
       }
     }
   });
@@ -511,7 +572,9 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
       trav::closest_hit<false>(sc, r, lds + threadIdx.x, BLOCK, best, nv, nt);
       int ln = P.mlight[i];
       const DLight& L = S.lights[ln];
-      const float* e = nullptr;
+      // pend_m already holds the contribution times the sampled light's
+      // emission; here only whether the ray reaches that light is decided
+      bool reaches = false;
       if (best.prim >= 0) {
         // the hit primitive's light must be the sampled AreaLight (Light.hs:47-50)
         if (best.prim >= S.ntri && L.type == LT_AREA) {
@@ -521,16 +584,13 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
             float eps;
             int mat, light;
             hit_dg(S, r, best.t, best.b1, best.b2, best.prim, &dgg, &dgs, &eps, &mat, &light);
-            if (dot3(dgg.n, neg3(r.d)) > 0.0f) e = S.lights[hl].L;  // lEmit with -wi
+            reaches = dot3(dgg.n, neg3(r.d)) > 0.0f;  // lEmit with -wi
           }
         }
       } else {
-        e = light_le(S, ln, r.d);
+        reaches = L.type == LT_INF;  // le of an infinite light (Light.hs:98-106)
       }
-      if (e) {
-#pragma unroll
-        for (int b = 0; b < 16; ++b) P.rad[b * Pn + i] += P.pend_m[b * Pn + i] * e[b];
-      }
+      if (reaches) xyz_accum(P.rad, Pn, i, P.pend_m);
     }
   });
 }
@@ -552,21 +612,11 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
     const DevJob& J = a.jobs[P.job[i]];
     ctr = J.pass_done + P.pass[i];
     if (!J.cancelled) {
-      Sp L;
-      bool bad = false;
-#pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        L.v[b] = P.rad[b * Pn + i];
-        bad = bad || isnan(L.v[b]) || isinf(L.v[b]);
-      }
+      // a NaN or infinite band makes its XYZ sums NaN or infinite, so the
+      // sample is dropped as the spectral check of addSample would
+      float X = P.rad[i], Y = P.rad[Pn + i], Z = P.rad[2 * Pn + i];
+      const bool bad = isnan(X) || isinf(X) || isnan(Y) || isinf(Y) || isnan(Z) || isinf(Z);
       if (!bad) {  // addSample (Image.hs:250-299) on the full image
-        float X = 0.0f, Y = 0.0f, Z = 0.0f;
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-          X = X + cCieX[b] * L.v[b];
-          Y = Y + cCieY[b] * L.v[b];
-          Z = Z + cCieZ[b] * L.v[b];
-        }
         X = X / BGPU_CIE_Y_SUM;
         Y = Y / BGPU_CIE_Y_SUM;
         Z = Z / BGPU_CIE_Y_SUM;
@@ -632,24 +682,25 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
 constexpr int kWideBlock = 1024;
 
 template <int BLOCK>
-static hipError_t enqueue_iter(const PtKernelArgs& a, hipStream_t s) {
+static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, hipStream_t s) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
   dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
   hipLaunchKernelGGL(k_extend<BLOCK>, g, b, 0, s, a);
-  hipLaunchKernelGGL(k_shade<BLOCK>, g, b, 0, s, a);
+  if (shade_waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
   hipLaunchKernelGGL(k_shadow<BLOCK>, g, b, 0, s, a);
   hipLaunchKernelGGL(k_mis<BLOCK>, g, b, 0, s, a);
   hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, int block, hipStream_t s) {
-  switch (block) {
-    case 64: return enqueue_iter<64>(a, s);
-    case 256: return enqueue_iter<256>(a, s);
-    default: return enqueue_iter<128>(a, s);
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, hipStream_t s) {
+  switch (cfg.block) {
+    case 64: return enqueue_iter<64>(a, cfg.shade_waves, s);
+    case 256: return enqueue_iter<256>(a, cfg.shade_waves, s);
+    default: return enqueue_iter<128>(a, cfg.shade_waves, s);
   }
 }
 

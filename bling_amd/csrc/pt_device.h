@@ -376,29 +376,76 @@ __device__ __forceinline__ float fr_diel_scalar(float etai, float etat, float co
   float rPerp = (ac - rpp) / (ac + rpp);
   return (rParl * rParl + rPerp * rPerp) * 0.5f;
 }
-// f := r * fresnel(cosi)  (per band)
-__device__ __forceinline__ void r_times_fresnel(const DBxdf& b, float cosi, Sp& out) {
-  if (b.fr == FR_DIEL) {
-    float f = fr_diel_scalar(b.ei, b.et, cosi);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) out.v[i] = b.r[i] * f;
-  } else if (b.fr == FR_COND) {  // Fresnel.hs:254-266
-    float ac = fabsf(cosi);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      float eta = b.eta[i], k = b.k[i];
-      float tmpF = eta * eta + k * k;
-      float tmp = tmpF * (ac * ac);
-      float ec2 = eta * (2.0f * ac);
-      float c2 = ac * ac;
-      float rPer2 = (tmpF - ec2 + c2) / (tmpF + ec2 + c2);
-      float rPar2 = (tmp - ec2 + 1.0f) / (tmp + ec2 + 1.0f);
-      out.v[i] = b.r[i] * ((rPer2 + rPar2) / 2.0f);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) out.v[i] = b.r[i] * 1.0f;
+// Spectral BSDF values are carried as at most two scalar-weighted terms and
+// expanded one band at a time where they are consumed, so that no 16-band
+// temporaries stay live across the shading code.  Each term reproduces the
+// per-band arithmetic of the reference expression it stands for.
+enum {
+  T_RS = 0,   // r[b] * a                      (Lambert eval r*s, Lambert/Oren sample)
+  T_ROS = 1,  // (r[b] * a) * c                (Oren-Nayar eval (r*o)*s)
+  T_RFX = 2,  // (r[b] * c) * a                (r times a scalar Fresnel c, then scaled)
+  T_RCX = 3,  // (r[b] * Fcond_b(c)) * a       (r times the conductor Fresnel at cosine c)
+  T_TRN = 4   // (a * r[b]) * c                (specular transmission ((1-fr)*r)*eta2)
+};
+struct Term {
+  const DBxdf* bx;
+  int op;
+  float a, c;
+};
+// value_b = V_SUM:   (0 + t0) + t1            (terms present: n)
+//           V_SCALE: t0 * k1
+//           V_MIX:   (t0 * k1 + fo) * k2      fo = n > 1 ? 0 + t1 : 0
+enum { V_SUM = 0, V_SCALE = 1, V_MIX = 2 };
+struct SpecVal {
+  Term t[2];
+  int n, mode;
+  float k1, k2;
+};
+
+__device__ __forceinline__ float fr_cond_band(const DBxdf& b, float cosi, int i) {  // Fresnel.hs:254-266
+  float ac = fabsf(cosi);
+  float eta = b.eta[i], k = b.k[i];
+  float tmpF = eta * eta + k * k;
+  float tmp = tmpF * (ac * ac);
+  float ec2 = eta * (2.0f * ac);
+  float c2 = ac * ac;
+  float rPer2 = (tmpF - ec2 + c2) / (tmpF + ec2 + c2);
+  float rPar2 = (tmp - ec2 + 1.0f) / (tmp + ec2 + 1.0f);
+  return (rPer2 + rPar2) / 2.0f;
+}
+__device__ __forceinline__ float term_at(const Term& t, int b) {
+  const DBxdf& x = *t.bx;
+  switch (t.op) {
+    case T_RS: return x.r[b] * t.a;
+    case T_ROS: return (x.r[b] * t.a) * t.c;
+    case T_RFX: return (x.r[b] * t.c) * t.a;
+    case T_RCX: return (x.r[b] * fr_cond_band(x, t.c, b)) * t.a;
+    default: return (t.a * x.r[b]) * t.c;
   }
+}
+__device__ __forceinline__ float spec_at(const SpecVal& v, int b) {
+  if (v.mode == V_SUM) {
+    float f = 0.0f;
+    if (v.n > 0) f = f + term_at(v.t[0], b);
+    if (v.n > 1) f = f + term_at(v.t[1], b);
+    return f;
+  }
+  if (v.mode == V_SCALE) return term_at(v.t[0], b) * v.k1;
+  float fo = v.n > 1 ? 0.0f + term_at(v.t[1], b) : 0.0f;
+  return (term_at(v.t[0], b) * v.k1 + fo) * v.k2;
+}
+__device__ __forceinline__ SpecVal spec_zero() {
+  SpecVal v;
+  v.n = 0;
+  v.mode = V_SUM;
+  v.k1 = v.k2 = 0.0f;
+  return v;
+}
+// r * fresnel(cosi) (Fresnel.hs), scaled by a, as a term
+__device__ __forceinline__ Term rf_term(const DBxdf& b, float cosi, float a) {
+  if (b.fr == FR_COND) return Term{&b, T_RCX, a, cosi};
+  float f = b.fr == FR_DIEL ? fr_diel_scalar(b.ei, b.et, cosi) : 1.0f;
+  return Term{&b, T_RFX, a, f};
 }
 
 __device__ __forceinline__ float mf_g(f3 wo, f3 wi, f3 wh) {
@@ -426,31 +473,30 @@ __device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // 
   return b.A + b.B * maxcos * sina * tanb;
 }
 
-// bxdfEval b wo wi, accumulated: acc += value
-__device__ __forceinline__ void bx_eval_acc(const DBxdf& b, f3 wo, f3 wi, Sp& acc) {
+// bxdfEval b wo wi as a term; false when the component contributes nothing
+__device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term* t) {
   if (b.kind == K_LAMBERT) {
-    float s = kInvPi * acos_t(wo);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + b.r[i] * s;
-  } else if (b.kind == K_OREN) {
-    float o = oren_scale(b, wo, wi), s = kInvPi * acos_t(wo);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + (b.r[i] * o) * s;
-  } else if (b.kind == K_MF) {  // Microfacet.hs:21-32
+    *t = Term{&b, T_RS, kInvPi * acos_t(wo), 0.0f};
+    return true;
+  }
+  if (b.kind == K_OREN) {
+    *t = Term{&b, T_ROS, oren_scale(b, wo, wi), kInvPi * acos_t(wo)};
+    return true;
+  }
+  if (b.kind == K_MF) {  // Microfacet.hs:21-32
     float costo = acos_t(wo), costi = acos_t(wi);
-    if (costi == 0.0f || costo == 0.0f) return;
+    if (costi == 0.0f || costo == 0.0f) return false;
     f3 whp = add3(wi, wo);
-    if (whp.x == 0.0f && whp.y == 0.0f && whp.z == 0.0f) return;
+    if (whp.x == 0.0f && whp.y == 0.0f && whp.z == 0.0f) return false;
     f3 wh = norm3(whp);
-    if (cos_t(wh) < 0.0f) return;
+    if (cos_t(wh) < 0.0f) return false;
     float costh = dot3(wi, wh);
     float D = (b.e + 2.0f) * kInvTwoPi * powf(acos_t(wh), b.e);
     float x = D * mf_g(wo, wi, wh) / (4.0f * costi);
-    Sp rf;
-    r_times_fresnel(b, costh, rf);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc.v[i] = acc.v[i] + rf.v[i] * x;
+    *t = rf_term(b, costh, x);
+    return true;
   }
+  return false;  // specular components evaluate to zero
 }
 __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
   if (b.kind == K_LAMBERT || b.kind == K_OREN) return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
@@ -463,20 +509,19 @@ __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
   }
   return 0.0f;
 }
-// bxdfSample with adj = False
-__device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float u2, Sp& f, f3* wi,
-                                          float* pdf) {
+// bxdfSample with adj = False: the sampled value as a term (zero: *zero set)
+__device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float u2, Term* f, bool* zero,
+                                          f3* wi, float* pdf) {
+  *zero = false;
   if (b.kind == K_LAMBERT || b.kind == K_OREN) {
     f3 w = cos_hemisphere(u1, u2);
     if (wo.z < 0.0f) w.z = -w.z;  // toSameHemisphere
     if (same_hemi(wo, w)) {
-      float o = b.kind == K_OREN ? oren_scale(b, wo, w) : 1.0f;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) f.v[i] = b.kind == K_OREN ? b.r[i] * o : b.r[i];
+      *f = Term{&b, T_RS, b.kind == K_OREN ? oren_scale(b, wo, w) : 1.0f, 0.0f};
       *wi = w;
       *pdf = kInvPi * acos_t(w);
     } else {
-      f = sp_const(0.0f);
+      *zero = true;
       *wi = b.kind == K_LAMBERT ? wo : w;
       *pdf = 0.0f;
     }
@@ -494,22 +539,18 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
     float costH = dot3(wo, wh);
     if (same_hemi(wo, w)) {
       float fact = d * fabsf(costH) / pd * mf_g(wo, w, wh);
-      Sp rf;
-      r_times_fresnel(b, costH, rf);
-      float sc = fact / acos_t(w);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) f.v[i] = rf.v[i] * sc;
+      *f = rf_term(b, costH, fact / acos_t(w));
       *wi = w;
       *pdf = pd / (4.0f * fabsf(costH));
     } else {
-      f = sp_const(0.0f);
+      *zero = true;
       *wi = wo;
       *pdf = 0.0f;
     }
     return;
   }
   if (b.kind == K_SPECR) {  // Specular.hs:460-465
-    r_times_fresnel(b, cos_t(wo), f);
+    *f = rf_term(b, cos_t(wo), 1.0f);
     *wi = mk3(-wo.x, -wo.y, wo.z);
     *pdf = 1.0f;
     return;
@@ -520,7 +561,7 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
   float eta = ei / et, eta2 = eta * eta;
   float sint2 = eta2 * sin_t2(wo);
   if (sint2 >= 1.0f) {
-    f = sp_const(0.0f);
+    *zero = true;
     *wi = wo;
     *pdf = 0.0f;
     return;
@@ -529,8 +570,7 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
   float cost = entering ? -c : c;
   *wi = mk3(eta * (-wo.x), eta * (-wo.y), cost);
   float fr = fr_diel_scalar(ei, et, cost);
-#pragma unroll
-  for (int i = 0; i < 16; ++i) f.v[i] = ((1.0f - fr) * b.r[i]) * eta2;
+  *f = Term{&b, T_TRN, 1.0f - fr, eta2};
   *pdf = 1.0f;
 }
 
@@ -545,16 +585,17 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, con
 }
 __device__ __forceinline__ int bsdf_n(const Bsdf& b) { return b.m ? b.m->n : 0; }
 
-__device__ __forceinline__ void eval_bsdf(const Bsdf& b, f3 woW, f3 wiW, Sp& f) {  // Reflection.hs:318-332
-  f = sp_const(0.0f);
+__device__ __forceinline__ SpecVal eval_bsdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:318-332
+  SpecVal f = spec_zero();
   float cosWo = dot3(woW, b.ng);
   float side = dot3(wiW, b.ng) / cosWo;
-  if (side == 0.0f || fabsf(cosWo) < 1e-5f) return;
+  if (side == 0.0f || fabsf(cosWo) < 1e-5f) return f;
   int need = side < 0.0f ? BX_T : BX_R;
   f3 wo = to_local(b.cs, woW), wi = to_local(b.cs, wiW);
   int n = bsdf_n(b);
   for (int i = 0; i < n; ++i)
-    if (b.m->c[i].type & need) bx_eval_acc(b.m->c[i], wi, wo, f);
+    if ((b.m->c[i].type & need) && bx_eval_term(b.m->c[i], wi, wo, &f.t[f.n])) f.n++;
+  return f;
 }
 __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:251-257
   int n = bsdf_n(b);
@@ -566,8 +607,8 @@ __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, f3 woW, f3 wiW) {  // R
 }
 // sampleBsdf (Reflection.hs:278-316), adj = False, all flags; returns pdf.
 __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp, float u1, float u2,
-                                             Sp& f, f3* wiW, int* type) {
-  f = sp_const(0.0f);
+                                             SpecVal& f, f3* wiW, int* type) {
+  f = spec_zero();
   *wiW = mk3(0.0f, 1.0f, 0.0f);
   *type = BX_R | BX_D;
   int cnt = bsdf_n(b);
@@ -578,10 +619,11 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
   sNum = sNum < 0 ? 0 : sNum;
   const DBxdf& bx = b.m->c[sNum];
-  Sp fs;
+  Term fs;
+  bool fzero;
   f3 wi;
   float pdfp;
-  bx_sample(bx, wo, u1, u2, fs, &wi, &pdfp);
+  bx_sample(bx, wo, u1, u2, &fs, &fzero, &wi, &pdfp);
   f3 ww = to_world(b.cs, wi);
   float side = dot3(ww, b.ng) / dot3(woW, b.ng);
   if (pdfp == 0.0f || side == 0.0f) return 0.0f;
@@ -589,26 +631,29 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   if (!(bx.type & need)) return 0.0f;
   *wiW = ww;
   *type = bx.type;
+  if (fzero) fs = Term{&bx, T_RS, 0.0f, 0.0f};  // a zero spectrum (r * 0)
+  f.t[0] = fs;
+  f.n = 1;
   if (bx.type & BX_S) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) f.v[i] = fs.v[i] * cntf;
+    f.mode = V_SCALE;
+    f.k1 = cntf;
     return pdfp * invCnt;
   }
   if (cnt == 1) {
-    f = fs;
+    f.mode = V_SCALE;
+    f.k1 = 1.0f;
     return pdfp;
   }
   float other = 0.0f;
-  Sp fo = sp_const(0.0f);
   for (int i = 0; i < cnt; ++i) {
     if (i == sNum) continue;
     other = other + bx_pdf(b.m->c[i], wo, wi);
-    if (b.m->c[i].type & need) bx_eval_acc(b.m->c[i], wi, wo, fo);
+    if ((b.m->c[i].type & need) && bx_eval_term(b.m->c[i], wi, wo, &f.t[1])) f.n = 2;
   }
   float pdf = (pdfp + other) * invCnt;
-  float ip = 1.0f / pdf;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) f.v[i] = (fs.v[i] * pdfp + fo.v[i]) * ip;
+  f.mode = V_MIX;
+  f.k1 = pdfp;
+  f.k2 = 1.0f / pdf;
   return pdf;
 }
 

@@ -59,6 +59,14 @@ class Context:
     def scene(self, flat):
         return GpuScene(self, flat)
 
+    def config_set(self, key, value):
+        _ck(self.lib.bgpu_config_set(self.h, key.encode(), int(value)))
+
+    def config_get(self, key):
+        v = C.c_int64()
+        _ck(self.lib.bgpu_config_get(self.h, key.encode(), C.byref(v)))
+        return v.value
+
 
 def _rays_struct(arrs):
     r = abi.Rays()

@@ -264,6 +264,16 @@ int bgpu_job_latency_ms(bgpu_job* job, double* ms);
 int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out);
 int bgpu_stats_reset(bgpu_ctx* ctx);
 
+/* Tuning knobs (no counterpart in the reference, whose CPU integrator has
+ * none): "trace.block" (64/128/256, intersect kernels), "pt.block"
+ * (64/128/256, bounce kernels), "pt.pool" (path slots, >= 1024),
+ * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
+ * two waves per SIMD).  Changing a pt.* knob is refused with BGPU_E_STATE
+ * while served jobs are queued or running; the engines are rebuilt on their
+ * next use.  bgpu_config_get reads the current value. */
+int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value);
+int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -198,3 +198,41 @@ def test_job_view_tracks_oracle_and_scene_film(gpu_ctx):
         assert e.value.code == abi.E_INVALID
     finally:
         gs.close()
+
+
+def test_tuning_knobs_keep_results(gpu_ctx):
+    """Kernel block size, pool size and the shading kernel's occupancy variant
+    change scheduling only: the image is the same up to film-atomic order."""
+    flat = scenes.cornell(32)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=4)
+    base, _ = render_gpu(gpu_ctx, flat, spec)
+    defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves")}
+    try:
+        for key, val in (("pt.shade_waves", 2), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096)):
+            gpu_ctx.config_set(key, val)
+            assert gpu_ctx.config_get(key) == val
+            img, _ = render_gpu(gpu_ctx, flat, spec)
+            assert np.array_equal(img[..., 0], base[..., 0]), key
+            np.testing.assert_allclose(img, base, rtol=1e-4, atol=1e-5, err_msg=key)
+            gpu_ctx.config_set(key, defaults[key])
+        with pytest.raises(abi.BgpuError) as e:
+            gpu_ctx.config_set("pt.block", 96)
+        assert e.value.code == abi.E_INVALID
+        with pytest.raises(abi.BgpuError) as e:
+            gpu_ctx.config_set("no.such.key", 1)
+        assert e.value.code == abi.E_INVALID
+        gs = gpu_ctx.scene(flat)
+        try:
+            j = Job(gs, spec)
+            with pytest.raises(abi.BgpuError) as e:
+                gpu_ctx.config_set("pt.pool", 8192)  # a served job handle is live
+            assert e.value.code == abi.E_STATE
+            j.wait(timeout=120)
+            j.release()
+            gpu_ctx.config_set("pt.pool", 8192)
+            gpu_ctx.config_set("pt.pool", defaults["pt.pool"])
+        finally:
+            gs.close()
+    finally:
+        for k, v in defaults.items():
+            gpu_ctx.config_set(k, v)
