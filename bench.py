@@ -95,7 +95,7 @@ def main():
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
-    for k in ("trace.block", "pt.block", "pt.pool", "pt.shade_waves"):
+    for k in ("bvh.width", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves"):
         knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,
@@ -235,6 +235,7 @@ def intersect_leg(ctx, args, out):
                                                          nr, args.res, args.res, nr),
         "rays_per_step": len(launches) * nr,
         "bvh_build_s": round(build_s, 2),
+        "bvh": gs.bvh_info(),
         "per_launch": per_launch,
         "shape_hit_fraction_incoherent": round(shape_frac, 4),
     })

@@ -87,6 +87,12 @@ class JobDesc(C.Structure):
                 ("pad", C.c_int32), ("camera", C.POINTER(Camera)), ("film", C.POINTER(Film))]
 
 
+class BvhInfo(C.Structure):
+    _fields_ = [("width", C.c_int32), ("nodes", C.c_int32), ("leaves", C.c_int32),
+                ("max_depth", C.c_int32), ("stack_bound", C.c_int32), ("pad", C.c_int32),
+                ("build_ms", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extend", C.c_uint64),
                 ("rays_shadow", C.c_uint64), ("rays_mis", C.c_uint64),
@@ -100,7 +106,7 @@ EXPORTS = [
     "bgpu_scene_destroy", "bgpu_scene_bytes", "bgpu_intersect_closest", "bgpu_intersect_any",
     "bgpu_last_kernel_ms", "bgpu_render", "bgpu_job_submit", "bgpu_job_poll", "bgpu_job_wait",
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
-    "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get",
+    "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get", "bgpu_scene_bvh_info",
 ]
 
 _lib = None
@@ -142,6 +148,7 @@ def load(path=LIB_PATH):
     lib.bgpu_stats_reset.argtypes = [vp]
     lib.bgpu_config_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.bgpu_config_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    lib.bgpu_scene_bvh_info.argtypes = [vp, C.POINTER(BvhInfo)]
     _lib = lib
     return lib
 

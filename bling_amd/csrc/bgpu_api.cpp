@@ -128,6 +128,13 @@ int bgpu_scene_destroy(bgpu_scene* scene) {
   return BGPU_OK;
 }
 
+int bgpu_scene_bvh_info(const bgpu_scene* scene, bgpu_bvh_info* out) {
+  if (!scene || !out) return fail(BGPU_E_INVALID, "bgpu_scene_bvh_info: NULL argument");
+  const BuildStats& b = scene->build;
+  *out = bgpu_bvh_info{b.width, b.nodes, b.leaves, b.max_depth, b.stack_bound, 0, b.build_ms};
+  return BGPU_OK;
+}
+
 int bgpu_scene_bytes(const bgpu_scene* scene, size_t* bytes) {
   if (!scene || !bytes) return fail(BGPU_E_INVALID, "bgpu_scene_bytes: NULL argument");
   *bytes = scene->device_bytes();
@@ -205,7 +212,9 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, b
   unsigned long long* cnt;
   if (int rc = count_begin(c, flags, &cnt)) return rc;
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
-  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, cnt));
+  if (!c->work_buf.ptr)
+    if (int rc = c->work_buf.alloc(c, 256)) return rc;
+  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, cnt, c->work_buf.as<unsigned int>()));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) {
     BGPU_HIP(hipMemcpyAsync(hits->t, dh.t, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -245,7 +254,9 @@ int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n, uint8
   unsigned long long* cnt;
   if (int rc = count_begin(c, flags, &cnt)) return rc;
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
-  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, cnt));
+  if (!c->work_buf.ptr)
+    if (int rc = c->work_buf.alloc(c, 256)) return rc;
+  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, cnt, c->work_buf.as<unsigned int>()));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) BGPU_HIP(hipMemcpyAsync(occluded, docc, n, hipMemcpyDeviceToHost, c->stream));
   BGPU_HIP(hipStreamSynchronize(c->stream));

@@ -66,6 +66,7 @@ struct Ctx {
   // staging for host-pointer ray batches
   DevBuf stage_rays, stage_hits;
   DevBuf count_buf;               // BGPU_COUNT traversal counters
+  DevBuf work_buf;                // ray counters of the persistent traversal kernels
   std::mutex stats_mu;
   bgpu_stats stats{};
   Scheduler* sched = nullptr;     // served path, created on first job_submit
@@ -74,6 +75,7 @@ struct Ctx {
   std::atomic<int> live_jobs{0};  // served job handles not yet deleted
   TraceConfig trace_cfg;
   PtConfig pt_cfg;
+  int bvh_width = 4;              // children per BVH node of scenes created next
   ~Ctx();
 };
 

@@ -273,23 +273,17 @@ void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
   if (stats) *stats = st;
 }
 
-void collapse_bvh(const std::vector<BNode>& bin, int width, std::vector<WNode>* out) {
-  out->clear();
-  if (bin.empty()) return;
-  // map BVH2 node index -> output index, assigned in DFS order
-  struct Item {
-    int src;
-  };
-  std::vector<int> remap(bin.size(), -1);
-  std::vector<int> todo{0};
-  remap[0] = 0;
+int collapse_bvh(const std::vector<BNode>& bin, int root, int width, std::vector<WNode>* out) {
+  // BVH2 inner index -> output index; children are assigned in DFS order
+  const int base = (int)out->size();
+  std::vector<std::pair<int, int>> todo{{root, base}};  // (BVH2 node, output node)
   out->push_back(WNode{});
   while (!todo.empty()) {
-    int s = todo.back();
+    const int s = todo.back().first, o = todo.back().second;
     todo.pop_back();
-    int o = remap[s];
     std::vector<Box> cb{bin[s].child[0], bin[s].child[1]};
     std::vector<int32_t> cl{bin[s].link[0], bin[s].link[1]};
+    // open the largest inner child until the node is `width` wide
     while ((int)cl.size() < width) {
       int pick = -1;
       float pa = -1.0f;
@@ -299,7 +293,7 @@ void collapse_bvh(const std::vector<BNode>& bin, int width, std::vector<WNode>* 
           pick = (int)i;
         }
       if (pick < 0) break;
-      int g = cl[pick];
+      const int g = cl[pick];
       cb[pick] = bin[g].child[0];
       cl[pick] = bin[g].link[0];
       cb.push_back(bin[g].child[1]);
@@ -307,16 +301,38 @@ void collapse_bvh(const std::vector<BNode>& bin, int width, std::vector<WNode>* 
     }
     for (size_t i = 0; i < cl.size(); ++i) {
       if (cl[i] >= 0) {
-        int ni = (int)out->size();
+        const int ni = (int)out->size();
         out->push_back(WNode{});
-        remap[cl[i]] = ni;
-        todo.push_back(cl[i]);
+        todo.push_back({cl[i], ni});
         cl[i] = ni;
       }
     }
     (*out)[o].child = cb;
     (*out)[o].link = cl;
   }
+  return base;
+}
+
+int stack_bound(const std::vector<WNode>& w, int root) {
+  // post-order over the tree: bound(n) = (children - 1) + max bound(inner child)
+  std::vector<int> bound(w.size(), -1);
+  std::vector<int> st{root};
+  while (!st.empty()) {
+    const int n = st.back();
+    bool ready = true;
+    for (int32_t l : w[n].link)
+      if (l >= 0 && bound[l] < 0) {
+        st.push_back(l);
+        ready = false;
+      }
+    if (!ready) continue;
+    st.pop_back();
+    int m = 0;
+    for (int32_t l : w[n].link)
+      if (l >= 0) m = std::max(m, bound[l]);
+    bound[n] = (int)w[n].link.size() - 1 + m;
+  }
+  return bound[root];
 }
 
 }  // namespace bgpu

@@ -31,6 +31,7 @@ struct BuildParams {
 
 struct BuildStats {
   int nodes = 0, leaves = 0, max_depth = 0, max_leaf_size = 0;
+  int width = 2, stack_bound = 0;
   double sah = 0.0, build_ms = 0.0;
 };
 
@@ -40,13 +41,20 @@ struct BuildStats {
 void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
                 std::vector<BNode>* nodes, std::vector<int32_t>* order, BuildStats* stats);
 
-// Collapses a BVH2 into a BVH of width w (4 or 8) by pulling grandchildren up
-// in SAH order; same leaf encoding.
+// A node of a BVH of width w: up to w children, inner links index the output
+// array, leaves keep the BVH2 encoding.
 struct WNode {
   std::vector<Box> child;
   std::vector<int32_t> link;
 };
-void collapse_bvh(const std::vector<BNode>& bin, int width, std::vector<WNode>* out);
+// Appends the subtree of `bin` rooted at `root` to `out` as a BVH of width
+// `width` (2, 4 or 8), opening the largest inner child until each node is
+// full; returns the index of the new root.
+int collapse_bvh(const std::vector<BNode>& bin, int root, int width, std::vector<WNode>* out);
+// Exact worst case of a traversal stack that pushes every hit child but the
+// one it descends into: the maximum over root-to-leaf paths of the summed
+// (children - 1).
+int stack_bound(const std::vector<WNode>& w, int root);
 
 Box pad_box(const Box& b);
 

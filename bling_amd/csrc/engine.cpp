@@ -758,6 +758,18 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     ctx->trace_cfg.block = (int)value;
     return BGPU_OK;
   }
+  if (k == "trace.persistent") {
+    if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "trace.persistent must be 0 or 1");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->trace_cfg.persistent = (int)value;
+    return BGPU_OK;
+  }
+  if (k == "bvh.width") {
+    if (value != 2 && value != 4 && value != 8) return fail(BGPU_E_INVALID, "bvh.width must be 2, 4 or 8");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->bvh_width = (int)value;
+    return BGPU_OK;
+  }
   PtConfig next = ctx->pt_cfg;
   if (k == "pt.block") {
     if (!block_ok(value)) return fail(BGPU_E_INVALID, "pt.block must be 64, 128 or 256");
@@ -791,6 +803,8 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   if (!ctx || !key || !value) return fail(BGPU_E_INVALID, "bgpu_config_get: NULL argument");
   const std::string k(key);
   if (k == "trace.block") *value = ctx->trace_cfg.block;
+  else if (k == "bvh.width") *value = ctx->bvh_width;
+  else if (k == "trace.persistent") *value = ctx->trace_cfg.persistent;
   else if (k == "pt.block") *value = ctx->pt_cfg.block;
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;

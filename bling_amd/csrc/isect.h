@@ -25,8 +25,14 @@ namespace bgpu {
 // evaluates c = |o|^2 - r^2 with cancellation and so reports hits up to about
 // 3*sqrt(2^-24)*|o| outside the true surface; a box padded only by rounding
 // would cull those hits, which the reference does report.
+// Traversal stack entries available per lane (LDS part plus private spill);
+// scene creation checks the tree's worst case against it.
+constexpr int kStackCapacity = 64;
+// link of an unused child slot in a node
+constexpr int32_t kNoChild = 0x7fffffff;
+
 struct DevScene {
-  const float4* nodes;
+  const float4* nodes;  // 8 * width floats per node
   const float4* leaf;
   const DShape* shapes;
   float root_lo[3], root_hi[3];
@@ -34,6 +40,8 @@ struct DevScene {
   int32_t shape_root;  // -1 if no shapes
   float shape_pad_k;
   int32_t num_nodes;
+  int32_t width;       // 2, 4 or 8 children per node
+  int32_t pad;
 };
 
 struct DevRays {
@@ -47,11 +55,16 @@ struct DevHits {
 // Launch-shape knobs chosen by the autotuner.
 struct TraceConfig {
   int block = 128;
+  int persistent = 1;  // lanes refill with new rays (1) or one ray per thread (0)
 };
 
+// `work`: two zeroed-on-launch counters for the persistent kernels (may be
+// null, which selects one ray per thread).
 hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,
-                          const TraceConfig& cfg, hipStream_t s, unsigned long long* counters);
+                          const TraceConfig& cfg, hipStream_t s, unsigned long long* counters,
+                          unsigned int* work);
 hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
-                      const TraceConfig& cfg, hipStream_t s, unsigned long long* counters);
+                      const TraceConfig& cfg, hipStream_t s, unsigned long long* counters,
+                      unsigned int* work);
 
 }  // namespace bgpu

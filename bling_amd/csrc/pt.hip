@@ -311,7 +311,7 @@ __device__ __forceinline__ void for_uniform_scene(bool valid, int my, F&& body) 
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
-  __shared__ int lds[trav::kShort * BLOCK];
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_act;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
@@ -524,7 +524,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
-  __shared__ int lds[trav::kShort * BLOCK];
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_shadow;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
-  __shared__ int lds[trav::kShort * BLOCK];
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_mis;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;

@@ -19,6 +19,7 @@ Ctx::~Ctx() {
   stage_rays.release();
   stage_hits.release();
   count_buf.release();
+  work_buf.release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -153,38 +154,32 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     extend_b(root, pb[d->num_triangles + s]);
   }
 
-  // -- BVHs: one over the triangles, one over the shapes, in one node array
-  std::vector<BNode> nodes;
+  // -- BVHs: one over the triangles, one over the shapes, built binary and
+  // collapsed to the context's width into one node array
+  std::vector<BNode> tnodes, snodes;
   std::vector<int32_t> order;
   BuildParams bp;
-  int32_t tri_root = -1, shape_root = -1;
   float aniso = 1.0f;
   if (d->num_triangles) {
     std::vector<Box> tb(pb.begin(), pb.begin() + d->num_triangles);
-    build_bvh2(tb, bp, &nodes, &order, &sc->build);
-    tri_root = 0;
+    build_bvh2(tb, bp, &tnodes, &order, &sc->build);
   }
   if (d->num_shapes) {
     std::vector<Box> sb(pb.begin() + d->num_triangles, pb.end());
-    std::vector<BNode> sn;
     std::vector<int32_t> so;
     BuildStats sst;
-    build_bvh2(sb, bp, &sn, &so, &sst);
-    int32_t noff = (int32_t)nodes.size(), loff = (int32_t)order.size();
-    for (BNode& n : sn)
-      for (int s = 0; s < 2; ++s) {
-        if (n.link[s] >= 0) {
-          n.link[s] += noff;
-        } else {
+    build_bvh2(sb, bp, &snodes, &so, &sst);
+    // the shape leaves follow the triangle leaves in the leaf records
+    const int32_t loff = (int32_t)order.size();
+    for (BNode& n : snodes)
+      for (int s = 0; s < 2; ++s)
+        if (n.link[s] < 0) {
           int enc = ~n.link[s];
           n.link[s] = ~((((enc >> kLeafCountBits) + loff) << kLeafCountBits) | (enc & kMaxLeaf));
         }
-      }
-    shape_root = noff;
-    nodes.insert(nodes.end(), sn.begin(), sn.end());
     for (int32_t id : so) order.push_back(id + (int32_t)d->num_triangles);
-    sc->build.nodes += sst.nodes;
     sc->build.leaves += sst.leaves;
+    sc->build.max_depth = std::max(sc->build.max_depth, sst.max_depth);
     // anisotropy of each o2w's linear part scales the false-hit radius
     for (uint32_t s = 0; s < d->num_shapes; ++s) {
       const float* m = d->shapes[s].o2w;
@@ -197,15 +192,40 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
       aniso = std::max(aniso, std::min(a, 1000.0f));
     }
   }
+  // the widest width whose worst-case traversal stack fits the device stack
+  std::vector<WNode> nodes;
+  int32_t tri_root = -1, shape_root = -1;
+  int width = ctx->bvh_width;
+  for (;; width /= 2) {
+    nodes.clear();
+    tri_root = tnodes.empty() ? -1 : collapse_bvh(tnodes, 0, width, &nodes);
+    shape_root = snodes.empty() ? -1 : collapse_bvh(snodes, 0, width, &nodes);
+    int bound = 0;
+    if (tri_root >= 0) bound = std::max(bound, stack_bound(nodes, tri_root));
+    if (shape_root >= 0) bound = std::max(bound, stack_bound(nodes, shape_root));
+    sc->build.stack_bound = bound;
+    if (bound <= kStackCapacity || width == 2) break;
+  }
+  if (sc->build.stack_bound > kStackCapacity)
+    return fail(BGPU_E_INVALID, "BVH too deep for the traversal stack");
+  sc->build.width = width;
+  sc->build.nodes = (int)nodes.size();
 
-  std::vector<float> hn(16 * nodes.size());
+  // node layout, 8 * width floats: lo.x[w] hi.x[w] lo.y[w] hi.y[w] lo.z[w]
+  // hi.z[w] link[w] pad[w]; unused slots carry the link kNoChild
+  const int W = width;
+  std::vector<float> hn((size_t)8 * W * std::max<size_t>(nodes.size(), 1), 0.0f);
   for (size_t i = 0; i < nodes.size(); ++i) {
-    const BNode& n = nodes[i];
-    float* o = &hn[16 * i];
-    o[0] = n.child[0].lo[0]; o[1] = n.child[0].hi[0]; o[2] = n.child[0].lo[1]; o[3] = n.child[0].hi[1];
-    o[4] = n.child[0].lo[2]; o[5] = n.child[0].hi[2]; o[6] = n.child[1].lo[0]; o[7] = n.child[1].hi[0];
-    o[8] = n.child[1].lo[1]; o[9] = n.child[1].hi[1]; o[10] = n.child[1].lo[2]; o[11] = n.child[1].hi[2];
-    o[12] = bitsf(n.link[0]); o[13] = bitsf(n.link[1]); o[14] = 0.0f; o[15] = 0.0f;
+    const WNode& n = nodes[i];
+    float* o = &hn[(size_t)8 * W * i];
+    for (int c = 0; c < W; ++c) {
+      const bool used = c < (int)n.link.size();
+      for (int k = 0; k < 3; ++k) {
+        o[(2 * k) * W + c] = used ? n.child[c].lo[k] : 0.0f;
+        o[(2 * k + 1) * W + c] = used ? n.child[c].hi[k] : 0.0f;
+      }
+      o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
+    }
   }
   std::vector<float> hl(12 * std::max<size_t>(order.size(), 1), 0.0f);
   for (size_t k = 0; k < order.size(); ++k) {
@@ -255,6 +275,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     ds.root_hi[k] = root.hi[k];
   }
   ds.num_nodes = (int32_t)nodes.size();
+  ds.width = width;
   ds.tri_root = tri_root;
   ds.shape_root = shape_root;
   // 3*sqrt(2^-24) = 7.3e-4 bounds the quadratic's false-hit radius per unit

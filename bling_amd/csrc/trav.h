@@ -1,32 +1,60 @@
 // Device BVH traversal shared by the ray-batch kernels and the path tracer.
-// One ray per lane.  The traversal stack keeps its top kShort entries in LDS
-// (laid out [depth][lane] so a wave's pushes hit 64 distinct banks) and spills
-// deeper entries to a private array (scratch); the builder bounds the tree
-// depth, hence the stack.  Node boxes are tested with explicit FMAs (the boxes
-// are padded outward, so the test only has to be conservative); primitives
-// are tested with the reference's exact arithmetic.
+// One ray per lane over a BVH of width 2, 4 or 8 (8 * width floats per node:
+// the children's slabs as six arrays of width floats, then the links).  Hit
+// children are sorted near to far; the nearest is entered and the others are
+// pushed with their entry distances, so that a closest-hit query skips popped
+// subtrees that start beyond its current hit.  The stack keeps its top kShort
+// entries in LDS (laid out [depth][lane] so a wave's pushes hit 64 distinct
+// banks) and spills deeper entries to a private array; scene creation bounds
+// the worst-case depth by kStackCapacity.  Node boxes are tested with explicit
+// FMAs (they are padded outward, so the test, and the culling by entry
+// distance, only have to be conservative); primitives are tested with the
+// reference's exact arithmetic.
 #pragma once
 #include "isect.h"
 
 namespace bgpu {
 namespace trav {
 
-constexpr int kShort = 16;
-constexpr int kSpill = 48;  // kShort + kSpill >= tree depth bound (60)
+// Stack entries kept in LDS, and whether entries carry their entry distance
+// (closest-hit queries then skip subtrees beyond the current hit).
+#ifndef BGPU_TRAV_SHORT
+#define BGPU_TRAV_SHORT 12
+#endif
+#ifndef BGPU_TRAV_CULL
+#define BGPU_TRAV_CULL 0
+#endif
+constexpr int kShort = BGPU_TRAV_SHORT;
+constexpr bool kCull = BGPU_TRAV_CULL != 0;
+constexpr int kSpill = kStackCapacity - kShort;
+// LDS ints per lane a kernel provides: node links, then entry distances
+constexpr int kLdsPerLane = kCull ? 2 * kShort : kShort;
 
 struct Stack {
-  int* lds;
+  int* lds;    // kLdsPerLane entries per lane, `stride` apart
   int stride;
   int sp;
-  int spill[kSpill];
-  __device__ __forceinline__ void push(int v) {
-    if (sp < kShort) lds[sp * stride] = v;
-    else spill[sp - kShort] = v;
+  int spill_n[kSpill];
+  float spill_t[kCull ? kSpill : 1];
+  __device__ __forceinline__ void push(int v, float t) {
+    if (sp < kShort) {
+      lds[sp * stride] = v;
+      if (kCull) lds[(kShort + sp) * stride] = __float_as_int(t);
+    } else {
+      spill_n[sp - kShort] = v;
+      if (kCull) spill_t[sp - kShort] = t;
+    }
     ++sp;
   }
-  __device__ __forceinline__ int pop() {
+  __device__ __forceinline__ void pop(int* v, float* t) {
     --sp;
-    return sp < kShort ? lds[sp * stride] : spill[sp - kShort];
+    if (sp < kShort) {
+      *v = lds[sp * stride];
+      *t = kCull ? __int_as_float(lds[(kShort + sp) * stride]) : 0.0f;
+    } else {
+      *v = spill_n[sp - kShort];
+      *t = kCull ? spill_t[sp - kShort] : 0.0f;
+    }
   }
 };
 
@@ -74,8 +102,37 @@ struct Best {
   int prim;
 };
 
+// Compare-exchange on (key, link) pairs, ascending by key.
+__device__ __forceinline__ void cex(float& ka, int& la, float& kb, int& lb) {
+  const bool s = kb < ka;
+  const float k0 = s ? kb : ka, k1 = s ? ka : kb;
+  const int l0 = s ? lb : la, l1 = s ? la : lb;
+  ka = k0; kb = k1; la = l0; lb = l1;
+}
+template <int W>
+__device__ __forceinline__ void sort_children(float (&k)[W], int (&l)[W]) {
+  if constexpr (W == 2) {
+    cex(k[0], l[0], k[1], l[1]);
+  } else if constexpr (W == 4) {
+    cex(k[0], l[0], k[1], l[1]); cex(k[2], l[2], k[3], l[3]);
+    cex(k[0], l[0], k[2], l[2]); cex(k[1], l[1], k[3], l[3]);
+    cex(k[1], l[1], k[2], l[2]);
+  } else {  // Batcher's odd-even merge sort of 8
+    cex(k[0], l[0], k[1], l[1]); cex(k[2], l[2], k[3], l[3]);
+    cex(k[4], l[4], k[5], l[5]); cex(k[6], l[6], k[7], l[7]);
+    cex(k[0], l[0], k[2], l[2]); cex(k[1], l[1], k[3], l[3]);
+    cex(k[4], l[4], k[6], l[6]); cex(k[5], l[5], k[7], l[7]);
+    cex(k[1], l[1], k[2], l[2]); cex(k[5], l[5], k[6], l[6]);
+    cex(k[0], l[0], k[4], l[4]); cex(k[1], l[1], k[5], l[5]);
+    cex(k[2], l[2], k[6], l[6]); cex(k[3], l[3], k[7], l[7]);
+    cex(k[2], l[2], k[4], l[4]); cex(k[3], l[3], k[5], l[5]);
+    cex(k[1], l[1], k[2], l[2]); cex(k[3], l[3], k[4], l[4]);
+    cex(k[5], l[5], k[6], l[6]);
+  }
+}
+
 // Traverses one tree.  CLOSEST: shrinks best.t; ANY: returns at the first hit.
-template <bool ANY, bool FAT, bool STATS>
+template <int W, bool ANY, bool FAT, bool STATS>
 __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRay& r, f3 inv,
                                          f3 oi, Stack& st, Best& best, unsigned long long& nvisit,
                                          unsigned long long& ntest) {
@@ -83,32 +140,39 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
   st.sp = 0;
   for (;;) {
     if (cur >= 0) {
-      const float4* nd = sc.nodes + 4 * cur;
-      float4 a = nd[0], b = nd[1], c = nd[2];
-      int4 l = reinterpret_cast<const int4*>(nd)[3];
-      if (STATS) ++nvisit;
-      float tlim = ANY ? r.tmax : best.t;
-      bool h0, h1;
-      float tn0, tn1;
-      if (FAT) {
-        float lx = a.x, hx = a.y, ly = a.z, hy = a.w, lz = b.x, hz = b.y;
-        inflate(lx, hx, ly, hy, lz, hz, r.o, sc.shape_pad_k);
-        slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h0, &tn0);
-        lx = b.z; hx = b.w; ly = c.x; hy = c.y; lz = c.z; hz = c.w;
-        inflate(lx, hx, ly, hy, lz, hz, r.o, sc.shape_pad_k);
-        slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h1, &tn1);
-      } else {
-        slab(a.x, a.y, a.z, a.w, b.x, b.y, inv, oi, r.tmin, tlim, &h0, &tn0);
-        slab(b.z, b.w, c.x, c.y, c.z, c.w, inv, oi, r.tmin, tlim, &h1, &tn1);
+      const float4* nd = sc.nodes + (size_t)cur * (2 * W);
+      float v[8 * W];
+#pragma unroll
+      for (int q = 0; q < (7 * W + 3) / 4; ++q) {
+        const float4 x = nd[q];
+        v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
       }
-      if (h0 && h1) {
-        bool first0 = tn0 <= tn1;
-        st.push(first0 ? l.y : l.x);
-        cur = first0 ? l.x : l.y;
+      if (STATS) ++nvisit;
+      const float tlim = ANY ? r.tmax : best.t;
+      float key[W];
+      int lk[W];
+      int nhit = 0;
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        float lx = v[c], hx = v[W + c], ly = v[2 * W + c], hy = v[3 * W + c], lz = v[4 * W + c],
+              hz = v[5 * W + c];
+        lk[c] = __float_as_int(v[6 * W + c]);
+        if (FAT) inflate(lx, hx, ly, hy, lz, hz, r.o, sc.shape_pad_k);
+        bool h;
+        float tn;
+        slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h, &tn);
+        h = h && lk[c] != kNoChild;
+        key[c] = h ? fminf(tn, 3.0e38f) : INFINITY;  // misses sort last
+        nhit += h ? 1 : 0;
+      }
+      if (nhit > 0) {
+        sort_children<W>(key, lk);
+#pragma unroll
+        for (int c = W - 1; c >= 1; --c)
+          if (c < nhit) st.push(lk[c], key[c]);
+        cur = lk[0];
         continue;
       }
-      if (h0) { cur = l.x; continue; }
-      if (h1) { cur = l.y; continue; }
     } else {
       int enc = ~cur;
       int first = enc >> 4, cnt = enc & 15;
@@ -147,19 +211,48 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
         }
       }
     }
-    if (st.sp == 0) break;
-    cur = st.pop();
+    // next entry whose subtree may still hold a hit at or before best.t (an
+    // equal t can still win the tie on the primitive id)
+    bool found = false;
+    while (st.sp > 0) {
+      int n;
+      float t;
+      st.pop(&n, &t);
+      if (ANY || !kCull || t <= best.t) {
+        cur = n;
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
   }
   return false;
 }
 
+template <int W, bool STATS>
+__device__ __forceinline__ void closest_w(const DevScene& sc, const DRay& r, f3 inv, f3 oi, Stack& st,
+                                          Best& best, unsigned long long& nvisit, unsigned long long& ntest) {
+  if (sc.tri_root >= 0) traverse<W, false, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest);
+  if (sc.shape_root >= 0) traverse<W, false, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest);
+}
+template <int W, bool STATS>
+__device__ __forceinline__ bool any_w(const DevScene& sc, const DRay& r, f3 inv, f3 oi, Stack& st,
+                                      unsigned long long& nvisit, unsigned long long& ntest) {
+  Best best{r.tmax, 0.0f, 0.0f, -1};
+  if (sc.tri_root >= 0 && traverse<W, true, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest))
+    return true;
+  if (sc.shape_root >= 0 && traverse<W, true, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest))
+    return true;
+  return false;
+}
 
 // Closest hit over both trees (triangles, then shapes), after the root-box
-// test of the accelerator primitive (Primitive/KdTree.hs:240).
-template <bool STATS>
-__device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                            Best& best, unsigned long long& nvisit,
-                                            unsigned long long& ntest) {
+// test of the accelerator primitive (Primitive/KdTree.hs:240), for a scene of
+// width W.
+template <int W, bool STATS>
+__device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
+                                              Best& best, unsigned long long& nvisit,
+                                              unsigned long long& ntest) {
   best.t = r.tmax;
   best.b1 = 0.0f;
   best.b2 = 0.0f;
@@ -170,26 +263,39 @@ __device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, i
   Stack st;
   st.lds = lds;
   st.stride = stride;
-  if (sc.tri_root >= 0) traverse<false, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest);
-  if (sc.shape_root >= 0) traverse<false, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest);
+  closest_w<W, STATS>(sc, r, inv, oi, st, best, nvisit, ntest);
 }
 
-// Any hit (Primitive/KdTree.hs:244-246).
-template <bool STATS>
-__device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                        unsigned long long& nvisit, unsigned long long& ntest) {
+// Any hit (Primitive/KdTree.hs:244-246) for a scene of width W.
+template <int W, bool STATS>
+__device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
+                                          unsigned long long& nvisit, unsigned long long& ntest) {
   if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return false;
   f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
   f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
   Stack st;
   st.lds = lds;
   st.stride = stride;
-  Best best{r.tmax, 0.0f, 0.0f, -1};
-  if (sc.tri_root >= 0 && traverse<true, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest))
-    return true;
-  if (sc.shape_root >= 0 && traverse<true, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest))
-    return true;
-  return false;
+  return any_w<W, STATS>(sc, r, inv, oi, st, nvisit, ntest);
+}
+
+// Width-dispatching forms for kernels whose rays may come from scenes of
+// different widths; the width is uniform per scene, so the branch is uniform
+// per wave where the scene is.
+template <bool STATS>
+__device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
+                                            Best& best, unsigned long long& nvisit,
+                                            unsigned long long& ntest) {
+  if (sc.width == 8) closest_hit_w<8, STATS>(sc, r, lds, stride, best, nvisit, ntest);
+  else if (sc.width == 4) closest_hit_w<4, STATS>(sc, r, lds, stride, best, nvisit, ntest);
+  else closest_hit_w<2, STATS>(sc, r, lds, stride, best, nvisit, ntest);
+}
+template <bool STATS>
+__device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
+                                        unsigned long long& nvisit, unsigned long long& ntest) {
+  if (sc.width == 8) return any_hit_w<8, STATS>(sc, r, lds, stride, nvisit, ntest);
+  if (sc.width == 4) return any_hit_w<4, STATS>(sc, r, lds, stride, nvisit, ntest);
+  return any_hit_w<2, STATS>(sc, r, lds, stride, nvisit, ntest);
 }
 
 }  // namespace trav

@@ -92,6 +92,11 @@ class GpuScene:
             _ck(self.ctx.lib.bgpu_scene_destroy(self.h))
             self.h = None
 
+    def bvh_info(self):
+        b = abi.BvhInfo()
+        _ck(self.ctx.lib.bgpu_scene_bvh_info(self.h, C.byref(b)))
+        return {k: getattr(b, k) for k, _ in abi.BvhInfo._fields_ if k != "pad"}
+
     def device_bytes(self):
         v = C.c_size_t()
         _ck(self.ctx.lib.bgpu_scene_bytes(self.h, C.byref(v)))

@@ -265,7 +265,11 @@ int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out);
 int bgpu_stats_reset(bgpu_ctx* ctx);
 
 /* Tuning knobs (no counterpart in the reference, whose CPU integrator has
- * none): "trace.block" (64/128/256, intersect kernels), "pt.block"
+ * none): "bvh.width" (2/4/8 children per BVH node of scenes created
+ * afterwards; a scene whose worst-case traversal stack would not fit falls
+ * back to a narrower width), "trace.block" (64/128/256, intersect kernels),
+ * "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
+ * ray when theirs is done; 0: one ray per thread), "pt.block"
  * (64/128/256, bounce kernels), "pt.pool" (path slots, >= 1024),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
  * two waves per SIMD).  Changing a pt.* knob is refused with BGPU_E_STATE
@@ -273,6 +277,18 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * next use.  bgpu_config_get reads the current value. */
 int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value);
 int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value);
+
+/* Acceleration structure of a scene (no counterpart in the reference). */
+typedef struct {
+  int32_t width;        /* children per node */
+  int32_t nodes;        /* nodes over both trees */
+  int32_t leaves;
+  int32_t max_depth;    /* of the binary build */
+  int32_t stack_bound;  /* worst-case traversal stack entries */
+  int32_t pad;
+  double build_ms;
+} bgpu_bvh_info;
+int bgpu_scene_bvh_info(const bgpu_scene* scene, bgpu_bvh_info* out);
 
 #ifdef __cplusplus
 }

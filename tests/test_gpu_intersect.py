@@ -167,3 +167,46 @@ def test_closed_form_cases_on_gpu(gpu_ctx, case):
             assert (b1[0], b2[0]) == (np.float32(exp["b1"]), np.float32(exp["b2"]))
     if "any" in case:
         assert occ[0] == case["any"]
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_bvh_widths_bit_exact(gpu_ctx, width, persistent):
+    """Every BVH width and both traversal modes (one ray per thread; lanes
+    refilled with new rays) give the oracle's results bit for bit (the
+    structure changes the work, never the answer): a mixed triangle + shape
+    scene under incoherent rays with random intervals, and a displaced
+    icosphere with coherent and incoherent rays."""
+    prev = gpu_ctx.config_get("bvh.width"), gpu_ctx.config_get("trace.persistent")
+    gpu_ctx.config_set("bvh.width", width)
+    gpu_ctx.config_set("trace.persistent", persistent)
+    try:
+        flat = scenes.small_scene(num_tris=4000, num_spheres=500, seed=21)
+        lo, hi = scenes.scene_bounds(flat)
+        rays = scenes.incoherent_rays(lo - 0.3, hi + 0.3, 50000, seed=6)
+        rng = np.random.default_rng(width)
+        rays[:, 6] = rng.uniform(0.0, 0.2, len(rays)).astype(np.float32)
+        rays[:, 7] = np.where(rng.random(len(rays)) < 0.5, np.inf, rng.uniform(0.5, 3.0, len(rays))).astype(np.float32)
+        gs = gpu_ctx.scene(flat)
+        info = gs.bvh_info()
+        g, gocc = gs.intersect(rays), gs.intersects(rays)
+        gs.close()
+        assert info["width"] == width and info["stack_bound"] <= 64, info
+        o = OracleScene(flat)
+        assert_same(g, o.intersect(rays, brute=True), "small/w%d" % width)
+        assert (gocc == o.intersects(rays, brute=True)).all()
+        o.close()
+        ico = scenes.icosphere_scene(levels=5, num_spheres=300, seed=3, res=128)
+        lo, hi = scenes.scene_bounds(ico)
+        batches = [scenes.primary_rays(ico), scenes.incoherent_rays(lo, hi, 30000, seed=8)]
+        gs = gpu_ctx.scene(ico)
+        res = [(gs.intersect(b), gs.intersects(b)) for b in batches]
+        gs.close()
+        o = OracleScene(ico)
+        for (g, gocc), b, name in zip(res, batches, ("coherent", "incoherent")):
+            assert_same(g, o.intersect(b), "ico/%s/w%d" % (name, width))
+            assert (gocc == o.intersects(b)).all()
+        o.close()
+    finally:
+        gpu_ctx.config_set("bvh.width", prev[0])
+        gpu_ctx.config_set("trace.persistent", prev[1])
