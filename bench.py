@@ -37,10 +37,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s and p50/p99 job latency, served vs plain GPU path at equal RMSE"
 PEAK_HBM_GBS = 8000.0
-# algorithmic bytes per unit of work (BVH2 node = 2 child boxes of 24 B + 2 links
-# of 4 B; triangle = 9 floats)
+# algorithmic bytes per unit of work: a ray is 8 floats in, a closest hit
+# 4 words out, an any-hit 1 byte out; a node of width w is w child boxes of
+# 24 B plus w links of 4 B (28 w B); a triangle test reads 9 floats
 B_RAY_IN, B_HIT_OUT, B_OCC_OUT = 32, 16, 1
-B_NODE, B_TRI = 56, 36
+B_NODE_PER_CHILD, B_TRI = 28, 36
 CORNELL_REF = os.path.join(ROOT, "tests", "golden", "cornell256_4096spp.npz")
 
 
@@ -181,10 +182,12 @@ def intersect_leg(ctx, args, out):
     p_c = dbat["incoherent"][1][3].cpu().numpy()
     shape_frac = float((p_c >= flat.num_triangles).mean())
 
+    b_node = B_NODE_PER_CHILD * gs.bvh_info()["width"]
+
     def alg_bytes(label):
         nodes, prims = work[label]
         o = B_HIT_OUT if label.startswith("closest") else B_OCC_OUT
-        return nr * (B_RAY_IN + o) + nodes * B_NODE + prims * B_TRI
+        return nr * (B_RAY_IN + o) + nodes * b_node + prims * B_TRI
 
     if args.copy_ref:
         # an elementwise kernel that reads exactly 1 GiB and writes exactly 1 GiB

@@ -9,6 +9,10 @@
                      seed 1) plus 10,000 spheres with radii log-uniform in
                      [0.001, 0.05]; coherent (pinhole primary) and incoherent
                      (uniform origin in the bounds, uniform direction) rays.
+  stack_scene()      random axis-aligned box stacks (12 triangles per box) on a
+                     floor, lit by a quad and a small sphere light.
+  grid_scene()       a displaced heightmap grid (Heightmap.hs) with shading
+                     normals, lit the same way.
 
 The meshes go through SceneBuilder.mesh, i.e. the same transPoint as the
 reference's mkTriangleMesh.
