@@ -70,8 +70,8 @@ def main():
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy, the reference for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--legs", default="intersect,render,serve",
-                    help="comma-separated subset of intersect,render,serve")
+    ap.add_argument("--legs", default="intersect,render,divergent,serve",
+                    help="comma-separated subset of intersect,render,divergent,serve")
     ap.add_argument("--render-steps", type=int, default=10)
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs per arrival rate")
     ap.add_argument("--serve-rates", default="5,50,200", help="Poisson arrival rates, jobs/s")
@@ -117,6 +117,8 @@ def main():
         intersect_leg(ctx, args, out)
     if "render" in legs:
         out["config"]["render"] = render_leg(ctx, args)
+    if "divergent" in legs:
+        out["config"]["divergent"] = divergent_leg(ctx, args)
     if "serve" in legs:
         out["config"]["serve"] = serve_leg(ctx, args)
     ctx.close()
@@ -346,6 +348,56 @@ def render_cpu(flat, mk, ref):
     return {"msamples_s": round(n / dt / 1e6, 4), "mrays_s": round(rays / dt / 1e6, 3),
             "cores": threads, "kind": "port", "seconds": round(dt, 2),
             "rmse_y_vs_4096spp": round(float(np.sqrt(np.mean((luminance(img) - ref) ** 2))), 6)}
+
+
+# --------------------------------------------------------------- divergent
+def divergent_leg(ctx, args):
+    """Divergent materials: 1024^2 heightmap + 20 tori of 20k triangles +
+    200 spheres (2,493,058 triangles), matte/plastic/metal/glass, a 512x256
+    sky map; 1024x1024, 64 spp (8x8), max depth 16, RR after depth 3."""
+    from bling_amd import scenes
+    from bling_amd.runtime import GpuRenderer, JobSpec
+    t0 = time.time()
+    flat = scenes.divergent_scene()
+    gen_s = time.time() - t0
+    t0 = time.time()
+    gs = ctx.scene(flat)
+    build_s = time.time() - t0
+    spec = JobSpec(spp_nu=8, spp_nv=8, max_depth=16, sample_depth=3, rr_start_depth=3, passes=1, seed=5)
+    GpuRenderer(JobSpec(spp_nu=1, spp_nv=1, max_depth=16, sample_depth=3, rr_start_depth=3,
+                        seed=6, view=flat.view(256, 256))).render_plain(gs)
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    img, _ = GpuRenderer(spec).render_plain(gs)
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    res = {
+        "workload": "divergent materials: %d triangles + %d spheres, 1024x1024, 8x8 spp, max depth 16, "
+                    "RR after depth 3, sky map 512x256, plain path" % (flat.num_triangles, flat.num_shapes),
+        "scene_gen_s": round(gen_s, 2), "bvh_build_s": round(build_s, 2), "bvh": gs.bvh_info(),
+        "ms_per_render": round(wall * 1e3, 1),
+        "msamples_s": round(st["samples"] / wall / 1e6, 2),
+        "mrays_s": round(rays_of(st) / wall / 1e6, 2),
+        "rays_per_sample": round(rays_of(st) / max(1, st["samples"]), 3),
+        "image_finite": bool(np.isfinite(img).all()),
+    }
+    gs.close()
+    if not args.no_cpu:
+        from oracle import pyoracle
+        threads = host_threads()
+        small = JobSpec(spp_nu=2, spp_nv=2, max_depth=16, sample_depth=3, rr_start_depth=3, seed=5,
+                        view=flat.view(96, 96))
+        osc = pyoracle.OracleScene(flat)  # its build is not timed
+        t0 = time.perf_counter()
+        _, _, rays = pyoracle.render(flat, small, threads=threads, scene=osc)
+        dt = time.perf_counter() - t0
+        osc.close()
+        n = 97 * 97 * 4
+        res["cpu"] = {"msamples_s": round(n / dt / 1e6, 4), "mrays_s": round(rays / dt / 1e6, 3),
+                      "cores": threads, "kind": "port", "seconds": round(dt, 2),
+                      "sample": "the same scene at 96x96, 2x2 spp (37,636 camera samples)"}
+    log("divergent:", json.dumps(res))
+    return res
 
 
 # ------------------------------------------------------------------- serve

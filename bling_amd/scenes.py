@@ -266,19 +266,14 @@ def stack_scene(num_tris, res=256, seed=0):
     return sb.build()
 
 
-def grid_scene(num_tris, res=256, seed=0):
-    """A displaced heightmap grid (fBm heights) with shading normals, lit."""
-    n = max(2, int(np.sqrt(num_tris / 2.0)) + 1)
-    noise_seed = seed + 1
-    sb = SceneBuilder()
-    _lit_room(sb, 2.0, height=1.0)
-    m = sb.material(Material.matte(rgb_refl(0.5, 0.6, 0.4)))
-    # fBm sampled once on a lattice of at most 513^2 points, interpolated
-    # bilinearly (the heightmap evaluates elev five times per vertex)
+def fbm_elevation(n, seed, scale=0.25, freq=4.0):
+    """elev(x, z) on [0,1]^2: scale * fBm(freq * (x, z, 0)) (6 octaves), sampled
+    once on a lattice of at most 513^2 points and interpolated bilinearly (the
+    heightmap evaluates elev five times per vertex)."""
     L = min(n, 512) + 1
     lx, lz = np.meshgrid(np.arange(L) / (L - 1.0), np.arange(L) / (L - 1.0))
-    lat = 0.25 * fbm(np.stack([lx.reshape(-1) * 4.0, lz.reshape(-1) * 4.0, np.zeros(L * L)], 1),
-                     octaves=6, omega=0.5, seed=noise_seed).reshape(L, L)
+    lat = scale * fbm(np.stack([lx.reshape(-1) * freq, lz.reshape(-1) * freq, np.zeros(L * L)], 1),
+                      octaves=6, omega=0.5, seed=seed).reshape(L, L)
 
     def elev(x, z):
         u = np.clip(x, 0.0, 1.0) * (L - 1)
@@ -288,11 +283,88 @@ def grid_scene(num_tris, res=256, seed=0):
         fu, fw = u - i, w - k
         return ((1 - fu) * (1 - fw) * lat[k, i] + fu * (1 - fw) * lat[k, i + 1]
                 + (1 - fu) * fw * lat[k + 1, i] + fu * fw * lat[k + 1, i + 1])
+    return elev
 
-    v, f, nrm, uv = heightmap_mesh(elev, n, n)
+
+def grid_scene(num_tris, res=256, seed=0):
+    """A displaced heightmap grid (fBm heights) with shading normals, lit."""
+    n = max(2, int(np.sqrt(num_tris / 2.0)) + 1)
+    sb = SceneBuilder()
+    _lit_room(sb, 2.0, height=1.0)
+    m = sb.material(Material.matte(rgb_refl(0.5, 0.6, 0.4)))
+    v, f, nrm, uv = heightmap_mesh(fbm_elevation(n, seed + 1), n, n)
     o2w = T.scale((3.0, 1.0, 3.0)) @ T.translate((-1.5, 0.1, -1.5))
     sb.mesh(v, f, o2w=o2w, material=m, normals=nrm, uvs=uv)
     c2w = T.look_at((0.0, 2.5, -3.5), (0.0, 0.2, 0.0), (0, 1, 0))
+    sb.look(c2w, 50.0, res, res)
+    return sb.build()
+
+
+def torus(major, minor, nu, nv):
+    """A torus around the y axis tessellated into 2 * nu * nv triangles."""
+    u = 2 * np.pi * np.arange(nu) / nu
+    v = 2 * np.pi * np.arange(nv) / nv
+    U, V = np.meshgrid(u, v, indexing="ij")
+    ring = major + minor * np.cos(V)
+    verts = np.stack([ring * np.cos(U), minor * np.sin(V), ring * np.sin(U)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    faces = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], 2).reshape(-1, 3)
+    return verts.astype(F), faces.astype(np.int32)
+
+
+def sky_gradient(w=512, h=256):
+    """A w x h spectral sky map: a bluish zenith over a pale horizon and a
+    dark ground, rows from the +z pole (v = 0) down."""
+    zen, hor, gnd = rgb_illum(0.3, 0.5, 1.0), rgb_illum(1.0, 0.95, 0.85), rgb_illum(0.15, 0.12, 0.1)
+    zen, hor, gnd = np.asarray(zen, F), np.asarray(hor, F), np.asarray(gnd, F)
+    v = (np.arange(h, dtype=np.float64) + 0.5) / h
+    rows = np.where((v < 0.5)[:, None], zen[None] + (hor - zen)[None] * (v / 0.5)[:, None] ** 0.7,
+                    gnd[None] + (hor - gnd)[None] * np.exp(-(v - 0.5) * 12.0)[:, None])
+    return np.ascontiguousarray(np.broadcast_to(rows[:, None, :], (h, w, 16)).astype(F))
+
+
+def divergent_scene(res=1024, hm=1024, n_tori=20, torus_tris=20000, n_spheres=200, sky=(512, 256),
+                    seed=4):
+    """Divergent materials: a fBm heightmap of hm x hm vertices, n_tori tori of
+    torus_tris triangles and n_spheres spheres, each object with a material
+    drawn uniformly from matte, plastic (Blinn exponent 10..1000), metal
+    (roughness 0..0.3) and glass (IOR 1.5), lit only by a sky-gradient
+    infinite light."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+
+    def pick():
+        k = rng.integers(4)
+        if k == 0:
+            return sb.material(Material.matte(rgb_refl(*rng.uniform(0.2, 0.9, 3))))
+        if k == 1:
+            return sb.material(Material.plastic(rgb_refl(*rng.uniform(0.1, 0.8, 3)), const(0.3),
+                                                1.0 / rng.uniform(10.0, 1000.0)))
+        if k == 2:
+            return sb.material(Material.metal(const(rng.uniform(0.2, 1.5)), const(rng.uniform(2.0, 4.0)),
+                                              rng.uniform(0.0, 0.3)))
+        return sb.material(Material.glass(1.5, const(1.0), const(1.0)))
+
+    terrain = sb.material(Material.matte(rgb_refl(0.45, 0.5, 0.35)))
+    v, f, nrm, uv = heightmap_mesh(fbm_elevation(hm, seed + 11, scale=0.06), hm, hm)
+    o2w = T.scale((8.0, 8.0, 8.0)) @ T.translate((-4.0, 0.0, -4.0))
+    sb.mesh(v, f, o2w=o2w, material=terrain, normals=nrm, uvs=uv)
+    nside = max(3, int(np.sqrt(torus_tris / 2.0)))
+    tv, tf = torus(0.35, 0.1, nside, max(3, torus_tris // (2 * nside)))
+    for _ in range(n_tori):
+        o = (T.rotate_x(rng.uniform(0, 360)) @ T.rotate_y(rng.uniform(0, 360))
+             @ T.translate((rng.uniform(-3, 3), rng.uniform(0.6, 1.6), rng.uniform(-3, 3))))
+        sb.mesh(tv, tf, o2w=o, material=pick())
+    for _ in range(n_spheres):
+        r = float(np.exp(rng.uniform(np.log(0.05), np.log(0.3))))
+        c = (rng.uniform(-3.5, 3.5), rng.uniform(0.3, 2.0), rng.uniform(-3.5, 3.5))
+        sb.shape(abi.SHAPE_SPHERE, (r,), o2w=T.translate(c), material=pick())
+    sb.infinite_light(sky_gradient(*sky), w2l=T.rotate_x(90))
+    c2w = T.look_at((0.0, 2.6, -5.5), (0.0, 0.6, 0.0), (0, 1, 0))
     sb.look(c2w, 50.0, res, res)
     return sb.build()
 

@@ -95,11 +95,12 @@ class OracleScene:
         return occ.astype(bool)
 
 
-def render(flat, spec, threads=None, passes=None):
+def render(flat, spec, threads=None, passes=None, scene=None):
     """Oracle render of a whole job: returns ((h, w, 4) wxyz, (h, w, 3) sum of
-    squared XYZ, ray queries traced)."""
+    squared XYZ, ray queries traced).  `scene`: an OracleScene of `flat` to
+    reuse (its build is then not repeated)."""
     lib = load()
-    o = OracleScene(flat)
+    o = scene if scene is not None else OracleScene(flat)
     h, w = spec.image_shape(flat)
     img = np.zeros((h, w, 4), np.float32)
     sq = np.zeros((h, w, 3), np.float32)
@@ -110,5 +111,6 @@ def render(flat, spec, threads=None, passes=None):
                                C.POINTER(C.c_uint64), C.c_int, C.c_int]
     lib.orc_render(o.h, C.addressof(d), img.ctypes.data, sq.ctypes.data,
                    threads or os.cpu_count(), C.byref(rays), p0, p1)
-    o.close()
+    if scene is None:
+        o.close()
     return img, sq, rays.value

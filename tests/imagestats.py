@@ -4,11 +4,18 @@ in exactly one pixel, up to measure-zero boundary samples)."""
 import numpy as np
 
 
+# Film sums are float32: E[x^2] - E[x]^2 cannot resolve a variance below a
+# few dozen ulps of E[x^2] (pixels whose samples are nearly equal, e.g. sky
+# seen directly), so the variance is floored there.
+VAR_FLOOR_ULPS = 64.0 * np.finfo(np.float32).eps
+
+
 def mean_se(img, sq):
     w = img[..., 0:1]
     n = np.maximum(w, 1.0)
     mean = img[..., 1:] / n
-    var = np.maximum(sq / n - mean * mean, 0.0)
+    ex2 = sq / n
+    var = np.maximum(ex2 - mean * mean, VAR_FLOOR_ULPS * ex2)
     se = np.sqrt(var / np.maximum(n - 1.0, 1.0))
     return mean, se
 

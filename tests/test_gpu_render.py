@@ -236,3 +236,55 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     finally:
         for k, v in defaults.items():
             gpu_ctx.config_set(k, v)
+
+
+def test_divergent_materials_matches_oracle(gpu_ctx):
+    """The divergent-materials workload at reduced size (heightmap, tori,
+    spheres of matte/plastic/metal/glass under a sky map, max depth 16):
+    independent seeds agree within 3 sigma per pixel, the same seed tracks
+    the oracle sample for sample."""
+    flat = scenes.divergent_scene(res=40, hm=32, n_tori=4, torus_tris=800, n_spheres=30)
+    mk = lambda seed: JobSpec(spp_nu=4, spp_nv=4, max_depth=16, sample_depth=3, rr_start_depth=3,
+                              passes=4, seed=seed, variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, mk(31))
+    assert np.isfinite(g).all()
+    o, osq, _ = pyoracle.render(flat, mk(32))
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    assert abs(zmean) < 0.08, zmean
+    o1, o1sq, _ = pyoracle.render(flat, mk(31))
+    assert np.array_equal(g[..., 0], o1[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o1, o1sq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+
+
+def test_batch_of_64_jobs(gpu_ctx):
+    """64 jobs over 8 resident scenes of different sizes, resolutions and spp
+    submitted at once: every job's image in the mix equals its image alone."""
+    rng = np.random.default_rng(64)
+    flats = [scenes.stack_scene(int(n), seed=k) if k % 2 == 0 else scenes.grid_scene(int(n), seed=k)
+             for k, n in enumerate(np.exp(rng.uniform(np.log(1e3), np.log(5e4), 8)))]
+    gss = [gpu_ctx.scene(f) for f in flats]
+    try:
+        jobs = []
+        for k in range(64):
+            si = int(rng.integers(8))
+            res = int(rng.choice([16, 24, 32, 48]))
+            spp = [(1, 1), (2, 1), (2, 2), (4, 2)][int(rng.integers(4))]
+            spec = JobSpec(spp_nu=spp[0], spp_nv=spp[1], max_depth=5, sample_depth=3, rr_start_depth=3,
+                           passes=1, seed=1000 + k, view=flats[si].view(res, res))
+            jobs.append((si, spec))
+        handles = [Job(gss[si], spec) for si, spec in jobs]
+        for (si, spec), h in zip(jobs, handles):
+            h.wait(timeout=120)
+            img, _, done, state = h.poll()
+            h.release()
+            alone, _ = GpuRenderer(spec).render_plain(gss[si])
+            assert state == abi.JOB_DONE and img.shape == alone.shape
+            assert np.array_equal(img[..., 0], alone[..., 0])
+            np.testing.assert_allclose(img, alone, rtol=1e-4, atol=1e-5)
+    finally:
+        for gs in gss:
+            gs.close()
