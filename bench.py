@@ -162,8 +162,11 @@ def intersect_leg(ctx, args, out):
     for name, (cols, outs, occ) in dbat.items():
         rp = [c.data_ptr() for c in cols]
         hp = [o.data_ptr() for o in outs]
-        launches.append(("closest/" + name, lambda rp=rp, hp=hp, **kw: gs.intersect_dev(rp, nr, hp, **kw)))
-        launches.append(("any/" + name, lambda rp=rp, op=occ.data_ptr(), **kw: gs.intersects_dev(rp, nr, op, **kw)))
+        coh = name == "coherent"  # camera rays: the caller's BGPU_COHERENT hint
+        launches.append(("closest/" + name,
+                         lambda rp=rp, hp=hp, coh=coh, **kw: gs.intersect_dev(rp, nr, hp, coherent=coh, **kw)))
+        launches.append(("any/" + name,
+                         lambda rp=rp, op=occ.data_ptr(), coh=coh, **kw: gs.intersects_dev(rp, nr, op, coherent=coh, **kw)))
 
     # counting pass: node visits and primitive tests per launch (not timed)
     work = {}

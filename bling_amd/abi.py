@@ -21,7 +21,7 @@ MESH_NORMALS, MESH_UVS = 1, 2
 LIGHT_INFINITE = 1
 CAMERA_PERSPECTIVE, CAMERA_ENVIRONMENT = 0, 1
 FILTER_BOX, FILTER_GAUSS, FILTER_MITCHELL, FILTER_SINC, FILTER_TRIANGLE = range(5)
-DEVICE_PTRS, TIMED, COUNT = 1, 2, 4
+DEVICE_PTRS, TIMED, COUNT, COHERENT = 1, 2, 4, 8
 JOB_VARIANCE = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = range(5)
 

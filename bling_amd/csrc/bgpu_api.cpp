@@ -214,7 +214,9 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, b
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
   if (!c->work_buf.ptr)
     if (int rc = c->work_buf.alloc(c, 256)) return rc;
-  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, c->trace_cfg, c->stream, cnt, c->work_buf.as<unsigned int>()));
+  TraceConfig tc = c->trace_cfg;
+  if (flags & BGPU_COHERENT) tc.persistent = 0;
+  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, tc, c->stream, cnt, c->work_buf.as<unsigned int>()));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) {
     BGPU_HIP(hipMemcpyAsync(hits->t, dh.t, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -256,7 +258,9 @@ int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n, uint8
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
   if (!c->work_buf.ptr)
     if (int rc = c->work_buf.alloc(c, 256)) return rc;
-  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, c->trace_cfg, c->stream, cnt, c->work_buf.as<unsigned int>()));
+  TraceConfig tc = c->trace_cfg;
+  if (flags & BGPU_COHERENT) tc.persistent = 0;
+  BGPU_HIP(launch_any(scene->dev, dr, (int)n, docc, tc, c->stream, cnt, c->work_buf.as<unsigned int>()));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   if (!dev) BGPU_HIP(hipMemcpyAsync(occluded, docc, n, hipMemcpyDeviceToHost, c->stream));
   BGPU_HIP(hipStreamSynchronize(c->stream));

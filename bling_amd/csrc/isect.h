@@ -8,23 +8,24 @@
 
 namespace bgpu {
 
-// BVH2 node, 64 B (one fetch tests both children):
-//   a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
-//   b = (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x)
-//   c = (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z)
-//   d = (link0, link1, 0, 0)  link >= 0: inner node; link < 0: leaf ~((first<<4)|count)
+// BVH node of width W (2, 4 or 8), 8 * W floats (W = 4: one 128-byte line):
+//   lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W] link[W] pad[W]
+//   link >= 0: inner node; link < 0: leaf ~((first<<4)|count); kNoChild: unused
 // Leaf records, 48 B each, in leaf order:
 //   r0 = (p1.xyz, prim id bits), r1 = (e1.xyz, kind bits), r2 = (e2.xyz, 0)
 //   kind >= 0: triangle (e1 = p2-p1, e2 = p3-p1); kind < 0: shape index ~kind.
 //
 // Triangles and analytic shapes live in two BVHs sharing these arrays: the
 // triangle tree is rooted at node tri_root, the shape tree at shape_root (-1
-// when absent).  Shape-tree boxes are tested inflated by
-// shape_pad_k * (distance from the ray origin to the box + box diagonal):
-// the reference's binary32 sphere/cylinder quadratic (Shape.hs:113-187)
-// evaluates c = |o|^2 - r^2 with cancellation and so reports hits up to about
-// 3*sqrt(2^-24)*|o| outside the true surface; a box padded only by rounding
-// would cull those hits, which the reference does report.
+// when absent).  The reference's binary32 sphere/cylinder quadratic
+// (Shape.hs:113-187) evaluates c = |o|^2 - r^2 with cancellation and so
+// reports hits up to about 3*sqrt(2^-24)*|o - centre| outside the true
+// surface (|.| in object space); a box padded only by rounding would cull
+// those hits, which the reference does report.  The children of a
+// shape-tree node are therefore tested grown by shape_pad_k times the L1
+// distance from the ray origin to the farthest corner of the node's box,
+// which bounds |o - centre| for every shape below it (shape_pad_k includes
+// the anisotropy of the shapes' transforms).
 // Traversal stack entries available per lane (LDS part plus private spill);
 // scene creation checks the tree's worst case against it.
 constexpr int kStackCapacity = 64;

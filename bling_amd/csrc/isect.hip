@@ -194,6 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(DevScene sc, DevRays rays, in
       }
       if (STATS) ++nvisit;
       const float tlim = ANY ? r.tmax : best.t;
+      const float pad = FAT ? trav::node_margin<W>(v, r.o, sc.shape_pad_k) : 0.0f;
       float key[W];
       int lk[W];
       int nhit = 0;
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(DevScene sc, DevRays rays, in
         float lx = v[c], hx = v[W + c], ly = v[2 * W + c], hy = v[3 * W + c], lz = v[4 * W + c],
               hz = v[5 * W + c];
         lk[c] = __float_as_int(v[6 * W + c]);
-        if (FAT) trav::inflate(lx, hx, ly, hy, lz, hz, r.o, sc.shape_pad_k);
+        if (FAT) trav::inflate(lx, hx, ly, hy, lz, hz, pad);
         bool h;
         float tn;
         trav::slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h, &tn);

@@ -212,7 +212,8 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   sc->build.nodes = (int)nodes.size();
 
   // node layout, 8 * width floats: lo.x[w] hi.x[w] lo.y[w] hi.y[w] lo.z[w]
-  // hi.z[w] link[w] pad[w]; unused slots carry the link kNoChild
+  // hi.z[w] link[w] pad[w]; unused slots carry the link kNoChild and repeat
+  // child 0's box (so that the union of a node's child boxes is its box)
   const int W = width;
   std::vector<float> hn((size_t)8 * W * std::max<size_t>(nodes.size(), 1), 0.0f);
   for (size_t i = 0; i < nodes.size(); ++i) {
@@ -220,9 +221,10 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     float* o = &hn[(size_t)8 * W * i];
     for (int c = 0; c < W; ++c) {
       const bool used = c < (int)n.link.size();
+      const Box& b = n.child[used ? c : 0];
       for (int k = 0; k < 3; ++k) {
-        o[(2 * k) * W + c] = used ? n.child[c].lo[k] : 0.0f;
-        o[(2 * k + 1) * W + c] = used ? n.child[c].hi[k] : 0.0f;
+        o[(2 * k) * W + c] = b.lo[k];
+        o[(2 * k + 1) * W + c] = b.hi[k];
       }
       o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
     }

@@ -86,13 +86,27 @@ __device__ __forceinline__ void slab(float lx, float hx, float ly, float hy, flo
   *tn = n;
 }
 
-// Grows a box by k * (L1 distance from o to the box + L1 diagonal).
+// Margin of the children of a shape-tree node for a ray from o: shape_pad_k
+// times the L1 distance from o to the farthest corner of the node's box (the
+// union of its children), an upper bound of |o - centre| for every shape
+// below the node.  Unused child slots repeat child 0's box, so the union is
+// exact.
+template <int W>
+__device__ __forceinline__ float node_margin(const float* v, f3 o, float k) {
+  float lx = v[0], hx = v[W], ly = v[2 * W], hy = v[3 * W], lz = v[4 * W], hz = v[5 * W];
+#pragma unroll
+  for (int c = 1; c < W; ++c) {
+    lx = fminf(lx, v[c]); hx = fmaxf(hx, v[W + c]);
+    ly = fminf(ly, v[2 * W + c]); hy = fmaxf(hy, v[3 * W + c]);
+    lz = fminf(lz, v[4 * W + c]); hz = fmaxf(hz, v[5 * W + c]);
+  }
+  const float fx = fmaxf(fabsf(o.x - lx), fabsf(o.x - hx));
+  const float fy = fmaxf(fabsf(o.y - ly), fabsf(o.y - hy));
+  const float fz = fmaxf(fabsf(o.z - lz), fabsf(o.z - hz));
+  return k * (fx + fy + fz);
+}
 __device__ __forceinline__ void inflate(float& lx, float& hx, float& ly, float& hy, float& lz,
-                                        float& hz, f3 o, float k) {
-  float dist = fmaxf(fmaxf(lx - o.x, o.x - hx), 0.0f) + fmaxf(fmaxf(ly - o.y, o.y - hy), 0.0f) +
-               fmaxf(fmaxf(lz - o.z, o.z - hz), 0.0f);
-  float diag = (hx - lx) + (hy - ly) + (hz - lz);
-  float p = k * (dist + diag);
+                                        float& hz, float p) {
   lx -= p; ly -= p; lz -= p;
   hx += p; hy += p; hz += p;
 }
@@ -149,6 +163,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
       }
       if (STATS) ++nvisit;
       const float tlim = ANY ? r.tmax : best.t;
+      const float pad = FAT ? node_margin<W>(v, r.o, sc.shape_pad_k) : 0.0f;
       float key[W];
       int lk[W];
       int nhit = 0;
@@ -157,7 +172,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
         float lx = v[c], hx = v[W + c], ly = v[2 * W + c], hy = v[3 * W + c], lz = v[4 * W + c],
               hz = v[5 * W + c];
         lk[c] = __float_as_int(v[6 * W + c]);
-        if (FAT) inflate(lx, hx, ly, hy, lz, hz, r.o, sc.shape_pad_k);
+        if (FAT) inflate(lx, hx, ly, hy, lz, hz, pad);
         bool h;
         float tn;
         slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h, &tn);

@@ -128,16 +128,16 @@ class GpuScene:
         return occ.astype(bool)
 
     # --- device-resident arrays (integer device addresses) -------------
-    def intersect_dev(self, ray_ptrs, n, hit_ptrs, timed=False):
+    def intersect_dev(self, ray_ptrs, n, hit_ptrs, timed=False, coherent=False):
         r = _rays_struct(ray_ptrs)
         h = abi.Hits(*hit_ptrs)
-        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0)
+        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0) | (abi.COHERENT if coherent else 0)
         _ck(self.ctx.lib.bgpu_intersect_closest(self.h, C.byref(r), n, C.byref(h), flags))
         return self.ctx.last_kernel_ms() if timed else None
 
-    def intersects_dev(self, ray_ptrs, n, occ_ptr, timed=False):
+    def intersects_dev(self, ray_ptrs, n, occ_ptr, timed=False, coherent=False):
         r = _rays_struct(ray_ptrs)
-        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0)
+        flags = abi.DEVICE_PTRS | (abi.TIMED if timed else 0) | (abi.COHERENT if coherent else 0)
         _ck(self.ctx.lib.bgpu_intersect_any(self.h, C.byref(r), n, occ_ptr, flags))
         return self.ctx.last_kernel_ms() if timed else None
 

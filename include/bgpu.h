@@ -183,8 +183,11 @@ typedef struct {
 enum {
   BGPU_DEVICE_PTRS = 1, /* ray/hit arrays are device pointers */
   BGPU_TIMED = 2,       /* record the kernel time, read by bgpu_last_kernel_ms */
-  BGPU_COUNT = 4        /* run the counting variant: adds node visits and
+  BGPU_COUNT = 4,       /* run the counting variant: adds node visits and
                            primitive tests to bgpu_stats (not for timing) */
+  BGPU_COHERENT = 8     /* hint: neighbouring rays take similar paths (camera
+                           rays); traced one ray per thread instead of with
+                           lane refill.  Results are identical either way. */
 };
 
 /* ---- jobs --------------------------------------------------------------- */

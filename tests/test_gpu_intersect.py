@@ -210,3 +210,29 @@ def test_bvh_widths_bit_exact(gpu_ctx, width, persistent):
     finally:
         gpu_ctx.config_set("bvh.width", prev[0])
         gpu_ctx.config_set("trace.persistent", prev[1])
+
+
+def test_coherent_hint_same_results(gpu_ctx):
+    """BGPU_COHERENT only chooses how lanes take rays: the results equal the
+    default launch bit for bit."""
+    import ctypes as C
+    flat = scenes.small_scene(num_tris=2000, num_spheres=200, seed=9)
+    gs = gpu_ctx.scene(flat)
+    try:
+        for rays in (scenes.primary_rays(flat, res=96),
+                     scenes.incoherent_rays(*scenes.scene_bounds(flat), 20000, seed=3)):
+            base = gs.intersect(rays)
+            occ = gs.intersects(rays)
+            n = len(rays)
+            cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
+            out = [np.empty(n, np.float32) for _ in range(3)] + [np.empty(n, np.int32)]
+            r = abi.Rays(*[c.ctypes.data for c in cols])
+            h = abi.Hits(*[o.ctypes.data for o in out])
+            abi.check(gpu_ctx.lib.bgpu_intersect_closest(gs.h, C.byref(r), n, C.byref(h), abi.COHERENT))
+            occ2 = np.empty(n, np.uint8)
+            abi.check(gpu_ctx.lib.bgpu_intersect_any(gs.h, C.byref(r), n, occ2.ctypes.data, abi.COHERENT))
+            for a, b in zip(base, out):
+                assert np.array_equal(a.view(np.int32), b.view(np.int32))
+            assert np.array_equal(occ, occ2.astype(bool))
+    finally:
+        gs.close()
