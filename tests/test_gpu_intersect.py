@@ -216,10 +216,10 @@ def test_coherent_hint_same_results(gpu_ctx):
     """BGPU_COHERENT only chooses how lanes take rays: the results equal the
     default launch bit for bit."""
     import ctypes as C
-    flat = scenes.small_scene(num_tris=2000, num_spheres=200, seed=9)
+    flat = scenes.icosphere_scene(levels=4, num_spheres=300, seed=9, res=96)
     gs = gpu_ctx.scene(flat)
     try:
-        for rays in (scenes.primary_rays(flat, res=96),
+        for rays in (scenes.primary_rays(flat),
                      scenes.incoherent_rays(*scenes.scene_bounds(flat), 20000, seed=3)):
             base = gs.intersect(rays)
             occ = gs.intersects(rays)
