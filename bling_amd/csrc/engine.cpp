@@ -86,6 +86,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   int P = 0;
   PtConfig cfg;
+  int width = 0;  // BVH width the captured traversal kernels assume (0: any)
   DevBuf pool_buf, jobs_buf, ring_buf, scenes_buf, cnt_buf, admit_buf, prog_buf;
   PtKernelArgs args{};
   AdmitReq* h_admit = nullptr;
@@ -183,14 +184,25 @@ struct Engine {
     for (int i = kMaxJobs - 1; i >= 0; --i) free_slots.push_back(i);
     BGPU_HIP(hipMemsetAsync(prog_buf.ptr, 0, sizeof(uint32_t) * kSkipWords, stream));
     BGPU_HIP(pt_enqueue_init(args, stream));
-    // capture one bounce iteration
+    if (int rc = capture(c->bvh_width)) return rc;
+    BGPU_HIP(hipStreamSynchronize(stream));
+    return BGPU_OK;
+  }
+
+  // Captures one bounce iteration whose traversal kernels are specialised
+  // for BVH width w (0: any width).
+  int capture(int w) {
+    if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
+    if (graph) BGPU_HIP(hipGraphDestroy(graph));
+    gexec = nullptr;
+    graph = nullptr;
     BGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = pt_enqueue_iteration(args, cfg, stream);
+    hipError_t e = pt_enqueue_iteration(args, cfg, w, stream);
     hipError_t e2 = hipStreamEndCapture(stream, &graph);
     if (e != hipSuccess) return hip_fail(e, "pt_enqueue_iteration (capture)");
     if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
     BGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    BGPU_HIP(hipStreamSynchronize(stream));
+    width = w;
     return BGPU_OK;
   }
 
@@ -271,6 +283,12 @@ struct Engine {
   // the caller retires the job, which returns what was taken.
   int admit(HostJob* j) {
     if (!can_admit(j)) return fail(BGPU_E_BUDGET, "job table or progress counters full");
+    // a scene of another width than the captured kernels assume: fall back
+    // to kernels that take any width (the stream is idle between iterations)
+    if (width != 0 && j->scene->dev.width != width) {
+      BGPU_HIP(hipStreamSynchronize(stream));
+      if (int rc = capture(0)) return rc;
+    }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
     const bgpu_job_desc& d = j->desc;

@@ -145,8 +145,9 @@ struct PtScene {
 int make_view(const bgpu_camera& cam, const bgpu_film& film, DevView* v);  // 0 or BGPU_E_INVALID
 
 // Enqueues one bounce iteration (reset, generate, extend, shade, shadow,
-// MIS, finish) on `s`.  Capturable into a hipGraph.
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, hipStream_t s);
+// MIS, finish) on `s`.  Capturable into a hipGraph.  `width`: the BVH width
+// of every scene the iteration will see (2, 4 or 8), or 0 for any width.
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s);
 // Enqueues the admission of jobs listed in `admit` (count n) on `s`.
 hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s);
 hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s);

@@ -309,7 +309,22 @@ __device__ __forceinline__ void for_uniform_scene(bool valid, int my, F&& body) 
   }
 }
 
-template <int BLOCK>
+// Traversal for a scene of width W (0: any width, chosen per scene).
+template <int W>
+__device__ __forceinline__ void pt_closest(const DevScene& sc, const DRay& r, int* lds, int stride,
+                                           trav::Best& best) {
+  unsigned long long nv = 0, nt = 0;
+  if constexpr (W == 0) trav::closest_hit<false>(sc, r, lds, stride, best, nv, nt);
+  else trav::closest_hit_w<W, false>(sc, r, lds, stride, best, nv, nt);
+}
+template <int W>
+__device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, int* lds, int stride) {
+  unsigned long long nv = 0, nt = 0;
+  if constexpr (W == 0) return trav::any_hit<false>(sc, r, lds, stride, nv, nt);
+  else return trav::any_hit_w<W, false>(sc, r, lds, stride, nv, nt);
+}
+
+template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
   __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
@@ -329,8 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
       r.tmin = P.ray[6 * Pn + i];
       r.tmax = INFINITY;
       trav::Best best;
-      unsigned long long nv = 0, nt = 0;
-      trav::closest_hit<false>(sc, r, lds + threadIdx.x, BLOCK, best, nv, nt);
+      pt_closest<W>(sc, r, lds + threadIdx.x, BLOCK, best);
       P.hit[i] = best.t;
       P.hit[Pn + i] = best.b1;
       P.hit[2 * Pn + i] = best.b2;
@@ -522,7 +536,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   if (want_mis) P.q_mis[qm] = (uint32_t)i;
 }
 
-template <int BLOCK>
+template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
   __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
@@ -540,15 +554,14 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
       r.d = mk3(P.sray[3 * Pn + i], P.sray[4 * Pn + i], P.sray[5 * Pn + i]);
       r.tmin = P.sray[6 * Pn + i];
       r.tmax = P.sray[7 * Pn + i];
-      unsigned long long nv = 0, nt = 0;
-      if (!trav::any_hit<false>(sc, r, lds + threadIdx.x, BLOCK, nv, nt)) {  // occluded (Scene.hs:45-47)
+      if (!pt_any<W>(sc, r, lds + threadIdx.x, BLOCK)) {  // occluded (Scene.hs:45-47)
         xyz_accum(P.rad, Pn, i, P.pend_s);
       }
     }
   });
 }
 
-template <int BLOCK>
+template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
   __shared__ int lds[trav::kLdsPerLane * BLOCK];
   const int q = blockIdx.x * BLOCK + threadIdx.x;
@@ -568,8 +581,7 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
       r.tmin = P.mray[6 * Pn + i];
       r.tmax = INFINITY;
       trav::Best best;
-      unsigned long long nv = 0, nt = 0;
-      trav::closest_hit<false>(sc, r, lds + threadIdx.x, BLOCK, best, nv, nt);
+      pt_closest<W>(sc, r, lds + threadIdx.x, BLOCK, best);
       int ln = P.mlight[i];
       const DLight& L = S.lights[ln];
       // pend_m already holds the contribution times the sampled light's
@@ -681,26 +693,36 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
 // block, so they run in the largest blocks
 constexpr int kWideBlock = 1024;
 
-template <int BLOCK>
+template <int BLOCK, int W>
 static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, hipStream_t s) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
   dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
-  hipLaunchKernelGGL(k_extend<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, 0, s, a);
   if (shade_waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
-  hipLaunchKernelGGL(k_shadow<BLOCK>, g, b, 0, s, a);
-  hipLaunchKernelGGL(k_mis<BLOCK>, g, b, 0, s, a);
+  hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, 0, s, a);
+  hipLaunchKernelGGL((k_mis<BLOCK, W>), g, b, 0, s, a);
   hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, hipStream_t s) {
+template <int BLOCK>
+static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, int shade_waves, hipStream_t s) {
+  switch (width) {
+    case 2: return enqueue_iter<BLOCK, 2>(a, shade_waves, s);
+    case 4: return enqueue_iter<BLOCK, 4>(a, shade_waves, s);
+    case 8: return enqueue_iter<BLOCK, 8>(a, shade_waves, s);
+    default: return enqueue_iter<BLOCK, 0>(a, shade_waves, s);
+  }
+}
+
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s) {
   switch (cfg.block) {
-    case 64: return enqueue_iter<64>(a, cfg.shade_waves, s);
-    case 256: return enqueue_iter<256>(a, cfg.shade_waves, s);
-    default: return enqueue_iter<128>(a, cfg.shade_waves, s);
+    case 64: return enqueue_iter_w<64>(a, width, cfg.shade_waves, s);
+    case 256: return enqueue_iter_w<256>(a, width, cfg.shade_waves, s);
+    default: return enqueue_iter_w<128>(a, width, cfg.shade_waves, s);
   }
 }
 

@@ -236,3 +236,28 @@ def test_coherent_hint_same_results(gpu_ctx):
             assert np.array_equal(occ, occ2.astype(bool))
     finally:
         gs.close()
+
+
+def test_autotuner_picks_a_valid_config(gpu_ctx):
+    """The tuner times every candidate, leaves the fastest set on the
+    context, and the tuned context still answers bit-exactly."""
+    from bling_amd import tune
+    keep = {k: gpu_ctx.config_get(k) for k in ("bvh.width", "trace.persistent", "trace.block")}
+    try:
+        flat = scenes.small_scene(num_tris=3000, num_spheres=300, seed=17)
+        rays = scenes.incoherent_rays(*scenes.scene_bounds(flat), 20000, seed=7)
+        best, table = tune.tune_trace(gpu_ctx, flat, rays, widths=(2, 4), blocks=(64, 128), modes=(1, 0),
+                                      repeats=1)
+        assert len(table) == 8 and all(row["ms"] > 0 for row in table)
+        assert best == min(table, key=lambda r: r["ms"])
+        for k in ("bvh.width", "trace.persistent", "trace.block"):
+            assert gpu_ctx.config_get(k) == best[k]
+        gs = gpu_ctx.scene(flat)
+        g = gs.intersect(rays)
+        gs.close()
+        o = OracleScene(flat)
+        assert_same(g, o.intersect(rays, brute=True), "tuned")
+        o.close()
+    finally:
+        for k, v in keep.items():
+            gpu_ctx.config_set(k, v)
