@@ -96,7 +96,8 @@ def main():
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
-    for k in ("bvh.width", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves"):
+    for k in ("bvh.width", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves",
+              "pt.persistent"):
         knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,

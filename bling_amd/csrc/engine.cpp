@@ -131,9 +131,9 @@ struct Engine {
     cfg = config;
     P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 8 int arrays + 52 float arrays of P, plus 3 queues
+    // pool layout: 9 int arrays + 52 float arrays of P, plus 3 queues
     size_t per = (size_t)P * 4;
-    size_t total = per * (8 + 52 + 3);
+    size_t total = per * (9 + 52 + 3);
     if (int rc = pool_buf.alloc(c, total)) return rc;
     char* b = pool_buf.as<char>();
     auto take = [&](size_t n) {
@@ -151,6 +151,7 @@ struct Engine {
     pp.dspec = (int32_t*)take(1);
     pp.hprim = (int32_t*)take(1);
     pp.mlight = (int32_t*)take(1);
+    pp.occ = (int32_t*)take(1);
     pp.sxy = (float*)take(2);
     pp.ray = (float*)take(7);
     pp.hit = (float*)take(3);
@@ -798,6 +799,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.shade_waves") {
     if (value != 0 && value != 2) return fail(BGPU_E_INVALID, "pt.shade_waves must be 0 or 2");
     next.shade_waves = (int)value;
+  } else if (k == "pt.persistent") {
+    if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "pt.persistent must be 0 or 1");
+    next.persistent = (int)value;
   } else {
     return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   }
@@ -826,6 +830,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "pt.block") *value = ctx->pt_cfg.block;
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
+  else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
   else return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   return BGPU_OK;
 }

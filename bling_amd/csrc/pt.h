@@ -115,6 +115,7 @@ struct PathPool {
   float* mray;      // 7P MIS ray
   float* pend_m;    // 3P XYZ of the BSDF sample if the MIS ray reaches the light
   int32_t* mlight;  // light index of the MIS ray
+  int32_t* occ;     // shadow ray occluded by a triangle (persistent passes)
   uint32_t* q_act;
   uint32_t* q_shadow;
   uint32_t* q_mis;
@@ -123,7 +124,8 @@ struct PathPool {
 struct PtConfig {
   int block = 128;
   int pool = 1 << 20;
-  int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2)
+  int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2 waves per SIMD)
+  int persistent = 0;   // traversal passes with lane refill (1) or one ray per thread (0)
 };
 
 struct PtKernelArgs {

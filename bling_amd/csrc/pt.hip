@@ -15,7 +15,7 @@
 // Kernel grids cover the pool; queue lengths are read on the device, so the
 // sequence replays from a captured hipGraph without host round trips.
 #include "pt_device.h"
-#include "trav.h"
+#include "trav_persist.h"
 
 namespace bgpu {
 
@@ -607,6 +607,160 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
   });
 }
 
+// ---- persistent traversal passes (scenes of one width W) -----------------
+// Each traversal is two passes over its queue, triangles then shapes, with
+// lanes refilled from one static range of queue entries per wave
+// (trav_persist.h).  Items carry their own scene, so jobs of different
+// scenes share a wave.
+__device__ __forceinline__ const DevScene& scene_of(const PtKernelArgs& a, int i) {
+  return a.scenes[a.jobs[a.pool.job[i]].scene].geo;
+}
+__device__ __forceinline__ trav::LaneScene lane_scene(const DevScene& g, bool fat) {
+  return trav::LaneScene{g.nodes, g.leaf, g.shapes, g.shape_pad_k, fat ? g.shape_root : g.tri_root};
+}
+
+template <bool FAT>
+struct ExtendSrc {  // closest hit of the path ray; result in hit / hprim
+  const PtKernelArgs& a;
+  __device__ __forceinline__ bool begin(int k, DRay& r, trav::LaneScene& ls, trav::Best& best) {
+    const PathPool& P = a.pool;
+    const int Pn = P.P, i = (int)P.q_act[k];
+    const DevScene& g = scene_of(a, i);
+    r.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
+    r.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
+    r.tmin = P.ray[6 * Pn + i];
+    r.tmax = INFINITY;
+    ls = lane_scene(g, FAT);
+    best = trav::Best{r.tmax, 0.0f, 0.0f, -1};
+    if (FAT && P.hprim[i] >= 0) best = trav::Best{P.hit[i], P.hit[Pn + i], P.hit[2 * Pn + i], P.hprim[i]};
+    return slab_haskell(g.root_lo, g.root_hi, r);
+  }
+  __device__ __forceinline__ void end(int k, const DRay&, const trav::LaneScene&, const trav::Best& best) {
+    if (FAT && best.prim < 0) return;
+    const PathPool& P = a.pool;
+    const int Pn = P.P, i = (int)P.q_act[k];
+    P.hit[i] = best.t;
+    P.hit[Pn + i] = best.b1;
+    P.hit[2 * Pn + i] = best.b2;
+    P.hprim[i] = best.prim;
+  }
+};
+
+template <bool FAT>
+struct ShadowSrc {  // any hit of the shadow ray; unoccluded adds the light sample
+  const PtKernelArgs& a;
+  __device__ __forceinline__ bool begin(int k, DRay& r, trav::LaneScene& ls, trav::Best& best) {
+    const PathPool& P = a.pool;
+    const int Pn = P.P, i = (int)P.q_shadow[k];
+    const DevScene& g = scene_of(a, i);
+    r.o = mk3(P.sray[i], P.sray[Pn + i], P.sray[2 * Pn + i]);
+    r.d = mk3(P.sray[3 * Pn + i], P.sray[4 * Pn + i], P.sray[5 * Pn + i]);
+    r.tmin = P.sray[6 * Pn + i];
+    r.tmax = P.sray[7 * Pn + i];
+    ls = lane_scene(g, FAT);
+    best = trav::Best{r.tmax, 0.0f, 0.0f, -1};
+    return slab_haskell(g.root_lo, g.root_hi, r) && (!FAT || P.occ[i] == 0);
+  }
+  __device__ __forceinline__ void end(int k, const DRay&, const trav::LaneScene&, const trav::Best& best) {
+    const PathPool& P = a.pool;
+    const int i = (int)P.q_shadow[k];
+    if (!FAT) {
+      P.occ[i] = best.prim >= 0 ? 1 : 0;
+    } else if (best.prim < 0 && P.occ[i] == 0) {  // unoccluded (Scene.hs:45-47, 61-69)
+      xyz_accum(P.rad, P.P, i, P.pend_s);
+    }
+  }
+};
+
+template <bool FAT>
+struct MisSrc {  // closest hit of the MIS ray into hit / hprim (resolved by k_mis_resolve)
+  const PtKernelArgs& a;
+  __device__ __forceinline__ bool begin(int k, DRay& r, trav::LaneScene& ls, trav::Best& best) {
+    const PathPool& P = a.pool;
+    const int Pn = P.P, i = (int)P.q_mis[k];
+    const DevScene& g = scene_of(a, i);
+    r.o = mk3(P.mray[i], P.mray[Pn + i], P.mray[2 * Pn + i]);
+    r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
+    r.tmin = P.mray[6 * Pn + i];
+    r.tmax = INFINITY;
+    ls = lane_scene(g, FAT);
+    best = trav::Best{r.tmax, 0.0f, 0.0f, -1};
+    if (FAT && P.hprim[i] >= 0) best = trav::Best{P.hit[i], P.hit[Pn + i], P.hit[2 * Pn + i], P.hprim[i]};
+    return slab_haskell(g.root_lo, g.root_hi, r);
+  }
+  __device__ __forceinline__ void end(int k, const DRay&, const trav::LaneScene&, const trav::Best& best) {
+    if (FAT && best.prim < 0) return;
+    const PathPool& P = a.pool;
+    const int Pn = P.P, i = (int)P.q_mis[k];
+    P.hit[i] = best.t;
+    P.hit[Pn + i] = best.b1;
+    P.hit[2 * Pn + i] = best.b2;
+    P.hprim[i] = best.prim;
+  }
+};
+
+template <int BLOCK, int W, bool FAT>
+__global__ __launch_bounds__(BLOCK) void k_extend_p(PtKernelArgs a) {
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  ExtendSrc<FAT> src{a};
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_act, nullptr, lds + threadIdx.x, BLOCK,
+                                            nullptr);
+}
+template <int BLOCK, int W, bool FAT>
+__global__ __launch_bounds__(BLOCK) void k_shadow_p(PtKernelArgs a) {
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  ShadowSrc<FAT> src{a};
+  trav::persist_trace<W, true, FAT, false>(src, (int)a.cnt->n_shadow, nullptr, lds + threadIdx.x, BLOCK,
+                                           nullptr);
+}
+template <int BLOCK, int W, bool FAT>
+__global__ __launch_bounds__(BLOCK) void k_mis_p(PtKernelArgs a) {
+  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  MisSrc<FAT> src{a};
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_mis, nullptr, lds + threadIdx.x, BLOCK,
+                                            nullptr);
+}
+
+// The MIS ray reaches the sampled light: the hit primitive is that AreaLight
+// seen from its emitting side, or an infinite light's ray escapes.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mis_resolve(PtKernelArgs a) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_mis;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < n;
+  int i = valid ? (int)P.q_mis[q] : 0;
+  for_uniform_scene(valid, valid ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
+    const DevPtScene& S = a.scenes[scene_u];
+    const int prim = P.hprim[i];
+    const DLight& L = S.lights[P.mlight[i]];
+    bool reaches = false;
+    if (prim >= 0) {
+      // the hit primitive's light must be the sampled AreaLight (Light.hs:47-50)
+      if (prim >= S.ntri && L.type == LT_AREA) {
+        int hl = S.geo.shapes[prim - S.ntri].light;
+        if (hl >= 0 && S.lights[hl].id == L.id) {
+          DRay r;
+          r.o = mk3(P.mray[i], P.mray[Pn + i], P.mray[2 * Pn + i]);
+          r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
+          r.tmin = P.mray[6 * Pn + i];
+          r.tmax = INFINITY;
+          DG dgg, dgs;
+          float eps;
+          int mat, light;
+          hit_dg(S, r, P.hit[i], P.hit[Pn + i], P.hit[2 * Pn + i], prim, &dgg, &dgs, &eps, &mat, &light);
+          reaches = dot3(dgg.n, neg3(r.d)) > 0.0f;  // lEmit with -wi
+        }
+      }
+    } else {
+      reaches = L.type == LT_INF;  // le of an infinite light (Light.hs:98-106)
+    }
+    if (reaches) xyz_accum(P.rad, Pn, i, P.pend_m);
+  });
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
   __shared__ unsigned long long s_key[BLOCK / 64 * kGroups];
@@ -693,15 +847,38 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
 // block, so they run in the largest blocks
 constexpr int kWideBlock = 1024;
 
+template <int BLOCK>
+static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
+  if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
+}
+
 template <int BLOCK, int W>
-static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, hipStream_t s) {
+static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool persistent, hipStream_t s) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
   dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
+  if constexpr (W != 0) {
+    if (persistent) {
+      static int ge = 0, gs = 0, gm = 0;  // grids that fill the device once
+      if (!ge) ge = trav::persistent_grid(k_extend_p<BLOCK, W, false>, BLOCK);
+      if (!gs) gs = trav::persistent_grid(k_shadow_p<BLOCK, W, false>, BLOCK);
+      if (!gm) gm = trav::persistent_grid(k_mis_p<BLOCK, W, false>, BLOCK);
+      hipLaunchKernelGGL((k_extend_p<BLOCK, W, false>), dim3(ge), b, 0, s, a);
+      hipLaunchKernelGGL((k_extend_p<BLOCK, W, true>), dim3(ge), b, 0, s, a);
+      launch_shade<BLOCK>(a, shade_waves, g, b, s);
+      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, false>), dim3(gs), b, 0, s, a);
+      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, true>), dim3(gs), b, 0, s, a);
+      hipLaunchKernelGGL((k_mis_p<BLOCK, W, false>), dim3(gm), b, 0, s, a);
+      hipLaunchKernelGGL((k_mis_p<BLOCK, W, true>), dim3(gm), b, 0, s, a);
+      hipLaunchKernelGGL((k_mis_resolve<BLOCK>), g, b, 0, s, a);
+      hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, 0, s, a);
-  if (shade_waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
+  launch_shade<BLOCK>(a, shade_waves, g, b, s);
   hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, 0, s, a);
   hipLaunchKernelGGL((k_mis<BLOCK, W>), g, b, 0, s, a);
   hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
@@ -709,20 +886,21 @@ static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, hipStream
 }
 
 template <int BLOCK>
-static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, int shade_waves, hipStream_t s) {
+static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, const PtConfig& cfg, hipStream_t s) {
+  const bool p = cfg.persistent != 0;
   switch (width) {
-    case 2: return enqueue_iter<BLOCK, 2>(a, shade_waves, s);
-    case 4: return enqueue_iter<BLOCK, 4>(a, shade_waves, s);
-    case 8: return enqueue_iter<BLOCK, 8>(a, shade_waves, s);
-    default: return enqueue_iter<BLOCK, 0>(a, shade_waves, s);
+    case 2: return enqueue_iter<BLOCK, 2>(a, cfg.shade_waves, p, s);
+    case 4: return enqueue_iter<BLOCK, 4>(a, cfg.shade_waves, p, s);
+    case 8: return enqueue_iter<BLOCK, 8>(a, cfg.shade_waves, p, s);
+    default: return enqueue_iter<BLOCK, 0>(a, cfg.shade_waves, p, s);
   }
 }
 
 hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s) {
   switch (cfg.block) {
-    case 64: return enqueue_iter_w<64>(a, width, cfg.shade_waves, s);
-    case 256: return enqueue_iter_w<256>(a, width, cfg.shade_waves, s);
-    default: return enqueue_iter_w<128>(a, width, cfg.shade_waves, s);
+    case 64: return enqueue_iter_w<64>(a, width, cfg, s);
+    case 256: return enqueue_iter_w<256>(a, width, cfg, s);
+    default: return enqueue_iter_w<128>(a, width, cfg, s);
   }
 }
 

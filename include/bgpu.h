@@ -273,7 +273,10 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * back to a narrower width), "trace.block" (64/128/256, intersect kernels),
  * "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
  * ray when theirs is done; 0: one ray per thread), "pt.block"
- * (64/128/256, bounce kernels), "pt.pool" (path slots, >= 1024),
+ * (64/128/256, bounce kernels), "pt.persistent" (1: the bounce loop's
+ * traversal runs as persistent passes that refill lanes with new rays;
+ * 0, the default: one ray per thread),
+ * "pt.pool" (path slots, >= 1024),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
  * two waves per SIMD).  Changing a pt.* knob is refused with BGPU_E_STATE
  * while served jobs are queued or running; the engines are rebuilt on their

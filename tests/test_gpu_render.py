@@ -206,9 +206,10 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     flat = scenes.cornell(32)
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=4)
     base, _ = render_gpu(gpu_ctx, flat, spec)
-    defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves")}
+    defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves", "pt.persistent")}
     try:
-        for key, val in (("pt.shade_waves", 2), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096)):
+        for key, val in (("pt.shade_waves", 0), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096),
+                         ("pt.persistent", 0)):
             gpu_ctx.config_set(key, val)
             assert gpu_ctx.config_get(key) == val
             img, _ = render_gpu(gpu_ctx, flat, spec)
