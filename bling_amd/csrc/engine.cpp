@@ -55,6 +55,8 @@ struct HostJob {
     desc.film = nullptr;
     px = (size_t)view.film_w * view.film_h;
     per_pass = (unsigned long long)view.ext_w * view.ext_h * (unsigned long long)(d.spp_nu * d.spp_nv);
+    // passes are counted by 32-bit device counters
+    if (per_pass > 0xffffffffull) return fail(BGPU_E_INVALID, "job has more than 2^32-1 samples per pass");
     if (per_pass > (1ull << 62) / (unsigned long long)d.passes)
       return fail(BGPU_E_INVALID, "job has too many samples");
     total = per_pass * (unsigned long long)d.passes;

@@ -121,6 +121,14 @@ def test_invalid_job_rejected(gpu_ctx):
             with pytest.raises(abi.BgpuError) as e:
                 GpuRenderer(bad).render_plain(gs)
             assert e.value.code == abi.E_INVALID
+        # 30001^2 extent pixels x 8 samples in one pass exceed the 32-bit pass
+        # counters: refused before the film is touched
+        import ctypes as C
+        big = JobSpec(spp_nu=4, spp_nv=2, view=flat.view(30000, 30000))
+        tiny = np.zeros(16, np.float32)
+        for call in (lambda: gpu_ctx.lib.bgpu_render(gs.h, C.byref(big.desc()), tiny.ctypes.data, None),
+                     lambda: gpu_ctx.lib.bgpu_job_submit(gs.h, C.byref(big.desc()), C.byref(C.c_void_p()))):
+            assert call() == abi.E_INVALID
     finally:
         gs.close()
 
