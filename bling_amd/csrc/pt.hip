@@ -272,9 +272,7 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
         P.ray[i] = r.o.x; P.ray[Pn + i] = r.o.y; P.ray[2 * Pn + i] = r.o.z;
         P.ray[3 * Pn + i] = r.d.x; P.ray[4 * Pn + i] = r.d.y; P.ray[5 * Pn + i] = r.d.z;
         P.ray[6 * Pn + i] = r.tmin;
-#pragma unroll
-        for (int b = 0; b < 16; ++b) P.thr[b * Pn + i] = 1.0f;
-        xyz_store(P.rad, Pn, i, Xyz{});
+        xyz_store(P.rad, Pn, i, Xyz{});  // throughput is 1 at depth 0, not stored
         P.state[i] = 1;
         fresh = true;
       }
@@ -377,6 +375,9 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
       int ds = P.dspec[i];
       int depth = ds & 0xffff;
       bool spec = (ds >> 16) != 0;
+      // throughput band b: exactly 1 for camera rays (not stored at depth 0)
+      const float* thr = P.thr + i;  // band b at thr[b * Pn]
+      auto thr_at = [&](int b) { return depth == 0 ? 1.0f : thr[b * Pn]; };
       if (prim < 0) {  // Path.hs:44-47
         if (spec) {
           for (int li = 0; li < S.nlights; ++li) {
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
             if (le) {
               Xyz c;
 #pragma unroll
-              for (int b = 0; b < 16; ++b) c.add(b, P.thr[b * Pn + i] * le[b]);
+              for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * le[b]);
               xyz_accum(P.rad, Pn, i, c);
             }
           }
@@ -411,13 +412,12 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         sm.get2(un, 2 + 3 * d, &bd1, &bd2);
         f3 wo = neg3(ray.d);
         f3 p = bsdf.p;
-        const float* thr = P.thr + i;  // band b at thr[b * Pn]
         // emission seen through a specular chain (Path.hs:53, intLe with rd)
         if (spec && light >= 0 && dot3(dgg.n, ray.d) > 0.0f) {
           const float* L = S.lights[light].L;
           Xyz c;
 #pragma unroll
-          for (int b = 0; b < 16; ++b) c.add(b, thr[b * Pn] * L[b]);
+          for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
           xyz_accum(P.rad, Pn, i, c);
         }
         int lc = S.nlights;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
                 float fv = spec_at(f, b), li = ls.li[b];
                 liBlack = liBlack && li == 0.0f;
                 fBlack = fBlack && fv == 0.0f;
-                c.add(b, thr[b * Pn] * (((fv * li) * k) * lscale));
+                c.add(b, thr_at(b) * (((fv * li) * k) * lscale));
               }
               if (!(liBlack || fBlack)) {
                 xyz_store(P.pend_s, Pn, i, c);
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
               for (int b = 0; b < 16; ++b) {
                 float fv = spec_at(fb, b);
                 fBlack = fBlack && fv == 0.0f;
-                c.add(b, (thr[b * Pn] * ((fv * w) * lscale)) * e[b]);
+                c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
               }
               if (!fBlack) {
                 xyz_store(P.pend_m, Pn, i, c);
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         if (depth > J.rr) {
           float y = 0.0f;  // sY (Spectrum.hs:371-373)
 #pragma unroll
-          for (int b = 0; b < 16; ++b) y = y + thr[b * Pn] * cCieY[b];
+          for (int b = 0; b < 16; ++b) y = y + thr_at(b) * cCieY[b];
           pc = hmin(0.75f, y / BGPU_CIE_Y_SUM);
         }
         float x = sm.get1(un, 3 + 4 * d);
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
             for (int b = 0; b < 16; ++b) {
               float fv = spec_at(f, b);
               fBlack = fBlack && fv == 0.0f;
-              thw[b * Pn] = (fv * thw[b * Pn]) * ipc;
+              thw[b * Pn] = (fv * thr_at(b)) * ipc;
             }
             if (fBlack) {
               cont = false;

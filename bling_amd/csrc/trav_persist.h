@@ -28,8 +28,15 @@ namespace trav {
 #ifndef BGPU_SLICE
 #define BGPU_SLICE 128
 #endif
+#ifndef BGPU_SPECULATIVE
+#define BGPU_SPECULATIVE 0
+#endif
 constexpr unsigned int kSlice = BGPU_SLICE;
 constexpr int kRefillMin = BGPU_REFILL_MIN;
+// A lane that reaches a leaf while other lanes of its wave still walk inner
+// nodes postpones the leaf and walks on (speculative traversal, Aila and
+// Laine 2009), so that the leaf step runs with more lanes active.
+constexpr bool kSpeculative = BGPU_SPECULATIVE != 0;
 
 // The part of a scene a traversal pass reads, per lane.
 struct LaneScene {
@@ -49,6 +56,7 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
   st.stride = stride;
   st.sp = 0;
   int item = -1, cur = kNoChild;
+  int pend = kNoChild;  // postponed leaf, kNoChild when none
   DRay r;
   LaneScene ls{nullptr, nullptr, nullptr, 0.0f, -1};
   f3 inv = mk3(0.0f, 0.0f, 0.0f), oi = inv;
@@ -88,6 +96,7 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
           inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
           oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
           st.sp = 0;
+          pend = kNoChild;
           cur = live && ls.root >= 0 ? ls.root : kNoChild;
         }
         const unsigned int took = (unsigned int)__popcll(m);
@@ -97,6 +106,15 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
     if (__ballot(item >= 0) == 0) {
       if (drained) break;
       continue;
+    }
+    if (kSpeculative && item >= 0 && cur < 0 && pend == kNoChild) {
+      pend = cur;
+      if (st.sp > 0) {
+        float t;
+        st.pop(&cur, &t);
+      } else {
+        cur = kNoChild;
+      }
     }
     // inner nodes until every lane is at a leaf or done
     while (item >= 0 && cur >= 0 && cur != kNoChild) {
@@ -138,10 +156,26 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
       } else {
         cur = kNoChild;
       }
+      if (kSpeculative) {
+        if (cur < 0 && pend == kNoChild) {  // postpone the first leaf, walk on
+          pend = cur;
+          if (st.sp > 0) {
+            float t;
+            st.pop(&cur, &t);
+          } else {
+            cur = kNoChild;
+          }
+        }
+        // stop once no lane still looks for its first leaf
+        if (__ballot(item >= 0 && pend == kNoChild && cur >= 0 && cur != kNoChild) == 0) break;
+      }
     }
-    // leaves
-    if (item >= 0 && cur < 0) {
-      const int enc = ~cur;
+    // leaves: the postponed one, else the one the walk stopped at
+    const bool at_leaf = kSpeculative ? (item >= 0 && pend != kNoChild) : (item >= 0 && cur < 0);
+    if (at_leaf) {
+      const int leaf = kSpeculative ? pend : cur;
+      if (kSpeculative) pend = kNoChild;
+      const int enc = ~leaf;
       const int first = enc >> 4, cnt = enc & 15;
       bool hit_any = false;
       for (int k = first; k < first + cnt; ++k) {
@@ -184,15 +218,18 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
       if (ANY && hit_any) {
         best.prim = 0;  // occluded
         cur = kNoChild;
-      } else if (st.sp > 0) {
-        float t;
-        st.pop(&cur, &t);
-      } else {
-        cur = kNoChild;
+        st.sp = 0;
+      } else if (!kSpeculative) {
+        if (st.sp > 0) {
+          float t;
+          st.pop(&cur, &t);
+        } else {
+          cur = kNoChild;
+        }
       }
     }
     // finished items commit their result and free the lane
-    if (item >= 0 && cur == kNoChild) {
+    if (item >= 0 && cur == kNoChild && (!kSpeculative || pend == kNoChild)) {
       src.end(item, r, ls, best);
       item = -1;
       best.prim = -1;
