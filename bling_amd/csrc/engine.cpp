@@ -7,8 +7,11 @@
 //                    jobs arriving while others run are admitted as job-table
 //                    slots and HBM (film buffers) free up, and their samples
 //                    share the path pool with everything in flight
-//                    (continuous batching).  Progress is polled per pass
-//                    (PassDone, Rendering.hs:165-169).
+//                    (continuous batching).  Admitted jobs hand their
+//                    samples to the device in chunks, earliest estimated
+//                    finish first, so short jobs overtake long ones.
+//                    Progress is polled per pass (PassDone,
+//                    Rendering.hs:165-169).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -30,6 +33,8 @@ struct HostJob {
   size_t px = 0;           // film pixels
   int slot = -1;
   unsigned long long total = 0, per_pass = 0;
+  unsigned long long fed = 0;  // samples handed to the device work space
+  unsigned long long seq = 0;  // arrival order
   DevBuf film, sq;
   std::vector<unsigned int> h_pass;  // host copy of per-pass counts
   unsigned long long h_skipped = 0;
@@ -106,6 +111,7 @@ struct Engine {
   // its reset step drops the finished job from the device's job ring, after
   // which the slot's table entry may be overwritten.
   std::vector<std::pair<int, unsigned int>> quarantine;  // (slot, iteration at retirement)
+  unsigned long long n_admitted = 0;
 
   ~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -167,14 +173,14 @@ struct Engine {
     pp.q_shadow = (uint32_t*)take(1);
     pp.q_mis = (uint32_t*)take(1);
     if (int rc = jobs_buf.alloc(c, sizeof(DevJob) * kMaxJobs)) return rc;
-    if (int rc = ring_buf.alloc(c, sizeof(uint32_t) * kMaxJobs)) return rc;
+    if (int rc = ring_buf.alloc(c, sizeof(RingChunk) * kRing)) return rc;
     if (int rc = scenes_buf.alloc(c, sizeof(DevPtScene) * kMaxScenes)) return rc;
     if (int rc = cnt_buf.alloc(c, sizeof(PtCounters))) return rc;
     if (int rc = admit_buf.alloc(c, sizeof(AdmitReq) * kMaxJobs)) return rc;
     if (int rc = prog_buf.alloc(c, sizeof(uint32_t) * kProgWords)) return rc;
     args.scenes = scenes_buf.as<DevPtScene>();
     args.jobs = jobs_buf.as<DevJob>();
-    args.ring = ring_buf.as<uint32_t>();
+    args.ring = ring_buf.as<RingChunk>();
     args.cnt = cnt_buf.as<PtCounters>();
     args.admit = admit_buf.as<AdmitReq>();
     BGPU_HIP(hipHostMalloc((void**)&h_admit, sizeof(AdmitReq) * kMaxJobs, hipHostMallocDefault));
@@ -333,12 +339,69 @@ struct Engine {
     J.cancelled = 0;
     J.view = j->view;
     BGPU_HIP(hipMemcpyAsync(args.jobs + slot, h_job, sizeof(DevJob), hipMemcpyHostToDevice, stream));
-    h_admit[0] = AdmitReq{slot, 1, j->total};
-    BGPU_HIP(hipMemcpyAsync(args.admit, h_admit, sizeof(AdmitReq), hipMemcpyHostToDevice, stream));
-    BGPU_HIP(pt_enqueue_admit(args, 1, stream));
     BGPU_HIP(hipStreamSynchronize(stream));  // pinned staging is reused
+    j->fed = 0;
+    j->seq = ++n_admitted;
     j->state = BGPU_JOB_RUNNING;
     return BGPU_OK;
+  }
+
+  // Appends chunks of samples to the device work space with one admit
+  // launch: reqs[k] = (job, samples).  Chunks of a job are handed over in
+  // sample order.
+  int feed(const std::vector<std::pair<HostJob*, unsigned long long>>& reqs) {
+    if (reqs.empty()) return BGPU_OK;
+    if (reqs.size() > (size_t)kMaxJobs) return fail(BGPU_E_STATE, "too many chunks in one admission");
+    for (size_t k = 0; k < reqs.size(); ++k) {
+      HostJob* j = reqs[k].first;
+      h_admit[k] = AdmitReq{j->slot, 1, reqs[k].second, j->fed};
+      j->fed += reqs[k].second;
+    }
+    BGPU_HIP(hipMemcpyAsync(args.admit, h_admit, sizeof(AdmitReq) * reqs.size(), hipMemcpyHostToDevice, stream));
+    BGPU_HIP(pt_enqueue_admit(args, (int)reqs.size(), stream));
+    BGPU_HIP(hipStreamSynchronize(stream));  // pinned staging is reused
+    return BGPU_OK;
+  }
+
+  // Earliest estimated finish first: while fewer than `ahead` samples wait
+  // in the device work space, hands chunks of at most `chunk` samples to the
+  // job whose arrival time plus remaining work at `per_ms` samples per
+  // millisecond is smallest (ties: arrival order).  Short jobs overtake
+  // long ones that arrived less than their difference in run time earlier,
+  // and no job waits behind work that arrived after its own estimated
+  // finish, so long jobs are not starved.  The counters in h_cnt must be
+  // fresh (read after the last iteration, no admission since).
+  int feed_next(const std::vector<HostJob*>& js, unsigned long long ahead, unsigned long long chunk,
+                double per_ms) {
+    const PtCounters& c = *h_cnt;
+    unsigned long long backlog = c.work_total > c.work_next ? c.work_total - c.work_next : 0;
+    unsigned int ring_free = (unsigned)kRing - (c.ring_tail - c.ring_head);
+    std::vector<std::pair<HostJob*, unsigned long long>> reqs;
+    std::vector<HostJob*> open;
+    for (HostJob* j : js)
+      if (j->slot >= 0 && !j->cancel_sent && j->fed < j->total) open.push_back(j);
+    const double inv = 1.0 / std::max(per_ms, 1.0);
+    auto key = [inv](const HostJob* j) { return j->t_submit + (double)(j->total - j->fed) * inv; };
+    std::sort(open.begin(), open.end(), [&](const HostJob* a, const HostJob* b) {
+      const double ka = key(a), kb = key(b);
+      return ka != kb ? ka < kb : a->seq < b->seq;
+    });
+    // chunks of one job are merged into one request per launch
+    std::vector<unsigned long long> take(open.size(), 0);
+    size_t k = 0;
+    while (k < open.size() && backlog < ahead && reqs.size() + 1 < ring_free && reqs.size() < (size_t)kMaxJobs) {
+      HostJob* j = open[k];
+      const unsigned long long left = j->total - j->fed - take[k];
+      const unsigned long long n = std::min(left, chunk);
+      take[k] += n;
+      backlog += n;
+      if (take[k] == j->total - j->fed) {
+        reqs.emplace_back(j, take[k]);
+        ++k;
+      }
+    }
+    if (k < open.size() && take[k] > 0) reqs.emplace_back(open[k], take[k]);
+    return feed(reqs);
   }
 
   int cancel(HostJob* j) {
@@ -378,7 +441,8 @@ struct Engine {
       }
       j->passes_seen = pd;
       j->samples_seen = n;
-      if (done && n >= j->total) done->push_back(j);
+      // a cancelled job ends once what was handed over is accounted for
+      if (done && n >= (j->cancel_sent ? j->fed : j->total)) done->push_back(j);
     }
     return BGPU_OK;
   }
@@ -484,6 +548,7 @@ struct Scheduler {
   void loop() {
     (void)hipSetDevice(ctx->device);
     PtCounters prev{};
+    double per_ms = 1e5;  // samples per millisecond
     while (true) {
       std::vector<HostJob*> to_admit;
       {
@@ -548,9 +613,13 @@ struct Scheduler {
       uint64_t seen_before = 0;
       for (HostJob* j : run_now) seen_before += j->samples_seen;
       std::vector<HostJob*> finished;
-      // jobs in `run_now` are deleted only by this thread, and poll writes
-      // fields no other thread reads; results are published under `mu`
-      int rc = eng.run(4);
+      // jobs in `run_now` are deleted only by this thread, and poll and
+      // feed write fields no other thread reads; results are published
+      // under `mu`.  Four pools of samples stay queued ahead of the device:
+      // enough for the iterations between polls, few enough that a short
+      // job arriving now is reached within milliseconds.
+      int rc = eng.feed_next(run_now, 4ull * (unsigned long long)eng.P, (unsigned long long)eng.P / 2, per_ms);
+      if (!rc) rc = eng.run(4);
       if (!rc) rc = eng.poll(run_now, &finished);
       if (rc) {
         std::lock_guard<std::mutex> lk(mu);
@@ -568,6 +637,8 @@ struct Scheduler {
       uint64_t seen_after = 0;
       for (HostJob* j : run_now) seen_after += j->samples_seen;
       accumulate_stats(ctx, prev, cur, t1 - t0, seen_after - seen_before);
+      if (t1 > t0 && seen_after > seen_before)  // throughput estimate for feed_next
+        per_ms = 0.9 * per_ms + 0.1 * (double)(seen_after - seen_before) / (t1 - t0);
       prev = cur;
       std::lock_guard<std::mutex> lk(mu);
       for (HostJob* j : run_now) j->passes_done = j->passes_seen;
@@ -638,6 +709,7 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   eng.release_quarantine();
   const PtCounters before = *eng.h_cnt;
   if (int rc = eng.admit(&j)) return bail(rc);
+  if (int rc = eng.feed({{&j, j.total}})) return bail(rc);
   // every path ends within max_depth + 1 bounces, so the job drains in at
   // most (total / P + 1) * (max_depth + 2) iterations
   unsigned long long cap = (j.total / (unsigned long long)eng.P + 2) * (unsigned long long)(job->max_depth + 2) + 64;

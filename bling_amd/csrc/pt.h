@@ -12,6 +12,7 @@ namespace bgpu {
 
 constexpr int kBands = 16;
 constexpr int kMaxJobs = 1024;   // job table slots (reused)
+constexpr int kRing = 2048;      // job chunks in the work queue (power of two)
 constexpr int kMaxScenes = 64;   // scenes referenced by the engine
 
 // BxDF flags and kinds (Reflection.hs:94-108, Reflection/*.hs)
@@ -64,11 +65,10 @@ struct DevView {
   float filter[256];
 };
 
-// One job in the device job table.  Its samples occupy the work indices
-// [begin, end) of a monotonic 64-bit work space; a work index maps to
-// (pass, pixel of the sample extent, sample n) pass-major.
+// One job in the device job table.  Its samples (numbered pass-major:
+// pass, pixel of the sample extent, sample n) enter the work space in
+// chunks (RingChunk).
 struct DevJob {
-  unsigned long long begin, end;
   unsigned long long per_pass;  // extent pixels * nu * nv
   uint64_t seed;
   int32_t scene, nu, nv, N, md, sd, rr, passes;
@@ -85,14 +85,22 @@ struct PtCounters {
   unsigned long long work_next, work_total;
   unsigned int n_act, n_shadow, n_mis, last_act;
   unsigned long long tot_camera, tot_extend, tot_shadow, tot_mis, tot_samples;
-  unsigned int ring_head, ring_tail;  // open-job ring [head, tail)
+  unsigned int ring_head, ring_tail;  // chunks not yet fully taken [head, tail)
   unsigned int iter, pad;
 };
 
-// Admission request consumed by the admit kernel.
+// A chunk of a job's samples in the monotonic 64-bit work space: work
+// indices [wbegin, wend) are the job's samples local, local + 1, ...
+struct RingChunk {
+  unsigned long long wbegin, wend, local;
+  int32_t slot, pad;
+};
+
+// Chunk request consumed by the admit kernel: `size` samples of job `slot`
+// starting at its sample `local`.
 struct AdmitReq {
   int32_t slot, valid;
-  unsigned long long size;
+  unsigned long long size, local;
 };
 
 // Path pool, structure of arrays of P slots.
@@ -132,7 +140,7 @@ struct PtKernelArgs {
   PathPool pool;
   const DevPtScene* scenes;
   DevJob* jobs;
-  uint32_t* ring;  // job slots in admission order (kMaxJobs entries, modulo)
+  RingChunk* ring;  // chunks in work order (kRing entries, modulo)
   PtCounters* cnt;
   AdmitReq* admit;
 };

@@ -156,7 +156,7 @@ __global__ void k_init(PtKernelArgs a) {
     PtCounters z = {};
     *a.cnt = z;
   }
-  if (i < kMaxJobs) a.ring[i] = 0;
+  if (i < kRing) a.ring[i] = RingChunk{0, 0, 0, 0, 0};
   if (i < a.pool.P) a.pool.state[i] = 0;
 }
 
@@ -164,14 +164,11 @@ __global__ void k_admit(PtKernelArgs a, int n) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   PtCounters* c = a.cnt;
   for (int k = 0; k < n; ++k) {
-    AdmitReq r = a.admit[k];
-    if (!r.valid) continue;
-    DevJob& J = a.jobs[r.slot];
-    unsigned long long b = c->work_total > c->work_next ? c->work_total : c->work_next;
-    J.begin = b;
-    J.end = b + r.size;
-    c->work_total = J.end;
-    a.ring[c->ring_tail % kMaxJobs] = (uint32_t)r.slot;
+    const AdmitReq r = a.admit[k];
+    if (!r.valid || c->ring_tail - c->ring_head >= (unsigned)kRing) continue;
+    const unsigned long long b = c->work_total > c->work_next ? c->work_total : c->work_next;
+    a.ring[c->ring_tail % kRing] = RingChunk{b, b + r.size, r.local, r.slot, 0};
+    c->work_total = b + r.size;
     c->ring_tail++;
   }
 }
@@ -187,23 +184,19 @@ __global__ void k_reset(PtKernelArgs a) {
   c->n_shadow = 0;
   c->n_mis = 0;
   c->iter++;
-  while (c->ring_head != c->ring_tail) {
-    const DevJob& J = a.jobs[a.ring[c->ring_head % kMaxJobs]];
-    if (J.end <= c->work_next) c->ring_head++;
-    else break;
-  }
+  while (c->ring_head != c->ring_tail && a.ring[c->ring_head % kRing].wend <= c->work_next) c->ring_head++;
 }
 
-// Job holding work index w: the last ring entry whose begin <= w.
-__device__ __forceinline__ int find_job(const PtKernelArgs& a, unsigned long long w, unsigned int head,
-                                        unsigned int tail) {
+// Chunk holding work index w: the last ring entry whose wbegin <= w.
+__device__ __forceinline__ const RingChunk& find_chunk(const PtKernelArgs& a, unsigned long long w,
+                                                       unsigned int head, unsigned int tail) {
   unsigned int lo = 0, hi = tail - head;  // search [lo, hi)
   while (hi - lo > 1) {
     unsigned int mid = (lo + hi) >> 1;
-    if (a.jobs[a.ring[(head + mid) % kMaxJobs]].begin <= w) lo = mid;
+    if (a.ring[(head + mid) % kRing].wbegin <= w) lo = mid;
     else hi = mid;
   }
-  return (int)a.ring[(head + lo) % kMaxJobs];
+  return a.ring[(head + lo) % kRing];
 }
 
 template <int BLOCK>
@@ -240,15 +233,15 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
   }
   bool fresh = false;
   const unsigned int head = a.cnt->ring_head, tail = a.cnt->ring_tail;
-  if (want && w < a.cnt->work_total && tail != head && tail - head <= (unsigned)kMaxJobs) {
-    int j = find_job(a, w, head, tail);
-    if (j < 0 || j >= kMaxJobs) j = 0;
+  if (want && w < a.cnt->work_total && tail != head && tail - head <= (unsigned)kRing) {
+    const RingChunk& ch = find_chunk(a, w, head, tail);
+    const int j = ch.slot >= 0 && ch.slot < kMaxJobs ? ch.slot : 0;
     const DevJob& J = a.jobs[j];
-    if (w >= J.begin && w < J.end) {
+    if (w >= ch.wbegin && w < ch.wend) {
       if (J.cancelled) {
         atomicAdd(J.skipped, 1ull);
       } else {
-        unsigned long long local = w - J.begin;
+        unsigned long long local = ch.local + (w - ch.wbegin);
         int pass = (int)(local / J.per_pass);
         unsigned long long rem = local % J.per_pass;
         int pix = (int)(rem / (unsigned long long)J.N);
@@ -833,8 +826,8 @@ hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s) {
     if (e != hipSuccess) return e;
     tables = true;
   }
-  int P = a.pool.P;
-  hipLaunchKernelGGL(k_init, dim3((P + 255) / 256), dim3(256), 0, s, a);
+  const int n = a.pool.P > kRing ? a.pool.P : kRing;
+  hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
