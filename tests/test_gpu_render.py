@@ -297,3 +297,52 @@ def test_batch_of_64_jobs(gpu_ctx):
     finally:
         for gs in gss:
             gs.close()
+
+
+def test_chunked_jobs_match_plain_overtake_and_cancel(gpu_ctx):
+    """With a small path pool every job reaches the device in many chunks:
+    chunked jobs equal their plain renders, a short job submitted after a
+    long one finishes first, and a long job cancelled part-way ends as
+    CANCELLED with its complete passes kept."""
+    import time
+    keep = gpu_ctx.config_get("pt.pool")
+    gpu_ctx.config_set("pt.pool", 4096)  # chunks of 2048 samples
+    flat = scenes.cornell(48)
+    gs = gpu_ctx.scene(flat)
+    try:
+        specs = [JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, passes=3, seed=21),
+                 JobSpec(spp_nu=1, spp_nv=1, max_depth=4, sample_depth=3, passes=1, seed=22,
+                         view=flat.view(16, 16)),
+                 JobSpec(spp_nu=4, spp_nv=2, max_depth=4, sample_depth=3, passes=2, seed=23,
+                         view=flat.view(40, 24))]
+        handles = [Job(gs, sp) for sp in specs]
+        for sp, h in zip(specs, handles):
+            h.wait(timeout=120)
+            img, _, done, state = h.poll()
+            h.release()
+            alone, _ = GpuRenderer(sp).render_plain(gs)
+            assert state == abi.JOB_DONE and done == sp.passes
+            assert np.array_equal(img[..., 0], alone[..., 0])
+            np.testing.assert_allclose(img, alone, rtol=1e-4, atol=1e-5)
+
+        big = Job(gs, JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, passes=50, seed=24))
+        small = Job(gs, JobSpec(spp_nu=1, spp_nv=1, max_depth=4, sample_depth=3, passes=1, seed=25,
+                                view=flat.view(16, 16)))
+        small.wait(timeout=120)
+        assert small.poll(want_image=False)[3] == abi.JOB_DONE
+        assert big.poll(want_image=False)[3] == abi.JOB_RUNNING  # overtaken
+        small.release()
+        t0 = time.time()
+        while big.poll(want_image=False)[2] < 1 and time.time() - t0 < 60:
+            time.sleep(0.002)
+        big.cancel()
+        with pytest.raises(abi.BgpuError) as e:
+            big.wait(timeout=120)
+        assert e.value.code == abi.E_CANCELLED
+        img, _, done, state = big.poll()
+        assert state == abi.JOB_CANCELLED and 1 <= done < 50
+        assert np.isfinite(img).all()
+        big.release()
+    finally:
+        gs.close()
+        gpu_ctx.config_set("pt.pool", keep)
