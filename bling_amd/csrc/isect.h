@@ -9,23 +9,38 @@
 namespace bgpu {
 
 // BVH node of width W (2, 4 or 8), 8 * W floats (W = 4: one 128-byte line):
-//   lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W] link[W] pad[W]
+//   lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W] link[W] h[W]
 //   link >= 0: inner node; link < 0: leaf ~((first<<4)|count); kNoChild: unused
+//   h: the shape tree's per-child margin factor (below); 0 in the triangle tree
 // Leaf records, 48 B each, in leaf order:
 //   r0 = (p1.xyz, prim id bits), r1 = (e1.xyz, kind bits), r2 = (e2.xyz, 0)
 //   kind >= 0: triangle (e1 = p2-p1, e2 = p3-p1); kind < 0: shape index ~kind.
 //
 // Triangles and analytic shapes live in two BVHs sharing these arrays: the
 // triangle tree is rooted at node tri_root, the shape tree at shape_root (-1
-// when absent).  The reference's binary32 sphere/cylinder quadratic
-// (Shape.hs:113-187) evaluates c = |o|^2 - r^2 with cancellation and so
-// reports hits up to about 3*sqrt(2^-24)*|o - centre| outside the true
-// surface (|.| in object space); a box padded only by rounding would cull
-// those hits, which the reference does report.  The children of a
-// shape-tree node are therefore tested grown by shape_pad_k times the L1
-// distance from the ray origin to the farthest corner of the node's box,
-// which bounds |o - centre| for every shape below it (shape_pad_k includes
-// the anisotropy of the shapes' transforms).
+// when absent).
+//
+// The reference's binary32 sphere quadratic (Shape.hs:173-187, solveQuadric
+// Math.hs:124-139) reports hits slightly outside the true surface, and a
+// shape-tree box must contain every point it reports.  In object space, with
+// u = 2^-24, the computed a = |d|^2, b = 2 o.d, c = |o|^2 - r^2 and
+// b*b - (4a)*c carry an absolute error of at most 64 u |o|^2 |d|^2 in
+// total, while the exact discriminant is 4 |d|^2 (r^2 - dist^2) (dist: the
+// ray line's distance from the centre).  A reported hit therefore has
+// dist^2 <= r^2 + 16 u |o|^2, and its t lies within 4 sqrt(u) |o| / |d| of
+// the closest approach, so the hit point is within
+// sqrt(r^2 + 32 u |o|^2) of the centre: an excess of at most
+//   min(K0 |o|, K0^2 |o|^2 / (2 r)),  K0 = kSphereK0 >= sqrt(32 u).
+// Mapped to world space with sigma(w2o) |o_world - c| >= |o| and a factor
+// sigma(o2w) on distances, the children of a shape-tree node are tested
+// grown by the margin computed in trav::node_margins: shape_pad_k
+// (= K0 * max sigma(o2w) sigma(w2o)) bounds the first form for every shape
+// kind, the per-child factor h = sigma(o2w) sigma(w2o)^2 / (2 r) (the largest
+// over the child's subtree) the second for spheres, and shape_floor_k covers the
+// rounding of the ray's transform and of t.  Cylinders (the same quadratic
+// in two dimensions, then range tests on the hit point), disks, quads and
+// boxes keep the first form.
+constexpr float kSphereK0 = 2e-3f;
 // Traversal stack entries available per lane (LDS part plus private spill);
 // scene creation checks the tree's worst case against it.
 constexpr int kStackCapacity = 64;
@@ -39,10 +54,10 @@ struct DevScene {
   float root_lo[3], root_hi[3];
   int32_t tri_root;    // -1 if no triangles
   int32_t shape_root;  // -1 if no shapes
-  float shape_pad_k;
+  float shape_pad_k;    // K0 * max sigma(o2w) sigma(w2o) over the shapes
   int32_t num_nodes;
-  int32_t width;       // 2, 4 or 8 children per node
-  int32_t pad;
+  int32_t width;        // 2, 4 or 8 children per node
+  float shape_floor_k;  // rounding term of the shape margins
 };
 
 struct DevRays {

@@ -112,7 +112,8 @@ struct BatchSrc {
   uint8_t* occ;
   __device__ __forceinline__ bool begin(int k, DRay& r, trav::LaneScene& ls, trav::Best& best) {
     r = trav::load_ray(rays, k);
-    ls = trav::LaneScene{sc.nodes, sc.leaf, sc.shapes, sc.shape_pad_k, FAT ? sc.shape_root : sc.tri_root};
+    ls = trav::LaneScene{sc.nodes, sc.leaf, sc.shapes, sc.shape_pad_k, sc.shape_floor_k,
+                         FAT ? sc.shape_root : sc.tri_root};
     bool live = slab_haskell(sc.root_lo, sc.root_hi, r);
     best = trav::Best{r.tmax, 0.0f, 0.0f, -1};
     if (FAT) {

@@ -609,7 +609,8 @@ __device__ __forceinline__ const DevScene& scene_of(const PtKernelArgs& a, int i
   return a.scenes[a.jobs[a.pool.job[i]].scene].geo;
 }
 __device__ __forceinline__ trav::LaneScene lane_scene(const DevScene& g, bool fat) {
-  return trav::LaneScene{g.nodes, g.leaf, g.shapes, g.shape_pad_k, fat ? g.shape_root : g.tri_root};
+  return trav::LaneScene{g.nodes, g.leaf, g.shapes, g.shape_pad_k, g.shape_floor_k,
+                         fat ? g.shape_root : g.tri_root};
 }
 
 template <bool FAT>

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "bgpu_internal.h"
 
@@ -102,6 +103,21 @@ inline uint32_t fbits(float f) {
   std::memcpy(&u, &f, 4);
   return u;
 }
+// Upper bound of the largest singular value of the linear part of a
+// row-major 4x4 matrix: sigma^2 <= the largest absolute row sum of L^T L.
+double sigma_bound(const float* m) {
+  double best = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    double row = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double g = 0.0;
+      for (int i = 0; i < 3; ++i) g += (double)m[4 * i + j] * (double)m[4 * i + k];
+      row += std::fabs(g);
+    }
+    best = std::max(best, row);
+  }
+  return std::sqrt(best) * (1.0 + 1e-9);
+}
 inline float bitsf(int32_t i) {
   float f;
   std::memcpy(&f, &i, 4);
@@ -160,6 +176,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   std::vector<int32_t> order;
   BuildParams bp;
   float aniso = 1.0f;
+  std::vector<float> hfac;  // per shape: sigma(o2w) sigma(w2o)^2 / (2 r), +inf if not a sphere
   if (d->num_triangles) {
     std::vector<Box> tb(pb.begin(), pb.begin() + d->num_triangles);
     build_bvh2(tb, bp, &tnodes, &order, &sc->build);
@@ -180,16 +197,17 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     for (int32_t id : so) order.push_back(id + (int32_t)d->num_triangles);
     sc->build.leaves += sst.leaves;
     sc->build.max_depth = std::max(sc->build.max_depth, sst.max_depth);
-    // anisotropy of each o2w's linear part scales the false-hit radius
+    // transform factors of the false-hit margin (isect.h)
+    hfac.resize(d->num_shapes);
     for (uint32_t s = 0; s < d->num_shapes; ++s) {
-      const float* m = d->shapes[s].o2w;
-      float cn[3];
-      for (int c = 0; c < 3; ++c)
-        cn[c] = std::sqrt(m[c] * m[c] + m[4 + c] * m[4 + c] + m[8 + c] * m[8 + c]);
-      float mx = std::max(cn[0], std::max(cn[1], cn[2]));
-      float mn = std::min(cn[0], std::min(cn[1], cn[2]));
-      float a = (mn > 0.0f && std::isfinite(mx / mn)) ? mx / mn : 1000.0f;
-      aniso = std::max(aniso, std::min(a, 1000.0f));
+      const bgpu_shape& sh = d->shapes[s];
+      const double s1 = sigma_bound(sh.o2w), s2 = sigma_bound(sh.w2o);
+      const double a = std::isfinite(s1 * s2) ? std::min(s1 * s2, 1e6) : 1e6;
+      aniso = std::max(aniso, (float)(a * (1.0 + 1e-6)));
+      const double h = s1 * s2 * s2 / (2.0 * (double)sh.params[0]);
+      hfac[s] = (sh.type == BGPU_SHAPE_SPHERE && sh.params[0] > 0.0f && std::isfinite(h) && h < 1e30)
+                  ? (float)(h * (1.0 + 1e-6))
+                  : INFINITY;
     }
   }
   // the widest width whose worst-case traversal stack fits the device stack
@@ -212,9 +230,27 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   sc->build.nodes = (int)nodes.size();
 
   // node layout, 8 * width floats: lo.x[w] hi.x[w] lo.y[w] hi.y[w] lo.z[w]
-  // hi.z[w] link[w] pad[w]; unused slots carry the link kNoChild and repeat
-  // child 0's box (so that the union of a node's child boxes is its box)
+  // hi.z[w] link[w] h[w]; unused slots carry the link kNoChild and repeat
+  // child 0's box and h (so that the union of a node's child boxes is its box)
   const int W = width;
+  std::vector<float> hchild(nodes.size() * W, 0.0f);  // shape tree: max h below each child
+  std::function<float(int32_t)> hmax = [&](int32_t link) -> float {
+    float m = 0.0f;
+    if (link >= 0) {
+      const WNode& n = nodes[link];
+      for (size_t c = 0; c < n.link.size(); ++c) {
+        const float v = hmax(n.link[c]);
+        hchild[(size_t)link * W + c] = v;
+        m = std::max(m, v);
+      }
+      return m;
+    }
+    const int enc = ~link;
+    for (int k = enc >> kLeafCountBits; k < (enc >> kLeafCountBits) + (enc & kMaxLeaf); ++k)
+      m = std::max(m, hfac[order[k] - (int32_t)d->num_triangles]);
+    return m;
+  };
+  if (shape_root >= 0) (void)hmax(shape_root);
   std::vector<float> hn((size_t)8 * W * std::max<size_t>(nodes.size(), 1), 0.0f);
   for (size_t i = 0; i < nodes.size(); ++i) {
     const WNode& n = nodes[i];
@@ -227,6 +263,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         o[(2 * k + 1) * W + c] = b.hi[k];
       }
       o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
+      o[7 * W + c] = hchild[i * W + (used ? c : 0)];
     }
   }
   std::vector<float> hl(12 * std::max<size_t>(order.size(), 1), 0.0f);
@@ -280,9 +317,8 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   ds.width = width;
   ds.tri_root = tri_root;
   ds.shape_root = shape_root;
-  // 3*sqrt(2^-24) = 7.3e-4 bounds the quadratic's false-hit radius per unit
-  // of origin distance; 2e-3 leaves a margin for the transformed origin.
-  ds.shape_pad_k = 2e-3f * aniso;
+  ds.shape_pad_k = kSphereK0 * aniso;
+  ds.shape_floor_k = 64.0f * 0x1p-24f * aniso;
   sc->film = d->film;
   sc->camera = d->camera;
   return BGPU_OK;

@@ -86,13 +86,18 @@ __device__ __forceinline__ void slab(float lx, float hx, float ly, float hy, flo
   *tn = n;
 }
 
-// Margin of the children of a shape-tree node for a ray from o: shape_pad_k
-// times the L1 distance from o to the farthest corner of the node's box (the
-// union of its children), an upper bound of |o - centre| for every shape
-// below the node.  Unused child slots repeat child 0's box, so the union is
-// exact.
+// Margins of the children of a shape-tree node for a ray from o (see the
+// false-hit bound in isect.h).  D is the distance from o to the farthest
+// corner of the node's box (the union of its children; unused child slots
+// repeat child 0's box, so the union is exact), an upper bound of |o - c| for
+// the centre c of every shape below the node.  Child c gets
+//   max(min(k * D, kSphereK0^2 * D^2 * h_c), floor_k * (|o|_1 + |box|_1))
+// with h_c = the largest sigma(o2w) * sigma(w2o)^2 / (2 r) of the subtree's
+// spheres (+inf when it holds another shape kind), stored in the node's
+// last W floats, and |box|_1 the L1 norm of the box corner farthest from the
+// world origin (rounding of the ray's transform and of t).
 template <int W>
-__device__ __forceinline__ float node_margin(const float* v, f3 o, float k) {
+__device__ __forceinline__ void node_margins(const float* v, f3 o, float k, float floor_k, float* pad) {
   float lx = v[0], hx = v[W], ly = v[2 * W], hy = v[3 * W], lz = v[4 * W], hz = v[5 * W];
 #pragma unroll
   for (int c = 1; c < W; ++c) {
@@ -103,7 +108,13 @@ __device__ __forceinline__ float node_margin(const float* v, f3 o, float k) {
   const float fx = fmaxf(fabsf(o.x - lx), fabsf(o.x - hx));
   const float fy = fmaxf(fabsf(o.y - ly), fabsf(o.y - hy));
   const float fz = fmaxf(fabsf(o.z - lz), fabsf(o.z - hz));
-  return k * (fx + fy + fz);
+  const float d2 = (fx * fx + fy * fy + fz * fz) * 1.0001f;  // rounded up
+  const float cap = k * __builtin_sqrtf(d2) * 1.0001f;
+  const float q = kSphereK0 * kSphereK0 * d2;
+  const float ext = fmaxf(fabsf(lx), fabsf(hx)) + fmaxf(fabsf(ly), fabsf(hy)) + fmaxf(fabsf(lz), fabsf(hz));
+  const float fl = floor_k * (fabsf(o.x) + fabsf(o.y) + fabsf(o.z) + ext);
+#pragma unroll
+  for (int c = 0; c < W; ++c) pad[c] = fmaxf(fminf(cap, q * v[7 * W + c]), fl);
 }
 __device__ __forceinline__ void inflate(float& lx, float& hx, float& ly, float& hy, float& lz,
                                         float& hz, float p) {
@@ -157,13 +168,14 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
       const float4* nd = sc.nodes + (size_t)cur * (2 * W);
       float v[8 * W];
 #pragma unroll
-      for (int q = 0; q < (7 * W + 3) / 4; ++q) {
+      for (int q = 0; q < (FAT ? 2 * W : (7 * W + 3) / 4); ++q) {
         const float4 x = nd[q];
         v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
       }
       if (STATS) ++nvisit;
       const float tlim = ANY ? r.tmax : best.t;
-      const float pad = FAT ? node_margin<W>(v, r.o, sc.shape_pad_k) : 0.0f;
+      float pad[W];
+      if (FAT) node_margins<W>(v, r.o, sc.shape_pad_k, sc.shape_floor_k, pad);
       float key[W];
       int lk[W];
       int nhit = 0;
@@ -172,7 +184,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
         float lx = v[c], hx = v[W + c], ly = v[2 * W + c], hy = v[3 * W + c], lz = v[4 * W + c],
               hz = v[5 * W + c];
         lk[c] = __float_as_int(v[6 * W + c]);
-        if (FAT) inflate(lx, hx, ly, hy, lz, hz, pad);
+        if (FAT) inflate(lx, hx, ly, hy, lz, hz, pad[c]);
         bool h;
         float tn;
         slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h, &tn);

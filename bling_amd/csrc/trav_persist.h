@@ -43,7 +43,7 @@ struct LaneScene {
   const float4* nodes;
   const float4* leaf;
   const DShape* shapes;
-  float pad_k;
+  float pad_k, floor_k;
   int root;  // root of the pass's tree, -1 when absent
 };
 
@@ -58,7 +58,7 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
   int item = -1, cur = kNoChild;
   int pend = kNoChild;  // postponed leaf, kNoChild when none
   DRay r;
-  LaneScene ls{nullptr, nullptr, nullptr, 0.0f, -1};
+  LaneScene ls{nullptr, nullptr, nullptr, 0.0f, 0.0f, -1};
   f3 inv = mk3(0.0f, 0.0f, 0.0f), oi = inv;
   Best best{0.0f, 0.0f, 0.0f, -1};
   unsigned long long nvisit = 0, ntest = 0;
@@ -121,13 +121,14 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
       const float4* nd = ls.nodes + (size_t)cur * (2 * W);
       float v[8 * W];
 #pragma unroll
-      for (int q = 0; q < (7 * W + 3) / 4; ++q) {
+      for (int q = 0; q < (FAT ? 2 * W : (7 * W + 3) / 4); ++q) {
         const float4 x = nd[q];
         v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
       }
       if (STATS) ++nvisit;
       const float tlim = ANY ? r.tmax : best.t;
-      const float pad = FAT ? node_margin<W>(v, r.o, ls.pad_k) : 0.0f;
+      float pad[W];
+      if (FAT) node_margins<W>(v, r.o, ls.pad_k, ls.floor_k, pad);
       float key[W];
       int lk[W];
       int nhit = 0;
@@ -136,7 +137,7 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
         float lx = v[c], hx = v[W + c], ly = v[2 * W + c], hy = v[3 * W + c], lz = v[4 * W + c],
               hz = v[5 * W + c];
         lk[c] = __float_as_int(v[6 * W + c]);
-        if (FAT) inflate(lx, hx, ly, hy, lz, hz, pad);
+        if (FAT) inflate(lx, hx, ly, hy, lz, hz, pad[c]);
         bool h;
         float tn;
         slab(lx, hx, ly, hy, lz, hz, inv, oi, r.tmin, tlim, &h, &tn);

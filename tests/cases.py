@@ -105,3 +105,43 @@ def materials_scene(res=48, env=True, seed=3):
     sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 45.0, res, res))
     sb.set_film(res, res)
     return sb.build()
+
+
+def grazing_spheres(n_sph=400, n=60000, seed=17, k0=2e-3, scaled=False):
+    """Spheres of radius 1e-4..0.05 on a grid and rays that graze them from
+    1..200 units away, at line distances from r - K/2 to r + 2K with
+    K = min(k0 |o - c|, k0^2 |o - c|^2 / (2 r)): the regime where the
+    reference's binary32 sphere test reports hits outside the surface.
+    `scaled`: the spheres carry a rotation and a non-uniform scale (the
+    stated radius is then the object-space one, and the targets are aimed at
+    the unscaled centre, so the grazing distances are approximate).
+    Returns (flat scene, rays (n, 8), centres (n_sph, 3), radii (n_sph,))."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n_sph]
+    cen = (g * 3.0).astype(np.float64) + rng.uniform(-0.5, 0.5, (n_sph, 3))
+    cen = cen.astype(np.float32).astype(np.float64)
+    rad = np.exp(rng.uniform(np.log(1e-4), np.log(0.05), n_sph)).astype(np.float32).astype(np.float64)
+    sb = SceneBuilder()
+    m = sb.material(Material.matte(const(0.5)))
+    for k, (c, r) in enumerate(zip(cen, rad)):
+        xf = T.translate(tuple(float(x) for x in c))
+        if scaled:
+            xf = (T.scale((1.0, 1.0 + 0.5 * (k % 3), 1.0)) @ T.rotate_x(float(rng.uniform(0, 360)))
+                  @ T.rotate_y(float(rng.uniform(0, 360))) @ xf)
+        sb.shape(abi.SHAPE_SPHERE, (float(r),), o2w=xf, material=m)
+    tgt = rng.integers(n_sph, size=n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = np.cross(u, rng.normal(size=(n, 3)))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    L = np.exp(rng.uniform(np.log(1.0), np.log(200.0), n))
+    r = rad[tgt]
+    bound = np.minimum(k0 * L, k0 * k0 * L * L / (2 * r))
+    off = r + bound * rng.uniform(-0.5, 2.0, n)
+    org = cen[tgt] + u * L[:, None]
+    d = (cen[tgt] + w * off[:, None]) - org
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = org
+    rays[:, 3:6] = d
+    rays[:, 7] = np.inf
+    return sb.build(), rays, cen, rad

@@ -261,3 +261,23 @@ def test_autotuner_picks_a_valid_config(gpu_ctx):
     finally:
         for k, v in keep.items():
             gpu_ctx.config_set(k, v)
+
+
+@pytest.mark.parametrize("scaled", [False, True])
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_grazing_sphere_hits_bit_exact(gpu_ctx, width, scaled):
+    """Rays that graze small spheres from far away, where the reference's
+    binary32 sphere test reports hits outside the surface: the shape tree's
+    grown boxes keep every such hit (plain and rotated, non-uniformly scaled
+    spheres; every BVH width)."""
+    from tests import cases
+    flat, rays, _, _ = cases.grazing_spheres(n=40000, seed=23 + width, scaled=scaled)
+    prev = gpu_ctx.config_get("bvh.width")
+    gpu_ctx.config_set("bvh.width", width)
+    try:
+        g, o, gocc, oocc = run_both(gpu_ctx, flat, rays, brute=True)
+    finally:
+        gpu_ctx.config_set("bvh.width", prev)
+    assert_same(g, o, "grazing/w%d/%s" % (width, "scaled" if scaled else "plain"))
+    assert (gocc == oocc).all(), "any-hit mismatches: %d" % (gocc != oocc).sum()
+    assert (g[3] >= 0).sum() > 1000
