@@ -755,6 +755,65 @@ __global__ __launch_bounds__(BLOCK) void k_mis_resolve(PtKernelArgs a) {
   });
 }
 
+// Lane of the (r+1)-th set bit of m (r < popcount(m)).
+__device__ __forceinline__ int nth_set_lane(unsigned long long m, int r) {
+  int lo = 0, hi = 63;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (__popcll(m & ((2ull << mid) - 1ull)) > r) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Film splats whose footprint is one pixel (the box filter's) are merged
+// within the wave before the atomics: the wave's splats are gathered in lane
+// order to lanes 0..cnt-1, runs of equal (job slot, pixel) keys are summed by
+// a segmented scan, and the last lane of each run adds the sums.  Samples of
+// a pixel are consecutive work indices, so they often finish in neighbouring
+// lanes, whose same-address atomics would otherwise serialise.
+__device__ __forceinline__ void merged_splat(const PtKernelArgs& a, bool single, unsigned long long key,
+                                             float v[7]) {
+  const unsigned long long part = __ballot(single);
+  if (part == 0ull) return;
+  const int lane = __lane_id();
+  const int cnt = __popcll(part);
+  const bool have = lane < cnt;
+  const int src = have ? nth_set_lane(part, lane) : lane;
+  const unsigned long long k = __shfl(key, src);
+  float u[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) u[c] = __shfl(v[c], src);
+  const unsigned long long kp = __shfl_up(k, 1);
+  const bool head = have && (lane == 0 || kp != k);
+  const unsigned long long heads = __ballot(head);
+  const int start = have ? 63 - __clzll(heads & ((2ull << lane) - 1ull)) : lane;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      const float t = __shfl_up(u[c], off);
+      if (lane - off >= start) u[c] += t;
+    }
+  }
+  const bool tail = have && (lane == cnt - 1 || (lane < 63 && ((heads >> (lane + 1)) & 1ull)));
+  if (tail) {
+    const DevJob& J = a.jobs[(int)(k >> 32)];
+    const size_t pix = (size_t)(k & 0xffffffffull);
+    float* o = J.wxyz + 4 * pix;
+    atomicAdd(o, u[0]);
+    atomicAdd(o + 1, u[1]);
+    atomicAdd(o + 2, u[2]);
+    atomicAdd(o + 3, u[3]);
+    if (J.sq) {
+      float* s2 = J.sq + 3 * pix;
+      atomicAdd(s2, u[4]);
+      atomicAdd(s2 + 1, u[5]);
+      atomicAdd(s2 + 2, u[6]);
+    }
+  }
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
   __shared__ unsigned long long s_key[BLOCK / 64 * kGroups];
@@ -768,8 +827,12 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
   const int i = (unsigned)q < n ? (int)P.q_act[q] : 0;
   const bool term = (unsigned)q < n && P.state[i] == 2;
   unsigned int* ctr = nullptr;
+  bool single = false;  // a one-pixel splat, merged below
+  unsigned long long key = 0ull;
+  float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (term) {
-    const DevJob& J = a.jobs[P.job[i]];
+    const int slot = P.job[i];
+    const DevJob& J = a.jobs[slot];
     ctr = J.pass_done + P.pass[i];
     if (!J.cancelled) {
       // a NaN or infinite band makes its XYZ sums NaN or infinite, so the
@@ -789,23 +852,40 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
         y0 = y0 < 0 ? 0 : y0;
         x1 = x1 > J.view.film_w - 1 ? J.view.film_w - 1 : x1;
         y1 = y1 > J.view.film_h - 1 ? J.view.film_h - 1 : y1;
-        for (int y = y0; y <= y1; ++y) {
-          int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
+        if (x0 == x1 && y0 == y1) {
+          int fy = (int)floorf(fabsf(((float)y0 - dy) * ifh * 16.0f));
           fy = fy < 15 ? fy : 15;
-          for (int x = x0; x <= x1; ++x) {
-            int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
-            fx = fx < 15 ? fx : 15;
-            float w = J.view.filter[fy * 16 + fx];
-            float* o = J.wxyz + 4 * ((size_t)y * J.view.film_w + x);
-            atomicAdd(o, w);
-            atomicAdd(o + 1, X * w);
-            atomicAdd(o + 2, Y * w);
-            atomicAdd(o + 3, Z * w);
-            if (J.sq) {
-              float* s2 = J.sq + 3 * ((size_t)y * J.view.film_w + x);
-              atomicAdd(s2, X * X * w);
-              atomicAdd(s2 + 1, Y * Y * w);
-              atomicAdd(s2 + 2, Z * Z * w);
+          int fx = (int)floorf(fabsf(((float)x0 - dx) * ifw * 16.0f));
+          fx = fx < 15 ? fx : 15;
+          const float w = J.view.filter[fy * 16 + fx];
+          single = true;
+          key = ((unsigned long long)slot << 32) | ((unsigned)y0 * (unsigned)J.view.film_w + (unsigned)x0);
+          v[0] = w;
+          v[1] = X * w;
+          v[2] = Y * w;
+          v[3] = Z * w;
+          v[4] = X * X * w;
+          v[5] = Y * Y * w;
+          v[6] = Z * Z * w;
+        } else {
+          for (int y = y0; y <= y1; ++y) {
+            int fy = (int)floorf(fabsf(((float)y - dy) * ifh * 16.0f));
+            fy = fy < 15 ? fy : 15;
+            for (int x = x0; x <= x1; ++x) {
+              int fx = (int)floorf(fabsf(((float)x - dx) * ifw * 16.0f));
+              fx = fx < 15 ? fx : 15;
+              float w = J.view.filter[fy * 16 + fx];
+              float* o = J.wxyz + 4 * ((size_t)y * J.view.film_w + x);
+              atomicAdd(o, w);
+              atomicAdd(o + 1, X * w);
+              atomicAdd(o + 2, Y * w);
+              atomicAdd(o + 3, Z * w);
+              if (J.sq) {
+                float* s2 = J.sq + 3 * ((size_t)y * J.view.film_w + x);
+                atomicAdd(s2, X * X * w);
+                atomicAdd(s2 + 1, Y * Y * w);
+                atomicAdd(s2 + 2, Z * Z * w);
+              }
             }
           }
         }
@@ -813,6 +893,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
     }
     P.state[i] = 0;
   }
+  merged_splat(a, single, key, v);
   block_increment<BLOCK>(ctr, term, s_key, s_cnt, s_n);
 }
 
