@@ -1,0 +1,199 @@
+"""The `.bling` reader (bling_amd/bling_format.py) against the grammar of
+src/lib/Graphics/Bling/IO/*.hs and against scenes built directly.
+
+- A Cornell box written in .bling (tests/golden/cornell_c1.bling) flattens
+  to the arrays of scenes.cornell(256): vertices, indices and shape records
+  bit for bit, materials per primitive, the camera value for value.
+- Grammar pieces: numbers rounded like GHC's `read`, transform composition,
+  polygon fans, spectra (rgbR/rgbI/spd/temp/hex), statement and light
+  order, defaults of the start state, errors.
+- The reference's own example scenes (read in place when the reference
+  checkout is present): the five the GPU path renders build, with the
+  primitive counts their text describes; the others fail either as
+  Unsupported or, where the reference's current grammar rejects them too,
+  as syntax errors.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from bling_amd import abi, scenes
+from bling_amd import bling_format as B
+from bling_amd import transform as T
+from bling_amd.scene import rgb_illum, rgb_refl, spd
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_EXAMPLES = "/root/reference/examples"
+F = np.float32
+
+
+def _mat_of_prims(flat):
+    """Material record (type, scalars, spectra) per primitive: triangles
+    through their mesh, then shapes."""
+    mats = [bytes(flat.materials[i]) for i in range(flat.desc.num_materials)]
+    tri = [mats[flat.meshes[m].material] for m in flat.tri_mesh]
+    return tri + [mats[flat.shapes[s].material] for s in range(flat.num_shapes)]
+
+
+def test_cornell_fixture_flattens_like_the_procedural_scene():
+    b = B.load(os.path.join(GOLDEN, "cornell_c1.bling"))
+    a = scenes.cornell(256)
+    f = b.flat
+    assert np.array_equal(a.vertices.view(np.int32), f.vertices.view(np.int32))
+    assert np.array_equal(a.indices, f.indices)
+    assert a.num_shapes == f.num_shapes == 1
+    sa, sf = a.shapes[0], f.shapes[0]
+    for name, _ in abi.Shape._fields_:
+        if name != "material":
+            assert bytes(getattr(sa, name)) == bytes(getattr(sf, name)) if hasattr(getattr(sa, name), "_length_") \
+                else getattr(sa, name) == getattr(sf, name), name
+    assert _mat_of_prims(a) == _mat_of_prims(f)
+    # the reference composes every transform block with the current one, so
+    # the camera matrix may carry -0.0 where the direct build has 0.0
+    for name in ("c2w", "r2c"):
+        assert list(getattr(a.desc.camera, name)) == list(getattr(f.desc.camera, name))
+    assert bytes(a.desc.film) == bytes(f.desc.film)
+    s = b.spec()
+    assert (s.spp_nu, s.spp_nv, s.max_depth, s.sample_depth, s.rr_start_depth) == (4, 4, 5, 3, 7)
+    v = f.view(128, 96)
+    assert v.size == (128, 96)
+
+
+def test_numbers_round_like_ghc_read():
+    # decimal -> binary32 in one rounding; a double-rounding case
+    # (1 + 2^-24 + 2^-60: nearest double is the midpoint, which rounds to even)
+    assert B.f32("1.0000000596046447753906250000000000000000000000000000000000000867") == F(1.0000001)
+    assert B.f32("0.1") == F(0.1)
+    p = B._Parser("imageSize 320 200 filter mitchell 2 2.5 0.333333 0.333333")
+    p.run()
+    assert p.res == (320, 200)
+    assert p.filter == (abi.FILTER_MITCHELL, (F(2), F(2.5), F(0.333333), F(0.333333)))
+
+
+def test_transform_blocks_compose_like_the_reference():
+    p = B._Parser("transform { rotateX 30 translate 1 2 3 } transform { scale 2 2 2 }")
+    p.run()
+    want = T.scale((2, 2, 2)) @ (T.rotate_x(30) @ T.translate((1, 2, 3)) @ T.Transform())
+    assert p.transform.m == want.m and p.transform.inv == want.inv
+    p = B._Parser("transform { rotateX 30 } newTransform { translate 1 0 0 }")
+    p.run()
+    assert p.transform.m == (T.translate((1, 0, 0)) @ T.Transform()).m
+    p = B._Parser("transform { matrix { m 1 0 0 4 m 0 1 0 5 m 0 0 1 6 m 0 0 0 1 } }")
+    p.run()
+    assert p.transform.m == (T.from_matrix([[1, 0, 0, 4], [0, 1, 0, 5], [0, 0, 1, 6], [0, 0, 0, 1]])
+                             @ T.Transform()).m
+
+
+def test_polygon_faces_become_fans():
+    b = B.parse("prim { mesh vertexCount 5 faceCount 2 v 0 0 0 v 1 0 0 v 1 1 0 v 0 1 0 v 2 2 0 "
+                "f 0 1 2 3 4 f 4 3 2 }")
+    assert b.flat.indices.tolist() == [[0, 1, 2], [0, 2, 3], [0, 3, 4], [4, 3, 2]]
+
+
+def test_spectra():
+    p = B._Parser("")
+    p.toks = B._tokens("rgbR 0.5 0.25 1 rgbI %ff8000 spd { 500 1, 400 0.5 } temp 3000")
+    assert np.array_equal(p.spectrum(), rgb_refl(0.5, 0.25, 1.0))
+    assert np.array_equal(p.spectrum(), rgb_illum(F(1), F(F(128) / F(255)), F(0)))
+    assert np.array_equal(p.spectrum(), spd([(400, 0.5), (500, 1)]))
+    bb = p.spectrum()
+    # Planck at 3000 K rises across 400-700 nm; band 0 is the mean of the
+    # values at 400 and 418.75 nm
+    assert np.all(np.diff(bb) > 0)
+    pl = lambda w: 0.4e-9 * 3.74183e-16 * (w * 1e-9) ** -5 / (np.exp(1.4388e-2 / (w * 1e-9 * 3000)) - 1)
+    assert abs(bb[0] / (0.5 * (pl(400) + pl(418.75))) - 1) < 1e-5
+    assert abs(bb[15] / (0.5 * (pl(681.25) + pl(700))) - 1) < 1e-5
+
+
+def test_statement_order_emission_and_light_ids():
+    b = B.parse("""
+        material { matte kd { constant rgbR 0.5 0.5 0.5 } sigma { constant 0.3 } }
+        prim { shape { sphere radius 1 } }
+        emission { rgbI 2 2 2 }
+        prim { shape { cylinder radius 1 zmin 2 zmax -1 phiMax 400 } }
+        emission { none }
+        prim { shape { disk height 0.5 radius 0.2 innerRadius 1 phiMax 90 } }
+        prim { mesh vertexCount 3 faceCount 1 v 0 0 0 v 1 0 0 v 0 1 0 f 0 1 2 }
+        light { infinite { rotateX 90 } l { constant rgbI 1 1 1 } }
+        light { infinite { } l { constant rgbI 3 3 3 } }
+    """)
+    f = b.flat
+    kinds = [f.shapes[i].type for i in range(f.num_shapes)]
+    assert kinds == [abi.SHAPE_DISK, abi.SHAPE_CYLINDER, abi.SHAPE_SPHERE]  # last statement first
+    assert [f.shapes[i].light_id for i in range(3)] == [2, 1, 0]          # the parse-order counter
+    assert [f.shapes[i].emissive for i in range(3)] == [0, 1, 0]
+    cyl, disk = f.shapes[1], f.shapes[0]
+    assert list(cyl.params)[:4] == [F(1), F(-1), F(2), T.radians(F(360))]  # mkCylinder orders z, clamps phi
+    assert list(disk.params)[:4] == [F(0.5), F(1), F(0.2), T.radians(F(90))]
+    m = f.materials[f.shapes[2].material]
+    assert m.type == abi.MAT_MATTE and m.scalar[0] == F(0.3)
+    assert f.num_triangles == 1
+    assert [lm[0, 0] for lm in f.light_maps] == [F(rgb_illum(3, 3, 3)[0]), F(rgb_illum(1, 1, 1)[0])]
+
+
+def test_start_state_defaults():
+    b = B.parse("prim { shape { sphere radius 1 } }")
+    assert (b.width, b.height) == (640, 480)
+    s = b.spec()
+    assert (s.spp_nu, s.spp_nv, s.max_depth, s.sample_depth) == (2, 2, 7, 3)
+    c2w, r2c = T.perspective_camera(T.translate((0, 0, -5)), 0, 1, 90, 640, 480)
+    assert list(b.flat.desc.camera.c2w) == c2w.m and list(b.flat.desc.camera.r2c) == r2c.m
+    m = b.flat.materials[0]
+    assert m.type == abi.MAT_MATTE and list(m.spec[0]) == list(rgb_refl(0.9, 0.9, 0.9))
+
+
+def test_errors():
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("renderer { sampler sampled { sampler { stratified xSamples 2 ySamples 2 } } }")
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("prim { shape { torus radius 1 } }")
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("imageSize 10")
+    with pytest.raises(B.Unsupported):
+        B.parse("prim { heightMap 4 4 { fbm 0 octaves 3 omega 0.5 } { } }")
+    with pytest.raises(B.Unsupported):
+        B.parse("material { matte kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
+    with pytest.raises(B.Unsupported):
+        B.parse("camera { environment }")
+    with pytest.raises(ValueError, match="out of bounds"):
+        B.parse("prim { mesh vertexCount 3 faceCount 1 v 0 0 0 v 1 0 0 v 0 1 0 f 0 1 5 }")
+    b = B.parse("renderer { sppm photonCount 100 maxDepth 5 radius 1 } prim { shape { sphere radius 1 } }")
+    with pytest.raises(B.Unsupported):
+        b.spec()
+
+
+EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit lights, size)
+    "cornell-box.bling": (30, 1, 0, (480, 480)),
+    "cornell-box-specular.bling": (10, 3, 0, (480, 480)),
+    "matte-test.bling": (2, 7, 0, (1280, 720)),
+    "plastic-test.bling": (2, 7, 0, (640, 360)),
+    "blackbody-emission.bling": (0, 16, 0, (640, 400)),
+}
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference checkout not present")
+def test_reference_examples():
+    seen = {}
+    for path in sorted(glob.glob(os.path.join(REF_EXAMPLES, "*.bling"))):
+        name = os.path.basename(path)
+        try:
+            b = B.load(path)
+            seen[name] = (b.flat.num_triangles, b.flat.num_shapes, len(b.flat.light_maps), (b.width, b.height))
+        except B.Unsupported:
+            seen[name] = "unsupported"
+        except B.BlingSyntaxError:
+            seen[name] = "syntax"
+    for name, want in EXPECT.items():
+        assert seen[name] == want, (name, seen[name])
+    assert sum(v == "unsupported" for v in seen.values()) >= 10
+    # cornell-box.bling: camera from its lookAt, film from its first filter
+    b = B.load(os.path.join(REF_EXAMPLES, "cornell-box.bling"))
+    c2w, _ = T.perspective_camera(T.look_at((278, 273, -800), (278, 273, 0), (0, 1, 0)) @ T.Transform(),
+                                  0, 10, 37.5, 480, 480)
+    assert list(b.flat.desc.camera.c2w) == c2w.m
+    assert b.flat.desc.film.filter == abi.FILTER_MITCHELL
+    with pytest.raises(B.Unsupported):  # its last renderer block is sppm
+        b.spec()
