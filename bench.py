@@ -97,7 +97,7 @@ def main():
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
     for k in ("bvh.width", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves",
-              "pt.persistent"):
+              "pt.persistent", "pt.sort"):
         knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,

@@ -139,9 +139,9 @@ struct Engine {
     cfg = config;
     P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 9 int arrays + 52 float arrays of P, plus 3 queues
+    // pool layout: 9 int arrays + 52 float arrays of P, plus 5 queues
     size_t per = (size_t)P * 4;
-    size_t total = per * (9 + 52 + 3);
+    size_t total = per * (9 + 52 + 5);
     if (int rc = pool_buf.alloc(c, total)) return rc;
     char* b = pool_buf.as<char>();
     auto take = [&](size_t n) {
@@ -172,6 +172,8 @@ struct Engine {
     pp.q_act = (uint32_t*)take(1);
     pp.q_shadow = (uint32_t*)take(1);
     pp.q_mis = (uint32_t*)take(1);
+    pp.q_sh = (uint32_t*)take(1);
+    pp.q_cls = (uint32_t*)take(1);
     if (int rc = jobs_buf.alloc(c, sizeof(DevJob) * kMaxJobs)) return rc;
     if (int rc = ring_buf.alloc(c, sizeof(RingChunk) * kRing)) return rc;
     if (int rc = scenes_buf.alloc(c, sizeof(DevPtScene) * kMaxScenes)) return rc;
@@ -193,14 +195,15 @@ struct Engine {
     for (int i = kMaxJobs - 1; i >= 0; --i) free_slots.push_back(i);
     BGPU_HIP(hipMemsetAsync(prog_buf.ptr, 0, sizeof(uint32_t) * kSkipWords, stream));
     BGPU_HIP(pt_enqueue_init(args, stream));
-    if (int rc = capture(c->bvh_width)) return rc;
+    if (int rc = capture(c->bvh_width, cfg.sort == 1)) return rc;
     BGPU_HIP(hipStreamSynchronize(stream));
     return BGPU_OK;
   }
 
   // Captures one bounce iteration whose traversal kernels are specialised
-  // for BVH width w (0: any width).
-  int capture(int w) {
+  // for BVH width w (0: any width), shading in class-grouped order if `sort`.
+  int capture(int w, bool sort) {
+    args.sort_shade = sort ? 1 : 0;
     if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
     if (graph) BGPU_HIP(hipGraphDestroy(graph));
     gexec = nullptr;
@@ -294,9 +297,18 @@ struct Engine {
     if (!can_admit(j)) return fail(BGPU_E_BUDGET, "job table or progress counters full");
     // a scene of another width than the captured kernels assume: fall back
     // to kernels that take any width (the stream is idle between iterations)
-    if (width != 0 && j->scene->dev.width != width) {
+    const int want_w = (width != 0 && j->scene->dev.width != width) ? 0 : width;
+    // shading-queue grouping (pt.sort 2): on while a resident scene mixes
+    // BxDF makeups (the table holds only scenes with jobs in flight)
+    bool want_sort = cfg.sort == 1;
+    if (cfg.sort == 2) {
+      want_sort = j->scene->pt.shade_kinds > 1;
+      for (Scene* s : scenes)
+        if (s && s->pt.shade_kinds > 1) want_sort = true;
+    }
+    if (want_w != width || want_sort != (args.sort_shade != 0)) {
       BGPU_HIP(hipStreamSynchronize(stream));
-      if (int rc = capture(0)) return rc;
+      if (int rc = capture(want_w, want_sort)) return rc;
     }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
@@ -873,6 +885,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.shade_waves") {
     if (value != 0 && value != 2) return fail(BGPU_E_INVALID, "pt.shade_waves must be 0 or 2");
     next.shade_waves = (int)value;
+  } else if (k == "pt.sort") {
+    if (value < 0 || value > 2) return fail(BGPU_E_INVALID, "pt.sort must be 0, 1 or 2");
+    next.sort = (int)value;
   } else if (k == "pt.persistent") {
     if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "pt.persistent must be 0 or 1");
     next.persistent = (int)value;
@@ -905,6 +920,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
   else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
+  else if (k == "pt.sort") *value = ctx->pt_cfg.sort;
   else return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   return BGPU_OK;
 }

@@ -81,12 +81,18 @@ struct DevJob {
 };
 
 // Device counters of the engine.
+// Classes of the shading work a path needs this iteration (shade_class):
+// the shade queue is grouped by class so that a wave runs one code path.
+constexpr int kShadeClasses = 32;
+
 struct PtCounters {
   unsigned long long work_next, work_total;
   unsigned int n_act, n_shadow, n_mis, last_act;
   unsigned long long tot_camera, tot_extend, tot_shadow, tot_mis, tot_samples;
   unsigned int ring_head, ring_tail;  // chunks not yet fully taken [head, tail)
   unsigned int iter, pad;
+  unsigned int cls_count[kShadeClasses];   // active paths per shading class
+  unsigned int cls_cursor[kShadeClasses];  // slots of the grouped queue taken per class
 };
 
 // A chunk of a job's samples in the monotonic 64-bit work space: work
@@ -127,6 +133,8 @@ struct PathPool {
   uint32_t* q_act;
   uint32_t* q_shadow;
   uint32_t* q_mis;
+  uint32_t* q_sh;   // q_act grouped by shading class (when sorting)
+  uint32_t* q_cls;  // shading class of q_act[q]
 };
 
 struct PtConfig {
@@ -134,6 +142,8 @@ struct PtConfig {
   int pool = 1 << 20;
   int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2 waves per SIMD)
   int persistent = 0;   // traversal passes with lane refill (1) or one ray per thread (0)
+  int sort = 2;         // shading queue grouped by material kind: 1 always, 0 never, 2 while a
+                        // resident scene has more than one BxDF makeup
 };
 
 struct PtKernelArgs {
@@ -143,12 +153,14 @@ struct PtKernelArgs {
   RingChunk* ring;  // chunks in work order (kRing entries, modulo)
   PtCounters* cnt;
   AdmitReq* admit;
+  int32_t sort_shade, pad;  // shade reads q_sh (grouped by class) instead of q_act
 };
 
 // Host-side shading data owned by a Scene.
 struct PtScene {
   std::vector<int32_t> shape_light_host;  // device light index per shape, -1 if none
   DevPtScene host{};                      // filled by build_pt_data, pointers are device
+  int shade_kinds = 0;                    // distinct BxDF makeups of the materials in use
 };
 
 // Fills a job view from a camera and film description.

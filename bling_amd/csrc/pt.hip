@@ -187,6 +187,14 @@ __global__ void k_reset(PtKernelArgs a) {
   while (c->ring_head != c->ring_tail && a.ring[c->ring_head % kRing].wend <= c->work_next) c->ring_head++;
 }
 
+// Zeroes the shading-class counters (one thread per class).
+__global__ void k_reset_classes(PtKernelArgs a) {
+  if (threadIdx.x < kShadeClasses) {
+    a.cnt->cls_count[threadIdx.x] = 0u;
+    a.cnt->cls_cursor[threadIdx.x] = 0u;
+  }
+}
+
 // Chunk holding work index w: the last ring entry whose wbegin <= w.
 __device__ __forceinline__ const RingChunk& find_chunk(const PtKernelArgs& a, unsigned long long w,
                                                        unsigned int head, unsigned int tail) {
@@ -344,6 +352,74 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
   });
 }
 
+// Shading class of active path i: 0 escaped, 1 at the depth cutoff, 2 no
+// BxDF, then one class per BxDF makeup of the hit's material (the kind of
+// its first component, or of its two components), so that grouped waves run
+// one shading code path.
+__device__ __forceinline__ int shade_class(const PtKernelArgs& a, int i) {
+  const PathPool& P = a.pool;
+  const int prim = P.hprim[i];
+  if (prim < 0) return 0;
+  const DevJob& J = a.jobs[P.job[i]];
+  if ((P.dspec[i] & 0xffff) == J.md) return 1;
+  const DevPtScene& S = a.scenes[J.scene];
+  const int m = prim < S.ntri ? S.tri_info[prim].x : S.geo.shapes[prim - S.ntri].material;
+  if (m < 0 || m >= S.nmats || S.mats[m].n == 0) return 2;
+  const DMaterial& M = S.mats[m];
+  const int c = M.n == 1 ? 3 + M.c[0].kind : 8 + 5 * M.c[0].kind + M.c[1].kind;
+  return c < kShadeClasses ? c : kShadeClasses - 1;
+}
+
+// Grouping the shade queue by class: counts per class (one atomic per class
+// per block), then each block reserves its range of every class and writes
+// its paths there (order within a class is arbitrary).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_class_count(PtKernelArgs a) {
+  __shared__ unsigned int h[kShadeClasses];
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
+  if (threadIdx.x < kShadeClasses) h[threadIdx.x] = 0u;
+  __syncthreads();
+  if ((unsigned)q < n) {
+    const int c = shade_class(a, (int)a.pool.q_act[q]);
+    a.pool.q_cls[q] = (uint32_t)c;
+    atomicAdd(&h[c], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kShadeClasses && h[threadIdx.x] != 0u)
+    atomicAdd(&a.cnt->cls_count[threadIdx.x], h[threadIdx.x]);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_class_scatter(PtKernelArgs a) {
+  static_assert(BLOCK >= 64, "class bases are scanned by the first wave");
+  __shared__ unsigned int h[kShadeClasses], base[kShadeClasses];
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
+  if (threadIdx.x < kShadeClasses) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const bool valid = (unsigned)q < n;
+  const int c = valid ? (int)a.pool.q_cls[q] : 0;
+  const unsigned int pos = valid ? atomicAdd(&h[c], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x < 64) {  // first wave: class bases by an exclusive scan of the counts
+    const int t = threadIdx.x;
+    const unsigned int cnt = t < kShadeClasses ? a.cnt->cls_count[t] : 0u;
+    unsigned int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < kShadeClasses; off <<= 1) {
+      const unsigned int v = __shfl_up(incl, off);
+      if (t >= off) incl += v;
+    }
+    if (t < kShadeClasses)
+      base[t] = incl - cnt + (h[t] != 0u ? atomicAdd(&a.cnt->cls_cursor[t], h[t]) : 0u);
+  }
+  __syncthreads();
+  if (valid) a.pool.q_sh[base[c] + pos] = a.pool.q_act[q];
+}
+
 template <int BLOCK, int MINW>
 __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   const int q = blockIdx.x * BLOCK + threadIdx.x;
@@ -352,7 +428,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   const PathPool& P = a.pool;
   const int Pn = P.P;
   bool valid = (unsigned)q < nact;
-  int i = valid ? (int)P.q_act[q] : 0;
+  int i = valid ? (int)(a.sort_shade ? P.q_sh[q] : P.q_act[q]) : 0;
   int jslot = valid ? P.job[i] : 0;
   bool want_shadow = false, want_mis = false;
   for_uniform_scene(valid, valid ? a.jobs[jslot].scene : 0, [&](int scene_u) {
@@ -924,6 +1000,13 @@ constexpr int kWideBlock = 1024;
 
 template <int BLOCK>
 static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
+  if (a.sort_shade) {  // group the queue by shading class first
+    constexpr int kSortBlock = 256;
+    dim3 gs((a.pool.P + kSortBlock - 1) / kSortBlock), bs(kSortBlock);
+    hipLaunchKernelGGL(k_reset_classes, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_class_count<kSortBlock>, gs, bs, 0, s, a);
+    hipLaunchKernelGGL(k_class_scatter<kSortBlock>, gs, bs, 0, s, a);
+  }
   if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
 }

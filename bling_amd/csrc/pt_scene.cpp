@@ -224,6 +224,28 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     }
   }
   if (int rc = sc->materials.upload(ctx, mats.data(), mats.size() * sizeof(DMaterial), st)) return rc;
+  // distinct BxDF makeups among the materials primitives use: a scene with
+  // more than one benefits from shading queues grouped by material kind
+  {
+    std::vector<char> used(mats.size(), 0);
+    for (uint32_t t = 0; t < d->num_triangles; ++t) {
+      const int32_t mi = d->tri_mesh ? d->tri_mesh[t] : 0;
+      const int32_t m = (d->meshes && mi >= 0 && (uint32_t)mi < std::max<uint32_t>(d->num_meshes, 1))
+                            ? d->meshes[mi].material
+                            : 0;
+      if (m >= 0 && (size_t)m < used.size()) used[m] = 1;
+    }
+    for (uint32_t s = 0; s < ns; ++s)
+      if (d->shapes[s].material >= 0 && (size_t)d->shapes[s].material < used.size()) used[d->shapes[s].material] = 1;
+    std::vector<int> sigs;
+    for (size_t i = 0; i < mats.size(); ++i) {
+      if (!used[i]) continue;
+      const DMaterial& m = mats[i];
+      const int sig = m.n | (m.n > 0 ? m.c[0].kind << 2 : 0) | (m.n > 1 ? m.c[1].kind << 5 : 0);
+      if (std::find(sigs.begin(), sigs.end(), sig) == sigs.end()) sigs.push_back(sig);
+    }
+    sc->pt.shade_kinds = (int)sigs.size();
+  }
 
   // lights: explicit first, then emissive shapes (Scene.hs:38-42)
   std::vector<DLight> lights;

@@ -278,7 +278,10 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * 0, the default: one ray per thread),
  * "pt.pool" (path slots, >= 1024),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
- * two waves per SIMD).  Changing a pt.* knob is refused with BGPU_E_STATE
+ * two waves per SIMD), "pt.sort" (1: the shading queue is grouped by
+ * material kind each bounce, so that a wave runs one BSDF code path; 0: path
+ * order; 2, the default: grouped while a scene whose materials mix BSDF
+ * kinds has jobs in flight).  Changing a pt.* knob is refused with BGPU_E_STATE
  * while served jobs are queued or running; the engines are rebuilt on their
  * next use.  bgpu_config_get reads the current value. */
 int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value);

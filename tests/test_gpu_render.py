@@ -214,10 +214,11 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     flat = scenes.cornell(32)
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=4)
     base, _ = render_gpu(gpu_ctx, flat, spec)
-    defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves", "pt.persistent")}
+    defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves", "pt.persistent",
+                                                   "pt.sort")}
     try:
         for key, val in (("pt.shade_waves", 0), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096),
-                         ("pt.persistent", 0)):
+                         ("pt.persistent", 0), ("pt.persistent", 1), ("pt.sort", 0), ("pt.sort", 1)):
             gpu_ctx.config_set(key, val)
             assert gpu_ctx.config_get(key) == val
             img, _ = render_gpu(gpu_ctx, flat, spec)
@@ -267,6 +268,17 @@ def test_divergent_materials_matches_oracle(gpu_ctx):
     mo, _ = mean_se(o1, o1sq)
     rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
     assert np.median(rel) < 1e-4, np.median(rel)
+    # grouping the shading queue by material kind (on by default for this
+    # mixed-material scene) changes the order of work only
+    keep = gpu_ctx.config_get("pt.sort")
+    try:
+        for mode in (0, 1):
+            gpu_ctx.config_set("pt.sort", mode)
+            g2, _ = render_gpu(gpu_ctx, flat, mk(31))
+            assert np.array_equal(g2[..., 0], g[..., 0]), mode
+            np.testing.assert_allclose(g2, g, rtol=1e-4, atol=1e-5, err_msg=str(mode))
+    finally:
+        gpu_ctx.config_set("pt.sort", keep)
 
 
 def test_batch_of_64_jobs(gpu_ctx):
