@@ -5,8 +5,9 @@ tune_trace()  picks the BVH width, the traversal mode (lane refill or one ray
               scene and a sample of its rays, timing each candidate with the
               library's own kernel timer (BGPU_TIMED);
 tune_render() picks the block size of the bounce kernels, the shading
-              kernel's occupancy variant and the path-pool size for a render
-              job, timing whole renders.
+              kernel's occupancy variant, the path-pool size and (when asked)
+              the shading-queue grouping for a render job, timing whole
+              renders.
 
 Both leave the winning configuration set on the context and return it with
 the table of timings.  Results never depend on the knobs (the tests check
@@ -63,30 +64,32 @@ def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(
 
 
 def tune_render(ctx, gscene, spec, blocks=(64, 128, 256), shade_waves=(2, 0),
-                pools=(1 << 19, 1 << 20, 1 << 21), repeats=2):
+                pools=(1 << 19, 1 << 20, 1 << 21), sorts=(2,), repeats=2):
     """Times the plain render of `spec` on `gscene` for every (pt.block,
-    pt.shade_waves, pt.pool); returns (best, table)."""
+    pt.shade_waves, pt.pool, pt.sort); returns (best, table)."""
     from .runtime import GpuRenderer
     table = []
-    keep = {k: ctx.config_get(k) for k in ("pt.block", "pt.shade_waves", "pt.pool")}
+    keys = ("pt.block", "pt.shade_waves", "pt.pool", "pt.sort")
+    keep = {k: ctx.config_get(k) for k in keys}
     try:
-        for block, waves, pool in itertools.product(blocks, shade_waves, pools):
+        for block, waves, pool, sort in itertools.product(blocks, shade_waves, pools, sorts):
             ctx.config_set("pt.block", block)
             ctx.config_set("pt.shade_waves", waves)
             ctx.config_set("pt.pool", pool)
+            ctx.config_set("pt.sort", sort)
             GpuRenderer(spec).render_plain(gscene)  # engine build + warm-up
             ms = []
             for _ in range(repeats):
                 t0 = time.perf_counter()
                 GpuRenderer(spec).render_plain(gscene)
                 ms.append((time.perf_counter() - t0) * 1e3)
-            table.append({"pt.block": block, "pt.shade_waves": waves, "pt.pool": pool,
+            table.append({"pt.block": block, "pt.shade_waves": waves, "pt.pool": pool, "pt.sort": sort,
                           "ms": float(np.median(ms))})
     finally:
         for k, v in keep.items():
             ctx.config_set(k, v)
     best = _best(table)
-    for k in ("pt.block", "pt.shade_waves", "pt.pool"):
+    for k in keys:
         ctx.config_set(k, best[k])
     return best, table
 
