@@ -103,9 +103,10 @@ class GpuScene:
         return v.value
 
     # --- host arrays ---------------------------------------------------
-    def intersect(self, rays):
+    def intersect(self, rays, coherent=False):
         """rays: (n, 8) float32 [ox oy oz dx dy dz tmin tmax]; returns
-        (t, b1, b2, prim) numpy arrays."""
+        (t, b1, b2, prim) numpy arrays.  `coherent`: the batch's rays are
+        coherent (e.g. camera rays), the BGPU_COHERENT hint."""
         rays = _host_f32(rays)
         n = len(rays)
         cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
@@ -115,16 +116,18 @@ class GpuScene:
         prim = np.empty(n, np.int32)
         r = _rays_struct([c.ctypes.data for c in cols])
         h = abi.Hits(t.ctypes.data, b1.ctypes.data, b2.ctypes.data, prim.ctypes.data)
-        _ck(self.ctx.lib.bgpu_intersect_closest(self.h, C.byref(r), n, C.byref(h), 0))
+        _ck(self.ctx.lib.bgpu_intersect_closest(self.h, C.byref(r), n, C.byref(h),
+                                                abi.COHERENT if coherent else 0))
         return t, b1, b2, prim
 
-    def intersects(self, rays):
+    def intersects(self, rays, coherent=False):
         rays = _host_f32(rays)
         n = len(rays)
         cols = [np.ascontiguousarray(rays[:, k]) for k in range(8)]
         occ = np.empty(n, np.uint8)
         r = _rays_struct([c.ctypes.data for c in cols])
-        _ck(self.ctx.lib.bgpu_intersect_any(self.h, C.byref(r), n, occ.ctypes.data, 0))
+        _ck(self.ctx.lib.bgpu_intersect_any(self.h, C.byref(r), n, occ.ctypes.data,
+                                            abi.COHERENT if coherent else 0))
         return occ.astype(bool)
 
     # --- device-resident arrays (integer device addresses) -------------

@@ -281,3 +281,47 @@ def test_grazing_sphere_hits_bit_exact(gpu_ctx, width, scaled):
     assert_same(g, o, "grazing/w%d/%s" % (width, "scaled" if scaled else "plain"))
     assert (gocc == oocc).all(), "any-hit mismatches: %d" % (gocc != oocc).sum()
     assert (g[3] >= 0).sum() > 1000
+
+
+@pytest.mark.parametrize("coherent", [False, True])
+def test_degenerate_rays_leave_valid_ones_exact(gpu_ctx, coherent):
+    """Rays outside the bit-exact domain (zero, NaN or 1e+-30-scaled
+    directions, NaN or infinite origins, empty intervals) mixed into a
+    ragged batch: the reference's own answer for them depends on its
+    kd-tree's visiting order (its binary32 quadric returns NaN or infinite
+    roots), so only sanity is required of them; every valid ray in the same
+    batch, with lanes refilled around the degenerate ones, stays bit-exact."""
+    flat = scenes.small_scene(num_tris=1500, num_spheres=150, seed=12)
+    lo, hi = scenes.scene_bounds(flat)
+    n = 24007  # not a multiple of any block or slice size
+    rays = scenes.incoherent_rays(lo, hi, n, seed=13)
+    rng = np.random.default_rng(14)
+    bad = rng.random(n) < 0.3
+    kind = rng.integers(0, 7, n)
+    for k, (sl, val) in enumerate(((slice(3, 6), 0.0), (slice(3, 4), np.nan), (slice(0, 1), np.nan),
+                                   (slice(0, 3), np.inf), (None, None), (slice(3, 6), 1e-30),
+                                   (slice(3, 6), 1e30))):
+        m = bad & (kind == k)
+        if k == 4:
+            rays[m, 6], rays[m, 7] = 2.0, 1.0
+        elif k in (5, 6):
+            rays[m, sl] *= np.float32(val)
+        else:
+            rays[m, sl] = val
+    gs = gpu_ctx.scene(flat)
+    try:
+        g = gs.intersect(rays, coherent=coherent)
+        gocc = gs.intersects(rays, coherent=coherent)
+    finally:
+        gs.close()
+    o = OracleScene(flat)
+    oc = o.intersect(rays, brute=True)
+    oocc = o.intersects(rays, brute=True)
+    o.close()
+    ok = ~bad
+    assert_same(tuple(x[ok] for x in g), tuple(x[ok] for x in oc[:4]) + (0,), "valid rays among degenerate")
+    assert (gocc[ok] == oocc[ok]).all()
+    prim = g[3][bad]
+    assert ((prim >= -1) & (prim < flat.num_prims)).all()
+    empty = bad & (kind == 4)
+    assert (g[3][empty] == -1).all() and not gocc[empty].any()
