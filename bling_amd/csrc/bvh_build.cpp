@@ -38,82 +38,175 @@ struct TNode {
   std::unique_ptr<TNode> kid[2];
 };
 
+// A primitive range's bounds and centroid bounds.
+struct Extent {
+  Box box;
+  float cmin[3], cmax[3];
+};
+inline Extent empty_extent() {
+  Extent x;
+  x.box = empty_box();
+  for (int k = 0; k < 3; ++k) {
+    x.cmin[k] = INFINITY;
+    x.cmax[k] = -INFINITY;
+  }
+  return x;
+}
+inline void grow(Extent& a, const Extent& b) {
+  grow(a.box, b.box);
+  for (int k = 0; k < 3; ++k) {
+    a.cmin[k] = std::min(a.cmin[k], b.cmin[k]);
+    a.cmax[k] = std::max(a.cmax[k], b.cmax[k]);
+  }
+}
+
+constexpr int kMaxBins = 64;
+// Per-axis bins of one pass: bounds, centroid bounds and counts.
+struct Bins {
+  Extent x[3][kMaxBins];
+  int c[3][kMaxBins];
+  void clear(int nb) {
+    for (int a = 0; a < 3; ++a)
+      for (int i = 0; i < nb; ++i) {
+        x[a][i] = empty_extent();
+        c[a][i] = 0;
+      }
+  }
+};
+
+// Runs f(begin, end) over [b, e) in up to `threads` contiguous chunks.
+template <class F>
+void parallel_chunks(int b, int e, int threads, F&& f) {
+  const int n = e - b;
+  const int t = std::max(1, std::min(threads, n / 65536));
+  if (t == 1) {
+    f(0, b, e);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int i = 0; i < t; ++i) {
+    const int cb = b + (int)((long long)n * i / t), ce = b + (int)((long long)n * (i + 1) / t);
+    th.emplace_back([&f, i, cb, ce] { f(i, cb, ce); });
+  }
+  for (auto& x : th) x.join();
+}
+
 struct Builder {
   const std::vector<Box>& pb;
   std::vector<float> cen;  // 3 per prim
   std::vector<int32_t> ids;
   BuildParams p;
   int par_depth = 0;
+  int threads = 1;
 
   Builder(const std::vector<Box>& b, const BuildParams& bp) : pb(b), p(bp) {
     cen.resize(3 * b.size());
-    for (size_t i = 0; i < b.size(); ++i)
-      for (int k = 0; k < 3; ++k) {
-        float lo = b[i].lo[k], hi = b[i].hi[k];
-        float c = 0.5f * (lo + hi);
-        cen[3 * i + k] = std::isfinite(c) ? c : 0.0f;
-      }
     ids.resize(b.size());
-    for (size_t i = 0; i < b.size(); ++i) ids[i] = (int32_t)i;
+    threads = p.threads > 0 ? p.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, 16);
+    parallel_chunks(0, (int)b.size(), threads, [&](int, int cb, int ce) {
+      for (int i = cb; i < ce; ++i) {
+        for (int k = 0; k < 3; ++k) {
+          float lo = b[i].lo[k], hi = b[i].hi[k];
+          float c = 0.5f * (lo + hi);
+          cen[3 * (size_t)i + k] = std::isfinite(c) ? c : 0.0f;
+        }
+        ids[i] = (int32_t)i;
+      }
+    });
   }
 
-  Box range_box(int b, int e) const {
-    Box r = empty_box();
-    for (int i = b; i < e; ++i) grow(r, pb[ids[i]]);
+  // Bounds and centroid bounds of [b, e) (one pass, chunked when large).
+  Extent range_extent(int b, int e, int nthreads) const {
+    std::vector<Extent> part(std::max(1, nthreads), empty_extent());
+    parallel_chunks(b, e, nthreads, [&](int t, int cb, int ce) {
+      Extent x = empty_extent();
+      for (int i = cb; i < ce; ++i) {
+        const int id = ids[i];
+        grow(x.box, pb[id]);
+        for (int k = 0; k < 3; ++k) {
+          x.cmin[k] = std::min(x.cmin[k], cen[3 * (size_t)id + k]);
+          x.cmax[k] = std::max(x.cmax[k], cen[3 * (size_t)id + k]);
+        }
+      }
+      part[t] = x;
+    });
+    Extent r = empty_extent();
+    for (const Extent& x : part) grow(r, x);
     return r;
   }
 
-  // Returns the split position (begin < mid < end); fills child boxes.
-  int split(int b, int e, int depth, bool* make_leaf) {
+  // Splits [b, e) with extent `ex`: returns the split position (begin < mid
+  // < end) and the children's extents, or sets *make_leaf.  Binning of all
+  // three axes is one pass over the range (chunked over threads when large);
+  // the children's extents are unions of the chosen bins.
+  int split(int b, int e, int depth, const Extent& ex, int nthreads, bool* make_leaf, Extent* le,
+            Extent* re) {
     int n = e - b;
     *make_leaf = false;
-    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
-    Box nb = empty_box();
-    for (int i = b; i < e; ++i) {
-      int id = ids[i];
-      grow(nb, pb[id]);
-      for (int k = 0; k < 3; ++k) {
-        cmin[k] = std::min(cmin[k], cen[3 * id + k]);
-        cmax[k] = std::max(cmax[k], cen[3 * id + k]);
-      }
-    }
+    const float* cmin = ex.cmin;
+    const float* cmax = ex.cmax;
     bool balanced = depth > 40;  // keeps the total depth under kMaxDepth
     float best_cost = INFINITY;
     int best_axis = -1, best_bin = -1;
-    const int NB = std::max(2, std::min(p.bins, 64));
-    float parea = area(nb);
+    const int NB = std::max(2, std::min(p.bins, kMaxBins));
+    float parea = area(ex.box);
+    std::unique_ptr<Bins> bins;
     if (!balanced && parea > 0.0f) {
+      float sc[3];
+      bool use[3];
       for (int ax = 0; ax < 3; ++ax) {
         float ext = cmax[ax] - cmin[ax];
-        if (!(ext > 0.0f)) continue;
-        Box bb[64];
-        int bc[64];
-        for (int i = 0; i < NB; ++i) {
-          bb[i] = empty_box();
-          bc[i] = 0;
+        use[ax] = ext > 0.0f;
+        sc[ax] = use[ax] ? NB / ext : 0.0f;
+      }
+      const int nt = std::max(1, std::min(nthreads, n / 65536));
+      std::vector<std::unique_ptr<Bins>> part(nt);
+      for (auto& x : part) {
+        x.reset(new Bins());
+        x->clear(NB);
+      }
+      parallel_chunks(b, e, nt, [&](int t, int cb, int ce) {
+        Bins& B = *part[t];
+        for (int i = cb; i < ce; ++i) {
+          const int id = ids[i];
+          for (int ax = 0; ax < 3; ++ax) {
+            if (!use[ax]) continue;
+            int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * (size_t)id + ax] - cmin[ax]) * sc[ax])));
+            Extent& x = B.x[ax][bi];
+            grow(x.box, pb[id]);
+            for (int k = 0; k < 3; ++k) {
+              x.cmin[k] = std::min(x.cmin[k], cen[3 * (size_t)id + k]);
+              x.cmax[k] = std::max(x.cmax[k], cen[3 * (size_t)id + k]);
+            }
+            B.c[ax][bi]++;
+          }
         }
-        float sc = NB / ext;
-        for (int i = b; i < e; ++i) {
-          int id = ids[i];
-          int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * id + ax] - cmin[ax]) * sc)));
-          grow(bb[bi], pb[id]);
-          bc[bi]++;
-        }
-        float ra[64];
-        int rc[64];
+      });
+      bins = std::move(part[0]);
+      for (int t = 1; t < nt; ++t)
+        for (int ax = 0; ax < 3; ++ax)
+          for (int i = 0; i < NB; ++i) {
+            grow(bins->x[ax][i], part[t]->x[ax][i]);
+            bins->c[ax][i] += part[t]->c[ax][i];
+          }
+      for (int ax = 0; ax < 3; ++ax) {
+        if (!use[ax]) continue;
+        float ra[kMaxBins];
+        int rc[kMaxBins];
         Box acc = empty_box();
         int cnt = 0;
         for (int i = NB - 1; i > 0; --i) {
-          grow(acc, bb[i]);
-          cnt += bc[i];
+          grow(acc, bins->x[ax][i].box);
+          cnt += bins->c[ax][i];
           ra[i] = area(acc);
           rc[i] = cnt;
         }
         acc = empty_box();
         cnt = 0;
         for (int i = 0; i < NB - 1; ++i) {
-          grow(acc, bb[i]);
-          cnt += bc[i];
+          grow(acc, bins->x[ax][i].box);
+          cnt += bins->c[ax][i];
           if (cnt == 0 || rc[i + 1] == 0) continue;
           float c = p.cost_node + (area(acc) * cnt + ra[i + 1] * rc[i + 1]) / parea;
           if (c < best_cost) {
@@ -141,11 +234,16 @@ struct Builder {
       float sc = NB / ext;
       int ax = best_axis;
       auto it = std::partition(ids.begin() + b, ids.begin() + e, [&](int id) {
-        int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * id + ax] - cmin[ax]) * sc)));
+        int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * (size_t)id + ax] - cmin[ax]) * sc)));
         return bi <= best_bin;
       });
       int mid = (int)(it - ids.begin());
-      if (mid > b && mid < e) return mid;
+      if (mid > b && mid < e) {
+        *le = empty_extent();
+        *re = empty_extent();
+        for (int i = 0; i < NB; ++i) grow(i <= best_bin ? *le : *re, bins->x[ax][i]);
+        return mid;
+      }
     }
     // balanced / degenerate: median split on the widest centroid axis
     int ax = 0;
@@ -153,33 +251,39 @@ struct Builder {
       if (cmax[k] - cmin[k] > cmax[ax] - cmin[ax]) ax = k;
     int mid = b + n / 2;
     std::nth_element(ids.begin() + b, ids.begin() + mid, ids.begin() + e,
-                     [&](int x, int y) { return cen[3 * x + ax] < cen[3 * y + ax]; });
+                     [&](int x, int y) { return cen[3 * (size_t)x + ax] < cen[3 * (size_t)y + ax]; });
+    *le = range_extent(b, mid, nthreads);
+    *re = range_extent(mid, e, nthreads);
     return mid;
   }
 
-  // Builds the subtree for [b, e); returns a node (never a bare leaf unless
-  // called for a child slot, see child()).
-  void child(TNode* parent, int slot, int b, int e, int depth) {
-    parent->box[slot] = pad_box(range_box(b, e));
-    bool leaf = false;
-    int mid = (e - b) <= 1 ? b : split(b, e, depth, &leaf);
-    if ((e - b) <= 1) leaf = true;
+  // Threads a subtree at `depth` may use: the builder's threads shared by
+  // the 2^depth subtrees that run at once.
+  int threads_at(int depth) const { return std::max(1, threads >> std::min(depth, 30)); }
+
+  // Builds the subtree for [b, e) with extent ex into `slot` of `parent`.
+  void child(TNode* parent, int slot, int b, int e, int depth, const Extent& ex) {
+    parent->box[slot] = pad_box(ex.box);
+    bool leaf = (e - b) <= 1;
+    Extent le, re;
+    int mid = b;
+    if (!leaf) mid = split(b, e, depth, ex, threads_at(depth), &leaf, &le, &re);
     if (leaf) {
       parent->leaf[slot] = ~((b << kLeafCountBits) | (e - b));
       return;
     }
     parent->kid[slot].reset(new TNode());
-    fill(parent->kid[slot].get(), b, mid, e, depth + 1);
+    fill(parent->kid[slot].get(), b, mid, e, depth + 1, le, re);
   }
 
-  void fill(TNode* node, int b, int mid, int e, int depth) {
+  void fill(TNode* node, int b, int mid, int e, int depth, const Extent& le, const Extent& re) {
     if (depth < par_depth && (e - b) > 65536) {
-      auto f = std::async(std::launch::async, [&] { child(node, 0, b, mid, depth); });
-      child(node, 1, mid, e, depth);
+      auto f = std::async(std::launch::async, [&] { child(node, 0, b, mid, depth, le); });
+      child(node, 1, mid, e, depth, re);
       f.get();
     } else {
-      child(node, 0, b, mid, depth);
-      child(node, 1, mid, e, depth);
+      child(node, 0, b, mid, depth, le);
+      child(node, 1, mid, e, depth, re);
     }
   }
 };
@@ -246,9 +350,15 @@ void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
     root.leaf[1] = ~0;
   } else {
     bool leaf = false;
-    int mid = bd.split(0, n, 0, &leaf);
-    if (leaf) mid = std::max(1, n / 2);  // the root is always an inner node
-    bd.fill(&root, 0, mid, n, 1);
+    const Extent all = bd.range_extent(0, n, bd.threads);
+    Extent le, re;
+    int mid = bd.split(0, n, 0, all, bd.threads, &leaf, &le, &re);
+    if (leaf) {  // the root is always an inner node
+      mid = std::max(1, n / 2);
+      le = bd.range_extent(0, mid, 1);
+      re = bd.range_extent(mid, n, 1);
+    }
+    bd.fill(&root, 0, mid, n, 1, le, re);
   }
   flatten(&root, nodes, order, bd.ids, 0, &st);
   st.nodes = (int)nodes->size();
@@ -275,40 +385,46 @@ void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
 
 int collapse_bvh(const std::vector<BNode>& bin, int root, int width, std::vector<WNode>* out) {
   // BVH2 inner index -> output index; children are assigned in DFS order
+  width = std::max(2, std::min(width, kMaxWidth));
+  out->reserve(out->size() + bin.size());  // at least one BVH2 node per wide node
   const int base = (int)out->size();
   std::vector<std::pair<int, int>> todo{{root, base}};  // (BVH2 node, output node)
   out->push_back(WNode{});
   while (!todo.empty()) {
     const int s = todo.back().first, o = todo.back().second;
     todo.pop_back();
-    std::vector<Box> cb{bin[s].child[0], bin[s].child[1]};
-    std::vector<int32_t> cl{bin[s].link[0], bin[s].link[1]};
+    WNode w;
+    w.child[0] = bin[s].child[0];
+    w.child[1] = bin[s].child[1];
+    w.link[0] = bin[s].link[0];
+    w.link[1] = bin[s].link[1];
+    w.n = 2;
     // open the largest inner child until the node is `width` wide
-    while ((int)cl.size() < width) {
+    while (w.n < width) {
       int pick = -1;
       float pa = -1.0f;
-      for (size_t i = 0; i < cl.size(); ++i)
-        if (cl[i] >= 0 && area(cb[i]) > pa) {
-          pa = area(cb[i]);
-          pick = (int)i;
+      for (int i = 0; i < w.n; ++i)
+        if (w.link[i] >= 0 && area(w.child[i]) > pa) {
+          pa = area(w.child[i]);
+          pick = i;
         }
       if (pick < 0) break;
-      const int g = cl[pick];
-      cb[pick] = bin[g].child[0];
-      cl[pick] = bin[g].link[0];
-      cb.push_back(bin[g].child[1]);
-      cl.push_back(bin[g].link[1]);
+      const int g = w.link[pick];
+      w.child[pick] = bin[g].child[0];
+      w.link[pick] = bin[g].link[0];
+      w.child[w.n] = bin[g].child[1];
+      w.link[w.n] = bin[g].link[1];
+      w.n++;
     }
-    for (size_t i = 0; i < cl.size(); ++i) {
-      if (cl[i] >= 0) {
+    for (int i = 0; i < w.n; ++i) {
+      if (w.link[i] >= 0) {
         const int ni = (int)out->size();
         out->push_back(WNode{});
-        todo.push_back({cl[i], ni});
-        cl[i] = ni;
+        todo.push_back({w.link[i], ni});
+        w.link[i] = ni;
       }
     }
-    (*out)[o].child = cb;
-    (*out)[o].link = cl;
+    (*out)[o] = w;
   }
   return base;
 }
@@ -320,17 +436,19 @@ int stack_bound(const std::vector<WNode>& w, int root) {
   while (!st.empty()) {
     const int n = st.back();
     bool ready = true;
-    for (int32_t l : w[n].link)
+    for (int i = 0; i < w[n].n; ++i) {
+      const int32_t l = w[n].link[i];
       if (l >= 0 && bound[l] < 0) {
         st.push_back(l);
         ready = false;
       }
+    }
     if (!ready) continue;
     st.pop_back();
     int m = 0;
-    for (int32_t l : w[n].link)
-      if (l >= 0) m = std::max(m, bound[l]);
-    bound[n] = (int)w[n].link.size() - 1 + m;
+    for (int i = 0; i < w[n].n; ++i)
+      if (w[n].link[i] >= 0) m = std::max(m, bound[w[n].link[i]]);
+    bound[n] = w[n].n - 1 + m;
   }
   return bound[root];
 }

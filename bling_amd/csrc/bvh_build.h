@@ -43,9 +43,11 @@ void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
 
 // A node of a BVH of width w: up to w children, inner links index the output
 // array, leaves keep the BVH2 encoding.
+constexpr int kMaxWidth = 8;
 struct WNode {
-  std::vector<Box> child;
-  std::vector<int32_t> link;
+  Box child[kMaxWidth];
+  int32_t link[kMaxWidth];
+  int n = 0;  // children in use
 };
 // Appends the subtree of `bin` rooted at `root` to `out` as a BVH of width
 // `width` (2, 4 or 8), opening the largest inner child until each node is

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <thread>
 
 #include "bgpu_internal.h"
 
@@ -118,6 +119,19 @@ double sigma_bound(const float* m) {
   }
   return std::sqrt(best) * (1.0 + 1e-9);
 }
+// Runs f(begin, end) over [0, n) in contiguous chunks on up to 16 threads.
+template <class F>
+void parallel_for(size_t n, F&& f) {
+  const size_t t = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+  const size_t k = std::min(t, n / 32768);
+  if (k <= 1) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t i = 0; i < k; ++i) th.emplace_back([&f, n, k, i] { f(n * i / k, n * (i + 1) / k); });
+  for (auto& x : th) x.join();
+}
 inline float bitsf(int32_t i) {
   float f;
   std::memcpy(&f, &i, 4);
@@ -158,17 +172,16 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
 
   // -- primitive bounds and the root box fold (KdTree.hs:111)
   std::vector<Box> pb(np);
-  Box root = empty_box();
-  for (uint32_t t = 0; t < d->num_triangles; ++t) {
-    Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
-    for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * (size_t)t + k]);
-    pb[t] = b;
-    extend_b(root, b);
-  }
-  for (uint32_t s = 0; s < d->num_shapes; ++s) {
-    pb[d->num_triangles + s] = shape_world_bounds(d->shapes[s]);
-    extend_b(root, pb[d->num_triangles + s]);
-  }
+  parallel_for(d->num_triangles, [&](size_t tb, size_t te) {
+    for (size_t t = tb; t < te; ++t) {
+      Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
+      for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * t + k]);
+      pb[t] = b;
+    }
+  });
+  for (uint32_t s = 0; s < d->num_shapes; ++s) pb[d->num_triangles + s] = shape_world_bounds(d->shapes[s]);
+  Box root = empty_box();  // the fold in primitive order
+  for (uint32_t i = 0; i < np; ++i) extend_b(root, pb[i]);
 
   // -- BVHs: one over the triangles, one over the shapes, built binary and
   // collapsed to the context's width into one node array
@@ -178,8 +191,12 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   float aniso = 1.0f;
   std::vector<float> hfac;  // per shape: sigma(o2w) sigma(w2o)^2 / (2 r), +inf if not a sphere
   if (d->num_triangles) {
-    std::vector<Box> tb(pb.begin(), pb.begin() + d->num_triangles);
-    build_bvh2(tb, bp, &tnodes, &order, &sc->build);
+    if (d->num_shapes == 0) {
+      build_bvh2(pb, bp, &tnodes, &order, &sc->build);
+    } else {
+      std::vector<Box> tb(pb.begin(), pb.begin() + d->num_triangles);
+      build_bvh2(tb, bp, &tnodes, &order, &sc->build);
+    }
   }
   if (d->num_shapes) {
     std::vector<Box> sb(pb.begin() + d->num_triangles, pb.end());
@@ -216,6 +233,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   int width = ctx->bvh_width;
   for (;; width /= 2) {
     nodes.clear();
+    nodes.reserve(tnodes.size() + snodes.size() + 2);
     tri_root = tnodes.empty() ? -1 : collapse_bvh(tnodes, 0, width, &nodes);
     shape_root = snodes.empty() ? -1 : collapse_bvh(snodes, 0, width, &nodes);
     int bound = 0;
@@ -238,7 +256,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     float m = 0.0f;
     if (link >= 0) {
       const WNode& n = nodes[link];
-      for (size_t c = 0; c < n.link.size(); ++c) {
+      for (int c = 0; c < n.n; ++c) {
         const float v = hmax(n.link[c]);
         hchild[(size_t)link * W + c] = v;
         m = std::max(m, v);
@@ -252,38 +270,42 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   };
   if (shape_root >= 0) (void)hmax(shape_root);
   std::vector<float> hn((size_t)8 * W * std::max<size_t>(nodes.size(), 1), 0.0f);
-  for (size_t i = 0; i < nodes.size(); ++i) {
-    const WNode& n = nodes[i];
-    float* o = &hn[(size_t)8 * W * i];
-    for (int c = 0; c < W; ++c) {
-      const bool used = c < (int)n.link.size();
-      const Box& b = n.child[used ? c : 0];
-      for (int k = 0; k < 3; ++k) {
-        o[(2 * k) * W + c] = b.lo[k];
-        o[(2 * k + 1) * W + c] = b.hi[k];
+  parallel_for(nodes.size(), [&](size_t ib, size_t ie) {
+    for (size_t i = ib; i < ie; ++i) {
+      const WNode& n = nodes[i];
+      float* o = &hn[(size_t)8 * W * i];
+      for (int c = 0; c < W; ++c) {
+        const bool used = c < n.n;
+        const Box& b = n.child[used ? c : 0];
+        for (int k = 0; k < 3; ++k) {
+          o[(2 * k) * W + c] = b.lo[k];
+          o[(2 * k + 1) * W + c] = b.hi[k];
+        }
+        o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
+        o[7 * W + c] = hchild[i * W + (used ? c : 0)];
       }
-      o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
-      o[7 * W + c] = hchild[i * W + (used ? c : 0)];
     }
-  }
+  });
   std::vector<float> hl(12 * std::max<size_t>(order.size(), 1), 0.0f);
-  for (size_t k = 0; k < order.size(); ++k) {
-    int id = order[k];
-    float* o = &hl[12 * k];
-    if ((uint32_t)id < d->num_triangles) {
-      const float* p1 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id];
-      const float* p2 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 1];
-      const float* p3 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 2];
-      o[0] = p1[0]; o[1] = p1[1]; o[2] = p1[2]; o[3] = bitsf(id);
-      // e1 = p2 - p1, e2 = p3 - p1: the same IEEE subtractions as TriangleMesh.hs:249
-      o[4] = p2[0] - p1[0]; o[5] = p2[1] - p1[1]; o[6] = p2[2] - p1[2]; o[7] = bitsf(0);
-      o[8] = p3[0] - p1[0]; o[9] = p3[1] - p1[1]; o[10] = p3[2] - p1[2]; o[11] = 0.0f;
-    } else {
-      int si = id - (int)d->num_triangles;
-      o[3] = bitsf(id);
-      o[7] = bitsf(~si);
+  parallel_for(order.size(), [&](size_t kb, size_t ke) {
+    for (size_t k = kb; k < ke; ++k) {
+      int id = order[k];
+      float* o = &hl[12 * k];
+      if ((uint32_t)id < d->num_triangles) {
+        const float* p1 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id];
+        const float* p2 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 1];
+        const float* p3 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 2];
+        o[0] = p1[0]; o[1] = p1[1]; o[2] = p1[2]; o[3] = bitsf(id);
+        // e1 = p2 - p1, e2 = p3 - p1: the same IEEE subtractions as TriangleMesh.hs:249
+        o[4] = p2[0] - p1[0]; o[5] = p2[1] - p1[1]; o[6] = p2[2] - p1[2]; o[7] = bitsf(0);
+        o[8] = p3[0] - p1[0]; o[9] = p3[1] - p1[1]; o[10] = p3[2] - p1[2]; o[11] = 0.0f;
+      } else {
+        int si = id - (int)d->num_triangles;
+        o[3] = bitsf(id);
+        o[7] = bitsf(~si);
+      }
     }
-  }
+  });
   std::vector<DShape> hs(std::max<uint32_t>(d->num_shapes, 1));
   for (uint32_t s = 0; s < d->num_shapes; ++s) {
     const bgpu_shape& in = d->shapes[s];
