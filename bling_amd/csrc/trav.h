@@ -30,12 +30,15 @@ constexpr int kSpill = kStackCapacity - kShort;
 // LDS ints per lane a kernel provides: node links, then entry distances
 constexpr int kLdsPerLane = kCull ? 2 * kShort : kShort;
 
+// The counters live in registers; only the spill arrays, which are indexed
+// by the stack depth, are private memory (they belong to the caller, so that
+// the compiler keeps sp and stride out of scratch).
 struct Stack {
   int* lds;    // kLdsPerLane entries per lane, `stride` apart
   int stride;
   int sp;
-  int spill_n[kSpill];
-  float spill_t[kCull ? kSpill : 1];
+  int* spill_n;    // kSpill entries
+  float* spill_t;  // kSpill entries when kCull
   __device__ __forceinline__ void push(int v, float t) {
     if (sp < kShort) {
       lds[sp * stride] = v;
@@ -287,9 +290,13 @@ __device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r,
   if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return;
   f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
   f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
+  int spill_n[kSpill];
+  float spill_t[kCull ? kSpill : 1];
   Stack st;
   st.lds = lds;
   st.stride = stride;
+  st.spill_n = spill_n;
+  st.spill_t = spill_t;
   closest_w<W, STATS>(sc, r, inv, oi, st, best, nvisit, ntest);
 }
 
@@ -300,9 +307,13 @@ __device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int
   if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return false;
   f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
   f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
+  int spill_n[kSpill];
+  float spill_t[kCull ? kSpill : 1];
   Stack st;
   st.lds = lds;
   st.stride = stride;
+  st.spill_n = spill_n;
+  st.spill_t = spill_t;
   return any_w<W, STATS>(sc, r, inv, oi, st, nvisit, ntest);
 }
 

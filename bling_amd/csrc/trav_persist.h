@@ -51,10 +51,14 @@ template <int W, bool ANY, bool FAT, bool STATS, class Src>
 __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* work, int* lds_lane,
                                               int stride, unsigned long long* counters) {
   const int lane = __lane_id();
+  int spill_n[kSpill];
+  float spill_t[kCull ? kSpill : 1];
   Stack st;
   st.lds = lds_lane;
   st.stride = stride;
   st.sp = 0;
+  st.spill_n = spill_n;
+  st.spill_t = spill_t;
   int item = -1, cur = kNoChild;
   int pend = kNoChild;  // postponed leaf, kNoChild when none
   DRay r;
