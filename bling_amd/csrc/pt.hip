@@ -1001,7 +1001,7 @@ constexpr int kWideBlock = 1024;
 template <int BLOCK>
 static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
   if (a.sort_shade) {  // group the queue by shading class first
-    constexpr int kSortBlock = 256;
+    constexpr int kSortBlock = 1024;  // fewer blocks, fewer atomics on the class counters
     dim3 gs((a.pool.P + kSortBlock - 1) / kSortBlock), bs(kSortBlock);
     hipLaunchKernelGGL(k_reset_classes, dim3(1), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_class_count<kSortBlock>, gs, bs, 0, s, a);
