@@ -91,27 +91,34 @@ void parallel_chunks(int b, int e, int threads, F&& f) {
   for (auto& x : th) x.join();
 }
 
+// A primitive's bounds, centroid and id, stored in build order: ranges are
+// partitioned in place, so every pass over a range reads it sequentially.
+struct Prim {
+  Box box;
+  float c[3];
+  int32_t id;
+};
+
 struct Builder {
-  const std::vector<Box>& pb;
-  std::vector<float> cen;  // 3 per prim
-  std::vector<int32_t> ids;
+  std::vector<Prim> prims;
   BuildParams p;
   int par_depth = 0;
   int threads = 1;
 
-  Builder(const std::vector<Box>& b, const BuildParams& bp) : pb(b), p(bp) {
-    cen.resize(3 * b.size());
-    ids.resize(b.size());
+  Builder(const std::vector<Box>& b, const BuildParams& bp) : p(bp) {
+    prims.resize(b.size());
     threads = p.threads > 0 ? p.threads : (int)std::max(1u, std::thread::hardware_concurrency());
     threads = std::min(threads, 16);
     parallel_chunks(0, (int)b.size(), threads, [&](int, int cb, int ce) {
       for (int i = cb; i < ce; ++i) {
+        Prim& q = prims[i];
+        q.box = b[i];
         for (int k = 0; k < 3; ++k) {
           float lo = b[i].lo[k], hi = b[i].hi[k];
           float c = 0.5f * (lo + hi);
-          cen[3 * (size_t)i + k] = std::isfinite(c) ? c : 0.0f;
+          q.c[k] = std::isfinite(c) ? c : 0.0f;
         }
-        ids[i] = (int32_t)i;
+        q.id = (int32_t)i;
       }
     });
   }
@@ -122,11 +129,11 @@ struct Builder {
     parallel_chunks(b, e, nthreads, [&](int t, int cb, int ce) {
       Extent x = empty_extent();
       for (int i = cb; i < ce; ++i) {
-        const int id = ids[i];
-        grow(x.box, pb[id]);
+        const Prim& q = prims[i];
+        grow(x.box, q.box);
         for (int k = 0; k < 3; ++k) {
-          x.cmin[k] = std::min(x.cmin[k], cen[3 * (size_t)id + k]);
-          x.cmax[k] = std::max(x.cmax[k], cen[3 * (size_t)id + k]);
+          x.cmin[k] = std::min(x.cmin[k], q.c[k]);
+          x.cmax[k] = std::max(x.cmax[k], q.c[k]);
         }
       }
       part[t] = x;
@@ -169,15 +176,15 @@ struct Builder {
       parallel_chunks(b, e, nt, [&](int t, int cb, int ce) {
         Bins& B = *part[t];
         for (int i = cb; i < ce; ++i) {
-          const int id = ids[i];
+          const Prim& q = prims[i];
           for (int ax = 0; ax < 3; ++ax) {
             if (!use[ax]) continue;
-            int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * (size_t)id + ax] - cmin[ax]) * sc[ax])));
+            int bi = std::min(NB - 1, std::max(0, (int)((q.c[ax] - cmin[ax]) * sc[ax])));
             Extent& x = B.x[ax][bi];
-            grow(x.box, pb[id]);
+            grow(x.box, q.box);
             for (int k = 0; k < 3; ++k) {
-              x.cmin[k] = std::min(x.cmin[k], cen[3 * (size_t)id + k]);
-              x.cmax[k] = std::max(x.cmax[k], cen[3 * (size_t)id + k]);
+              x.cmin[k] = std::min(x.cmin[k], q.c[k]);
+              x.cmax[k] = std::max(x.cmax[k], q.c[k]);
             }
             B.c[ax][bi]++;
           }
@@ -233,11 +240,11 @@ struct Builder {
       float ext = cmax[best_axis] - cmin[best_axis];
       float sc = NB / ext;
       int ax = best_axis;
-      auto it = std::partition(ids.begin() + b, ids.begin() + e, [&](int id) {
-        int bi = std::min(NB - 1, std::max(0, (int)((cen[3 * (size_t)id + ax] - cmin[ax]) * sc)));
+      auto it = std::partition(prims.begin() + b, prims.begin() + e, [&](const Prim& q) {
+        int bi = std::min(NB - 1, std::max(0, (int)((q.c[ax] - cmin[ax]) * sc)));
         return bi <= best_bin;
       });
-      int mid = (int)(it - ids.begin());
+      int mid = (int)(it - prims.begin());
       if (mid > b && mid < e) {
         *le = empty_extent();
         *re = empty_extent();
@@ -250,8 +257,8 @@ struct Builder {
     for (int k = 1; k < 3; ++k)
       if (cmax[k] - cmin[k] > cmax[ax] - cmin[ax]) ax = k;
     int mid = b + n / 2;
-    std::nth_element(ids.begin() + b, ids.begin() + mid, ids.begin() + e,
-                     [&](int x, int y) { return cen[3 * (size_t)x + ax] < cen[3 * (size_t)y + ax]; });
+    std::nth_element(prims.begin() + b, prims.begin() + mid, prims.begin() + e,
+                     [&](const Prim& x, const Prim& y) { return x.c[ax] < y.c[ax]; });
     *le = range_extent(b, mid, nthreads);
     *re = range_extent(mid, e, nthreads);
     return mid;
@@ -360,7 +367,11 @@ void build_bvh2(const std::vector<Box>& prim_bounds, const BuildParams& p,
     }
     bd.fill(&root, 0, mid, n, 1, le, re);
   }
-  flatten(&root, nodes, order, bd.ids, 0, &st);
+  std::vector<int32_t> ids(n);
+  parallel_chunks(0, n, bd.threads, [&](int, int cb, int ce) {
+    for (int i = cb; i < ce; ++i) ids[i] = bd.prims[i].id;
+  });
+  flatten(&root, nodes, order, ids, 0, &st);
   st.nodes = (int)nodes->size();
   // SAH cost estimate (node visits + prim tests, per unit root area)
   double ra = 0.0;
