@@ -1108,6 +1108,10 @@ static Ray fireRay(const bgpu_camera& cam, float ix, float iy, float lu, float l
 
 extern "C" {
 
+// The sampler's primitives, for known-answer and bijection tests.
+uint64_t orc_mix64(uint64_t x) { return orc::mix64(x); }
+uint32_t orc_permute(uint32_t i, uint32_t n, uint32_t p) { return orc::permute(i, n, p); }
+
 // Sampler values of every sample n of pixel (pix, pix) in `pass`, for the
 // stratification tests: per row [camera x, camera y, 1D table dim 0,
 // 2D table dim 0 (u, v), 1D dim 100 (past the tables), lens (u, v)].

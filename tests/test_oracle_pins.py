@@ -120,6 +120,27 @@ def test_sampler_stratification():
     assert len(set(np.round(raw1, 6))) > N // 2  # beyond the tables: plain uniform
 
 
+def test_sampler_primitives_known_answers():
+    """mix64 is the SplitMix64 finaliser (S. Vigna): fed the generator's
+    states 1, 2, 3 x 0x9e3779b97f4a7c15 (seed 0) it must return the
+    generator's published first outputs.  permute(., n, p) must be a
+    bijection of [0, n) for every key (it stands in for `shuffle`)."""
+    import ctypes as C
+    lib = pyoracle.load()
+    lib.orc_mix64.restype = C.c_uint64
+    lib.orc_mix64.argtypes = [C.c_uint64]
+    lib.orc_permute.restype = C.c_uint32
+    lib.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    gamma = 0x9E3779B97F4A7C15
+    got = [lib.orc_mix64((k * gamma) & (2 ** 64 - 1)) for k in (1, 2, 3)]
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 7, 16, 100, 1000, 4097):
+        for p in rng.integers(0, 2 ** 32, 4):
+            img = [lib.orc_permute(i, n, int(p)) for i in range(n)]
+            assert sorted(img) == list(range(n)), (n, p)
+
+
 def test_spectral_tables():
     y = np.array(TABLES["cie_y"], np.float32)
     acc = np.float32(0)
