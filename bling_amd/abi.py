@@ -36,9 +36,18 @@ class Shape(C.Structure):
                 ("w2o", F16), ("emission", F16)]
 
 
+TEX_NONE, TEX_CHECKER, TEX_GRAPHPAPER = 0, 1, 2
+MAP_UV, MAP_PLANAR = 0, 1
+
+
+class Texture(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("mapping", C.c_int32), ("slots", C.c_int32), ("pad", C.c_int32),
+                ("params", C.c_float * 12), ("alt", (C.c_float * 16) * 2)]
+
+
 class Material(C.Structure):
     _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("scalar", C.c_float * 2),
-                ("spec", (C.c_float * 16) * 2)]
+                ("spec", (C.c_float * 16) * 2), ("tex", Texture)]
 
 
 class Mesh(C.Structure):

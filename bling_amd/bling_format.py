@@ -315,16 +315,60 @@ class _Parser:
 
     # -- textures and materials (MaterialParser.hs)
     def spectrum_texture(self, name):
+        """A spectrum texture (MaterialParser.hs:207-241): a constant spectrum,
+        or a selector ("checker" | "graphpaper", mapping, params, tex1, tex2)
+        over two constants; a blend of constants by a constant is folded."""
         self.named(name)
         self.open()
         t = self.word()
-        if t in ("blend", "image", "checker", "gradient", "graphPaper"):
-            raise Unsupported("spectrum texture %r (only constant textures are on the GPU path)" % t)
-        if t != "constant":
+        if t == "constant":
+            v = self.spectrum()
+        elif t == "blend":
+            t1 = self.spectrum_texture("tex1")
+            t2 = self.spectrum_texture("tex2")
+            f = self.scalar_texture("f")
+            if not (isinstance(t1, np.ndarray) and isinstance(t2, np.ndarray)):
+                raise Unsupported("blend of non-constant textures")
+            if f <= F(0):  # spectrumBlend (Texture.hs:135-145)
+                v = t1
+            elif f >= F(1):
+                v = t2
+            else:
+                v = np.array([F(F(a * F(F(1) - f)) + F(b * f)) for a, b in zip(t1, t2)], F)
+        elif t == "checker":
+            scale = self.vec()
+            t1 = self.spectrum_texture("tex1")
+            t2 = self.spectrum_texture("tex2")
+            v = ("checker", abi.MAP_UV, list(scale), t1, t2)
+        elif t == "graphPaper":
+            lw = self.flt()
+            mapping, mp = self.mapping_2d("map")
+            t1 = self.spectrum_texture("tex1")
+            t2 = self.spectrum_texture("tex2")
+            v = ("graphpaper", mapping, [lw] + mp, t1, t2)
+        elif t in ("image", "gradient"):
+            raise Unsupported("spectrum texture %r" % t)
+        else:
             self.fail("unknown texture type " + t)
-        s = self.spectrum()
+        if isinstance(v, tuple) and not (isinstance(v[3], np.ndarray) and isinstance(v[4], np.ndarray)):
+            raise Unsupported("%s texture over non-constant textures" % v[0])
         self.close()
-        return s
+        return v
+
+    def mapping_2d(self, name):
+        """pTextureMapping2d (MaterialParser.hs:159-178): (mapping, params)."""
+        self.named(name)
+        self.open()
+        t = self.word()
+        if t == "uv":
+            mapping, params = abi.MAP_UV, [self.flt(), self.flt(), self.flt(), self.flt()]
+        elif t == "planar":
+            vu, vv = self.vec(), self.vec()
+            mapping, params = abi.MAP_PLANAR, list(vu) + list(vv) + [self.flt(), self.flt()]
+        else:
+            self.fail("unknown 2d mapping " + t)
+        self.close()
+        return mapping, params
 
     def scalar_texture(self, name):
         self.named(name)
@@ -338,28 +382,52 @@ class _Parser:
         self.close()
         return v
 
+    @staticmethod
+    def textured(make, slots):
+        """The material `make(spectra)` with selector textures in its
+        spectrum slots: every textured slot must use the same selector."""
+        tex = [v for v in slots if isinstance(v, tuple)]
+        m = make([v[3] if isinstance(v, tuple) else v for v in slots])
+        if not tex:
+            return m
+        kind, mapping, params = tex[0][:3]
+        if any(v[:3] != (kind, mapping, params) for v in tex[1:]):
+            raise Unsupported("material with differently textured parameters")
+        bits = sum(1 << k for k, v in enumerate(slots) if isinstance(v, tuple))
+        alt = [v[4] if isinstance(v, tuple) else v for v in slots] + [None]
+        code = abi.TEX_CHECKER if kind == "checker" else abi.TEX_GRAPHPAPER
+        out = m.textured(0, code, params, alt[0], mapping) if bits & 1 else m.textured(1, code, params, alt[1], mapping)
+        if bits == 3:
+            out.tex_slots = 3
+            out.alt = [np.asarray(alt[0], F), np.asarray(alt[1], F)]
+        return out
+
     def material_body(self):
         t = self.word()
         if t == "blackbody":
             return Material.black()
         if t == "matte":
             kd = self.spectrum_texture("kd")
-            return Material.matte(kd, self.scalar_texture("sigma"))
+            sigma = self.scalar_texture("sigma")
+            return self.textured(lambda s: Material.matte(s[0], sigma), [kd])
         if t == "plastic":
             kd = self.spectrum_texture("kd")
             ks = self.spectrum_texture("ks")
-            return Material.plastic(kd, ks, self.scalar_texture("rough"))
+            rough = self.scalar_texture("rough")
+            return self.textured(lambda s: Material.plastic(s[0], s[1], rough), [kd, ks])
         if t == "metal":
             eta = self.spectrum_texture("eta")
             k = self.spectrum_texture("k")
-            return Material.metal(eta, k, self.scalar_texture("rough"))
+            rough = self.scalar_texture("rough")
+            return self.textured(lambda s: Material.metal(s[0], s[1], rough), [eta, k])
         if t == "glass":
             ior = self.scalar_texture("ior")
             kr = self.spectrum_texture("kr")
             kt = self.spectrum_texture("kt")
-            return Material.glass(ior, kr, kt)
+            return self.textured(lambda s: Material.glass(ior, s[0], s[1]), [kr, kt])
         if t == "mirror":
-            return Material.mirror(self.spectrum_texture("kr"))
+            kr = self.spectrum_texture("kr")
+            return self.textured(lambda s: Material.mirror(s[0]), [kr])
         if t in ("bumpMap", "substrate", "shinyMetal", "transMatte"):
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)

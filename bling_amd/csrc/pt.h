@@ -27,7 +27,9 @@ struct DBxdf {
 };
 struct DMaterial {
   int32_t n;
-  int32_t pad[3];
+  int32_t tex_kind, tex_map;  // selector texture (bgpu_texture), BGPU_TEX_NONE if constant
+  int32_t alt;                // table index of the tex2 variant of this material
+  float tex[12];
   DBxdf c[2];
 };
 

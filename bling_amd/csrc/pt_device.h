@@ -165,6 +165,7 @@ __device__ __forceinline__ f3 set_comp(int d, float t, f3 v) {
 struct DG {
   f3 p, n, dpdu, dpdv;
   float b1, b2;
+  float u = 0.0f, v = 0.0f;  // surface parameters (0 where mkDg' builds the DG)
 };
 __device__ __forceinline__ DG mk_dg(f3 p, f3 dpdu, f3 dpdv) {
   return DG{p, norm3(cross3(dpdu, dpdv)), dpdu, dpdv, 0.0f, 0.0f};
@@ -174,7 +175,8 @@ __device__ __forceinline__ DG mk_dg1(f3 p, f3 n) {
   return DG{p, n, f.s, f.t, 0.0f, 0.0f};
 }
 
-// Object-space DG of a shape hit (Shape.hs:86-229), without u/v.
+// Object-space DG of a shape hit (Shape.hs:86-229); u/v as the quad and
+// the sphere set them, 0 for the shapes built with mkDg'.
 __device__ __forceinline__ DG shape_dg(const DShape& s, const DRay& ray, float t, int axis) {
   f3 p = ray_at(ray, t);
   switch (s.type) {
@@ -191,8 +193,12 @@ __device__ __forceinline__ DG shape_dg(const DShape& s, const DRay& ray, float t
     }
     case SH_DISK:
       return mk_dg1(p, mk3(0.0f, 0.0f, -1.0f));
-    case SH_QUAD:
-      return mk_dg(p, mk3(s.p[0], 0.0f, 0.0f), mk3(0.0f, s.p[1], 0.0f));
+    case SH_QUAD: {  // Shape.hs:157-171
+      DG d = mk_dg(p, mk3(s.p[0], 0.0f, 0.0f), mk3(0.0f, s.p[1], 0.0f));
+      d.u = (s.p[0] + p.x) / (2.0f * s.p[0]);
+      d.v = (s.p[1] + p.y) / (2.0f * s.p[1]);
+      return d;
+    }
     default: {
       float r = s.p[0];
       const float thetaMin = kPi, thetaMax = 0.0f, phiMax = kTwoPi;
@@ -202,7 +208,10 @@ __device__ __forceinline__ DG shape_dg(const DShape& s, const DRay& ray, float t
       float cosphi = p.x * iz, sinphi = p.y * iz;
       f3 dpdu = mk3(-(phiMax * p.y), phiMax * p.x, 0.0f);
       f3 dpdv = scl3(mk3(p.z * cosphi, p.z * sinphi, -(r * sinf(theta))), thetaMax - thetaMin);
-      return mk_dg(p, dpdu, dpdv);
+      DG d = mk_dg(p, dpdu, dpdv);
+      d.u = atan2p(p.y, p.x) / phiMax;  // Shape.hs:189-192
+      d.v = (theta - thetaMin) / (thetaMax - thetaMin);
+      return d;
     }
   }
 }
@@ -240,6 +249,11 @@ __device__ __forceinline__ void hit_dg(const DevPtScene& S, const DRay& ray, flo
     *dgg = mk_dg(ray_at(ray, t), dpdu, dpdv);
     dgg->b1 = b1;
     dgg->b2 = b2;
+    {  // interpolated u, v (TriangleMesh.hs:195-198)
+      const float b0 = 1.0f - b1 - b2;
+      dgg->u = b0 * uv00 + b1 * uv10 + b2 * uv20;
+      dgg->v = b0 * uv01 + b1 * uv11 + b2 * uv21;
+    }
     *dgs = *dgg;
     if ((info.y & BGPU_MESH_NORMALS) && S.normals) {  // TriangleMesh.hs:202-214
       f3 n0 = mk3(S.normals[3 * i0], S.normals[3 * i0 + 1], S.normals[3 * i0 + 2]);
@@ -302,6 +316,8 @@ __device__ __forceinline__ void hit_dg(const DevPtScene& S, const DRay& ray, flo
   dgg->dpdu = xform_vector(o2w, d.dpdu);
   dgg->dpdv = xform_vector(o2w, d.dpdv);
   dgg->b1 = dgg->b2 = 0.0f;
+  dgg->u = d.u;
+  dgg->v = d.v;
   *dgs = *dgg;
   *eps = 5e-4f * t;
   *material = s.material;
@@ -574,9 +590,31 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
   *pdf = 1.0f;
 }
 
+// Selector textures (Texture.hs:166-219) on the shading DG: true where the
+// material takes its tex2 spectra (the alternate material entry).
+__device__ __forceinline__ bool tex_selects_alt(const DMaterial& m, const DG& dg) {
+  const float* q = m.tex;
+  if (m.tex_kind == BGPU_TEX_CHECKER) {  // checkerBoard: tex1 where the sum is even
+    const long long s = (long long)floorf(dg.p.x * q[0]) + (long long)floorf(dg.p.y * q[1]) + (long long)floorf(dg.p.z * q[2]);
+    return (s & 1) != 0;
+  }
+  float x, z;  // graphPaper over its mapping
+  if (m.tex_map == BGPU_MAP_PLANAR) {
+    x = dot3(dg.p, mk3(q[1], q[2], q[3])) + q[7];
+    z = dot3(dg.p, mk3(q[4], q[5], q[6])) + q[8];
+  } else {
+    x = q[1] * dg.u + q[3];
+    z = q[2] * dg.v + q[4];
+  }
+  const float xf = fabsf(x - truncf(x)), zf = fabsf(z - truncf(z));  // properFraction
+  const float lo = q[0] / 2.0f, hi = 1.0f - lo;
+  return xf < lo || zf < lo || xf > hi || zf > hi;
+}
+
 __device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, const DG& dgg, const DG& dgs) {
   Bsdf b;
   b.m = (material >= 0 && material < S.nmats) ? &S.mats[material] : nullptr;
+  if (b.m && b.m->tex_kind != BGPU_TEX_NONE && tex_selects_alt(*b.m, dgs)) b.m = &S.mats[b.m->alt];
   f3 nn = dgs.n, sn = norm3(dgs.dpdu);
   b.cs = Frame{sn, cross3(nn, sn), nn};
   b.p = dgs.p;

@@ -139,14 +139,14 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     float e = 1.0f / rough;
     return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
   };
-  for (uint32_t i = 0; i < d->num_materials; ++i) {
-    const bgpu_material& m = d->materials[i];
-    DMaterial& o = mats[i];
+  // one material from its type and scalars, with spectra spec[0], spec[1]
+  auto build = [&](const bgpu_material& m, const float (*spec)[16]) {
+    DMaterial o;
     std::memset(&o, 0, sizeof(o));
     switch (m.type) {
       case BGPU_MAT_MATTE: {
         DBxdf b = bx_zero();
-        std::memcpy(b.r, m.spec[0], 64);
+        std::memcpy(b.r, spec[0], 64);
         float sig = m.scalar[0];
         if (sig == 0.0f) {
           b.kind = K_LAMBERT;
@@ -165,11 +165,11 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         DBxdf dd = bx_zero();
         dd.kind = K_LAMBERT;
         dd.type = BX_R | BX_D;
-        std::memcpy(dd.r, m.spec[0], 64);
+        std::memcpy(dd.r, spec[0], 64);
         DBxdf g = bx_zero();
         g.kind = K_MF;
         g.type = BX_R | BX_G;
-        std::memcpy(g.r, m.spec[1], 64);
+        std::memcpy(g.r, spec[1], 64);
         g.e = blinn_e(m.scalar[0]);
         g.fr = FR_DIEL;
         g.ei = 1.0f;
@@ -185,8 +185,8 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         for (int k = 0; k < 16; ++k) g.r[k] = 1.0f;
         g.e = blinn_e(m.scalar[0]);
         g.fr = FR_COND;
-        std::memcpy(g.eta, m.spec[0], 64);
-        std::memcpy(g.k, m.spec[1], 64);
+        std::memcpy(g.eta, spec[0], 64);
+        std::memcpy(g.k, spec[1], 64);
         o.c[o.n++] = g;
         break;
       }
@@ -203,8 +203,8 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         t.ei = 1.0f;
         t.et = m.scalar[0];
         for (int k = 0; k < 16; ++k) {  // sClamp' (Spectrum.hs:459-461)
-          r.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[0][k]));
-          t.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[1][k]));
+          r.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
+          t.r[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k]));
         }
         o.c[o.n++] = r;
         o.c[o.n++] = t;
@@ -215,13 +215,31 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         r.kind = K_SPECR;
         r.type = BX_R | BX_S;
         r.fr = FR_NOOP;
-        for (int k = 0; k < 16; ++k) r.r[k] = hmaxf(0.0f, hminf(1.0f, m.spec[0][k]));
+        for (int k = 0; k < 16; ++k) r.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
         o.c[o.n++] = r;
         break;
       }
       default:
         break;
     }
+    return o;
+  };
+  for (uint32_t i = 0; i < d->num_materials; ++i) {
+    const bgpu_material& m = d->materials[i];
+    mats[i] = build(m, m.spec);
+    const bgpu_texture& tx = m.tex;
+    if (tx.kind == BGPU_TEX_NONE) continue;
+    if ((tx.kind != BGPU_TEX_CHECKER && tx.kind != BGPU_TEX_GRAPHPAPER) ||
+        (tx.mapping != BGPU_MAP_UV && tx.mapping != BGPU_MAP_PLANAR) || tx.slots < 1 || tx.slots > 3)
+      return fail(BGPU_E_INVALID, "material texture: unknown kind, mapping or slots");
+    // the tex2 variant: the textured slots take the alternate spectra
+    float alt[2][16];
+    for (int k = 0; k < 2; ++k) std::memcpy(alt[k], ((tx.slots >> k) & 1) ? tx.alt[k] : m.spec[k], 64);
+    mats[i].tex_kind = tx.kind;
+    mats[i].tex_map = tx.mapping;
+    std::memcpy(mats[i].tex, tx.params, sizeof(mats[i].tex));
+    mats[i].alt = (int32_t)mats.size();
+    mats.push_back(build(m, alt));
   }
   if (int rc = sc->materials.upload(ctx, mats.data(), mats.size() * sizeof(DMaterial), st)) return rc;
   // distinct BxDF makeups among the materials primitives use: a scene with

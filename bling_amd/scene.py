@@ -120,6 +120,22 @@ class Material:
         self.spec0 = const(0) if spec0 is None else np.asarray(spec0, F)
         self.spec1 = const(0) if spec1 is None else np.asarray(spec1, F)
         self.s0, self.s1 = F(s0), F(s1)
+        # selector texture (bgpu_texture): none by default
+        self.tex_kind, self.tex_map, self.tex_slots = abi.TEX_NONE, abi.MAP_UV, 0
+        self.tex_params = [0.0] * 12
+        self.alt = [const(0), const(0)]
+
+    def textured(self, slot, kind, params, alt, mapping=abi.MAP_UV):
+        """This material with spectrum slot `slot` (0 or 1) textured: `kind`
+        (TEX_CHECKER: params sx sy sz; TEX_GRAPHPAPER: lw and the mapping's
+        parameters) selects per hit between the slot's spectrum (tex1) and
+        `alt` (tex2)."""
+        m = Material(self.kind, self.spec0, self.spec1, self.s0, self.s1)
+        m.tex_kind, m.tex_map, m.tex_slots = kind, mapping, 1 << slot
+        m.tex_params = [float(F(x)) for x in params] + [0.0] * (12 - len(params))
+        m.alt = [self.spec0.copy(), self.spec1.copy()]
+        m.alt[slot] = np.asarray(alt, F)
+        return m
 
     @staticmethod
     def matte(kd, sigma=0.0):
@@ -328,6 +344,13 @@ class FlatScene:
             for k in range(16):
                 self.materials[i].spec[0][k] = m.spec0[k]
                 self.materials[i].spec[1][k] = m.spec1[k]
+            t = self.materials[i].tex
+            t.kind, t.mapping, t.slots = m.tex_kind, m.tex_map, m.tex_slots
+            for k in range(12):
+                t.params[k] = m.tex_params[k]
+            for k in range(16):
+                t.alt[0][k] = m.alt[0][k]
+                t.alt[1][k] = m.alt[1][k]
         meshes = b.meshes or [(0, 0)]
         self.meshes = (abi.Mesh * len(meshes))()
         for i, (mat, fl) in enumerate(meshes):

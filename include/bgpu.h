@@ -87,11 +87,34 @@ enum {
   BGPU_MAT_GLASS = 4,   /* spec[0]=reflect, spec[1]=transmit (both clamped to [0,1]), scalar[0]=ior */
   BGPU_MAT_MIRROR = 5   /* spec[0]=reflect (clamped to [0,1]) */
 };
+/* A texture that selects, per hit, between two constant spectra
+ * (Texture.hs:191-219): spec[k] of the material is tex1's value and
+ * alt[k] tex2's, for every slot k in `slots` (bit k).
+ *   CHECKER     checkerBoard (sx sy sz) on the world hit point: tex1 where
+ *               floor(x sx) + floor(y sy) + floor(z sz) is even.
+ *               params: sx sy sz
+ *   GRAPHPAPER  graphPaper lw over a 2d mapping: tex2 on the lines (the
+ *               fractional part of either coordinate within lw/2 of an
+ *               integer), tex1 elsewhere.  params: lw, then the mapping:
+ *               UV      su sv ou ov            (Texture.hs:166-170)
+ *               PLANAR  vu.xyz vv.xyz ou ov    (Texture.hs:172-179) */
+enum { BGPU_TEX_NONE = 0, BGPU_TEX_CHECKER = 1, BGPU_TEX_GRAPHPAPER = 2 };
+enum { BGPU_MAP_UV = 0, BGPU_MAP_PLANAR = 1 };
+typedef struct {
+  int32_t kind;
+  int32_t mapping;
+  int32_t slots;
+  int32_t pad;
+  float params[12];
+  float alt[2][16];
+} bgpu_texture;
+
 typedef struct {
   int32_t type;
   int32_t pad;
   float scalar[2];
   float spec[2][16];
+  bgpu_texture tex; /* kind BGPU_TEX_NONE: constant spectra */
 } bgpu_material;
 
 /* A triangle mesh made by mkTriangleMesh (Primitive/TriangleMesh.hs:119-140). */

@@ -737,6 +737,30 @@ static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, 
   }
 }
 
+// Selector textures on the shading DG, the material's `tex dgs`: checkerBoard
+// (Texture.hs:209-219) and graphPaper (Texture.hs:191-207) over uvMapping /
+// planarMapping (Texture.hs:166-179).  True where tex2 is taken.
+static bool texSelectsTex2(const bgpu_texture& t, const DG& dg) {
+  const float* q = t.params;
+  if (t.kind == BGPU_TEX_CHECKER) {
+    long long s = (long long)std::floor(dg.p.x * q[0]) + (long long)std::floor(dg.p.y * q[1]) +
+                  (long long)std::floor(dg.p.z * q[2]);
+    long long m = ((s % 2) + 2) % 2;  // Haskell `mod`
+    return m != 0;
+  }
+  float x, z;
+  if (t.mapping == BGPU_MAP_PLANAR) {
+    x = dot(dg.p, mk(q[1], q[2], q[3])) + q[7];
+    z = dot(dg.p, mk(q[4], q[5], q[6])) + q[8];
+  } else {
+    x = q[1] * dg.u + q[3];
+    z = q[2] * dg.v + q[4];
+  }
+  float xf = std::fabs(x - std::trunc(x)), zf = std::fabs(z - std::trunc(z));  // properFraction
+  float lo = q[0] / 2.0f, hi = 1.0f - lo;
+  return xf < lo || zf < lo || xf > hi || zf > hi;
+}
+
 static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
   Bsdf b;
   b.n = 0;
@@ -747,7 +771,13 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
   b.ng = dgg.n;
   if (material < 0 || material >= (int)c.s->materials.size()) return b;
   const bgpu_material& m = c.s->materials[material];
-  Spec s0 = specOf(m.spec[0]), s1 = specOf(m.spec[1]);
+  const float* sp0 = m.spec[0];
+  const float* sp1 = m.spec[1];
+  if (m.tex.kind != BGPU_TEX_NONE && texSelectsTex2(m.tex, dgs)) {
+    if (m.tex.slots & 1) sp0 = m.tex.alt[0];
+    if (m.tex.slots & 2) sp1 = m.tex.alt[1];
+  }
+  Spec s0 = specOf(sp0), s1 = specOf(sp1);
   auto blinnE = [](float rough) {  // mkBlinn (1 / rough), fixExponent (Microfacet.hs:127-134)
     float e = 1.0f / rough;
     return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
@@ -1111,6 +1141,15 @@ extern "C" {
 // The sampler's primitives, for known-answer and bijection tests.
 uint64_t orc_mix64(uint64_t x) { return orc::mix64(x); }
 uint32_t orc_permute(uint32_t i, uint32_t n, uint32_t p) { return orc::permute(i, n, p); }
+
+// Whether texture `t` picks tex2 at point p with surface parameters (u, v).
+int orc_tex_selects(const bgpu_texture* t, float px, float py, float pz, float u, float v) {
+  orc::DG dg{};
+  dg.p = orc::mk(px, py, pz);
+  dg.u = u;
+  dg.v = v;
+  return orc::texSelectsTex2(*t, dg) ? 1 : 0;
+}
 
 // Sampler values of every sample n of pixel (pix, pix) in `pass`, for the
 // stratification tests: per row [camera x, camera y, 1D table dim 0,

@@ -107,6 +107,41 @@ def materials_scene(res=48, env=True, seed=3):
     return sb.build()
 
 
+def textured_scene(res=48, seed=3):
+    """Selector textures on every mapping and primitive kind: a triangle
+    floor with a planar-mapped checker, a uv graph-paper sphere, a plastic
+    sphere whose two spectra are both textured, and a quad with planar
+    graph paper, under a quad light."""
+    from bling_amd.scene import rgb_refl, rgb_illum
+    sb = SceneBuilder()
+    floor = Material.matte(rgb_refl(0.7, 0.7, 0.7)).textured(
+        0, abi.TEX_CHECKER, (1.0, 1.0, 1.0), rgb_refl(0.1, 0.2, 0.6))
+    fm = sb.material(floor)
+    paper = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.6)).textured(
+        0, abi.TEX_GRAPHPAPER, (0.1, 8.0, 6.0, 0.0, 0.0), rgb_refl(0.02, 0.02, 0.02)))
+    pl = Material.plastic(rgb_refl(0.2, 0.3, 0.7), const(0.4), 0.05).textured(
+        0, abi.TEX_CHECKER, (4.0, 4.0, 4.0), rgb_refl(0.8, 0.1, 0.1))
+    pl.tex_slots = 3
+    pl.alt[1] = np.asarray(const(0.05), np.float32)
+    plastic = sb.material(pl)
+    wall = sb.material(Material.matte(rgb_refl(0.8, 0.8, 0.2)).textured(
+        0, abi.TEX_GRAPHPAPER, (0.2, 2.0, 0.0, 0.0, 0.0, 0.0, 2.0, 0.25, 0.25),
+        rgb_refl(0.1, 0.1, 0.1), mapping=abi.MAP_PLANAR))
+    sb.mesh(np.array([[-4, -4, 0], [4, -4, 0], [4, 4, 0], [-4, 4, 0]], np.float32),
+            np.array([[0, 1, 2], [0, 2, 3]]), material=fm,
+            uvs=np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32))
+    sb.shape(abi.SHAPE_SPHERE, (0.6,), o2w=T.translate((-0.8, 0.0, 0.6)), material=paper)
+    sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((0.8, 0.3, 0.5)), material=plastic)
+    sb.shape(abi.SHAPE_QUAD, (2.0, 1.0), o2w=T.translate((0.0, 1.5, 1.0)) @ T.rotate_x(90),
+             material=wall)
+    lt = T.rotate_x(180) @ T.translate((0.0, 0.0, 3.0))
+    sb.shape(abi.SHAPE_QUAD, (0.7, 0.7), o2w=lt, material=fm, emission=rgb_illum(8, 8, 8))
+    c2w = T.look_at((0.0, -4.5, 2.0), (0.0, 0.0, 0.6), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 50.0, res, res))
+    sb.set_film(res, res)
+    return sb.build()
+
+
 def grazing_spheres(n_sph=400, n=60000, seed=17, k0=2e-3, scaled=False):
     """Spheres of radius 1e-4..0.05 on a grid and rays that graze them from
     1..200 units away, at line distances from r - K/2 to r + 2K with

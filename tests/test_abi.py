@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 
 STRUCTS = {
-    "bgpu_shape": abi.Shape, "bgpu_material": abi.Material, "bgpu_mesh": abi.Mesh,
+    "bgpu_shape": abi.Shape, "bgpu_material": abi.Material, "bgpu_texture": abi.Texture, "bgpu_mesh": abi.Mesh,
     "bgpu_light": abi.Light, "bgpu_camera": abi.Camera, "bgpu_film": abi.Film,
     "bgpu_scene_desc": abi.SceneDesc, "bgpu_rays": abi.Rays, "bgpu_hits": abi.Hits,
     "bgpu_job_desc": abi.JobDesc, "bgpu_stats": abi.Stats, "bgpu_bvh_info": abi.BvhInfo,

@@ -8,7 +8,7 @@ src/lib/Graphics/Bling/IO/*.hs and against scenes built directly.
   polygon fans, spectra (rgbR/rgbI/spd/temp/hex), statement and light
   order, defaults of the start state, errors.
 - The reference's own example scenes (read in place when the reference
-  checkout is present): the five the GPU path renders build, with the
+  checkout is present): the six the GPU path renders build, with the
   primitive counts their text describes; the others fail either as
   Unsupported or, where the reference's current grammar rejects them too,
   as syntax errors.
@@ -154,8 +154,12 @@ def test_errors():
     with pytest.raises(B.Unsupported):
         B.parse("prim { heightMap 4 4 { fbm 0 octaves 3 omega 0.5 } { } }")
     with pytest.raises(B.Unsupported):
-        B.parse("material { matte kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } "
-                "tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
+        B.parse("material { matte kd { checker 1 1 1 tex1 { checker 2 2 2 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
+    with pytest.raises(B.Unsupported):  # two slots under different selectors
+        B.parse("material { plastic kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } ks { checker 2 1 1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } rough { constant 0.1 } }")
     with pytest.raises(B.Unsupported):
         B.parse("camera { environment }")
     with pytest.raises(ValueError, match="out of bounds"):
@@ -165,12 +169,47 @@ def test_errors():
         b.spec()
 
 
+def test_selector_textures():
+    """checker / graphPaper over constants (MaterialParser.hs:207-241,
+    159-178) become the material's selector texture; blend of constants by
+    a constant folds to a constant like spectrumBlend (Texture.hs)."""
+    hdr = "prim { shape { sphere radius 1 } } "
+    b = B.parse("material { matte kd { checker 1 2 3 tex1 { constant rgbR 0.5 0.5 0.5 } "
+                "tex2 { constant rgbR 0.1 0.2 0.3 } } sigma { constant 0 } } " + hdr)
+    m = b.flat.materials[0]
+    assert (m.tex.kind, m.tex.mapping, m.tex.slots) == (abi.TEX_CHECKER, abi.MAP_UV, 1)
+    assert list(m.tex.params)[:3] == [1.0, 2.0, 3.0]
+    assert list(m.spec[0]) == list(rgb_refl(0.5, 0.5, 0.5))
+    assert list(m.tex.alt[0]) == list(rgb_refl(0.1, 0.2, 0.3))
+    b = B.parse("material { matte kd { graphPaper 0.05 map { planar 1 0 0 0 0 1 0.5 0.25 } "
+                "tex1 { constant rgbR 0.6 0.6 0.6 } tex2 { constant rgbR 0.02 0.02 0.02 } } "
+                "sigma { constant 0 } } " + hdr)
+    m = b.flat.materials[0]
+    assert (m.tex.kind, m.tex.mapping) == (abi.TEX_GRAPHPAPER, abi.MAP_PLANAR)
+    assert list(m.tex.params)[:9] == [F(0.05), 1, 0, 0, 0, 0, 1, 0.5, 0.25]
+    # both plastic spectra under one selector
+    b = B.parse("material { plastic kd { graphPaper 0.1 map { uv 4 4 0 0 } tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } ks { graphPaper 0.1 map { uv 4 4 0 0 } "
+                "tex1 { constant rgbR 0.5 0.5 0.5 } tex2 { constant rgbR 0.2 0.2 0.2 } } "
+                "rough { constant 0.1 } } " + hdr)
+    m = b.flat.materials[0]
+    assert m.tex.slots == 3 and list(m.tex.alt[1]) == list(rgb_refl(0.2, 0.2, 0.2))
+    # blend: (1 - f) tex1 + f tex2 per band; f clamps the ends
+    for f, want in ((0.25, [F(F(F(0.8) * F(0.75)) + F(F(0.4) * F(0.25)))]), (1.5, [F(0.4)]), (-1, [F(0.8)])):
+        b = B.parse("material { matte kd { blend tex1 { constant spd { 400 0.8, 700 0.8 } } "
+                    "tex2 { constant spd { 400 0.4, 700 0.4 } } f { constant %s } } "
+                    "sigma { constant 0 } } " % f + hdr)
+        assert b.flat.materials[0].tex.kind == abi.TEX_NONE
+        assert list(b.flat.materials[0].spec[0])[:3] == want * 3, f
+
+
 EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit lights, size)
     "cornell-box.bling": (30, 1, 0, (480, 480)),
     "cornell-box-specular.bling": (10, 3, 0, (480, 480)),
     "matte-test.bling": (2, 7, 0, (1280, 720)),
     "plastic-test.bling": (2, 7, 0, (640, 360)),
     "blackbody-emission.bling": (0, 16, 0, (640, 400)),
+    "glass-torus.bling": (0, 4, 0, (640, 360)),
 }
 
 
@@ -197,3 +236,9 @@ def test_reference_examples():
     assert b.flat.desc.film.filter == abi.FILTER_MITCHELL
     with pytest.raises(B.Unsupported):  # its last renderer block is sppm
         b.spec()
+    # glass-torus.bling: the floor's graph paper (0.05, uv 15 15 0 0)
+    b = B.load(os.path.join(REF_EXAMPLES, "glass-torus.bling"))
+    tex = [b.flat.materials[i].tex for i in range(b.flat.desc.num_materials)]
+    paper = [t for t in tex if t.kind == abi.TEX_GRAPHPAPER]
+    assert len(paper) == 1 and paper[0].mapping == abi.MAP_UV
+    assert list(paper[0].params)[:5] == [F(0.05), 15, 15, 0, 0]

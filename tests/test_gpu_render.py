@@ -376,3 +376,29 @@ def test_bling_file_renders_like_the_procedural_scene(gpu_ctx):
     assert np.array_equal(a_img[..., 0], b_img[..., 0])
     np.testing.assert_allclose(b_img, a_img, rtol=1e-4, atol=1e-5)
     assert b_img[..., 2].mean() > 0.01
+
+
+def test_textured_scene_matches_oracle(gpu_ctx):
+    """Checker and graph-paper selectors on triangles (uv from the mesh),
+    spheres, quads, uv and planar mappings, and a material with both
+    spectra textured: same seed tracks the oracle, another seed is within
+    3 sigma."""
+    from tests import cases
+    flat = cases.textured_scene(res=48)
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, rr_start_depth=3, passes=4,
+                   seed=31, variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    spec2 = JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, rr_start_depth=3, passes=4,
+                    seed=32, variance=True)
+    o2, osq2, _ = pyoracle.render(flat, spec2)
+    frac, zmean, zmax = compare(g, gsq, o2, osq2)
+    assert frac < 0.015, (frac, zmean, zmax)
+    assert abs(zmean) < 0.08, zmean

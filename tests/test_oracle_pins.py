@@ -246,3 +246,70 @@ def test_sphere_false_hits_stay_within_the_traversal_margin():
     assert (line > rr * (1 + 1e-6) + 1e-7).sum() > 50
     # ... and every reported hit point lies within it
     assert np.all(excess <= allowed), float(np.max(excess / allowed))
+
+
+def _tex(kind, params, mapping=None):
+    t = abi.Texture()
+    t.kind = kind
+    t.mapping = abi.MAP_UV if mapping is None else mapping
+    t.slots = 1
+    for k, x in enumerate(params):
+        t.params[k] = x
+    return t
+
+
+def test_texture_selectors_at_closed_form_points():
+    """checkerBoard (Texture.hs:209-219) picks tex2 where
+    floor(x sx) + floor(y sy) + floor(z sz) is odd (negative sums included);
+    graphPaper (Texture.hs:191-207) picks tex2 (the line) where the
+    fractional part of either mapped coordinate lies within lw/2 of an
+    integer, measured by |x - truncate x| on both sides of zero."""
+    import ctypes as C
+    lib = pyoracle.load()
+    lib.orc_tex_selects.restype = C.c_int
+    lib.orc_tex_selects.argtypes = [C.POINTER(abi.Texture)] + [C.c_float] * 5
+
+    def sel(t, p, u=0.0, v=0.0):
+        return lib.orc_tex_selects(C.byref(t), *p, u, v)
+
+    chk = _tex(abi.TEX_CHECKER, (1.0, 2.0, 1.0))
+    assert sel(chk, (0.5, 0.2, 0.5)) == 0
+    assert sel(chk, (1.5, 0.2, 0.5)) == 1
+    assert sel(chk, (0.5, 0.7, 0.5)) == 1          # y scaled by 2: floor(1.4) = 1
+    assert sel(chk, (-0.5, 0.2, 0.5)) == 1         # floor, not truncation: -1
+    assert sel(chk, (-0.5, -0.2, 0.5)) == 0        # -1 + -1 = -2: even
+    assert sel(chk, (-2.5, 0.2, 0.5)) == 1         # -3: odd under Haskell's mod
+    # uv graph paper, su = 2, sv = 1, offsets 0, line width 0.1
+    gp = _tex(abi.TEX_GRAPHPAPER, (0.1, 2.0, 1.0, 0.0, 0.0))
+    p0 = (0.0, 0.0, 0.0)
+    assert sel(gp, p0, 0.25, 0.5) == 0             # x = 0.5, z = 0.5
+    assert sel(gp, p0, 0.51, 0.5) == 1             # x = 1.02: near a line
+    assert sel(gp, p0, 0.25, 0.98) == 1            # z = 0.98 > 0.95
+    assert sel(gp, p0, 0.5, 0.25) == 1             # u on a line (x = 1.0), v not
+    assert sel(gp, p0, 0.25 / 2, 0.25) == 0        # x = 0.25
+    assert sel(gp, p0, -0.01, 0.5) == 1            # x = -0.02: |frac| = 0.02
+    assert sel(gp, p0, -0.24, 0.5) == 0            # x = -0.48
+    # offsets: x = 2u + 0.5
+    gpo = _tex(abi.TEX_GRAPHPAPER, (0.1, 2.0, 1.0, 0.5, 0.0))
+    assert sel(gpo, p0, 0.25, 0.5) == 1 and sel(gpo, p0, 0.0, 0.5) == 0
+    # planar: x = p . (1, 0, 0) + 0.25, z = p . (0, 0, 2)
+    pl = _tex(abi.TEX_GRAPHPAPER, (0.1, 1.0, 0.0, 0.0, 0.0, 0.0, 2.0, 0.25, 0.0), abi.MAP_PLANAR)
+    assert sel(pl, (0.25, 9.0, 0.25), 0.5, 0.5) == 0   # x = 0.5, z = 0.5
+    assert sel(pl, (0.74, 9.0, 0.25), 0.5, 0.5) == 1   # x = 0.99
+    assert sel(pl, (0.25, 9.0, 0.49), 0.5, 0.5) == 1   # z = 0.98
+    assert sel(pl, (0.25, -3.0, 0.25), 0.0, 0.0) == 0  # uv and y are ignored
+
+
+def test_textured_scene_differs_from_untextured():
+    """The selectors reach the rendered image: the textured scene's mean
+    differs from the same scene with every texture switched off."""
+    from tests import cases
+    flat = cases.textured_scene(res=16)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=1,
+                   seed=5, variance=True)
+    a, _, _ = pyoracle.render(flat, spec)
+    for i in range(flat.desc.num_materials):
+        flat.materials[i].tex.kind = abi.TEX_NONE
+    b, _, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    assert abs(a[..., 2].mean() - b[..., 2].mean()) > 0.02 * abs(b[..., 2].mean())
