@@ -116,6 +116,7 @@ EXPORTS = [
     "bgpu_last_kernel_ms", "bgpu_render", "bgpu_job_submit", "bgpu_job_poll", "bgpu_job_wait",
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
     "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get", "bgpu_scene_bvh_info",
+    "bgpu_scene_write", "bgpu_scene_read", "bgpu_scene_desc_free", "bgpu_scene_load",
 ]
 
 _lib = None
@@ -158,6 +159,10 @@ def load(path=LIB_PATH):
     lib.bgpu_config_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.bgpu_config_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     lib.bgpu_scene_bvh_info.argtypes = [vp, C.POINTER(BvhInfo)]
+    lib.bgpu_scene_write.argtypes = [C.POINTER(SceneDesc), C.c_char_p]
+    lib.bgpu_scene_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(SceneDesc))]
+    lib.bgpu_scene_desc_free.argtypes = [C.POINTER(SceneDesc)]
+    lib.bgpu_scene_load.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
     _lib = lib
     return lib
 

@@ -59,6 +59,14 @@ class Context:
     def scene(self, flat):
         return GpuScene(self, flat)
 
+    def load_scene(self, path):
+        """A scene created from a flat binary scene file (bgpu_scene_load).
+        Its `flat` is the file's header data (film, primitive counts)."""
+        from .scene import SceneFile
+        meta = SceneFile(path)
+        meta.close()
+        return GpuScene(self, meta, path=path)
+
     def config_set(self, key, value):
         _ck(self.lib.bgpu_config_set(self.h, key.encode(), int(value)))
 
@@ -80,11 +88,14 @@ def _host_f32(a):
 
 
 class GpuScene:
-    def __init__(self, ctx, flat):
+    def __init__(self, ctx, flat, path=None):
         self.ctx = ctx
         self.flat = flat
         h = C.c_void_p()
-        _ck(ctx.lib.bgpu_scene_create(ctx.h, C.byref(flat.desc), C.byref(h)))
+        if path is not None:
+            _ck(ctx.lib.bgpu_scene_load(ctx.h, str(path).encode(), C.byref(h)))
+        else:
+            _ck(ctx.lib.bgpu_scene_create(ctx.h, C.byref(flat.desc), C.byref(h)))
         self.h = h
 
     def close(self):

@@ -395,6 +395,11 @@ class FlatScene:
         self.film = b.film
         self.lens = getattr(b, "lens", None)
 
+    def save(self, path):
+        """Writes the flattened scene as a flat binary scene file
+        (bgpu_scene_write); `SceneFile(path)` reads it back."""
+        abi.check(abi.load().bgpu_scene_write(C.byref(self.desc), str(path).encode()))
+
     def view(self, width, height):
         """The scene's camera re-made for a `width` x `height` image (a
         RenderJob's camera follows its imageSize, Rendering.hs:133-137)."""
@@ -402,3 +407,43 @@ class FlatScene:
             raise ValueError("scene was not built with SceneBuilder.look")
         c2w, fov, lr, fd, ft, fp = self.lens
         return View.perspective(c2w, fov, width, height, lr, fd, ft, fp)
+
+
+class SceneFile:
+    """A flat binary scene file read by bgpu_scene_read: `desc` is the
+    descriptor (its arrays live in the library's block until `close`), and
+    `film`, `num_triangles`, `num_shapes` and `num_prims` mirror FlatScene's."""
+
+    def __init__(self, path):
+        self._lib = abi.load()
+        p = C.POINTER(abi.SceneDesc)()
+        abi.check(self._lib.bgpu_scene_read(str(path).encode(), C.byref(p)))
+        self._p = p
+        self.desc = p.contents
+        d = self.desc
+        self.film = (d.film.width, d.film.height, d.film.filter, list(d.film.fparams))
+        self.num_triangles = d.num_triangles
+        self.num_shapes = d.num_shapes
+        self.num_prims = d.num_triangles + d.num_shapes
+
+    def array(self, name, dtype, cols):
+        """The descriptor's array `name` (vertices, normals, uvs, indices,
+        tri_mesh) as a numpy copy, or None."""
+        ptr = getattr(self.desc, name)
+        if not ptr:
+            return None
+        n = self.desc.num_vertices if name in ("vertices", "normals", "uvs") else self.desc.num_triangles
+        a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))), (n * cols,))
+        return a.reshape(n, cols).copy() if cols > 1 else a.copy()
+
+    def close(self):
+        if self._p:
+            abi.check(self._lib.bgpu_scene_desc_free(self._p))
+            self._p = None
+            self.desc = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -264,6 +264,18 @@ int bgpu_scene_destroy(bgpu_scene* scene);
 /* Device bytes held by a scene. */
 int bgpu_scene_bytes(const bgpu_scene* scene, size_t* bytes);
 
+/* Flat binary scene files (no counterpart in the reference, which parses
+ * .bling text on every run, IO/ParserCore.hs:114-147).  bgpu_scene_write
+ * stores a descriptor, its arrays and light maps in one file; the record
+ * layouts are recorded in it and checked on reading.  bgpu_scene_read
+ * returns a descriptor owned by the library, whose arrays point into one
+ * block, freed with bgpu_scene_desc_free; a malformed or truncated file is
+ * BGPU_E_INVALID.  bgpu_scene_load reads a file and creates the scene. */
+int bgpu_scene_write(const bgpu_scene_desc* desc, const char* path);
+int bgpu_scene_read(const char* path, bgpu_scene_desc** out);
+int bgpu_scene_desc_free(bgpu_scene_desc* desc);
+int bgpu_scene_load(bgpu_ctx* ctx, const char* path, bgpu_scene** out);
+
 /* `intersect` of the accelerator primitive (Primitive/KdTree.hs:240-242). */
 int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
                            bgpu_hits* hits, int flags);

@@ -402,3 +402,22 @@ def test_textured_scene_matches_oracle(gpu_ctx):
     frac, zmean, zmax = compare(g, gsq, o2, osq2)
     assert frac < 0.015, (frac, zmean, zmax)
     assert abs(zmean) < 0.08, zmean
+
+
+def test_scene_file_renders_like_the_flat_scene(gpu_ctx, tmp_path):
+    """A scene loaded from its flat binary file (bgpu_scene_load) renders
+    the image of the scene created from the descriptor."""
+    from tests import cases
+    flat = cases.textured_scene(res=32)
+    path = tmp_path / "t.bgs"
+    flat.save(path)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, passes=2,
+                   seed=13, variance=True)
+    a, _ = render_gpu(gpu_ctx, flat, spec)
+    gs = gpu_ctx.load_scene(path)
+    try:
+        b, _ = GpuRenderer(spec).render_plain(gs)
+    finally:
+        gs.close()
+    assert np.array_equal(a[..., 0], b[..., 0])
+    np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6)
