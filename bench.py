@@ -512,8 +512,27 @@ def serve_leg(ctx, args):
     named = serve_scenes(args.serve_max_tris)
     flats = [f for _, f in named]
     log("serve: %d scenes generated in %.1fs" % (len(flats), time.time() - t0))
+    # scene admission: creation with the host BVH build, then with the device
+    # build (the one kept); both include the upload of the shading data
     t0 = time.time()
-    gss = [ctx.scene(f) for f in flats]
+    gss, admission = [], []
+    for (name, f) in named:
+        ctx.config_set("bvh.device", 0)
+        t1 = time.perf_counter()
+        g = ctx.scene(f)
+        host_s = time.perf_counter() - t1
+        g.close()
+        ctx.config_set("bvh.device", 1)
+        t1 = time.perf_counter()
+        g = ctx.scene(f)
+        dev_s = time.perf_counter() - t1
+        info = g.bvh_info()
+        gss.append(g)
+        admission.append({"scene": name, "tris": f.num_triangles, "host_build_s": round(host_s, 3),
+                          "device_build_s": round(dev_s, 3), "device_tree": info["device"],
+                          "device_bvh2_ms": round(info["build_ms"], 1)})
+        log("serve: scene %s %d tris created in %.3fs (host BVH) / %.3fs (device BVH)" % (
+            name, f.num_triangles, host_s, dev_s))
     log("serve: uploaded + BVHs in %.1fs, %.1f MB" % (time.time() - t0,
                                                        sum(g.device_bytes() for g in gss) / 1e6))
     res_set = [int(v) for v in args.serve_res.split(",")]
@@ -558,6 +577,7 @@ def serve_leg(ctx, args):
                     % (", ".join("%s %d tris" % (k, f.num_triangles) for k, f in named),
                        res_set, spp_set),
         "rates": rows,
+        "scene_admission": admission,
     }
 
 

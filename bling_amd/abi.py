@@ -98,7 +98,7 @@ class JobDesc(C.Structure):
 
 class BvhInfo(C.Structure):
     _fields_ = [("width", C.c_int32), ("nodes", C.c_int32), ("leaves", C.c_int32),
-                ("max_depth", C.c_int32), ("stack_bound", C.c_int32), ("pad", C.c_int32),
+                ("max_depth", C.c_int32), ("stack_bound", C.c_int32), ("device", C.c_int32),
                 ("build_ms", C.c_double)]
 
 

@@ -131,7 +131,7 @@ int bgpu_scene_destroy(bgpu_scene* scene) {
 int bgpu_scene_bvh_info(const bgpu_scene* scene, bgpu_bvh_info* out) {
   if (!scene || !out) return fail(BGPU_E_INVALID, "bgpu_scene_bvh_info: NULL argument");
   const BuildStats& b = scene->build;
-  *out = bgpu_bvh_info{b.width, b.nodes, b.leaves, b.max_depth, b.stack_bound, 0, b.build_ms};
+  *out = bgpu_bvh_info{b.width, b.nodes, b.leaves, b.max_depth, b.stack_bound, b.device, b.build_ms};
   return BGPU_OK;
 }
 

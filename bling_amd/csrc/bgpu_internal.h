@@ -14,6 +14,7 @@
 
 #include "../../include/bgpu.h"
 #include "bvh_build.h"
+#include "bvh_device.h"
 #include "isect.h"
 #include "pt.h"
 
@@ -76,6 +77,7 @@ struct Ctx {
   TraceConfig trace_cfg;
   PtConfig pt_cfg;
   int bvh_width = 4;              // children per BVH node of scenes created next
+  int bvh_device = 1;             // 1: build large triangle trees on the GPU
   ~Ctx();
 };
 

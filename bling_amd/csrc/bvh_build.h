@@ -32,6 +32,7 @@ struct BuildParams {
 struct BuildStats {
   int nodes = 0, leaves = 0, max_depth = 0, max_leaf_size = 0;
   int width = 2, stack_bound = 0;
+  int device = 0;  // 1: the triangle tree was built on the GPU
   double sah = 0.0, build_ms = 0.0;
 };
 

@@ -875,6 +875,12 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     ctx->bvh_width = (int)value;
     return BGPU_OK;
   }
+  if (k == "bvh.device") {
+    if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "bvh.device must be 0 or 1");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->bvh_device = (int)value;
+    return BGPU_OK;
+  }
   PtConfig next = ctx->pt_cfg;
   if (k == "pt.block") {
     if (!block_ok(value)) return fail(BGPU_E_INVALID, "pt.block must be 64, 128 or 256");
@@ -915,6 +921,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   const std::string k(key);
   if (k == "trace.block") *value = ctx->trace_cfg.block;
   else if (k == "bvh.width") *value = ctx->bvh_width;
+  else if (k == "bvh.device") *value = ctx->bvh_device;
   else if (k == "trace.persistent") *value = ctx->trace_cfg.persistent;
   else if (k == "pt.block") *value = ctx->pt_cfg.block;
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;

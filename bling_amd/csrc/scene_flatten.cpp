@@ -121,9 +121,9 @@ double sigma_bound(const float* m) {
 }
 // Runs f(begin, end) over [0, n) in contiguous chunks on up to 16 threads.
 template <class F>
-void parallel_for(size_t n, F&& f) {
+void parallel_for(size_t n, F&& f, size_t grain = 32768) {
   const size_t t = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
-  const size_t k = std::min(t, n / 32768);
+  const size_t k = std::min(t, n / grain);
   if (k <= 1) {
     f((size_t)0, n);
     return;
@@ -170,18 +170,53 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   sc->nshape = d->num_shapes;
   const uint32_t np = d->num_triangles + d->num_shapes;
 
+  // shading data first: the device build reads its vertices and indices
+  if (int rc = build_pt_data(ctx, d, sc)) return rc;
+
+  // -- the triangle tree on the GPU when the scene is large enough; it
+  // reports a fallback for inputs the host builder handles differently
+  DeviceBvh dbvh;
+  bool dev_tri = false;
+  if (ctx->bvh_device && d->num_triangles > (uint32_t)kSmall) {
+    bool fallback = true;
+    if (int rc = dbvh.build(ctx, sc->verts.as<float>(), sc->tri_idx.as<int32_t>(), (int)d->num_triangles, &fallback))
+      return rc;
+    dev_tri = !fallback;
+  }
+
   // -- primitive bounds and the root box fold (KdTree.hs:111)
-  std::vector<Box> pb(np);
-  parallel_for(d->num_triangles, [&](size_t tb, size_t te) {
-    for (size_t t = tb; t < te; ++t) {
-      Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
-      for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * t + k]);
-      pb[t] = b;
-    }
-  });
-  for (uint32_t s = 0; s < d->num_shapes; ++s) pb[d->num_triangles + s] = shape_world_bounds(d->shapes[s]);
+  std::vector<Box> pb(dev_tri ? d->num_shapes : np);
+  const uint32_t soff = dev_tri ? 0 : d->num_triangles;  // shapes' first index in pb
   Box root = empty_box();  // the fold in primitive order
-  for (uint32_t i = 0; i < np; ++i) extend_b(root, pb[i]);
+  if (!dev_tri) {
+    parallel_for(d->num_triangles, [&](size_t tb, size_t te) {
+      for (size_t t = tb; t < te; ++t) {
+        Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
+        for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * t + k]);
+        pb[t] = b;
+      }
+    });
+    for (uint32_t i = 0; i < d->num_triangles; ++i) extend_b(root, pb[i]);
+  } else {
+    // the same fold without storing the boxes: per-chunk folds combined in
+    // chunk order (extendAABB keeps the earlier value on ties)
+    std::vector<Box> part(64, empty_box());
+    const size_t nt = d->num_triangles;
+    parallel_for(64, [&](size_t cb, size_t ce) {
+      for (size_t c = cb; c < ce; ++c) {
+        Box acc = empty_box();
+        for (size_t t = nt * c / 64; t < nt * (c + 1) / 64; ++t) {
+          Box b = empty_box();
+          for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * t + k]);
+          extend_b(acc, b);
+        }
+        part[c] = acc;
+      }
+    }, 1);
+    for (const Box& b : part) extend_b(root, b);
+  }
+  for (uint32_t s = 0; s < d->num_shapes; ++s) pb[soff + s] = shape_world_bounds(d->shapes[s]);
+  for (uint32_t s = 0; s < d->num_shapes; ++s) extend_b(root, pb[soff + s]);
 
   // -- BVHs: one over the triangles, one over the shapes, built binary and
   // collapsed to the context's width into one node array
@@ -190,7 +225,13 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   BuildParams bp;
   float aniso = 1.0f;
   std::vector<float> hfac;  // per shape: sigma(o2w) sigma(w2o)^2 / (2 r), +inf if not a sphere
-  if (d->num_triangles) {
+  if (dev_tri) {
+    sc->build = BuildStats{};
+    sc->build.device = 1;
+    sc->build.leaves = dbvh.leaves;
+    sc->build.max_depth = dbvh.max_depth;
+    sc->build.build_ms = dbvh.build_ms;
+  } else if (d->num_triangles) {
     if (d->num_shapes == 0) {
       build_bvh2(pb, bp, &tnodes, &order, &sc->build);
     } else {
@@ -199,12 +240,12 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     }
   }
   if (d->num_shapes) {
-    std::vector<Box> sb(pb.begin() + d->num_triangles, pb.end());
+    std::vector<Box> sb(pb.begin() + soff, pb.end());
     std::vector<int32_t> so;
     BuildStats sst;
     build_bvh2(sb, bp, &snodes, &so, &sst);
     // the shape leaves follow the triangle leaves in the leaf records
-    const int32_t loff = (int32_t)order.size();
+    const int32_t loff = (int32_t)d->num_triangles;
     for (BNode& n : snodes)
       for (int s = 0; s < 2; ++s)
         if (n.link[s] < 0) {
@@ -228,16 +269,26 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     }
   }
   // the widest width whose worst-case traversal stack fits the device stack
+  // `nodes` holds the host-collapsed trees; with the device triangle tree
+  // it holds the shape tree only, placed after the dev_count device nodes
   std::vector<WNode> nodes;
   int32_t tri_root = -1, shape_root = -1;
+  int dev_count = 0;
   int width = ctx->bvh_width;
   for (;; width /= 2) {
     nodes.clear();
     nodes.reserve(tnodes.size() + snodes.size() + 2);
-    tri_root = tnodes.empty() ? -1 : collapse_bvh(tnodes, 0, width, &nodes);
-    shape_root = snodes.empty() ? -1 : collapse_bvh(snodes, 0, width, &nodes);
     int bound = 0;
-    if (tri_root >= 0) bound = std::max(bound, stack_bound(nodes, tri_root));
+    if (dev_tri) {
+      int tb = 0;
+      if (int rc = dbvh.collapse(ctx, width, &dev_count, &tb)) return rc;
+      tri_root = 0;
+      bound = tb;
+    } else {
+      tri_root = tnodes.empty() ? -1 : collapse_bvh(tnodes, 0, width, &nodes);
+      if (tri_root >= 0) bound = std::max(bound, stack_bound(nodes, tri_root));
+    }
+    shape_root = snodes.empty() ? -1 : collapse_bvh(snodes, 0, width, &nodes);
     if (shape_root >= 0) bound = std::max(bound, stack_bound(nodes, shape_root));
     sc->build.stack_bound = bound;
     if (bound <= kStackCapacity || width == 2) break;
@@ -245,7 +296,8 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   if (sc->build.stack_bound > kStackCapacity)
     return fail(BGPU_E_INVALID, "BVH too deep for the traversal stack");
   sc->build.width = width;
-  sc->build.nodes = (int)nodes.size();
+  sc->build.nodes = dev_count + (int)nodes.size();
+  const int obase = dev_tri ? (int)d->num_triangles : 0;  // leaf position of order[0]
 
   // node layout, 8 * width floats: lo.x[w] hi.x[w] lo.y[w] hi.y[w] lo.z[w]
   // hi.z[w] link[w] h[w]; unused slots carry the link kNoChild and repeat
@@ -265,7 +317,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     }
     const int enc = ~link;
     for (int k = enc >> kLeafCountBits; k < (enc >> kLeafCountBits) + (enc & kMaxLeaf); ++k)
-      m = std::max(m, hfac[order[k] - (int32_t)d->num_triangles]);
+      m = std::max(m, hfac[order[k - obase] - (int32_t)d->num_triangles]);
     return m;
   };
   if (shape_root >= 0) (void)hmax(shape_root);
@@ -281,11 +333,14 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
           o[(2 * k) * W + c] = b.lo[k];
           o[(2 * k + 1) * W + c] = b.hi[k];
         }
-        o[6 * W + c] = bitsf(used ? n.link[c] : kNoChild);
+        const int32_t lk = used ? n.link[c] : kNoChild;
+        o[6 * W + c] = bitsf(lk >= 0 && lk != kNoChild ? lk + dev_count : lk);
         o[7 * W + c] = hchild[i * W + (used ? c : 0)];
       }
     }
   });
+  // leaf records of order[k] at position obase + k (the device writes the
+  // triangles' records itself)
   std::vector<float> hl(12 * std::max<size_t>(order.size(), 1), 0.0f);
   parallel_for(order.size(), [&](size_t kb, size_t ke) {
     for (size_t k = kb; k < ke; ++k) {
@@ -319,9 +374,23 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   }
 
   hipStream_t st = ctx->stream;
-  if (int rc = sc->nodes.upload(ctx, hn.data(), hn.size() * 4, st)) return rc;
-  if (int rc = sc->leaf.upload(ctx, hl.data(), hl.size() * 4, st)) return rc;
-  if (int rc = build_pt_data(ctx, d, sc)) return rc;
+  if (dev_tri) {
+    const size_t nb = (size_t)8 * W * 4;  // bytes per node
+    if (int rc = sc->nodes.alloc(ctx, nb * (dev_count + nodes.size()))) return rc;
+    BGPU_HIP(hipMemcpyAsync(sc->nodes.ptr, dbvh.wide_nodes(), nb * dev_count, hipMemcpyDeviceToDevice, st));
+    if (!nodes.empty())
+      BGPU_HIP(hipMemcpyAsync(static_cast<char*>(sc->nodes.ptr) + nb * dev_count, hn.data(), nb * nodes.size(),
+                              hipMemcpyHostToDevice, st));
+    if (int rc = sc->leaf.alloc(ctx, 48ull * (d->num_triangles + d->num_shapes))) return rc;
+    if (int rc = dbvh.leaves_to(ctx, sc->verts.as<float>(), sc->tri_idx.as<int32_t>(), sc->leaf.as<float4>()))
+      return rc;
+    if (!order.empty())
+      BGPU_HIP(hipMemcpyAsync(static_cast<char*>(sc->leaf.ptr) + 48ull * d->num_triangles, hl.data(),
+                              48ull * order.size(), hipMemcpyHostToDevice, st));
+  } else {
+    if (int rc = sc->nodes.upload(ctx, hn.data(), hn.size() * 4, st)) return rc;
+    if (int rc = sc->leaf.upload(ctx, hl.data(), hl.size() * 4, st)) return rc;
+  }
   // shape light indices are assigned by build_pt_data (light list order)
   for (uint32_t s = 0; s < d->num_shapes; ++s) hs[s].light = sc->pt.shape_light_host.empty() ? -1 : sc->pt.shape_light_host[s];
   if (int rc = sc->shapes.upload(ctx, hs.data(), hs.size() * sizeof(DShape), st)) return rc;
@@ -335,10 +404,10 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     ds.root_lo[k] = root.lo[k];
     ds.root_hi[k] = root.hi[k];
   }
-  ds.num_nodes = (int32_t)nodes.size();
+  ds.num_nodes = (int32_t)(dev_count + nodes.size());
   ds.width = width;
   ds.tri_root = tri_root;
-  ds.shape_root = shape_root;
+  ds.shape_root = shape_root < 0 ? -1 : shape_root + dev_count;
   ds.shape_pad_k = kSphereK0 * aniso;
   ds.shape_floor_k = 64.0f * 0x1p-24f * aniso;
   sc->film = d->film;

@@ -306,7 +306,9 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * none): "bvh.width" (2/4/8 children per BVH node of scenes created
  * afterwards; a scene whose worst-case traversal stack would not fit falls
  * back to a narrower width), "trace.block" (64/128/256, intersect kernels),
- * "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
+ * "bvh.device" (1, the default: scenes created afterwards with more than
+ * 1024 triangles build their triangle tree on the GPU, the same tree as the
+ * host builder's; 0: on the host), "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
  * ray when theirs is done; 0: one ray per thread), "pt.block"
  * (64/128/256, bounce kernels), "pt.persistent" (1: the bounce loop's
  * traversal runs as persistent passes that refill lanes with new rays;
@@ -329,8 +331,8 @@ typedef struct {
   int32_t leaves;
   int32_t max_depth;    /* of the binary build */
   int32_t stack_bound;  /* worst-case traversal stack entries */
-  int32_t pad;
-  double build_ms;
+  int32_t device;       /* 1: the triangle tree was built on the GPU */
+  double build_ms;      /* binary build of the triangle tree */
 } bgpu_bvh_info;
 int bgpu_scene_bvh_info(const bgpu_scene* scene, bgpu_bvh_info* out);
 
