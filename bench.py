@@ -21,6 +21,8 @@ Two more legs are reported in the same line (skip them with --legs):
           the plain path (one job at a time, in arrival order): Mrays/s,
           samples/s and p50/p99 job latency per arrival rate, and whether
           every job's image is the same on both paths.
+The optional `build` leg (--legs build) times scene creation of the largest
+serving scene with the host and the device BVH build.
 
 Prints one JSON line.  Only N=1 is supported: this build targets one GPU.
 """
@@ -78,6 +80,7 @@ def main():
     ap.add_argument("--serve-max-tris", type=float, default=8e6)
     ap.add_argument("--serve-res", default="128,256,512,1024")
     ap.add_argument("--serve-spp", default="4,16,64")
+    ap.add_argument("--build-reps", type=int, default=3, help="scene creations per builder in the build leg")
     ap.add_argument("--config", default="",
                     help="tuning knobs, key=value[,key=value] (bgpu_config_set keys)")
     args = ap.parse_args()
@@ -96,7 +99,7 @@ def main():
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
-    for k in ("bvh.width", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves",
+    for k in ("bvh.width", "bvh.device", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves",
               "pt.persistent", "pt.sort"):
         knobs[k] = ctx.config_get(k)
     out = {
@@ -122,6 +125,8 @@ def main():
         out["config"]["divergent"] = divergent_leg(ctx, args)
     if "serve" in legs:
         out["config"]["serve"] = serve_leg(ctx, args)
+    if "build" in legs:
+        out["config"]["build"] = build_leg(ctx, args)
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -422,6 +427,34 @@ def divergent_leg(ctx, args):
 
 
 # ------------------------------------------------------------------- serve
+def build_leg(ctx, args):
+    """Scene creation (upload, BVH build and collapse) of the largest serving
+    scene, a displaced grid of about --serve-max-tris triangles, with the
+    host BVH build and with the device build; best of --build-reps."""
+    from bling_amd import scenes
+    t0 = time.time()
+    flat = scenes.grid_scene(int(args.serve_max_tris), seed=6)
+    log("build: grid %d tris generated in %.1fs" % (flat.num_triangles, time.time() - t0))
+    res = {}
+    for dev in (0, 1):
+        ctx.config_set("bvh.device", dev)
+        best, info = None, None
+        for _ in range(args.build_reps):
+            t1 = time.perf_counter()
+            g = ctx.scene(flat)
+            dt = time.perf_counter() - t1
+            info = g.bvh_info()
+            g.close()
+            best = dt if best is None else min(best, dt)
+        res["device" if dev else "host"] = {"create_s": round(best, 4), "bvh2_ms": round(info["build_ms"], 1),
+                                            "device_tree": info["device"], "nodes": info["nodes"]}
+        log("build: bvh.device=%d: scene created in %.3fs (binary build %.1f ms)" % (dev, best, info["build_ms"]))
+    ctx.config_set("bvh.device", 1)
+    return {"workload": "scene creation of a displaced grid of %d triangles (upload, BVH build, collapse to "
+                        "width %d); best of %d" % (flat.num_triangles, ctx.config_get("bvh.width"), args.build_reps),
+            **res, "speedup": round(res["host"]["create_s"] / max(1e-9, res["device"]["create_s"]), 2)}
+
+
 def serve_scenes(max_tris):
     """The Cornell box plus 7 procedural scenes, 0.1M..max_tris triangles
     (log-spaced), alternating box stacks, displaced grids and an icosphere."""

@@ -16,6 +16,8 @@
 //      scan over the range's chunks;
 //   3. ranges of at most kSmall primitives: one workgroup builds the whole
 //      subtree depth first (bins in LDS, a wave-ballot stable partition);
+//      each partition moves a range between the two primitive buffers, and
+//      leaves copy their primitives back to the first;
 //   4. collapse to width W level by level (opening the largest inner child
 //      first, as collapse_bvh does) and the exact traversal stack bound,
 //      bottom up;
@@ -58,7 +60,9 @@ struct Ext {
   float lo[3], hi[3], cmin[3], cmax[3];
 };
 struct BTask {
-  int32_t b, e, depth, parent, slot, pad[3];
+  int32_t b, e, depth, parent, slot;
+  int32_t buf;  // which primitive buffer holds the range (0: A, 1: B)
+  int32_t pad[2];
   Ext x;
 };
 struct Dec {
@@ -229,6 +233,57 @@ __device__ __forceinline__ void bin_prim(uint32_t* bins, const DPrim& p, const b
   }
 }
 
+// Bins the primitives [i0, i1) of one thread, consecutive ones: per axis
+// the run of primitives that fall into the same bin is reduced in registers
+// and merged into the LDS bins once (neighbouring primitives of a range are
+// close in space, so runs are long and LDS atomics few).
+__device__ void bin_run(uint32_t* bins, const DPrim* __restrict__ src, int i0, int i1, const bool* use,
+                        const float* cmin, const float* sc) {
+  int cur[3] = {-1, -1, -1};
+  float acc[3][12];
+  uint32_t cnt[3] = {0, 0, 0};
+  auto flush = [&](int ax) {
+    uint32_t* w = bins + (ax * NB + cur[ax]) * kBinWords;
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&w[k], ord(acc[ax][k]));
+      atomicMax(&w[3 + k], ord(acc[ax][3 + k]));
+      atomicMin(&w[6 + k], ord(acc[ax][6 + k]));
+      atomicMax(&w[9 + k], ord(acc[ax][9 + k]));
+    }
+    atomicAdd(&w[12], cnt[ax]);
+  };
+  for (int i = i0; i < i1; ++i) {
+    const DPrim p = src[i];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (!use[ax]) continue;
+      const int bi = bin_of(p.c[ax], cmin[ax], sc[ax]);
+      if (bi != cur[ax]) {
+        if (cur[ax] >= 0) flush(ax);
+        cur[ax] = bi;
+        cnt[ax] = 0;
+        for (int k = 0; k < 3; ++k) {
+          acc[ax][k] = p.lo[k];
+          acc[ax][3 + k] = p.hi[k];
+          acc[ax][6 + k] = p.c[k];
+          acc[ax][9 + k] = p.c[k];
+        }
+      } else {
+        for (int k = 0; k < 3; ++k) {
+          acc[ax][k] = fminf(acc[ax][k], p.lo[k]);
+          acc[ax][3 + k] = fmaxf(acc[ax][3 + k], p.hi[k]);
+          acc[ax][6 + k] = fminf(acc[ax][6 + k], p.c[k]);
+          acc[ax][9 + k] = fmaxf(acc[ax][9 + k], p.c[k]);
+        }
+      }
+      cnt[ax]++;
+    }
+  }
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (use[ax] && cur[ax] >= 0) flush(ax);
+}
+
 __device__ __forceinline__ void bin_setup(const Ext& x, bool* use, float* cmin, float* sc) {
   for (int ax = 0; ax < 3; ++ax) {
     const float ext = x.cmax[ax] - x.cmin[ax];
@@ -293,7 +348,8 @@ __global__ void k_root_task(const uint32_t* ext_words, int n, BTask* task) {
   t.depth = 0;
   t.parent = -1;
   t.slot = 0;
-  t.pad[0] = t.pad[1] = t.pad[2] = 0;
+  t.buf = 0;
+  t.pad[0] = t.pad[1] = 0;
   for (int k = 0; k < 3; ++k) {
     t.x.lo[k] = unord(ext_words[k]);
     t.x.hi[k] = unord(ext_words[3 + k]);
@@ -348,8 +404,8 @@ __global__ void k_clear_bins(uint32_t* bins, int nwords) {
 }
 
 __global__ __launch_bounds__(kLevelBlock) void k_bin(const BTask* __restrict__ tasks, int ntask,
-                                                    const int* __restrict__ blk_first,
-                                                    const DPrim* __restrict__ A, uint32_t* gbins) {
+                                                    const int* __restrict__ blk_first, const DPrim* P0,
+                                                    const DPrim* P1, uint32_t* gbins) {
   __shared__ uint32_t bins[kBinsWords];
   const int blk = blockIdx.x;
   if (blk >= blk_first[ntask]) return;
@@ -362,7 +418,9 @@ __global__ __launch_bounds__(kLevelBlock) void k_bin(const BTask* __restrict__ t
   float cmin[3], sc[3];
   bin_setup(task.x, use, cmin, sc);
   const int cb = task.b + (blk - blk_first[t]) * kChunk, ce = min(task.e, cb + kChunk);
-  for (int i = cb + (int)threadIdx.x; i < ce; i += blockDim.x) bin_prim(bins, A[i], use, cmin, sc);
+  constexpr int per = kChunk / kLevelBlock;
+  const int i0 = cb + (int)threadIdx.x * per;
+  bin_run(bins, task.buf ? P1 : P0, i0, min(ce, i0 + per), use, cmin, sc);
   __syncthreads();
   uint32_t* g = gbins + (size_t)t * kBinsWords;
   for (int w = threadIdx.x; w < kBinsWords; w += blockDim.x) {
@@ -375,15 +433,16 @@ __global__ __launch_bounds__(kLevelBlock) void k_bin(const BTask* __restrict__ t
   }
 }
 
-__device__ void emit_child(int b, int e, int depth, int parent, int slot, const Ext& x, BTask* next, int next_cap,
-                           BTask* small, int small_cap, Counters* ctr) {
+__device__ void emit_child(int b, int e, int depth, int parent, int slot, int buf, const Ext& x, BTask* next,
+                           int next_cap, BTask* small, int small_cap, Counters* ctr) {
   BTask c;
   c.b = b;
   c.e = e;
   c.depth = depth;
   c.parent = parent;
   c.slot = slot;
-  c.pad[0] = c.pad[1] = c.pad[2] = 0;
+  c.buf = buf;
+  c.pad[0] = c.pad[1] = 0;
   c.x = x;
   if (e - b > kSmall) {
     const int q = atomicAdd(&ctr->next, 1);
@@ -421,8 +480,10 @@ __global__ void k_split(const BTask* __restrict__ tasks, int ntask, const uint32
       d.node = node;
       if (task.parent >= 0) nodes[task.parent].link[task.slot] = node;
       atomicMax(&ctr->max_depth, task.depth);
-      emit_child(task.b, task.b + d.nleft, task.depth + 1, node, 0, d.le, next, next_cap, small, small_cap, ctr);
-      emit_child(task.b + d.nleft, task.e, task.depth + 1, node, 1, d.re, next, next_cap, small, small_cap, ctr);
+      emit_child(task.b, task.b + d.nleft, task.depth + 1, node, 0, task.buf ^ 1, d.le, next, next_cap, small,
+                 small_cap, ctr);
+      emit_child(task.b + d.nleft, task.e, task.depth + 1, node, 1, task.buf ^ 1, d.re, next, next_cap, small,
+                 small_cap, ctr);
     }
   }
   decs[t] = d;
@@ -431,7 +492,7 @@ __global__ void k_split(const BTask* __restrict__ tasks, int ntask, const uint32
 // left-side primitives per chunk of the splitting tasks
 __global__ __launch_bounds__(kLevelBlock) void k_count(const BTask* __restrict__ tasks, int ntask,
                                                       const int* __restrict__ blk_first, const Dec* __restrict__ decs,
-                                                      const DPrim* __restrict__ A, int* cnt) {
+                                                      const DPrim* P0, const DPrim* P1, int* cnt) {
   __shared__ int s;
   const int blk = blockIdx.x;
   if (blk >= blk_first[ntask]) return;
@@ -444,6 +505,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_count(const BTask* __restrict__
   if (threadIdx.x == 0) s = 0;
   __syncthreads();
   const BTask& task = tasks[t];
+  const DPrim* A = task.buf ? P1 : P0;
   const int cb = task.b + (blk - blk_first[t]) * kChunk, ce = min(task.e, cb + kChunk);
   int c = 0;
   for (int i = cb + (int)threadIdx.x; i < ce; i += blockDim.x) c += bin_of(A[i].c[d.axis], d.cmin, d.sc) <= d.bin;
@@ -489,8 +551,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_scan_chunks(int ntask, const in
 
 __global__ __launch_bounds__(kLevelBlock) void k_scatter(const BTask* __restrict__ tasks, int ntask,
                                                         const int* __restrict__ blk_first, const Dec* __restrict__ decs,
-                                                        const int* __restrict__ off, const DPrim* __restrict__ A,
-                                                        DPrim* __restrict__ B) {
+                                                        const int* __restrict__ off, DPrim* P0, DPrim* P1) {
   __shared__ int sh[kLevelBlock];
   const int blk = blockIdx.x;
   if (blk >= blk_first[ntask]) return;
@@ -498,6 +559,8 @@ __global__ __launch_bounds__(kLevelBlock) void k_scatter(const BTask* __restrict
   const Dec d = decs[t];
   if (d.kind != kDecSplit) return;
   const BTask& task = tasks[t];
+  const DPrim* A = task.buf ? P1 : P0;  // the range moves to the other buffer
+  DPrim* B = task.buf ? P0 : P1;
   const int cb = task.b + (blk - blk_first[t]) * kChunk, ce = min(task.e, cb + kChunk);
   int lrun = task.b + off[blk];
   int rrun = task.b + d.nleft + (cb - task.b - off[blk]);
@@ -519,35 +582,105 @@ __global__ __launch_bounds__(kLevelBlock) void k_scatter(const BTask* __restrict
   }
 }
 
-__global__ __launch_bounds__(kLevelBlock) void k_copyback(const BTask* __restrict__ tasks, int ntask,
-                                                         const int* __restrict__ blk_first, const Dec* __restrict__ decs,
-                                                         const DPrim* __restrict__ B, DPrim* __restrict__ A) {
-  const int blk = blockIdx.x;
-  if (blk >= blk_first[ntask]) return;
-  const int t = task_of_block(blk_first, ntask, blk);
-  if (decs[t].kind != kDecSplit) return;
-  const BTask& task = tasks[t];
-  const int cb = task.b + (blk - blk_first[t]) * kChunk, ce = min(task.e, cb + kChunk);
-  for (int i = cb + (int)threadIdx.x; i < ce; i += blockDim.x) A[i] = B[i];
+// ---- phase 3: one workgroup per small subtree ------------------------------
+
+// The split decision of `decide`, evaluated by the 64 lanes of one wave:
+// lane l < 33 prices the split after bin l % 11 on axis l / 11, and a wave
+// arg-min keeps the first cheapest candidate in (axis, bin) order, as the
+// serial loop's strict comparison does.  Lane 0 writes the result to d.
+__device__ void decide_wave(const uint32_t* bins, int n, int depth, const Ext& x, Dec& d, int lane) {
+  const bool balanced = depth > kBalancedDepth;
+  const float parea = area3(x.lo, x.hi);
+  float cost = INFINITY;
+  int key = 0x7fffffff;
+  if (!balanced && parea > 0.0f && lane < 3 * (NB - 1)) {
+    const int ax = lane / (NB - 1), sb = lane % (NB - 1);
+    if (x.cmax[ax] - x.cmin[ax] > 0.0f) {
+      float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      int cl = 0, cr = 0;
+      for (int i = 0; i < NB; ++i) {
+        const uint32_t* w = bins + (ax * NB + i) * kBinWords;
+        if (i <= sb) {
+          for (int k = 0; k < 3; ++k) {
+            llo[k] = smin(llo[k], unord(w[k]));
+            lhi[k] = smax(lhi[k], unord(w[3 + k]));
+          }
+          cl += (int)w[12];
+        } else {
+          for (int k = 0; k < 3; ++k) {
+            rlo[k] = smin(rlo[k], unord(w[k]));
+            rhi[k] = smax(rhi[k], unord(w[3 + k]));
+          }
+          cr += (int)w[12];
+        }
+      }
+      if (cl != 0 && cr != 0) {
+        const float c = 1.0f + (area3(llo, lhi) * cl + area3(rlo, rhi) * cr) / parea;
+        if (c < INFINITY) {
+          cost = c;
+          key = lane;
+        }
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float oc = __shfl_xor(cost, o);
+    const int ok = __shfl_xor(key, o);
+    if (oc < cost || (oc == cost && ok < key)) {
+      cost = oc;
+      key = ok;
+    }
+  }
+  if (lane != 0) return;
+  const bool found = key != 0x7fffffff;
+  const int best_axis = found ? key / (NB - 1) : -1, best_bin = found ? key % (NB - 1) : -1;
+  const float best_cost = cost;
+  d.kind = kDecFallback;
+  if (!balanced && n <= kMaxLeafSah && (best_axis < 0 || (float)n <= best_cost)) d.kind = kDecLeaf;
+  else if (!balanced && best_axis < 0 && n <= kMaxLeaf) d.kind = kDecLeaf;
+  else if (balanced && n <= kMaxLeaf) d.kind = kDecLeaf;
+  else if (best_axis >= 0) {
+    d.kind = kDecSplit;
+    d.axis = best_axis;
+    d.bin = best_bin;
+    d.cmin = x.cmin[best_axis];
+    d.sc = (float)NB / (x.cmax[best_axis] - x.cmin[best_axis]);
+    int nl = 0;
+    for (int i = 0; i <= best_bin; ++i) nl += (int)bins[(best_axis * NB + i) * kBinWords + 12];
+    d.nleft = nl;
+    bins_extent(bins, best_axis, 0, best_bin, d.le);
+    bins_extent(bins, best_axis, best_bin + 1, NB - 1, d.re);
+  }
 }
 
-// ---- phase 3: one workgroup per small subtree ------------------------------
+// A pending range of a subtree: its task and which buffer holds its
+// primitives (0: A, 1: B).  Partitions alternate between the buffers; a
+// leaf's primitives are copied to A, where the leaf records are read.
+struct SubTask {
+  BTask t;
+  int32_t buf, pad[3];
+};
 
 __global__ __launch_bounds__(kTreeBlock) void k_subtree(const BTask* __restrict__ small, int nsmall, DPrim* A,
                                                        DPrim* B, BNode* nodes, int node_cap, Counters* ctr) {
   __shared__ uint32_t bins[kBinsWords];
-  __shared__ BTask stack[kTreeStack];
+  __shared__ SubTask stack[kTreeStack];
+  __shared__ SubTask cur;
   __shared__ Dec dec;
-  __shared__ int sp, base, used, cap, stop, iters;
+  __shared__ int sp, base, used, cap, stop, iters, nleaves, maxd;  // counters merged once at the end
   const int tid = threadIdx.x;
   if (blockIdx.x >= nsmall) return;
   if (tid == 0) {
     const BTask r = small[blockIdx.x];
-    stack[0] = r;
+    stack[0].t = r;
+    stack[0].buf = r.buf;
     sp = 1;
     stop = 0;
     used = 0;
     iters = 0;
+    nleaves = 0;
+    maxd = 0;
     const int n = r.e - r.b;
     cap = n > 1 ? n - 1 : 0;  // a binary tree over n primitives has at most n - 1 inner nodes
     base = cap ? atomicAdd(&ctr->nodes, cap) : 0;
@@ -558,95 +691,107 @@ __global__ __launch_bounds__(kTreeBlock) void k_subtree(const BTask* __restrict_
   }
   for (;;) {
     __syncthreads();
-    const int s = sp;
-    if (s == 0 || stop) break;
-    const BTask task = stack[s - 1];
+    if (sp == 0 || stop) break;
     __syncthreads();
-    const int n = task.e - task.b;
     if (tid == 0) {
-      sp = s - 1;
-      nodes[task.parent].child[task.slot] = pad_box_d(task.x.lo, task.x.hi);
+      cur = stack[--sp];
+      nodes[cur.t.parent].child[cur.t.slot] = pad_box_d(cur.t.x.lo, cur.t.x.hi);
       if (++iters > 2 * cap + 2) {  // a subtree over n primitives has at most 2n - 1 ranges
         atomicOr(&ctr->flag, kFlagOverflow);
         stop = 1;
       }
     }
-    if (n <= 1) {
-      if (tid == 0) {
-        nodes[task.parent].link[task.slot] = leaf_link(task.b, n);
-        atomicAdd(&ctr->leaves, 1);
+    __syncthreads();
+    const int b = cur.t.b, e = cur.t.e, n = e - b;
+    const DPrim* src = cur.buf ? B : A;
+    DPrim* dst = cur.buf ? A : B;
+    bool leaf = n <= 1;
+    if (!leaf) {
+      if (binned(cur.t.depth, cur.t.x)) {
+        for (int w = tid; w < kBinsWords; w += kTreeBlock) bins[w] = empty_word(w);
+        __syncthreads();
+        bool use[3];
+        float cmin[3], sc[3];
+        bin_setup(cur.t.x, use, cmin, sc);
+        const int per = (n + kTreeBlock - 1) / kTreeBlock;
+        const int i0 = b + tid * per;
+        bin_run(bins, src, i0, min(e, i0 + per), use, cmin, sc);
       }
+      __syncthreads();
+      decide_wave(bins, n, cur.t.depth, cur.t.x, dec, tid);
+      if (tid == 0) {
+        if (dec.kind == kDecSplit) {
+          if (used >= cap || sp + 2 > kTreeStack) {
+            atomicOr(&ctr->flag, kFlagOverflow);
+            dec.kind = kDecFallback;
+          } else {
+            dec.node = base + used++;
+            nodes[cur.t.parent].link[cur.t.slot] = dec.node;
+            maxd = max(maxd, cur.t.depth);
+          }
+        } else if (dec.kind == kDecFallback) {
+          atomicOr(&ctr->flag, kFlagFallback);
+        }
+        if (dec.kind == kDecFallback) stop = 1;
+      }
+      __syncthreads();
+      if (dec.kind == kDecFallback) continue;
+      leaf = dec.kind == kDecLeaf;
+    }
+    if (leaf) {
+      if (tid == 0) {
+        nodes[cur.t.parent].link[cur.t.slot] = leaf_link(b, n);
+        nleaves++;
+      }
+      if (cur.buf)
+        for (int i = b + tid; i < e; i += kTreeBlock) A[i] = B[i];
       continue;
     }
-    if (binned(task.depth, task.x)) {
-      for (int w = tid; w < kBinsWords; w += kTreeBlock) bins[w] = empty_word(w);
-      __syncthreads();
-      bool use[3];
-      float cmin[3], sc[3];
-      bin_setup(task.x, use, cmin, sc);
-      for (int i = task.b + tid; i < task.e; i += kTreeBlock) bin_prim(bins, A[i], use, cmin, sc);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      decide(bins, n, task.depth, task.x, dec);
-      if (dec.kind == kDecLeaf) {
-        nodes[task.parent].link[task.slot] = leaf_link(task.b, n);
-        atomicAdd(&ctr->leaves, 1);
-      } else if (dec.kind == kDecSplit) {
-        if (used >= cap || sp + 2 > kTreeStack) {
-          atomicOr(&ctr->flag, kFlagOverflow);
-          dec.kind = kDecFallback;
-        } else {
-          dec.node = base + used++;
-          nodes[task.parent].link[task.slot] = dec.node;
-          atomicMax(&ctr->max_depth, task.depth);
-        }
-      } else {
-        atomicOr(&ctr->flag, kFlagFallback);
-      }
-      if (dec.kind == kDecFallback) stop = 1;
-    }
-    __syncthreads();
-    if (dec.kind != kDecSplit) continue;
-    // stable partition of [b, e) into B, then back into A
-    int lrun = task.b, rrun = task.b + dec.nleft;
-    for (int r = task.b; r < task.e; r += kTreeBlock) {
+    // stable partition of [b, e) from src into dst
+    int lrun = b, rrun = b + dec.nleft;
+    for (int r = b; r < e; r += kTreeBlock) {
       const int i = r + tid;
-      const bool act = i < task.e;
+      const bool act = i < e;
       DPrim p;
       bool f = false;
       if (act) {
-        p = A[i];
-        f = bin_of(p.c[dec.axis], dec.cmin, dec.sc) <= dec.bin;
+        p = src[i];
+        const float cv = dec.axis == 0 ? p.c[0] : (dec.axis == 1 ? p.c[1] : p.c[2]);
+        f = bin_of(cv, dec.cmin, dec.sc) <= dec.bin;
       }
       const unsigned long long m = __ballot(f);
       const int below = __popcll(m & ((1ull << tid) - 1ull));
-      if (act) B[f ? lrun + below : rrun + (tid - below)] = p;
+      if (act) dst[f ? lrun + below : rrun + (tid - below)] = p;
       const int nl = __popcll(m);
       lrun += nl;
-      rrun += min(kTreeBlock, task.e - r) - nl;
+      rrun += min(kTreeBlock, e - r) - nl;
     }
-    __syncthreads();
-    for (int i = task.b + tid; i < task.e; i += kTreeBlock) A[i] = B[i];
     if (tid == 0) {
-      BTask c;
-      c.depth = task.depth + 1;
-      c.parent = dec.node;
+      SubTask c;
+      c.buf = cur.buf ^ 1;
       c.pad[0] = c.pad[1] = c.pad[2] = 0;
-      c.b = task.b + dec.nleft;
-      c.e = task.e;
-      c.slot = 1;
-      c.x = dec.re;
+      c.t.depth = cur.t.depth + 1;
+      c.t.parent = dec.node;
+      c.t.buf = 0;
+      c.t.pad[0] = c.t.pad[1] = 0;
+      c.t.b = b + dec.nleft;
+      c.t.e = e;
+      c.t.slot = 1;
+      c.t.x = dec.re;
       stack[sp++] = c;
-      c.b = task.b;
-      c.e = task.b + dec.nleft;
-      c.slot = 0;
-      c.x = dec.le;
+      c.t.b = b;
+      c.t.e = b + dec.nleft;
+      c.t.slot = 0;
+      c.t.x = dec.le;
       stack[sp++] = c;
     }
   }
   __syncthreads();
-  if (tid == 0) atomicAdd(&ctr->small_used, used);
+  if (tid == 0) {
+    atomicAdd(&ctr->small_used, used);
+    atomicAdd(&ctr->leaves, nleaves);
+    if (used) atomicMax(&ctr->max_depth, maxd);
+  }
 }
 
 // ---- phase 4: collapse to width W and the traversal stack bound ----------
@@ -825,15 +970,14 @@ int DeviceBvh::build(Ctx* ctx, const float* d_verts, const int32_t* d_idx, int n
     hipLaunchKernelGGL(k_blockmap, dim3(1), dim3(1024), 0, st, tasks, ntask, bf);
     hipLaunchKernelGGL(k_clear_bins, dim3(grid_of((long long)ntask * kBinsWords, 256)), dim3(256), 0, st,
                        I.bins.as<uint32_t>(), ntask * kBinsWords);
-    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, A, I.bins.as<uint32_t>());
+    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, A, B, I.bins.as<uint32_t>());
     BGPU_HIP(hipMemsetAsync(&dctr->next, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_split, dim3(grid_of(ntask, 64)), dim3(64), 0, st, tasks, ntask, I.bins.as<uint32_t>(),
                        I.decs.as<Dec>(), N2, I.node_cap, next, I.ntask_cap, I.small.as<BTask>(), I.small_cap, dctr);
-    hipLaunchKernelGGL(k_count, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, decs, A, I.cnt.as<int>());
+    hipLaunchKernelGGL(k_count, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, decs, A, B, I.cnt.as<int>());
     hipLaunchKernelGGL(k_scan_chunks, dim3(ntask), dim3(kLevelBlock), 0, st, ntask, bf, decs, I.cnt.as<int>(),
                        I.off.as<int>());
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, decs, I.off.as<int>(), A, B);
-    hipLaunchKernelGGL(k_copyback, dim3(grid), dim3(kLevelBlock), 0, st, tasks, ntask, bf, decs, B, A);
     BGPU_HIP(hipGetLastError());
     BGPU_HIP(hipMemcpyAsync(&h, dctr, sizeof(h), hipMemcpyDeviceToHost, st));
     BGPU_HIP(hipStreamSynchronize(st));
