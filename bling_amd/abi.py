@@ -23,6 +23,7 @@ CAMERA_PERSPECTIVE, CAMERA_ENVIRONMENT = 0, 1
 FILTER_BOX, FILTER_GAUSS, FILTER_MITCHELL, FILTER_SINC, FILTER_TRIANGLE = range(5)
 DEVICE_PTRS, TIMED, COUNT, COHERENT = 1, 2, 4, 8
 JOB_VARIANCE = 1
+INTEGRATOR_PATH, INTEGRATOR_DIRECT = 0, 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = range(5)
 
 F16 = C.c_float * 16
@@ -93,7 +94,7 @@ class JobDesc(C.Structure):
     _fields_ = [("spp_nu", C.c_int32), ("spp_nv", C.c_int32), ("max_depth", C.c_int32),
                 ("sample_depth", C.c_int32), ("rr_start_depth", C.c_int32),
                 ("passes", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
-                ("pad", C.c_int32), ("camera", C.POINTER(Camera)), ("film", C.POINTER(Film))]
+                ("integrator", C.c_int32), ("camera", C.POINTER(Camera)), ("film", C.POINTER(Film))]
 
 
 class BvhInfo(C.Structure):

@@ -13,13 +13,15 @@ The grammar follows the reference's Parsec parser:
   IO/CameraParser.hs          perspective camera (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
-                              blackbody with constant textures
+                              blackbody with constant textures, checker and
+                              graphPaper over constants, blends of constants
   IO/PrimitiveParser.hs       mesh (polygon faces triangulated as fans,
                               TriangleMesh.hs:23-29) and shape primitives
   IO/LightParser.hs           infinite lights with a constant map; emission
   IO/RendererParser.hs,       renderer blocks (every kind is read; only the
   IO/IntegratorParser.hs      sampler renderer with a stratified sampler and
-                              the path integrator maps to a GPU job)
+                              the path or directLighting integrator maps
+                              to a GPU job)
 
 The reader builds a `SceneBuilder` in the order `mkScene` sees the
 primitives: the parser prepends each statement's primitives
@@ -663,15 +665,19 @@ class BlingScene:
         """The GPU job of the file's renderer block: `sampled { sampler {
         stratified nu nv } integrator { path maxDepth md sampleDepth sd } }`,
         with the reference's hard-coded Russian-roulette depth 7
-        (Integrator/Path.hs:70)."""
+        (Integrator/Path.hs:70), or `integrator { directLighting maxDepth md }`."""
         from .runtime import JobSpec
         r = self.renderer
         if r[0] != "sampler":
             raise Unsupported("renderer %r (the GPU path renders the sampler renderer)" % r[0])
         if r[1][0] != "stratified":
             raise Unsupported("sampler %r (the GPU path uses the stratified sampler)" % r[1][0])
+        if r[2][0] == "directLighting":  # IntegratorParser.hs:37-39
+            return JobSpec(spp_nu=r[1][1], spp_nv=r[1][2], max_depth=r[2][1], sample_depth=0,
+                           rr_start_depth=7, passes=passes, seed=seed, integrator="direct")
         if r[2][0] != "path":
-            raise Unsupported("integrator %r (the GPU path is the path integrator)" % r[2][0])
+            raise Unsupported("integrator %r (the GPU path runs the path and direct-lighting integrators)"
+                              % r[2][0])
         return JobSpec(spp_nu=r[1][1], spp_nv=r[1][2], max_depth=r[2][1], sample_depth=r[2][2],
                        rr_start_depth=7, passes=passes, seed=seed)
 

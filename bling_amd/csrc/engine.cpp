@@ -139,9 +139,10 @@ struct Engine {
     cfg = config;
     P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 9 int arrays + 52 float arrays of P, plus 5 queues
+    // pool layout: 9 int arrays + 52 float arrays of P, plus 5 queues, plus
+    // the direct-lighting branch stacks (1 + kDlWords * dl_depth arrays)
     size_t per = (size_t)P * 4;
-    size_t total = per * (9 + 52 + 5);
+    size_t total = per * (9 + 52 + 5 + 1 + (size_t)kDlWords * cfg.dl_depth);
     if (int rc = pool_buf.alloc(c, total)) return rc;
     char* b = pool_buf.as<char>();
     auto take = [&](size_t n) {
@@ -174,6 +175,9 @@ struct Engine {
     pp.q_mis = (uint32_t*)take(1);
     pp.q_sh = (uint32_t*)take(1);
     pp.q_cls = (uint32_t*)take(1);
+    pp.dl_sp = (int32_t*)take(1);
+    pp.dl_stk = (float*)take((size_t)kDlWords * cfg.dl_depth);
+    pp.dl_cap = cfg.dl_depth;
     if (int rc = jobs_buf.alloc(c, sizeof(DevJob) * kMaxJobs)) return rc;
     if (int rc = ring_buf.alloc(c, sizeof(RingChunk) * kRing)) return rc;
     if (int rc = scenes_buf.alloc(c, sizeof(DevPtScene) * kMaxScenes)) return rc;
@@ -195,15 +199,16 @@ struct Engine {
     for (int i = kMaxJobs - 1; i >= 0; --i) free_slots.push_back(i);
     BGPU_HIP(hipMemsetAsync(prog_buf.ptr, 0, sizeof(uint32_t) * kSkipWords, stream));
     BGPU_HIP(pt_enqueue_init(args, stream));
-    if (int rc = capture(c->bvh_width, cfg.sort == 1)) return rc;
+    if (int rc = capture(c->bvh_width, cfg.sort == 1, false)) return rc;
     BGPU_HIP(hipStreamSynchronize(stream));
     return BGPU_OK;
   }
 
   // Captures one bounce iteration whose traversal kernels are specialised
   // for BVH width w (0: any width), shading in class-grouped order if `sort`.
-  int capture(int w, bool sort) {
+  int capture(int w, bool sort, bool direct) {
     args.sort_shade = sort ? 1 : 0;
+    args.direct = direct ? 1 : 0;
     if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
     if (graph) BGPU_HIP(hipGraphDestroy(graph));
     gexec = nullptr;
@@ -306,9 +311,13 @@ struct Engine {
       for (Scene* s : scenes)
         if (s && s->pt.shade_kinds > 1) want_sort = true;
     }
-    if (want_w != width || want_sort != (args.sort_shade != 0)) {
+    // the direct-lighting shading kernel runs while such a job is resident
+    bool want_direct = j->desc.integrator == BGPU_INTEGRATOR_DIRECT;
+    for (HostJob* o : slots)
+      if (o && o->desc.integrator == BGPU_INTEGRATOR_DIRECT) want_direct = true;
+    if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0)) {
       BGPU_HIP(hipStreamSynchronize(stream));
-      if (int rc = capture(want_w, want_sort)) return rc;
+      if (int rc = capture(want_w, want_sort, want_direct)) return rc;
     }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
@@ -344,6 +353,7 @@ struct Engine {
     J.sd = d.sample_depth;
     J.rr = d.rr_start_depth;
     J.passes = d.passes;
+    J.integ = d.integrator;
     J.wxyz = j->film.as<float>();
     J.sq = j->sq.ptr ? j->sq.as<float>() : nullptr;
     J.pass_done = prog + off;
@@ -487,8 +497,13 @@ struct Engine {
 
 void destroy_engine(Engine* e) { delete e; }
 
-static int validate_job(const bgpu_job_desc* d) {
+static int validate_job(const bgpu_job_desc* d, const PtConfig& cfg) {
   if (!d) return fail(BGPU_E_INVALID, "job description is NULL");
+  if (d->integrator != BGPU_INTEGRATOR_PATH && d->integrator != BGPU_INTEGRATOR_DIRECT)
+    return fail(BGPU_E_INVALID, "integrator must be BGPU_INTEGRATOR_PATH or BGPU_INTEGRATOR_DIRECT");
+  if (d->integrator == BGPU_INTEGRATOR_DIRECT && (d->max_depth < 1 || d->max_depth - 1 > cfg.dl_depth))
+    return fail(BGPU_E_INVALID, "directLighting maxDepth must be in [1, pt.dl_depth + 1] (pt.dl_depth = " +
+                                    std::to_string(cfg.dl_depth) + ")");
   if (d->spp_nu <= 0 || d->spp_nv <= 0)
     return fail(BGPU_E_INVALID, "mkStratifiedSampler: nx and ny must be > 0");
   if ((long long)d->spp_nu * d->spp_nv > (1 << 20)) return fail(BGPU_E_INVALID, "too many samples per pixel");
@@ -693,7 +708,12 @@ extern "C" {
 
 int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float* sq) {
   if (!scene || !wxyz) return fail(BGPU_E_INVALID, "bgpu_render: NULL argument");
-  if (int rc = validate_job(job)) return rc;
+  PtConfig cfg_now;
+  {
+    std::lock_guard<std::mutex> lk(scene->ctx->mu);
+    cfg_now = scene->ctx->pt_cfg;
+  }
+  if (int rc = validate_job(job, cfg_now)) return rc;
   Ctx* c = scene->ctx;
   BGPU_HIP(hipSetDevice(c->device));
   HostJob j;
@@ -724,7 +744,12 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
   if (int rc = eng.feed({{&j, j.total}})) return bail(rc);
   // every path ends within max_depth + 1 bounces, so the job drains in at
   // most (total / P + 1) * (max_depth + 2) iterations
-  unsigned long long cap = (j.total / (unsigned long long)eng.P + 2) * (unsigned long long)(job->max_depth + 2) + 64;
+  // (a direct-lighting sample traces at most 2^max_depth - 1 branches, one
+  // bounce each, in its slot)
+  const unsigned long long per_path = job->integrator == BGPU_INTEGRATOR_DIRECT
+                                          ? (1ull << std::min(job->max_depth, 20)) + 1
+                                          : (unsigned long long)(job->max_depth + 2);
+  unsigned long long cap = (j.total / (unsigned long long)eng.P + 2) * per_path + 64;
   unsigned long long iters = 0;
   std::vector<HostJob*> one{&j}, done;
   while (done.empty()) {
@@ -752,7 +777,12 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
 int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out) {
   if (!scene || !out) return fail(BGPU_E_INVALID, "bgpu_job_submit: NULL argument");
   *out = nullptr;
-  if (int rc = validate_job(job)) return rc;
+  PtConfig cfg_now;
+  {
+    std::lock_guard<std::mutex> lk(scene->ctx->mu);
+    cfg_now = scene->ctx->pt_cfg;
+  }
+  if (int rc = validate_job(job, cfg_now)) return rc;
   Ctx* c = scene->ctx;
   bgpu_job* j = new bgpu_job();
   if (int rc = j->prepare(scene, *job)) {
@@ -897,6 +927,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.persistent") {
     if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "pt.persistent must be 0 or 1");
     next.persistent = (int)value;
+  } else if (k == "pt.dl_depth") {
+    if (value < 0 || value > 15) return fail(BGPU_E_INVALID, "pt.dl_depth must be in [0, 15]");
+    next.dl_depth = (int)value;
   } else {
     return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   }
@@ -928,6 +961,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
   else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
   else if (k == "pt.sort") *value = ctx->pt_cfg.sort;
+  else if (k == "pt.dl_depth") *value = ctx->pt_cfg.dl_depth;
   else return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   return BGPU_OK;
 }

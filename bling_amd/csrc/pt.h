@@ -78,7 +78,8 @@ struct DevJob {
   float* sq;
   unsigned int* pass_done;      // finished samples per pass
   unsigned long long* skipped;  // samples dropped after cancellation
-  int32_t cancelled, pad;
+  int32_t cancelled;
+  int32_t integ;                // BGPU_INTEGRATOR_PATH or BGPU_INTEGRATOR_DIRECT
   DevView view;
 };
 
@@ -112,6 +113,8 @@ struct AdmitReq {
 };
 
 // Path pool, structure of arrays of P slots.
+constexpr int kDlWords = 24;  // a pending direct-lighting branch: ray (7), throughput (16), depth
+
 struct PathPool {
   int32_t P;
   int32_t* state;   // 0 free, 1 active, 2 done
@@ -132,6 +135,12 @@ struct PathPool {
   float* pend_m;    // 3P XYZ of the BSDF sample if the MIS ray reaches the light
   int32_t* mlight;  // light index of the MIS ray
   int32_t* occ;     // shadow ray occluded by a triangle (persistent passes)
+  // direct lighting: specular branches waiting to be traced, a stack of
+  // dl_cap entries per slot; entry e, word k of slot i at dl_stk[(e * kDlWords + k) * P + i]
+  // (ray ox oy oz dx dy dz tmin, 16 throughput bands, depth)
+  int32_t* dl_sp;
+  float* dl_stk;
+  int32_t dl_cap, dl_pad;
   uint32_t* q_act;
   uint32_t* q_shadow;
   uint32_t* q_mis;
@@ -146,6 +155,7 @@ struct PtConfig {
   int persistent = 0;   // traversal passes with lane refill (1) or one ray per thread (0)
   int sort = 2;         // shading queue grouped by material kind: 1 always, 0 never, 2 while a
                         // resident scene has more than one BxDF makeup
+  int dl_depth = 4;     // pending specular branches per direct-lighting path
 };
 
 struct PtKernelArgs {
@@ -155,7 +165,8 @@ struct PtKernelArgs {
   RingChunk* ring;  // chunks in work order (kRing entries, modulo)
   PtCounters* cnt;
   AdmitReq* admit;
-  int32_t sort_shade, pad;  // shade reads q_sh (grouped by class) instead of q_act
+  int32_t sort_shade;  // shade reads q_sh (grouped by class) instead of q_act
+  int32_t direct;      // direct-lighting jobs may be resident: run their shading kernel
 };
 
 // Host-side shading data owned by a Scene.

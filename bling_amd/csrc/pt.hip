@@ -268,6 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
         P.pix[i] = pix;
         P.samp[i] = n;
         P.dspec[i] = 1 << 16;  // depth 0, specular (camera ray)
+        P.dl_sp[i] = 0;        // no pending direct-lighting branches
         P.sxy[i] = sx;
         P.sxy[Pn + i] = sy;
         P.ray[i] = r.o.x; P.ray[Pn + i] = r.o.y; P.ray[2 * Pn + i] = r.o.z;
@@ -420,6 +421,80 @@ __global__ __launch_bounds__(BLOCK) void k_class_scatter(PtKernelArgs a) {
   if (valid) a.pool.q_sh[base[c] + pos] = a.pool.q_act[q];
 }
 
+// sampleOneLight (Scene.hs:110-118) with sampleLightMis / sampleBsdfMis
+// (:61-82): the light sample's shadow ray and the BSDF sample's MIS ray are
+// queued with the contributions they carry if unoccluded / if they reach the
+// light, scaled by the path throughput thr_at(band).
+template <class Thr>
+__device__ __forceinline__ void sample_one_light(const DevPtScene& S, const PathPool& P, int Pn, int i,
+                                                 const Bsdf& bsdf, f3 wo, f3 p, float eps, float lNumU, float ld1,
+                                                 float ld2, float bComp, float bd1, float bd2, Thr thr_at,
+                                                 bool& want_shadow, bool& want_mis) {
+  int lc = S.nlights;
+  if (lc > 0) {  // sampleOneLight (Scene.hs:110-118)
+    int ln = 0;
+    float lscale = 1.0f;
+    if (lc > 1) {
+      ln = (int)floorf(lNumU * (float)lc);
+      ln = ln < lc - 1 ? ln : lc - 1;
+      lscale = (float)lc;
+    }
+    {  // sampleLightMis (Scene.hs:61-69)
+      LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
+      if (ls.pdf != 0.0f && ls.li) {
+        SpecVal f = eval_bsdf(bsdf, wo, ls.wi);
+        float w = power_heuristic(ls.pdf, bsdf_pdf(bsdf, wo, ls.wi));
+        float k = w / ls.pdf;
+        Xyz c;
+        bool liBlack = true, fBlack = true;
+#pragma unroll 4
+        for (int b = 0; b < 16; ++b) {
+          float fv = spec_at(f, b), li = ls.li[b];
+          liBlack = liBlack && li == 0.0f;
+          fBlack = fBlack && fv == 0.0f;
+          c.add(b, thr_at(b) * (((fv * li) * k) * lscale));
+        }
+        if (!(liBlack || fBlack)) {
+          xyz_store(P.pend_s, Pn, i, c);
+          P.sray[i] = ls.ray.o.x; P.sray[Pn + i] = ls.ray.o.y; P.sray[2 * Pn + i] = ls.ray.o.z;
+          P.sray[3 * Pn + i] = ls.ray.d.x; P.sray[4 * Pn + i] = ls.ray.d.y; P.sray[5 * Pn + i] = ls.ray.d.z;
+          P.sray[6 * Pn + i] = ls.ray.tmin; P.sray[7 * Pn + i] = ls.ray.tmax;
+          want_shadow = true;
+        }
+      }
+    }
+    {  // sampleBsdfMis (Scene.hs:71-82)
+      SpecVal fb;
+      f3 wib;
+      int tb;
+      float bpdf = sample_bsdf(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
+      if (bpdf != 0.0f) {
+        float lpdf = light_pdf(S, ln, p, wib);
+        float w = power_heuristic(bpdf, lpdf);
+        // the emission a hit on the sampled light returns is known here:
+        // the area light's radiance, or the infinite light's along wib
+        const float* e = S.lights[ln].type == LT_INF ? light_le(S, ln, wib) : S.lights[ln].L;
+        Xyz c;
+        bool fBlack = true;
+#pragma unroll 4
+        for (int b = 0; b < 16; ++b) {
+          float fv = spec_at(fb, b);
+          fBlack = fBlack && fv == 0.0f;
+          c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
+        }
+        if (!fBlack) {
+          xyz_store(P.pend_m, Pn, i, c);
+          P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
+          P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
+          P.mray[6 * Pn + i] = eps;
+          P.mlight[i] = ln;
+          want_mis = true;
+        }
+      }
+    }
+  }
+}
+
 template <int BLOCK, int MINW>
 __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   const int q = blockIdx.x * BLOCK + threadIdx.x;
@@ -430,6 +505,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   bool valid = (unsigned)q < nact;
   int i = valid ? (int)(a.sort_shade ? P.q_sh[q] : P.q_act[q]) : 0;
   int jslot = valid ? P.job[i] : 0;
+  if (a.direct && valid && a.jobs[jslot].integ == BGPU_INTEGRATOR_DIRECT) valid = false;  // k_shade_direct's
   bool want_shadow = false, want_mis = false;
   for_uniform_scene(valid, valid ? a.jobs[jslot].scene : 0, [&](int scene_u) {
     {
@@ -489,69 +565,8 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
           for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
           xyz_accum(P.rad, Pn, i, c);
         }
-        int lc = S.nlights;
-        if (lc > 0) {  // sampleOneLight (Scene.hs:110-118)
-          int ln = 0;
-          float lscale = 1.0f;
-          if (lc > 1) {
-            ln = (int)floorf(lNumU * (float)lc);
-            ln = ln < lc - 1 ? ln : lc - 1;
-            lscale = (float)lc;
-          }
-          {  // sampleLightMis (Scene.hs:61-69)
-            LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
-            if (ls.pdf != 0.0f && ls.li) {
-              SpecVal f = eval_bsdf(bsdf, wo, ls.wi);
-              float w = power_heuristic(ls.pdf, bsdf_pdf(bsdf, wo, ls.wi));
-              float k = w / ls.pdf;
-              Xyz c;
-              bool liBlack = true, fBlack = true;
-#pragma unroll 4
-              for (int b = 0; b < 16; ++b) {
-                float fv = spec_at(f, b), li = ls.li[b];
-                liBlack = liBlack && li == 0.0f;
-                fBlack = fBlack && fv == 0.0f;
-                c.add(b, thr_at(b) * (((fv * li) * k) * lscale));
-              }
-              if (!(liBlack || fBlack)) {
-                xyz_store(P.pend_s, Pn, i, c);
-                P.sray[i] = ls.ray.o.x; P.sray[Pn + i] = ls.ray.o.y; P.sray[2 * Pn + i] = ls.ray.o.z;
-                P.sray[3 * Pn + i] = ls.ray.d.x; P.sray[4 * Pn + i] = ls.ray.d.y; P.sray[5 * Pn + i] = ls.ray.d.z;
-                P.sray[6 * Pn + i] = ls.ray.tmin; P.sray[7 * Pn + i] = ls.ray.tmax;
-                want_shadow = true;
-              }
-            }
-          }
-          {  // sampleBsdfMis (Scene.hs:71-82)
-            SpecVal fb;
-            f3 wib;
-            int tb;
-            float bpdf = sample_bsdf(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
-            if (bpdf != 0.0f) {
-              float lpdf = light_pdf(S, ln, p, wib);
-              float w = power_heuristic(bpdf, lpdf);
-              // the emission a hit on the sampled light returns is known here:
-              // the area light's radiance, or the infinite light's along wib
-              const float* e = S.lights[ln].type == LT_INF ? light_le(S, ln, wib) : S.lights[ln].L;
-              Xyz c;
-              bool fBlack = true;
-#pragma unroll 4
-              for (int b = 0; b < 16; ++b) {
-                float fv = spec_at(fb, b);
-                fBlack = fBlack && fv == 0.0f;
-                c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
-              }
-              if (!fBlack) {
-                xyz_store(P.pend_m, Pn, i, c);
-                P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
-                P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
-                P.mray[6 * Pn + i] = eps;
-                P.mlight[i] = ln;
-                want_mis = true;
-              }
-            }
-          }
-        }
+        sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at, want_shadow,
+                         want_mis);
         // Russian roulette (Path.hs:70-72)
         float pc = 1.0f;
         if (depth > J.rr) {
@@ -597,6 +612,122 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         P.state[i] = cont ? 1 : 2;
       }
     }
+  });
+  __shared__ unsigned int s32[BLOCK / 64 + 1];
+  const unsigned int qs = block_append<BLOCK>(&a.cnt->n_shadow, want_shadow, s32);
+  if (want_shadow) P.q_shadow[qs] = (uint32_t)i;
+  const unsigned int qm = block_append<BLOCK>(&a.cnt->n_mis, want_mis, s32);
+  if (want_mis) P.q_mis[qm] = (uint32_t)i;
+}
+
+// Shading of direct-lighting paths (Integrator/DirectLighting.hs:23-63).
+// A slot walks its sample's tree of specular branches depth first: at a hit
+// it adds the emission towards wo and sampleOneLight (dimensions 2d, 2d + 1),
+// then continues along the specular reflection and pushes the specular
+// transmission (both sampled with uComp 0.5, uDir (0.5, 0.5)) while depth + 1
+// < maxDepth, each branch weighted by its sampled f.  A path that escapes
+// (black) or has no branch left pops the next pending one, or ends.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int nact = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= nact) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  bool valid = (unsigned)q < nact;
+  int i = valid ? (int)(a.sort_shade ? P.q_sh[q] : P.q_act[q]) : 0;
+  int jslot = valid ? P.job[i] : 0;
+  valid = valid && a.jobs[jslot].integ == BGPU_INTEGRATOR_DIRECT;
+  bool want_shadow = false, want_mis = false;
+  for_uniform_scene(valid, valid ? a.jobs[jslot].scene : 0, [&](int scene_u) {
+    const DevPtScene& S = a.scenes[scene_u];
+    const DevJob& J = a.jobs[jslot];
+    DRay ray;
+    ray.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
+    ray.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
+    ray.tmin = P.ray[6 * Pn + i];
+    ray.tmax = INFINITY;
+    const int prim = P.hprim[i];
+    const int depth = P.dspec[i] & 0xffff;
+    const float* thr = P.thr + i;
+    auto thr_at = [&](int b) { return depth == 0 ? 1.0f : thr[b * Pn]; };
+    bool more = false;  // the slot continues with a branch of this vertex
+    if (prim >= 0) {
+      float th = P.hit[i], hb1 = P.hit[Pn + i], hb2 = P.hit[2 * Pn + i];
+      int pix = P.pix[i], n = P.samp[i];
+      int ix = J.view.ext_x0 + pix % J.view.ext_w, iy = J.view.ext_y0 + pix / J.view.ext_w;
+      // SurfaceIntegrator li (2 md) (2 md): tables for every dimension used
+      Sampler sm{pixel_key(J.seed, (uint32_t)P.pass[i], ix, iy), (uint32_t)J.nu, (uint32_t)J.nv, (uint32_t)J.N,
+                 2u * (uint32_t)J.md, 2u * (uint32_t)J.md};
+      DG dgg, dgs;
+      float eps;
+      int mat, light;
+      hit_dg(S, ray, th, hb1, hb2, prim, &dgg, &dgs, &eps, &mat, &light);
+      Bsdf bsdf = make_bsdf(S, mat, dgg, dgs);
+      const uint32_t k = 2u * (uint32_t)depth, un = (uint32_t)n;
+      const float uln = sm.get1(un, k), ubc = sm.get1(un, k + 1);
+      float uld1, uld2, ubd1, ubd2;
+      sm.get2(un, k, &uld1, &uld2);
+      sm.get2(un, k + 1, &ubd1, &ubd2);
+      const f3 wo = neg3(ray.d), p = bsdf.p;
+      if (light >= 0 && dot3(dgg.n, wo) > 0.0f) {  // intLe int wo (Primitive.hs:68-76, Light.hs:85-93)
+        const float* L = S.lights[light].L;
+        Xyz c;
+#pragma unroll
+        for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
+        xyz_accum(P.rad, Pn, i, c);
+      }
+      sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2, thr_at, want_shadow,
+                       want_mis);
+      if (depth + 1 < J.md) {  // cont (DirectLighting.hs:49-63)
+        SpecVal fr, ft;
+        f3 wr, wt;
+        const float pr = sample_bsdf_flt(bsdf, wo, BX_S | BX_R, fr, &wr);
+        const float pt = sample_bsdf_flt(bsdf, wo, BX_S | BX_T, ft, &wt);
+        const bool br = pr != 0.0f, bt = pt != 0.0f;
+        if (br && bt) {  // reflection continues here, transmission waits
+          const int e = P.dl_sp[i];
+          if (e < P.dl_cap) {
+            float* st = P.dl_stk + (size_t)e * kDlWords * Pn + i;
+            st[0] = p.x; st[Pn] = p.y; st[2 * Pn] = p.z;
+            st[3 * Pn] = wt.x; st[4 * Pn] = wt.y; st[5 * Pn] = wt.z;
+            st[6 * Pn] = eps;
+#pragma unroll 4
+            for (int b = 0; b < 16; ++b) st[(7 + b) * Pn] = spec_at(ft, b) * thr_at(b);
+            st[23 * Pn] = __int_as_float(depth + 1);
+            P.dl_sp[i] = e + 1;
+          }
+        }
+        if (br || bt) {
+          const SpecVal& f = br ? fr : ft;
+          const f3 wi = br ? wr : wt;
+          float* thw = P.thr + i;
+#pragma unroll 4
+          for (int b = 0; b < 16; ++b) thw[b * Pn] = spec_at(f, b) * thr_at(b);
+          P.ray[i] = p.x; P.ray[Pn + i] = p.y; P.ray[2 * Pn + i] = p.z;
+          P.ray[3 * Pn + i] = wi.x; P.ray[4 * Pn + i] = wi.y; P.ray[5 * Pn + i] = wi.z;
+          P.ray[6 * Pn + i] = eps;
+          P.dspec[i] = depth + 1;
+          more = true;
+        }
+      }
+    }
+    if (!more) {
+      const int e = P.dl_sp[i];
+      if (e > 0) {  // the next pending branch
+        const float* st = P.dl_stk + (size_t)(e - 1) * kDlWords * Pn + i;
+        P.ray[i] = st[0]; P.ray[Pn + i] = st[Pn]; P.ray[2 * Pn + i] = st[2 * Pn];
+        P.ray[3 * Pn + i] = st[3 * Pn]; P.ray[4 * Pn + i] = st[4 * Pn]; P.ray[5 * Pn + i] = st[5 * Pn];
+        P.ray[6 * Pn + i] = st[6 * Pn];
+        float* thw = P.thr + i;
+#pragma unroll 4
+        for (int b = 0; b < 16; ++b) thw[b * Pn] = st[(7 + b) * Pn];
+        P.dspec[i] = __float_as_int(st[23 * Pn]);
+        P.dl_sp[i] = e - 1;
+        more = true;
+      }
+    }
+    P.state[i] = more ? 1 : 2;
   });
   __shared__ unsigned int s32[BLOCK / 64 + 1];
   const unsigned int qs = block_append<BLOCK>(&a.cnt->n_shadow, want_shadow, s32);
@@ -1009,6 +1140,7 @@ static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipSt
   }
   if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
+  if (a.direct) hipLaunchKernelGGL((k_shade_direct<BLOCK>), g, b, 0, s, a);
 }
 
 template <int BLOCK, int W>

@@ -695,6 +695,56 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   return pdf;
 }
 
+// sampleBsdf' flags with uComp 0.5 and uDir (0.5, 0.5) (Reflection.hs:278-316,
+// DirectLighting.hs:53): only the components whose type is a subset of
+// `flags` (bxdfMatches) take part.  Returns the pdf; f as sample_bsdf's.
+__device__ __forceinline__ float sample_bsdf_flt(const Bsdf& b, f3 woW, int flags, SpecVal& f, f3* wiW) {
+  f = spec_zero();
+  *wiW = mk3(0.0f, 1.0f, 0.0f);
+  const int n = bsdf_n(b);
+  int m[2], cnt = 0;
+  for (int i = 0; i < n; ++i)
+    if ((b.m->c[i].type & flags) == b.m->c[i].type) m[cnt++] = i;
+  if (cnt == 0) return 0.0f;
+  const f3 wo = to_local(b.cs, woW);
+  const float cntf = (float)cnt, invCnt = 1.0f / cntf;
+  int sNum = (int)floorf(0.5f * cntf);
+  sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
+  sNum = sNum < 0 ? 0 : sNum;
+  const DBxdf& bx = b.m->c[sNum == 0 ? m[0] : m[1]];
+  Term fs;
+  bool fzero;
+  f3 wi;
+  float pdfp;
+  bx_sample(bx, wo, 0.5f, 0.5f, &fs, &fzero, &wi, &pdfp);
+  const f3 ww = to_world(b.cs, wi);
+  const float side = dot3(ww, b.ng) / dot3(woW, b.ng);
+  if (pdfp == 0.0f || side == 0.0f) return 0.0f;
+  const int need = side < 0.0f ? BX_T : BX_R;
+  if (!(bx.type & need)) return 0.0f;
+  *wiW = ww;
+  if (fzero) fs = Term{&bx, T_RS, 0.0f, 0.0f};
+  f.t[0] = fs;
+  f.n = 1;
+  if (bx.type & BX_S) {
+    f.mode = V_SCALE;
+    f.k1 = cntf;
+    return pdfp * invCnt;
+  }
+  if (cnt == 1) {
+    f.mode = V_SCALE;
+    f.k1 = 1.0f;
+    return pdfp;
+  }
+  const DBxdf& o = b.m->c[sNum == 0 ? m[1] : m[0]];  // the other matching component
+  const float pdf = (pdfp + bx_pdf(o, wo, wi)) * invCnt;
+  if ((o.type & need) && bx_eval_term(o, wi, wo, &f.t[1])) f.n = 2;
+  f.mode = V_MIX;
+  f.k1 = pdfp;
+  f.k2 = 1.0f / pdf;
+  return pdf;
+}
+
 // -------------------------------------------------------------- lights
 __device__ __forceinline__ float shape_area(const DShape& s) {  // Shape.hs:314-328
   switch (s.type) {

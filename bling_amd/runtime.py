@@ -159,7 +159,9 @@ class GpuScene:
 @dataclass
 class JobSpec:
     """A render job: `sampler { stratified nu nv }` and
-    `integrator { path maxDepth md sampleDepth sd }`, `passes` passes."""
+    `integrator { path maxDepth md sampleDepth sd }` (integrator "path") or
+    `integrator { directLighting maxDepth md }` (integrator "direct"),
+    `passes` passes."""
     spp_nu: int = 2
     spp_nv: int = 2
     max_depth: int = 7
@@ -169,6 +171,7 @@ class JobSpec:
     seed: int = 1
     variance: bool = False
     view: object = None  # scene.View; None renders the scene's own camera and film
+    integrator: str = "path"
 
     def desc(self):
         d = abi.JobDesc()
@@ -177,6 +180,9 @@ class JobSpec:
         d.rr_start_depth, d.passes = self.rr_start_depth, self.passes
         d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
         d.flags = abi.JOB_VARIANCE if self.variance else 0
+        if self.integrator not in ("path", "direct"):
+            raise ValueError("integrator must be 'path' or 'direct'")
+        d.integrator = abi.INTEGRATOR_DIRECT if self.integrator == "direct" else abi.INTEGRATOR_PATH
         if self.view is not None:
             d.camera = C.pointer(self.view.camera)
             d.film = C.pointer(self.view.film)

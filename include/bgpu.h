@@ -216,9 +216,14 @@ enum {
 /* ---- jobs --------------------------------------------------------------- */
 /* One render job: `renderer { gpu sampled { sampler { stratified nu nv }
  * integrator { path maxDepth md sampleDepth sd } } }` (RendererParser.hs:56-73,
- * IntegratorParser.hs:41-44), `passes` progressive passes of nu*nv samples per
- * pixel over the film's sample extent (Image.hs:162-168, Rendering.hs:207-236). */
+ * IntegratorParser.hs:37-44), `passes` progressive passes of nu*nv samples per
+ * pixel over the film's sample extent (Image.hs:162-168, Rendering.hs:207-236).
+ * integrator BGPU_INTEGRATOR_DIRECT is `directLighting maxDepth md`
+ * (Integrator/DirectLighting.hs): sample_depth and rr_start_depth are unused,
+ * and max_depth - 1 must not exceed the "pt.dl_depth" knob (its specular
+ * branches wait in a per-path stack of that many entries). */
 enum { BGPU_JOB_VARIANCE = 1 /* also accumulate per-pixel sum of squared XYZ */ };
+enum { BGPU_INTEGRATOR_PATH = 0, BGPU_INTEGRATOR_DIRECT = 1 };
 typedef struct {
   int32_t spp_nu, spp_nv;
   int32_t max_depth, sample_depth;
@@ -226,7 +231,7 @@ typedef struct {
   int32_t passes;         /* number of progressive passes, >= 1 */
   uint64_t seed;
   int32_t flags;
-  int32_t pad;
+  int32_t integrator;     /* BGPU_INTEGRATOR_PATH (default) or BGPU_INTEGRATOR_DIRECT */
   /* Per-job camera and film (a RenderJob's camera and imageSize,
    * Rendering.hs:133-137); NULL selects the scene description's. */
   const bgpu_camera* camera;
@@ -313,7 +318,9 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * (64/128/256, bounce kernels), "pt.persistent" (1: the bounce loop's
  * traversal runs as persistent passes that refill lanes with new rays;
  * 0, the default: one ray per thread),
- * "pt.pool" (path slots, >= 1024),
+ * "pt.pool" (path slots, >= 1024), "pt.dl_depth" (0..15, default 4:
+ * pending specular branches a direct-lighting path can hold, so its jobs may
+ * use maxDepth <= pt.dl_depth + 1; 96 bytes per path slot per entry),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
  * two waves per SIMD), "pt.sort" (1: the shading queue is grouped by
  * material kind each bounce, so that a wave runs one BSDF code path; 0: path

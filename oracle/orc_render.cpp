@@ -888,16 +888,21 @@ struct BsdfSample {
   Spec f;
   V3 wi;
 };
-static BsdfSample sampleBsdf(const Bsdf& b, V3 woW, float uComp, float u1, float u2) {  // :278-316
+constexpr int kBxAll = BX_R | BX_T | BX_D | BX_G | BX_S;
+// sampleBsdf' flags (Reflection.hs:278-316): the components whose type is a
+// subset of `flags` (bxdfMatches) take part; sampleBsdf is flags = all.
+static BsdfSample sampleBsdf(const Bsdf& b, V3 woW, float uComp, float u1, float u2, int flags = kBxAll) {
   BsdfSample empty{BX_R | BX_D, 0.0f, sconst(0), mk(0, 1, 0)};
-  if (b.n == 0) return empty;
+  int m[8], cnt = 0;
+  for (int i = 0; i < b.n; ++i)
+    if ((b.c[i].type & flags) == b.c[i].type) m[cnt++] = i;
+  if (cnt == 0) return empty;
   V3 wo = worldToLocal(b.cs, woW);
-  int cnt = b.n;
   float cntf = (float)cnt, invCnt = 1.0f / cntf;
   int sNum = (int)std::floor(uComp * cntf);
   sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
   sNum = sNum < 0 ? 0 : sNum;
-  const Bxdf& bx = b.c[sNum];
+  const Bxdf& bx = b.c[m[sNum]];
   Spec fS;
   V3 wi;
   float pdfp;
@@ -911,10 +916,11 @@ static BsdfSample sampleBsdf(const Bsdf& b, V3 woW, float uComp, float u1, float
   if (cnt == 1) return BsdfSample{bx.type, pdfp, fS, wiW};
   float other = 0.0f;
   Spec fOthers = sconst(0);
-  for (int i = 0; i < cnt; ++i) {
-    if (i == sNum) continue;
-    other = other + bxPdf(b.c[i], wo, wi);
-    if (b.c[i].type & need) fOthers = fOthers + bxEval(b.c[i], wi, wo);
+  for (int k = 0; k < cnt; ++k) {
+    if (k == sNum) continue;
+    const Bxdf& o = b.c[m[k]];
+    other = other + bxPdf(o, wo, wi);
+    if (o.type & need) fOthers = fOthers + bxEval(o, wi, wo);
   }
   float pdf = (pdfp + other) * invCnt;
   Spec fSum = sscale(sscale(fS, pdfp) + fOthers, 1.0f / pdf);
@@ -1033,6 +1039,49 @@ static Spec li(Ctx& c, const Ray& camRay, const PixelSampler& ps, uint32_t n, in
     c.rays++;
     hit = intersectFull(*c.s, ray, &is);
   }
+}
+
+// DirectLighting (Integrator/DirectLighting.hs:23-63): at every hit the
+// emission towards wo, sampleOneLight with the sample dimensions 2d / 2d + 1,
+// and the specular reflection and transmission continuations (uComp 0.5,
+// uDir (0.5, 0.5)) up to maxDepth, each weighted by its sampled f; escaping
+// rays are black.
+static Spec sampleOneLightAt(Ctx& c, V3 p, float eps, V3 nrm, V3 wo, const Bsdf& bsdf, float lNumU, float lDir1,
+                             float lDir2, float uBC, float uBD1, float uBD2) {  // Scene.hs:110-118
+  int lc = (int)c.lights.size();
+  if (lc == 1) return estimateDirect(c, 0, p, eps, nrm, wo, bsdf, lDir1, lDir2, uBC, uBD1, uBD2);
+  if (lc > 1) {
+    int ln = (int)std::floor(lNumU * (float)lc);
+    ln = ln < lc - 1 ? ln : lc - 1;
+    return sscale(estimateDirect(c, ln, p, eps, nrm, wo, bsdf, lDir1, lDir2, uBC, uBD1, uBD2), (float)lc);
+  }
+  return sconst(0);
+}
+
+static Spec liDirect(Ctx& c, const Ray& ray, const PixelSampler& ps, uint32_t n, int d, int md) {
+  Isect is;
+  c.rays++;
+  if (!intersectFull(*c.s, ray, &is)) return sconst(0);
+  uint32_t k = 2 * (uint32_t)d;
+  float uln = ps.get1(n, k), ubc = ps.get1(n, k + 1);
+  float uld1, uld2, ubd1, ubd2;
+  ps.get2(n, k, &uld1, &uld2);
+  ps.get2(n, k + 1, &ubd1, &ubd2);
+  const Scene& s = *c.s;
+  int mat = is.prim < s.ntri ? s.meshes[s.tri_mesh[is.prim]].material : s.shapes[is.prim - s.ntri].material;
+  Bsdf bsdf = makeBsdf(c, mat, is.dgg, is.dgs);
+  V3 p = bsdf.p, nrm = bsdf.cs.n, wo = -ray.d;
+  float e = is.eps;
+  Spec l = sampleOneLightAt(c, p, e, nrm, wo, bsdf, uln, uld1, uld2, ubc, ubd1, ubd2);
+  auto cont = [&](int flags) {  // cont (DirectLighting.hs:49-63)
+    if (d + 1 == md) return sconst(0);
+    BsdfSample smp = sampleBsdf(bsdf, wo, 0.5f, 0.5f, 0.5f, flags);
+    if (smp.pdf == 0.0f) return sconst(0);
+    return smp.f * liDirect(c, Ray{p, smp.wi, e, kInf}, ps, n, d + 1, md);
+  };
+  Spec re = cont(BX_S | BX_R);
+  Spec tr = cont(BX_S | BX_T);
+  return l + re + tr + lEmitAt(c, is, wo);
 }
 
 // --------------------------------------------------------------------- film
@@ -1219,19 +1268,23 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
   int ew = x1 - x0 + 1, eh = y1 - y0 + 1;
   uint32_t nu = job->spp_nu, nv = job->spp_nv, N = nu * nv;
   int md = job->max_depth, sd = job->sample_depth, rr = job->rr_start_depth;
+  const bool direct = job->integrator == BGPU_INTEGRATOR_DIRECT;
+  // the sampler's table sizes: SurfaceIntegrator s1d s2d (Path.hs:18-30,
+  // DirectLighting.hs:14-20)
+  const uint32_t n1d = direct ? 2u * (uint32_t)md : 4u * (uint32_t)sd;
+  const uint32_t n2d = direct ? 2u * (uint32_t)md : 3u * (uint32_t)sd;
   for (int pass = pass_begin; pass < pass_end; ++pass) {
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
     for (long pix = 0; pix < (long)ew * eh; ++pix) {
       int ix = x0 + (int)(pix % ew), iy = y0 + (int)(pix / ew);
-      PixelSampler ps{pixelKey(job->seed, (uint32_t)pass, ix, iy), nu, nv, N, 4u * (uint32_t)sd,
-                      3u * (uint32_t)sd};
+      PixelSampler ps{pixelKey(job->seed, (uint32_t)pass, ix, iy), nu, nv, N, n1d, n2d};
       for (uint32_t n = 0; n < N; ++n) {
         float ox, oy, lu, lv;
         ps.camera(n, &ox, &oy);
         ps.lens(n, &lu, &lv);
         float sx = (float)ix + ox, sy = (float)iy + oy;
         Ray r = fireRay(camera, sx, sy, lu, lv);
-        Spec L = li(c, r, ps, n, md, rr);
+        Spec L = direct ? liDirect(c, r, ps, n, 0, md) : li(c, r, ps, n, md, rr);
         addSample(c, wxyz, sq, W, H, sx, sy, L);
       }
     }

@@ -132,7 +132,7 @@ def textured_scene(res=48, seed=3):
             uvs=np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32))
     sb.shape(abi.SHAPE_SPHERE, (0.6,), o2w=T.translate((-0.8, 0.0, 0.6)), material=paper)
     sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((0.8, 0.3, 0.5)), material=plastic)
-    sb.shape(abi.SHAPE_QUAD, (2.0, 1.0), o2w=T.translate((0.0, 1.5, 1.0)) @ T.rotate_x(90),
+    sb.shape(abi.SHAPE_QUAD, (2.0, 1.0), o2w=T.rotate_x(90) @ T.translate((0.0, 1.5, 1.0)),
              material=wall)
     lt = T.rotate_x(180) @ T.translate((0.0, 0.0, 3.0))
     sb.shape(abi.SHAPE_QUAD, (0.7, 0.7), o2w=lt, material=fm, emission=rgb_illum(8, 8, 8))

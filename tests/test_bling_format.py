@@ -236,6 +236,9 @@ def test_reference_examples():
     assert b.flat.desc.film.filter == abi.FILTER_MITCHELL
     with pytest.raises(B.Unsupported):  # its last renderer block is sppm
         b.spec()
+    # blackbody-emission.bling renders with `directLighting maxDepth 5`
+    s = B.load(os.path.join(REF_EXAMPLES, "blackbody-emission.bling")).spec()
+    assert (s.integrator, s.max_depth) == ("direct", 5)
     # glass-torus.bling: the floor's graph paper (0.05, uv 15 15 0 0)
     b = B.load(os.path.join(REF_EXAMPLES, "glass-torus.bling"))
     tex = [b.flat.materials[i].tex for i in range(b.flat.desc.num_materials)]

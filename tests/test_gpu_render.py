@@ -421,3 +421,73 @@ def test_scene_file_renders_like_the_flat_scene(gpu_ctx, tmp_path):
         gs.close()
     assert np.array_equal(a[..., 0], b[..., 0])
     np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6)
+
+
+def test_direct_lighting_matches_oracle(gpu_ctx):
+    """The direct-lighting integrator (Integrator/DirectLighting.hs) on the
+    materials scene, whose glass and mirror spheres branch: the same seed
+    tracks the oracle, another seed is within 3 sigma."""
+    from tests import cases
+    flat = cases.materials_scene(res=40, env=True)
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=0, passes=4, seed=41, variance=True,
+                   integrator="direct")
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    assert np.array_equal(g[..., 0], o[..., 0])
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    spec2 = JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=0, passes=4, seed=42, variance=True,
+                    integrator="direct")
+    o2, osq2, _ = pyoracle.render(flat, spec2)
+    frac, zmean, zmax = compare(g, gsq, o2, osq2)
+    assert frac < 0.015, (frac, zmean, zmax)
+    assert abs(zmean) < 0.08, zmean
+
+
+def test_direct_lighting_mirror_closed_form(gpu_ctx):
+    """r Le (w + 1) through a mirror under a quad light (the oracle's pin of
+    the same scene), r Le w at maxDepth 1."""
+    from tests.test_oracle_pins import mirror_under_light
+    r, Le, theta = 0.9, 3.0, 60.0
+    c = np.cos(np.radians(theta))
+    pdf_l = (3.0 / c) ** 2 / (c * (4 * 100.0 * 100.0))
+    w = 1.0 / (1.0 + pdf_l ** 2)
+    flat = mirror_under_light(r=r, Le=Le, theta=theta)
+    for md, want in ((1, r * Le * w), (2, r * Le * (w + 1.0)), (5, r * Le * (w + 1.0))):
+        spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=md, sample_depth=0, passes=1, seed=3, integrator="direct")
+        img, _ = render_gpu(gpu_ctx, flat, spec)
+        np.testing.assert_allclose(img[..., 2] / img[..., 0], want, rtol=1e-5)
+
+
+def test_direct_and_path_jobs_served_together(gpu_ctx):
+    """Direct-lighting and path jobs in flight together on the served path
+    each give their plain result; a direct-lighting job deeper than the
+    branch stacks hold is refused."""
+    from tests import cases
+    flat = cases.materials_scene(res=24, env=True)
+    specs = [JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=0, passes=2, seed=s, integrator="direct")
+             for s in (1, 2)]
+    specs += [JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, passes=2, seed=3)]
+    gs = gpu_ctx.scene(flat)
+    try:
+        plain = [GpuRenderer(sp).render_plain(gs) for sp in specs]
+        jobs = [Job(gs, sp) for sp in specs]
+        for sp, j, (p_img, _) in zip(specs, jobs, plain):
+            j.wait(timeout=120)
+            img, _, done, state = j.poll()
+            assert state == abi.JOB_DONE and done == sp.passes
+            assert np.array_equal(img[..., 0], p_img[..., 0])
+            np.testing.assert_allclose(img, p_img, rtol=1e-4, atol=1e-5)
+            j.release()
+        assert gpu_ctx.config_get("pt.dl_depth") == 4
+        deep = JobSpec(spp_nu=1, spp_nv=1, max_depth=6, passes=1, integrator="direct")
+        with pytest.raises(abi.BgpuError) as e:
+            GpuRenderer(deep).render_plain(gs)
+        assert e.value.code == abi.E_INVALID and "pt.dl_depth" in str(e.value)
+    finally:
+        gs.close()

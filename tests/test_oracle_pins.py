@@ -313,3 +313,73 @@ def test_textured_scene_differs_from_untextured():
     b, _, _ = pyoracle.render(flat, spec)
     assert np.isfinite(a).all() and np.isfinite(b).all()
     assert abs(a[..., 2].mean() - b[..., 2].mean()) > 0.02 * abs(b[..., 2].mean())
+
+
+def _direct(flat, md, spp=4, passes=2, seed=5):
+    spec = JobSpec(spp_nu=spp, spp_nv=spp, max_depth=md, sample_depth=0, passes=passes, seed=seed,
+                   variance=True, integrator="direct")
+    img, sq, _ = pyoracle.render(flat, spec)
+    return mean_se(img, sq)
+
+
+def test_direct_lighting_analytic_and_black_escapes():
+    """DirectLighting (Integrator/DirectLighting.hs): a Lambertian plane
+    under a uniform sky reflects rho * L and a sphere light gives
+    rho * Le * (R / D)^2 below it, as for the path integrator's direct term;
+    a ray that escapes is black (the integrator's `maybe black`), so a
+    camera that sees only sky renders zero, where the path integrator
+    renders the sky."""
+    m, se = _direct(cases.plane_under_sky(rho=0.5, sky=1.0), md=3, spp=8, passes=4)
+    y, s = m[..., 1].mean(), np.sqrt((se[..., 1] ** 2).sum()) / m[..., 1].size
+    assert abs(y - 0.5) < 4 * s + 1e-4, (y, s)
+    R, D, Le, rho = 0.25, 2.0, 4.0, 0.5
+    m, se = _direct(cases.sphere_light_over_plane(rho=rho, Le=Le, R=R, D=D), md=2, spp=8, passes=4)
+    y, s = m[..., 1].mean(), np.sqrt((se[..., 1] ** 2).sum()) / m[..., 1].size
+    want = rho * Le * (R / D) ** 2
+    assert abs(y - want) < 4 * s + 1e-4 * want, (y, want, s)
+    sky = cases.sky_only(sky=2.0)
+    m, _ = _direct(sky, md=4)
+    assert (m[..., 1:] == 0).all()
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=1, passes=1, seed=5)
+    img, _, _ = pyoracle.render(sky, spec)
+    assert img[..., 2].sum() > 0
+
+
+def mirror_under_light(r=0.9, Le=3.0, theta=60.0, res=4):
+    """A mirror (r, no Fresnel) at z = 0 seen at `theta` degrees from its
+    normal by a narrow camera, under a large emissive quad at z = 3 facing
+    down, behind the camera's rays, with the blackbody material."""
+    sb = SceneBuilder()
+    mir = sb.material(Material.mirror(const(r)))
+    blk = sb.material(Material.black())
+    sb.shape(abi.SHAPE_QUAD, (50.0, 50.0), material=mir)
+    sb.shape(abi.SHAPE_QUAD, (100.0, 100.0), o2w=T.rotate_x(180) @ T.translate((0.0, 0.0, 3.0)), material=blk,
+             emission=const(Le))
+    t = np.radians(theta)
+    c2w = T.look_at((0.0, -4.0 * np.sin(t), 4.0 * np.cos(t)), (0.0, 0.0, 0.0), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, 0.02, res, res))
+    sb.set_film(res, res)
+    return sb.build()
+
+
+def test_direct_lighting_specular_continuation():
+    """At a mirror the light sample is zero (a specular BSDF evaluates to
+    0) and the BSDF sample reaches the light with the power-heuristic weight
+    w = 1 / (1 + pdf_light^2), pdf_light = dist^2 / (cos theta_l area); with
+    maxDepth >= 2 the specular continuation (`cont`, DirectLighting.hs:49-63)
+    adds f * Le, where f = r * fr(cos theta_o) = r is specRefl's sampled
+    value with the no-op Fresnel of the mirror (Reflection/Specular.hs:10-18;
+    no cosine and no pdf division).  The pixel is r Le (w + 1), and r Le w at
+    maxDepth 1; the blackbody light adds no further branch."""
+    r, Le, theta = 0.9, 3.0, 60.0
+    c = np.cos(np.radians(theta))
+    dist = 3.0 / c  # mirror point to the light along the reflected ray
+    pdf_l = dist ** 2 / (c * (4 * 100.0 * 100.0))  # solid-angle pdf of the quad light
+    w = 1.0 / (1.0 + pdf_l ** 2)
+    flat = mirror_under_light(r=r, Le=Le, theta=theta)
+    m1, _ = _direct(flat, md=1)
+    m2, _ = _direct(flat, md=2)
+    m5, _ = _direct(flat, md=5)
+    np.testing.assert_allclose(m1[..., 1], r * Le * w, rtol=1e-5)
+    np.testing.assert_allclose(m2[..., 1], r * Le * (w + 1.0), rtol=1e-5)
+    np.testing.assert_allclose(m5[..., 1], m2[..., 1], rtol=1e-6)  # the light is black: no further branch
