@@ -291,13 +291,26 @@ def cpu_baseline(flat, batches, seconds):
         reps += 1
         if total_t >= seconds:
             break
+    # the same restatement on one core, over a quarter of the time
+    n1 = 4096
+    q1, t1 = 0, 0.0
+    while t1 < seconds / 4:
+        t0 = time.perf_counter()
+        for k, v in samples.items():
+            o.intersect(v[:n1], threads=1)
+            o.intersects(v[:n1], threads=1)
+        t1 += time.perf_counter() - t0
+        q1 += 4 * n1
     o.close()
-    log("cpu baseline: %d queries in %.2fs on %d threads" % (total_q, total_t, threads))
+    log("cpu baseline: %d queries in %.2fs on %d threads; %d in %.2fs on 1" % (total_q, total_t, threads, q1, t1))
     return {"value": round(total_q / total_t / 1e6, 4), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
             "sample": "%d rays from each of the 4 query batches (closest+any, coherent+incoherent), "
                       "%d time(s) over, %.1f s; oracle C++ restatement with its own median BVH, "
-                      "%d OpenMP threads" % (n, reps, total_t, threads)}
+                      "%d OpenMP threads" % (n, reps, total_t, threads),
+            "value_1core": round(q1 / t1 / 1e6, 4),
+            "sample_1core": "4096 rays from each of the 4 query batches, %d queries in %.1f s, one thread"
+                            % (q1, t1)}
 
 
 # ------------------------------------------------------------------ render
