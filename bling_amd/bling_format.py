@@ -48,6 +48,7 @@ from fractions import Fraction
 import numpy as np
 
 from . import abi
+from . import noise
 from . import transform as T
 from .scene import Material, SceneBuilder, rgb_illum, rgb_refl, spd
 
@@ -151,6 +152,36 @@ def _clamp(v, lo, hi):  # Math.hs:84-87
 
 
 # ----------------------------------------------------------------- parsing
+def heightmap_mesh(elev, ns, nt):
+    """heightMap elev (ns, nt) (Primitive/Heightmap.hs:14-47) in binary32:
+    vertices (x, elev(x, z), z) on the (ns x nt) lattice of [0, 1]^2 (z outer),
+    two triangles per cell, shading normals -normalize(dx, ex + ez, dz) from
+    central differences with ex = 1 / ns, ez = 1 / nt, and uv
+    (x / (ns - 1), z / (ns - 1)) -- both divided by ns - 1, as the reference
+    writes it.  Returns (vertices, faces, normals, uvs) in object space."""
+    fns, fnt = F(ns), F(nt)
+    xs = np.arange(ns, dtype=F) / (fns - F(1))
+    zs = np.arange(nt, dtype=F) / (fnt - F(1))
+    cx = np.tile(xs, nt)
+    cz = np.repeat(zs, ns)
+    verts = np.stack([cx, elev(cx, cz), cz], 1).astype(F)
+    ex, ez = F(1) / fns, F(1) / fnt
+    dx = elev(cx - ex, cz) - elev(cx + ex, cz)
+    dz = elev(cx, cz - ez) - elev(cx, cz + ez)
+    dy = np.full_like(dx, ex + ez)
+    sq = (dx * dx + dy * dy) + dz * dz
+    inv = F(1) / np.sqrt(sq)
+    nrm = np.where((sq != 0)[:, None], np.stack([dx * inv, dy * inv, dz * inv], 1),
+                   np.array([0, 1, 0], F)[None, :])
+    nrm = (-nrm).astype(F)
+    uvs = np.stack([cx / (fns - F(1)), cz / (fns - F(1))], 1).astype(F)
+    x, y = np.meshgrid(np.arange(ns - 1), np.arange(nt - 1))
+    x, y = x.reshape(-1), y.reshape(-1)
+    v = lambda a, b: a + b * ns
+    faces = np.stack([v(x, y), v(x + 1, y), v(x + 1, y + 1), v(x, y), v(x + 1, y + 1), v(x, y + 1)], 1)
+    return verts, faces.reshape(-1, 3).astype(np.int64), nrm, uvs
+
+
 class _Parser:
     def __init__(self, text):
         self.toks = _tokens(text)
@@ -434,6 +465,30 @@ class _Parser:
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)
 
+    def scalar_map_2d(self):
+        """pScalarMap2d (MaterialParser.hs:232-245): `fbm z octaves n omega w`
+        (texMap3dTo2d (fbm n w) z) or `scale f map`; returns elev(x, y) over
+        binary32 arrays."""
+        self.open()
+        t = self.word()
+        if t == "fbm":
+            z = self.flt()
+            octaves = self.named_int("octaves")
+            omega = self.named_flt("omega")
+
+            def elev(x, y):
+                return noise.fbm(octaves, omega, x, y, np.full(np.shape(x), z, F))
+        elif t == "scale":
+            k = F(self.flt())
+            inner = self.scalar_map_2d()
+
+            def elev(x, y):
+                return k * inner(x, y)
+        else:
+            self.fail("unknown scalar map type " + t)
+        self.close()
+        return elev
+
     # -- primitives (PrimitiveParser.hs)
     def prim_block(self):
         self.open()
@@ -455,13 +510,20 @@ class _Parser:
                 for k in range(len(rest) - 1):
                     flat += [f0, rest[k], rest[k + 1]]
             stmt = [("mesh", np.array(verts, F).reshape(-1, 3), np.array(flat, np.int64).reshape(-1, 3),
-                     self.transform, self.material)]
+                     self.transform, self.material, None, None)]
+        elif t == "heightMap":  # PrimitiveParser.hs:39-45: its own transform block
+            ns = self.integ()
+            nt = self.integ()
+            elev = self.scalar_map_2d()
+            xf = self.transform_block()
+            verts, faces, norms, uvs = heightmap_mesh(elev, ns, nt)
+            stmt = [("mesh", verts, faces, xf, self.material, norms, uvs)]
         elif t == "shape":
             kind, params = self.shape_block()
             sid = self.curr_id
             self.curr_id += 1
             stmt = [("shape", kind, params, self.transform, self.material, self.emit, sid)]
-        elif t in ("heightMap", "waveFront", "julia", "mandelbulb", "bezier"):
+        elif t in ("waveFront", "julia", "mandelbulb", "bezier"):
             raise Unsupported("primitive %r" % t)
         else:
             self.fail("unknown primitive type " + t)
@@ -642,9 +704,9 @@ class BlingScene:
         for stmt in reversed(p.prims):  # mkScene order (RenderJob.hs:49-52)
             for rec in stmt:
                 if rec[0] == "mesh":
-                    _, verts, faces, xf, m = rec
+                    _, verts, faces, xf, m, norms, uvs = rec
                     if len(faces):
-                        sb.mesh(verts, faces, o2w=xf, material=mat(m))
+                        sb.mesh(verts, faces, o2w=xf, material=mat(m), normals=norms, uvs=uvs)
                 else:
                     _, kind, params, xf, m, emit, sid = rec
                     sb.shape(kind, params, o2w=xf, material=mat(m), emission=emit)

@@ -8,7 +8,7 @@ src/lib/Graphics/Bling/IO/*.hs and against scenes built directly.
   polygon fans, spectra (rgbR/rgbI/spd/temp/hex), statement and light
   order, defaults of the start state, errors.
 - The reference's own example scenes (read in place when the reference
-  checkout is present): the six the GPU path renders build, with the
+  checkout is present): the seven the GPU path renders build, with the
   primitive counts their text describes; the others fail either as
   Unsupported or, where the reference's current grammar rejects them too,
   as syntax errors.
@@ -152,7 +152,7 @@ def test_errors():
     with pytest.raises(B.BlingSyntaxError):
         B.parse("imageSize 10")
     with pytest.raises(B.Unsupported):
-        B.parse("prim { heightMap 4 4 { fbm 0 octaves 3 omega 0.5 } { } }")
+        B.parse("prim { mandelbulb order 8 epsilon 0.001 iterations 20 }")
     with pytest.raises(B.Unsupported):
         B.parse("material { matte kd { checker 1 1 1 tex1 { checker 2 2 2 tex1 { constant rgbR 1 1 1 } "
                 "tex2 { constant rgbR 0 0 0 } } tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
@@ -210,6 +210,7 @@ EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit li
     "plastic-test.bling": (2, 7, 0, (640, 360)),
     "blackbody-emission.bling": (0, 16, 0, (640, 400)),
     "glass-torus.bling": (0, 4, 0, (640, 360)),
+    "cornell-box-underwater.bling": (79210, 3, 0, (720, 720)),
 }
 
 
@@ -245,3 +246,61 @@ def test_reference_examples():
     paper = [t for t in tex if t.kind == abi.TEX_GRAPHPAPER]
     assert len(paper) == 1 and paper[0].mapping == abi.MAP_UV
     assert list(paper[0].params)[:5] == [F(0.05), 15, 15, 0, 0]
+
+
+def test_noise_lattice_and_gradients():
+    """perlin3d (Texture.hs:358-397) is 0 on the integer lattice, and its
+    gradient at a lattice point is the gradient vector its hash selects:
+    with h = perm[perm[perm[i] + j] + k] & 15, the 12 + 4 set
+    u = dx if h < 8 or h in (12, 13) else dy, v = dy if h < 4 or h in (12, 13)
+    else dz, signs from bits 0 and 1 (the fade weights have zero slope there,
+    so the corner's own term is the whole first-order change)."""
+    from bling_amd import noise
+    from bling_amd.noise_table import PERM
+    rng = np.random.default_rng(1)
+    ijk = rng.integers(-40, 300, (64, 3)).astype(np.float32)
+    assert (noise.perlin3d(ijk[:, 0], ijk[:, 1], ijk[:, 2]) == 0).all()
+    P = PERM + PERM
+    eps = np.float32(1e-3)
+    for i, j, k in ijk.astype(np.int64)[:24]:
+        h = P[P[P[i & 255] + (j & 255)] + (k & 255)] & 15
+        g = np.zeros(3)
+        ua = 0 if (h < 8 or h in (12, 13)) else 1
+        va = 1 if (h < 4 or h in (12, 13)) else 2
+        g[ua] += -1.0 if h & 1 else 1.0
+        g[va] += -1.0 if h & 2 else 1.0
+        fd = []
+        for a in range(3):
+            d = np.zeros(3, np.float32)
+            d[a] = eps
+            p = np.float32([i, j, k]) + d
+            fd.append(float(noise.perlin3d(p[0:1], p[1:2], p[2:3])[0]) / float(eps))
+        np.testing.assert_allclose(fd, g, atol=2e-2)
+    x = rng.uniform(-50, 50, (4096, 3)).astype(np.float32)
+    v = noise.perlin3d(x[:, 0], x[:, 1], x[:, 2])
+    assert np.abs(v).max() < 1.5 and v.std() > 0.1
+    # fbm's first octave is the noise itself; later octaves add omega^i-weighted terms
+    np.testing.assert_array_equal(noise.fbm(1, 0.5, x[:, 0], x[:, 1], x[:, 2]), v)
+    assert not np.array_equal(noise.fbm(2, 0.5, x[:, 0], x[:, 1], x[:, 2]), v)
+
+
+def test_heightmap_mesh_closed_forms():
+    """heightMap (Primitive/Heightmap.hs:14-47): ns x nt lattice on [0, 1]^2
+    with z outer, two triangles per cell, uv divided by ns - 1 on both axes,
+    and normals -normalize(dx, ex + ez, dz) by central differences; for a
+    plane elev = a x + c the normal is -(-2 a ex, ex + ez, 0) / len."""
+    ns, nt, a = 5, 3, 0.5
+    v, f, n, uv = B.heightmap_mesh(lambda x, z: F(a) * x + F(0.25), ns, nt)
+    assert v.shape == (15, 3) and f.shape == (2 * 4 * 2, 3)
+    np.testing.assert_array_equal(f[:2], [[0, 1, 6], [0, 6, 5]])
+    np.testing.assert_array_equal(v[7], F([F(2) / F(4), F(a) * (F(2) / F(4)) + F(0.25), F(1) / F(2)]))
+    np.testing.assert_array_equal(uv[-1], F([F(1) / F(4), F(1) / F(4)]))  # (1, 1) / (ns - 1)
+    ex, ez = 1.0 / ns, 1.0 / nt
+    want = -np.array([-2 * a * ex, ex + ez, 0.0])
+    want /= np.linalg.norm(want)
+    np.testing.assert_allclose(n, np.broadcast_to(want, n.shape), rtol=1e-6, atol=1e-7)
+    b = B.parse("material { matte kd { constant rgbR 0.5 0.5 0.5 } sigma { constant 0 } } "
+                "prim { heightMap 6 4 { scale 0 { fbm 0 octaves 2 omega 0.5 } } { scale 2 1 2 } }")
+    f2 = b.flat
+    assert f2.num_triangles == 5 * 3 * 2 and f2.normals is not None and f2.uvs is not None
+    assert (f2.vertices[:, 1] == 0).all() and np.allclose(f2.normals, [0, -1, 0])

@@ -491,3 +491,39 @@ def test_direct_and_path_jobs_served_together(gpu_ctx):
         assert e.value.code == abi.E_INVALID and "pt.dl_depth" in str(e.value)
     finally:
         gs.close()
+
+
+HEIGHTMAP_BLING = """
+imageSize 48 48
+filter box
+transform { lookAt { pos 0 3 -3 look 0 0 0 up 0 1 0 } }
+camera { perspective fov 50 lensRadius 0 focalDistance 1 }
+newTransform { }
+material { plastic kd { constant rgbR 0.3 0.5 0.2 } ks { constant rgbR 0.4 0.4 0.4 } rough { constant 0.05 } }
+prim { heightMap 40 30 { scale 0.6 { fbm 0 octaves 4 omega 0.5 } } { translate -0.5 0 -0.5 scale 4 1 4 } }
+material { blackbody }
+emission { rgbI 6 6 6 }
+transform { rotateX -90 translate 0 3 0 }
+prim { shape { quad 0.7 0.7 } }
+emission { none }
+renderer { sampler sampled { sampler { stratified 4 4 } integrator { path maxDepth 5 sampleDepth 3 } } }
+"""
+
+
+def test_heightmap_bling_scene_matches_oracle(gpu_ctx):
+    """A heightMap primitive from .bling text (fBm elevation baked into a
+    mesh with shading normals and uvs, Primitive/Heightmap.hs) renders on
+    the GPU as in the oracle."""
+    from bling_amd import bling_format
+    b = bling_format.parse(HEIGHTMAP_BLING)
+    assert b.flat.num_triangles == 39 * 29 * 2 and b.flat.normals is not None
+    spec = b.spec(passes=2, seed=17)
+    spec.variance = True
+    g, gsq = render_gpu(gpu_ctx, b.flat, spec)
+    o, osq, _ = pyoracle.render(b.flat, spec)
+    assert np.isfinite(g).all() and g[..., 2].mean() > 0
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    assert np.median(np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)) < 1e-4
