@@ -13,7 +13,7 @@ The grammar follows the reference's Parsec parser:
   IO/CameraParser.hs          perspective camera (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
-                              blackbody with constant textures, checker and
+                              transMatte, blackbody with constant textures, checker and
                               graphPaper over constants, blends of constants
   IO/PrimitiveParser.hs       mesh (polygon faces triangulated as fans,
                               TriangleMesh.hs:23-29) and shape primitives
@@ -461,7 +461,12 @@ class _Parser:
         if t == "mirror":
             kr = self.spectrum_texture("kr")
             return self.textured(lambda s: Material.mirror(s[0]), [kr])
-        if t in ("bumpMap", "substrate", "shinyMetal", "transMatte"):
+        if t == "transMatte":  # pMatteTranslucent (MaterialParser.hs:92-97)
+            kr = self.spectrum_texture("kr")
+            kt = self.spectrum_texture("kt")
+            ks = self.scalar_texture("ks")
+            return self.textured(lambda s: Material.trans_matte(s[0], s[1], ks), [kr, kt])
+        if t in ("bumpMap", "substrate", "shinyMetal"):
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)
 

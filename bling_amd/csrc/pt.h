@@ -21,7 +21,8 @@ enum { K_LAMBERT = 0, K_OREN = 1, K_MF = 2, K_SPECR = 3, K_SPECT = 4 };
 enum { FR_NOOP = 0, FR_DIEL = 1, FR_COND = 2 };
 
 struct DBxdf {
-  int32_t kind, type, fr, pad;
+  int32_t kind, type, fr;
+  int32_t flip;  // brdfToBtdf: the diffuse lobe with wi mirrored to the other hemisphere
   float A, B, e, ei, et, pad2[3];
   float r[16], eta[16], k[16];
 };

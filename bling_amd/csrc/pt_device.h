@@ -491,6 +491,7 @@ __device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // 
 
 // bxdfEval b wo wi as a term; false when the component contributes nothing
 __device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term* t) {
+  if (b.flip) wi.z = -wi.z;  // brdfToBtdf: e wo wi = bxdfEval brdf wo (otherHemisphere wi)
   if (b.kind == K_LAMBERT) {
     *t = Term{&b, T_RS, kInvPi * acos_t(wo), 0.0f};
     return true;
@@ -515,6 +516,7 @@ __device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term*
   return false;  // specular components evaluate to zero
 }
 __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
+  if (b.flip) wi.z = -wi.z;
   if (b.kind == K_LAMBERT || b.kind == K_OREN) return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
   if (b.kind == K_MF) {
     f3 whp = add3(wo, wi);
@@ -541,6 +543,7 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
       *wi = b.kind == K_LAMBERT ? wo : w;
       *pdf = 0.0f;
     }
+    if (b.flip) wi->z = -wi->z;  // brdfToBtdf: the sampled direction's other hemisphere
     return;
   }
   if (b.kind == K_MF) {  // Microfacet.hs:42-54,175-182

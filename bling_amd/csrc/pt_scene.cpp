@@ -210,6 +210,33 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         o.c[o.n++] = t;
         break;
       }
+      case BGPU_MAT_TRANSMATTE: {  // translucentMatte (Material.hs:43-53)
+        const float sig = m.scalar[0];
+        auto lobe = [&](const float* s, bool flip) {
+          DBxdf b = bx_zero();
+          std::memcpy(b.r, s, 64);
+          if (sig == 0.0f) {
+            b.kind = K_LAMBERT;
+          } else {
+            b.kind = K_OREN;
+            float c = clampf(sig, 0.0f, 1.0f);
+            float c2 = c * c;
+            b.A = 1.0f - (c2 / (2.0f * (c2 + 0.33f)));
+            b.B = 0.45f * c2 / (c2 + 0.09f);
+          }
+          b.type = flip ? (BX_T | BX_D) : (BX_R | BX_D);  // bxdfTypeFlip [Reflection, Transmission]
+          b.flip = flip ? 1 : 0;
+          return b;
+        };
+        float r[16], t[16];
+        for (int k = 0; k < 16; ++k) {
+          r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
+          t[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k])) * (1.0f - r[k]);
+        }
+        o.c[o.n++] = lobe(r, false);
+        o.c[o.n++] = lobe(t, true);
+        break;
+      }
       case BGPU_MAT_MIRROR: {
         DBxdf r = bx_zero();
         r.kind = K_SPECR;

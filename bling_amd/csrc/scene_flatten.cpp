@@ -179,8 +179,13 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   bool dev_tri = false;
   if (ctx->bvh_device && d->num_triangles > (uint32_t)kSmall) {
     bool fallback = true;
-    if (int rc = dbvh.build(ctx, sc->verts.as<float>(), sc->tri_idx.as<int32_t>(), (int)d->num_triangles, &fallback))
+    const int rc = dbvh.build(ctx, sc->verts.as<float>(), sc->tri_idx.as<int32_t>(), (int)d->num_triangles, &fallback);
+    if (rc == BGPU_E_BUDGET || rc == BGPU_E_NOMEM) {
+      dbvh.release();  // its scratch does not fit: the host builds with less device memory
+      fallback = true;
+    } else if (rc) {
       return rc;
+    }
     dev_tri = !fallback;
   }
 

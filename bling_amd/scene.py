@@ -158,6 +158,11 @@ class Material:
         return Material(abi.MAT_MIRROR, kr)
 
     @staticmethod
+    def trans_matte(kr, kt, sigma=0.0):
+        """translucentMatte kr kt sigma (Material.hs:43-53)."""
+        return Material(abi.MAT_TRANSMATTE, kr, kt, sigma)
+
+    @staticmethod
     def black():
         return Material(abi.MAT_BLACK)
 

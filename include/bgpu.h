@@ -85,7 +85,10 @@ enum {
   BGPU_MAT_PLASTIC = 2, /* spec[0]=kd, spec[1]=ks, scalar[0]=roughness */
   BGPU_MAT_METAL = 3,   /* spec[0]=eta, spec[1]=k, scalar[0]=roughness */
   BGPU_MAT_GLASS = 4,   /* spec[0]=reflect, spec[1]=transmit (both clamped to [0,1]), scalar[0]=ior */
-  BGPU_MAT_MIRROR = 5   /* spec[0]=reflect (clamped to [0,1]) */
+  BGPU_MAT_MIRROR = 5,  /* spec[0]=reflect (clamped to [0,1]) */
+  BGPU_MAT_TRANSMATTE = 6 /* translucentMatte (Material.hs:43-53): spec[0]=kr, spec[1]=kt (both clamped
+                             to [0,1], kt then scaled by 1 - kr), scalar[0]=sigma; a diffuse reflection and
+                             the same lobe turned into transmission (brdfToBtdf, Reflection.hs:188-195) */
 };
 /* A texture that selects, per hit, between two constant spectra
  * (Texture.hs:191-219): spec[k] of the material is tex1's value and

@@ -562,6 +562,7 @@ struct Bxdf {
   int fr;
   float ei, et; // dielectric Fresnel / transmission etas
   Spec eta, k;  // conductor
+  int flip;     // brdfToBtdf (Reflection.hs:188-195): wi mirrored to the other hemisphere
 };
 
 struct Bsdf {
@@ -636,6 +637,7 @@ static Spec orenNayar(const Bxdf& b, V3 wo, V3 wi) {  // Diffuse.hs:431-443
 }
 
 static Spec bxEval(const Bxdf& b, V3 wo, V3 wi) {
+  if (b.flip) wi.z = -wi.z;
   switch (b.kind) {
     case K_LAMBERT: return sscale(b.r, kInvPi * absCosT(wo));
     case K_OREN: return sscale(orenNayar(b, wo, wi), kInvPi * absCosT(wo));
@@ -654,6 +656,7 @@ static Spec bxEval(const Bxdf& b, V3 wo, V3 wi) {
   }
 }
 static float bxPdf(const Bxdf& b, V3 wo, V3 wi) {
+  if (b.flip) wi.z = -wi.z;
   switch (b.kind) {
     case K_LAMBERT:
     case K_OREN: return cosPdf(wo, wi);
@@ -682,6 +685,7 @@ static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, 
         *wi = b.kind == K_LAMBERT ? wo : w;
         *pdf = 0.0f;
       }
+      if (b.flip) wi->z = -wi->z;
       return;
     }
     case K_MF: {  // Microfacet.hs:42-54,175-182
@@ -846,6 +850,29 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
       t.et = ior;
       b.c[b.n++] = r;
       b.c[b.n++] = t;
+      break;
+    }
+    case BGPU_MAT_TRANSMATTE: {  // translucentMatte (Material.hs:43-53)
+      float sig = m.scalar[0];
+      Spec r = sclamp(s0, 0.0f, 1.0f);
+      Spec t = sclamp(s1, 0.0f, 1.0f);
+      for (int k = 0; k < 16; ++k) t[k] = t[k] * (1.0f - r[k]);
+      for (int side = 0; side < 2; ++side) {
+        Bxdf x{};
+        x.r = side ? t : r;
+        if (sig == 0.0f) {
+          x.kind = K_LAMBERT;
+        } else {
+          x.kind = K_OREN;
+          float sc = clampf(sig, 0.0f, 1.0f);
+          float sig2 = sc * sc;
+          x.A = 1.0f - (sig2 / (2.0f * (sig2 + 0.33f)));
+          x.B = 0.45f * sig2 / (sig2 + 0.09f);
+        }
+        x.type = side ? (BX_T | BX_D) : (BX_R | BX_D);
+        x.flip = side;
+        b.c[b.n++] = x;
+      }
       break;
     }
     case BGPU_MAT_MIRROR: {  // Material.hs:343-347

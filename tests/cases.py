@@ -37,11 +37,12 @@ def sky_map(spec, w=1, h=1):
     return np.tile(np.asarray(spec, np.float32), (h, w, 1))
 
 
-def plane_under_sky(rho=0.5, sky=1.0, res=8, fov=2.0):
+def plane_under_sky(rho=0.5, sky=1.0, res=8, fov=2.0, material=None):
     """A large diffuse quad at z = 0 under a uniform infinite light; the camera
-    looks down at the origin.  Direct reflected radiance is rho * L."""
+    looks down at the origin.  Direct reflected radiance is rho * L (for a
+    matte plane; `material` replaces it)."""
     sb = SceneBuilder()
-    m = sb.material(Material.matte(const(rho)))
+    m = sb.material(material if material is not None else Material.matte(const(rho)))
     sb.shape(abi.SHAPE_QUAD, (1000.0, 1000.0), material=m)
     sb.infinite_light(sky_map(const(sky)))
     c2w = T.look_at((0.0, -3.0, 4.0), (0.0, 0.0, 0.0), (0, 0, 1))

@@ -8,7 +8,7 @@ src/lib/Graphics/Bling/IO/*.hs and against scenes built directly.
   polygon fans, spectra (rgbR/rgbI/spd/temp/hex), statement and light
   order, defaults of the start state, errors.
 - The reference's own example scenes (read in place when the reference
-  checkout is present): the seven the GPU path renders build, with the
+  checkout is present): the eight the GPU path renders build, with the
   primitive counts their text describes; the others fail either as
   Unsupported or, where the reference's current grammar rejects them too,
   as syntax errors.
@@ -174,6 +174,10 @@ def test_selector_textures():
     159-178) become the material's selector texture; blend of constants by
     a constant folds to a constant like spectrumBlend (Texture.hs)."""
     hdr = "prim { shape { sphere radius 1 } } "
+    t = B.parse("material { transMatte kr { constant rgbR 0.2 0.2 0.2 } kt { constant rgbR 0.5 0.5 0.5 } "
+                "ks { constant 0.3 } } " + hdr).flat.materials[0]
+    assert (t.type, t.scalar[0]) == (abi.MAT_TRANSMATTE, F(0.3))
+    assert list(t.spec[1]) == list(rgb_refl(0.5, 0.5, 0.5))
     b = B.parse("material { matte kd { checker 1 2 3 tex1 { constant rgbR 0.5 0.5 0.5 } "
                 "tex2 { constant rgbR 0.1 0.2 0.3 } } sigma { constant 0 } } " + hdr)
     m = b.flat.materials[0]
@@ -211,6 +215,7 @@ EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit li
     "blackbody-emission.bling": (0, 16, 0, (640, 400)),
     "glass-torus.bling": (0, 4, 0, (640, 360)),
     "cornell-box-underwater.bling": (79210, 3, 0, (720, 720)),
+    "trans-matte.bling": (28, 2, 0, (480, 480)),
 }
 
 

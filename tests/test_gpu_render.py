@@ -527,3 +527,23 @@ def test_heightmap_bling_scene_matches_oracle(gpu_ctx):
     mg, _ = mean_se(g, gsq)
     mo, _ = mean_se(o, osq)
     assert np.median(np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)) < 1e-4
+
+
+def test_translucent_matte_matches_oracle_and_closed_form(gpu_ctx):
+    """transMatte's transmission lobe (brdfToBtdf of a Lambertian): a
+    translucent plane under a surrounding uniform sky has direct radiance
+    (kr + kt (1 - kr)) L; the same seed tracks the oracle."""
+    from bling_amd.scene import Material, const
+    from tests import cases
+    kr, kt = 0.3, 0.7
+    flat = cases.plane_under_sky(sky=1.0, material=Material.trans_matte(const(kr), const(kt), 0.0))
+    spec = JobSpec(spp_nu=8, spp_nv=8, max_depth=1, sample_depth=1, passes=8, seed=5, variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    mg, se = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    y = mg[..., 1].mean()
+    s = np.sqrt((se[..., 1] ** 2).sum()) / mg[..., 1].size
+    want = kr + kt * (1 - kr)
+    assert abs(y - want) < 4 * s + 1e-4, (y, want, s)
+    np.testing.assert_allclose(mg, mo, rtol=1e-4, atol=1e-6)

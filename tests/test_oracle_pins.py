@@ -84,6 +84,16 @@ def test_diffuse_plane_under_uniform_sky():
     _analytic(cases.plane_under_sky(rho=0.5, sky=1.0), 0.5)
 
 
+def test_translucent_plane_under_uniform_sky():
+    """translucentMatte (Material.hs:43-53): a Lambertian reflection of
+    r = kr and the same lobe turned to transmission (brdfToBtdf) with
+    t = kt (1 - kr); under a sky that surrounds the plane the direct radiance
+    is (r + t) L, the transmitted part coming from below."""
+    kr, kt = 0.3, 0.7
+    m = Material.trans_matte(const(kr), const(kt), 0.0)
+    _analytic(cases.plane_under_sky(sky=1.0, material=m), kr + kt * (1 - kr))
+
+
 def test_sphere_light_over_plane():
     """Cone-sampled sphere light (Shape.hs:364-375) with MIS: at the point
     below the light, Lo = rho * Le * (R / D)^2."""
