@@ -139,10 +139,10 @@ struct Engine {
     cfg = config;
     P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // pool layout: 9 int arrays + 52 float arrays of P, plus 5 queues, plus
+    // pool layout: 9 int arrays + 53 float arrays of P, plus 5 queues, plus
     // the direct-lighting branch stacks (1 + kDlWords * dl_depth arrays)
     size_t per = (size_t)P * 4;
-    size_t total = per * (9 + 52 + 5 + 1 + (size_t)kDlWords * cfg.dl_depth);
+    size_t total = per * (9 + 53 + 5 + 1 + (size_t)kDlWords * cfg.dl_depth);
     if (int rc = pool_buf.alloc(c, total)) return rc;
     char* b = pool_buf.as<char>();
     auto take = [&](size_t n) {
@@ -168,7 +168,7 @@ struct Engine {
     pp.rad = (float*)take(3);
     pp.sray = (float*)take(8);
     pp.pend_s = (float*)take(3);
-    pp.mray = (float*)take(7);
+    pp.mray = (float*)take(8);
     pp.pend_m = (float*)take(3);
     pp.q_act = (uint32_t*)take(1);
     pp.q_shadow = (uint32_t*)take(1);

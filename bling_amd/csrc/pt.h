@@ -132,7 +132,7 @@ struct PathPool {
   float* rad;       // 3P radiance as unnormalised XYZ sums
   float* sray;      // 8P shadow ray
   float* pend_s;    // 3P XYZ of the light sample if the shadow ray is unoccluded
-  float* mray;      // 7P MIS ray
+  float* mray;      // 8P MIS ray: ox oy oz dx dy dz tmin tmax
   float* pend_m;    // 3P XYZ of the BSDF sample if the MIS ray reaches the light
   int32_t* mlight;  // light index of the MIS ray
   int32_t* occ;     // shadow ray occluded by a triangle (persistent passes)

@@ -310,12 +310,12 @@ __device__ __forceinline__ void for_uniform_scene(bool valid, int my, F&& body) 
 }
 
 // Traversal for a scene of width W (0: any width, chosen per scene).
-template <int W>
+template <int W, bool ROOT_INF = false>
 __device__ __forceinline__ void pt_closest(const DevScene& sc, const DRay& r, int* lds, int stride,
                                            trav::Best& best) {
   unsigned long long nv = 0, nt = 0;
-  if constexpr (W == 0) trav::closest_hit<false>(sc, r, lds, stride, best, nv, nt);
-  else trav::closest_hit_w<W, false>(sc, r, lds, stride, best, nv, nt);
+  if constexpr (W == 0) trav::closest_hit<false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
+  else trav::closest_hit_w<W, false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
 }
 template <int W>
 __device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, int* lds, int stride) {
@@ -482,11 +482,30 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
           fBlack = fBlack && fv == 0.0f;
           c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
         }
-        if (!fBlack) {
+        // An area light is reached only if the ray hits its shape: the shape
+        // test with tmax = inf (the traversal's own arithmetic, monotone in
+        // tmax) culls rays that miss it, and its hit bounds the traversal,
+        // since nothing beyond the light can be the closest hit.
+        float mt = INFINITY;
+        bool misses = false;
+        if (S.lights[ln].type == LT_AREA) {
+          const DShape& lsh = S.geo.shapes[S.lights[ln].shape];
+          DRay ro;
+          ro.o = xform_point(lsh.w2o, p);
+          ro.d = xform_vector(lsh.w2o, wib);
+          ro.tmin = eps;
+          ro.tmax = INFINITY;
+          float tl;
+          int ax;
+          if (shape_intersect(lsh, ro, &tl, &ax)) mt = nextafterf(tl, INFINITY);
+          else misses = true;
+        }
+        if (!fBlack && !misses) {
           xyz_store(P.pend_m, Pn, i, c);
           P.mray[i] = p.x; P.mray[Pn + i] = p.y; P.mray[2 * Pn + i] = p.z;
           P.mray[3 * Pn + i] = wib.x; P.mray[4 * Pn + i] = wib.y; P.mray[5 * Pn + i] = wib.z;
           P.mray[6 * Pn + i] = eps;
+          P.mray[7 * Pn + i] = mt;
           P.mlight[i] = ln;
           want_mis = true;
         }
@@ -779,9 +798,9 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
       r.o = mk3(P.mray[i], P.mray[Pn + i], P.mray[2 * Pn + i]);
       r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
       r.tmin = P.mray[6 * Pn + i];
-      r.tmax = INFINITY;
+      r.tmax = P.mray[7 * Pn + i];  // bound by the sampled light's hit (the root box sees tmax = inf)
       trav::Best best;
-      pt_closest<W>(sc, r, lds + threadIdx.x, BLOCK, best);
+      pt_closest<W, true>(sc, r, lds + threadIdx.x, BLOCK, best);
       int ln = P.mlight[i];
       const DLight& L = S.lights[ln];
       // pend_m already holds the contribution times the sampled light's
@@ -884,10 +903,12 @@ struct MisSrc {  // closest hit of the MIS ray into hit / hprim (resolved by k_m
     r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
     r.tmin = P.mray[6 * Pn + i];
     r.tmax = INFINITY;
+    const bool root = slab_haskell(g.root_lo, g.root_hi, r);  // the reference's ray: tmax = inf
+    r.tmax = P.mray[7 * Pn + i];  // bound by the sampled light's hit
     ls = lane_scene(g, FAT);
     best = trav::Best{r.tmax, 0.0f, 0.0f, -1};
     if (FAT && P.hprim[i] >= 0) best = trav::Best{P.hit[i], P.hit[Pn + i], P.hit[2 * Pn + i], P.hprim[i]};
-    return slab_haskell(g.root_lo, g.root_hi, r);
+    return root;
   }
   __device__ __forceinline__ void end(int k, const DRay&, const trav::LaneScene&, const trav::Best& best) {
     if (FAT && best.prim < 0) return;

@@ -279,7 +279,7 @@ __device__ __forceinline__ bool any_w(const DevScene& sc, const DRay& r, f3 inv,
 // Closest hit over both trees (triangles, then shapes), after the root-box
 // test of the accelerator primitive (Primitive/KdTree.hs:240), for a scene of
 // width W.
-template <int W, bool STATS>
+template <int W, bool STATS, bool ROOT_INF = false>
 __device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
                                               Best& best, unsigned long long& nvisit,
                                               unsigned long long& ntest) {
@@ -287,7 +287,13 @@ __device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r,
   best.b1 = 0.0f;
   best.b2 = 0.0f;
   best.prim = -1;
-  if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return;
+  if (ROOT_INF) {  // the root box test of the unbounded ray; r.tmax only bounds the traversal
+    DRay ru = r;
+    ru.tmax = INFINITY;
+    if (!slab_haskell(sc.root_lo, sc.root_hi, ru)) return;
+  } else if (!slab_haskell(sc.root_lo, sc.root_hi, r)) {
+    return;
+  }
   f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
   f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
   int spill_n[kSpill];
@@ -320,13 +326,13 @@ __device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int
 // Width-dispatching forms for kernels whose rays may come from scenes of
 // different widths; the width is uniform per scene, so the branch is uniform
 // per wave where the scene is.
-template <bool STATS>
+template <bool STATS, bool ROOT_INF = false>
 __device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
                                             Best& best, unsigned long long& nvisit,
                                             unsigned long long& ntest) {
-  if (sc.width == 8) closest_hit_w<8, STATS>(sc, r, lds, stride, best, nvisit, ntest);
-  else if (sc.width == 4) closest_hit_w<4, STATS>(sc, r, lds, stride, best, nvisit, ntest);
-  else closest_hit_w<2, STATS>(sc, r, lds, stride, best, nvisit, ntest);
+  if (sc.width == 8) closest_hit_w<8, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
+  else if (sc.width == 4) closest_hit_w<4, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
+  else closest_hit_w<2, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
 }
 template <bool STATS>
 __device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
