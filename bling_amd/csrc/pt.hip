@@ -317,11 +317,11 @@ __device__ __forceinline__ void pt_closest(const DevScene& sc, const DRay& r, in
   if constexpr (W == 0) trav::closest_hit<false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
   else trav::closest_hit_w<W, false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
 }
-template <int W>
+template <int W, bool EXACT = false>
 __device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, int* lds, int stride) {
   unsigned long long nv = 0, nt = 0;
-  if constexpr (W == 0) return trav::any_hit<false>(sc, r, lds, stride, nv, nt);
-  else return trav::any_hit_w<W, false>(sc, r, lds, stride, nv, nt);
+  if constexpr (W == 0) return trav::any_hit<false, EXACT>(sc, r, lds, stride, nv, nt);
+  else return trav::any_hit_w<W, false, EXACT>(sc, r, lds, stride, nv, nt);
 }
 
 template <int BLOCK, int W>
@@ -799,10 +799,17 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
       r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
       r.tmin = P.mray[6 * Pn + i];
       r.tmax = P.mray[7 * Pn + i];  // bound by the sampled light's hit (the root box sees tmax = inf)
-      trav::Best best;
-      pt_closest<W, true>(sc, r, lds + threadIdx.x, BLOCK, best);
       int ln = P.mlight[i];
       const DLight& L = S.lights[ln];
+      trav::Best best;
+      best.prim = -1;
+      if (L.type == LT_INF) {
+        // an infinite light is reached iff nothing is hit: any hit, with the
+        // closest-hit query's shape tests, decides it (tmax is inf here)
+        if (pt_any<W, true>(sc, r, lds + threadIdx.x, BLOCK)) best.prim = 0;
+      } else {
+        pt_closest<W, true>(sc, r, lds + threadIdx.x, BLOCK, best);
+      }
       // pend_m already holds the contribution times the sampled light's
       // emission; here only whether the ray reaches that light is decided
       bool reaches = false;

@@ -160,7 +160,9 @@ __device__ __forceinline__ void sort_children(float (&k)[W], int (&l)[W]) {
 }
 
 // Traverses one tree.  CLOSEST: shrinks best.t; ANY: returns at the first hit.
-template <int W, bool ANY, bool FAT, bool STATS>
+// EXACT (with ANY): shapes are tested with `intersect` rather than
+// `intersects`, so that "any hit" is exactly "a closest hit exists".
+template <int W, bool ANY, bool FAT, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRay& r, f3 inv,
                                          f3 oi, Stack& st, Best& best, unsigned long long& nvisit,
                                          unsigned long long& ntest) {
@@ -224,7 +226,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
           ro.d = xform_vector(s.w2o, r.d);
           ro.tmin = r.tmin;
           ro.tmax = ANY ? r.tmax : best.t;
-          if (ANY) {
+          if (ANY && !EXACT) {
             hit = shape_intersects(s, ro);
           } else {
             int ax;
@@ -265,13 +267,15 @@ __device__ __forceinline__ void closest_w(const DevScene& sc, const DRay& r, f3 
   if (sc.tri_root >= 0) traverse<W, false, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest);
   if (sc.shape_root >= 0) traverse<W, false, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest);
 }
-template <int W, bool STATS>
+template <int W, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool any_w(const DevScene& sc, const DRay& r, f3 inv, f3 oi, Stack& st,
                                       unsigned long long& nvisit, unsigned long long& ntest) {
   Best best{r.tmax, 0.0f, 0.0f, -1};
-  if (sc.tri_root >= 0 && traverse<W, true, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest))
+  if (sc.tri_root >= 0 &&
+      traverse<W, true, false, STATS, EXACT>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest))
     return true;
-  if (sc.shape_root >= 0 && traverse<W, true, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest))
+  if (sc.shape_root >= 0 &&
+      traverse<W, true, true, STATS, EXACT>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest))
     return true;
   return false;
 }
@@ -306,8 +310,9 @@ __device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r,
   closest_w<W, STATS>(sc, r, inv, oi, st, best, nvisit, ntest);
 }
 
-// Any hit (Primitive/KdTree.hs:244-246) for a scene of width W.
-template <int W, bool STATS>
+// Any hit (Primitive/KdTree.hs:244-246) for a scene of width W; EXACT:
+// whether `intersect` (KdTree.hs:240-242) would find a hit.
+template <int W, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
                                           unsigned long long& nvisit, unsigned long long& ntest) {
   if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return false;
@@ -320,7 +325,7 @@ __device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int
   st.stride = stride;
   st.spill_n = spill_n;
   st.spill_t = spill_t;
-  return any_w<W, STATS>(sc, r, inv, oi, st, nvisit, ntest);
+  return any_w<W, STATS, EXACT>(sc, r, inv, oi, st, nvisit, ntest);
 }
 
 // Width-dispatching forms for kernels whose rays may come from scenes of
@@ -334,12 +339,12 @@ __device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, i
   else if (sc.width == 4) closest_hit_w<4, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
   else closest_hit_w<2, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
 }
-template <bool STATS>
+template <bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
                                         unsigned long long& nvisit, unsigned long long& ntest) {
-  if (sc.width == 8) return any_hit_w<8, STATS>(sc, r, lds, stride, nvisit, ntest);
-  if (sc.width == 4) return any_hit_w<4, STATS>(sc, r, lds, stride, nvisit, ntest);
-  return any_hit_w<2, STATS>(sc, r, lds, stride, nvisit, ntest);
+  if (sc.width == 8) return any_hit_w<8, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
+  if (sc.width == 4) return any_hit_w<4, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
+  return any_hit_w<2, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
 }
 
 }  // namespace trav
