@@ -7,6 +7,21 @@
 //   Montecarlo.hs:40-186                     Camera.hs:49-68   Image.hs:40-61,162-168,250-299
 //   DifferentialGeometry.hs:41-81            Primitive/TriangleMesh.hs:202-287
 //   Rendering.hs:207-246 (passes over the sample extent)
+//   Integrator/DirectLighting.hs:23-63       Material.hs:43-53 (translucentMatte)
+//   Texture.hs:191-219 (checkerBoard, graphPaper selectors)
+//
+// Pins (tests/test_oracle_pins.py):
+//   path integrator, NEE + MIS   rho L under a uniform sky; rho Le (R/D)^2 below a
+//                                sphere light; sky-only renders return the sky exactly
+//   directLighting               the same closed forms, zero on escapes, and
+//                                r Le w / r Le (w + 1) through a mirror (maxDepth 1 / >= 2)
+//   translucentMatte             (kr + kt (1 - kr)) L inside a uniform sky
+//   texture selectors            closed-form points of checkerBoard and graphPaper
+//   sampler                      SplitMix64 known answers, permutation bijectivity,
+//                                stratification of every table dimension
+//   film, camera                 job views equal scenes built at that size
+// Parity unpinned: the bits of glibc sinf/cosf/acosf/atan2f/powf/expf as GHC
+// calls them (they move sampled directions, compared statistically).
 #include <array>
 #include <atomic>
 #include <cmath>
