@@ -519,6 +519,8 @@ class _Parser:
         elif t == "heightMap":  # PrimitiveParser.hs:39-45: its own transform block
             ns = self.integ()
             nt = self.integ()
+            if ns < 2 or nt < 2:  # the lattice divides by ns - 1 and nt - 1
+                raise Unsupported("heightMap with fewer than 2 x 2 samples")
             elev = self.scalar_map_2d()
             xf = self.transform_block()
             verts, faces, norms, uvs = heightmap_mesh(elev, ns, nt)
