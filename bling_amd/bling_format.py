@@ -466,7 +466,17 @@ class _Parser:
             kt = self.spectrum_texture("kt")
             ks = self.scalar_texture("ks")
             return self.textured(lambda s: Material.trans_matte(s[0], s[1], ks), [kr, kt])
-        if t in ("bumpMap", "substrate", "shinyMetal"):
+        if t == "substrate":  # pSubstrateMaterial (MaterialParser.hs:50-58)
+            kd = self.spectrum_texture("kd")
+            ks = self.spectrum_texture("ks")
+            ka = self.spectrum_texture("ka")
+            urough = self.scalar_texture("urough")
+            vrough = self.scalar_texture("vrough")
+            depth = self.scalar_texture("depth")
+            if isinstance(ka, tuple):
+                raise Unsupported("substrate with a textured ka (selectors cover kd and ks)")
+            return self.textured(lambda s: Material.substrate(s[0], s[1], ka, urough, vrough, depth), [kd, ks])
+        if t in ("bumpMap", "shinyMetal"):
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)
 

@@ -17,9 +17,12 @@ constexpr int kMaxScenes = 64;   // scenes referenced by the engine
 
 // BxDF flags and kinds (Reflection.hs:94-108, Reflection/*.hs)
 enum { BX_R = 1, BX_T = 2, BX_D = 4, BX_G = 8, BX_S = 16 };
-enum { K_LAMBERT = 0, K_OREN = 1, K_MF = 2, K_SPECR = 3, K_SPECT = 4 };
+enum { K_LAMBERT = 0, K_OREN = 1, K_MF = 2, K_SPECR = 3, K_SPECT = 4, K_FB = 5 };
 enum { FR_NOOP = 0, FR_DIEL = 1, FR_COND = 2 };
 
+// K_FB (FresnelBlend over the anisotropic distribution) reuses the fields:
+// r = rd (diffuse), eta = rs (specular), k = ra (absorption), e = ex,
+// ei = ey, et = coating depth.  It is always a material's only component.
 struct DBxdf {
   int32_t kind, type, fr;
   int32_t flip;  // brdfToBtdf: the diffuse lobe with wi mirrored to the other hemisphere

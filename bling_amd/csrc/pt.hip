@@ -354,9 +354,9 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
 }
 
 // Shading class of active path i: 0 escaped, 1 at the depth cutoff, 2 no
-// BxDF, then one class per BxDF makeup of the hit's material (the kind of
-// its first component, or of its two components), so that grouped waves run
-// one shading code path.
+// BxDF, then one class per BxDF makeup of the hit's material: 3 + kind for
+// one component (kinds 0..5), 9 + 5 kind0 + kind1 for two (kinds 0..4), so
+// that grouped waves run one shading code path.
 __device__ __forceinline__ int shade_class(const PtKernelArgs& a, int i) {
   const PathPool& P = a.pool;
   const int prim = P.hprim[i];
@@ -367,7 +367,7 @@ __device__ __forceinline__ int shade_class(const PtKernelArgs& a, int i) {
   const int m = prim < S.ntri ? S.tri_info[prim].x : S.geo.shapes[prim - S.ntri].material;
   if (m < 0 || m >= S.nmats || S.mats[m].n == 0) return 2;
   const DMaterial& M = S.mats[m];
-  const int c = M.n == 1 ? 3 + M.c[0].kind : 8 + 5 * M.c[0].kind + M.c[1].kind;
+  const int c = M.n == 1 ? 3 + M.c[0].kind : 9 + 5 * M.c[0].kind + M.c[1].kind;
   return c < kShadeClasses ? c : kShadeClasses - 1;
 }
 

@@ -401,7 +401,10 @@ enum {
   T_ROS = 1,  // (r[b] * a) * c                (Oren-Nayar eval (r*o)*s)
   T_RFX = 2,  // (r[b] * c) * a                (r times a scalar Fresnel c, then scaled)
   T_RCX = 3,  // (r[b] * Fcond_b(c)) * a       (r times the conductor Fresnel at cosine c)
-  T_TRN = 4   // (a * r[b]) * c                (specular transmission ((1-fr)*r)*eta2)
+  T_TRN = 4,  // (a * r[b]) * c                (specular transmission ((1-fr)*r)*eta2)
+  // FresnelBlend's two terms, expanded only by fb_band (value mode V_FB):
+  T_FBD = 5,  // ((A_b * rd[b]) * (1 - rs[b])) * a, A_b = exp(ra[b] * c) under a coating, else 1
+  T_FBS = 6   // (rs[b] + (1 - rs[b]) * c) * a   (the Schlick-weighted specular)
 };
 struct Term {
   const DBxdf* bx;
@@ -411,7 +414,10 @@ struct Term {
 // value_b = V_SUM:   (0 + t0) + t1            (terms present: n)
 //           V_SCALE: t0 * k1
 //           V_MIX:   (t0 * k1 + fo) * k2      fo = n > 1 ? 0 + t1 : 0
-enum { V_SUM = 0, V_SCALE = 1, V_MIX = 2 };
+//           V_FB:    (t0 + t1) * k1           FresnelBlend's diffuse + specular terms
+// term_at knows ops 0-4 only; V_FB keeps FresnelBlend out of the per-band
+// dispatch that every other material runs.
+enum { V_SUM = 0, V_SCALE = 1, V_MIX = 2, V_FB = 3 };
 struct SpecVal {
   Term t[2];
   int n, mode;
@@ -447,6 +453,13 @@ __device__ __forceinline__ float spec_at(const SpecVal& v, int b) {
     return f;
   }
   if (v.mode == V_SCALE) return term_at(v.t[0], b) * v.k1;
+  if (v.mode == V_FB) {  // t0: T_FBD (a = diffuse scale, c = absorption exponent), t1: T_FBS
+    const DBxdf& x = *v.t[0].bx;
+    const float ab = x.et > 0.0f ? expf(x.k[b] * v.t[0].c) : 1.0f;
+    const float diff = ((ab * x.r[b]) * (1.0f - x.eta[b])) * v.t[0].a;
+    const float spec = (x.eta[b] + (1.0f - x.eta[b]) * v.t[1].c) * v.t[1].a;
+    return (diff + spec) * v.k1;
+  }
   float fo = v.n > 1 ? 0.0f + term_at(v.t[1], b) : 0.0f;
   return (term_at(v.t[0], b) * v.k1 + fo) * v.k2;
 }
@@ -467,6 +480,88 @@ __device__ __forceinline__ Term rf_term(const DBxdf& b, float cosi, float a) {
 __device__ __forceinline__ float mf_g(f3 wo, f3 wi, f3 wh) {
   float nh = acos_t(wh), no = acos_t(wo), ni = acos_t(wi), oh = fabsf(dot3(wo, wh));
   return hmin(1.0f, hmin(2.0f * nh * no / oh, 2.0f * nh * ni / oh));
+}
+// The anisotropic microfacet distribution with exponents ex, ey
+// (Microfacet.hs:136-196): D, its sampling pdf, and the sampler.
+__device__ __forceinline__ float aniso_exp(float ex, float ey, f3 wh, float den) {
+  return (ex * wh.x * wh.x + ey * wh.y * wh.y) / den;
+}
+__device__ __forceinline__ float aniso_d(float ex, float ey, f3 wh) {
+  const float costh = acos_t(wh), d = 1.0f - costh * costh;
+  if (d == 0.0f) return 0.0f;
+  return __builtin_sqrtf((ex + 2.0f) * (ey + 2.0f)) * kInvTwoPi * powf(costh, aniso_exp(ex, ey, wh, d));
+}
+__device__ __forceinline__ float aniso_pdf(float ex, float ey, f3 wh) {
+  const float costh = acos_t(wh);
+  const float e = aniso_exp(ex, ey, wh, hmax(0.0f, 1.0f - costh * costh));
+  return __builtin_sqrtf((ex + 1.0f) * (ey + 1.0f)) * kInvTwoPi * powf(costh, e);
+}
+// The azimuth is drawn in the first quadrant and mirrored into the quadrant
+// u1 selects; the returned pdf is aniso_pdf at the returned half vector.
+__device__ __forceinline__ f3 aniso_sample(float ex, float ey, float u1, float u2, float* pdf) {
+  float up;
+  int q;
+  if (u1 < 0.25f) {
+    up = 4.0f * u1, q = 0;
+  } else if (u1 < 0.5f) {
+    up = 4.0f * (0.5f - u1), q = 1;
+  } else if (u1 < 0.75f) {
+    up = 4.0f * (u1 - 0.5f), q = 2;
+  } else {
+    up = 4.0f * (1.0f - u1), q = 3;
+  }
+  const float p = ex == ey ? kPi * up * 0.5f
+                           : atanf(__builtin_sqrtf((ex + 1.0f) / (ey + 1.0f)) * tanf(kPi * up * 0.5f));
+  const float cp = cosf(p), sp = sinf(p);
+  const float cost = powf(u2, 1.0f / (ex * cp * cp + ey * sp * sp + 1.0f));
+  const float phi = q == 0 ? p : (q == 1 ? kPi - p : (q == 2 ? p + kPi : kTwoPi - p));
+  const float sint = __builtin_sqrtf(hmax(0.0f, 1.0f - cost * cost));
+  const f3 wh = mk3(sint * cosf(phi), sint * sinf(phi), cost);
+  const float f = kInvTwoPi * powf(cost, aniso_exp(ex, ey, wh, 1.0f - cost * cost));
+  *pdf = __builtin_sqrtf((ex + 1.0f) * (ey + 1.0f)) * f;
+  return wh;
+}
+// FresnelBlend (Microfacet.hs:56-104), a material's only component: e wo wi
+// as its diffuse and specular terms, its pdf, and its sample with adj False
+// (f = e wo wi / pdf; false when the distribution's pdf is zero).
+__device__ __forceinline__ f3 fb_half(f3 wi, f3 wo) {
+  const f3 wh = norm3(add3(wi, wo));
+  return wh.z < 0.0f ? neg3(wh) : wh;
+}
+__device__ __forceinline__ void fb_terms(const DBxdf& b, f3 wo, f3 wi, Term* td, Term* ts) {
+  const float costi = acos_t(wi), costo = acos_t(wo);
+  const float ka = -b.et * (costi + costo) / (costi * costo);
+  const float dd = (costo * 28.0f / 23.0f * kPi) * (1.0f - powf(1.0f - 0.5f * costi, 5.0f)) *
+                   (1.0f - powf(1.0f - 0.5f * costo, 5.0f));
+  const f3 wh = fb_half(wi, wo);
+  const float costih = fabsf(dot3(wi, wh));
+  const float ds = aniso_d(b.e, b.ei, wh) * costo / (4.0f * costih * hmax(costi, costo));
+  *td = Term{&b, T_FBD, dd, ka};
+  *ts = Term{&b, T_FBS, ds, powf(1.0f - costih, 5.0f)};
+}
+__device__ __forceinline__ float fb_pdf(const DBxdf& b, f3 wo, f3 wi) {
+  if (!same_hemi(wo, wi)) return 0.0f;
+  const f3 wh = fb_half(wi, wo);
+  return 0.5f * (acos_t(wi) * kInvPi + aniso_pdf(b.e, b.ei, wh) / (4.0f * fabsf(dot3(wo, wh))));
+}
+__device__ __forceinline__ bool fb_sample(const DBxdf& b, f3 wo, float u1, float u2, Term* td, Term* ts, f3* wi,
+                                          float* pdf) {
+  f3 w, wh;
+  float pdfp;
+  if (u1 < 0.5f) {
+    w = cos_hemisphere(u1 * 2.0f, u2);
+    if (wo.z < 0.0f) w.z = -w.z;  // toSameHemisphere
+    wh = fb_half(w, wo);
+    pdfp = aniso_pdf(b.e, b.ei, wh);
+  } else {
+    wh = aniso_sample(b.e, b.ei, 2.0f * (u1 - 0.5f), u2, &pdfp);
+    w = sub3(sscale3(2.0f * dot3(wo, wh), wh), wo);
+  }
+  *wi = w;
+  if (pdfp == 0.0f) return false;
+  *pdf = 0.5f * (acos_t(w) * kInvPi + pdfp / (4.0f * fabsf(dot3(wo, wh))));
+  fb_terms(b, wo, w, td, ts);
+  return true;
 }
 __device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // Diffuse.hs:431-443
   float sinti = sin_t(wi), sinto = sin_t(wo);
@@ -518,6 +613,7 @@ __device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term*
 __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
   if (b.flip) wi.z = -wi.z;
   if (b.kind == K_LAMBERT || b.kind == K_OREN) return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
+  if (b.kind == K_FB) return fb_pdf(b, wo, wi);
   if (b.kind == K_MF) {
     f3 whp = add3(wo, wi);
     if (sqlen3(whp) == 0.0f) return 0.0f;
@@ -634,8 +730,18 @@ __device__ __forceinline__ SpecVal eval_bsdf(const Bsdf& b, f3 woW, f3 wiW) {  /
   int need = side < 0.0f ? BX_T : BX_R;
   f3 wo = to_local(b.cs, woW), wi = to_local(b.cs, wiW);
   int n = bsdf_n(b);
-  for (int i = 0; i < n; ++i)
-    if ((b.m->c[i].type & need) && bx_eval_term(b.m->c[i], wi, wo, &f.t[f.n])) f.n++;
+  for (int i = 0; i < n; ++i) {
+    const DBxdf& c = b.m->c[i];
+    if (!(c.type & need)) continue;
+    if (c.kind == K_FB) {  // the only component: 0 + (diff + spec) = (diff + spec) * 1
+      fb_terms(c, wi, wo, &f.t[0], &f.t[1]);
+      f.n = 2;
+      f.mode = V_FB;
+      f.k1 = 1.0f;
+    } else if (bx_eval_term(c, wi, wo, &f.t[f.n])) {
+      f.n++;
+    }
+  }
   return f;
 }
 __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:251-257
@@ -660,6 +766,20 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
   sNum = sNum < 0 ? 0 : sNum;
   const DBxdf& bx = b.m->c[sNum];
+  if (bx.kind == K_FB) {  // the only component, a reflection: f = e wo wi / pdf = (t0 + t1) * (1 / pdf)
+    f3 wi;
+    float pdfp;
+    if (!fb_sample(bx, wo, u1, u2, &f.t[0], &f.t[1], &wi, &pdfp)) return 0.0f;
+    const f3 ww = to_world(b.cs, wi);
+    const float side = dot3(ww, b.ng) / dot3(woW, b.ng);
+    if (side == 0.0f || side < 0.0f) return 0.0f;
+    *wiW = ww;
+    *type = bx.type;
+    f.n = 2;
+    f.mode = V_FB;
+    f.k1 = 1.0f / pdfp;
+    return pdfp;
+  }
   Term fs;
   bool fzero;
   f3 wi;

@@ -237,6 +237,21 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         o.c[o.n++] = lobe(t, true);
         break;
       }
+      case BGPU_MAT_SUBSTRATE: {  // mkSubstrate (Material.hs:111-127)
+        DBxdf b = bx_zero();
+        b.kind = K_FB;
+        b.type = BX_R | BX_G;
+        for (int k = 0; k < 16; ++k) {  // sClamp 0 1
+          b.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
+          b.eta[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k]));
+          b.k[k] = hmaxf(0.0f, hminf(1.0f, m.spec[2][k]));
+        }
+        b.e = blinn_e(hmaxf(0.0f, m.scalar[0]));  // mkAnisotropic (1 / u) (1 / v), u = max 0 urough
+        b.ei = blinn_e(hmaxf(0.0f, m.scalar[1]));
+        b.et = m.scalar[2];
+        o.c[o.n++] = b;
+        break;
+      }
       case BGPU_MAT_MIRROR: {
         DBxdf r = bx_zero();
         r.kind = K_SPECR;

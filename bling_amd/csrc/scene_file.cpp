@@ -26,7 +26,7 @@ namespace bgpu {
 namespace {
 
 constexpr char kMagic[8] = {'B', 'G', 'P', 'U', 'S', 'C', 'N', '1'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: materials carry three spectra and four scalars
 constexpr uint32_t kByteOrder = 0x01020304u;
 constexpr uint32_t kHasNormals = 1, kHasUvs = 2, kHasTriMesh = 4;
 constexpr uint64_t kAlign = 64;

@@ -115,11 +115,12 @@ def xyz_to_rgb(xyz):
 class Material:
     """Material parameters in the descriptor's layout (bgpu_material)."""
 
-    def __init__(self, kind, spec0=None, spec1=None, s0=0.0, s1=0.0):
+    def __init__(self, kind, spec0=None, spec1=None, s0=0.0, s1=0.0, spec2=None, s2=0.0):
         self.kind = kind
         self.spec0 = const(0) if spec0 is None else np.asarray(spec0, F)
         self.spec1 = const(0) if spec1 is None else np.asarray(spec1, F)
-        self.s0, self.s1 = F(s0), F(s1)
+        self.spec2 = const(0) if spec2 is None else np.asarray(spec2, F)
+        self.s0, self.s1, self.s2 = F(s0), F(s1), F(s2)
         # selector texture (bgpu_texture): none by default
         self.tex_kind, self.tex_map, self.tex_slots = abi.TEX_NONE, abi.MAP_UV, 0
         self.tex_params = [0.0] * 12
@@ -130,7 +131,7 @@ class Material:
         (TEX_CHECKER: params sx sy sz; TEX_GRAPHPAPER: lw and the mapping's
         parameters) selects per hit between the slot's spectrum (tex1) and
         `alt` (tex2)."""
-        m = Material(self.kind, self.spec0, self.spec1, self.s0, self.s1)
+        m = Material(self.kind, self.spec0, self.spec1, self.s0, self.s1, self.spec2, self.s2)
         m.tex_kind, m.tex_map, m.tex_slots = kind, mapping, 1 << slot
         m.tex_params = [float(F(x)) for x in params] + [0.0] * (12 - len(params))
         m.alt = [self.spec0.copy(), self.spec1.copy()]
@@ -161,6 +162,13 @@ class Material:
     def trans_matte(kr, kt, sigma=0.0):
         """translucentMatte kr kt sigma (Material.hs:43-53)."""
         return Material(abi.MAT_TRANSMATTE, kr, kt, sigma)
+
+    @staticmethod
+    def substrate(kd, ks, ka, urough, vrough, depth=0.0):
+        """mkSubstrate kd ks ka urough vrough depth (Material.hs:111-127): a
+        FresnelBlend coating over a diffuse base with an anisotropic
+        microfacet distribution of exponents 1/urough and 1/vrough."""
+        return Material(abi.MAT_SUBSTRATE, kd, ks, urough, vrough, ka, depth)
 
     @staticmethod
     def black():
@@ -346,9 +354,11 @@ class FlatScene:
             self.materials[i].type = m.kind
             self.materials[i].scalar[0] = m.s0
             self.materials[i].scalar[1] = m.s1
+            self.materials[i].scalar[2] = m.s2
             for k in range(16):
                 self.materials[i].spec[0][k] = m.spec0[k]
                 self.materials[i].spec[1][k] = m.spec1[k]
+                self.materials[i].spec[2][k] = m.spec2[k]
             t = self.materials[i].tex
             t.kind, t.mapping, t.slots = m.tex_kind, m.tex_map, m.tex_slots
             for k in range(12):

@@ -86,9 +86,13 @@ enum {
   BGPU_MAT_METAL = 3,   /* spec[0]=eta, spec[1]=k, scalar[0]=roughness */
   BGPU_MAT_GLASS = 4,   /* spec[0]=reflect, spec[1]=transmit (both clamped to [0,1]), scalar[0]=ior */
   BGPU_MAT_MIRROR = 5,  /* spec[0]=reflect (clamped to [0,1]) */
-  BGPU_MAT_TRANSMATTE = 6 /* translucentMatte (Material.hs:43-53): spec[0]=kr, spec[1]=kt (both clamped
-                             to [0,1], kt then scaled by 1 - kr), scalar[0]=sigma; a diffuse reflection and
-                             the same lobe turned into transmission (brdfToBtdf, Reflection.hs:188-195) */
+  BGPU_MAT_TRANSMATTE = 6, /* translucentMatte (Material.hs:43-53): spec[0]=kr, spec[1]=kt (both clamped
+                              to [0,1], kt then scaled by 1 - kr), scalar[0]=sigma; a diffuse reflection and
+                              the same lobe turned into transmission (brdfToBtdf, Reflection.hs:188-195) */
+  BGPU_MAT_SUBSTRATE = 7   /* mkSubstrate (Material.hs:111-127): spec[0]=kd, spec[1]=ks, spec[2]=ka (each
+                              clamped to [0,1]), scalar[0]=urough, scalar[1]=vrough, scalar[2]=depth; one
+                              FresnelBlend BRDF over the anisotropic distribution with exponents
+                              1/max(0, urough) and 1/max(0, vrough) (Microfacet.hs:56-104,136-196) */
 };
 /* A texture that selects, per hit, between two constant spectra
  * (Texture.hs:191-219): spec[k] of the material is tex1's value and
@@ -115,9 +119,9 @@ typedef struct {
 typedef struct {
   int32_t type;
   int32_t pad;
-  float scalar[2];
-  float spec[2][16];
-  bgpu_texture tex; /* kind BGPU_TEX_NONE: constant spectra */
+  float scalar[4];
+  float spec[3][16];
+  bgpu_texture tex; /* kind BGPU_TEX_NONE: constant spectra; selectors cover spec[0] and spec[1] */
 } bgpu_material;
 
 /* A triangle mesh made by mkTriangleMesh (Primitive/TriangleMesh.hs:119-140). */

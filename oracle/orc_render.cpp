@@ -566,7 +566,7 @@ static Spec lightLe(const LightRec& l, const Ray& r) {  // Light.hs:98-106
 
 // -------------------------------------------------------------------- BSDF
 enum { BX_R = 1, BX_T = 2, BX_D = 4, BX_G = 8, BX_S = 16 };
-enum { K_LAMBERT, K_OREN, K_MF, K_SPECR, K_SPECT };
+enum { K_LAMBERT, K_OREN, K_MF, K_SPECR, K_SPECT, K_FB };
 enum { FR_NOOP, FR_DIEL, FR_COND };
 
 struct Bxdf {
@@ -578,6 +578,9 @@ struct Bxdf {
   float ei, et; // dielectric Fresnel / transmission etas
   Spec eta, k;  // conductor
   int flip;     // brdfToBtdf (Reflection.hs:188-195): wi mirrored to the other hemisphere
+  // FresnelBlend (Microfacet.hs:56-104): r is the diffuse layer rd
+  Spec rs, ra;              // specular reflectance, absorption
+  float ex, ey, depth;      // anisotropic exponents, coating thickness
 };
 
 struct Bsdf {
@@ -640,6 +643,77 @@ static float mfG(V3 wo, V3 wi, V3 wh) {  // Microfacet.hs:113-120
 static float blinnD(float e, V3 wh) { return (e + 2.0f) * kInvTwoPi * std::pow(absCosT(wh), e); }
 static float blinnPdf(float e, V3 wh) { return (e + 1.0f) * std::pow(absCosT(wh), e) * kInvTwoPi; }
 
+// The anisotropic distribution (Microfacet.hs:136-196).
+static float anisoExponent(float ex, float ey, V3 wh, float den) {
+  return (ex * wh.x * wh.x + ey * wh.y * wh.y) / den;
+}
+static float anisoD(float ex, float ey, V3 wh) {  // mfDistD
+  float costh = absCosT(wh);
+  float d = 1.0f - costh * costh;
+  if (d == 0.0f) return 0.0f;
+  return std::sqrt((ex + 2.0f) * (ey + 2.0f)) * kInvTwoPi * std::pow(costh, anisoExponent(ex, ey, wh, d));
+}
+static float anisoPdf(float ex, float ey, V3 wh) {  // mfDistPdf
+  float costh = absCosT(wh);
+  float e = anisoExponent(ex, ey, wh, hmax(0.0f, 1.0f - costh * costh));
+  return std::sqrt((ex + 1.0f) * (ey + 1.0f)) * kInvTwoPi * std::pow(costh, e);
+}
+static V3 anisoSample(float ex, float ey, float u1, float u2, float* dOut, float* pdfOut) {  // mfDistSample
+  // smpFirstQuadrand, mirrored into the quadrant that u1 selects
+  auto first = [&](float u, float* p, float* c) {
+    *p = ex == ey ? kPi * u * 0.5f : std::atan(std::sqrt((ex + 1.0f) / (ey + 1.0f)) * std::tan(kPi * u * 0.5f));
+    float cp = std::cos(*p), sp = std::sin(*p);
+    *c = std::pow(u2, 1.0f / (ex * cp * cp + ey * sp * sp + 1.0f));
+  };
+  float p, cost, phi;
+  if (u1 < 0.25f) {
+    first(4.0f * u1, &p, &cost);
+    phi = p;
+  } else if (u1 < 0.5f) {
+    first(4.0f * (0.5f - u1), &p, &cost);
+    phi = kPi - p;
+  } else if (u1 < 0.75f) {
+    first(4.0f * (u1 - 0.5f), &p, &cost);
+    phi = p + kPi;
+  } else {
+    first(4.0f * (1.0f - u1), &p, &cost);
+    phi = kTwoPi - p;
+  }
+  float sint = std::sqrt(hmax(0.0f, 1.0f - cost * cost));
+  V3 wh = mk(sint * std::cos(phi), sint * std::sin(phi), cost);
+  float f = kInvTwoPi * std::pow(cost, anisoExponent(ex, ey, wh, 1.0f - cost * cost));
+  *dOut = std::sqrt((ex + 2.0f) * (ey + 2.0f)) * f;
+  *pdfOut = std::sqrt((ex + 1.0f) * (ey + 1.0f)) * f;
+  return wh;
+}
+// FresnelBlend's half vector: normalize (wi + wo), turned into the upper hemisphere
+static V3 fbHalf(V3 wi, V3 wo) {
+  V3 wh = normalize(wi + wo);
+  return wh.z < 0.0f ? -wh : wh;
+}
+static Spec fbEval(const Bxdf& b, V3 wo, V3 wi) {  // e (Microfacet.hs:65-86)
+  float costi = absCosT(wi), costo = absCosT(wo);
+  Spec white = sconst(1.0f);
+  Spec a = white;
+  if (b.depth > 0.0f) {  // absorption in the coating
+    Spec x = sscale(b.ra, -b.depth * (costi + costo) / (costi * costo));
+    for (int k = 0; k < 16; ++k) a[k] = std::exp(x[k]);
+  }
+  Spec diff = sscale(a * b.r * (white - b.rs), (costo * 28.0f / 23.0f * kPi) *
+                                                   (1.0f - std::pow(1.0f - 0.5f * costi, 5.0f)) *
+                                                   (1.0f - std::pow(1.0f - 0.5f * costo, 5.0f)));
+  V3 wh = fbHalf(wi, wo);
+  float costih = absDot(wi, wh);
+  Spec schlick = b.rs + sscale(white - b.rs, std::pow(1.0f - costih, 5.0f));
+  Spec spec = sscale(schlick, anisoD(b.ex, b.ey, wh) * costo / (4.0f * costih * hmax(costi, costo)));
+  return diff + spec;
+}
+static float fbPdf(const Bxdf& b, V3 wo, V3 wi) {  // p (Microfacet.hs:101-104)
+  if (!sameHemi(wo, wi)) return 0.0f;
+  V3 wh = fbHalf(wi, wo);
+  return 0.5f * (absCosT(wi) * kInvPi + anisoPdf(b.ex, b.ey, wh) / (4.0f * absDot(wo, wh)));
+}
+
 static Spec orenNayar(const Bxdf& b, V3 wo, V3 wi) {  // Diffuse.hs:431-443
   float sinti = sinT(wi), sinto = sinT(wo);
   float maxcos = 0.0f;
@@ -667,6 +741,7 @@ static Spec bxEval(const Bxdf& b, V3 wo, V3 wi) {
       float x = blinnD(b.e, wh) * mfG(wo, wi, wh) / (4.0f * costi);
       return sscale(b.r * fresnel(b, costh), x);
     }
+    case K_FB: return fbEval(b, wo, wi);
     default: return sconst(0);
   }
 }
@@ -682,6 +757,7 @@ static float bxPdf(const Bxdf& b, V3 wo, V3 wi) {
       if (cosT(wh) < 0.0f) return 0.0f;
       return blinnPdf(b.e, wh) / (4.0f * absDot(wo, wh));
     }
+    case K_FB: return fbPdf(b, wo, wi);
     default: return 0.0f;
   }
 }
@@ -725,6 +801,28 @@ static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, 
         *wi = wo;
         *pdf = 0.0f;
       }
+      return;
+    }
+    case K_FB: {  // s False (Microfacet.hs:88-99): the sampled value is e wo wi / pdf
+      V3 w, wh;
+      float pdfp;
+      if (u1 < 0.5f) {
+        w = toSameHemi(wo, cosineSampleHemisphere(u1 * 2.0f, u2));
+        wh = fbHalf(w, wo);
+        pdfp = anisoPdf(b.ex, b.ey, wh);
+      } else {
+        float d;
+        wh = anisoSample(b.ex, b.ey, 2.0f * (u1 - 0.5f), u2, &d, &pdfp);
+        w = sscale(2.0f * dot(wo, wh), wh) - wo;
+      }
+      *wi = w;
+      if (pdfp == 0.0f) {
+        *f = sconst(0);
+        *pdf = 0.0f;
+        return;
+      }
+      *pdf = 0.5f * (absCosT(w) * kInvPi + pdfp / (4.0f * absDot(wo, wh)));
+      *f = sscale(fbEval(b, wo, w), 1.0f / *pdf);
       return;
     }
     case K_SPECR:  // Specular.hs:460-465
@@ -780,6 +878,8 @@ static bool texSelectsTex2(const bgpu_texture& t, const DG& dg) {
   return xf < lo || zf < lo || xf > hi || zf > hi;
 }
 
+static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1, Bsdf& b);
+
 static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
   Bsdf b;
   b.n = 0;
@@ -796,7 +896,13 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
     if (m.tex.slots & 1) sp0 = m.tex.alt[0];
     if (m.tex.slots & 2) sp1 = m.tex.alt[1];
   }
-  Spec s0 = specOf(sp0), s1 = specOf(sp1);
+  addComponents(m, specOf(sp0), specOf(sp1), b);
+  return b;
+}
+
+// The BxDFs of material m (Material.hs:32-127), with spectra s0 and s1 (the
+// texture's choice for the first two slots).
+static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1, Bsdf& b) {
   auto blinnE = [](float rough) {  // mkBlinn (1 / rough), fixExponent (Microfacet.hs:127-134)
     float e = 1.0f / rough;
     return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
@@ -867,6 +973,19 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
       b.c[b.n++] = t;
       break;
     }
+    case BGPU_MAT_SUBSTRATE: {  // mkSubstrate (Material.hs:111-127)
+      Bxdf x{};
+      x.kind = K_FB;
+      x.type = BX_R | BX_G;
+      x.r = sclamp(s0, 0.0f, 1.0f);
+      x.rs = sclamp(s1, 0.0f, 1.0f);
+      x.ra = sclamp(specOf(m.spec[2]), 0.0f, 1.0f);
+      x.ex = blinnE(hmax(0.0f, m.scalar[0]));  // mkAnisotropic (1 / u) (1 / v): fixExponent of each
+      x.ey = blinnE(hmax(0.0f, m.scalar[1]));
+      x.depth = m.scalar[2];
+      b.c[b.n++] = x;
+      break;
+    }
     case BGPU_MAT_TRANSMATTE: {  // translucentMatte (Material.hs:43-53)
       float sig = m.scalar[0];
       Spec r = sclamp(s0, 0.0f, 1.0f);
@@ -902,7 +1021,6 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
     default:
       break;
   }
-  return b;
 }
 
 static Spec evalBsdf(const Bsdf& b, V3 woW, V3 wiW) {  // Reflection.hs:318-332, adj = False
@@ -1240,6 +1358,53 @@ int orc_tex_selects(const bgpu_texture* t, float px, float py, float pz, float u
   dg.u = u;
   dg.v = v;
   return orc::texSelectsTex2(*t, dg) ? 1 : 0;
+}
+
+// The anisotropic distribution (ex, ey) at n half vectors wh (3 n floats):
+// D into d[i], its pdf into pdf[i]; with u (2 n floats) set, the sampler's
+// half vectors go to wh and their pdfs to pdf instead.
+void orc_aniso(float ex, float ey, int n, const float* u, float* wh, float* d, float* pdf) {
+  using namespace orc;
+  for (int i = 0; i < n; ++i) {
+    if (u) {
+      float dd;
+      V3 w = anisoSample(ex, ey, u[2 * i], u[2 * i + 1], &dd, &pdf[i]);
+      wh[3 * i] = w.x, wh[3 * i + 1] = w.y, wh[3 * i + 2] = w.z;
+      d[i] = dd;
+    } else {
+      V3 w = mk(wh[3 * i], wh[3 * i + 1], wh[3 * i + 2]);
+      d[i] = anisoD(ex, ey, w);
+      pdf[i] = anisoPdf(ex, ey, w);
+    }
+  }
+}
+
+// Component 0 of material m in its local shading frame, for n directions wo
+// (3 n floats): op 0 evaluates bxdfEval wo wi into f (16 n) and bxdfPdf wo wi
+// into pdf; op 1 samples with (u1, u2) = u[2 i..], writing wi, f and pdf.
+int orc_bxdf(const bgpu_material* m, int op, int n, const float* wo, const float* u, float* wi, float* f,
+             float* pdf) {
+  using namespace orc;
+  Bsdf b;
+  b.n = 0;
+  addComponents(*m, specOf(m->spec[0]), specOf(m->spec[1]), b);
+  if (b.n == 0) return -1;
+  const Bxdf& x = b.c[0];
+  for (int i = 0; i < n; ++i) {
+    V3 o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+    Spec s;
+    if (op == 0) {
+      V3 w = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+      s = bxEval(x, o, w);
+      pdf[i] = bxPdf(x, o, w);
+    } else {
+      V3 w;
+      bxSample(x, o, u[2 * i], u[2 * i + 1], &s, &w, &pdf[i]);
+      wi[3 * i] = w.x, wi[3 * i + 1] = w.y, wi[3 * i + 2] = w.z;
+    }
+    for (int k = 0; k < 16; ++k) f[16 * i + k] = s[k];
+  }
+  return 0;
 }
 
 // Sampler values of every sample n of pixel (pix, pix) in `pass`, for the

@@ -114,3 +114,53 @@ def render(flat, spec, threads=None, passes=None, scene=None):
     if scene is None:
         o.close()
     return img, sq, rays.value
+
+
+def aniso(ex, ey, wh=None, u=None):
+    """The anisotropic microfacet distribution of exponents (ex, ey)
+    (Microfacet.hs:136-196).  With half vectors wh (n, 3): (D, pdf) there.
+    With sample points u (n, 2): the sampler's (wh, d, pdf)."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_aniso.argtypes = [C.c_float, C.c_float, C.c_int, vp, vp, vp, vp]
+    if u is not None:
+        u = np.ascontiguousarray(u, np.float32)
+        n = len(u)
+        wh = np.zeros((n, 3), np.float32)
+    else:
+        wh = np.ascontiguousarray(wh, np.float32)
+        n = len(wh)
+    d = np.zeros(n, np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib.orc_aniso(ex, ey, n, None if u is None else u.ctypes.data, wh.ctypes.data, d.ctypes.data, pdf.ctypes.data)
+    return (wh, d, pdf) if u is not None else (d, pdf)
+
+
+def bxdf_eval(material, wo, wi):
+    """Component 0 of `material` (an abi.Material) in its local frame:
+    bxdfEval wo wi as (n, 16) spectra and bxdfPdf wo wi."""
+    return _bxdf(material, 0, wo, wi, None)[1:]
+
+
+def bxdf_sample(material, wo, u):
+    """bxdfSample (adj False) of component 0 of `material` at sample points u
+    (n, 2): returns (wi, f, pdf)."""
+    return _bxdf(material, 1, wo, None, u)
+
+
+def _bxdf(material, op, wo, wi, u):
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_bxdf.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    lib.orc_bxdf.restype = C.c_int
+    wo = np.ascontiguousarray(wo, np.float32)
+    n = len(wo)
+    wi = np.zeros((n, 3), np.float32) if wi is None else np.ascontiguousarray(wi, np.float32).copy()
+    u = None if u is None else np.ascontiguousarray(u, np.float32)
+    f = np.zeros((n, 16), np.float32)
+    pdf = np.zeros(n, np.float32)
+    rc = lib.orc_bxdf(C.addressof(material), op, n, wo.ctypes.data, None if u is None else u.ctypes.data,
+                      wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    if rc != 0:
+        raise ValueError("material has no BxDF")
+    return wi, f, pdf

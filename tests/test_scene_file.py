@@ -101,7 +101,7 @@ def test_malformed_files_are_refused(tmp_path):
     refused(raw + b"\0" * 64, "trailing bytes")
     refused(b"XGPUSCN1" + raw[8:], "magic")
     bad = bytearray(raw)
-    bad[8] = 2  # version
+    bad[8] = 0xEE  # version
     refused(bytes(bad), "version")
     bad = bytearray(raw)
     bad[40] ^= 1  # flags: a normals section the file does not hold
