@@ -9,6 +9,8 @@
 //   Rendering.hs:207-246 (passes over the sample extent)
 //   Integrator/DirectLighting.hs:23-63       Material.hs:43-53 (translucentMatte)
 //   Texture.hs:191-219 (checkerBoard, graphPaper selectors)
+//   Material.hs:111-127 (mkSubstrate)        Reflection/Microfacet.hs:56-104,136-196
+//                                            (FresnelBlend, the anisotropic distribution)
 //
 // Pins (tests/test_oracle_pins.py):
 //   path integrator, NEE + MIS   rho L under a uniform sky; rho Le (R/D)^2 below a
@@ -20,8 +22,14 @@
 //   sampler                      SplitMix64 known answers, permutation bijectivity,
 //                                stratification of every table dimension
 //   film, camera                 job views equal scenes built at that size
+// Pins (tests/test_oracle_microfacet.py):
+//   anisotropic distribution     D cos and pdf integrate to 1; ex = ey is Blinn's D,
+//                                pdf and sampler in closed form; sampled moments
+//   FresnelBlend                 sample = eval / pdf at its own pdf; the coating's
+//                                absorption is Beer-Lambert
 // Parity unpinned: the bits of glibc sinf/cosf/acosf/atan2f/powf/expf as GHC
-// calls them (they move sampled directions, compared statistically).
+// calls them (they move sampled directions, compared statistically), and
+// FresnelBlend's diffuse and specular factors (the reference's own forms).
 #include <array>
 #include <atomic>
 #include <cmath>
