@@ -609,3 +609,15 @@ def test_substrate_scene_matches_oracle(gpu_ctx):
     frac, zmean, zmax = compare(g, gsq, o2, osq2)
     assert frac < 0.015, (frac, zmean, zmax)
     assert abs(zmean) < 0.08, zmean
+    # the direct-lighting integrator's shading kernel on the same materials
+    spec3 = b.spec(passes=4, seed=25)
+    spec3.variance = True
+    spec3.integrator = "direct"
+    g3, gsq3 = render_gpu(gpu_ctx, b.flat, spec3)
+    o3, osq3, _ = pyoracle.render(b.flat, spec3)
+    assert np.isfinite(g3).all() and g3[..., 2].mean() > 0
+    frac, zmean, zmax = compare(g3, gsq3, o3, osq3)
+    assert frac < 0.01, (frac, zmean, zmax)
+    mg, _ = mean_se(g3, gsq3)
+    mo, _ = mean_se(o3, osq3)
+    assert np.median(np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)) < 1e-4
