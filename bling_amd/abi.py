@@ -107,7 +107,8 @@ class Stats(C.Structure):
     _fields_ = [("rays_camera", C.c_uint64), ("rays_extend", C.c_uint64),
                 ("rays_shadow", C.c_uint64), ("rays_mis", C.c_uint64),
                 ("samples", C.c_uint64), ("jobs_done", C.c_uint64), ("gpu_ms", C.c_double),
-                ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64)]
+                ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64),
+                ("branches_dropped", C.c_uint64)]
 
 
 # every symbol declared in include/bgpu.h
@@ -118,6 +119,7 @@ EXPORTS = [
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
     "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get", "bgpu_scene_bvh_info",
     "bgpu_scene_write", "bgpu_scene_read", "bgpu_scene_desc_free", "bgpu_scene_load",
+    "bgpu_mem_info",
 ]
 
 _lib = None
@@ -157,6 +159,7 @@ def load(path=LIB_PATH):
     lib.bgpu_job_latency_ms.argtypes = [vp, C.POINTER(C.c_double)]
     lib.bgpu_stats_get.argtypes = [vp, C.POINTER(Stats)]
     lib.bgpu_stats_reset.argtypes = [vp]
+    lib.bgpu_mem_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     lib.bgpu_config_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.bgpu_config_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     lib.bgpu_scene_bvh_info.argtypes = [vp, C.POINTER(BvhInfo)]

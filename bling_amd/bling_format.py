@@ -744,7 +744,7 @@ class BlingScene:
         """The GPU job of the file's renderer block: `sampled { sampler {
         stratified nu nv } integrator { path maxDepth md sampleDepth sd } }`,
         with the reference's hard-coded Russian-roulette depth 7
-        (Integrator/Path.hs:70), or `integrator { directLighting maxDepth md }`."""
+        (Integrator/Path.hs:68), or `integrator { directLighting maxDepth md }`."""
         from .runtime import JobSpec
         r = self.renderer
         if r[0] != "sampler":

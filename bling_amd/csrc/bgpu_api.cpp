@@ -102,6 +102,13 @@ int bgpu_init(int device, size_t hbm_budget_bytes, bgpu_ctx** out) {
   return BGPU_OK;
 }
 
+int bgpu_mem_info(bgpu_ctx* ctx, size_t* used, size_t* budget) {
+  if (!ctx || !used || !budget) return fail(BGPU_E_INVALID, "bgpu_mem_info: NULL argument");
+  *used = ctx->used.load();
+  *budget = ctx->budget;
+  return BGPU_OK;
+}
+
 int bgpu_shutdown(bgpu_ctx* ctx) {
   if (!ctx) return fail(BGPU_E_INVALID, "bgpu_shutdown: NULL context");
   (void)hipSetDevice(ctx->device);

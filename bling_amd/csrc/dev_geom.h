@@ -3,8 +3,8 @@
 // -ffp-contract=off; '/' and sqrt are correctly rounded in HIP by default),
 // so that hit/miss, t and barycentrics are bit-identical to the reference's
 // Float arithmetic:
-//   Math.hs:124-139,328-353   AABB.hs:79-94   Transform.hs:341-372
-//   Primitive/TriangleMesh.hs:220-261   Shape.hs:81-284
+//   Math.hs:124-139,328-353   AABB.hs:79-94   Transform.hs:247-280
+//   Primitive/TriangleMesh.hs:140-207   Shape.hs:81-284
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -96,7 +96,7 @@ __device__ __forceinline__ bool slab_haskell(const float* lo, const float* hi, c
   return !(nearT > farT);
 }
 
-// Moeller-Trumbore exactly as triangleIntersect (TriangleMesh.hs:240-261),
+// Moeller-Trumbore exactly as triangleIntersect (TriangleMesh.hs:160-207),
 // with e1 = p2 - p1 and e2 = p3 - p1 precomputed on the host (same IEEE op).
 __device__ __forceinline__ bool tri_test(f3 p1, f3 e1, f3 e2, f3 ro, f3 rd, float tmin, float tmax,
                                          float* to, float* b1o, float* b2o) {

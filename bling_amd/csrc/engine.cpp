@@ -2,7 +2,7 @@
 // bounce iteration, replayed until the admitted work is done.
 //
 //   bgpu_render      the plain path: one job on a private engine, blocking
-//                    (prender, Rendering.hs:207-236, all passes).
+//                    (prender, Rendering.hs:111-140, all passes).
 //   bgpu_job_*       the served path: a scheduler thread owns one engine;
 //                    jobs arriving while others run are admitted as job-table
 //                    slots and HBM (film buffers) free up, and their samples
@@ -11,7 +11,7 @@
 //                    samples to the device in chunks, earliest estimated
 //                    finish first, so short jobs overtake long ones.
 //                    Progress is polled per pass (PassDone,
-//                    Rendering.hs:165-169).
+//                    Rendering.hs:60-75).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -523,6 +523,7 @@ static void accumulate_stats(Ctx* ctx, const PtCounters& before, const PtCounter
   s.rays_mis += (after.tot_mis + after.n_mis) - (before.tot_mis + before.n_mis);
   s.samples += samples;
   s.gpu_ms += ms;
+  s.branches_dropped += after.dl_dropped - before.dl_dropped;
 }
 
 // ---------------------------------------------------------------- served
@@ -911,6 +912,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     ctx->bvh_device = (int)value;
     return BGPU_OK;
   }
+  // the copy, its validation and the assignment all happen under ctx->mu, so
+  // that concurrent settings of different pt.* keys neither race nor get lost
+  std::lock_guard<std::mutex> lk(ctx->mu);
   PtConfig next = ctx->pt_cfg;
   if (k == "pt.block") {
     if (!block_ok(value)) return fail(BGPU_E_INVALID, "pt.block must be 64, 128 or 256");
@@ -934,7 +938,6 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   }
   // engines capture their kernels and pool at creation: drop the idle ones
-  std::lock_guard<std::mutex> lk(ctx->mu);
   if (ctx->live_jobs.load() > 0) return fail(BGPU_E_STATE, "served job handles are still live");
   if (ctx->sched) {
     destroy_scheduler(ctx->sched);
@@ -952,6 +955,7 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
 int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   if (!ctx || !key || !value) return fail(BGPU_E_INVALID, "bgpu_config_get: NULL argument");
   const std::string k(key);
+  std::lock_guard<std::mutex> lk(ctx->mu);
   if (k == "trace.block") *value = ctx->trace_cfg.block;
   else if (k == "bvh.width") *value = ctx->bvh_width;
   else if (k == "bvh.device") *value = ctx->bvh_device;

@@ -98,6 +98,7 @@ struct PtCounters {
   unsigned long long tot_camera, tot_extend, tot_shadow, tot_mis, tot_samples;
   unsigned int ring_head, ring_tail;  // chunks not yet fully taken [head, tail)
   unsigned int iter, pad;
+  unsigned long long dl_dropped;  // direct-lighting branches lost to a full branch stack (must stay 0)
   unsigned int cls_count[kShadeClasses];   // active paths per shading class
   unsigned int cls_cursor[kShadeClasses];  // slots of the grouped queue taken per class
 };

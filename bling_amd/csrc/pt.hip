@@ -576,7 +576,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         sm.get2(un, 2 + 3 * d, &bd1, &bd2);
         f3 wo = neg3(ray.d);
         f3 p = bsdf.p;
-        // emission seen through a specular chain (Path.hs:53, intLe with rd)
+        // emission seen through a specular chain (Path.hs:60, intLe with rd)
         if (spec && light >= 0 && dot3(dgg.n, ray.d) > 0.0f) {
           const float* L = S.lights[light].L;
           Xyz c;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         }
         sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at, want_shadow,
                          want_mis);
-        // Russian roulette (Path.hs:70-72)
+        // Russian roulette (Path.hs:68-72)
         float pc = 1.0f;
         if (depth > J.rr) {
           float y = 0.0f;  // sY (Spectrum.hs:371-373)
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   if (want_mis) P.q_mis[qm] = (uint32_t)i;
 }
 
-// Shading of direct-lighting paths (Integrator/DirectLighting.hs:23-63).
+// Shading of direct-lighting paths (Integrator/DirectLighting.hs:23-59).
 // A slot walks its sample's tree of specular branches depth first: at a hit
 // it adds the emission towards wo and sampleOneLight (dimensions 2d, 2d + 1),
 // then continues along the specular reflection and pushes the specular
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
       }
       sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2, thr_at, want_shadow,
                        want_mis);
-      if (depth + 1 < J.md) {  // cont (DirectLighting.hs:49-63)
+      if (depth + 1 < J.md) {  // cont (DirectLighting.hs:45-59)
         SpecVal fr, ft;
         f3 wr, wt;
         const float pr = sample_bsdf_flt(bsdf, wo, BX_S | BX_R, fr, &wr);
@@ -706,6 +706,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
         const bool br = pr != 0.0f, bt = pt != 0.0f;
         if (br && bt) {  // reflection continues here, transmission waits
           const int e = P.dl_sp[i];
+          // validate_job bounds maxDepth by dl_cap + 1 and at most maxDepth - 1
+          // branches are pending at once, so a full stack is unreachable;
+          // a dropped branch would bias the image, so it is counted
+          if (e >= P.dl_cap) atomicAdd(&a.cnt->dl_dropped, 1ull);
           if (e < P.dl_cap) {
             float* st = P.dl_stk + (size_t)e * kDlWords * Pn + i;
             st[0] = p.x; st[Pn] = p.y; st[2 * Pn] = p.z;

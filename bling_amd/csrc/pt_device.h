@@ -2,7 +2,7 @@
 //   Integrator/Path.hs:41-87, Scene.hs:61-118 (NEE with MIS), Reflection.hs:201-335,
 //   Reflection/{Diffuse,Specular,Microfacet}.hs, Fresnel.hs:31-69, Material.hs:32-96,
 //   Light.hs:72-229, Shape.hs:314-410, Montecarlo.hs:40-186, Camera.hs:49-68,
-//   DifferentialGeometry.hs:41-81, Primitive/TriangleMesh.hs:202-287.
+//   DifferentialGeometry.hs:41-81, Primitive/TriangleMesh.hs:119-207.
 // Spectra are 16 binary32 bands in registers; material and light spectra are
 // read from HBM where they are used.
 #pragma once
@@ -255,7 +255,7 @@ __device__ __forceinline__ void hit_dg(const DevPtScene& S, const DRay& ray, flo
       dgg->v = b0 * uv01 + b1 * uv11 + b2 * uv21;
     }
     *dgs = *dgg;
-    if ((info.y & BGPU_MESH_NORMALS) && S.normals) {  // TriangleMesh.hs:202-214
+    if ((info.y & BGPU_MESH_NORMALS) && S.normals) {  // TriangleMesh.hs:122-134
       f3 n0 = mk3(S.normals[3 * i0], S.normals[3 * i0 + 1], S.normals[3 * i0 + 2]);
       f3 n1 = mk3(S.normals[3 * i1], S.normals[3 * i1 + 1], S.normals[3 * i1 + 2]);
       f3 n2 = mk3(S.normals[3 * i2], S.normals[3 * i2 + 1], S.normals[3 * i2 + 2]);
@@ -424,7 +424,7 @@ struct SpecVal {
   float k1, k2;
 };
 
-__device__ __forceinline__ float fr_cond_band(const DBxdf& b, float cosi, int i) {  // Fresnel.hs:254-266
+__device__ __forceinline__ float fr_cond_band(const DBxdf& b, float cosi, int i) {  // Fresnel.hs:58-70
   float ac = fabsf(cosi);
   float eta = b.eta[i], k = b.k[i];
   float tmpF = eta * eta + k * k;
@@ -563,7 +563,7 @@ __device__ __forceinline__ bool fb_sample(const DBxdf& b, f3 wo, float u1, float
   fb_terms(b, wo, w, td, ts);
   return true;
 }
-__device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // Diffuse.hs:431-443
+__device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // Diffuse.hs:53-65
   float sinti = sin_t(wi), sinto = sin_t(wo);
   float maxcos = 0.0f;
   if (sinti > 1e-4f && sinto > 1e-4f) {
@@ -664,13 +664,13 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
     }
     return;
   }
-  if (b.kind == K_SPECR) {  // Specular.hs:460-465
+  if (b.kind == K_SPECR) {  // Specular.hs:11-20
     *f = rf_term(b, cos_t(wo), 1.0f);
     *wi = mk3(-wo.x, -wo.y, wo.z);
     *pdf = 1.0f;
     return;
   }
-  // specTrans (Specular.hs:478-502)
+  // specTrans (Specular.hs:33-58)
   bool entering = cos_t(wo) > 0.0f;
   float ei = entering ? b.ei : b.et, et = entering ? b.et : b.ei;
   float eta = ei / et, eta2 = eta * eta;

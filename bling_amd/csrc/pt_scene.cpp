@@ -3,7 +3,7 @@
 //   lights     -> explicit lights, then geometric (emissive shape) lights in
 //                 shape order, the list of mkScene (Scene.hs:37-43)
 //   env maps   -> the Dist2D of mkInfiniteAreaLight (Light.hs:76-77,
-//                 Montecarlo.hs:263-317), built in binary32 in the same order
+//                 Montecarlo.hs:34-104), built in binary32 in the same order
 //   film       -> 16x16 filter table (Image.hs:48-61) and sample extent (Image.hs:162-168)
 #include <algorithm>
 #include <cmath>
@@ -66,7 +66,7 @@ float eval_filter(const bgpu_film& f, float x, float y) {  // Filter.hs:68-96
   }
 }
 
-// Dist1D [n, func[n], cdf[n+1], fi] (Montecarlo.hs:269-277)
+// Dist1D [n, func[n], cdf[n+1], fi] (Montecarlo.hs:40-48)
 void push_dist1d(std::vector<float>& out, const std::vector<float>& f) {
   int n = (int)f.size();
   float nb;
@@ -283,6 +283,12 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     mats[i].alt = (int32_t)mats.size();
     mats.push_back(build(m, alt));
   }
+  // eval_bsdf / sample_bsdf take a FresnelBlend lobe (K_FB) to be its
+  // material's only component: its value replaces the per-component sum
+  for (const DMaterial& dm : mats)
+    for (int c = 0; c < dm.n; ++c)
+      if (dm.c[c].kind == K_FB && dm.n != 1)
+        return fail(BGPU_E_INVALID, "material: a FresnelBlend lobe must be the material's only BxDF");
   if (int rc = sc->materials.upload(ctx, mats.data(), mats.size() * sizeof(DMaterial), st)) return rc;
   // distinct BxDF makeups among the materials primitives use: a scene with
   // more than one benefits from shading queues grouped by material kind

@@ -1,6 +1,6 @@
 // Scene flattening: the description (world-space vertices after the host's
 // transPoint, shapes with their matrices, materials, lights, camera, film) is
-// validated like mkTriangleMesh (Primitive/TriangleMesh.hs:127-136), bounded
+// validated like mkTriangleMesh (Primitive/TriangleMesh.hs:46-55), bounded
 // like mkKdTree (Primitive/KdTree.hs:107-112), organised into a BVH and copied
 // once into device-resident arrays.
 #include <algorithm>
@@ -32,7 +32,7 @@ namespace {
 inline float hmax_h(float x, float y) { return (x <= y) ? y : x; }
 inline float hmin_h(float x, float y) { return (x <= y) ? x : y; }
 
-inline void xp_host(const float* m, const float* p, float* o) {  // Transform.hs:341-350
+inline void xp_host(const float* m, const float* p, float* o) {  // Transform.hs:247-258
   float xp = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
   float yp = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
   float zp = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
@@ -65,7 +65,7 @@ void extend_b(Box& a, const Box& b) {  // extendAABB (AABB.hs:34-41)
   }
 }
 
-// transBox o2w (objectBounds s)  (Shape.hs:286-311, Transform.hs:375-386)
+// transBox o2w (objectBounds s)  (Shape.hs:286-311, Transform.hs:281-296)
 Box shape_world_bounds(const bgpu_shape& s) {
   float lo[3], hi[3];
   const float* p = s.params;
@@ -143,7 +143,7 @@ inline float bitsf(int32_t i) {
 int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc);
 
 int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
-  // -- validation (TriangleMesh.hs:127-136 raise `error`; here: BGPU_E_INVALID)
+  // -- validation (TriangleMesh.hs:46-55 raise `error`; here: BGPU_E_INVALID)
   if (d->num_triangles && (!d->indices || !d->vertices))
     return fail(BGPU_E_INVALID, "triangles given without vertices/indices");
   if (d->num_shapes && !d->shapes) return fail(BGPU_E_INVALID, "shapes pointer is NULL");
@@ -196,7 +196,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   if (!dev_tri) {
     parallel_for(d->num_triangles, [&](size_t tb, size_t te) {
       for (size_t t = tb; t < te; ++t) {
-        Box b = empty_box();  // triangleBounds (TriangleMesh.hs:216-218)
+        Box b = empty_box();  // triangleBounds (TriangleMesh.hs:136-138)
         for (int k = 0; k < 3; ++k) extend_p(b, d->vertices + 3 * (size_t)d->indices[3 * t + k]);
         pb[t] = b;
       }
@@ -356,7 +356,7 @@ int flatten_and_upload(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
         const float* p2 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 1];
         const float* p3 = d->vertices + 3 * (size_t)d->indices[3 * (size_t)id + 2];
         o[0] = p1[0]; o[1] = p1[1]; o[2] = p1[2]; o[3] = bitsf(id);
-        // e1 = p2 - p1, e2 = p3 - p1: the same IEEE subtractions as TriangleMesh.hs:249
+        // e1 = p2 - p1, e2 = p3 - p1: the same IEEE subtractions as TriangleMesh.hs:169
         o[4] = p2[0] - p1[0]; o[5] = p2[1] - p1[1]; o[6] = p2[2] - p1[2]; o[7] = bitsf(0);
         o[8] = p3[0] - p1[0]; o[9] = p3[1] - p1[1]; o[10] = p3[2] - p1[2]; o[11] = 0.0f;
       } else {

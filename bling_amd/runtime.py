@@ -7,9 +7,9 @@ C ABI (include/bgpu.h).
                   a miss is prim id -1, as `Nothing`.
   Renderer     -- `GpuRenderer.render(job, report)` follows
                   `class Renderer a where render :: a -> RenderJob ->
-                  ProgressReporter -> IO ()` (Rendering.hs:173-174): it calls
+                  ProgressReporter -> IO ()` (Rendering.hs:77-78): it calls
                   report(PassDone(pass, image, 1)) after every pass and stops
-                  when report returns False (Rendering.hs:223-236).
+                  when report returns False (Rendering.hs:111-140).
 
 Everything runs on the GPU through libbgpu.so; there is no CPU fallback.
 """
@@ -52,6 +52,12 @@ class Context:
         s = abi.Stats()
         _ck(self.lib.bgpu_stats_get(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in abi.Stats._fields_}
+
+    def mem_info(self):
+        """(device bytes held, HBM budget) of the context."""
+        used, budget = C.c_size_t(), C.c_size_t()
+        _ck(self.lib.bgpu_mem_info(self.h, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
 
     def reset_stats(self):
         _ck(self.lib.bgpu_stats_reset(self.h))
@@ -197,7 +203,7 @@ class JobSpec:
 
 @dataclass
 class PassDone:
-    """Progress event after a pass (Rendering.hs:165-169)."""
+    """Progress event after a pass (Rendering.hs:60-75)."""
     pass_num: int
     final_img: np.ndarray  # (h, w, 4) float32 = (weight, X, Y, Z)
     splat_weight: float = 1.0
@@ -243,7 +249,7 @@ class Job:
 
 
 class GpuRenderer:
-    """The `gpu` renderer: one job, progressive passes (Rendering.hs:207-236)."""
+    """The `gpu` renderer: one job, progressive passes (Rendering.hs:111-140)."""
 
     def __init__(self, spec):
         self.spec = spec

@@ -3,7 +3,7 @@
 primitives (src/lib/Graphics/Bling/IO/PrimitiveParser.hs:28-139):
 
   * meshes are pre-transformed to world space with transPoint/transNormal
-    (Primitive/TriangleMesh.hs:119-140);
+    (Primitive/TriangleMesh.hs:39-59);
   * shapes keep their object-to-world transform and its inverse and get an
     AreaLight with a fresh id when an emission is active (Primitive/Geometry.hs:14-37,
     IO/PrimitiveParser.hs:63-67);
@@ -417,7 +417,7 @@ class FlatScene:
 
     def view(self, width, height):
         """The scene's camera re-made for a `width` x `height` image (a
-        RenderJob's camera follows its imageSize, Rendering.hs:133-137)."""
+        RenderJob's camera follows its imageSize, IO/CameraParser.hs:30-38)."""
         if self.lens is None:
             raise ValueError("scene was not built with SceneBuilder.look")
         c2w, fov, lr, fd, ft, fp = self.lens

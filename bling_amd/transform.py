@@ -116,7 +116,7 @@ class Transform:
         self.m = list(IDENTITY) if m is None else [F(v) for v in m]
         self.inv = list(IDENTITY) if inv is None else [F(v) for v in inv]
 
-    def __matmul__(self, other):  # concatTrans (Transform.hs:335-338)
+    def __matmul__(self, other):  # concatTrans (Transform.hs:241-245)
         return Transform(mat_mul(self.m, other.m), mat_mul(other.inv, self.inv))
 
     def inverse(self):
@@ -128,7 +128,7 @@ class Transform:
     def np_inv(self):
         return np.array(self.inv, dtype=np.float32)
 
-    def point(self, p):  # transPoint (Transform.hs:341-350)
+    def point(self, p):  # transPoint (Transform.hs:247-258)
         m = self.m
         x, y, z = (F(v) for v in p)
         xp = F(F(F(F(m[0] * x) + F(m[1] * y)) + F(m[2] * z)) + m[3])
@@ -159,7 +159,7 @@ class Transform:
         return out.astype(np.float32)
 
     def normals(self, ns):
-        """transNormal (Transform.hs:361-366) over (n, 3): uses the inverse, transposed."""
+        """transNormal (Transform.hs:267-274) over (n, 3): uses the inverse, transposed."""
         n = np.asarray(ns, dtype=np.float32)
         i = self.np_inv()
         x, y, z = n[:, 0], n[:, 1], n[:, 2]
@@ -203,7 +203,7 @@ def rotate_z(deg):
 
 
 def from_matrix(rows):
-    """fromMatrix' (Transform.hs:234-235): matrix plus its computed inverse."""
+    """fromMatrix' (Transform.hs:140-141): matrix plus its computed inverse."""
     m = [F(v) for r in rows for v in r]
     return Transform(m, invert(m))
 
@@ -222,7 +222,7 @@ def _v_cross(u, v):  # cross (Math.hs:336-339)
             F(F(u[0] * v[1]) - F(u[1] * v[0])))
 
 
-def look_at(pos, look, up):  # Transform.hs:316-329
+def look_at(pos, look, up):  # Transform.hs:222-237
     p = tuple(F(v) for v in pos)
     l = tuple(F(v) for v in look)
     d = _v_norm((F(l[0] - p[0]), F(l[1] - p[1]), F(l[2] - p[2])))
@@ -233,7 +233,7 @@ def look_at(pos, look, up):  # Transform.hs:316-329
     return Transform(m, invert(m))
 
 
-def perspective(fov, n, f):  # Transform.hs:301-313
+def perspective(fov, n, f):  # Transform.hs:207-221
     n, f = F(n), F(f)
     i_tan = F(F(1) / tanf(F(radians(fov) / F(2))))
     fn = F(f - n)

@@ -8,7 +8,7 @@
  *     `occluded` (Scene.hs:45-51)  -> bgpu_intersect_closest / bgpu_intersect_any;
  *   - the path integrator `mkPathIntegrator md sd` (Integrator/Path.hs:30-87)
  *     driven per camera sample by the sampler renderer `prender`
- *     (Rendering.hs:207-246)  -> bgpu_job_submit / bgpu_job_poll / bgpu_render;
+ *     (Rendering.hs:111-151)  -> bgpu_job_submit / bgpu_job_poll / bgpu_render;
  *   - the film accumulation `addSample` (Image.hs:250-299), returned as the
  *     (w, X, Y, Z) float quadruple per pixel of `Image` (Image.hs:123-129).
  *
@@ -35,7 +35,7 @@ extern "C" {
 /* ---- error codes ------------------------------------------------------ */
 enum {
   BGPU_OK = 0,
-  BGPU_E_INVALID = -1,   /* bad argument or malformed scene (cf. TriangleMesh.hs:127-136) */
+  BGPU_E_INVALID = -1,   /* bad argument or malformed scene (cf. TriangleMesh.hs:46-55) */
   BGPU_E_NOMEM = -2,     /* host or device allocation failed */
   BGPU_E_HIP = -3,       /* HIP runtime error */
   BGPU_E_BUDGET = -4,    /* job or scene does not fit the context's HBM budget */
@@ -74,7 +74,7 @@ typedef struct {
   int32_t light_id;   /* the AreaLight id used by light equality (Light.hs:47-50) */
   float params[6];
   float o2w[16];      /* object-to-world matrix, row-major 4x4 */
-  float w2o[16];      /* its stored inverse (Transform.hs:218-221) */
+  float w2o[16];      /* its stored inverse (Transform.hs:123-127) */
   float emission[16]; /* spectral radiance when emissive */
 } bgpu_shape;
 
@@ -124,7 +124,7 @@ typedef struct {
   bgpu_texture tex; /* kind BGPU_TEX_NONE: constant spectra; selectors cover spec[0] and spec[1] */
 } bgpu_material;
 
-/* A triangle mesh made by mkTriangleMesh (Primitive/TriangleMesh.hs:119-140). */
+/* A triangle mesh made by mkTriangleMesh (Primitive/TriangleMesh.hs:39-59). */
 enum { BGPU_MESH_NORMALS = 1, BGPU_MESH_UVS = 2 };
 typedef struct {
   int32_t material;
@@ -204,7 +204,7 @@ typedef struct {
 } bgpu_rays;
 
 /* Closest-hit result: t, the triangle barycentrics (b1, b2) of
- * TriangleMesh.hs:254-261 (0 for shapes) and the primitive id, -1 on a miss. */
+ * TriangleMesh.hs:175-180 (0 for shapes) and the primitive id, -1 on a miss. */
 typedef struct {
   float *t, *b1, *b2;
   int32_t* prim;
@@ -224,7 +224,7 @@ enum {
 /* One render job: `renderer { gpu sampled { sampler { stratified nu nv }
  * integrator { path maxDepth md sampleDepth sd } } }` (RendererParser.hs:56-73,
  * IntegratorParser.hs:37-44), `passes` progressive passes of nu*nv samples per
- * pixel over the film's sample extent (Image.hs:162-168, Rendering.hs:207-236).
+ * pixel over the film's sample extent (Image.hs:162-168, Rendering.hs:111-140).
  * integrator BGPU_INTEGRATOR_DIRECT is `directLighting maxDepth md`
  * (Integrator/DirectLighting.hs): sample_depth and rr_start_depth are unused,
  * and max_depth - 1 must not exceed the "pt.dl_depth" knob (its specular
@@ -234,13 +234,13 @@ enum { BGPU_INTEGRATOR_PATH = 0, BGPU_INTEGRATOR_DIRECT = 1 };
 typedef struct {
   int32_t spp_nu, spp_nv;
   int32_t max_depth, sample_depth;
-  int32_t rr_start_depth; /* Russian roulette when depth > this (Path.hs:70 uses 7) */
+  int32_t rr_start_depth; /* Russian roulette when depth > this (Path.hs:68 uses 7) */
   int32_t passes;         /* number of progressive passes, >= 1 */
   uint64_t seed;
   int32_t flags;
   int32_t integrator;     /* BGPU_INTEGRATOR_PATH (default) or BGPU_INTEGRATOR_DIRECT */
   /* Per-job camera and film (a RenderJob's camera and imageSize,
-   * Rendering.hs:133-137); NULL selects the scene description's. */
+   * IO/CameraParser.hs:30-38); NULL selects the scene description's. */
   const bgpu_camera* camera;
   const bgpu_film* film;
 } bgpu_job_desc;
@@ -260,6 +260,8 @@ typedef struct {
   double gpu_ms;             /* device time of the render launches */
   uint64_t nodes_visited;    /* BGPU_COUNT queries: BVH node fetches */
   uint64_t prims_tested;     /* BGPU_COUNT queries: primitive tests */
+  uint64_t branches_dropped; /* direct-lighting specular branches lost to a full
+                                branch stack; 0 unless an internal bound is broken */
 } bgpu_stats;
 
 /* ---- entry points ------------------------------------------------------ */
@@ -297,13 +299,13 @@ int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
 /* Duration in ms of the last BGPU_TIMED intersect launch on this context. */
 int bgpu_last_kernel_ms(bgpu_ctx* ctx, double* ms);
 
-/* Plain path: render one job to completion, blocking (prender, Rendering.hs:207-236).
+/* Plain path: render one job to completion, blocking (prender, Rendering.hs:111-140).
  * wxyz: width*height*4 floats (w, X, Y, Z); sq: width*height*3 or NULL. */
 int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float* sq);
 
 /* Served path: asynchronous jobs that share the device (continuous batching). */
 int bgpu_job_submit(bgpu_scene* scene, const bgpu_job_desc* job, bgpu_job** out);
-/* Progressive result (PassDone, Rendering.hs:165-169): copies the film so far. */
+/* Progressive result (PassDone, Rendering.hs:60-75): copies the film so far. */
 int bgpu_job_poll(bgpu_job* job, float* wxyz, float* sq, int32_t* passes_done, int32_t* state);
 int bgpu_job_wait(bgpu_job* job, double timeout_s);
 int bgpu_job_cancel(bgpu_job* job);
@@ -312,6 +314,10 @@ int bgpu_job_release(bgpu_job* job);
 int bgpu_job_latency_ms(bgpu_job* job, double* ms);
 
 int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out);
+/* Device bytes the context holds (scenes, engines, resident job films) and
+ * its HBM budget (no counterpart in the reference).  Jobs whose film would
+ * take `used` past `budget` wait in the queue until others release theirs. */
+int bgpu_mem_info(bgpu_ctx* ctx, size_t* used, size_t* budget);
 int bgpu_stats_reset(bgpu_ctx* ctx);
 
 /* Tuning knobs (no counterpart in the reference, whose CPU integrator has

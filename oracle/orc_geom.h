@@ -12,7 +12,7 @@ struct TriHit {
   float t, b1, b2;
 };
 
-// triangleIntersect (TriangleMesh.hs:240-261): the hit test part.
+// triangleIntersect (TriangleMesh.hs:160-207): the hit test part.
 inline bool triangleIntersect(V3 p1, V3 p2, V3 p3, const Ray& r, TriHit* h) {
   V3 e1 = p2 - p1, e2 = p3 - p1;
   V3 s1 = cross(r.d, e2);
@@ -33,7 +33,7 @@ inline bool triangleIntersect(V3 p1, V3 p2, V3 p3, const Ray& r, TriHit* h) {
   return true;
 }
 
-// triangleIntersects (TriangleMesh.hs:220-238): same arithmetic, Bool.
+// triangleIntersects (TriangleMesh.hs:140-158): same arithmetic, Bool.
 inline bool triangleIntersects(V3 p1, V3 p2, V3 p3, const Ray& r) {
   TriHit h;
   return triangleIntersect(p1, p2, p3, r, &h);
@@ -222,7 +222,7 @@ inline AABB objectBounds(const Shape& s) {
   }
 }
 
-// transBox (Transform.hs:375-386)
+// transBox (Transform.hs:281-296)
 inline AABB transBox(const M4& t, const AABB& b) {
   float mx = b.lo.x, my = b.lo.y, mz = b.lo.z, nx = b.hi.x, ny = b.hi.y, nz = b.hi.z;
   V3 ps[8] = {mk(mx, my, mz), mk(mx, my, nz), mk(mx, ny, mz), mk(mx, ny, nz),
@@ -232,7 +232,7 @@ inline AABB transBox(const M4& t, const AABB& b) {
   return r;
 }
 
-// triangleBounds (TriangleMesh.hs:216-218)
+// triangleBounds (TriangleMesh.hs:136-138)
 inline AABB triangleBounds(V3 p1, V3 p2, V3 p3) {
   return extendP(extendP(extendP(emptyAABB(), p1), p2), p3);
 }

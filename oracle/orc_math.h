@@ -108,7 +108,7 @@ struct M4 {
   float m[16];
   float at(int r, int c) const { return m[r * 4 + c]; }
 };
-// transPoint (Transform.hs:341-350)
+// transPoint (Transform.hs:247-258)
 inline V3 transPoint(const M4& M, V3 p) {
   float xp = M.at(0, 0) * p.x + M.at(0, 1) * p.y + M.at(0, 2) * p.z + M.at(0, 3);
   float yp = M.at(1, 0) * p.x + M.at(1, 1) * p.y + M.at(1, 2) * p.z + M.at(1, 3);
@@ -117,19 +117,19 @@ inline V3 transPoint(const M4& M, V3 p) {
   if (wp == 1.0f) return mk(xp, yp, zp);
   return mk(xp / wp, yp / wp, zp / wp);
 }
-// transVector (Transform.hs:353-358)
+// transVector (Transform.hs:259-266)
 inline V3 transVector(const M4& M, V3 v) {
   return mk(M.at(0, 0) * v.x + M.at(0, 1) * v.y + M.at(0, 2) * v.z,
             M.at(1, 0) * v.x + M.at(1, 1) * v.y + M.at(1, 2) * v.z,
             M.at(2, 0) * v.x + M.at(2, 1) * v.y + M.at(2, 2) * v.z);
 }
-// transNormal (Transform.hs:361-366): uses the *inverse* matrix, transposed.
+// transNormal (Transform.hs:267-274): uses the *inverse* matrix, transposed.
 inline V3 transNormal(const M4& Inv, V3 n) {
   return mk(Inv.at(0, 0) * n.x + Inv.at(1, 0) * n.y + Inv.at(2, 0) * n.z,
             Inv.at(0, 1) * n.x + Inv.at(1, 1) * n.y + Inv.at(2, 1) * n.z,
             Inv.at(0, 2) * n.x + Inv.at(1, 2) * n.y + Inv.at(2, 2) * n.z);
 }
-inline Ray transRay(const M4& M, const Ray& r) {  // Transform.hs:369-372
+inline Ray transRay(const M4& M, const Ray& r) {  // Transform.hs:275-280
   return Ray{transPoint(M, r.o), transVector(M, r.d), r.tmin, r.tmax};
 }
 

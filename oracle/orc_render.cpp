@@ -5,9 +5,9 @@
 //   Reflection.hs:201-335 (BSDF)             Reflection/{Diffuse,Specular,Microfacet}.hs
 //   Fresnel.hs:31-69   Material.hs:32-96     Light.hs:72-229   Shape.hs:314-410
 //   Montecarlo.hs:40-186                     Camera.hs:49-68   Image.hs:40-61,162-168,250-299
-//   DifferentialGeometry.hs:41-81            Primitive/TriangleMesh.hs:202-287
-//   Rendering.hs:207-246 (passes over the sample extent)
-//   Integrator/DirectLighting.hs:23-63       Material.hs:43-53 (translucentMatte)
+//   DifferentialGeometry.hs:41-81            Primitive/TriangleMesh.hs:119-207
+//   Rendering.hs:111-151 (passes over the sample extent)
+//   Integrator/DirectLighting.hs:23-59       Material.hs:43-53 (translucentMatte)
 //   Texture.hs:191-219 (checkerBoard, graphPaper selectors)
 //   Material.hs:111-127 (mkSubstrate)        Reflection/Microfacet.hs:56-104,136-196
 //                                            (FresnelBlend, the anisotropic distribution)
@@ -41,6 +41,26 @@
 #include "orc_scene.h"
 
 namespace orc {
+
+// Planted mistakes, for the tests that show each closed-form pin fails when
+// the function it pins is wrong (orc_set_mutant; 0 = the restatement as is).
+enum {
+  MUT_NONE = 0,
+  MUT_FR_DIEL_ETA = 1,     // frDielectric: Snell's sin^2 scaled by 1 / etat, not 1 / etat^2
+  MUT_FR_COND_K = 2,       // frConductor: k^2 dropped from eta^2 + k^2
+  MUT_BLINN_NORM = 3,      // Blinn D: (e + 1) / 2pi instead of (e + 2) / 2pi
+  MUT_MF_G = 4,            // Torrance-Sparrow G: factor 2 dropped
+  MUT_TRANS_ETA2 = 5,      // specTrans: f scaled by eta instead of eta^2
+  MUT_TRANS_DIR = 6,       // specTrans: tangential direction not reversed
+  MUT_OREN_SWAP = 7,       // Oren-Nayar: alpha / beta taken from the wrong side
+  MUT_ENV_SIN = 8,         // infinite light pdf: the 1 / sin(theta) Jacobian dropped
+  MUT_ENV_ROW = 9,         // Dist2D rows built from the map's next row
+  MUT_PATH_PC = 10,        // path throughput: 1 / pc dropped
+  MUT_PATH_T = 11,         // path throughput: t' = f (the old t dropped)
+  MUT_POWER = 12,          // balance heuristic instead of the power heuristic
+  MUT_FILTER_CLIP = 13     // film splats not clipped at the tile origin
+};
+static int g_mutant = MUT_NONE;
 
 // ---------------------------------------------------------------- spectra
 using Spec = std::array<float, 16>;
@@ -188,7 +208,7 @@ static void triUVs(const Scene& s, int t, float* uv) {
       uv[2 * k] = s.uvs[2 * o];
       uv[2 * k + 1] = s.uvs[2 * o + 1];
     }
-  } else {  // triangleDefaultUVs (TriangleMesh.hs:199-200)
+  } else {  // triangleDefaultUVs (TriangleMesh.hs:119-120)
     uv[0] = 0; uv[1] = 0; uv[2] = 1; uv[3] = 0; uv[4] = 1; uv[5] = 1;
   }
 }
@@ -224,7 +244,7 @@ static DG triDG(const Scene& s, int t, const Ray& r, float th, float b1, float b
   return d;
 }
 
-// triangleShadingGeometry with o2w = identity (TriangleMesh.hs:202-214, Primitive.hs:65)
+// triangleShadingGeometry with o2w = identity (TriangleMesh.hs:122-134, Primitive.hs:65)
 static DG triShading(const Scene& s, int t, const DG& dgg) {
   V3 n0 = s.normals[s.idx[3 * t]], n1 = s.normals[s.idx[3 * t + 1]], n2 = s.normals[s.idx[3 * t + 2]];
   float b0 = 1.0f - dgg.b1 - dgg.b2;
@@ -268,11 +288,12 @@ static bool intersectFull(const Scene& s, const Ray& r, Isect* is) {
 }
 
 // ------------------------------------------------------------- monte carlo
-static inline float powerHeuristic(float fPdf, float gPdf) {  // Montecarlo.hs:342-345
+static inline float powerHeuristic(float fPdf, float gPdf) {  // Montecarlo.hs:113-116
   float f = 1.0f * fPdf, g = 1.0f * gPdf;
+  if (g_mutant == MUT_POWER) return f / (f + g);
   return (f * f) / (f * f + g * g);
 }
-static void concentricSampleDisk(float u1, float u2, float* ox, float* oy) {  // Montecarlo.hs:389-406
+static void concentricSampleDisk(float u1, float u2, float* ox, float* oy) {  // Montecarlo.hs:160-179
   float sx = u1 * 2.0f - 1.0f, sy = u2 * 2.0f - 1.0f;
   if (sx == 0.0f && sy == 0.0f) {
     *ox = 0.0f;
@@ -299,7 +320,7 @@ static void concentricSampleDisk(float u1, float u2, float* ox, float* oy) {  //
   *ox = r * std::cos(theta);
   *oy = r * std::sin(theta);
 }
-static V3 cosineSampleHemisphere(float u1, float u2) {  // Montecarlo.hs:375-378
+static V3 cosineSampleHemisphere(float u1, float u2) {  // Montecarlo.hs:146-158
   float x, y;
   concentricSampleDisk(u1, u2, &x, &y);
   return mk(x, y, std::sqrt(hmax(0.0f, 1.0f - x * x - y * y)));
@@ -312,14 +333,14 @@ static V3 uniformSampleCone(const Frame& f, float cosThetaMax, float u1, float u
          scale(f.n, cosTheta);
 }
 static float uniformConePdf(float c) { return c >= 1.0f ? 0.0f : 1.0f / (kTwoPi * (1.0f - c)); }
-static V3 uniformSampleSphere(float u1, float u2) {  // Montecarlo.hs:410-415
+static V3 uniformSampleSphere(float u1, float u2) {  // Montecarlo.hs:181-186
   float u = u1 * 2.0f - 1.0f;
   float s = std::sqrt(1.0f - u * u);
   float omega = u2 * 2.0f * kPi;
   return mk(s * std::cos(omega), s * std::sin(omega), u);
 }
 
-// Dist1D / Dist2D (Montecarlo.hs:263-333)
+// Dist1D / Dist2D (Montecarlo.hs:34-104)
 struct Dist1D {
   std::vector<float> func, cdf;
   float fi;
@@ -382,7 +403,8 @@ static Dist2D buildDist(const LightRec& l) {  // mkDist2D (texSize rmap) (sY . e
   std::vector<float> fi;
   for (int v = 0; v < l.h; ++v) {
     std::vector<float> row(l.w);
-    for (int u = 0; u < l.w; ++u) row[u] = sY(texEval(l, (float)u / sx, (float)v / sy));
+    const int vr = g_mutant == MUT_ENV_ROW ? (v + 1) % l.h : v;
+    for (int u = 0; u < l.w; ++u) row[u] = sY(texEval(l, (float)u / sx, (float)vr / sy));
     d.cond.push_back(mkDist1D(row));
     fi.push_back(d.cond.back().fi);
   }
@@ -515,7 +537,7 @@ static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, 
     mapPdf = pdf0 * pdf1;
     Ray r0{mk(0, 0, 0), mk(0, 1, 0), 0.0f, 1.0f};
     if (mapPdf == 0.0f) return LightSample{sconst(0), mk(0, 1, 0), r0, 0.0f};
-    float phi = uv[0] * 2.0f * kPi, theta = uv[1] * kPi;  // cartToSph (Types.hs:512-514)
+    float phi = uv[0] * 2.0f * kPi, theta = uv[1] * kPi;  // cartToSph (Types.hs:31-33)
     float sint = std::sin(theta);
     if (sint == 0.0f) return LightSample{sconst(0), mk(0, 1, 0), r0, 0.0f};
     V3 dl = mk(std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta));
@@ -523,7 +545,7 @@ static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, 
     ls.li = texEval(l, uv[0], uv[1]);
     ls.wi = wi;
     ls.ray = Ray{pW, wi, e, kInf};
-    ls.pdf = mapPdf / (2.0f * kPi * kPi * sint);
+    ls.pdf = g_mutant == MUT_ENV_SIN ? mapPdf / (2.0f * kPi * kPi) : mapPdf / (2.0f * kPi * kPi * sint);
     return ls;
   }
   const ShapePrim& sp = c.s->shapes[l.shape];  // Light.hs:152-160
@@ -557,7 +579,7 @@ static float lightPdf(const Ctx& c, const LightRec& l, V3 p, V3 wi) {  // Light.
     float pd;
     if (d.marg.fi * d.cond[iv].fi == 0.0f) pd = 0.0f;
     else pd = (d.cond[iv].func[iu] * d.marg.func[iv]) / (d.cond[iv].fi * d.marg.fi);
-    return pd / (2.0f * kPi * kPi * sint);
+    return g_mutant == MUT_ENV_SIN ? pd / (2.0f * kPi * kPi) : pd / (2.0f * kPi * kPi * sint);
   }
   const ShapePrim& sp = c.s->shapes[l.shape];
   return shapePdf(sp.shape, transPoint(sp.w2o, p), transVector(sp.w2o, wi));
@@ -613,9 +635,10 @@ static inline float sinPhi(V3 v) {
 static inline bool sameHemi(V3 a, V3 b) { return a.z * b.z > 0.0f; }
 static inline V3 toSameHemi(V3 wo, V3 wi) { return wo.z < 0.0f ? mk(wi.x, wi.y, -wi.z) : wi; }
 
-static Spec frDielectric(float etai, float etat, float cosi) {  // Fresnel.hs:227-251
+static Spec frDielectric(float etai, float etat, float cosi) {  // Fresnel.hs:21-55
   float c = hmax(0.0f, 1.0f - cosi * cosi);
-  float costp = cosi > 0.0f ? c / (etat * etat) : c * (etat * etat);
+  const float e2 = g_mutant == MUT_FR_DIEL_ETA ? etat : etat * etat;
+  float costp = cosi > 0.0f ? c / e2 : c * e2;
   float cost = std::sqrt(1.0f - clampf(costp, 0.0f, 1.0f));
   float ac = std::fabs(cosi);
   float eta = etat / etai;
@@ -625,9 +648,9 @@ static Spec frDielectric(float etai, float etat, float cosi) {  // Fresnel.hs:22
   float rPerp = (ac - rPerpp) / (ac + rPerpp);
   return sconst((rParl * rParl + rPerp * rPerp) * 0.5f);
 }
-static Spec frConductor(const Spec& eta, const Spec& k, float cosi) {  // Fresnel.hs:254-266
+static Spec frConductor(const Spec& eta, const Spec& k, float cosi) {  // Fresnel.hs:58-70
   float ac = std::fabs(cosi);
-  Spec tmpF = eta * eta + k * k;
+  Spec tmpF = g_mutant == MUT_FR_COND_K ? eta * eta : eta * eta + k * k;
   Spec tmp = sscale(tmpF, ac * ac);
   Spec ec2 = sscale(eta, 2.0f * ac);
   Spec c2 = sconst(ac * ac);
@@ -646,9 +669,12 @@ static float cosPdf(V3 wo, V3 wi) { return sameHemi(wo, wi) ? kInvPi * absCosT(w
 
 static float mfG(V3 wo, V3 wi, V3 wh) {  // Microfacet.hs:113-120
   float nDotWh = absCosT(wh), nDotWo = absCosT(wo), nDotWi = absCosT(wi), woDotWh = absDot(wo, wh);
-  return hmin(1.0f, hmin(2.0f * nDotWh * nDotWo / woDotWh, 2.0f * nDotWh * nDotWi / woDotWh));
+  const float two = g_mutant == MUT_MF_G ? 1.0f : 2.0f;
+  return hmin(1.0f, hmin(two * nDotWh * nDotWo / woDotWh, two * nDotWh * nDotWi / woDotWh));
 }
-static float blinnD(float e, V3 wh) { return (e + 2.0f) * kInvTwoPi * std::pow(absCosT(wh), e); }
+static float blinnD(float e, V3 wh) {
+  return (g_mutant == MUT_BLINN_NORM ? e + 1.0f : e + 2.0f) * kInvTwoPi * std::pow(absCosT(wh), e);
+}
 static float blinnPdf(float e, V3 wh) { return (e + 1.0f) * std::pow(absCosT(wh), e) * kInvTwoPi; }
 
 // The anisotropic distribution (Microfacet.hs:136-196).
@@ -722,13 +748,13 @@ static float fbPdf(const Bxdf& b, V3 wo, V3 wi) {  // p (Microfacet.hs:101-104)
   return 0.5f * (absCosT(wi) * kInvPi + anisoPdf(b.ex, b.ey, wh) / (4.0f * absDot(wo, wh)));
 }
 
-static Spec orenNayar(const Bxdf& b, V3 wo, V3 wi) {  // Diffuse.hs:431-443
+static Spec orenNayar(const Bxdf& b, V3 wo, V3 wi) {  // Diffuse.hs:53-65
   float sinti = sinT(wi), sinto = sinT(wo);
   float maxcos = 0.0f;
   if (sinti > 1e-4f && sinto > 1e-4f)
     maxcos = hmax(0.0f, cosPhi(wi) * cosPhi(wo) + sinPhi(wi) * sinPhi(wo));
   float sina, tanb;
-  if (absCosT(wi) > absCosT(wo)) { sina = sinto; tanb = sinti / absCosT(wi); }
+  if ((absCosT(wi) > absCosT(wo)) != (g_mutant == MUT_OREN_SWAP)) { sina = sinto; tanb = sinti / absCosT(wi); }
   else { sina = sinti; tanb = sinto / absCosT(wo); }
   return sscale(b.r, b.A + b.B * maxcos * sina * tanb);
 }
@@ -773,7 +799,7 @@ static float bxPdf(const Bxdf& b, V3 wo, V3 wi) {
 static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, float* pdf) {
   switch (b.kind) {
     case K_LAMBERT:
-    case K_OREN: {  // Diffuse.hs:392-400,416-420
+    case K_OREN: {  // Diffuse.hs:14-22,38-42
       V3 w = toSameHemi(wo, cosineSampleHemisphere(u1, u2));
       if (sameHemi(wo, w)) {
         *f = b.kind == K_LAMBERT ? b.r : orenNayar(b, wo, w);
@@ -833,12 +859,12 @@ static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, 
       *f = sscale(fbEval(b, wo, w), 1.0f / *pdf);
       return;
     }
-    case K_SPECR:  // Specular.hs:460-465
+    case K_SPECR:  // Specular.hs:11-20
       *f = b.r * fresnel(b, cosT(wo));
       *wi = mk(-wo.x, -wo.y, wo.z);
       *pdf = 1.0f;
       return;
-    default: {  // specTrans (Specular.hs:478-502), adj = False
+    default: {  // specTrans (Specular.hs:33-58), adj = False
       bool entering = cosT(wo) > 0.0f;
       float ei = entering ? b.ei : b.et, et = entering ? b.et : b.ei;
       float sini2 = sinT2(wo);
@@ -853,9 +879,9 @@ static void bxSample(const Bxdf& b, V3 wo, float u1, float u2, Spec* f, V3* wi, 
       }
       float c = std::sqrt(hmax(0.0f, 1.0f - sint2));
       float cost = entering ? -c : c;
-      *wi = mk(eta * (-wo.x), eta * (-wo.y), cost);
+      *wi = g_mutant == MUT_TRANS_DIR ? mk(eta * wo.x, eta * wo.y, cost) : mk(eta * (-wo.x), eta * (-wo.y), cost);
       Spec fr = frDielectric(ei, et, cost);
-      *f = sscale((sconst(1.0f) - fr) * b.r, eta2);
+      *f = sscale((sconst(1.0f) - fr) * b.r, g_mutant == MUT_TRANS_ETA2 ? eta : eta2);
       *pdf = 1.0f;
       return;
     }
@@ -916,7 +942,7 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
     return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
   };
   switch (m.type) {
-    case BGPU_MAT_MATTE: {  // Material.hs:307-316
+    case BGPU_MAT_MATTE: {  // Material.hs:32-41
       float sig = m.scalar[0];
       Bxdf x{};
       x.r = s0;
@@ -933,7 +959,7 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
       b.c[b.n++] = x;
       break;
     }
-    case BGPU_MAT_PLASTIC: {  // Material.hs:351-361
+    case BGPU_MAT_PLASTIC: {  // Material.hs:76-87
       Bxdf d{};
       d.kind = K_LAMBERT;
       d.type = BX_R | BX_D;
@@ -950,7 +976,7 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
       b.c[b.n++] = g;
       break;
     }
-    case BGPU_MAT_METAL: {  // Material.hs:364-371
+    case BGPU_MAT_METAL: {  // Material.hs:89-96
       Bxdf g{};
       g.kind = K_MF;
       g.type = BX_R | BX_G;
@@ -962,7 +988,7 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
       b.c[b.n++] = g;
       break;
     }
-    case BGPU_MAT_GLASS: {  // Material.hs:331-341
+    case BGPU_MAT_GLASS: {  // Material.hs:56-66
       float ior = m.scalar[0];
       Bxdf r{};
       r.kind = K_SPECR;
@@ -1017,7 +1043,7 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
       }
       break;
     }
-    case BGPU_MAT_MIRROR: {  // Material.hs:343-347
+    case BGPU_MAT_MIRROR: {  // Material.hs:68-74
       Bxdf r{};
       r.kind = K_SPECR;
       r.type = BX_R | BX_S;
@@ -1193,7 +1219,7 @@ static Spec li(Ctx& c, const Ray& camRay, const PixelSampler& ps, uint32_t n, in
     }
     Spec lHere = intl + ld;
     L = L + t * lHere;
-    float pc = depth <= rrDepth ? 1.0f : hmin(0.75f, sY(t));  // Path.hs:70
+    float pc = depth <= rrDepth ? 1.0f : hmin(0.75f, sY(t));  // Path.hs:68
     float x = ps.get1(n, 3 + 4 * d);
     if (x > pc) return L;
     float uc = ps.get1(n, 0 + 4 * d);
@@ -1202,14 +1228,16 @@ static Spec li(Ctx& c, const Ray& camRay, const PixelSampler& ps, uint32_t n, in
     if (smp.pdf == 0.0f || isBlack(smp.f)) return L;
     ray = Ray{p, smp.wi, eps, kInf};
     spec = (smp.type & BX_S) != 0;
-    t = sscale(smp.f * t, 1.0f / pc);
+    if (g_mutant == MUT_PATH_PC) t = smp.f * t;
+    else if (g_mutant == MUT_PATH_T) t = sscale(smp.f, 1.0f / pc);
+    else t = sscale(smp.f * t, 1.0f / pc);
     depth++;
     c.rays++;
     hit = intersectFull(*c.s, ray, &is);
   }
 }
 
-// DirectLighting (Integrator/DirectLighting.hs:23-63): at every hit the
+// DirectLighting (Integrator/DirectLighting.hs:23-59): at every hit the
 // emission towards wo, sampleOneLight with the sample dimensions 2d / 2d + 1,
 // and the specular reflection and transmission continuations (uComp 0.5,
 // uDir (0.5, 0.5)) up to maxDepth, each weighted by its sampled f; escaping
@@ -1241,7 +1269,7 @@ static Spec liDirect(Ctx& c, const Ray& ray, const PixelSampler& ps, uint32_t n,
   V3 p = bsdf.p, nrm = bsdf.cs.n, wo = -ray.d;
   float e = is.eps;
   Spec l = sampleOneLightAt(c, p, e, nrm, wo, bsdf, uln, uld1, uld2, ubc, ubd1, ubd2);
-  auto cont = [&](int flags) {  // cont (DirectLighting.hs:49-63)
+  auto cont = [&](int flags) {  // cont (DirectLighting.hs:45-59)
     if (d + 1 == md) return sconst(0);
     BsdfSample smp = sampleBsdf(bsdf, wo, 0.5f, 0.5f, 0.5f, flags);
     if (smp.pdf == 0.0f) return sconst(0);
@@ -1387,17 +1415,75 @@ void orc_aniso(float ex, float ey, int n, const float* u, float* wh, float* d, f
   }
 }
 
-// Component 0 of material m in its local shading frame, for n directions wo
-// (3 n floats): op 0 evaluates bxdfEval wo wi into f (16 n) and bxdfPdf wo wi
-// into pdf; op 1 samples with (u1, u2) = u[2 i..], writing wi, f and pdf.
-int orc_bxdf(const bgpu_material* m, int op, int n, const float* wo, const float* u, float* wi, float* f,
-             float* pdf) {
+// Selects a planted mistake (the MUT_* list); 0 restores the restatement.
+void orc_set_mutant(int m) { orc::g_mutant = m; }
+
+// frDielectric etai etat cosi (kind 0) or frConductor eta k cosi (kind 1) as
+// 16 bands (Fresnel.hs:21-70).
+void orc_fresnel(int kind, float etai, float etat, const float* eta, const float* k, float cosi, float* out) {
+  using namespace orc;
+  Spec r = kind == 0 ? frDielectric(etai, etat, cosi) : frConductor(specOf(eta), specOf(k), cosi);
+  for (int b = 0; b < 16; ++b) out[b] = r[b];
+}
+
+// The Blinn distribution of exponent e at n half vectors wh (3 n floats):
+// D and the half-vector sampling pdf (Microfacet.hs:175-196 via mkBlinn).
+void orc_blinn(float e, int n, const float* wh, float* d, float* pdf) {
+  using namespace orc;
+  for (int i = 0; i < n; ++i) {
+    V3 w = mk(wh[3 * i], wh[3 * i + 1], wh[3 * i + 2]);
+    d[i] = blinnD(e, w);
+    pdf[i] = blinnPdf(e, w);
+  }
+}
+
+// An infinite area light with a w x h spectral map (mkInfiniteAreaLight,
+// Light.hs:72-82,130-141,215-229): op 0 samples n directions with (u1, u2) =
+// u[2 i..] into wi (3 n floats), their pdf and radiance li (16 n); op 1
+// evaluates the pdf and le at the given wi.  w2l and l2w are row-major 4x4.
+void orc_envlight(int w, int h, const float* map, const float* w2l, const float* l2w, int op, int n,
+                  const float* u, float* wi, float* pdf, float* li) {
+  using namespace orc;
+  LightRec l{};
+  l.type = 1;
+  l.shape = -1;
+  l.id = -1;
+  l.w = w;
+  l.h = h;
+  l.map = map;
+  std::memcpy(l.w2l.m, w2l, sizeof(l.w2l.m));
+  std::memcpy(l.l2w.m, l2w, sizeof(l.l2w.m));
+  l.dist = buildDist(l);
+  Ctx c;
+  c.s = nullptr;
+  for (int i = 0; i < n; ++i) {
+    Spec le;
+    if (op == 0) {
+      LightSample ls = sampleLight(c, l, mk(0, 0, 0), 0.0f, u[2 * i], u[2 * i + 1]);
+      wi[3 * i] = ls.wi.x, wi[3 * i + 1] = ls.wi.y, wi[3 * i + 2] = ls.wi.z;
+      pdf[i] = ls.pdf;
+      le = ls.li;
+    } else {
+      V3 d = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+      pdf[i] = lightPdf(c, l, mk(0, 0, 0), d);
+      le = lightLe(l, Ray{mk(0, 0, 0), d, 0.0f, kInf});
+    }
+    for (int b = 0; b < 16; ++b) li[16 * i + b] = le[b];
+  }
+}
+
+// Component `comp` of material m in its local shading frame, for n
+// directions wo (3 n floats): op 0 evaluates bxdfEval wo wi into f (16 n) and
+// bxdfPdf wo wi into pdf; op 1 samples with (u1, u2) = u[2 i..], writing wi,
+// f and pdf.  Returns -1 when the material has no such component.
+int orc_bxdf_comp(const bgpu_material* m, int comp, int op, int n, const float* wo, const float* u, float* wi,
+                  float* f, float* pdf) {
   using namespace orc;
   Bsdf b;
   b.n = 0;
   addComponents(*m, specOf(m->spec[0]), specOf(m->spec[1]), b);
-  if (b.n == 0) return -1;
-  const Bxdf& x = b.c[0];
+  if (comp < 0 || comp >= b.n) return -1;
+  const Bxdf& x = b.c[comp];
   for (int i = 0; i < n; ++i) {
     V3 o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
     Spec s;
@@ -1413,6 +1499,12 @@ int orc_bxdf(const bgpu_material* m, int op, int n, const float* wo, const float
     for (int k = 0; k < 16; ++k) f[16 * i + k] = s[k];
   }
   return 0;
+}
+
+// Component 0 (orc_bxdf_comp with comp 0).
+int orc_bxdf(const bgpu_material* m, int op, int n, const float* wo, const float* u, float* wi, float* f,
+             float* pdf) {
+  return orc_bxdf_comp(m, 0, op, n, wo, u, wi, f, pdf);
 }
 
 // Sampler values of every sample n of pixel (pix, pix) in `pass`, for the
@@ -1465,7 +1557,7 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
     c.shapeLight[i] = (int)c.lights.size();
     c.lights.push_back(l);
   }
-  // a job's camera and film follow its imageSize (Rendering.hs:133-137)
+  // a job's camera and film follow its imageSize (IO/CameraParser.hs:30-38, Rendering.hs:43-44)
   const bgpu_film& film = job->film ? *job->film : s.film;
   const bgpu_camera& camera = job->camera ? *job->camera : s.camera;
   filterSize(film, &c.fw, &c.fh);

@@ -1,6 +1,6 @@
 // ORACLE -- test infrastructure only (see orc_math.h).
 // Counter-based restatement of the stratified sampler of
-// src/lib/Graphics/Bling/Sampling.hs:241-377 (runSample Stratified,
+// src/lib/Graphics/Bling/Sampling.hs:85-221 (runSample Stratified,
 // stratified1D/2D, fill, rnd', rnd2D').  The reference draws its jitter and
 // shuffles from an entropy-seeded MWC stream; here every value is a pure
 // function of (job seed, pass, pixel, sample, dimension):
@@ -61,29 +61,29 @@ enum : uint32_t {
   D_R1 = 0x6000, D_R2X = 0x7000, D_R2Y = 0x8000
 };
 
-static const float kAlmostOne = 0x1.fffffep-1f;  // Sampling.hs:310-311
+static const float kAlmostOne = 0x1.fffffep-1f;  // Sampling.hs:154-155
 
 struct PixelSampler {
   uint64_t key;
   uint32_t nu, nv, N;
   uint32_t n1d, n2d;  // table sizes 4*sd, 3*sd (Integrator/Path.hs:30-39)
 
-  // stratified2D entry i (Sampling.hs:320-327): (u, v) = quotRem i nu
+  // stratified2D entry i (Sampling.hs:164-171): (u, v) = quotRem i nu
   void strat2(uint32_t i, float ju, float jv, float* u, float* v) const {
     float du = 1.0f / (float)nu, dv = 1.0f / (float)nv;
     float a = ((float)(i / nu) + ju) * du, b = ((float)(i % nu) + jv) * dv;
     *u = a < kAlmostOne ? a : kAlmostOne;
     *v = b < kAlmostOne ? b : kAlmostOne;
   }
-  // camera sample n (pixel offsets, not shuffled; Sampling.hs:273,284-288)
+  // camera sample n (pixel offsets, not shuffled; Sampling.hs:117,128-132)
   void camera(uint32_t n, float* ox, float* oy) const {
     strat2(n, uni(rngBits(key, D_CAM_X, n)), uni(rngBits(key, D_CAM_Y, n)), ox, oy);
   }
-  void lens(uint32_t n, float* u, float* v) const {  // shuffled (Sampling.hs:274-277)
+  void lens(uint32_t n, float* u, float* v) const {  // shuffled (Sampling.hs:118-121)
     uint32_t i = permute(n, N, (uint32_t)rngBits(key, D_LENS_P, 0));
     strat2(i, uni(rngBits(key, D_LENS_X, i)), uni(rngBits(key, D_LENS_Y, i)), u, v);
   }
-  // rnd' k (Sampling.hs:359-367)
+  // rnd' k (Sampling.hs:203-211)
   float get1(uint32_t n, uint32_t k) const {
     if (k < n1d) {
       uint32_t i = permute(n, N, (uint32_t)rngBits(key, D_T1_P + k, 0));
@@ -93,7 +93,7 @@ struct PixelSampler {
     }
     return uni(rngBits(key, D_R1 + k, n));
   }
-  // rnd2D' k (Sampling.hs:369-377)
+  // rnd2D' k (Sampling.hs:213-221)
   void get2(uint32_t n, uint32_t k, float* u, float* v) const {
     if (k < n2d) {
       uint32_t i = permute(n, N, (uint32_t)rngBits(key, D_T2_P + k, 0));

@@ -33,12 +33,12 @@ bool primIntersect(const Scene& s, int id, const Ray& r, Hit* h) {
     h->b1 = th.b1;
     h->b2 = th.b2;
     h->prim = id;
-    h->eps = 1e-3f * th.t;  // TriangleMesh.hs:286
+    h->eps = 1e-3f * th.t;  // TriangleMesh.hs:206
     h->axis = 0;
     return true;
   }
   const ShapePrim& sp = s.shapes[id - s.ntri];
-  Ray ro = transRay(sp.w2o, r);  // Geometry.hs:320-323
+  Ray ro = transRay(sp.w2o, r);  // Geometry.hs:33-36
   ShapeHit sh;
   if (!shapeIntersect(sp.shape, ro, &sh)) return false;
   h->t = sh.t;
@@ -57,7 +57,7 @@ bool primIntersects(const Scene& s, int id, const Ray& r) {
     return triangleIntersects(a, b, c, r);
   }
   const ShapePrim& sp = s.shapes[id - s.ntri];
-  return shapeIntersects(sp.shape, transRay(sp.w2o, r));  // Geometry.hs:313-314
+  return shapeIntersects(sp.shape, transRay(sp.w2o, r));  // Geometry.hs:26-27
 }
 
 // (t, id) lexicographic minimum; counts exact ties at the running minimum.
@@ -265,7 +265,7 @@ Scene* buildScene(const bgpu_scene_desc* d, const char** err) {
   }
   if (d->uvs) s->uvs.assign(d->uvs, d->uvs + 2 * (size_t)d->num_vertices);
   s->idx.assign(d->indices, d->indices + 3 * (size_t)d->num_triangles);
-  for (int v : s->idx) {  // mkTriangleMesh checks (TriangleMesh.hs:127-133)
+  for (int v : s->idx) {  // mkTriangleMesh checks (TriangleMesh.hs:46-55)
     if (v < 0) { *err = "mkTriangleMesh: contains negative indices"; delete s; return nullptr; }
     if ((uint32_t)v >= d->num_vertices) {
       *err = "mkTriangleMesh: contains out of bounds indices";

@@ -7,7 +7,7 @@
 // The "nearest hit over all prims" is defined structure-independently as the
 // minimum t over all primitives whose own test accepts the ray, ties broken
 // by the lowest primitive id (the reference's winner at an exact tie depends
-// on kd-tree visit order, Primitive.hs:29-43 with TriangleMesh.hs:246).
+// on kd-tree visit order, Primitive.hs:29-43 with TriangleMesh.hs:166).
 #pragma once
 #include <vector>
 

@@ -136,31 +136,107 @@ def aniso(ex, ey, wh=None, u=None):
     return (wh, d, pdf) if u is not None else (d, pdf)
 
 
-def bxdf_eval(material, wo, wi):
-    """Component 0 of `material` (an abi.Material) in its local frame:
+def bxdf_eval(material, wo, wi, comp=0):
+    """Component `comp` of `material` (an abi.Material) in its local frame:
     bxdfEval wo wi as (n, 16) spectra and bxdfPdf wo wi."""
-    return _bxdf(material, 0, wo, wi, None)[1:]
+    return _bxdf(material, 0, wo, wi, None, comp)[1:]
 
 
-def bxdf_sample(material, wo, u):
-    """bxdfSample (adj False) of component 0 of `material` at sample points u
-    (n, 2): returns (wi, f, pdf)."""
-    return _bxdf(material, 1, wo, None, u)
+def bxdf_sample(material, wo, u, comp=0):
+    """bxdfSample (adj False) of component `comp` of `material` at sample
+    points u (n, 2): returns (wi, f, pdf)."""
+    return _bxdf(material, 1, wo, None, u, comp)
 
 
-def _bxdf(material, op, wo, wi, u):
+def _bxdf(material, op, wo, wi, u, comp=0):
     lib = load()
     vp = C.c_void_p
-    lib.orc_bxdf.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
-    lib.orc_bxdf.restype = C.c_int
+    lib.orc_bxdf_comp.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    lib.orc_bxdf_comp.restype = C.c_int
     wo = np.ascontiguousarray(wo, np.float32)
     n = len(wo)
     wi = np.zeros((n, 3), np.float32) if wi is None else np.ascontiguousarray(wi, np.float32).copy()
     u = None if u is None else np.ascontiguousarray(u, np.float32)
     f = np.zeros((n, 16), np.float32)
     pdf = np.zeros(n, np.float32)
-    rc = lib.orc_bxdf(C.addressof(material), op, n, wo.ctypes.data, None if u is None else u.ctypes.data,
-                      wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    rc = lib.orc_bxdf_comp(C.addressof(material), comp, op, n, wo.ctypes.data,
+                           None if u is None else u.ctypes.data, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
     if rc != 0:
-        raise ValueError("material has no BxDF")
+        raise ValueError("material has no BxDF component %d" % comp)
     return wi, f, pdf
+
+
+def fresnel_dielectric(etai, etat, cosi):
+    """frDielectric etai etat cosi (Fresnel.hs:21-55), band 0."""
+    lib = load()
+    lib.orc_fresnel.argtypes = [C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    out = np.zeros(16, np.float32)
+    lib.orc_fresnel(0, etai, etat, None, None, cosi, out.ctypes.data)
+    return out[0]
+
+
+def fresnel_conductor(eta, k, cosi):
+    """frConductor eta k cosi (Fresnel.hs:58-70) for 16-band eta and k."""
+    lib = load()
+    lib.orc_fresnel.argtypes = [C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    eta = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, np.float32), (16,)))
+    k = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.float32), (16,)))
+    out = np.zeros(16, np.float32)
+    lib.orc_fresnel(1, 0.0, 0.0, eta.ctypes.data, k.ctypes.data, cosi, out.ctypes.data)
+    return out
+
+
+def blinn(e, wh):
+    """The Blinn distribution's D and half-vector pdf at half vectors wh (n, 3)."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_blinn.argtypes = [C.c_float, C.c_int, vp, vp, vp]
+    wh = np.ascontiguousarray(wh, np.float32)
+    d = np.zeros(len(wh), np.float32)
+    pdf = np.zeros(len(wh), np.float32)
+    lib.orc_blinn(e, len(wh), wh.ctypes.data, d.ctypes.data, pdf.ctypes.data)
+    return d, pdf
+
+
+def envlight(radiance_map, u=None, wi=None, w2l=None):
+    """An infinite light with the (h, w, 16) map: with u (n, 2) its samples
+    (wi, pdf, li); with wi (n, 3) its (pdf, le) there."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_envlight.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
+    m = np.ascontiguousarray(radiance_map, np.float32)
+    h, w = m.shape[:2]
+    eye = np.eye(4, dtype=np.float32)
+    w2l = eye if w2l is None else np.ascontiguousarray(w2l, np.float32)
+    l2w = np.ascontiguousarray(np.linalg.inv(w2l.astype(np.float64)).astype(np.float32))
+    if u is not None:
+        u = np.ascontiguousarray(u, np.float32)
+        n = len(u)
+        wi = np.zeros((n, 3), np.float32)
+    else:
+        wi = np.ascontiguousarray(wi, np.float32)
+        n = len(wi)
+    pdf = np.zeros(n, np.float32)
+    li = np.zeros((n, 16), np.float32)
+    lib.orc_envlight(w, h, m.ctypes.data, w2l.ctypes.data, l2w.ctypes.data, 0 if u is not None else 1, n,
+                     None if u is None else u.ctypes.data, wi.ctypes.data, pdf.ctypes.data, li.ctypes.data)
+    return (wi, pdf, li) if u is not None else (pdf, li)
+
+
+class mutant:
+    """Context manager planting mistake `m` (orc_render.cpp's MUT_* list) in
+    the oracle for the duration of a block."""
+    FR_DIEL_ETA, FR_COND_K, BLINN_NORM, MF_G, TRANS_ETA2, TRANS_DIR, OREN_SWAP = 1, 2, 3, 4, 5, 6, 7
+    ENV_SIN, ENV_ROW, PATH_PC, PATH_T, POWER, FILTER_CLIP = 8, 9, 10, 11, 12, 13
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib = load()
+        lib.orc_set_mutant.argtypes = [C.c_int]
+        lib.orc_set_mutant(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        load().orc_set_mutant(0)

@@ -181,3 +181,41 @@ def grazing_spheres(n_sph=400, n=60000, seed=17, k0=2e-3, scaled=False):
     rays[:, 3:6] = d
     rays[:, 7] = np.inf
     return sb.build(), rays, cen, rad
+
+
+def material_abi(m):
+    """The bgpu_material record of a scene.Material."""
+    a = abi.Material()
+    a.type = m.kind
+    a.scalar[0], a.scalar[1], a.scalar[2] = m.s0, m.s1, m.s2
+    for k in range(16):
+        a.spec[0][k], a.spec[1][k], a.spec[2][k] = m.spec0[k], m.spec1[k], m.spec2[k]
+    return a
+
+
+def frame(n, p):
+    """Object-to-world transform whose z axis is the unit normal n, placed at p."""
+    n = np.asarray(n, np.float64)
+    n = n / np.linalg.norm(n)
+    a = np.array([0.0, 1.0, 0.0]) if abs(n[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    t1 = np.cross(a, n)
+    t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    rows = [[t1[0], t2[0], n[0], p[0]], [t1[1], t2[1], n[1], p[1]], [t1[2], t2[2], n[2], p[2]], [0, 0, 0, 1]]
+    return T.from_matrix(rows)
+
+
+def two_mirrors_under_sky(r=0.9, sky=2.0, res=4, fov=1.0):
+    """A camera at the origin looks along +z at mirror 1 (normal (-1, 0, -1)),
+    which sends its rays along -x to mirror 2 at x = -10 (normal (1, 1, 0)),
+    which sends them up (+y) into a uniform sky.  Both mirrors reflect r with
+    no Fresnel term; every camera ray takes this two-bounce path."""
+    sb = SceneBuilder()
+    mir = sb.material(Material.mirror(const(r)))
+    sb.shape(abi.SHAPE_QUAD, (2.0, 2.0), o2w=frame((-1.0, 0.0, -1.0), (0.0, 0.0, 5.0)), material=mir)
+    sb.shape(abi.SHAPE_QUAD, (3.0, 3.0), o2w=frame((1.0, 1.0, 0.0), (-10.0, 0.0, 5.0)), material=mir)
+    sb.infinite_light(sky_map(const(sky), 4, 2))
+    c2w = T.look_at((0.0, 0.0, 0.0), (0.0, 0.0, 5.0), (0, 1, 0))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
+    sb.set_film(res, res)
+    return sb.build()

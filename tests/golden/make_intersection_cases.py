@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Writes tests/golden/intersection_cases.json: ray/primitive cases whose
 results follow in closed form from the reference's formulas
-(Primitive/TriangleMesh.hs:240-261 Moeller-Trumbore, Shape.hs:81-284 shapes,
+(Primitive/TriangleMesh.hs:160-207 Moeller-Trumbore, Shape.hs:81-284 shapes,
 Math.hs:124-139 solveQuadric), with every quantity dyadic so binary32
 arithmetic is exact.  Expected values are computed here with exact rationals;
 nothing is computed by the oracle or the GPU.

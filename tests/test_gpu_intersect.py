@@ -80,7 +80,7 @@ def test_ray_intervals_and_edge_batches(gpu_ctx):
     rng = np.random.default_rng(9)
     rays[:, 6] = rng.uniform(0, 0.5, len(rays)) * (rng.random(len(rays)) < 0.5)
     rays[:, 7] = np.where(rng.random(len(rays)) < 0.5, rng.uniform(0.1, 2.0, len(rays)), np.inf)
-    # non-normalised directions, as transformed rays carry (Transform.hs:369-372)
+    # non-normalised directions, as transformed rays carry (Transform.hs:275-280)
     rays[:, 3:6] *= rng.uniform(0.2, 3.0, (len(rays), 1)).astype(np.float32)
     g, o, gocc, oocc = run_both(gpu_ctx, flat, rays, brute=True)
     assert_same(g, o, "intervals")

@@ -431,7 +431,9 @@ def test_direct_lighting_matches_oracle(gpu_ctx):
     flat = cases.materials_scene(res=40, env=True)
     spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=0, passes=4, seed=41, variance=True,
                    integrator="direct")
+    before = gpu_ctx.stats()["branches_dropped"]
     g, gsq = render_gpu(gpu_ctx, flat, spec)
+    assert gpu_ctx.stats()["branches_dropped"] == before, "a pending specular branch was dropped"
     o, osq, _ = pyoracle.render(flat, spec)
     assert np.isfinite(g).all()
     assert np.array_equal(g[..., 0], o[..., 0])

@@ -1,0 +1,408 @@
+"""Closed-form pins of the oracle's Fresnel terms, BSDFs, environment-light
+sampling and path recursion, each against what the mathematics fixes
+independently of the restated code:
+
+- frDielectric 1 n (Fresnel.hs:21-55) is the textbook unpolarised Fresnel
+  reflectance seen from either side, total internal reflection included;
+- frConductor (Fresnel.hs:58-70) at normal incidence is
+  ((eta-1)^2 + k^2) / ((eta+1)^2 + k^2), and 1 at grazing incidence;
+- the Blinn distribution (Microfacet.hs:175-196) and its sampling pdf are
+  normalised, and the Torrance-Sparrow value (Microfacet.hs:19-54) is
+  F D G / (4 cos_i) with G = min(1, 2 cos_i) when wo is the normal; it is
+  reciprocal, its pdf integrates to what the half-vector Jacobian predicts,
+  and its samples follow the pdf;
+- specRefl / specTrans (Reflection/Specular.hs:11-58) obey Snell's law and
+  R + T / eta^2 = 1 entering the glass; leaving it they reproduce the
+  reference's constant transmission Fresnel term and total internal
+  reflection;
+- Oren-Nayar (Reflection/Diffuse.hs:29-65) is the published qualitative
+  model, reciprocal, and Lambert as sigma -> 0;
+- the infinite light's Dist2D (Montecarlo.hs:75-104, Light.hs:72-82,130-141,
+  215-229) has a pdf that integrates to 1 over the sphere and samples that
+  follow the map's luminance;
+- the path recursion (Integrator/Path.hs:66-87) with and without Russian
+  roulette: two mirrors under a uniform sky give r^2 L (1 + w).
+
+`test_each_pin_fails_on_its_planted_mistake` plants a plausible mistake in
+the oracle (a runtime switch of the CPU restatement) and checks that the pin
+written for that function then fails."""
+import numpy as np
+import pytest
+
+from bling_amd.runtime import JobSpec
+from bling_amd.scene import Material, const
+from oracle import pyoracle
+from oracle.pyoracle import mutant
+from tests import cases
+from tests.imagestats import mean_se
+
+
+def _fresnel_textbook(ci, n1, n2):
+    """Unpolarised Fresnel reflectance for light arriving at cos ci from the
+    n1 side of an n1 / n2 interface (float64)."""
+    ci = np.asarray(ci, np.float64)
+    st2 = (n1 / n2) ** 2 * (1.0 - ci * ci)
+    out = np.ones_like(ci)
+    ok = st2 < 1.0
+    ct = np.sqrt(1.0 - st2[ok])
+    c = ci[ok]
+    rs = (n1 * c - n2 * ct) / (n1 * c + n2 * ct)
+    rp = (n2 * c - n1 * ct) / (n2 * c + n1 * ct)
+    out[ok] = 0.5 * (rs * rs + rp * rp)
+    return out
+
+
+def _unit(rng, n, upper=True, min_z=0.02):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    if upper:
+        v[:, 2] = np.abs(v[:, 2])
+        v = v[v[:, 2] > min_z]
+    return v.astype(np.float32)
+
+
+# ---------------------------------------------------------------- Fresnel
+def pin_fresnel_dielectric():
+    n = 1.5
+    c = np.linspace(1e-3, 1.0, 500).astype(np.float32)
+    got = np.array([pyoracle.fresnel_dielectric(1.0, n, x) for x in c], np.float64)
+    np.testing.assert_allclose(got, _fresnel_textbook(c, 1.0, n), rtol=2e-5, atol=2e-6)
+    # from inside (cos < 0): glass to air, with total internal reflection
+    got_in = np.array([pyoracle.fresnel_dielectric(1.0, n, -x) for x in c], np.float64)
+    np.testing.assert_allclose(got_in, _fresnel_textbook(c, n, 1.0), rtol=2e-5, atol=2e-6)
+    tir = np.sqrt(1.0 - c.astype(np.float64) ** 2) * n > 1.0 + 1e-5
+    assert tir.sum() > 100 and np.all(got_in[tir] == 1.0)
+    for eta in (1.33, 1.5, 2.4):  # ((n - 1) / (n + 1))^2 at normal incidence
+        assert abs(pyoracle.fresnel_dielectric(1.0, eta, 1.0) - ((eta - 1) / (eta + 1)) ** 2) < 1e-6
+
+
+def pin_fresnel_conductor():
+    for eta, k in ((0.2, 3.0), (1.5, 0.0), (0.05, 4.2), (2.9, 3.1)):
+        want = ((eta - 1) ** 2 + k * k) / ((eta + 1) ** 2 + k * k)
+        np.testing.assert_allclose(pyoracle.fresnel_conductor(eta, k, 1.0), want, rtol=2e-6)
+        assert np.all(pyoracle.fresnel_conductor(eta, k, 0.0) == 1.0)
+        a = pyoracle.fresnel_conductor(eta, k, 0.37)
+        assert np.array_equal(a, pyoracle.fresnel_conductor(eta, k, -0.37))
+        assert np.all((a > 0) & (a <= 1))
+
+
+def test_fresnel_dielectric_is_textbook():
+    pin_fresnel_dielectric()
+
+
+def test_fresnel_conductor_closed_forms():
+    pin_fresnel_conductor()
+
+
+# ------------------------------------------------------------ microfacets
+def pin_blinn_normalised():
+    """int D cos(theta_h) dw = 1 and int pdf dw = 1 (D depends on cos only:
+    the azimuth gives 2 pi; the integral over x = cos is done by a midpoint
+    rule fine enough for e = 1000)."""
+    m = 2_000_000
+    x = (np.arange(m) + 0.5) / m
+    wh = np.stack([np.sqrt(1 - x * x), np.zeros(m), x], 1).astype(np.float32)
+    x32 = wh[:, 2].astype(np.float64)
+    for e in (1.0, 10.0, 100.0, 1000.0):
+        d, pdf = pyoracle.blinn(e, wh)
+        assert abs(np.sum(d * x32) * 2 * np.pi / m - 1.0) < 1e-4, e
+        assert abs(np.sum(pdf.astype(np.float64)) * 2 * np.pi / m - 1.0) < 1e-4, e
+
+
+def _plastic(rough=0.05, ks=0.8):
+    return cases.material_abi(Material.plastic(const(0.0), const(ks), rough))
+
+
+def pin_torrance_sparrow_eval():
+    """With wo the normal: F(cos_h) D(cos_h) min(1, 2 cos_i) / (4 cos_i) ks,
+    F the textbook dielectric term at 1.5 and D = (e + 2) / 2pi cos_h^e."""
+    rough, ks = 0.05, 0.8
+    e = 1.0 / np.float32(rough)
+    rng = np.random.default_rng(11)
+    wi = _unit(rng, 3000, min_z=0.05)
+    wo = np.tile(np.float32([0, 0, 1]), (len(wi), 1))
+    f, _ = pyoracle.bxdf_eval(_plastic(rough, ks), wo, wi, comp=1)
+    wi64 = wi.astype(np.float64)
+    ci = wi64[:, 2]
+    wh = wi64 + np.array([0, 0, 1.0])
+    wh /= np.linalg.norm(wh, axis=1, keepdims=True)
+    ch = wh[:, 2]
+    want = ks * _fresnel_textbook(ch, 1.0, 1.5) * (e + 2) / (2 * np.pi) * ch ** e * np.minimum(1, 2 * ci) / (4 * ci)
+    np.testing.assert_allclose(f[:, 0], want, rtol=2e-4, atol=1e-7)
+    # metal at normal incidence: the conductor term times (e + 2) / 8pi
+    eta, k = 0.2, 3.0
+    met = cases.material_abi(Material.metal(const(eta), const(k), rough))
+    n1 = np.float32([[0, 0, 1]])
+    fm, _ = pyoracle.bxdf_eval(met, n1, n1)
+    want_m = ((eta - 1) ** 2 + k * k) / ((eta + 1) ** 2 + k * k) * (e + 2) / (8 * np.pi)
+    np.testing.assert_allclose(fm[0], want_m, rtol=2e-5)
+
+
+def pin_torrance_sparrow_reciprocal():
+    """f(wo, wi) = bxdfEval wo wi / |cos wo| is symmetric."""
+    rng = np.random.default_rng(12)
+    wo = _unit(rng, 4000, min_z=0.05)
+    wi = _unit(rng, len(wo), min_z=0.05)[: len(wo)]
+    wo = wo[: len(wi)]
+    m = _plastic(0.1)
+    a, _ = pyoracle.bxdf_eval(m, wo, wi, comp=1)
+    b, _ = pyoracle.bxdf_eval(m, wi, wo, comp=1)
+    fa = a[:, 0] / np.abs(wo[:, 2])
+    fb = b[:, 0] / np.abs(wi[:, 2])
+    big = fa > 1e-6
+    np.testing.assert_allclose(fa[big], fb[big], rtol=1e-4)
+
+
+def test_blinn_distribution_is_normalised():
+    pin_blinn_normalised()
+
+
+def test_torrance_sparrow_closed_form_and_reciprocity():
+    pin_torrance_sparrow_eval()
+    pin_torrance_sparrow_reciprocal()
+
+
+def test_microfacet_pdf_jacobian_and_samples():
+    """bxdfPdf wo wi = pdf_h / (4 |wo . wh|): its integral over the wi of
+    the upper hemisphere equals the half-vector pdf's integral over the half
+    vectors (facing wo) whose mirror direction stays above the surface (a
+    wh-space quadrature of the closed form).
+    Samples: their pdf is bxdfPdf, their value bxdfEval / pdf, and their half
+    vectors satisfy P(cos_h <= x) = x^(e + 1)."""
+    rough = 0.05
+    e = 1.0 / np.float32(rough)
+    m = _plastic(rough)
+    nt, nphi = 2000, 720
+    th = (np.arange(nt) + 0.5) * (np.pi / nt)
+    ph = (np.arange(nphi) + 0.5) * (2 * np.pi / nphi)
+    T_, P_ = np.meshgrid(th, ph, indexing="ij")
+    w = np.stack([np.sin(T_) * np.cos(P_), np.sin(T_) * np.sin(P_), np.cos(T_)], -1).reshape(-1, 3)
+    dw = (np.sin(T_) * (np.pi / nt) * (2 * np.pi / nphi)).reshape(-1)
+    hemi = w[:, 2] > 0
+    wh, dwh = w[hemi], dw[hemi]
+    for deg in (0.0, 30.0, 60.0):
+        t = np.radians(deg)
+        wo = np.array([np.sin(t), 0.0, np.cos(t)])
+        _, pdf = pyoracle.bxdf_eval(m, np.tile(wo.astype(np.float32), (len(w), 1)), w.astype(np.float32), comp=1)
+        lhs = np.sum(pdf.astype(np.float64)[hemi] * dwh)
+        refl = 2 * (wh @ wo)[:, None] * wh - wo
+        pdf_h = (e + 1) / (2 * np.pi) * wh[:, 2] ** e
+        keep = (refl[:, 2] > 0) & (wh @ wo > 0)
+        rhs = np.sum(pdf_h[keep] * dwh[keep])
+        assert abs(lhs - rhs) < 3e-3, (deg, lhs, rhs)
+    rng = np.random.default_rng(13)
+    n = 200_000
+    wo = _unit(rng, n, min_z=0.2)
+    u = rng.random((len(wo), 2)).astype(np.float32)
+    wi, f, pdf = pyoracle.bxdf_sample(m, wo, u, comp=1)
+    ok = pdf > 0  # samples whose mirror direction leaves the hemisphere are dropped
+    assert ok.mean() > 0.8
+    fe, pe = pyoracle.bxdf_eval(m, wo[ok], wi[ok], comp=1)
+    np.testing.assert_allclose(pe, pdf[ok], rtol=1e-3)
+    fin = fe[:, 0] > 1e-8
+    np.testing.assert_allclose(f[ok][fin, 0] * pdf[ok][fin], fe[fin, 0], rtol=1e-3)
+    # at normal view and e = 100 no sample is dropped (that needs cos_h < 0.707,
+    # probability 0.707^101): the half vectors' cosines follow x^(e + 1)
+    e2 = 1.0 / np.float32(0.01)
+    wo = np.tile(np.float32([0, 0, 1]), (n, 1))
+    wi, f, pdf = pyoracle.bxdf_sample(_plastic(0.01), wo, rng.random((n, 2)).astype(np.float32), comp=1)
+    assert np.all(pdf > 0)
+    h = wo.astype(np.float64) + wi.astype(np.float64)
+    ch = h[:, 2] / np.linalg.norm(h, axis=1)
+    cdf = np.sort(ch) ** (e2 + 1)  # uniform on [0, 1] if cos_h has density (e + 1) x^e
+    ks = np.max(np.abs(cdf - (np.arange(n) + 0.5) / n))
+    assert ks < 1.6 / np.sqrt(n), ks
+
+
+# ---------------------------------------------------------------- glass
+def _glass(n=1.5):
+    return cases.material_abi(Material.glass(n, const(1.0), const(1.0)))
+
+
+def pin_spec_trans():
+    n = 1.5
+    m = _glass(n)
+    rng = np.random.default_rng(21)
+    wo = _unit(rng, 5000, min_z=0.01)
+    u = rng.random((len(wo), 2)).astype(np.float32)
+    wr, fr, pr = pyoracle.bxdf_sample(m, wo, u, comp=0)
+    wt, ft, pt = pyoracle.bxdf_sample(m, wo, u, comp=1)
+    co = wo[:, 2].astype(np.float64)
+    assert np.all(pr == 1) and np.all(pt == 1)
+    assert np.array_equal(wr, wo * np.float32([-1, -1, 1]))
+    np.testing.assert_allclose(fr[:, 0], _fresnel_textbook(co, 1.0, n), rtol=2e-5, atol=2e-6)
+    # Snell: the tangential part scales by 1/n and reverses, the ray goes through
+    np.testing.assert_allclose(wt[:, :2], -wo[:, :2] / n, rtol=1e-6, atol=1e-7)
+    assert np.all(wt[:, 2] < 0)
+    np.testing.assert_allclose(np.linalg.norm(wt.astype(np.float64), axis=1), 1.0, atol=2e-6)
+    st = np.linalg.norm(wt[:, :2].astype(np.float64), axis=1)
+    np.testing.assert_allclose(st, np.linalg.norm(wo[:, :2].astype(np.float64), axis=1) / n, rtol=1e-5, atol=1e-7)
+    # entering a lossless interface: R + T / eta^2 = 1 (eta = 1 / n)
+    np.testing.assert_allclose(fr[:, 0] + ft[:, 0] * n * n, 1.0, atol=1e-4)  # float32 cancellation at grazing angles
+    # leaving the glass: the reference's transmission term evaluates
+    # frDielectric n 1 at the transmitted cosine, which reduces to the constant
+    # ((n - 1) / (n + 1))^2; radiance scales by eta^2 = n^2
+    wi_in = wo * np.float32([1, 1, -1])
+    wr2, fr2, _ = pyoracle.bxdf_sample(m, wi_in, u, comp=0)
+    wt2, ft2, pt2 = pyoracle.bxdf_sample(m, wi_in, u, comp=1)
+    sin_o = np.sqrt(1 - co ** 2)
+    inside = sin_o * n < 1 - 1e-5
+    tir = sin_o * n > 1 + 1e-5
+    assert inside.sum() > 1000 and tir.sum() > 1000
+    np.testing.assert_allclose(ft2[inside, 0], (1 - ((n - 1) / (n + 1)) ** 2) * n * n, rtol=2e-5)
+    np.testing.assert_allclose(fr2[inside, 0], _fresnel_textbook(co[inside], n, 1.0), rtol=2e-5, atol=2e-6)
+    assert np.all(wt2[inside, 2] > 0)
+    # total internal reflection: no transmission, full reflection
+    assert np.all(pt2[tir] == 0) and np.all(ft2[tir] == 0)
+    assert np.all(fr2[tir, 0] == 1.0)
+
+
+def test_specular_transmission_snell_and_energy():
+    pin_spec_trans()
+
+
+# ------------------------------------------------------------ Oren-Nayar
+def _oren_closed_form(wo, wi, sigma, kd):
+    wo = wo.astype(np.float64)
+    wi = wi.astype(np.float64)
+    s2 = sigma * sigma
+    A = 1 - s2 / (2 * (s2 + 0.33))
+    B = 0.45 * s2 / (s2 + 0.09)
+    ti, to = np.arccos(np.abs(wi[:, 2])), np.arccos(np.abs(wo[:, 2]))
+    pi_, po = np.arctan2(wi[:, 1], wi[:, 0]), np.arctan2(wo[:, 1], wo[:, 0])
+    ok = (np.sin(ti) > 1e-4) & (np.sin(to) > 1e-4)
+    mc = np.where(ok, np.maximum(0.0, np.cos(pi_ - po)), 0.0)
+    al, be = np.maximum(ti, to), np.minimum(ti, to)
+    return kd / np.pi * (A + B * mc * np.sin(al) * np.tan(be)) * np.abs(wo[:, 2])
+
+
+def pin_oren_nayar():
+    rng = np.random.default_rng(31)
+    wo = _unit(rng, 4000, min_z=0.05)
+    wi = _unit(rng, len(wo), min_z=0.05)
+    k = min(len(wo), len(wi))
+    wo, wi = wo[:k], wi[:k]
+    kd = 0.5
+    for sigma in (0.25, 0.6):
+        m = cases.material_abi(Material.matte(const(kd), sigma))
+        f, pdf = pyoracle.bxdf_eval(m, wo, wi)
+        np.testing.assert_allclose(f[:, 0], _oren_closed_form(wo, wi, sigma, kd), rtol=3e-4)
+        g, _ = pyoracle.bxdf_eval(m, wi, wo)
+        np.testing.assert_allclose(f[:, 0] / np.abs(wo[:, 2]), g[:, 0] / np.abs(wi[:, 2]), rtol=3e-4)
+        np.testing.assert_allclose(pdf, np.abs(wi[:, 2]) / np.pi, rtol=1e-6)
+    m = cases.material_abi(Material.matte(const(kd), 1e-6))
+    f, _ = pyoracle.bxdf_eval(m, wo, wi)
+    np.testing.assert_allclose(f[:, 0], kd / np.pi * np.abs(wo[:, 2]), rtol=1e-6)
+
+
+def test_oren_nayar_closed_form():
+    pin_oren_nayar()
+
+
+# ------------------------------------------------------- environment light
+def _env_map(h=8, w=16, seed=41):
+    rng = np.random.default_rng(seed)
+    vals = rng.uniform(0.1, 5.0, size=(h, w)).astype(np.float32)
+    vals[2, 3] = 40.0  # one bright cell
+    return np.repeat(vals[:, :, None], 16, axis=2), vals
+
+
+def pin_env_pdf_normalised():
+    m, _ = _env_map()
+    nt, nphi = 720, 1440
+    th = (np.arange(nt) + 0.5) * (np.pi / nt)
+    ph = (np.arange(nphi) + 0.5) * (2 * np.pi / nphi)
+    T_, P_ = np.meshgrid(th, ph, indexing="ij")
+    w = np.stack([np.sin(T_) * np.cos(P_), np.sin(T_) * np.sin(P_), np.cos(T_)], -1).reshape(-1, 3)
+    dw = (np.sin(T_) * (np.pi / nt) * (2 * np.pi / nphi)).reshape(-1)
+    pdf, _ = pyoracle.envlight(m, wi=w)
+    assert abs(np.sum(pdf.astype(np.float64) * dw) - 1.0) < 2e-3
+
+
+def pin_env_samples_follow_the_map():
+    """Cell (iu, iv) of the sampled (phi / 2pi, theta / pi) holds the mass of
+    the map's luminance at the cell's corner (mkDist2D evaluates the map at
+    (iu / w, iv / h); the map lookup takes row floor(-v h) mod h)."""
+    m, vals = _env_map()
+    h, w = vals.shape
+    rng = np.random.default_rng(42)
+    n = 400_000
+    wi, pdf, li = pyoracle.envlight(m, u=rng.random((n, 2)).astype(np.float32))
+    ok = pdf > 0
+    assert ok.mean() > 0.999
+    d = wi[ok].astype(np.float64)
+    phi = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+    theta = np.arccos(np.clip(d[:, 2], -1, 1))
+    iu = np.minimum((phi / (2 * np.pi) * w).astype(int), w - 1)
+    iv = np.minimum((theta / np.pi * h).astype(int), h - 1)
+    obs = np.bincount(iv * w + iu, minlength=h * w).astype(np.float64)
+    mass = vals[(h - np.arange(h)) % h, :].astype(np.float64).reshape(-1)
+    exp = mass / mass.sum() * ok.sum()
+    chi2 = np.sum((obs - exp) ** 2 / exp)
+    dof = h * w - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), chi2
+    # the pdf of a sampled direction and its radiance, recomputed from it
+    pdf2, le = pyoracle.envlight(m, wi=wi[ok])
+    assert np.mean(np.isclose(pdf2, pdf[ok], rtol=1e-3)) > 0.995
+    assert np.mean(np.all(le == li[ok], axis=1)) > 0.99
+
+
+def test_env_light_pdf_and_samples():
+    pin_env_pdf_normalised()
+    pin_env_samples_follow_the_map()
+
+
+# ------------------------------------------------------- path recursion
+def pin_two_mirrors(rr_start):
+    """Camera -> mirror 1 -> mirror 2 -> sky L.  At mirror 2 the BSDF-sampled
+    MIS ray escapes: r (r L w), w = 1 / (1 + pdf_l^2), pdf_l = 1 / (2 pi^2)
+    for a uniform map seen at theta = 90 degrees; the continuation escapes
+    after a specular bounce: r^2 L.  Pixel = r^2 L (1 + w) from maxDepth 2 on,
+    0 at maxDepth 1.  With Russian roulette from depth 1 (pc = 0.75 for
+    r = 0.9) and 2 x 2 samples, the stratified roulette sample continues
+    exactly 3 of the 4 paths of a pixel, each weighted 1 / pc: the pixel mean
+    is the same closed form."""
+    r, L = 0.9, 2.0
+    flat = cases.two_mirrors_under_sky(r=r, sky=L)
+    pdf_l = 1.0 / (2 * np.pi ** 2)
+    w = 1.0 / (1.0 + pdf_l ** 2)
+    for md, want in ((1, 0.0), (2, r * r * L * (1 + w)), (5, r * r * L * (1 + w))):
+        spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=md, sample_depth=3, rr_start_depth=rr_start, passes=1, seed=9)
+        img, _, _ = pyoracle.render(flat, spec)
+        y = img[..., 2] / img[..., 0]
+        np.testing.assert_allclose(y, want, rtol=1e-5, atol=1e-6)
+
+
+def test_path_recursion_two_mirrors():
+    pin_two_mirrors(rr_start=7)
+
+
+def test_path_recursion_two_mirrors_with_roulette():
+    pin_two_mirrors(rr_start=0)
+
+
+# ------------------------------------------------------------- mutations
+MUTATIONS = [
+    (mutant.FR_DIEL_ETA, pin_fresnel_dielectric),
+    (mutant.FR_COND_K, pin_fresnel_conductor),
+    (mutant.BLINN_NORM, pin_blinn_normalised),
+    (mutant.BLINN_NORM, pin_torrance_sparrow_eval),
+    (mutant.MF_G, pin_torrance_sparrow_eval),
+    (mutant.TRANS_ETA2, pin_spec_trans),
+    (mutant.TRANS_DIR, pin_spec_trans),
+    (mutant.FR_DIEL_ETA, pin_spec_trans),
+    (mutant.OREN_SWAP, pin_oren_nayar),
+    (mutant.ENV_SIN, pin_env_pdf_normalised),
+    (mutant.ENV_ROW, pin_env_samples_follow_the_map),
+    (mutant.PATH_PC, lambda: pin_two_mirrors(rr_start=0)),
+    (mutant.PATH_T, lambda: pin_two_mirrors(rr_start=7)),
+    (mutant.POWER, lambda: pin_two_mirrors(rr_start=7)),
+]
+
+
+@pytest.mark.parametrize("m,pin", MUTATIONS, ids=lambda x: str(x) if isinstance(x, int) else "")
+def test_each_pin_fails_on_its_planted_mistake(m, pin):
+    with mutant(m):
+        with pytest.raises(AssertionError):
+            pin()
+    pin()  # and passes again without it

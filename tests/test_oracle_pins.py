@@ -111,7 +111,7 @@ def test_material_scene_is_finite_and_energy_bounded():
 def test_sampler_stratification():
     """Per pixel the camera offsets fill the nu x nv strata, and every 1D/2D
     table dimension puts exactly one sample in each of the N strata
-    (stratified1D / stratified2D + shuffle, Sampling.hs:272-327)."""
+    (stratified1D / stratified2D + shuffle, Sampling.hs:134-171)."""
     import ctypes as C
     lib = pyoracle.load()
     if not hasattr(lib, "orc_sampler_table"):
@@ -176,7 +176,7 @@ def test_transforms():
     m = rng.normal(size=(4, 4)).astype(np.float32)
     inv = np.array(T.invert(m.reshape(-1).tolist()), np.float32).reshape(4, 4)
     np.testing.assert_allclose(inv @ m, np.eye(4), atol=2e-5)
-    # a @ b applies a first (concatTrans, Transform.hs:335-338)
+    # a @ b applies a first (concatTrans, Transform.hs:241-245)
     ab = T.translate((1, 0, 0)) @ T.scale((2, 2, 2))
     assert ab.point((0, 0, 0)) == (2.0, 0.0, 0.0)
     rt = T.rotate_z(90) @ T.translate((5, 0, 0))
@@ -376,9 +376,9 @@ def test_direct_lighting_specular_continuation():
     """At a mirror the light sample is zero (a specular BSDF evaluates to
     0) and the BSDF sample reaches the light with the power-heuristic weight
     w = 1 / (1 + pdf_light^2), pdf_light = dist^2 / (cos theta_l area); with
-    maxDepth >= 2 the specular continuation (`cont`, DirectLighting.hs:49-63)
+    maxDepth >= 2 the specular continuation (`cont`, DirectLighting.hs:45-59)
     adds f * Le, where f = r * fr(cos theta_o) = r is specRefl's sampled
-    value with the no-op Fresnel of the mirror (Reflection/Specular.hs:10-18;
+    value with the no-op Fresnel of the mirror (Reflection/Specular.hs:11-20;
     no cosine and no pdf division).  The pixel is r Le (w + 1), and r Le w at
     maxDepth 1; the blackbody light adds no further branch."""
     r, Le, theta = 0.9, 3.0, 60.0
