@@ -199,16 +199,19 @@ struct Engine {
     for (int i = kMaxJobs - 1; i >= 0; --i) free_slots.push_back(i);
     BGPU_HIP(hipMemsetAsync(prog_buf.ptr, 0, sizeof(uint32_t) * kSkipWords, stream));
     BGPU_HIP(pt_enqueue_init(args, stream));
-    if (int rc = capture(c->bvh_width, cfg.sort == 1, false)) return rc;
+    if (int rc = capture(c->bvh_width, cfg.sort == 1, false, cfg.sort == 1 ? SHADE_MODE_SPLIT : SHADE_MODE_ALL))
+      return rc;
     BGPU_HIP(hipStreamSynchronize(stream));
     return BGPU_OK;
   }
 
   // Captures one bounce iteration whose traversal kernels are specialised
-  // for BVH width w (0: any width), shading in class-grouped order if `sort`.
-  int capture(int w, bool sort, bool direct) {
+  // for BVH width w (0: any width), shading in class-grouped order if `sort`
+  // with the variants of `shade_mode` (SHADE_MODE_SPLIT needs `sort`).
+  int capture(int w, bool sort, bool direct, int shade_mode) {
     args.sort_shade = sort ? 1 : 0;
     args.direct = direct ? 1 : 0;
+    args.shade_mode = shade_mode;
     if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
     if (graph) BGPU_HIP(hipGraphDestroy(graph));
     gexec = nullptr;
@@ -311,13 +314,20 @@ struct Engine {
       for (Scene* s : scenes)
         if (s && s->pt.shade_kinds > 1) want_sort = true;
     }
+    // shading variants: the diffuse one alone while every resident scene is
+    // diffuse only; split by class (diffuse, then the rest) on a grouped queue
+    bool all_diffuse = j->scene->pt.diffuse_only;
+    for (Scene* s : scenes)
+      if (s && !s->pt.diffuse_only) all_diffuse = false;
+    const int want_mode = all_diffuse ? SHADE_MODE_DIFFUSE : (want_sort ? SHADE_MODE_SPLIT : SHADE_MODE_ALL);
     // the direct-lighting shading kernel runs while such a job is resident
     bool want_direct = j->desc.integrator == BGPU_INTEGRATOR_DIRECT;
     for (HostJob* o : slots)
       if (o && o->desc.integrator == BGPU_INTEGRATOR_DIRECT) want_direct = true;
-    if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0)) {
+    if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0) ||
+        want_mode != args.shade_mode) {
       BGPU_HIP(hipStreamSynchronize(stream));
-      if (int rc = capture(want_w, want_sort, want_direct)) return rc;
+      if (int rc = capture(want_w, want_sort, want_direct, want_mode)) return rc;
     }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;

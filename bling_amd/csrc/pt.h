@@ -90,7 +90,10 @@ struct DevJob {
 // Device counters of the engine.
 // Classes of the shading work a path needs this iteration (shade_class):
 // the shade queue is grouped by class so that a wave runs one code path.
-constexpr int kShadeClasses = 32;
+constexpr int kShadeClasses = 40;
+// classes [0, kDiffuseClasses): escaped, depth cutoff, no BxDF, and the
+// makeups of Lambert / Oren-Nayar components only
+constexpr int kDiffuseClasses = 9;
 
 struct PtCounters {
   unsigned long long work_next, work_total;
@@ -172,13 +175,21 @@ struct PtKernelArgs {
   AdmitReq* admit;
   int32_t sort_shade;  // shade reads q_sh (grouped by class) instead of q_act
   int32_t direct;      // direct-lighting jobs may be resident: run their shading kernel
+  int32_t shade_mode;  // SHADE_MODE_*: which shading variants the iteration runs
 };
+
+// Shading variants of an iteration: one kernel for every BxDF kind; the
+// diffuse variant alone (every resident scene's materials are Lambert /
+// Oren-Nayar); or, on the class-grouped queue, the diffuse variant over the
+// diffuse classes and the full kernel over the rest.
+enum { SHADE_MODE_ALL = 0, SHADE_MODE_DIFFUSE = 1, SHADE_MODE_SPLIT = 2 };
 
 // Host-side shading data owned by a Scene.
 struct PtScene {
   std::vector<int32_t> shape_light_host;  // device light index per shape, -1 if none
   DevPtScene host{};                      // filled by build_pt_data, pointers are device
   int shade_kinds = 0;                    // distinct BxDF makeups of the materials in use
+  bool diffuse_only = false;              // every material in use has Lambert / Oren-Nayar components only
 };
 
 // Fills a job view from a camera and film description.

@@ -354,9 +354,12 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
 }
 
 // Shading class of active path i: 0 escaped, 1 at the depth cutoff, 2 no
-// BxDF, then one class per BxDF makeup of the hit's material: 3 + kind for
-// one component (kinds 0..5), 9 + 5 kind0 + kind1 for two (kinds 0..4), so
-// that grouped waves run one shading code path.
+// BxDF, then one class per BxDF makeup of the hit's material, so that grouped
+// waves run one shading code path.  The makeups of diffuse kinds only
+// (Lambert 0, Oren-Nayar 1) come first: 3 + kind for one component,
+// 5 + 2 kind0 + kind1 for two; classes below kDiffuseClasses are exactly the
+// paths the diffuse shading variant may take.  Then 9 + (kind - 2) for one
+// other component (kinds 2..5) and 13 + 5 kind0 + kind1 for two (kinds 0..4).
 __device__ __forceinline__ int shade_class(const PtKernelArgs& a, int i) {
   const PathPool& P = a.pool;
   const int prim = P.hprim[i];
@@ -367,7 +370,12 @@ __device__ __forceinline__ int shade_class(const PtKernelArgs& a, int i) {
   const int m = prim < S.ntri ? S.tri_info[prim].x : S.geo.shapes[prim - S.ntri].material;
   if (m < 0 || m >= S.nmats || S.mats[m].n == 0) return 2;
   const DMaterial& M = S.mats[m];
-  const int c = M.n == 1 ? 3 + M.c[0].kind : 9 + 5 * M.c[0].kind + M.c[1].kind;
+  const int k0 = M.c[0].kind, k1 = M.n > 1 ? M.c[1].kind : 0;
+  const bool diffuse = k0 <= K_OREN && (M.n == 1 || k1 <= K_OREN);
+  int c;
+  if (diffuse) c = M.n == 1 ? 3 + k0 : 5 + 2 * k0 + k1;
+  else c = M.n == 1 ? 9 + (k0 - K_MF) : 13 + 5 * k0 + k1;
+  // a texture's alternate material has the same BxDF makeup (the same type)
   return c < kShadeClasses ? c : kShadeClasses - 1;
 }
 
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(BLOCK) void k_class_scatter(PtKernelArgs a) {
 // (:61-82): the light sample's shadow ray and the BSDF sample's MIS ray are
 // queued with the contributions they carry if unoccluded / if they reach the
 // light, scaled by the path throughput thr_at(band).
-template <class Thr>
+template <int KM, class Thr>
 __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const PathPool& P, int Pn, int i,
                                                  const Bsdf& bsdf, f3 wo, f3 p, float eps, float lNumU, float ld1,
                                                  float ld2, float bComp, float bd1, float bd2, Thr thr_at,
@@ -442,14 +450,14 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
     {  // sampleLightMis (Scene.hs:61-69)
       LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
       if (ls.pdf != 0.0f && ls.li) {
-        SpecVal f = eval_bsdf(bsdf, wo, ls.wi);
-        float w = power_heuristic(ls.pdf, bsdf_pdf(bsdf, wo, ls.wi));
+        SpecVal f = eval_bsdf<KM>(bsdf, wo, ls.wi);
+        float w = power_heuristic(ls.pdf, bsdf_pdf<KM>(bsdf, wo, ls.wi));
         float k = w / ls.pdf;
         Xyz c;
         bool liBlack = true, fBlack = true;
 #pragma unroll 4
         for (int b = 0; b < 16; ++b) {
-          float fv = spec_at(f, b), li = ls.li[b];
+          float fv = spec_at<KM>(f, b), li = ls.li[b];
           liBlack = liBlack && li == 0.0f;
           fBlack = fBlack && fv == 0.0f;
           c.add(b, thr_at(b) * (((fv * li) * k) * lscale));
@@ -467,7 +475,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
       SpecVal fb;
       f3 wib;
       int tb;
-      float bpdf = sample_bsdf(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
+      float bpdf = sample_bsdf<KM>(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
       if (bpdf != 0.0f) {
         float lpdf = light_pdf(S, ln, p, wib);
         float w = power_heuristic(bpdf, lpdf);
@@ -478,7 +486,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
         bool fBlack = true;
 #pragma unroll 4
         for (int b = 0; b < 16; ++b) {
-          float fv = spec_at(fb, b);
+          float fv = spec_at<KM>(fb, b);
           fBlack = fBlack && fv == 0.0f;
           c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
         }
@@ -514,14 +522,38 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
   }
 }
 
-template <int BLOCK, int MINW>
-__global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
-  const int q = blockIdx.x * BLOCK + threadIdx.x;
+// Queue ranges of the shading variants: the whole queue, or (grouped queue)
+// its diffuse classes [0, n_diff) and the rest [n_diff, n_act).
+enum { SHADE_ALL = 0, SHADE_DIFFUSE_PART = 1, SHADE_OTHER_PART = 2 };
+
+template <int PART>
+__device__ __forceinline__ void shade_range(const PtKernelArgs& a, unsigned int* lo, unsigned int* hi) {
   const unsigned int nact = a.cnt->n_act;
-  if ((unsigned)(blockIdx.x * BLOCK) >= nact) return;
+  if constexpr (PART == SHADE_ALL) {
+    *lo = 0u;
+    *hi = nact;
+  } else {
+    unsigned int nd = 0u;
+#pragma unroll
+    for (int c = 0; c < kDiffuseClasses; ++c) nd += a.cnt->cls_count[c];
+    nd = nd < nact ? nd : nact;
+    *lo = PART == SHADE_DIFFUSE_PART ? 0u : nd;
+    *hi = PART == SHADE_DIFFUSE_PART ? nd : nact;
+  }
+}
+
+// Path shading (Path.hs:41-87).  KM: the BxDF kinds compiled in (kAllKinds,
+// or kDiffuseKinds for the variant that shades only diffuse materials with
+// far fewer registers); PART: the queue range it shades.
+template <int BLOCK, int MINW, int KM, int PART>
+__global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
+  unsigned int lo, hi;
+  shade_range<PART>(a, &lo, &hi);
+  if ((unsigned)(blockIdx.x * BLOCK) >= hi - lo) return;
+  const int q = (int)lo + blockIdx.x * BLOCK + threadIdx.x;
   const PathPool& P = a.pool;
   const int Pn = P.P;
-  bool valid = (unsigned)q < nact;
+  bool valid = (unsigned)q < hi;
   int i = valid ? (int)(a.sort_shade ? P.q_sh[q] : P.q_act[q]) : 0;
   int jslot = valid ? P.job[i] : 0;
   if (a.direct && valid && a.jobs[jslot].integ == BGPU_INTEGRATOR_DIRECT) valid = false;  // k_shade_direct's
@@ -584,7 +616,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
           for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
           xyz_accum(P.rad, Pn, i, c);
         }
-        sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at, want_shadow,
+        sample_one_light<KM>(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at, want_shadow,
                          want_mis);
         // Russian roulette (Path.hs:68-72)
         float pc = 1.0f;
@@ -603,7 +635,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
           SpecVal f;
           f3 wi;
           int tp;
-          float spdf = sample_bsdf(bsdf, wo, uc, u1, u2, f, &wi, &tp);
+          float spdf = sample_bsdf<KM>(bsdf, wo, uc, u1, u2, f, &wi, &tp);
           if (spdf == 0.0f) {
             cont = false;
           } else {
@@ -614,7 +646,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
             float* thw = P.thr + i;
 #pragma unroll 4
             for (int b = 0; b < 16; ++b) {
-              float fv = spec_at(f, b);
+              float fv = spec_at<KM>(f, b);
               fBlack = fBlack && fv == 0.0f;
               thw[b * Pn] = (fv * thr_at(b)) * ipc;
             }
@@ -696,7 +728,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
         for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
         xyz_accum(P.rad, Pn, i, c);
       }
-      sample_one_light(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2, thr_at, want_shadow,
+      sample_one_light<kAllKinds>(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2, thr_at, want_shadow,
                        want_mis);
       if (depth + 1 < J.md) {  // cont (DirectLighting.hs:45-59)
         SpecVal fr, ft;
@@ -1078,7 +1110,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
       // sample is dropped as the spectral check of addSample would
       float X = P.rad[i], Y = P.rad[Pn + i], Z = P.rad[2 * Pn + i];
       const bool bad = isnan(X) || isinf(X) || isnan(Y) || isinf(Y) || isnan(Z) || isinf(Z);
-      if (!bad) {  // addSample (Image.hs:250-299) on the full image
+      if (!bad) {  // addSample (Image.hs:250-299) into the sample's tile
         X = X / BGPU_CIE_Y_SUM;
         Y = Y / BGPU_CIE_Y_SUM;
         Z = Z / BGPU_CIE_Y_SUM;
@@ -1087,10 +1119,22 @@ __global__ __launch_bounds__(BLOCK) void k_finish(PtKernelArgs a) {
         float dx = sx - 0.5f, dy = sy - 0.5f;
         int x0 = (int)ceilf(dx - fw), x1 = (int)floorf(dx + fw);
         int y0 = (int)ceilf(dy - fh), y1 = (int)floorf(dy + fh);
-        x0 = x0 < 0 ? 0 : x0;
-        y0 = y0 < 0 ? 0 : y0;
-        x1 = x1 > J.view.film_w - 1 ? J.view.film_w - 1 : x1;
-        y1 = y1 > J.view.film_h - 1 ? J.view.film_h - 1 : y1;
+        {  // prender's tile of the sample's pixel: splitWindow's 16 x 16
+           // window of the sample extent and the image mkImageTile makes for
+           // it (Sampling.hs:55-58, Image.hs:108-120); addTile drops what
+           // lies beyond the film (Image.hs:178-199)
+          const DevView& V = J.view;
+          const int pix = P.pix[i];
+          const int ix = pix % V.ext_w, iy = pix / V.ext_w;  // relative to the extent
+          const int xs = V.ext_x0 + (ix / 16) * 16, ys = V.ext_y0 + (iy / 16) * 16;
+          const int xe = min(xs + 15, V.ext_x0 + V.ext_w - 1), ye = min(ys + 15, V.ext_y0 + V.ext_h - 1);
+          const int lx = max(0, xs), ly = max(0, ys);
+          const int rx = xe + (int)floorf(0.5f + fw) - 1, ry = ye + (int)floorf(0.5f + fh) - 1;
+          x0 = max(x0, lx);
+          y0 = max(y0, ly);
+          x1 = min(min(x1, rx), V.film_w - 1);
+          y1 = min(min(y1, ry), V.film_h - 1);
+        }
         if (x0 == x1 && y0 == y1) {
           int fy = (int)floorf(fabsf(((float)y0 - dy) * ifh * 16.0f));
           fy = fy < 15 ? fy : 15;
@@ -1161,6 +1205,13 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
 // block, so they run in the largest blocks
 constexpr int kWideBlock = 1024;
 
+// Waves per SIMD the diffuse shading variant is compiled for (its register
+// budget: 512 / waves VGPRs; what does not fit spills to scratch).
+#ifndef BGPU_DIFFUSE_WAVES
+#define BGPU_DIFFUSE_WAVES 4
+#endif
+constexpr int kDiffuseWaves = BGPU_DIFFUSE_WAVES;
+
 template <int BLOCK>
 static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
   if (a.sort_shade) {  // group the queue by shading class first
@@ -1170,8 +1221,17 @@ static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipSt
     hipLaunchKernelGGL(k_class_count<kSortBlock>, gs, bs, 0, s, a);
     hipLaunchKernelGGL(k_class_scatter<kSortBlock>, gs, bs, 0, s, a);
   }
-  if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_shade<BLOCK, 1>), g, b, 0, s, a);
+  if (a.shade_mode == SHADE_MODE_DIFFUSE) {
+    hipLaunchKernelGGL((k_shade<BLOCK, kDiffuseWaves, kDiffuseKinds, SHADE_ALL>), g, b, 0, s, a);
+  } else if (a.shade_mode == SHADE_MODE_SPLIT) {
+    hipLaunchKernelGGL((k_shade<BLOCK, kDiffuseWaves, kDiffuseKinds, SHADE_DIFFUSE_PART>), g, b, 0, s, a);
+    if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2, kAllKinds, SHADE_OTHER_PART>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_shade<BLOCK, 1, kAllKinds, SHADE_OTHER_PART>), g, b, 0, s, a);
+  } else if (waves == 2) {
+    hipLaunchKernelGGL((k_shade<BLOCK, 2, kAllKinds, SHADE_ALL>), g, b, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_shade<BLOCK, 1, kAllKinds, SHADE_ALL>), g, b, 0, s, a);
+  }
   if (a.direct) hipLaunchKernelGGL((k_shade_direct<BLOCK>), g, b, 0, s, a);
 }
 

@@ -424,6 +424,19 @@ struct SpecVal {
   float k1, k2;
 };
 
+// BxDF kinds a shading variant compiles (bit k: kind k).  The BSDF routines
+// below take the set as a template argument and leave out the code of the
+// other kinds, so that a kernel shading only diffuse materials carries none
+// of the microfacet, FresnelBlend and specular code and its registers; the
+// caller guarantees that every component it passes is of a kind in the set.
+constexpr int kAllKinds = (1 << K_LAMBERT) | (1 << K_OREN) | (1 << K_MF) | (1 << K_SPECR) | (1 << K_SPECT) |
+                          (1 << K_FB);
+constexpr int kDiffuseKinds = (1 << K_LAMBERT) | (1 << K_OREN);
+template <int KM>
+__device__ __forceinline__ constexpr bool has_kind(int k) {
+  return ((KM >> k) & 1) != 0;
+}
+
 __device__ __forceinline__ float fr_cond_band(const DBxdf& b, float cosi, int i) {  // Fresnel.hs:58-70
   float ac = fabsf(cosi);
   float eta = b.eta[i], k = b.k[i];
@@ -435,8 +448,11 @@ __device__ __forceinline__ float fr_cond_band(const DBxdf& b, float cosi, int i)
   float rPar2 = (tmp - ec2 + 1.0f) / (tmp + ec2 + 1.0f);
   return (rPer2 + rPar2) / 2.0f;
 }
+template <int KM = kAllKinds>
 __device__ __forceinline__ float term_at(const Term& t, int b) {
   const DBxdf& x = *t.bx;
+  // diffuse kinds make only T_RS and T_ROS terms
+  if constexpr ((KM & ~kDiffuseKinds) == 0) return t.op == T_ROS ? (x.r[b] * t.a) * t.c : x.r[b] * t.a;
   switch (t.op) {
     case T_RS: return x.r[b] * t.a;
     case T_ROS: return (x.r[b] * t.a) * t.c;
@@ -445,23 +461,24 @@ __device__ __forceinline__ float term_at(const Term& t, int b) {
     default: return (t.a * x.r[b]) * t.c;
   }
 }
+template <int KM = kAllKinds>
 __device__ __forceinline__ float spec_at(const SpecVal& v, int b) {
   if (v.mode == V_SUM) {
     float f = 0.0f;
-    if (v.n > 0) f = f + term_at(v.t[0], b);
-    if (v.n > 1) f = f + term_at(v.t[1], b);
+    if (v.n > 0) f = f + term_at<KM>(v.t[0], b);
+    if (v.n > 1) f = f + term_at<KM>(v.t[1], b);
     return f;
   }
-  if (v.mode == V_SCALE) return term_at(v.t[0], b) * v.k1;
-  if (v.mode == V_FB) {  // t0: T_FBD (a = diffuse scale, c = absorption exponent), t1: T_FBS
+  if (v.mode == V_SCALE) return term_at<KM>(v.t[0], b) * v.k1;
+  if (has_kind<KM>(K_FB) && v.mode == V_FB) {  // t0: T_FBD (a = diffuse scale, c = absorption exponent), t1: T_FBS
     const DBxdf& x = *v.t[0].bx;
     const float ab = x.et > 0.0f ? expf(x.k[b] * v.t[0].c) : 1.0f;
     const float diff = ((ab * x.r[b]) * (1.0f - x.eta[b])) * v.t[0].a;
     const float spec = (x.eta[b] + (1.0f - x.eta[b]) * v.t[1].c) * v.t[1].a;
     return (diff + spec) * v.k1;
   }
-  float fo = v.n > 1 ? 0.0f + term_at(v.t[1], b) : 0.0f;
-  return (term_at(v.t[0], b) * v.k1 + fo) * v.k2;
+  float fo = v.n > 1 ? 0.0f + term_at<KM>(v.t[1], b) : 0.0f;
+  return (term_at<KM>(v.t[0], b) * v.k1 + fo) * v.k2;
 }
 __device__ __forceinline__ SpecVal spec_zero() {
   SpecVal v;
@@ -585,17 +602,18 @@ __device__ __forceinline__ float oren_scale(const DBxdf& b, f3 wo, f3 wi) {  // 
 }
 
 // bxdfEval b wo wi as a term; false when the component contributes nothing
+template <int KM = kAllKinds>
 __device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term* t) {
   if (b.flip) wi.z = -wi.z;  // brdfToBtdf: e wo wi = bxdfEval brdf wo (otherHemisphere wi)
-  if (b.kind == K_LAMBERT) {
+  if (has_kind<KM>(K_LAMBERT) && b.kind == K_LAMBERT) {
     *t = Term{&b, T_RS, kInvPi * acos_t(wo), 0.0f};
     return true;
   }
-  if (b.kind == K_OREN) {
+  if (has_kind<KM>(K_OREN) && b.kind == K_OREN) {
     *t = Term{&b, T_ROS, oren_scale(b, wo, wi), kInvPi * acos_t(wo)};
     return true;
   }
-  if (b.kind == K_MF) {  // Microfacet.hs:21-32
+  if (has_kind<KM>(K_MF) && b.kind == K_MF) {  // Microfacet.hs:21-32
     float costo = acos_t(wo), costi = acos_t(wi);
     if (costi == 0.0f || costo == 0.0f) return false;
     f3 whp = add3(wi, wo);
@@ -610,11 +628,13 @@ __device__ __forceinline__ bool bx_eval_term(const DBxdf& b, f3 wo, f3 wi, Term*
   }
   return false;  // specular components evaluate to zero
 }
+template <int KM = kAllKinds>
 __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
   if (b.flip) wi.z = -wi.z;
-  if (b.kind == K_LAMBERT || b.kind == K_OREN) return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
-  if (b.kind == K_FB) return fb_pdf(b, wo, wi);
-  if (b.kind == K_MF) {
+  if ((has_kind<KM>(K_LAMBERT) && b.kind == K_LAMBERT) || (has_kind<KM>(K_OREN) && b.kind == K_OREN))
+    return same_hemi(wo, wi) ? kInvPi * acos_t(wi) : 0.0f;
+  if (has_kind<KM>(K_FB) && b.kind == K_FB) return fb_pdf(b, wo, wi);
+  if (has_kind<KM>(K_MF) && b.kind == K_MF) {
     f3 whp = add3(wo, wi);
     if (sqlen3(whp) == 0.0f) return 0.0f;
     f3 wh = norm3(whp);
@@ -624,14 +644,15 @@ __device__ __forceinline__ float bx_pdf(const DBxdf& b, f3 wo, f3 wi) {
   return 0.0f;
 }
 // bxdfSample with adj = False: the sampled value as a term (zero: *zero set)
+template <int KM = kAllKinds>
 __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float u2, Term* f, bool* zero,
                                           f3* wi, float* pdf) {
   *zero = false;
-  if (b.kind == K_LAMBERT || b.kind == K_OREN) {
+  if ((has_kind<KM>(K_LAMBERT) && b.kind == K_LAMBERT) || (has_kind<KM>(K_OREN) && b.kind == K_OREN)) {
     f3 w = cos_hemisphere(u1, u2);
     if (wo.z < 0.0f) w.z = -w.z;  // toSameHemisphere
     if (same_hemi(wo, w)) {
-      *f = Term{&b, T_RS, b.kind == K_OREN ? oren_scale(b, wo, w) : 1.0f, 0.0f};
+      *f = Term{&b, T_RS, (has_kind<KM>(K_OREN) && b.kind == K_OREN) ? oren_scale(b, wo, w) : 1.0f, 0.0f};
       *wi = w;
       *pdf = kInvPi * acos_t(w);
     } else {
@@ -642,7 +663,7 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
     if (b.flip) wi->z = -wi->z;  // brdfToBtdf: the sampled direction's other hemisphere
     return;
   }
-  if (b.kind == K_MF) {  // Microfacet.hs:42-54,175-182
+  if (has_kind<KM>(K_MF) && b.kind == K_MF) {  // Microfacet.hs:42-54,175-182
     float cost = powf(u1, 1.0f / (b.e + 1.0f));
     float sint = __builtin_sqrtf(hmax(0.0f, 1.0f - cost * cost));
     float phi = u2 * 2.0f * kPi;
@@ -664,10 +685,16 @@ __device__ __forceinline__ void bx_sample(const DBxdf& b, f3 wo, float u1, float
     }
     return;
   }
-  if (b.kind == K_SPECR) {  // Specular.hs:11-20
+  if (has_kind<KM>(K_SPECR) && b.kind == K_SPECR) {  // Specular.hs:11-20
     *f = rf_term(b, cos_t(wo), 1.0f);
     *wi = mk3(-wo.x, -wo.y, wo.z);
     *pdf = 1.0f;
+    return;
+  }
+  if (!has_kind<KM>(K_SPECT)) {  // not reached: every component is of a kind in KM
+    *zero = true;
+    *wi = wo;
+    *pdf = 0.0f;
     return;
   }
   // specTrans (Specular.hs:33-58)
@@ -722,6 +749,7 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, con
 }
 __device__ __forceinline__ int bsdf_n(const Bsdf& b) { return b.m ? b.m->n : 0; }
 
+template <int KM = kAllKinds>
 __device__ __forceinline__ SpecVal eval_bsdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:318-332
   SpecVal f = spec_zero();
   float cosWo = dot3(woW, b.ng);
@@ -733,26 +761,28 @@ __device__ __forceinline__ SpecVal eval_bsdf(const Bsdf& b, f3 woW, f3 wiW) {  /
   for (int i = 0; i < n; ++i) {
     const DBxdf& c = b.m->c[i];
     if (!(c.type & need)) continue;
-    if (c.kind == K_FB) {  // the only component: 0 + (diff + spec) = (diff + spec) * 1
+    if (has_kind<KM>(K_FB) && c.kind == K_FB) {  // the only component: 0 + (diff + spec) = (diff + spec) * 1
       fb_terms(c, wi, wo, &f.t[0], &f.t[1]);
       f.n = 2;
       f.mode = V_FB;
       f.k1 = 1.0f;
-    } else if (bx_eval_term(c, wi, wo, &f.t[f.n])) {
+    } else if (bx_eval_term<KM>(c, wi, wo, &f.t[f.n])) {
       f.n++;
     }
   }
   return f;
 }
+template <int KM = kAllKinds>
 __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, f3 woW, f3 wiW) {  // Reflection.hs:251-257
   int n = bsdf_n(b);
   if (n == 0) return 0.0f;
   f3 wo = to_local(b.cs, woW), wi = to_local(b.cs, wiW);
   float s = 0.0f;
-  for (int i = 0; i < n; ++i) s = s + bx_pdf(b.m->c[i], wo, wi);
+  for (int i = 0; i < n; ++i) s = s + bx_pdf<KM>(b.m->c[i], wo, wi);
   return s / (float)n;
 }
 // sampleBsdf (Reflection.hs:278-316), adj = False, all flags; returns pdf.
+template <int KM = kAllKinds>
 __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp, float u1, float u2,
                                              SpecVal& f, f3* wiW, int* type) {
   f = spec_zero();
@@ -766,7 +796,7 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   sNum = cnt - 1 < sNum ? cnt - 1 : sNum;
   sNum = sNum < 0 ? 0 : sNum;
   const DBxdf& bx = b.m->c[sNum];
-  if (bx.kind == K_FB) {  // the only component, a reflection: f = e wo wi / pdf = (t0 + t1) * (1 / pdf)
+  if (has_kind<KM>(K_FB) && bx.kind == K_FB) {  // the only component, a reflection: f = e wo wi / pdf = (t0 + t1) * (1 / pdf)
     f3 wi;
     float pdfp;
     if (!fb_sample(bx, wo, u1, u2, &f.t[0], &f.t[1], &wi, &pdfp)) return 0.0f;
@@ -784,7 +814,7 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   bool fzero;
   f3 wi;
   float pdfp;
-  bx_sample(bx, wo, u1, u2, &fs, &fzero, &wi, &pdfp);
+  bx_sample<KM>(bx, wo, u1, u2, &fs, &fzero, &wi, &pdfp);
   f3 ww = to_world(b.cs, wi);
   float side = dot3(ww, b.ng) / dot3(woW, b.ng);
   if (pdfp == 0.0f || side == 0.0f) return 0.0f;
@@ -808,8 +838,8 @@ __device__ __forceinline__ float sample_bsdf(const Bsdf& b, f3 woW, float uComp,
   float other = 0.0f;
   for (int i = 0; i < cnt; ++i) {
     if (i == sNum) continue;
-    other = other + bx_pdf(b.m->c[i], wo, wi);
-    if ((b.m->c[i].type & need) && bx_eval_term(b.m->c[i], wi, wo, &f.t[1])) f.n = 2;
+    other = other + bx_pdf<KM>(b.m->c[i], wo, wi);
+    if ((b.m->c[i].type & need) && bx_eval_term<KM>(b.m->c[i], wi, wo, &f.t[1])) f.n = 2;
   }
   float pdf = (pdfp + other) * invCnt;
   f.mode = V_MIX;

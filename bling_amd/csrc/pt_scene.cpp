@@ -303,14 +303,19 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     }
     for (uint32_t s = 0; s < ns; ++s)
       if (d->shapes[s].material >= 0 && (size_t)d->shapes[s].material < used.size()) used[d->shapes[s].material] = 1;
+    for (size_t i = 0; i < d->num_materials; ++i)  // a texture may select the alternate entry
+      if (used[i] && mats[i].tex_kind != BGPU_TEX_NONE) used[mats[i].alt] = 1;
     std::vector<int> sigs;
+    bool diffuse = true;
     for (size_t i = 0; i < mats.size(); ++i) {
       if (!used[i]) continue;
       const DMaterial& m = mats[i];
       const int sig = m.n | (m.n > 0 ? m.c[0].kind << 2 : 0) | (m.n > 1 ? m.c[1].kind << 5 : 0);
       if (std::find(sigs.begin(), sigs.end(), sig) == sigs.end()) sigs.push_back(sig);
+      for (int c = 0; c < m.n; ++c) diffuse = diffuse && (m.c[c].kind == K_LAMBERT || m.c[c].kind == K_OREN);
     }
     sc->pt.shade_kinds = (int)sigs.size();
+    sc->pt.diffuse_only = diffuse;
   }
 
   // lights: explicit first, then emissive shapes (Scene.hs:38-42)

@@ -1324,17 +1324,40 @@ static void filterSize(const bgpu_film& f, float* fw, float* fh) {
   else { *fw = f.fparams[0]; *fh = f.fparams[1]; }
 }
 
-static void addSample(const Ctx& c, float* wxyz, float* sq, int W, int H, float sx, float sy,
-                      const Spec& L) {  // Image.hs:250-299 on the full image
+// The tile a sample of pixel (ix, iy) is rendered into: prender splits the
+// sample extent [ex0, ex1] x [ey0, ey1] into 16 x 16 windows (splitWindow,
+// Sampling.hs:55-58) and gives each its own image (mkImageTile,
+// Image.hs:108-120) at offset (max 0 xStart, max 0 yStart), of width
+// xEnd - px + floor (0.5 + fw); addSample clips a splat to that image and
+// addTile (Image.hs:178-199) drops what lies beyond the film.
+struct TileClip {
+  int lx, rx, ly, ry;
+};
+static TileClip tileClip(int ix, int iy, int ex0, int ex1, int ey0, int ey1, float fw, float fh) {
+  const int xs = ex0 + ((ix - ex0) / 16) * 16, ys = ey0 + ((iy - ey0) / 16) * 16;
+  const int xe = std::min(xs + 15, ex1), ye = std::min(ys + 15, ey1);
+  const int px = std::max(0, xs), py = std::max(0, ys);
+  const int w = xe - px + (int)std::floor(0.5f + fw), h = ye - py + (int)std::floor(0.5f + fh);
+  return TileClip{px, px + w - 1, py, py + h - 1};
+}
+
+static void addSample(const Ctx& c, float* wxyz, float* sq, int W, int H, float sx, float sy, const TileClip& tc,
+                      const Spec& L) {  // Image.hs:250-299 into the sample's tile
   for (float v : L)
     if (std::isnan(v) || std::isinf(v)) return;
   float X, Y, Z;
   toXYZ(L, &X, &Y, &Z);
   float fw = c.fw, fh = c.fh;
-  float ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:263 (sic)
+  float ifw = 1.0f / fw, ifh = 1.0f / fw;  // Image.hs:262-263 (sic)
   float dx = sx - 0.5f, dy = sy - 0.5f;
-  int x0 = std::max(0, (int)std::ceil(dx - fw)), x1 = std::min(W - 1, (int)std::floor(dx + fw));
-  int y0 = std::max(0, (int)std::ceil(dy - fh)), y1 = std::min(H - 1, (int)std::floor(dy + fh));
+  int x0 = (int)std::ceil(dx - fw), x1 = (int)std::floor(dx + fw);
+  int y0 = (int)std::ceil(dy - fh), y1 = (int)std::floor(dy + fh);
+  if (g_mutant != MUT_FILTER_CLIP) {
+    x0 = std::max(tc.lx, x0), x1 = std::min(tc.rx, x1);
+    y0 = std::max(tc.ly, y0), y1 = std::min(tc.ry, y1);
+  }
+  x0 = std::max(0, x0), x1 = std::min(W - 1, x1);
+  y0 = std::max(0, y0), y1 = std::min(H - 1, y1);
   if (x1 - x0 < 0 || y1 - y0 < 0) return;
   for (int y = y0; y <= y1; ++y) {
     int iy = std::min((int)std::floor(std::fabs(((float)y - dy) * ifh * 16.0f)), 15);
@@ -1413,6 +1436,30 @@ void orc_aniso(float ex, float ey, int n, const float* u, float* wh, float* d, f
       pdf[i] = anisoPdf(ex, ey, w);
     }
   }
+}
+
+// Splats n samples at image positions (sx[i], sy[i]) with XYZ-giving
+// constant spectra: sample i has the spectrum sConst v[i] and belongs to
+// pixel (px[i], py[i]) of the film's sample extent (prender's tiles).
+void orc_film_splat(const bgpu_film* film, int n, const float* sx, const float* sy, const int* px, const int* py,
+                    const float* v, float* wxyz) {
+  using namespace orc;
+  Ctx c;
+  c.s = nullptr;
+  filterSize(*film, &c.fw, &c.fh);
+  for (int y = 0; y < 16; ++y) {  // mkTableFilter (Image.hs:48-61)
+    float fy = ((float)y + 0.5f) * c.fh / 16.0f;
+    for (int x = 0; x < 16; ++x) {
+      float fx = ((float)x + 0.5f) * c.fw / 16.0f;
+      c.filter[y * 16 + x] = evalFilter(*film, fx, fy);
+    }
+  }
+  const int W = film->width, H = film->height;
+  int x0 = (int)std::floor(0.5f - c.fw), x1 = (int)std::floor(0.5f + (float)W + c.fw);
+  int y0 = (int)std::floor(0.5f - c.fh), y1 = (int)std::floor(0.5f + (float)H + c.fh);
+  for (int i = 0; i < n; ++i)
+    addSample(c, wxyz, nullptr, W, H, sx[i], sy[i], tileClip(px[i], py[i], x0, x1, y0, y1, c.fw, c.fh),
+              sconst(v[i]));
 }
 
 // Selects a planted mistake (the MUT_* list); 0 restores the restatement.
@@ -1592,7 +1639,7 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
         float sx = (float)ix + ox, sy = (float)iy + oy;
         Ray r = fireRay(camera, sx, sy, lu, lv);
         Spec L = direct ? liDirect(c, r, ps, n, 0, md) : li(c, r, ps, n, md, rr);
-        addSample(c, wxyz, sq, W, H, sx, sy, L);
+        addSample(c, wxyz, sq, W, H, sx, sy, tileClip(ix, iy, x0, x1, y0, y1, c.fw, c.fh), L);
       }
     }
   }

@@ -623,3 +623,48 @@ def test_substrate_scene_matches_oracle(gpu_ctx):
     mg, _ = mean_se(g3, gsq3)
     mo, _ = mean_se(o3, osq3)
     assert np.median(np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)) < 1e-4
+
+
+def test_path_recursion_two_mirrors_closed_form(gpu_ctx):
+    """Two mirrors under a uniform sky (the oracle's pin of the path
+    recursion, Integrator/Path.hs:66-87): r^2 L (1 + w) from maxDepth 2 on,
+    0 at maxDepth 1, with and without Russian roulette from depth 1 (the
+    stratified roulette sample continues exactly 3 of a pixel's 4 paths,
+    each weighted 1 / 0.75)."""
+    from tests import cases
+    r, L = 0.9, 2.0
+    flat = cases.two_mirrors_under_sky(r=r, sky=L)
+    pdf_l = 1.0 / (2 * np.pi ** 2)
+    w = 1.0 / (1.0 + pdf_l ** 2)
+    for rr in (7, 0):
+        for md, want in ((1, 0.0), (2, r * r * L * (1 + w)), (5, r * r * L * (1 + w))):
+            spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=md, sample_depth=3, rr_start_depth=rr, passes=1, seed=9)
+            img, _ = render_gpu(gpu_ctx, flat, spec)
+            np.testing.assert_allclose(img[..., 2] / img[..., 0], want, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["gauss", "mitchell", "sinc", "triangle", "box"])
+def test_film_filters_track_oracle(gpu_ctx, name):
+    """Non-box filters splat a sample over several pixels, clipped to the
+    16 x 16 tile prender renders it in (mkImageTile / addTile,
+    Image.hs:108-120,178-199).  Same seed on both sides: per-pixel weight
+    sums agree to float rounding (GPU atomics add in another order) and the
+    pixel means track the oracle's."""
+    from tests.test_oracle_closed_forms import FILTERS
+    kind, p = FILTERS[name]
+    flat = scenes.cornell(48)
+    view = flat.view(45, 38)
+    view.film.filter = kind
+    for k in range(4):
+        view.film.fparams[k] = p[k]
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=2, seed=17,
+                   variance=True, view=view)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    np.testing.assert_allclose(g[..., 0], o[..., 0], rtol=1e-5, atol=1e-5)
+    mg = g[..., 1:] / g[..., :1]
+    mo = o[..., 1:] / o[..., :1]
+    ok = np.abs(o[..., 0]) > 1e-3
+    rel = np.abs(mg - mo)[ok] / np.maximum(np.abs(mo[ok]), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)

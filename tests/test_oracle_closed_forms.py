@@ -381,6 +381,145 @@ def test_path_recursion_two_mirrors_with_roulette():
     pin_two_mirrors(rr_start=0)
 
 
+# ------------------------------------------------------------------ film
+FILTERS = {  # bgpu_film filter kinds with their parameters (Filter.hs:12-96)
+    "box": (0, (0.0, 0.0, 0.0, 0.0)),
+    "gauss": (1, (2.0, 1.5, 2.0, 0.0)),
+    "mitchell": (2, (2.0, 2.0, 1.0 / 3, 1.0 / 3)),
+    "sinc": (3, (3.0, 3.0, 3.0, 0.0)),
+    "triangle": (4, (1.5, 2.0, 0.0, 0.0)),
+}
+
+
+def _filter_eval(kind, p, x, y):
+    """evalFilter (Filter.hs:68-96) in float64."""
+    if kind == 0:
+        return float(abs(x) < 0.5 and abs(y) < 0.5)
+    if kind == 1:
+        w, h, a = p[0], p[1], p[2]
+        return max(0.0, np.exp(-a * x * x) - np.exp(-a * w * w)) * max(0.0, np.exp(-a * y * y) - np.exp(-a * h * h))
+    if kind == 2:
+        b, c = p[2], p[3]
+
+        def m(v):
+            v = abs(2 * v)
+            if v > 1:
+                return ((-b - 6 * c) * v ** 3 + (6 * b + 30 * c) * v ** 2 + (-12 * b - 48 * c) * v + (8 * b + 24 * c)) / 6
+            return ((12 - 9 * b - 6 * c) * v ** 3 + (-18 + 12 * b + 6 * c) * v ** 2 + (6 - 2 * b)) / 6
+        return m(x / p[0]) * m(y / p[1])
+    if kind == 3:
+        tau = p[2]
+
+        def s(v):
+            if abs(v) > 1:
+                return 0.0
+            if abs(v) < 1e-5:
+                return 1.0
+            z = abs(v) * np.pi
+            return np.sin(z) / z * np.sin(z * tau) / (z * tau)
+        return s(x / p[0]) * s(y / p[1])
+    return max(0.0, p[0] - abs(2 * x)) * max(0.0, p[1] - abs(2 * y)) / (p[0] * p[1])
+
+
+def _tiled_film(kind, p, W, H, sx, sy, px, py, v):
+    """prender's film, literally: 16 x 16 windows of the sample extent
+    (splitWindow), one tile image per window (mkImageTile), addSample into
+    the tile, addTile into the film (Image.hs:40-61,108-120,178-199,250-299,
+    Sampling.hs:55-58).  Returns (H, W, 2) sums of weight and weight * v."""
+    fw, fh = (0.5, 0.5) if kind == 0 else (p[0], p[1])
+    table = np.array([[_filter_eval(kind, p, (x + 0.5) * fw / 16, (y + 0.5) * fh / 16) for x in range(16)]
+                      for y in range(16)])
+    ifw = ifh = 1.0 / fw  # Image.hs:262-263 (sic)
+    ex0, ex1 = int(np.floor(0.5 - fw)), int(np.floor(0.5 + W + fw))
+    ey0, ey1 = int(np.floor(0.5 - fh)), int(np.floor(0.5 + H + fh))
+    img = np.zeros((H, W, 2))
+    for ys in range(ey0, ey1 + 1, 16):
+        for xs in range(ex0, ex1 + 1, 16):
+            xe, ye = min(xs + 15, ex1), min(ys + 15, ey1)
+            ox, oy = max(0, xs), max(0, ys)
+            tw, th = xe - ox + int(np.floor(0.5 + fw)), ye - oy + int(np.floor(0.5 + fh))
+            tile = np.zeros((th, tw, 2))
+            sel = np.nonzero((px >= xs) & (px <= xe) & (py >= ys) & (py <= ye))[0]
+            for i in sel:
+                dx, dy = sx[i] - 0.5, sy[i] - 0.5
+                x0, x1 = max(ox, int(np.ceil(dx - fw))), min(ox + tw - 1, int(np.floor(dx + fw)))
+                y0, y1 = max(oy, int(np.ceil(dy - fh))), min(oy + th - 1, int(np.floor(dy + fh)))
+                for y in range(y0, y1 + 1):
+                    fy = min(int(np.floor(abs((y - dy) * ifh * 16))), 15)
+                    for x in range(x0, x1 + 1):
+                        fx = min(int(np.floor(abs((x - dx) * ifw * 16))), 15)
+                        wt = table[fy, fx]
+                        tile[y - oy, x - ox] += (wt, wt * v[i])
+            for y in range(th):
+                for x in range(tw):
+                    if y + oy < H and x + ox < W:
+                        img[y + oy, x + ox] += tile[y, x]
+    return img
+
+
+def pin_film_tiles():
+    """The oracle's splats (clipped to the sample's tile) equal prender's
+    tile images merged into the film, for every filter."""
+    import ctypes as C
+    from bling_amd import abi
+    lib = pyoracle.load()
+    lib.orc_film_splat.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+    W, H = 37, 29
+    rng = np.random.default_rng(51)
+    for name, (kind, p) in FILTERS.items():
+        fw, fh = (0.5, 0.5) if kind == 0 else (p[0], p[1])
+        ex0, ex1 = int(np.floor(0.5 - fw)), int(np.floor(0.5 + W + fw))
+        ey0, ey1 = int(np.floor(0.5 - fh)), int(np.floor(0.5 + H + fh))
+        gx, gy = np.meshgrid(np.arange(ex0, ex1 + 1), np.arange(ey0, ey1 + 1))
+        px = np.repeat(gx.reshape(-1), 2).astype(np.int32)
+        py = np.repeat(gy.reshape(-1), 2).astype(np.int32)
+        sx = (px + rng.random(len(px))).astype(np.float32)
+        sy = (py + rng.random(len(py))).astype(np.float32)
+        v = rng.uniform(0.5, 2.0, len(px)).astype(np.float32)
+        film = abi.Film()
+        film.width, film.height, film.filter = W, H, kind
+        for k in range(4):
+            film.fparams[k] = p[k]
+        out = np.zeros((H, W, 4), np.float32)
+        lib.orc_film_splat(C.addressof(film), len(px), sx.ctypes.data, sy.ctypes.data, px.ctypes.data,
+                           py.ctypes.data, v.ctypes.data, out.ctypes.data)
+        want = _tiled_film(kind, p, W, H, sx.astype(np.float64), sy.astype(np.float64), px, py, v.astype(np.float64))
+        np.testing.assert_allclose(out[..., 0], want[..., 0], rtol=2e-5, atol=2e-5, err_msg=name)
+        np.testing.assert_allclose(out[..., 2], want[..., 1], rtol=2e-5, atol=2e-5, err_msg=name)
+
+
+def test_film_splats_follow_prenders_tiles():
+    pin_film_tiles()
+
+
+def test_filters_at_closed_form_points():
+    """Values the filter definitions fix: every filter at the origin (gauss
+    (1 - e^(-a w^2))(1 - e^(-a h^2)), mitchell ((6 - 2B) / 6)^2, sinc and
+    triangle 1) and zero at the extent; the film's first table entry (at
+    (fw / 32, fh / 32), mkTableFilter) is the filter's value there."""
+    import ctypes as C
+    from bling_amd import abi
+    lib = pyoracle.load()
+    lib.orc_film_splat.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+    for name, (kind, p) in FILTERS.items():
+        film = abi.Film()
+        film.width, film.height, film.filter = 3, 3, kind
+        for k in range(4):
+            film.fparams[k] = p[k]
+        # one sample at the centre of pixel (1, 1): the table entry of
+        # distance (1/32 fw, 1/32 fh) lands there
+        out = np.zeros((3, 3, 4), np.float32)
+        pos, pix, val = np.float32([1.5]), np.int32([1]), np.float32([1.0])
+        lib.orc_film_splat(C.addressof(film), 1, pos.ctypes.data, pos.ctypes.data, pix.ctypes.data, pix.ctypes.data,
+                           val.ctypes.data, out.ctypes.data)
+        fw, fh = (0.5, 0.5) if kind == 0 else (p[0], p[1])
+        at0 = {0: 1.0, 1: (1 - np.exp(-p[2] * p[0] ** 2)) * (1 - np.exp(-p[2] * p[1] ** 2)),
+               2: ((6 - 2 * p[2]) / 6) ** 2, 3: 1.0, 4: 1.0}[kind]
+        np.testing.assert_allclose(out[1, 1, 0], _filter_eval(kind, p, fw / 32, fh / 32), rtol=2e-5, err_msg=name)
+        assert abs(_filter_eval(kind, p, 0.0, 0.0) - at0) < 1e-12
+        assert abs(_filter_eval(kind, p, fw, 0.0)) < 1e-12
+
+
 # ------------------------------------------------------------- mutations
 MUTATIONS = [
     (mutant.FR_DIEL_ETA, pin_fresnel_dielectric),
@@ -397,6 +536,7 @@ MUTATIONS = [
     (mutant.PATH_PC, lambda: pin_two_mirrors(rr_start=0)),
     (mutant.PATH_T, lambda: pin_two_mirrors(rr_start=7)),
     (mutant.POWER, lambda: pin_two_mirrors(rr_start=7)),
+    (mutant.FILTER_CLIP, pin_film_tiles),
 ]
 
 
