@@ -1,28 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the ray-tracing hot path on one MI355X.
 
-Workload (one "step"): the intersection microbenchmark -- a displaced
-icosphere subdivided to 1,310,720 triangles (0.05 * fBm, 6 octaves, seed 1)
-plus 10,000 spheres (radius log-uniform 0.001..0.05), a binned-SAH BVH
-(12 bins) built on the host, and four launches over rays already resident in
-HBM: closest hit and any hit on 2048x2048 coherent pinhole primary rays and
-on 4,194,304 incoherent rays (origin uniform in the bounds, direction uniform
-on the sphere).  `value` is ray queries per second over the whole step.
+Headline (`value`): Mrays/s of the served path on the serving stream at its
+saturating arrival rate -- Poisson arrivals (200 jobs/s) of render jobs over
+8 resident scenes (the Cornell box and 7 procedural box-stack /
+displaced-grid / icosphere scenes of 0.1M-8M triangles), resolution uniform
+in {128, 256, 512, 1024}^2, spp uniform in {4, 16, 64}, path integrator with
+max depth 5 and Russian roulette after depth 3.  One step is 10 consecutive
+jobs of the stream; `--steps K` times 10 K jobs, after `--warmup W` jobs.
+The same stream goes through the plain path (bgpu_render, one job at a time
+in arrival order) with the same seeds, so both paths render the same images
+(equal RMSE); each path reports Mrays/s, samples/s, p50/p99 job latency and
+its GPU-busy fraction (device time of the bounce loop / makespan).  Rates 50
+and 5 jobs/s are reported beside it.
 
-Two more legs are reported in the same line (skip them with --legs):
-  render  the Cornell box workload (256x256, 4x4 stratified samples per pixel,
-          max depth 5, Russian roulette after depth 3) on the plain path:
-          samples/s, device-counted Mrays/s, and the image RMSE (luminance)
-          against a 4096-spp oracle render committed under tests/golden;
-  serve   a Poisson stream of render jobs over 8 resident scenes (the Cornell
-          box and 7 procedural box-stack / displaced-grid / icosphere scenes of
-          0.1M-8M triangles), each job with its own resolution and spp, run
-          once through the served path (continuous batching) and once through
-          the plain path (one job at a time, in arrival order): Mrays/s,
-          samples/s and p50/p99 job latency per arrival rate, and whether
-          every job's image is the same on both paths.
-The optional `build` leg (--legs build) times scene creation of the largest
-serving scene with the host and the device BVH build.
+Further legs, each in `config` (select with --legs):
+  batch      64 jobs submitted at once, each on its own scene of log-uniform
+             1k-2M triangles at 128^2-1024^2 with 1-32 spp, served vs plain;
+  intersect  closest and any hit on 4,194,304 coherent and 4,194,304
+             incoherent rays against a displaced icosphere of 1,310,720
+             triangles plus 10,000 spheres (rays resident in HBM), checked ray
+             for ray against the oracle;
+  render     the Cornell box, 256^2, 16 spp, max depth 5, plain path, with the
+             image RMSE against a 4096-spp oracle render;
+  divergent  matte/plastic/metal/glass over 2.49M triangles and 200 spheres
+             under a sky map, 1024^2, 64 spp, max depth 16;
+  build      (not default) scene creation with the host and the device BVH build.
+
+`roofline` is the headline workload's dominant kernel, timed per phase with
+HIP events in a profiling pass over its first jobs; `cpu_baseline` is the
+oracle (the C++ restatement of the reference) on a bounded sample of the
+same jobs, on this host's cores.
 
 Prints one JSON line.  Only N=1 is supported: this build targets one GPU.
 """
@@ -39,11 +47,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s and p50/p99 job latency, served vs plain GPU path at equal RMSE"
 PEAK_HBM_GBS = 8000.0
+STEP_JOBS = 10  # jobs of the serving stream per step
 # algorithmic bytes per unit of work: a ray is 8 floats in, a closest hit
 # 4 words out, an any-hit 1 byte out; a node of width w is w child boxes of
-# 24 B plus w links of 4 B (28 w B); a triangle test reads 9 floats
+# 24 B plus w links of 4 B (28 w B); a triangle test reads 9 floats; a
+# shape test reads the 128-byte shape record
 B_RAY_IN, B_HIT_OUT, B_OCC_OUT = 32, 16, 1
-B_NODE_PER_CHILD, B_TRI = 28, 36
+B_NODE_PER_CHILD, B_TRI, B_SHAPE = 28, 36, 128
+# path-pool bytes the shading kernel reads and writes per active path: reads
+# queue entry, job, ray (7 words), hit (3), primitive, depth, pixel, sample,
+# pass, throughput (16), radiance (3) = 144 B and the triangle's indices and
+# vertices (48 B); writes ray, depth, state, throughput, radiance, the shadow
+# ray with its pending XYZ (11 words), the MIS ray with its pending XYZ and
+# light (12 words) and two queue entries = 224 B
+B_SHADE_PATH = 144 + 48 + 224
 CORNELL_REF = os.path.join(ROOT, "tests", "golden", "cornell256_4096spp.npz")
 
 
@@ -61,25 +78,28 @@ def host_threads():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20, help="steps of %d serving jobs (saturating rate)" % STEP_JOBS)
+    ap.add_argument("--warmup", type=int, default=3, help="warmup jobs per path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--res", type=int, default=2048)
+    ap.add_argument("--isect-steps", type=int, default=20, help="timed repetitions of the 4 intersection launches")
     ap.add_argument("--only", default=None,
                     help="time only this launch (e.g. closest/incoherent); for counter runs")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy, the reference for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--legs", default="intersect,render,divergent,serve",
-                    help="comma-separated subset of intersect,render,divergent,serve")
+    ap.add_argument("--legs", default="serve,batch,intersect,render,divergent",
+                    help="comma-separated subset of serve,batch,intersect,render,divergent,build")
     ap.add_argument("--render-steps", type=int, default=10)
-    ap.add_argument("--serve-jobs", type=int, default=32, help="jobs per arrival rate")
-    ap.add_argument("--serve-rates", default="5,50,200", help="Poisson arrival rates, jobs/s")
+    ap.add_argument("--serve-rates", default="200,50,5", help="Poisson arrival rates, jobs/s; the first is the headline")
+    ap.add_argument("--serve-slow-jobs", type=int, default=24, help="jobs at rates below 20 jobs/s (arrival-bound)")
     ap.add_argument("--serve-max-tris", type=float, default=8e6)
     ap.add_argument("--serve-res", default="128,256,512,1024")
     ap.add_argument("--serve-spp", default="4,16,64")
+    ap.add_argument("--profile-jobs", type=int, default=20, help="jobs of the per-phase profiling pass")
+    ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--build-reps", type=int, default=3, help="scene creations per builder in the build leg")
     ap.add_argument("--config", default="",
                     help="tuning knobs, key=value[,key=value] (bgpu_config_set keys)")
@@ -117,14 +137,16 @@ def main():
         "data": "synthetic",
         "config": {"knobs": knobs},
     }
+    if "serve" in legs:
+        serve_leg(ctx, args, out)
+    if "batch" in legs:
+        out["config"]["batch"] = batch_leg(ctx, args)
     if "intersect" in legs:
-        intersect_leg(ctx, args, out)
+        out["config"]["intersect"] = intersect_leg(ctx, args)
     if "render" in legs:
         out["config"]["render"] = render_leg(ctx, args)
     if "divergent" in legs:
         out["config"]["divergent"] = divergent_leg(ctx, args)
-    if "serve" in legs:
-        out["config"]["serve"] = serve_leg(ctx, args)
     if "build" in legs:
         out["config"]["build"] = build_leg(ctx, args)
     ctx.close()
@@ -132,7 +154,10 @@ def main():
 
 
 # ------------------------------------------------------------ intersection
-def intersect_leg(ctx, args, out):
+def intersect_leg(ctx, args):
+    """Closest and any hit on 2048^2 coherent and 4,194,304 incoherent rays
+    (intersection microbenchmark), resident in HBM; every result checked
+    against the oracle's."""
     import ctypes as C
 
     import torch
@@ -146,13 +171,13 @@ def intersect_leg(ctx, args, out):
         "coherent": scenes.primary_rays(flat),
         "incoherent": scenes.incoherent_rays(lo, hi, nr, seed=2),
     }
-    log("workload built in %.1fs: %d tris, %d shapes, 2x%d rays" % (
+    log("intersect: workload built in %.1fs: %d tris, %d shapes, 2x%d rays" % (
         time.time() - t0, flat.num_triangles, flat.num_shapes, nr))
 
     t0 = time.time()
     gs = ctx.scene(flat)
     build_s = time.time() - t0
-    log("scene upload + BVH build %.2fs, %.1f MB on device" % (build_s, gs.device_bytes() / 1e6))
+    log("intersect: scene upload + BVH build %.2fs, %.1f MB on device" % (build_s, gs.device_bytes() / 1e6))
 
     dev = torch.device("cuda:0")
     dbat = {}
@@ -211,106 +236,114 @@ def intersect_leg(ctx, args, out):
     if args.only:
         launches = [(l, f) for l, f in launches if l == args.only]
 
-    # warmup + timed steps
-    for _ in range(args.warmup):
+    # warmup + timed repetitions
+    for _ in range(max(1, args.warmup)):
         for _, fn in launches:
             fn()
     torch.cuda.synchronize()
     kms = {label: [] for label, _ in launches}
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.isect_steps):
         for label, fn in launches:
             kms[label].append(fn(timed=True))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    queries = len(launches) * nr * args.steps
+    queries = len(launches) * nr * args.isect_steps
     avg = {k: float(np.mean(v)) for k, v in kms.items()}
     dom = max(avg, key=lambda k: avg[k])
-    achieved = alg_bytes(dom) / (avg[dom] * 1e-3) / 1e9
     per_launch = {k: {"ms": round(avg[k], 4), "mrays_s": round(nr / avg[k] / 1e3, 1),
                       "nodes_per_ray": round(work[k][0] / nr, 2),
                       "prims_per_ray": round(work[k][1] / nr, 2),
                       "alg_bytes": int(alg_bytes(k)),
                       "alg_GBps": round(alg_bytes(k) / (avg[k] * 1e-3) / 1e9, 1)} for k in avg}
-    log("per launch:", json.dumps(per_launch))
+    log("intersect: per launch:", json.dumps(per_launch))
 
+    # measured HBM traffic of closest/incoherent (committed counter runs)
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", "closest_incoherent_c2_pmc.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    out["value"] = round(queries / wall / 1e6, 2)
-    out["ms_per_step"] = round(wall / args.steps * 1e3, 3)
-    out["config"].update({
-        "workload": "intersection microbench: displaced icosphere %d tris + %d spheres, "
-                    "binned-SAH BVH2 (12 bins); closest+any hit on %d coherent (%dx%d pinhole) "
-                    "and %d incoherent rays per step" % (flat.num_triangles, flat.num_shapes,
-                                                         nr, args.res, args.res, nr),
-        "rays_per_step": len(launches) * nr,
+    achieved = alg_bytes(dom) / (avg[dom] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+        "frac_model": round(achieved / PEAK_HBM_GBS, 4),
+        "frac_hbm": (round(traffic / (avg[dom] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                     if traffic and dom == "closest/incoherent" else None),
+        "note": "achieved counts every node and leaf fetch of the traversal (the stats build's visit counts), "
+                "cache hits included; frac_hbm divides the measured DRAM traffic (FETCH_SIZE + WRITE_SIZE) "
+                "by the same time",
+    }
+    res = {
+        "workload": "intersection microbench: displaced icosphere %d tris + %d spheres, binned-SAH BVH "
+                    "(12 bins, width %d, built on the GPU); closest+any hit on %d coherent (%dx%d pinhole) "
+                    "and %d incoherent rays, resident in HBM" % (
+                        flat.num_triangles, flat.num_shapes, gs.bvh_info()["width"], nr, args.res, args.res, nr),
+        "mrays_s": round(queries / wall / 1e6, 2),
+        "ms_per_step": round(wall / args.isect_steps * 1e3, 3),
+        "steps": args.isect_steps,
         "bvh_build_s": round(build_s, 2),
         "bvh": gs.bvh_info(),
         "per_launch": per_launch,
         "shape_hit_fraction_incoherent": round(shape_frac, 4),
-    })
-    out["roofline"] = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 1),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4),
-        "traffic": traffic,
+        "roofline": roofline,
     }
-    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(flat, batches, args.cpu_seconds)
+    if not args.no_cpu and not args.only:
+        res["parity"], res["cpu_baseline"] = isect_parity_and_cpu(flat, batches, dbat, args.cpu_seconds)
     gs.close()
+    return res
 
 
-def cpu_baseline(flat, batches, seconds):
-    """The oracle (C++ restatement, OpenMP over rays) on a bounded sample of
-    the same four query batches, on this host's cores."""
+def isect_parity_and_cpu(flat, batches, dbat, seconds):
+    """The oracle (C++ restatement, OpenMP over rays) on all four full
+    batches, timed on this host's cores, and its answers compared with the
+    GPU's ray for ray: primitive ids, the bits of t, b1 and b2 of every hit,
+    and every any-hit answer."""
     from oracle.pyoracle import OracleScene
     threads = host_threads()
     o = OracleScene(flat)
-    n = 4096
-    rng = np.random.default_rng(0)
-    samples = {k: v[rng.permutation(len(v))] for k, v in batches.items()}
-    total_q, total_t, reps = 0, 0.0, 0
-    full = len(next(iter(samples.values())))
-    while True:  # grow the sample to the whole batch, then repeat it
+    parity = {}
+    total_q, total_t = 0, 0.0
+    for name, rays in batches.items():
+        cols, outs, occ = dbat[name]
         t0 = time.perf_counter()
-        for k, v in samples.items():
-            o.intersect(v[:n], threads=threads)
-            o.intersects(v[:n], threads=threads)
-        dt = time.perf_counter() - t0
-        if n < full and dt * 4 <= seconds:
-            n = min(full, n * 4)
-            continue
-        total_q += 4 * n
-        total_t += dt
-        reps += 1
-        if total_t >= seconds:
-            break
+        ot, ob1, ob2, oprim, ties = o.intersect(rays, threads=threads)
+        oocc = o.intersects(rays, threads=threads)
+        total_t += time.perf_counter() - t0
+        total_q += 2 * len(rays)
+        gt, gb1, gb2, gprim = (x.cpu().numpy() for x in outs)
+        gocc = occ.cpu().numpy().astype(bool)
+        hit = (gprim == oprim) & (oprim >= 0)
+        parity[name] = {
+            "rays": int(len(rays)), "hits": int((oprim >= 0).sum()),
+            "prim_mismatch": int((gprim != oprim).sum()),
+            "t_bits_mismatch": int((gt[hit].view(np.int32) != ot[hit].view(np.int32)).sum()),
+            "b1_bits_mismatch": int((gb1[hit].view(np.int32) != ob1[hit].view(np.int32)).sum()),
+            "b2_bits_mismatch": int((gb2[hit].view(np.int32) != ob2[hit].view(np.int32)).sum()),
+            "any_mismatch": int((gocc != oocc).sum()),
+            "exact_ties": int(ties),
+        }
+    log("intersect: parity vs oracle:", json.dumps(parity))
     # the same restatement on one core, over a quarter of the time
-    n1 = 4096
+    rng = np.random.default_rng(0)
+    samples = {k: v[rng.permutation(len(v))[:4096]] for k, v in batches.items()}
     q1, t1 = 0, 0.0
     while t1 < seconds / 4:
         t0 = time.perf_counter()
-        for k, v in samples.items():
-            o.intersect(v[:n1], threads=1)
-            o.intersects(v[:n1], threads=1)
+        for v in samples.values():
+            o.intersect(v, threads=1)
+            o.intersects(v, threads=1)
         t1 += time.perf_counter() - t0
-        q1 += 4 * n1
+        q1 += 4 * 4096
     o.close()
-    log("cpu baseline: %d queries in %.2fs on %d threads; %d in %.2fs on 1" % (total_q, total_t, threads, q1, t1))
-    return {"value": round(total_q / total_t / 1e6, 4), "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d rays from each of the 4 query batches (closest+any, coherent+incoherent), "
-                      "%d time(s) over, %.1f s; oracle C++ restatement with its own median BVH, "
-                      "%d OpenMP threads" % (n, reps, total_t, threads),
-            "value_1core": round(q1 / t1 / 1e6, 4),
-            "sample_1core": "4096 rays from each of the 4 query batches, %d queries in %.1f s, one thread"
-                            % (q1, t1)}
+    log("intersect: cpu %d queries in %.2fs on %d threads; %d in %.2fs on 1" % (total_q, total_t, threads, q1, t1))
+    cpu = {"value": round(total_q / total_t / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": "the four full batches (closest+any on 4,194,304 coherent and 4,194,304 incoherent rays), "
+                     "%.1f s; oracle C++ restatement with its own median BVH, %d OpenMP threads" % (total_t, threads),
+           "value_1core": round(q1 / t1 / 1e6, 4),
+           "sample_1core": "4096 rays from each batch, %d queries in %.1f s, one thread" % (q1, t1)}
+    return parity, cpu
 
 
 # ------------------------------------------------------------------ render
@@ -468,44 +501,6 @@ def build_leg(ctx, args):
             **res, "speedup": round(res["host"]["create_s"] / max(1e-9, res["device"]["create_s"]), 2)}
 
 
-def serve_scenes(max_tris):
-    """The Cornell box plus 7 procedural scenes, 0.1M..max_tris triangles
-    (log-spaced), alternating box stacks, displaced grids and an icosphere."""
-    from bling_amd import scenes
-    sizes = np.exp(np.linspace(np.log(1e5), np.log(max_tris), 7))
-    out = [("cornell", scenes.cornell(256))]
-    kinds = ["stack", "grid", "icosphere", "stack", "grid", "stack", "grid"]
-    for k, (kind, n) in enumerate(zip(kinds, sizes)):
-        if kind == "stack":
-            out.append(("stack", scenes.stack_scene(int(n), seed=k)))
-        elif kind == "grid":
-            out.append(("grid", scenes.grid_scene(int(n), seed=k)))
-        else:
-            lv = int(np.clip(np.round(np.log(n / 20.0) / np.log(4.0)), 3, 9))
-            out.append(("icosphere", scenes.icosphere_scene(levels=lv, num_spheres=1000, seed=k, res=256)))
-    return out
-
-
-def job_stream(n, rate, nscenes, res_set, spp_set, seed):
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, n)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    jobs = []
-    for k in range(n):
-        jobs.append({"arrival": float(arrivals[k]), "scene": int(rng.integers(nscenes)),
-                     "res": int(rng.choice(res_set)), "spp": int(rng.choice(spp_set)),
-                     "seed": int(seed * 100003 + k)})
-    return jobs
-
-
-def spec_for(job, flat):
-    from bling_amd.runtime import JobSpec
-    s = int(round(np.sqrt(job["spp"])))
-    nu, nv = (s, job["spp"] // s) if s * (job["spp"] // s) == job["spp"] else (job["spp"], 1)
-    return JobSpec(spp_nu=nu, spp_nv=nv, max_depth=5, sample_depth=3, rr_start_depth=3, passes=1,
-                   seed=job["seed"], view=flat.view(job["res"], job["res"]))
-
-
 def lat_stats(lat):
     a = np.array(lat) * 1e3
     return {"p50_ms": round(float(np.percentile(a, 50)), 2), "p99_ms": round(float(np.percentile(a, 99)), 2),
@@ -513,7 +508,9 @@ def lat_stats(lat):
 
 
 def run_plain(ctx, gss, flats, jobs):
+    """The plain path: bgpu_render one job at a time, in arrival order."""
     from bling_amd.runtime import GpuRenderer
+    from bling_amd.workloads import spec_for
     ctx.reset_stats()
     t_start = time.perf_counter()
     lat, imgs, t_end = [], [], 0.0
@@ -530,7 +527,10 @@ def run_plain(ctx, gss, flats, jobs):
 
 
 def run_served(ctx, gss, flats, jobs):
+    """The served path: each job submitted when it arrives (bgpu_job_submit),
+    continuous batching inside the library."""
     from bling_amd.runtime import Job
+    from bling_amd.workloads import spec_for
     ctx.reset_stats()
     t_start = time.perf_counter()
     handles, lag = [], []
@@ -541,7 +541,7 @@ def run_served(ctx, gss, flats, jobs):
         lag.append(time.perf_counter() - t_start - job["arrival"])
         handles.append(Job(gss[job["scene"]], spec_for(job, flats[job["scene"]])))
     lat, imgs = [], []
-    for h, lg in zip(handles, lag):
+    for h in handles:
         h.wait(timeout=600)
     t_end = time.perf_counter() - t_start
     for h, lg in zip(handles, lag):
@@ -553,78 +553,204 @@ def run_served(ctx, gss, flats, jobs):
     return lat, imgs, min(t_end, makespan), st
 
 
-def serve_leg(ctx, args):
+def path_row(st, span, lat):
+    return dict(mrays_s=round(rays_of(st) / span / 1e6, 2), msamples_s=round(st["samples"] / span / 1e6, 2),
+                makespan_s=round(span, 3), gpu_busy=round(st["gpu_ms"] * 1e-3 / span, 3), **lat_stats(lat))
+
+
+def compare_paths(ctx, gss, flats, jobs):
+    """Both paths over the same jobs: per-path figures and the largest
+    relative difference between a job's two images."""
+    p_lat, p_img, p_span, p_st = run_plain(ctx, gss, flats, jobs)
+    s_lat, s_img, s_span, s_st = run_served(ctx, gss, flats, jobs)
+    worst, weights_equal = 0.0, True
+    for a, b in zip(s_img, p_img):
+        weights_equal = weights_equal and bool(np.array_equal(a[..., 0], b[..., 0]))
+        d = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+        worst = max(worst, float(d.max()))
+    row = {
+        "jobs": len(jobs),
+        "samples": int(s_st["samples"]),
+        "served": path_row(s_st, s_span, s_lat),
+        "plain": path_row(p_st, p_span, p_lat),
+        "rays_served": int(rays_of(s_st)),
+        "rays_plain": int(rays_of(p_st)),
+        "image_max_rel_diff": worst,
+        "image_weights_equal": weights_equal,
+    }
+    row["speedup_mrays"] = round(row["served"]["mrays_s"] / max(1e-9, row["plain"]["mrays_s"]), 3)
+    return row
+
+
+def create_scenes(ctx, named, admission=None):
+    """Device scenes of `named` [(name, flat)]; with `admission` a list, also
+    times creation with the host BVH build and records both."""
+    gss = []
+    for (name, f) in named:
+        host_s = None
+        if admission is not None:
+            ctx.config_set("bvh.device", 0)
+            t1 = time.perf_counter()
+            g = ctx.scene(f)
+            host_s = time.perf_counter() - t1
+            g.close()
+            ctx.config_set("bvh.device", 1)
+        t1 = time.perf_counter()
+        g = ctx.scene(f)
+        dev_s = time.perf_counter() - t1
+        gss.append(g)
+        if admission is not None:
+            info = g.bvh_info()
+            admission.append({"scene": name, "tris": f.num_triangles, "host_build_s": round(host_s, 3),
+                              "device_build_s": round(dev_s, 3), "device_tree": info["device"],
+                              "device_bvh2_ms": round(info["build_ms"], 1)})
+    return gss
+
+
+def profile_pass(ctx, gss, flats, jobs):
+    """Per-phase device time of the bounce loop over `jobs` on the plain
+    path, the iterations launched one by one with events between the
+    phases (pt.profile 1); outside the timed runs."""
+    from bling_amd.runtime import GpuRenderer
+    from bling_amd.workloads import spec_for
+    ctx.config_set("pt.profile", 1)
+    ctx.reset_stats()
+    for job in jobs:
+        GpuRenderer(spec_for(job, flats[job["scene"]])).render_plain(gss[job["scene"]])
+    phases, iters = ctx.profile()
+    ctx.config_set("pt.profile", 0)
+    total = sum(ms for ms, _ in phases.values())
+    return {ph: {"ms": round(ms, 3), "share": round(ms / total, 4), "items": int(n),
+                 "us_per_iteration": round(1e3 * ms / max(1, iters), 2)} for ph, (ms, n) in phases.items()}, iters
+
+
+def serve_cpu(flats, jobs, seconds):
+    """The oracle integrator on the stream's jobs in arrival order, skipping
+    those over 4M samples, until `seconds` of work."""
+    from oracle import pyoracle
+    from bling_amd.workloads import spec_for
+    threads = host_threads()
+    osc = {}
+    dt, rays, samples, done = 0.0, 0, 0, 0
+    for job in jobs:
+        n = (job["res"] + 1) ** 2 * job["spp"]
+        if n > 4_000_000:
+            continue
+        k = job["scene"]
+        if k not in osc:
+            osc[k] = pyoracle.OracleScene(flats[k])  # its build is not timed
+        t0 = time.perf_counter()
+        _, _, r = pyoracle.render(flats[k], spec_for(job, flats[k]), threads=threads, scene=osc[k])
+        dt += time.perf_counter() - t0
+        rays += r
+        samples += n
+        done += 1
+        if dt >= seconds:
+            break
+    for o in osc.values():
+        o.close()
+    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "msamples_s": round(samples / dt / 1e6, 4),
+            "sample": "the first %d jobs of the saturating-rate stream with at most 4M camera samples each "
+                      "(%d samples, %.1f s), oracle C++ restatement of the path integrator, %d OpenMP threads"
+                      % (done, samples, dt, threads)}
+
+
+def serve_leg(ctx, args, out):
+    """The headline: the serving stream at the saturating rate, served vs plain."""
+    from bling_amd.workloads import job_stream, serve_scenes
     t0 = time.time()
     named = serve_scenes(args.serve_max_tris)
     flats = [f for _, f in named]
     log("serve: %d scenes generated in %.1fs" % (len(flats), time.time() - t0))
-    # scene admission: creation with the host BVH build, then with the device
-    # build (the one kept); both include the upload of the shading data
     t0 = time.time()
-    gss, admission = [], []
-    for (name, f) in named:
-        ctx.config_set("bvh.device", 0)
-        t1 = time.perf_counter()
-        g = ctx.scene(f)
-        host_s = time.perf_counter() - t1
-        g.close()
-        ctx.config_set("bvh.device", 1)
-        t1 = time.perf_counter()
-        g = ctx.scene(f)
-        dev_s = time.perf_counter() - t1
-        info = g.bvh_info()
-        gss.append(g)
-        admission.append({"scene": name, "tris": f.num_triangles, "host_build_s": round(host_s, 3),
-                          "device_build_s": round(dev_s, 3), "device_tree": info["device"],
-                          "device_bvh2_ms": round(info["build_ms"], 1)})
-        log("serve: scene %s %d tris created in %.3fs (host BVH) / %.3fs (device BVH)" % (
-            name, f.num_triangles, host_s, dev_s))
-    log("serve: uploaded + BVHs in %.1fs, %.1f MB" % (time.time() - t0,
-                                                       sum(g.device_bytes() for g in gss) / 1e6))
+    admission = []
+    gss = create_scenes(ctx, named, admission)
+    log("serve: uploaded + BVHs in %.1fs, %.1f MB" % (time.time() - t0, sum(g.device_bytes() for g in gss) / 1e6))
     res_set = [int(v) for v in args.serve_res.split(",")]
     spp_set = [int(v) for v in args.serve_spp.split(",")]
-    # warm both paths (engine creation, graph capture)
-    warm = job_stream(2, 1000.0, len(flats), [128], [1], 999)
+    # warmup: engine creation and graph capture on both paths
+    warm = job_stream(max(1, args.warmup), 1000.0, len(flats), [128], [4], seed=999)
     run_plain(ctx, gss, flats, warm)
     run_served(ctx, gss, flats, warm)
-    rows = []
+    rows, streams = [], []
     for ri, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
-        jobs = job_stream(args.serve_jobs, rate, len(flats), res_set, spp_set, seed=ri + 1)
-        p_lat, p_img, p_span, p_st = run_plain(ctx, gss, flats, jobs)
-        s_lat, s_img, s_span, s_st = run_served(ctx, gss, flats, jobs)
-        worst = 0.0
-        for a, b in zip(s_img, p_img):
-            d = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
-            worst = max(worst, float(d.max()))
-        row = {
-            "rate_jobs_s": rate,
-            "jobs": len(jobs),
-            "samples": int(s_st["samples"]),
-            "served": dict(mrays_s=round(rays_of(s_st) / s_span / 1e6, 2),
-                           msamples_s=round(s_st["samples"] / s_span / 1e6, 2),
-                           makespan_s=round(s_span, 3), **lat_stats(s_lat)),
-            "plain": dict(mrays_s=round(rays_of(p_st) / p_span / 1e6, 2),
-                          msamples_s=round(p_st["samples"] / p_span / 1e6, 2),
-                          makespan_s=round(p_span, 3), **lat_stats(p_lat)),
-            "rays_served": int(rays_of(s_st)),
-            "rays_plain": int(rays_of(p_st)),
-            "image_max_rel_diff": worst,
-        }
-        row["speedup_mrays"] = round(row["served"]["mrays_s"] / max(1e-9, row["plain"]["mrays_s"]), 3)
+        n = STEP_JOBS * args.steps if rate >= 20 else args.serve_slow_jobs
+        jobs = job_stream(n, rate, len(flats), res_set, spp_set, seed=ri + 1)
+        row = {"rate_jobs_s": rate, **compare_paths(ctx, gss, flats, jobs)}
         log("serve:", json.dumps(row))
         rows.append(row)
+        streams.append(jobs)
+    head, jobs = rows[0], streams[0]
+    phases, iters = profile_pass(ctx, gss, flats, jobs[:args.profile_jobs])
+    log("serve: profile:", json.dumps(phases))
+    dom = max(phases, key=lambda k: phases[k]["ms"])
+    sh = phases["shade"]
+    achieved = sh["items"] * B_SHADE_PATH / (sh["ms"] * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "shade_serve_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            traffic = json.load(f).get("hbm_bytes_per_path")
+    out["value"] = head["served"]["mrays_s"]
+    out["ms_per_step"] = round(head["served"]["makespan_s"] * 1e3 / args.steps, 3)
+    out["roofline"] = {
+        "bound": "hbm", "kernel": "k_shade (path shading)", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+        "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": None if traffic is None else int(traffic * sh["items"] / max(1, iters)),
+        "share_of_loop": sh["share"], "dominant_phase": dom,
+        "note": "achieved = %d path-pool bytes per shaded path x paths shaded / shade time, per-phase HIP "
+                "events over the first %d jobs (%d iterations); traffic = measured DRAM bytes per shading "
+                "launch (FETCH_SIZE + WRITE_SIZE per path shaded x mean paths per launch)"
+                % (B_SHADE_PATH, len(jobs[:args.profile_jobs]), iters),
+    }
+    out["cpu_baseline"] = None if args.no_cpu else serve_cpu(flats, jobs, args.cpu_seconds)
     for g in gss:
         g.close()
-    return {
-        "workload": "Poisson job stream over 8 resident scenes (%s), resolution uniform in %s, "
-                    "spp uniform in %s, max depth 5, RR after depth 3; served = continuous batching "
-                    "through bgpu_job_submit, plain = bgpu_render one job at a time in arrival order; "
-                    "same seeds on both paths (equal RMSE: the images agree to film-atomic rounding)"
-                    % (", ".join("%s %d tris" % (k, f.num_triangles) for k, f in named),
-                       res_set, spp_set),
-        "rates": rows,
-        "scene_admission": admission,
-    }
+    out["config"].update({
+        "workload": "serving stream at %g jobs/s: %d Poisson jobs over 8 resident scenes (%s), resolution "
+                    "uniform in %s, spp uniform in %s, path integrator, max depth 5, RR after depth 3; "
+                    "value = Mrays/s of the served path (continuous batching through bgpu_job_submit); the "
+                    "plain path (bgpu_render, one job at a time in arrival order) renders the same jobs with "
+                    "the same seeds (equal RMSE)" % (head["rate_jobs_s"], head["jobs"],
+                                                      ", ".join("%s %d tris" % (k, f.num_triangles) for k, f in named),
+                                                      res_set, spp_set),
+        "step": "%d consecutive jobs of the stream" % STEP_JOBS,
+        "served_vs_plain": {"mrays_s": [head["served"]["mrays_s"], head["plain"]["mrays_s"]],
+                            "p50_ms": [head["served"]["p50_ms"], head["plain"]["p50_ms"]],
+                            "p99_ms": [head["served"]["p99_ms"], head["plain"]["p99_ms"]],
+                            "gpu_busy": [head["served"]["gpu_busy"], head["plain"]["gpu_busy"]],
+                            "speedup_mrays": head["speedup_mrays"]},
+        "serve": {"rates": rows, "scene_admission": admission, "profile": phases, "profile_iterations": iters},
+    })
+
+
+def batch_leg(ctx, args):
+    """64 jobs submitted at once, each on its own scene (log-uniform 1k-2M
+    triangles, 128^2-1024^2, 1-32 spp): served (one batch, continuous
+    batching) against plain (one job at a time)."""
+    from bling_amd.workloads import batch_workload
+    t0 = time.time()
+    flats, jobs = batch_workload(n_jobs=args.batch_jobs)
+    log("batch: %d scenes (%d tris) generated in %.1fs" % (len(flats), sum(f.num_triangles for f in flats),
+                                                            time.time() - t0))
+    t0 = time.time()
+    gss = create_scenes(ctx, [("s%d" % k, f) for k, f in enumerate(flats)])
+    create_s = time.time() - t0
+    warm = [dict(j, res=64, spp=1) for j in jobs[:2]]
+    run_plain(ctx, gss, flats, warm)
+    run_served(ctx, gss, flats, warm)
+    row = compare_paths(ctx, gss, flats, jobs)
+    for g in gss:
+        g.close()
+    res = {"workload": "%d jobs at once, each on its own scene: box stacks / displaced grids of %d..%d "
+                       "triangles (log-uniform), resolution %d..%d (log-uniform), spp 1..32; path integrator, "
+                       "max depth 5, RR after depth 3" % (len(jobs), min(f.num_triangles for f in flats),
+                                                          max(f.num_triangles for f in flats),
+                                                          min(j["res"] for j in jobs), max(j["res"] for j in jobs)),
+           "scenes_created_s": round(create_s, 2), **row}
+    log("batch:", json.dumps(res))
+    return res
 
 
 if __name__ == "__main__":

@@ -97,6 +97,13 @@ class JobDesc(C.Structure):
                 ("integrator", C.c_int32), ("camera", C.POINTER(Camera)), ("film", C.POINTER(Film))]
 
 
+PHASES = ("generate", "extend", "classify", "shade", "shadow", "mis", "finish")
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 7), ("items", C.c_uint64 * 7), ("iterations", C.c_uint64)]
+
+
 class BvhInfo(C.Structure):
     _fields_ = [("width", C.c_int32), ("nodes", C.c_int32), ("leaves", C.c_int32),
                 ("max_depth", C.c_int32), ("stack_bound", C.c_int32), ("device", C.c_int32),
@@ -119,7 +126,7 @@ EXPORTS = [
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
     "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get", "bgpu_scene_bvh_info",
     "bgpu_scene_write", "bgpu_scene_read", "bgpu_scene_desc_free", "bgpu_scene_load",
-    "bgpu_mem_info",
+    "bgpu_mem_info", "bgpu_profile_get",
 ]
 
 _lib = None
@@ -160,6 +167,7 @@ def load(path=LIB_PATH):
     lib.bgpu_stats_get.argtypes = [vp, C.POINTER(Stats)]
     lib.bgpu_stats_reset.argtypes = [vp]
     lib.bgpu_mem_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    lib.bgpu_profile_get.argtypes = [vp, C.POINTER(Profile)]
     lib.bgpu_config_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.bgpu_config_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     lib.bgpu_scene_bvh_info.argtypes = [vp, C.POINTER(BvhInfo)]

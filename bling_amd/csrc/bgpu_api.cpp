@@ -180,7 +180,7 @@ static int count_begin(Ctx* c, int flags, unsigned long long** cnt) {
   if (!(flags & BGPU_COUNT)) return BGPU_OK;
   if (!c->count_buf.ptr)
     if (int rc = c->count_buf.alloc(c, 2 * sizeof(unsigned long long))) return rc;
-  BGPU_HIP(hipMemsetAsync(c->count_buf.ptr, 0, 2 * sizeof(unsigned long long), c->stream));
+  BGPU_HIP(zero_words(c->count_buf.as<unsigned int>(), 4, c->stream));
   *cnt = c->count_buf.as<unsigned long long>();
   return BGPU_OK;
 }
@@ -297,6 +297,7 @@ int bgpu_stats_reset(bgpu_ctx* ctx) {
   if (!ctx) return fail(BGPU_E_INVALID, "bgpu_stats_reset: NULL context");
   std::lock_guard<std::mutex> lk(ctx->stats_mu);
   ctx->stats = bgpu_stats{};
+  ctx->prof = bgpu_profile{};
   return BGPU_OK;
 }
 

@@ -60,6 +60,7 @@ struct Ctx {
   int num_cus = 0;
   size_t budget = 0;
   std::atomic<size_t> used{0};
+  std::atomic<size_t> film_bytes{0};  // job films allocated (running or finished, not yet released)
   hipStream_t stream = nullptr;   // intersect / plain render stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -70,6 +71,7 @@ struct Ctx {
   DevBuf work_buf;                // ray counters of the persistent traversal kernels
   std::mutex stats_mu;
   bgpu_stats stats{};
+  bgpu_profile prof{};            // per-phase times while pt.profile is 1
   Scheduler* sched = nullptr;     // served path, created on first job_submit
   Engine* plain = nullptr;        // plain path engine, reused by bgpu_render calls
   std::mutex plain_mu;            // one plain render at a time

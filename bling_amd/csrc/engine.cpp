@@ -71,6 +71,7 @@ struct HostJob {
   }
   virtual ~HostJob() {
     if (counted) counted->live_jobs--;
+    if (bytes && scene) scene->ctx->film_bytes -= bytes;
     film.release();
     sq.release();
     if (scene && --scene->refs == 0) {
@@ -112,9 +113,18 @@ struct Engine {
   // which the slot's table entry may be overwritten.
   std::vector<std::pair<int, unsigned int>> quarantine;  // (slot, iteration at retirement)
   unsigned long long n_admitted = 0;
+  // device time of the iterations: events around each run() (read by poll)
+  hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+  bool run_timed = false;
+  double dev_ms = 0.0;  // total, monotonic
+  hipEvent_t marks[PH_COUNT + 1] = {};  // phase boundaries while profiling
 
   ~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (ev_run0) (void)hipEventDestroy(ev_run0);
+    if (ev_run1) (void)hipEventDestroy(ev_run1);
+    for (hipEvent_t& e : marks)
+      if (e) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (h_admit) (void)hipHostFree(h_admit);
@@ -139,6 +149,10 @@ struct Engine {
     cfg = config;
     P = std::max(1024, (cfg.pool + 1023) / 1024 * 1024);
     BGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    BGPU_HIP(hipEventCreate(&ev_run0));
+    BGPU_HIP(hipEventCreate(&ev_run1));
+    if (cfg.profile)
+      for (hipEvent_t& e : marks) BGPU_HIP(hipEventCreate(&e));
     // pool layout: 9 int arrays + 53 float arrays of P, plus 5 queues, plus
     // the direct-lighting branch stacks (1 + kDlWords * dl_depth arrays)
     size_t per = (size_t)P * 4;
@@ -336,6 +350,8 @@ struct Engine {
     if (int rc = j->film.alloc(ctx, px * 16)) return rc;
     if (d.flags & BGPU_JOB_VARIANCE)
       if (int rc = j->sq.alloc(ctx, px * 12)) return rc;
+    j->bytes = job_bytes(j);
+    ctx->film_bytes += j->bytes;
     int slot = free_slots.back();
     free_slots.pop_back();
     j->slot = slot;
@@ -446,8 +462,40 @@ struct Engine {
     return BGPU_OK;
   }
 
+  // Replays `iters` bounce iterations; their device time is read by poll.
   int run(int iters) {
+    if (cfg.profile) return run_profiled(iters);
+    BGPU_HIP(hipEventRecord(ev_run0, stream));
     for (int k = 0; k < iters; ++k) BGPU_HIP(hipGraphLaunch(gexec, stream));
+    BGPU_HIP(hipEventRecord(ev_run1, stream));
+    run_timed = true;
+    return BGPU_OK;
+  }
+
+  // Profiling (pt.profile 1): the same kernels launched one iteration at a
+  // time without the graph, with events between the phases; each
+  // iteration's phase times and queue sizes go to the context's profile.
+  int run_profiled(int iters) {
+    for (int k = 0; k < iters; ++k) {
+      BGPU_HIP(pt_enqueue_iteration(args, cfg, width, stream, marks));
+      BGPU_HIP(hipMemcpyAsync(h_cnt, args.cnt, sizeof(PtCounters), hipMemcpyDeviceToHost, stream));
+      BGPU_HIP(hipStreamSynchronize(stream));
+      float ms[PH_COUNT];
+      float total = 0.0f;
+      for (int ph = 0; ph < PH_COUNT; ++ph) {
+        BGPU_HIP(hipEventElapsedTime(&ms[ph], marks[ph], marks[ph + 1]));
+        total += ms[ph];
+      }
+      const uint64_t act = h_cnt->n_act;
+      const uint64_t items[PH_COUNT] = {act, act, act, act, h_cnt->n_shadow, h_cnt->n_mis, act};
+      std::lock_guard<std::mutex> lk(ctx->stats_mu);
+      for (int ph = 0; ph < PH_COUNT; ++ph) {
+        ctx->prof.ms[ph] += ms[ph];
+        ctx->prof.items[ph] += items[ph];
+      }
+      ctx->prof.iterations++;
+      dev_ms += total;
+    }
     return BGPU_OK;
   }
 
@@ -459,6 +507,12 @@ struct Engine {
     if (!js.empty())
       BGPU_HIP(hipMemcpyAsync(h_prog, prog_buf.ptr, (size_t)arena_end() * 4, hipMemcpyDeviceToHost, stream));
     BGPU_HIP(hipStreamSynchronize(stream));
+    if (run_timed) {
+      float ms = 0.0f;
+      BGPU_HIP(hipEventElapsedTime(&ms, ev_run0, ev_run1));
+      dev_ms += ms;
+      run_timed = false;
+    }
     for (HostJob* j : js) {
       if (j->slot < 0 || !j->has_prog) continue;
       std::memcpy(j->h_pass.data(), h_prog + j->prog_off, (size_t)j->desc.passes * 4);
@@ -606,17 +660,38 @@ struct Scheduler {
             ++it;
           }
         }
-        // admit by HBM budget, in arrival order
+        // admit by HBM budget, in arrival order.  Films of jobs that are
+        // running or finished but not yet released count against it; a job
+        // that would not fit even once they are all released fails.
         size_t admitting = 0;
+        bool blocked = false;
         while (!pending.empty()) {
           HostJob* j = pending.front();
           size_t need = Engine::job_bytes(j);
-          if (ctx->used.load() + admitting + need > ctx->budget && !(running.empty() && to_admit.empty()))
+          const size_t used = ctx->used.load(), films = ctx->film_bytes.load();
+          if (used - std::min(used, films) + need > ctx->budget) {
+            pending.pop_front();
+            j->state = BGPU_JOB_FAILED;
+            j->t_done = now_ms();
+            set_error(BGPU_E_BUDGET, "job film does not fit the HBM budget beside the resident scenes and engines");
+            if (j->released) delete j;
+            done_cv.notify_all();
+            continue;
+          }
+          if (used + admitting + need > ctx->budget) {
+            blocked = true;
             break;
+          }
           if (!eng.can_admit(j) || to_admit.size() >= eng.free_slots.size()) break;
           pending.pop_front();
           to_admit.push_back(j);
           admitting += need;
+        }
+        // nothing to run and the next job waits for films to be released:
+        // sleep until a release (or a new submission) changes that
+        if (blocked && running.empty() && to_admit.empty()) {
+          cv.wait_for(lk, std::chrono::milliseconds(50));
+          continue;
         }
       }
       for (size_t k = 0; k < to_admit.size(); ++k) {
@@ -648,6 +723,7 @@ struct Scheduler {
         run_now = running;
       }
       double t0 = now_ms();
+      const double dev0 = eng.dev_ms;
       uint64_t seen_before = 0;
       for (HostJob* j : run_now) seen_before += j->samples_seen;
       std::vector<HostJob*> finished;
@@ -674,7 +750,7 @@ struct Scheduler {
       PtCounters cur = *eng.h_cnt;
       uint64_t seen_after = 0;
       for (HostJob* j : run_now) seen_after += j->samples_seen;
-      accumulate_stats(ctx, prev, cur, t1 - t0, seen_after - seen_before);
+      accumulate_stats(ctx, prev, cur, eng.dev_ms - dev0, seen_after - seen_before);
       if (t1 > t0 && seen_after > seen_before)  // throughput estimate for feed_next
         per_ms = 0.9 * per_ms + 0.1 * (double)(seen_after - seen_before) / (t1 - t0);
       prev = cur;
@@ -747,10 +823,10 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
     c->plain = nullptr;
     return rc;
   };
-  double t0 = now_ms();
   if (int rc = eng.poll({}, nullptr)) return bail(rc);
   eng.release_quarantine();
   const PtCounters before = *eng.h_cnt;
+  const double dev0 = eng.dev_ms;
   if (int rc = eng.admit(&j)) return bail(rc);
   if (int rc = eng.feed({{&j, j.total}})) return bail(rc);
   // every path ends within max_depth + 1 bounces, so the job drains in at
@@ -769,9 +845,8 @@ int bgpu_render(bgpu_scene* scene, const bgpu_job_desc* job, float* wxyz, float*
     iters += 4;
     if (iters > cap && done.empty()) return bail(fail(BGPU_E_STATE, "render did not drain within the iteration bound"));
   }
-  double t1 = now_ms();
   j.passes_done = j.passes_seen;
-  accumulate_stats(c, before, *eng.h_cnt, t1 - t0, j.total);
+  accumulate_stats(c, before, *eng.h_cnt, eng.dev_ms - dev0, j.total);
   {
     std::lock_guard<std::mutex> lk2(c->stats_mu);
     c->stats.jobs_done++;
@@ -877,6 +952,7 @@ int bgpu_job_release(bgpu_job* job) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (job->state >= BGPU_JOB_DONE) {
     delete job;
+    s->cv.notify_all();  // its film may let a waiting job in
   } else {
     job->released = true;
     job->cancel_req = true;
@@ -944,6 +1020,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.dl_depth") {
     if (value < 0 || value > 15) return fail(BGPU_E_INVALID, "pt.dl_depth must be in [0, 15]");
     next.dl_depth = (int)value;
+  } else if (k == "pt.profile") {
+    if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "pt.profile must be 0 or 1");
+    next.profile = (int)value;
   } else {
     return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   }
@@ -962,6 +1041,13 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   return BGPU_OK;
 }
 
+int bgpu_profile_get(bgpu_ctx* ctx, bgpu_profile* out) {
+  if (!ctx || !out) return fail(BGPU_E_INVALID, "bgpu_profile_get: NULL argument");
+  std::lock_guard<std::mutex> lk(ctx->stats_mu);
+  *out = ctx->prof;
+  return BGPU_OK;
+}
+
 int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   if (!ctx || !key || !value) return fail(BGPU_E_INVALID, "bgpu_config_get: NULL argument");
   const std::string k(key);
@@ -976,6 +1062,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
   else if (k == "pt.sort") *value = ctx->pt_cfg.sort;
   else if (k == "pt.dl_depth") *value = ctx->pt_cfg.dl_depth;
+  else if (k == "pt.profile") *value = ctx->pt_cfg.profile;
   else return fail(BGPU_E_INVALID, "unknown configuration key: " + k);
   return BGPU_OK;
 }

@@ -74,6 +74,9 @@ struct TraceConfig {
   int persistent = 1;  // lanes refill with new rays (1) or one ray per thread (0)
 };
 
+// Zeroes n 32-bit words on stream s with a kernel.
+hipError_t zero_words(unsigned int* p, int n, hipStream_t s);
+
 // `work`: two zeroed-on-launch counters for the persistent kernels (may be
 // null, which selects one ray per thread).
 hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,

@@ -7,6 +7,13 @@ namespace bgpu {
 
 namespace {
 
+// Zeroes n words (a kernel rather than a memset node, so that a caller
+// capturing the launches into a graph replays it correctly).
+__global__ void k_zero_words(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 __device__ __forceinline__ DRay load_ray(const DevRays& r, int i) {
   DRay ray;
   ray.o = mk3(r.ox[i], r.oy[i], r.oz[i]);
@@ -152,8 +159,7 @@ template <int BLOCK, int W, bool ANY, bool STATS>
 hipError_t run_persist(const DevScene& sc, const DevRays& r, int n, const DevHits& h, uint8_t* occ,
                        unsigned int* work, hipStream_t s, unsigned long long* counters) {
   // one counter per pass, zeroed together
-  hipError_t e = hipMemsetAsync(work, 0, 2 * sizeof(unsigned int), s);
-  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, work, 2);
   static int g_tri = 0, g_shape = 0;
   if (!g_tri) g_tri = trav::persistent_grid(k_persist<BLOCK, W, ANY, false, STATS>, BLOCK);
   if (!g_shape) g_shape = trav::persistent_grid(k_persist<BLOCK, W, ANY, true, STATS>, BLOCK);
@@ -179,6 +185,12 @@ hipError_t run_persist_any_w(const DevScene& sc, const DevRays& r, int n, const 
 }
 
 }  // namespace
+
+hipError_t zero_words(unsigned int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_words, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h,
                           const TraceConfig& cfg, hipStream_t s, unsigned long long* counters,

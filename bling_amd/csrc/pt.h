@@ -164,7 +164,12 @@ struct PtConfig {
   int sort = 2;         // shading queue grouped by material kind: 1 always, 0 never, 2 while a
                         // resident scene has more than one BxDF makeup
   int dl_depth = 4;     // pending specular branches per direct-lighting path
+  int profile = 0;      // 1: iterations run as plain launches, timed per phase (no graph)
 };
+
+// Phases of one bounce iteration, as timed by profiling (bgpu_profile).
+enum { PH_GENERATE = 0, PH_EXTEND = 1, PH_CLASSIFY = 2, PH_SHADE = 3, PH_SHADOW = 4, PH_MIS = 5, PH_FINISH = 6,
+       PH_COUNT = 7 };
 
 struct PtKernelArgs {
   PathPool pool;
@@ -198,7 +203,10 @@ int make_view(const bgpu_camera& cam, const bgpu_film& film, DevView* v);  // 0 
 // Enqueues one bounce iteration (reset, generate, extend, shade, shadow,
 // MIS, finish) on `s`.  Capturable into a hipGraph.  `width`: the BVH width
 // of every scene the iteration will see (2, 4 or 8), or 0 for any width.
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s);
+// `marks`: null, or PH_COUNT + 1 events recorded before the first phase and
+// after each phase (for profiling; not for capture).
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s,
+                                hipEvent_t* marks = nullptr);
 // Enqueues the admission of jobs listed in `admit` (count n) on `s`.
 hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s);
 hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s);

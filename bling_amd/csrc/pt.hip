@@ -1212,8 +1212,10 @@ constexpr int kWideBlock = 1024;
 #endif
 constexpr int kDiffuseWaves = BGPU_DIFFUSE_WAVES;
 
+static inline void mark(hipEvent_t* marks, int k, hipStream_t s);
+
 template <int BLOCK>
-static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
+static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s, hipEvent_t* marks) {
   if (a.sort_shade) {  // group the queue by shading class first
     constexpr int kSortBlock = 1024;  // fewer blocks, fewer atomics on the class counters
     dim3 gs((a.pool.P + kSortBlock - 1) / kSortBlock), bs(kSortBlock);
@@ -1221,6 +1223,7 @@ static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipSt
     hipLaunchKernelGGL(k_class_count<kSortBlock>, gs, bs, 0, s, a);
     hipLaunchKernelGGL(k_class_scatter<kSortBlock>, gs, bs, 0, s, a);
   }
+  mark(marks, PH_CLASSIFY + 1, s);
   if (a.shade_mode == SHADE_MODE_DIFFUSE) {
     hipLaunchKernelGGL((k_shade<BLOCK, kDiffuseWaves, kDiffuseKinds, SHADE_ALL>), g, b, 0, s, a);
   } else if (a.shade_mode == SHADE_MODE_SPLIT) {
@@ -1233,14 +1236,23 @@ static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipSt
     hipLaunchKernelGGL((k_shade<BLOCK, 1, kAllKinds, SHADE_ALL>), g, b, 0, s, a);
   }
   if (a.direct) hipLaunchKernelGGL((k_shade_direct<BLOCK>), g, b, 0, s, a);
+  mark(marks, PH_SHADE + 1, s);
+}
+
+// Records the phase boundary event k when profiling (marks non-null).
+static inline void mark(hipEvent_t* marks, int k, hipStream_t s) {
+  if (marks) (void)hipEventRecord(marks[k], s);
 }
 
 template <int BLOCK, int W>
-static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool persistent, hipStream_t s) {
+static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool persistent, hipStream_t s,
+                               hipEvent_t* marks) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
   dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
+  mark(marks, 0, s);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
+  mark(marks, PH_GENERATE + 1, s);
   if constexpr (W != 0) {
     if (persistent) {
       static int ge = 0, gs = 0, gm = 0;  // grids that fill the device once
@@ -1249,40 +1261,50 @@ static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool pers
       if (!gm) gm = trav::persistent_grid(k_mis_p<BLOCK, W, false>, BLOCK);
       hipLaunchKernelGGL((k_extend_p<BLOCK, W, false>), dim3(ge), b, 0, s, a);
       hipLaunchKernelGGL((k_extend_p<BLOCK, W, true>), dim3(ge), b, 0, s, a);
-      launch_shade<BLOCK>(a, shade_waves, g, b, s);
+      mark(marks, PH_EXTEND + 1, s);
+      launch_shade<BLOCK>(a, shade_waves, g, b, s, marks);
       hipLaunchKernelGGL((k_shadow_p<BLOCK, W, false>), dim3(gs), b, 0, s, a);
       hipLaunchKernelGGL((k_shadow_p<BLOCK, W, true>), dim3(gs), b, 0, s, a);
+      mark(marks, PH_SHADOW + 1, s);
       hipLaunchKernelGGL((k_mis_p<BLOCK, W, false>), dim3(gm), b, 0, s, a);
       hipLaunchKernelGGL((k_mis_p<BLOCK, W, true>), dim3(gm), b, 0, s, a);
       hipLaunchKernelGGL((k_mis_resolve<BLOCK>), g, b, 0, s, a);
+      mark(marks, PH_MIS + 1, s);
       hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
+      mark(marks, PH_FINISH + 1, s);
       return hipGetLastError();
     }
   }
   hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, 0, s, a);
-  launch_shade<BLOCK>(a, shade_waves, g, b, s);
+  mark(marks, PH_EXTEND + 1, s);
+  launch_shade<BLOCK>(a, shade_waves, g, b, s, marks);
   hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, 0, s, a);
+  mark(marks, PH_SHADOW + 1, s);
   hipLaunchKernelGGL((k_mis<BLOCK, W>), g, b, 0, s, a);
+  mark(marks, PH_MIS + 1, s);
   hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
+  mark(marks, PH_FINISH + 1, s);
   return hipGetLastError();
 }
 
 template <int BLOCK>
-static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, const PtConfig& cfg, hipStream_t s) {
+static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, const PtConfig& cfg, hipStream_t s,
+                                 hipEvent_t* marks) {
   const bool p = cfg.persistent != 0;
   switch (width) {
-    case 2: return enqueue_iter<BLOCK, 2>(a, cfg.shade_waves, p, s);
-    case 4: return enqueue_iter<BLOCK, 4>(a, cfg.shade_waves, p, s);
-    case 8: return enqueue_iter<BLOCK, 8>(a, cfg.shade_waves, p, s);
-    default: return enqueue_iter<BLOCK, 0>(a, cfg.shade_waves, p, s);
+    case 2: return enqueue_iter<BLOCK, 2>(a, cfg.shade_waves, p, s, marks);
+    case 4: return enqueue_iter<BLOCK, 4>(a, cfg.shade_waves, p, s, marks);
+    case 8: return enqueue_iter<BLOCK, 8>(a, cfg.shade_waves, p, s, marks);
+    default: return enqueue_iter<BLOCK, 0>(a, cfg.shade_waves, p, s, marks);
   }
 }
 
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s) {
+hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s,
+                                hipEvent_t* marks) {
   switch (cfg.block) {
-    case 64: return enqueue_iter_w<64>(a, width, cfg, s);
-    case 256: return enqueue_iter_w<256>(a, width, cfg, s);
-    default: return enqueue_iter_w<128>(a, width, cfg, s);
+    case 64: return enqueue_iter_w<64>(a, width, cfg, s, marks);
+    case 256: return enqueue_iter_w<256>(a, width, cfg, s, marks);
+    default: return enqueue_iter_w<128>(a, width, cfg, s, marks);
   }
 }
 

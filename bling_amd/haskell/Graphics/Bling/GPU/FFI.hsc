@@ -386,6 +386,19 @@ instance Storable BvhInfo where
     #{poke bgpu_bvh_info, device} p (biDevice b)
     #{poke bgpu_bvh_info, build_ms} p (biBuildMs b)
 
+-- | bgpu_profile: device time per phase of the bounce loop (pt.profile 1).
+data Profile = Profile { pfMs :: [Double], pfItems :: [Word64], pfIterations :: !Word64 }
+
+instance Storable Profile where
+  sizeOf _ = #{size bgpu_profile}
+  alignment _ = #{alignment bgpu_profile}
+  peek p = Profile <$> peekArray #{const BGPU_PHASES} (#{ptr bgpu_profile, ms} p)
+    <*> peekArray #{const BGPU_PHASES} (#{ptr bgpu_profile, items} p) <*> #{peek bgpu_profile, iterations} p
+  poke p f = do
+    pokeArray (#{ptr bgpu_profile, ms} p) (take #{const BGPU_PHASES} (pfMs f ++ repeat 0))
+    pokeArray (#{ptr bgpu_profile, items} p) (take #{const BGPU_PHASES} (pfItems f ++ repeat 0))
+    #{poke bgpu_profile, iterations} p (pfIterations f)
+
 -- * Entry points
 
 foreign import ccall unsafe "bgpu_last_error" c_lastError :: IO CString
@@ -415,6 +428,7 @@ foreign import ccall unsafe "bgpu_job_latency_ms" c_jobLatencyMs :: Ptr Job -> P
 foreign import ccall unsafe "bgpu_stats_get" c_statsGet :: Ptr Ctx -> Ptr Stats -> IO CInt
 foreign import ccall unsafe "bgpu_stats_reset" c_statsReset :: Ptr Ctx -> IO CInt
 foreign import ccall unsafe "bgpu_mem_info" c_memInfo :: Ptr Ctx -> Ptr CSize -> Ptr CSize -> IO CInt
+foreign import ccall unsafe "bgpu_profile_get" c_profileGet :: Ptr Ctx -> Ptr Profile -> IO CInt
 foreign import ccall safe "bgpu_config_set" c_configSet :: Ptr Ctx -> CString -> Int64 -> IO CInt
 foreign import ccall unsafe "bgpu_config_get" c_configGet :: Ptr Ctx -> CString -> Ptr Int64 -> IO CInt
 foreign import ccall unsafe "bgpu_scene_bvh_info" c_sceneBvhInfo :: Ptr Scene -> Ptr BvhInfo -> IO CInt

@@ -59,6 +59,13 @@ class Context:
         _ck(self.lib.bgpu_mem_info(self.h, C.byref(used), C.byref(budget)))
         return used.value, budget.value
 
+    def profile(self):
+        """Per-phase device time of the bounce loop (pt.profile 1):
+        {phase: (ms, items)} and the iteration count."""
+        p = abi.Profile()
+        _ck(self.lib.bgpu_profile_get(self.h, C.byref(p)))
+        return {ph: (p.ms[k], p.items[k]) for k, ph in enumerate(abi.PHASES)}, p.iterations
+
     def reset_stats(self):
         _ck(self.lib.bgpu_stats_reset(self.h))
 

@@ -146,7 +146,10 @@ def cornell(res=256, light_radiance=15.0, height=None):
     return sb.build()
 
 
-def icosphere_scene(levels=8, num_spheres=10000, seed=1, amplitude=0.05, res=2048):
+def icosphere_scene(levels=8, num_spheres=10000, seed=1, amplitude=0.05, res=2048, lit=False):
+    """A displaced icosphere (amplitude * fBm) among spheres of log-uniform
+    radius; `lit` adds a uniform sky (for rendering; the ray queries do not
+    need light)."""
     v, f = icosphere(levels)
     r = 1.0 + amplitude * fbm(v, octaves=6, omega=0.5, seed=seed)
     v = (v * r[:, None]).astype(F)
@@ -158,6 +161,8 @@ def icosphere_scene(levels=8, num_spheres=10000, seed=1, amplitude=0.05, res=204
     rad = np.exp(rng.uniform(np.log(0.001), np.log(0.05), num_spheres))
     for c, rr in zip(cen, rad):
         sb.shape(abi.SHAPE_SPHERE, (float(rr),), o2w=T.translate(tuple(c)), material=m)
+    if lit:
+        sb.infinite_light(np.tile(const(1.0), (2, 4, 1)))
     c2w = T.look_at((0.6, 0.9, -3.4), (0.0, 0.0, 0.0), (0, 1, 0))
     sb.look(c2w, 50.0, res, res)
     return sb.build()

@@ -257,7 +257,8 @@ typedef struct {
   uint64_t rays_camera, rays_extend, rays_shadow, rays_mis;
   uint64_t samples;          /* camera samples completed */
   uint64_t jobs_done;
-  double gpu_ms;             /* device time of the render launches */
+  double gpu_ms;             /* device time of the bounce-loop iterations (events
+                                around each batch of iterations) */
   uint64_t nodes_visited;    /* BGPU_COUNT queries: BVH node fetches */
   uint64_t prims_tested;     /* BGPU_COUNT queries: primitive tests */
   uint64_t branches_dropped; /* direct-lighting specular branches lost to a full
@@ -320,6 +321,22 @@ int bgpu_stats_get(bgpu_ctx* ctx, bgpu_stats* out);
 int bgpu_mem_info(bgpu_ctx* ctx, size_t* used, size_t* budget);
 int bgpu_stats_reset(bgpu_ctx* ctx);
 
+/* Device time per phase of the bounce loop (no counterpart in the
+ * reference), collected while the "pt.profile" knob is 1: the iterations
+ * then run as plain kernel launches with events between the phases instead
+ * of graph replays.  Phases: 0 generate (camera rays), 1 extend (closest
+ * hit of the path rays), 2 classify (grouping by material), 3 shade, 4 shadow
+ * (any hit), 5 MIS (closest hit of the BSDF-sampled rays), 6 finish (film).
+ * items: queue entries of the phase (active paths; shadow and MIS rays).
+ * Cleared by bgpu_stats_reset. */
+enum { BGPU_PHASES = 7 };
+typedef struct {
+  double ms[BGPU_PHASES];
+  uint64_t items[BGPU_PHASES];
+  uint64_t iterations;
+} bgpu_profile;
+int bgpu_profile_get(bgpu_ctx* ctx, bgpu_profile* out);
+
 /* Tuning knobs (no counterpart in the reference, whose CPU integrator has
  * none): "bvh.width" (2/4/8 children per BVH node of scenes created
  * afterwards; a scene whose worst-case traversal stack would not fit falls
@@ -334,6 +351,8 @@ int bgpu_stats_reset(bgpu_ctx* ctx);
  * "pt.pool" (path slots, >= 1024), "pt.dl_depth" (0..15, default 4:
  * pending specular branches a direct-lighting path can hold, so its jobs may
  * use maxDepth <= pt.dl_depth + 1; 96 bytes per path slot per entry),
+ * "pt.profile" (1: time the bounce loop per phase, bgpu_profile_get; 0, the
+ * default: replay the captured graph),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
  * two waves per SIMD), "pt.sort" (1: the shading queue is grouped by
  * material kind each bounce, so that a wave runs one BSDF code path; 0: path

@@ -41,7 +41,7 @@ def test_hbm_budget_admission(gpu_ctx):
         gs = _warm(a, flat)
         used0, _ = a.mem_info()
         gs.close()
-    budget = used0 + film + film // 2
+    budget = used0 + film + film // 2  # room for one film, not two
     with Context(0, hbm_budget=budget) as b:
         gs = _warm(b, flat)
         try:
@@ -53,13 +53,14 @@ def test_hbm_budget_admission(gpu_ctx):
             assert e.value.code == abi.E_BUDGET
             jobs = [Job(gs, sp) for sp in specs]
             waited = False
+            served, lat = {}, {}
             deadline = time.time() + 120
-            while True:
+            while len(served) < len(jobs):
                 # later jobs first: a later job seen running implies that
                 # every earlier one had finished before that observation
                 st = [None] * len(jobs)
                 for k in reversed(range(len(jobs))):
-                    st[k] = jobs[k].poll(want_image=False)[3]
+                    st[k] = abi.JOB_DONE if k in served else jobs[k].poll(want_image=False)[3]
                 for k, s in enumerate(st):
                     assert s != abi.JOB_FAILED
                     if s == abi.JOB_RUNNING:
@@ -68,16 +69,26 @@ def test_hbm_budget_admission(gpu_ctx):
                         waited = waited or k + 1 < len(jobs)
                 used, _ = b.mem_info()
                 assert used <= budget
-                if all(s == abi.JOB_DONE for s in st):
-                    break
+                # a finished job keeps its film until released: take the
+                # image and release it, which lets the next job in
+                for k, s in enumerate(st):
+                    if s == abi.JOB_DONE and k not in served:
+                        served[k] = jobs[k].poll()[0]
+                        lat[k] = jobs[k].latency_ms()
+                        jobs[k].release()
                 assert time.time() < deadline, st
                 time.sleep(0.0005)
             assert waited, "no job was seen waiting for the budget"
-            lat = [j.latency_ms() for j in jobs]
             assert lat[0] < lat[1] < lat[2], lat
-            served = [j.poll()[0] for j in jobs]
-            for j in jobs:
-                j.release()
+            served = [served[k] for k in range(len(jobs))]
+            # a job whose film cannot fit even with every other film released fails
+            lone = JobSpec(spp_nu=1, spp_nv=1, max_depth=2, sample_depth=2, view=flat.view(512, 512))
+            j = Job(gs, lone)
+            with pytest.raises(abi.BgpuError) as e:
+                j.wait(timeout=60)
+            assert e.value.code != abi.E_TIMEOUT
+            assert j.poll(want_image=False)[3] == abi.JOB_FAILED
+            j.release()
         finally:
             gs.close()
     # each result equals the job rendered alone
