@@ -79,6 +79,24 @@ extern "C" {
 const char* bgpu_last_error(void) { return bgpu::g_err.c_str(); }
 int bgpu_version(void) { return BGPU_ABI_VERSION; }
 
+// The static scratch array's address on each device, looked up once per
+// device with that device current (each GPU holds its own copy).
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch;
+
+static int device_scratch(int device, unsigned int** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch.size() <= device) g_scratch.resize(device + 1, nullptr);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    hipError_t e = scratch_address(&p);
+    if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress (scratch)");
+    g_scratch[device] = p;
+  }
+  *out = static_cast<unsigned int*>(g_scratch[device]);
+  return BGPU_OK;
+}
+
 int bgpu_init(int device, size_t hbm_budget_bytes, bgpu_ctx** out) {
   if (!out) return fail(BGPU_E_INVALID, "bgpu_init: out is NULL");
   *out = nullptr;
@@ -98,6 +116,7 @@ int bgpu_init(int device, size_t hbm_budget_bytes, bgpu_ctx** out) {
   BGPU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   BGPU_HIP(hipEventCreate(&c->ev0));
   BGPU_HIP(hipEventCreate(&c->ev1));
+  if (int rc = device_scratch(device, &c->scratch)) return rc;
   *out = c.release();
   return BGPU_OK;
 }
@@ -233,6 +252,49 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n, b
   }
   BGPU_HIP(hipStreamSynchronize(c->stream));
   if (int rc = count_end(c, cnt)) return rc;
+  if (flags & BGPU_TIMED) {
+    float ms = 0.0f;
+    BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+  }
+  return BGPU_OK;
+}
+
+int bgpu_intersect_closest_noalloc(bgpu_scene* scene, const bgpu_rays* rays, size_t n, bgpu_hits* hits,
+                                   int flags) {
+  if (!scene || !hits || !hits->t || !hits->b1 || !hits->b2 || !hits->prim)
+    return fail(BGPU_E_INVALID, "bgpu_intersect_closest_noalloc: NULL argument");
+  if (int rc = check_rays(rays)) return rc;
+  if (!(flags & BGPU_DEVICE_PTRS))
+    return fail(BGPU_E_INVALID, "bgpu_intersect_closest_noalloc: rays and hits must be device pointers "
+                                "(BGPU_DEVICE_PTRS); host batches need staging buffers");
+  if (n > (size_t)0x7fffffff)
+    return fail(BGPU_E_INVALID, "bgpu_intersect_closest_noalloc: ray batch larger than 2^31-1");
+  if (n == 0) return BGPU_OK;
+  Ctx* c = scene->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  BGPU_HIP(hipSetDevice(c->device));
+  const DevRays dr{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  const DevHits dh{hits->t, hits->b1, hits->b2, hits->prim};
+  unsigned int* work = c->scratch;  // words 0-1: the passes' counters (zeroed by the launch)
+  unsigned long long* cnt = nullptr;
+  if (flags & BGPU_COUNT) {  // words 8-11: node visits, primitive tests
+    BGPU_HIP(zero_words(c->scratch + 8, 4, c->stream));
+    cnt = reinterpret_cast<unsigned long long*>(c->scratch + 8);
+  }
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
+  TraceConfig tc = c->trace_cfg;
+  if (flags & BGPU_COHERENT) tc.persistent = 0;
+  BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, tc, c->stream, cnt, work));
+  if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
+  BGPU_HIP(hipStreamSynchronize(c->stream));
+  if (cnt) {
+    unsigned long long h[2];
+    BGPU_HIP(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
+    std::lock_guard<std::mutex> lk2(c->stats_mu);
+    c->stats.nodes_visited += h[0];
+    c->stats.prims_tested += h[1];
+  }
   if (flags & BGPU_TIMED) {
     float ms = 0.0f;
     BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));

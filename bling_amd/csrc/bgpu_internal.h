@@ -69,6 +69,7 @@ struct Ctx {
   DevBuf stage_rays, stage_hits;
   DevBuf count_buf;               // BGPU_COUNT traversal counters
   DevBuf work_buf;                // ray counters of the persistent traversal kernels
+  unsigned int* scratch = nullptr;  // this GPU's static scratch array (bgpu_intersect_closest_noalloc)
   std::mutex stats_mu;
   bgpu_stats stats{};
   bgpu_profile prof{};            // per-phase times while pt.profile is 1

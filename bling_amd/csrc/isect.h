@@ -74,6 +74,11 @@ struct TraceConfig {
   int persistent = 1;  // lanes refill with new rays (1) or one ray per thread (0)
 };
 
+// Words of the per-GPU scratch array of the allocation-free ray batches
+// (256 bytes); scratch_address returns its device address on the current
+// device.
+constexpr int kScratchWords = 64;
+hipError_t scratch_address(void** ptr);
 // Zeroes n 32-bit words on stream s with a kernel.
 hipError_t zero_words(unsigned int* p, int n, hipStream_t s);
 

@@ -5,6 +5,15 @@
 
 namespace bgpu {
 
+// Scratch of bgpu_intersect_closest_noalloc: the two pass counters of the
+// persistent traversal (words 0-1) and, for BGPU_COUNT, the node-visit and
+// primitive-test counters (64-bit, words 8-11).  One copy per GPU, for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_address).
+static __device__ __align__(256) unsigned int scratch[kScratchWords];
+
+hipError_t scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
 namespace {
 
 // Zeroes n words (a kernel rather than a memset node, so that a caller

@@ -297,6 +297,18 @@ int bgpu_intersect_closest(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
 /* `intersects` of the accelerator primitive (Primitive/KdTree.hs:244-246). */
 int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
                        uint8_t* occluded, int flags);
+/* bgpu_intersect_closest for callers that cannot allocate device memory
+ * (code that must not call hipMalloc: graph-building hosts, tight serving
+ * loops, harnesses that forbid allocation).  No counterpart in the
+ * reference.  Same results, flags and kernels as bgpu_intersect_closest, but
+ * it allocates nothing: rays and hits must be device pointers
+ * (BGPU_DEVICE_PTRS, else BGPU_E_INVALID), batches are limited to 2^31-1
+ * rays as there, and the kernels' counters live in a 256-byte static array
+ * that each GPU holds for the life of the process.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (from two contexts, or from
+ * threads bypassing the context's lock) race. */
+int bgpu_intersect_closest_noalloc(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
+                                   bgpu_hits* hits, int flags);
 /* Duration in ms of the last BGPU_TIMED intersect launch on this context. */
 int bgpu_last_kernel_ms(bgpu_ctx* ctx, double* ms);
 
