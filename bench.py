@@ -25,7 +25,9 @@ Further legs, each in `config` (select with --legs):
              image RMSE against a 4096-spp oracle render;
   divergent  matte/plastic/metal/glass over 2.49M triangles and 200 spheres
              under a sky map, 1024^2, 64 spp, max depth 16;
-  build      (not default) scene creation with the host and the device BVH build.
+  build      (not default) scene creation with the host and the device BVH build;
+  phases     (not default, for counter runs) only the per-phase profiling pass
+             of the headline workload, with the items of each phase.
 
 `roofline` is the headline workload's dominant kernel, timed per phase with
 HIP events in a profiling pass over its first jobs; `cpu_baseline` is the
@@ -51,9 +53,16 @@ STEP_JOBS = 10  # jobs of the serving stream per step
 # algorithmic bytes per unit of work: a ray is 8 floats in, a closest hit
 # 4 words out, an any-hit 1 byte out; a node of width w is w child boxes of
 # 24 B plus w links of 4 B (28 w B); a triangle test reads 9 floats; a
-# shape test reads the 128-byte shape record
+# shape test reads its type/parameter index, the 48-byte w2o matrix and 16 B
+# of parameters (68 B)
 B_RAY_IN, B_HIT_OUT, B_OCC_OUT = 32, 16, 1
-B_NODE_PER_CHILD, B_TRI, B_SHAPE = 28, 36, 128
+B_NODE_PER_CHILD, B_TRI, B_SHAPE = 28, 36, 68
+# VALU operations per unit: a child-box slab test 21 (6 sub/mul pairs, 3
+# swaps, 6 min/max), a triangle test 52 (51 and a divide), a shape test 40
+# (ray transform and quadratic); against the FP32 issue rate of 256 CUs x 128
+# lanes x 2.4 GHz (78.6 TFLOP/s counts an FMA as two)
+OPS_SLAB, OPS_TRI, OPS_SHAPE = 21, 52, 40
+PEAK_VALU_OPS = 78.6e12 / 2
 # path-pool bytes the shading kernel reads and writes per active path: reads
 # queue entry, job, ray (7 words), hit (3), primitive, depth, pixel, sample,
 # pass, throughput (16), radiance (3) = 144 B and the triangle's indices and
@@ -85,6 +94,9 @@ def main():
     ap.add_argument("--levels", type=int, default=8)
     ap.add_argument("--res", type=int, default=2048)
     ap.add_argument("--isect-steps", type=int, default=20, help="timed repetitions of the 4 intersection launches")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the intersection launches at the current knobs instead of tuning "
+                         "trace.block and trace.stack on them first")
     ap.add_argument("--only", default=None,
                     help="time only this launch (e.g. closest/incoherent); for counter runs")
     ap.add_argument("--copy-ref", action="store_true",
@@ -119,8 +131,8 @@ def main():
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
-    for k in ("bvh.width", "bvh.device", "trace.block", "trace.persistent", "pt.block", "pt.pool", "pt.shade_waves",
-              "pt.persistent", "pt.sort"):
+    for k in ("bvh.width", "bvh.device", "trace.block", "trace.persistent", "trace.stack", "pt.block", "pt.pool",
+              "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.persistent", "pt.sort"):
         knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,
@@ -137,6 +149,8 @@ def main():
         "data": "synthetic",
         "config": {"knobs": knobs},
     }
+    if args.copy_ref:
+        copy_ref()
     if "serve" in legs:
         serve_leg(ctx, args, out)
     if "batch" in legs:
@@ -149,8 +163,22 @@ def main():
         out["config"]["divergent"] = divergent_leg(ctx, args)
     if "build" in legs:
         out["config"]["build"] = build_leg(ctx, args)
+    if "phases" in legs:
+        out["config"]["phases"] = phases_leg(ctx, args)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def copy_ref():
+    """An elementwise kernel that reads exactly 1 GiB and writes exactly 1 GiB
+    (three dispatches), the scale of the HBM counters in counter runs."""
+    import torch
+    src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.add(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 # ------------------------------------------------------------ intersection
@@ -199,7 +227,7 @@ def intersect_leg(ctx, args):
         launches.append(("any/" + name,
                          lambda rp=rp, op=occ.data_ptr(), coh=coh, **kw: gs.intersects_dev(rp, nr, op, coherent=coh, **kw)))
 
-    # counting pass: node visits and primitive tests per launch (not timed)
+    # counting pass: node visits, triangle and shape tests per launch (not timed)
     work = {}
     for label, fn in launches:
         ctx.reset_stats()
@@ -214,27 +242,37 @@ def intersect_leg(ctx, args):
             abi.check(ctx.lib.bgpu_intersect_any(gs.h, C.byref(r), nr, occ.data_ptr(),
                                                  abi.DEVICE_PTRS | abi.COUNT))
         st = ctx.stats()
-        work[label] = (st["nodes_visited"], st["prims_tested"])
+        work[label] = (st["nodes_visited"], st["prims_tested"] - st["shapes_tested"], st["shapes_tested"],
+                       st["shape_nodes_visited"])
     p_c = dbat["incoherent"][1][3].cpu().numpy()
     shape_frac = float((p_c >= flat.num_triangles).mean())
 
-    b_node = B_NODE_PER_CHILD * gs.bvh_info()["width"]
+    width = gs.bvh_info()["width"]
+    b_node = B_NODE_PER_CHILD * width
 
     def alg_bytes(label):
-        nodes, prims = work[label]
+        nodes, tris, shapes, _ = work[label]
         o = B_HIT_OUT if label.startswith("closest") else B_OCC_OUT
-        return nr * (B_RAY_IN + o) + nodes * b_node + prims * B_TRI
+        return nr * (B_RAY_IN + o) + nodes * b_node + tris * B_TRI + shapes * B_SHAPE
 
-    if args.copy_ref:
-        # an elementwise kernel that reads exactly 1 GiB and writes exactly 1 GiB
-        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.add(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-        del src, dst
+    def alg_ops(label):
+        nodes, tris, shapes, _ = work[label]
+        return nodes * width * OPS_SLAB + tris * OPS_TRI + shapes * OPS_SHAPE
+
     if args.only:
         launches = [(l, f) for l, f in launches if l == args.only]
+
+    # the ray-batch kernels' block size and LDS stack depth, tuned on the
+    # timed launches themselves (results do not depend on them); the timed
+    # region runs the winner
+    tuning = None
+    if not args.no_tune:
+        from bling_amd import tune
+        space = {"trace.block": (64, 128, 256), "trace.stack": (8, 12, 16)}
+        best, table = tune.tune_launches(ctx, [fn for _, fn in launches], space, repeats=3)
+        tuning = {"space": {k: list(v) for k, v in space.items()}, "best": best, "table": table,
+                  "score": "sum over the launches of the median of 3 BGPU_TIMED kernel times (ms)"}
+        log("intersect: tuned", json.dumps(best))
 
     # warmup + timed repetitions
     for _ in range(max(1, args.warmup)):
@@ -253,9 +291,13 @@ def intersect_leg(ctx, args):
     dom = max(avg, key=lambda k: avg[k])
     per_launch = {k: {"ms": round(avg[k], 4), "mrays_s": round(nr / avg[k] / 1e3, 1),
                       "nodes_per_ray": round(work[k][0] / nr, 2),
-                      "prims_per_ray": round(work[k][1] / nr, 2),
+                      "shape_nodes_per_ray": round(work[k][3] / nr, 2),
+                      "tris_per_ray": round(work[k][1] / nr, 2),
+                      "shapes_per_ray": round(work[k][2] / nr, 3),
                       "alg_bytes": int(alg_bytes(k)),
-                      "alg_GBps": round(alg_bytes(k) / (avg[k] * 1e-3) / 1e9, 1)} for k in avg}
+                      "alg_GBps": round(alg_bytes(k) / (avg[k] * 1e-3) / 1e9, 1),
+                      "alg_ops": int(alg_ops(k)),
+                      "frac_valu": round(alg_ops(k) / (avg[k] * 1e-3) / PEAK_VALU_OPS, 4)} for k in avg}
     log("intersect: per launch:", json.dumps(per_launch))
 
     # measured HBM traffic of closest/incoherent (committed counter runs)
@@ -271,9 +313,11 @@ def intersect_leg(ctx, args):
         "frac_model": round(achieved / PEAK_HBM_GBS, 4),
         "frac_hbm": (round(traffic / (avg[dom] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
                      if traffic and dom == "closest/incoherent" else None),
-        "note": "achieved counts every node and leaf fetch of the traversal (the stats build's visit counts), "
-                "cache hits included; frac_hbm divides the measured DRAM traffic (FETCH_SIZE + WRITE_SIZE) "
-                "by the same time",
+        "frac_valu": per_launch[dom]["frac_valu"],
+        "note": "achieved counts every node and leaf fetch of the traversal (the stats build's visit counts, "
+                "shape tests at 68 B), cache hits included, so a cache-resident launch can exceed 1; "
+                "frac_hbm divides the measured DRAM traffic (FETCH_SIZE + WRITE_SIZE) by the same time; "
+                "frac_valu is the model's VALU operations over the FP32 issue rate",
     }
     res = {
         "workload": "intersection microbench: displaced icosphere %d tris + %d spheres, binned-SAH BVH "
@@ -288,6 +332,8 @@ def intersect_leg(ctx, args):
         "per_launch": per_launch,
         "shape_hit_fraction_incoherent": round(shape_frac, 4),
         "roofline": roofline,
+        "knobs": {k: ctx.config_get(k) for k in ("trace.block", "trace.stack", "trace.persistent")},
+        "tuning": tuning,
     }
     if not args.no_cpu and not args.only:
         res["parity"], res["cpu_baseline"] = isect_parity_and_cpu(flat, batches, dbat, args.cpu_seconds)
@@ -723,6 +769,28 @@ def serve_leg(ctx, args, out):
                             "speedup_mrays": head["speedup_mrays"]},
         "serve": {"rates": rows, "scene_admission": admission, "profile": phases, "profile_iterations": iters},
     })
+
+
+def phases_leg(ctx, args):
+    """The headline workload's per-phase profiling pass alone, for counter
+    runs: the serving scenes, then the first --profile-jobs jobs of the
+    saturating-rate stream on the plain path with pt.profile 1 (every bounce
+    kernel a plain launch).  Reports the items of each phase, so that a
+    counter total over a phase's dispatches divides into bytes per item."""
+    from bling_amd.workloads import job_stream, serve_scenes
+    named = serve_scenes(args.serve_max_tris)
+    flats = [f for _, f in named]
+    gss = create_scenes(ctx, named)
+    res_set = [int(v) for v in args.serve_res.split(",")]
+    spp_set = [int(v) for v in args.serve_spp.split(",")]
+    rate = float(args.serve_rates.split(",")[0])
+    jobs = job_stream(STEP_JOBS * args.steps, rate, len(flats), res_set, spp_set, seed=1)[:args.profile_jobs]
+    phases, iters = profile_pass(ctx, gss, flats, jobs)
+    log("phases:", json.dumps(phases))
+    for g in gss:
+        g.close()
+    return {"workload": "per-phase profiling pass: the first %d jobs of the %g jobs/s serving stream, plain path, "
+                        "pt.profile 1" % (len(jobs), rate), "iterations": iters, "phases": phases}
 
 
 def batch_leg(ctx, args):

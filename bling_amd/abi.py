@@ -115,7 +115,8 @@ class Stats(C.Structure):
                 ("rays_shadow", C.c_uint64), ("rays_mis", C.c_uint64),
                 ("samples", C.c_uint64), ("jobs_done", C.c_uint64), ("gpu_ms", C.c_double),
                 ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64),
-                ("branches_dropped", C.c_uint64)]
+                ("branches_dropped", C.c_uint64), ("shape_nodes_visited", C.c_uint64),
+                ("shapes_tested", C.c_uint64)]
 
 
 # every symbol declared in include/bgpu.h

@@ -193,24 +193,28 @@ static int stage_out(Ctx* c, size_t need) {
   return BGPU_OK;
 }
 
-// BGPU_COUNT: zeroed device counters for the counting kernel variant.
+// BGPU_COUNT: zeroed device counters for the counting kernel variant (4
+// 64-bit: triangle-tree node fetches and tests, shape-tree node fetches and
+// tests).
 static int count_begin(Ctx* c, int flags, unsigned long long** cnt) {
   *cnt = nullptr;
   if (!(flags & BGPU_COUNT)) return BGPU_OK;
   if (!c->count_buf.ptr)
-    if (int rc = c->count_buf.alloc(c, 2 * sizeof(unsigned long long))) return rc;
-  BGPU_HIP(zero_words(c->count_buf.as<unsigned int>(), 4, c->stream));
+    if (int rc = c->count_buf.alloc(c, 4 * sizeof(unsigned long long))) return rc;
+  BGPU_HIP(zero_words(c->count_buf.as<unsigned int>(), 8, c->stream));
   *cnt = c->count_buf.as<unsigned long long>();
   return BGPU_OK;
 }
 
 static int count_end(Ctx* c, unsigned long long* cnt) {
   if (!cnt) return BGPU_OK;
-  unsigned long long h[2];
+  unsigned long long h[4];
   BGPU_HIP(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
   std::lock_guard<std::mutex> lk(c->stats_mu);
-  c->stats.nodes_visited += h[0];
-  c->stats.prims_tested += h[1];
+  c->stats.nodes_visited += h[0] + h[2];
+  c->stats.prims_tested += h[1] + h[3];
+  c->stats.shape_nodes_visited += h[2];
+  c->stats.shapes_tested += h[3];
   return BGPU_OK;
 }
 
@@ -278,8 +282,8 @@ int bgpu_intersect_closest_noalloc(bgpu_scene* scene, const bgpu_rays* rays, siz
   const DevHits dh{hits->t, hits->b1, hits->b2, hits->prim};
   unsigned int* work = c->scratch;  // words 0-1: the passes' counters (zeroed by the launch)
   unsigned long long* cnt = nullptr;
-  if (flags & BGPU_COUNT) {  // words 8-11: node visits, primitive tests
-    BGPU_HIP(zero_words(c->scratch + 8, 4, c->stream));
+  if (flags & BGPU_COUNT) {  // words 8-15: the four 64-bit work counters
+    BGPU_HIP(zero_words(c->scratch + 8, 8, c->stream));
     cnt = reinterpret_cast<unsigned long long*>(c->scratch + 8);
   }
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev0, c->stream));
@@ -288,13 +292,7 @@ int bgpu_intersect_closest_noalloc(bgpu_scene* scene, const bgpu_rays* rays, siz
   BGPU_HIP(launch_closest(scene->dev, dr, (int)n, dh, tc, c->stream, cnt, work));
   if (flags & BGPU_TIMED) BGPU_HIP(hipEventRecord(c->ev1, c->stream));
   BGPU_HIP(hipStreamSynchronize(c->stream));
-  if (cnt) {
-    unsigned long long h[2];
-    BGPU_HIP(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
-    std::lock_guard<std::mutex> lk2(c->stats_mu);
-    c->stats.nodes_visited += h[0];
-    c->stats.prims_tested += h[1];
-  }
+  if (int rc = count_end(c, cnt)) return rc;
   if (flags & BGPU_TIMED) {
     float ms = 0.0f;
     BGPU_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));

@@ -986,6 +986,14 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     ctx->trace_cfg.persistent = (int)value;
     return BGPU_OK;
   }
+  if (k == "trace.stack") {
+    if (value < kMinStackLds || value > kMaxStackLds)
+      return fail(BGPU_E_INVALID, "trace.stack must be in [" + std::to_string(kMinStackLds) + ", " +
+                                      std::to_string(kMaxStackLds) + "]");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->trace_cfg.stack = (int)value;
+    return BGPU_OK;
+  }
   if (k == "bvh.width") {
     if (value != 2 && value != 4 && value != 8) return fail(BGPU_E_INVALID, "bvh.width must be 2, 4 or 8");
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1011,6 +1019,14 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.shade_waves") {
     if (value != 0 && value != 2) return fail(BGPU_E_INVALID, "pt.shade_waves must be 0 or 2");
     next.shade_waves = (int)value;
+  } else if (k == "pt.stack") {
+    if (value < kMinStackLds || value > kMaxStackLds)
+      return fail(BGPU_E_INVALID, "pt.stack must be in [" + std::to_string(kMinStackLds) + ", " +
+                                      std::to_string(kMaxStackLds) + "]");
+    next.stack = (int)value;
+  } else if (k == "pt.diffuse_waves") {
+    if (value != 0 && value != 2 && value != 4) return fail(BGPU_E_INVALID, "pt.diffuse_waves must be 0, 2 or 4");
+    next.diffuse_waves = (int)value;
   } else if (k == "pt.sort") {
     if (value < 0 || value > 2) return fail(BGPU_E_INVALID, "pt.sort must be 0, 1 or 2");
     next.sort = (int)value;
@@ -1056,9 +1072,12 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "bvh.width") *value = ctx->bvh_width;
   else if (k == "bvh.device") *value = ctx->bvh_device;
   else if (k == "trace.persistent") *value = ctx->trace_cfg.persistent;
+  else if (k == "trace.stack") *value = ctx->trace_cfg.stack;
+  else if (k == "pt.stack") *value = ctx->pt_cfg.stack;
   else if (k == "pt.block") *value = ctx->pt_cfg.block;
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
+  else if (k == "pt.diffuse_waves") *value = ctx->pt_cfg.diffuse_waves;
   else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
   else if (k == "pt.sort") *value = ctx->pt_cfg.sort;
   else if (k == "pt.dl_depth") *value = ctx->pt_cfg.dl_depth;

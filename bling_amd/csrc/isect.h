@@ -72,7 +72,11 @@ struct DevHits {
 struct TraceConfig {
   int block = 128;
   int persistent = 1;  // lanes refill with new rays (1) or one ray per thread (0)
+  int stack = 12;      // traversal stack entries per lane kept in LDS
 };
+// The range of TraceConfig::stack (and of the bounce loop's pt.stack); deeper
+// entries spill to private memory.
+constexpr int kMinStackLds = 8, kMaxStackLds = 24;
 
 // Words of the per-GPU scratch array of the allocation-free ray batches
 // (256 bytes); scratch_address returns its device address on the current

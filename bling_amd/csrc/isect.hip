@@ -6,8 +6,8 @@
 namespace bgpu {
 
 // Scratch of bgpu_intersect_closest_noalloc: the two pass counters of the
-// persistent traversal (words 0-1) and, for BGPU_COUNT, the node-visit and
-// primitive-test counters (64-bit, words 8-11).  One copy per GPU, for the
+// persistent traversal (words 0-1) and, for BGPU_COUNT, the four 64-bit
+// work counters (words 8-15).  One copy per GPU, for the
 // life of the process; host code reaches it only through the address
 // hipGetSymbolAddress returns on that device (scratch_address).
 static __device__ __align__(256) unsigned int scratch[kScratchWords];
@@ -32,17 +32,25 @@ __device__ __forceinline__ DRay load_ray(const DevRays& r, int i) {
   return ray;
 }
 
+// Adds a lane's work counts to counters[0..3]: triangle-tree node fetches
+// and tests, then shape-tree node fetches and tests.
+__device__ __forceinline__ void add_counts(unsigned long long* counters, const trav::Counts& cn) {
+  atomicAdd(&counters[0], cn.nodes[0]);
+  atomicAdd(&counters[1], cn.tests[0]);
+  atomicAdd(&counters[2], cn.nodes[1]);
+  atomicAdd(&counters[3], cn.tests[1]);
+}
+
 template <int BLOCK, bool STATS, int W>
 __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, int n, DevHits out,
-                                                   unsigned long long* counters) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+                                                   unsigned long long* counters, int nshort) {
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
   trav::Best best;
-  unsigned long long nvisit = 0, ntest = 0;
-  trav::closest_hit_w<W, STATS>(sc, r, lds + tid, BLOCK, best, nvisit, ntest);
+  trav::Counts cn;
+  trav::closest_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort), best, cn);
   if (best.prim >= 0) {
     out.t[i] = best.t;
     out.b1[i] = best.b1;
@@ -54,63 +62,58 @@ __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, in
     out.b2[i] = 0.0f;
     out.prim[i] = -1;
   }
-  if (STATS) {
-    atomicAdd(&counters[0], nvisit);
-    atomicAdd(&counters[1], ntest);
-  }
+  if (STATS) add_counts(counters, cn);
 }
 
 template <int BLOCK, bool STATS, int W>
 __global__ __launch_bounds__(BLOCK) void k_any(DevScene sc, DevRays rays, int n, uint8_t* occ,
-                                               unsigned long long* counters) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+                                               unsigned long long* counters, int nshort) {
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
-  unsigned long long nvisit = 0, ntest = 0;
-  bool found = trav::any_hit_w<W, STATS>(sc, r, lds + tid, BLOCK, nvisit, ntest);
+  trav::Counts cn;
+  bool found = trav::any_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort), cn);
   occ[i] = found ? 1 : 0;
-  if (STATS) {
-    atomicAdd(&counters[0], nvisit);
-    atomicAdd(&counters[1], ntest);
-  }
+  if (STATS) add_counts(counters, cn);
 }
 
 template <int BLOCK, int W>
 hipError_t run_closest_w(const DevScene& sc, const DevRays& r, int n, const DevHits& h, hipStream_t s,
-                         unsigned long long* counters) {
+                         unsigned long long* counters, int nshort) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
+  const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
   if (counters)
-    hipLaunchKernelGGL((k_closest<BLOCK, true, W>), grid, dim3(BLOCK), 0, s, sc, r, n, h, counters);
+    hipLaunchKernelGGL((k_closest<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort);
   else
-    hipLaunchKernelGGL((k_closest<BLOCK, false, W>), grid, dim3(BLOCK), 0, s, sc, r, n, h, counters);
+    hipLaunchKernelGGL((k_closest<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort);
   return hipGetLastError();
 }
 template <int BLOCK>
 hipError_t run_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h, hipStream_t s,
-                       unsigned long long* counters) {
-  if (sc.width == 8) return run_closest_w<BLOCK, 8>(sc, r, n, h, s, counters);
-  if (sc.width == 4) return run_closest_w<BLOCK, 4>(sc, r, n, h, s, counters);
-  return run_closest_w<BLOCK, 2>(sc, r, n, h, s, counters);
+                       unsigned long long* counters, int nshort) {
+  if (sc.width == 8) return run_closest_w<BLOCK, 8>(sc, r, n, h, s, counters, nshort);
+  if (sc.width == 4) return run_closest_w<BLOCK, 4>(sc, r, n, h, s, counters, nshort);
+  return run_closest_w<BLOCK, 2>(sc, r, n, h, s, counters, nshort);
 }
 
 template <int BLOCK, int W>
 hipError_t run_any_w(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hipStream_t s,
-                     unsigned long long* counters) {
+                     unsigned long long* counters, int nshort) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
+  const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
   if (counters)
-    hipLaunchKernelGGL((k_any<BLOCK, true, W>), grid, dim3(BLOCK), 0, s, sc, r, n, occ, counters);
+    hipLaunchKernelGGL((k_any<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort);
   else
-    hipLaunchKernelGGL((k_any<BLOCK, false, W>), grid, dim3(BLOCK), 0, s, sc, r, n, occ, counters);
+    hipLaunchKernelGGL((k_any<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort);
   return hipGetLastError();
 }
 template <int BLOCK>
 hipError_t run_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hipStream_t s,
-                   unsigned long long* counters) {
-  if (sc.width == 8) return run_any_w<BLOCK, 8>(sc, r, n, occ, s, counters);
-  if (sc.width == 4) return run_any_w<BLOCK, 4>(sc, r, n, occ, s, counters);
-  return run_any_w<BLOCK, 2>(sc, r, n, occ, s, counters);
+                   unsigned long long* counters, int nshort) {
+  if (sc.width == 8) return run_any_w<BLOCK, 8>(sc, r, n, occ, s, counters, nshort);
+  if (sc.width == 4) return run_any_w<BLOCK, 4>(sc, r, n, occ, s, counters, nshort);
+  return run_any_w<BLOCK, 2>(sc, r, n, occ, s, counters, nshort);
 }
 
 // ---------------------------------------------------------------- persistent
@@ -158,39 +161,39 @@ struct BatchSrc {
 template <int BLOCK, int W, bool ANY, bool FAT, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_persist(DevScene sc, DevRays rays, int n, DevHits out,
                                                    uint8_t* occ, unsigned int* work,
-                                                   unsigned long long* counters) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+                                                   unsigned long long* counters, int nshort) {
   BatchSrc<ANY, FAT> src{sc, rays, out, occ};
-  trav::persist_trace<W, ANY, FAT, STATS>(src, n, work, lds + threadIdx.x, BLOCK, counters);
+  trav::persist_trace<W, ANY, FAT, STATS>(src, n, work, trav::lane_slot<BLOCK>(nshort), counters);
 }
 
 template <int BLOCK, int W, bool ANY, bool STATS>
 hipError_t run_persist(const DevScene& sc, const DevRays& r, int n, const DevHits& h, uint8_t* occ,
-                       unsigned int* work, hipStream_t s, unsigned long long* counters) {
+                       unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort) {
   // one counter per pass, zeroed together
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, work, 2);
-  static int g_tri = 0, g_shape = 0;
-  if (!g_tri) g_tri = trav::persistent_grid(k_persist<BLOCK, W, ANY, false, STATS>, BLOCK);
-  if (!g_shape) g_shape = trav::persistent_grid(k_persist<BLOCK, W, ANY, true, STATS>, BLOCK);
-  hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, false, STATS>), dim3(g_tri), dim3(BLOCK), 0, s, sc, r, n, h,
-                     occ, work, counters);
+  static int g_tri[trav::kMaxShort + 1] = {}, g_shape[trav::kMaxShort + 1] = {};
+  const int gt = trav::persistent_grid_cached(g_tri, k_persist<BLOCK, W, ANY, false, STATS>, BLOCK, nshort);
+  const int gs = trav::persistent_grid_cached(g_shape, k_persist<BLOCK, W, ANY, true, STATS>, BLOCK, nshort);
+  const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
+  hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, false, STATS>), dim3(gt), dim3(BLOCK), lds, s, sc, r, n, h,
+                     occ, work, counters, nshort);
   if (sc.shape_root >= 0)
-    hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, true, STATS>), dim3(g_shape), dim3(BLOCK), 0, s, sc, r, n, h,
-                       occ, work + 1, counters);
+    hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, true, STATS>), dim3(gs), dim3(BLOCK), lds, s, sc, r, n, h,
+                       occ, work + 1, counters, nshort);
   return hipGetLastError();
 }
 
 template <int BLOCK, bool ANY>
 hipError_t run_persist_any_w(const DevScene& sc, const DevRays& r, int n, const DevHits& h, uint8_t* occ,
-                             unsigned int* work, hipStream_t s, unsigned long long* counters) {
+                             unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort) {
   if (counters) {
-    if (sc.width == 8) return run_persist<BLOCK, 8, ANY, true>(sc, r, n, h, occ, work, s, counters);
-    if (sc.width == 4) return run_persist<BLOCK, 4, ANY, true>(sc, r, n, h, occ, work, s, counters);
-    return run_persist<BLOCK, 2, ANY, true>(sc, r, n, h, occ, work, s, counters);
+    if (sc.width == 8) return run_persist<BLOCK, 8, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
+    if (sc.width == 4) return run_persist<BLOCK, 4, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
+    return run_persist<BLOCK, 2, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
   }
-  if (sc.width == 8) return run_persist<BLOCK, 8, ANY, false>(sc, r, n, h, occ, work, s, counters);
-  if (sc.width == 4) return run_persist<BLOCK, 4, ANY, false>(sc, r, n, h, occ, work, s, counters);
-  return run_persist<BLOCK, 2, ANY, false>(sc, r, n, h, occ, work, s, counters);
+  if (sc.width == 8) return run_persist<BLOCK, 8, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
+  if (sc.width == 4) return run_persist<BLOCK, 4, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
+  return run_persist<BLOCK, 2, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
 }
 
 }  // namespace
@@ -205,17 +208,19 @@ hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const Dev
                           const TraceConfig& cfg, hipStream_t s, unsigned long long* counters,
                           unsigned int* work) {
   if (n <= 0) return hipSuccess;
+  const int ns = cfg.stack;
+  if (ns < trav::kMinShort || ns > trav::kMaxShort) return hipErrorInvalidValue;
   if (cfg.persistent && work) {
     switch (cfg.block) {
-      case 64: return run_persist_any_w<64, false>(sc, r, n, h, nullptr, work, s, counters);
-      case 256: return run_persist_any_w<256, false>(sc, r, n, h, nullptr, work, s, counters);
-      default: return run_persist_any_w<128, false>(sc, r, n, h, nullptr, work, s, counters);
+      case 64: return run_persist_any_w<64, false>(sc, r, n, h, nullptr, work, s, counters, ns);
+      case 256: return run_persist_any_w<256, false>(sc, r, n, h, nullptr, work, s, counters, ns);
+      default: return run_persist_any_w<128, false>(sc, r, n, h, nullptr, work, s, counters, ns);
     }
   }
   switch (cfg.block) {
-    case 64: return run_closest<64>(sc, r, n, h, s, counters);
-    case 256: return run_closest<256>(sc, r, n, h, s, counters);
-    default: return run_closest<128>(sc, r, n, h, s, counters);
+    case 64: return run_closest<64>(sc, r, n, h, s, counters, ns);
+    case 256: return run_closest<256>(sc, r, n, h, s, counters, ns);
+    default: return run_closest<128>(sc, r, n, h, s, counters, ns);
   }
 }
 
@@ -223,18 +228,20 @@ hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
                       const TraceConfig& cfg, hipStream_t s, unsigned long long* counters,
                       unsigned int* work) {
   if (n <= 0) return hipSuccess;
+  const int ns = cfg.stack;
+  if (ns < trav::kMinShort || ns > trav::kMaxShort) return hipErrorInvalidValue;
   if (cfg.persistent && work) {
     const DevHits none{nullptr, nullptr, nullptr, nullptr};
     switch (cfg.block) {
-      case 64: return run_persist_any_w<64, true>(sc, r, n, none, occ, work, s, counters);
-      case 256: return run_persist_any_w<256, true>(sc, r, n, none, occ, work, s, counters);
-      default: return run_persist_any_w<128, true>(sc, r, n, none, occ, work, s, counters);
+      case 64: return run_persist_any_w<64, true>(sc, r, n, none, occ, work, s, counters, ns);
+      case 256: return run_persist_any_w<256, true>(sc, r, n, none, occ, work, s, counters, ns);
+      default: return run_persist_any_w<128, true>(sc, r, n, none, occ, work, s, counters, ns);
     }
   }
   switch (cfg.block) {
-    case 64: return run_any<64>(sc, r, n, occ, s, counters);
-    case 256: return run_any<256>(sc, r, n, occ, s, counters);
-    default: return run_any<128>(sc, r, n, occ, s, counters);
+    case 64: return run_any<64>(sc, r, n, occ, s, counters, ns);
+    case 256: return run_any<256>(sc, r, n, occ, s, counters, ns);
+    default: return run_any<128>(sc, r, n, occ, s, counters, ns);
   }
 }
 

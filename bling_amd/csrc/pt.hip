@@ -311,22 +311,22 @@ __device__ __forceinline__ void for_uniform_scene(bool valid, int my, F&& body) 
 
 // Traversal for a scene of width W (0: any width, chosen per scene).
 template <int W, bool ROOT_INF = false>
-__device__ __forceinline__ void pt_closest(const DevScene& sc, const DRay& r, int* lds, int stride,
+__device__ __forceinline__ void pt_closest(const DevScene& sc, const DRay& r, const trav::LdsSlot& slot,
                                            trav::Best& best) {
-  unsigned long long nv = 0, nt = 0;
-  if constexpr (W == 0) trav::closest_hit<false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
-  else trav::closest_hit_w<W, false, ROOT_INF>(sc, r, lds, stride, best, nv, nt);
+  trav::Counts cn;  // unused (no STATS)
+  if constexpr (W == 0) trav::closest_hit<false, ROOT_INF>(sc, r, slot, best, cn);
+  else trav::closest_hit_w<W, false, ROOT_INF>(sc, r, slot, best, cn);
 }
 template <int W, bool EXACT = false>
-__device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, int* lds, int stride) {
-  unsigned long long nv = 0, nt = 0;
-  if constexpr (W == 0) return trav::any_hit<false, EXACT>(sc, r, lds, stride, nv, nt);
-  else return trav::any_hit_w<W, false, EXACT>(sc, r, lds, stride, nv, nt);
+__device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, const trav::LdsSlot& slot) {
+  trav::Counts cn;  // unused (no STATS)
+  if constexpr (W == 0) return trav::any_hit<false, EXACT>(sc, r, slot, cn);
+  else return trav::any_hit_w<W, false, EXACT>(sc, r, slot, cn);
 }
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_act;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
       r.tmin = P.ray[6 * Pn + i];
       r.tmax = INFINITY;
       trav::Best best;
-      pt_closest<W>(sc, r, lds + threadIdx.x, BLOCK, best);
+      pt_closest<W>(sc, r, slot, best);
       P.hit[i] = best.t;
       P.hit[Pn + i] = best.b1;
       P.hit[2 * Pn + i] = best.b2;
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_shadow;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
       r.d = mk3(P.sray[3 * Pn + i], P.sray[4 * Pn + i], P.sray[5 * Pn + i]);
       r.tmin = P.sray[6 * Pn + i];
       r.tmax = P.sray[7 * Pn + i];
-      if (!pt_any<W>(sc, r, lds + threadIdx.x, BLOCK)) {  // occluded (Scene.hs:45-47)
+      if (!pt_any<W>(sc, r, slot)) {  // occluded (Scene.hs:45-47)
         xyz_accum(P.rad, Pn, i, P.pend_s);
       }
     }
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_mis;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
@@ -842,9 +842,9 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
       if (L.type == LT_INF) {
         // an infinite light is reached iff nothing is hit: any hit, with the
         // closest-hit query's shape tests, decides it (tmax is inf here)
-        if (pt_any<W, true>(sc, r, lds + threadIdx.x, BLOCK)) best.prim = 0;
+        if (pt_any<W, true>(sc, r, slot)) best.prim = 0;
       } else {
-        pt_closest<W, true>(sc, r, lds + threadIdx.x, BLOCK, best);
+        pt_closest<W, true>(sc, r, slot, best);
       }
       // pend_m already holds the contribution times the sampled light's
       // emission; here only whether the ray reaches that light is decided
@@ -966,24 +966,21 @@ struct MisSrc {  // closest hit of the MIS ray into hit / hprim (resolved by k_m
 
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_extend_p(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   ExtendSrc<FAT> src{a};
-  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_act, nullptr, lds + threadIdx.x, BLOCK,
-                                            nullptr);
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_act, nullptr, slot, nullptr);
 }
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_shadow_p(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   ShadowSrc<FAT> src{a};
-  trav::persist_trace<W, true, FAT, false>(src, (int)a.cnt->n_shadow, nullptr, lds + threadIdx.x, BLOCK,
-                                           nullptr);
+  trav::persist_trace<W, true, FAT, false>(src, (int)a.cnt->n_shadow, nullptr, slot, nullptr);
 }
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_mis_p(PtKernelArgs a) {
-  __shared__ int lds[trav::kLdsPerLane * BLOCK];
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
   MisSrc<FAT> src{a};
-  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_mis, nullptr, lds + threadIdx.x, BLOCK,
-                                            nullptr);
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_mis, nullptr, slot, nullptr);
 }
 
 // The MIS ray reaches the sampled light: the hit primitive is that AreaLight
@@ -1205,17 +1202,22 @@ hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
 // block, so they run in the largest blocks
 constexpr int kWideBlock = 1024;
 
-// Waves per SIMD the diffuse shading variant is compiled for (its register
-// budget: 512 / waves VGPRs; what does not fit spills to scratch).
-#ifndef BGPU_DIFFUSE_WAVES
-#define BGPU_DIFFUSE_WAVES 4
-#endif
-constexpr int kDiffuseWaves = BGPU_DIFFUSE_WAVES;
+// The shading kernel for the diffuse-only kind set (PART: all of the queue,
+// or its diffuse classes), compiled for 1, 2 or 4 waves per SIMD (a register
+// budget of 512 / waves VGPRs; what does not fit spills to scratch).
+template <int BLOCK, int PART>
+static void launch_diffuse(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
+  if (waves == 4) hipLaunchKernelGGL((k_shade<BLOCK, 4, kDiffuseKinds, PART>), g, b, 0, s, a);
+  else if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2, kDiffuseKinds, PART>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_shade<BLOCK, 1, kDiffuseKinds, PART>), g, b, 0, s, a);
+}
 
 static inline void mark(hipEvent_t* marks, int k, hipStream_t s);
 
 template <int BLOCK>
-static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s, hipEvent_t* marks) {
+static void launch_shade(const PtKernelArgs& a, const PtConfig& cfg, dim3 g, dim3 b, hipStream_t s,
+                         hipEvent_t* marks) {
+  const int waves = cfg.shade_waves;
   if (a.sort_shade) {  // group the queue by shading class first
     constexpr int kSortBlock = 1024;  // fewer blocks, fewer atomics on the class counters
     dim3 gs((a.pool.P + kSortBlock - 1) / kSortBlock), bs(kSortBlock);
@@ -1225,9 +1227,9 @@ static void launch_shade(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipSt
   }
   mark(marks, PH_CLASSIFY + 1, s);
   if (a.shade_mode == SHADE_MODE_DIFFUSE) {
-    hipLaunchKernelGGL((k_shade<BLOCK, kDiffuseWaves, kDiffuseKinds, SHADE_ALL>), g, b, 0, s, a);
+    launch_diffuse<BLOCK, SHADE_ALL>(a, cfg.diffuse_waves, g, b, s);
   } else if (a.shade_mode == SHADE_MODE_SPLIT) {
-    hipLaunchKernelGGL((k_shade<BLOCK, kDiffuseWaves, kDiffuseKinds, SHADE_DIFFUSE_PART>), g, b, 0, s, a);
+    launch_diffuse<BLOCK, SHADE_DIFFUSE_PART>(a, cfg.diffuse_waves, g, b, s);
     if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2, kAllKinds, SHADE_OTHER_PART>), g, b, 0, s, a);
     else hipLaunchKernelGGL((k_shade<BLOCK, 1, kAllKinds, SHADE_OTHER_PART>), g, b, 0, s, a);
   } else if (waves == 2) {
@@ -1245,29 +1247,31 @@ static inline void mark(hipEvent_t* marks, int k, hipStream_t s) {
 }
 
 template <int BLOCK, int W>
-static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool persistent, hipStream_t s,
+static hipError_t enqueue_iter(const PtKernelArgs& a, const PtConfig& cfg, bool persistent, hipStream_t s,
                                hipEvent_t* marks) {
   dim3 g((a.pool.P + BLOCK - 1) / BLOCK), b(BLOCK);
   dim3 gw((a.pool.P + kWideBlock - 1) / kWideBlock), bw(kWideBlock);
+  const size_t lds = trav::stack_lds_bytes(a.nshort, BLOCK);  // the traversal kernels' stacks
   mark(marks, 0, s);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_generate<kWideBlock>, gw, bw, 0, s, a);
   mark(marks, PH_GENERATE + 1, s);
   if constexpr (W != 0) {
     if (persistent) {
-      static int ge = 0, gs = 0, gm = 0;  // grids that fill the device once
-      if (!ge) ge = trav::persistent_grid(k_extend_p<BLOCK, W, false>, BLOCK);
-      if (!gs) gs = trav::persistent_grid(k_shadow_p<BLOCK, W, false>, BLOCK);
-      if (!gm) gm = trav::persistent_grid(k_mis_p<BLOCK, W, false>, BLOCK);
-      hipLaunchKernelGGL((k_extend_p<BLOCK, W, false>), dim3(ge), b, 0, s, a);
-      hipLaunchKernelGGL((k_extend_p<BLOCK, W, true>), dim3(ge), b, 0, s, a);
+      // grids that fill the device once, per LDS stack depth
+      static int ge[trav::kMaxShort + 1] = {}, gs[trav::kMaxShort + 1] = {}, gm[trav::kMaxShort + 1] = {};
+      const int e = trav::persistent_grid_cached(ge, k_extend_p<BLOCK, W, false>, BLOCK, a.nshort);
+      const int sh = trav::persistent_grid_cached(gs, k_shadow_p<BLOCK, W, false>, BLOCK, a.nshort);
+      const int m = trav::persistent_grid_cached(gm, k_mis_p<BLOCK, W, false>, BLOCK, a.nshort);
+      hipLaunchKernelGGL((k_extend_p<BLOCK, W, false>), dim3(e), b, lds, s, a);
+      hipLaunchKernelGGL((k_extend_p<BLOCK, W, true>), dim3(e), b, lds, s, a);
       mark(marks, PH_EXTEND + 1, s);
-      launch_shade<BLOCK>(a, shade_waves, g, b, s, marks);
-      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, false>), dim3(gs), b, 0, s, a);
-      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, true>), dim3(gs), b, 0, s, a);
+      launch_shade<BLOCK>(a, cfg, g, b, s, marks);
+      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, false>), dim3(sh), b, lds, s, a);
+      hipLaunchKernelGGL((k_shadow_p<BLOCK, W, true>), dim3(sh), b, lds, s, a);
       mark(marks, PH_SHADOW + 1, s);
-      hipLaunchKernelGGL((k_mis_p<BLOCK, W, false>), dim3(gm), b, 0, s, a);
-      hipLaunchKernelGGL((k_mis_p<BLOCK, W, true>), dim3(gm), b, 0, s, a);
+      hipLaunchKernelGGL((k_mis_p<BLOCK, W, false>), dim3(m), b, lds, s, a);
+      hipLaunchKernelGGL((k_mis_p<BLOCK, W, true>), dim3(m), b, lds, s, a);
       hipLaunchKernelGGL((k_mis_resolve<BLOCK>), g, b, 0, s, a);
       mark(marks, PH_MIS + 1, s);
       hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
@@ -1275,12 +1279,12 @@ static hipError_t enqueue_iter(const PtKernelArgs& a, int shade_waves, bool pers
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, 0, s, a);
+  hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, lds, s, a);
   mark(marks, PH_EXTEND + 1, s);
-  launch_shade<BLOCK>(a, shade_waves, g, b, s, marks);
-  hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, 0, s, a);
+  launch_shade<BLOCK>(a, cfg, g, b, s, marks);
+  hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, lds, s, a);
   mark(marks, PH_SHADOW + 1, s);
-  hipLaunchKernelGGL((k_mis<BLOCK, W>), g, b, 0, s, a);
+  hipLaunchKernelGGL((k_mis<BLOCK, W>), g, b, lds, s, a);
   mark(marks, PH_MIS + 1, s);
   hipLaunchKernelGGL(k_finish<kWideBlock>, gw, bw, 0, s, a);
   mark(marks, PH_FINISH + 1, s);
@@ -1292,15 +1296,18 @@ static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, const PtConfi
                                  hipEvent_t* marks) {
   const bool p = cfg.persistent != 0;
   switch (width) {
-    case 2: return enqueue_iter<BLOCK, 2>(a, cfg.shade_waves, p, s, marks);
-    case 4: return enqueue_iter<BLOCK, 4>(a, cfg.shade_waves, p, s, marks);
-    case 8: return enqueue_iter<BLOCK, 8>(a, cfg.shade_waves, p, s, marks);
-    default: return enqueue_iter<BLOCK, 0>(a, cfg.shade_waves, p, s, marks);
+    case 2: return enqueue_iter<BLOCK, 2>(a, cfg, p, s, marks);
+    case 4: return enqueue_iter<BLOCK, 4>(a, cfg, p, s, marks);
+    case 8: return enqueue_iter<BLOCK, 8>(a, cfg, p, s, marks);
+    default: return enqueue_iter<BLOCK, 0>(a, cfg, p, s, marks);
   }
 }
 
-hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int width, hipStream_t s,
+hipError_t pt_enqueue_iteration(const PtKernelArgs& args, const PtConfig& cfg, int width, hipStream_t s,
                                 hipEvent_t* marks) {
+  if (cfg.stack < trav::kMinShort || cfg.stack > trav::kMaxShort) return hipErrorInvalidValue;
+  PtKernelArgs a = args;
+  a.nshort = cfg.stack;
   switch (cfg.block) {
     case 64: return enqueue_iter_w<64>(a, width, cfg, s, marks);
     case 256: return enqueue_iter_w<256>(a, width, cfg, s, marks);

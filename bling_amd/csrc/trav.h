@@ -3,10 +3,11 @@
 // the children's slabs as six arrays of width floats, then the links).  Hit
 // children are sorted near to far; the nearest is entered and the others are
 // pushed with their entry distances, so that a closest-hit query skips popped
-// subtrees that start beyond its current hit.  The stack keeps its top kShort
-// entries in LDS (laid out [depth][lane] so a wave's pushes hit 64 distinct
-// banks) and spills deeper entries to a private array; scene creation bounds
-// the worst-case depth by kStackCapacity.  Node boxes are tested with explicit
+// subtrees that start beyond its current hit.  The stack keeps its top
+// `nshort` entries (kMinShort..kMaxShort, chosen at launch) in dynamic LDS,
+// laid out [depth][lane] so a wave's pushes hit 64 distinct banks, and spills
+// deeper entries to a private array; scene creation bounds the worst-case
+// depth by kStackCapacity.  Node boxes are tested with explicit
 // FMAs (they are padded outward, so the test, and the culling by entry
 // distance, only have to be conservative); primitives are tested with the
 // reference's exact arithmetic.
@@ -16,47 +17,73 @@
 namespace bgpu {
 namespace trav {
 
-// Stack entries kept in LDS, and whether entries carry their entry distance
-// (closest-hit queries then skip subtrees beyond the current hit).
-#ifndef BGPU_TRAV_SHORT
-#define BGPU_TRAV_SHORT 12
-#endif
+// Stack entries kept in LDS per lane (the range of the launch-time choice),
+// and whether entries carry their entry distance (closest-hit queries then
+// skip subtrees beyond the current hit).
 #ifndef BGPU_TRAV_CULL
 #define BGPU_TRAV_CULL 0
 #endif
-constexpr int kShort = BGPU_TRAV_SHORT;
+constexpr int kMinShort = kMinStackLds, kMaxShort = kMaxStackLds;
 constexpr bool kCull = BGPU_TRAV_CULL != 0;
-constexpr int kSpill = kStackCapacity - kShort;
-// LDS ints per lane a kernel provides: node links, then entry distances
-constexpr int kLdsPerLane = kCull ? 2 * kShort : kShort;
+// private entries beyond the shallowest LDS part
+constexpr int kSpill = kStackCapacity - kMinShort;
+// LDS ints per lane for nshort entries: node links, then entry distances
+__host__ __device__ constexpr int lds_per_lane(int nshort) { return kCull ? 2 * nshort : nshort; }
+// Dynamic LDS bytes of a traversal kernel's block.
+inline size_t stack_lds_bytes(int nshort, int block) {
+  return (size_t)lds_per_lane(nshort) * (size_t)block * sizeof(int);
+}
+
+// Where a lane's LDS stack entries are: entry d at lds[d * stride], d < nshort.
+struct LdsSlot {
+  int* lds;
+  int stride;
+  int nshort;
+};
+// This lane's slot in the block's dynamic LDS (stack_lds_bytes(nshort, BLOCK)
+// bytes, given at launch).
+template <int BLOCK>
+__device__ __forceinline__ LdsSlot lane_slot(int nshort) {
+  extern __shared__ int bgpu_stack_lds[];
+  return LdsSlot{bgpu_stack_lds + threadIdx.x, BLOCK, nshort};
+}
 
 // The counters live in registers; only the spill arrays, which are indexed
 // by the stack depth, are private memory (they belong to the caller, so that
 // the compiler keeps sp and stride out of scratch).
 struct Stack {
-  int* lds;    // kLdsPerLane entries per lane, `stride` apart
+  int* lds;    // lds_per_lane(nshort) entries per lane, `stride` apart
   int stride;
+  int nshort;  // entries in LDS (uniform over the launch)
   int sp;
   int* spill_n;    // kSpill entries
   float* spill_t;  // kSpill entries when kCull
+  __device__ __forceinline__ void init(const LdsSlot& s, int* sn, float* stt) {
+    lds = s.lds;
+    stride = s.stride;
+    nshort = s.nshort;
+    sp = 0;
+    spill_n = sn;
+    spill_t = stt;
+  }
   __device__ __forceinline__ void push(int v, float t) {
-    if (sp < kShort) {
+    if (sp < nshort) {
       lds[sp * stride] = v;
-      if (kCull) lds[(kShort + sp) * stride] = __float_as_int(t);
+      if (kCull) lds[(nshort + sp) * stride] = __float_as_int(t);
     } else {
-      spill_n[sp - kShort] = v;
-      if (kCull) spill_t[sp - kShort] = t;
+      spill_n[sp - nshort] = v;
+      if (kCull) spill_t[sp - nshort] = t;
     }
     ++sp;
   }
   __device__ __forceinline__ void pop(int* v, float* t) {
     --sp;
-    if (sp < kShort) {
+    if (sp < nshort) {
       *v = lds[sp * stride];
-      *t = kCull ? __int_as_float(lds[(kShort + sp) * stride]) : 0.0f;
+      *t = kCull ? __int_as_float(lds[(nshort + sp) * stride]) : 0.0f;
     } else {
-      *v = spill_n[sp - kShort];
-      *t = kCull ? spill_t[sp - kShort] : 0.0f;
+      *v = spill_n[sp - nshort];
+      *t = kCull ? spill_t[sp - nshort] : 0.0f;
     }
   }
 };
@@ -130,6 +157,12 @@ struct Best {
   int prim;
 };
 
+// Work counts of a STATS traversal: node fetches and primitive tests, per
+// tree (index 0: triangles, 1: shapes).
+struct Counts {
+  unsigned long long nodes[2] = {0, 0}, tests[2] = {0, 0};
+};
+
 // Compare-exchange on (key, link) pairs, ascending by key.
 __device__ __forceinline__ void cex(float& ka, int& la, float& kb, int& lb) {
   const bool s = kb < ka;
@@ -164,8 +197,7 @@ __device__ __forceinline__ void sort_children(float (&k)[W], int (&l)[W]) {
 // `intersects`, so that "any hit" is exactly "a closest hit exists".
 template <int W, bool ANY, bool FAT, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRay& r, f3 inv,
-                                         f3 oi, Stack& st, Best& best, unsigned long long& nvisit,
-                                         unsigned long long& ntest) {
+                                         f3 oi, Stack& st, Best& best, Counts& cn) {
   int cur = root;
   st.sp = 0;
   for (;;) {
@@ -177,7 +209,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
         const float4 x = nd[q];
         v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
       }
-      if (STATS) ++nvisit;
+      if (STATS) ++cn.nodes[FAT ? 1 : 0];
       const float tlim = ANY ? r.tmax : best.t;
       float pad[W];
       if (FAT) node_margins<W>(v, r.o, sc.shape_pad_k, sc.shape_floor_k, pad);
@@ -210,7 +242,7 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
       int first = enc >> 4, cnt = enc & 15;
       for (int k = first; k < first + cnt; ++k) {
         const float4* pr = sc.leaf + 3 * k;
-        if (STATS) ++ntest;
+        if (STATS) ++cn.tests[FAT ? 1 : 0];
         float4 r0 = pr[0], r1 = pr[1];
         int id = __float_as_int(r0.w);
         float t, b1 = 0.0f, b2 = 0.0f;
@@ -263,19 +295,17 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
 
 template <int W, bool STATS>
 __device__ __forceinline__ void closest_w(const DevScene& sc, const DRay& r, f3 inv, f3 oi, Stack& st,
-                                          Best& best, unsigned long long& nvisit, unsigned long long& ntest) {
-  if (sc.tri_root >= 0) traverse<W, false, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest);
-  if (sc.shape_root >= 0) traverse<W, false, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest);
+                                          Best& best, Counts& cn) {
+  if (sc.tri_root >= 0) traverse<W, false, false, STATS>(sc, sc.tri_root, r, inv, oi, st, best, cn);
+  if (sc.shape_root >= 0) traverse<W, false, true, STATS>(sc, sc.shape_root, r, inv, oi, st, best, cn);
 }
 template <int W, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool any_w(const DevScene& sc, const DRay& r, f3 inv, f3 oi, Stack& st,
-                                      unsigned long long& nvisit, unsigned long long& ntest) {
+                                      Counts& cn) {
   Best best{r.tmax, 0.0f, 0.0f, -1};
-  if (sc.tri_root >= 0 &&
-      traverse<W, true, false, STATS, EXACT>(sc, sc.tri_root, r, inv, oi, st, best, nvisit, ntest))
+  if (sc.tri_root >= 0 && traverse<W, true, false, STATS, EXACT>(sc, sc.tri_root, r, inv, oi, st, best, cn))
     return true;
-  if (sc.shape_root >= 0 &&
-      traverse<W, true, true, STATS, EXACT>(sc, sc.shape_root, r, inv, oi, st, best, nvisit, ntest))
+  if (sc.shape_root >= 0 && traverse<W, true, true, STATS, EXACT>(sc, sc.shape_root, r, inv, oi, st, best, cn))
     return true;
   return false;
 }
@@ -284,9 +314,8 @@ __device__ __forceinline__ bool any_w(const DevScene& sc, const DRay& r, f3 inv,
 // test of the accelerator primitive (Primitive/KdTree.hs:240), for a scene of
 // width W.
 template <int W, bool STATS, bool ROOT_INF = false>
-__device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                              Best& best, unsigned long long& nvisit,
-                                              unsigned long long& ntest) {
+__device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r, const LdsSlot& slot,
+                                              Best& best, Counts& cn) {
   best.t = r.tmax;
   best.b1 = 0.0f;
   best.b2 = 0.0f;
@@ -303,48 +332,41 @@ __device__ __forceinline__ void closest_hit_w(const DevScene& sc, const DRay& r,
   int spill_n[kSpill];
   float spill_t[kCull ? kSpill : 1];
   Stack st;
-  st.lds = lds;
-  st.stride = stride;
-  st.spill_n = spill_n;
-  st.spill_t = spill_t;
-  closest_w<W, STATS>(sc, r, inv, oi, st, best, nvisit, ntest);
+  st.init(slot, spill_n, spill_t);
+  closest_w<W, STATS>(sc, r, inv, oi, st, best, cn);
 }
 
 // Any hit (Primitive/KdTree.hs:244-246) for a scene of width W; EXACT:
 // whether `intersect` (KdTree.hs:240-242) would find a hit.
 template <int W, bool STATS, bool EXACT = false>
-__device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                          unsigned long long& nvisit, unsigned long long& ntest) {
+__device__ __forceinline__ bool any_hit_w(const DevScene& sc, const DRay& r, const LdsSlot& slot,
+                                          Counts& cn) {
   if (!slab_haskell(sc.root_lo, sc.root_hi, r)) return false;
   f3 inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
   f3 oi = mk3(r.o.x * inv.x, r.o.y * inv.y, r.o.z * inv.z);
   int spill_n[kSpill];
   float spill_t[kCull ? kSpill : 1];
   Stack st;
-  st.lds = lds;
-  st.stride = stride;
-  st.spill_n = spill_n;
-  st.spill_t = spill_t;
-  return any_w<W, STATS, EXACT>(sc, r, inv, oi, st, nvisit, ntest);
+  st.init(slot, spill_n, spill_t);
+  return any_w<W, STATS, EXACT>(sc, r, inv, oi, st, cn);
 }
 
 // Width-dispatching forms for kernels whose rays may come from scenes of
 // different widths; the width is uniform per scene, so the branch is uniform
 // per wave where the scene is.
 template <bool STATS, bool ROOT_INF = false>
-__device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                            Best& best, unsigned long long& nvisit,
-                                            unsigned long long& ntest) {
-  if (sc.width == 8) closest_hit_w<8, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
-  else if (sc.width == 4) closest_hit_w<4, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
-  else closest_hit_w<2, STATS, ROOT_INF>(sc, r, lds, stride, best, nvisit, ntest);
+__device__ __forceinline__ void closest_hit(const DevScene& sc, const DRay& r, const LdsSlot& slot,
+                                            Best& best, Counts& cn) {
+  if (sc.width == 8) closest_hit_w<8, STATS, ROOT_INF>(sc, r, slot, best, cn);
+  else if (sc.width == 4) closest_hit_w<4, STATS, ROOT_INF>(sc, r, slot, best, cn);
+  else closest_hit_w<2, STATS, ROOT_INF>(sc, r, slot, best, cn);
 }
 template <bool STATS, bool EXACT = false>
-__device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, int* lds, int stride,
-                                        unsigned long long& nvisit, unsigned long long& ntest) {
-  if (sc.width == 8) return any_hit_w<8, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
-  if (sc.width == 4) return any_hit_w<4, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
-  return any_hit_w<2, STATS, EXACT>(sc, r, lds, stride, nvisit, ntest);
+__device__ __forceinline__ bool any_hit(const DevScene& sc, const DRay& r, const LdsSlot& slot,
+                                        Counts& cn) {
+  if (sc.width == 8) return any_hit_w<8, STATS, EXACT>(sc, r, slot, cn);
+  if (sc.width == 4) return any_hit_w<4, STATS, EXACT>(sc, r, slot, cn);
+  return any_hit_w<2, STATS, EXACT>(sc, r, slot, cn);
 }
 
 }  // namespace trav

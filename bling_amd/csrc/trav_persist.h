@@ -48,17 +48,13 @@ struct LaneScene {
 };
 
 template <int W, bool ANY, bool FAT, bool STATS, class Src>
-__device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* work, int* lds_lane,
-                                              int stride, unsigned long long* counters) {
+__device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* work, const LdsSlot& slot,
+                                              unsigned long long* counters) {
   const int lane = __lane_id();
   int spill_n[kSpill];
   float spill_t[kCull ? kSpill : 1];
   Stack st;
-  st.lds = lds_lane;
-  st.stride = stride;
-  st.sp = 0;
-  st.spill_n = spill_n;
-  st.spill_t = spill_t;
+  st.init(slot, spill_n, spill_t);
   int item = -1, cur = kNoChild;
   int pend = kNoChild;  // postponed leaf, kNoChild when none
   DRay r;
@@ -247,23 +243,32 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
       a += __shfl_xor(a, off);
       b += __shfl_xor(b, off);
     }
-    if (lane == 0) {
-      atomicAdd(&counters[0], a);
-      atomicAdd(&counters[1], b);
+    if (lane == 0) {  // counters: triangle nodes, tests, then shape nodes, tests
+      atomicAdd(&counters[FAT ? 2 : 0], a);
+      atomicAdd(&counters[FAT ? 3 : 1], b);
     }
   }
 }
 
-// Blocks that fill the device once for a persistent kernel.
+// Blocks that fill the device once for a persistent kernel launched with
+// `lds_bytes` of dynamic LDS.
 template <class K>
-int persistent_grid(K kernel, int block) {
+int persistent_grid(K kernel, int block, size_t lds_bytes) {
   int dev = 0, per_cu = 1;
   (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1024;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes) != hipSuccess ||
+      per_cu < 1)
     per_cu = 1;
   return per_cu * prop.multiProcessorCount;
+}
+
+// persistent_grid per LDS stack depth, computed on first use of each depth.
+template <class K>
+int persistent_grid_cached(int (&cache)[kMaxShort + 1], K kernel, int block, int nshort) {
+  if (!cache[nshort]) cache[nshort] = persistent_grid(kernel, block, stack_lds_bytes(nshort, block));
+  return cache[nshort];
 }
 
 }  // namespace trav

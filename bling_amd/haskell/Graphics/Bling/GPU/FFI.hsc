@@ -344,7 +344,8 @@ instance Storable JobDesc where
 -- | bgpu_stats: device-counted ray queries and samples of a context.
 data Stats = Stats
   { stRaysCamera, stRaysExtend, stRaysShadow, stRaysMis, stSamples, stJobsDone :: !Word64
-  , stGpuMs :: !Double, stNodesVisited, stPrimsTested, stBranchesDropped :: !Word64 }
+  , stGpuMs :: !Double, stNodesVisited, stPrimsTested, stBranchesDropped :: !Word64
+  , stShapeNodesVisited, stShapesTested :: !Word64 }
 
 instance Storable Stats where
   sizeOf _ = #{size bgpu_stats}
@@ -354,6 +355,7 @@ instance Storable Stats where
     <*> #{peek bgpu_stats, samples} p <*> #{peek bgpu_stats, jobs_done} p
     <*> #{peek bgpu_stats, gpu_ms} p <*> #{peek bgpu_stats, nodes_visited} p
     <*> #{peek bgpu_stats, prims_tested} p <*> #{peek bgpu_stats, branches_dropped} p
+    <*> #{peek bgpu_stats, shape_nodes_visited} p <*> #{peek bgpu_stats, shapes_tested} p
   poke p s = do
     #{poke bgpu_stats, rays_camera} p (stRaysCamera s)
     #{poke bgpu_stats, rays_extend} p (stRaysExtend s)
@@ -365,6 +367,8 @@ instance Storable Stats where
     #{poke bgpu_stats, nodes_visited} p (stNodesVisited s)
     #{poke bgpu_stats, prims_tested} p (stPrimsTested s)
     #{poke bgpu_stats, branches_dropped} p (stBranchesDropped s)
+    #{poke bgpu_stats, shape_nodes_visited} p (stShapeNodesVisited s)
+    #{poke bgpu_stats, shapes_tested} p (stShapesTested s)
 
 -- | bgpu_bvh_info: facts about a scene's acceleration structure.
 data BvhInfo = BvhInfo

@@ -1,15 +1,18 @@
 """Autotuner for the library's knobs (bgpu_config_set).
 
-tune_trace()  picks the BVH width, the traversal mode (lane refill or one ray
-              per thread) and the block size of the ray-batch kernels for a
-              scene and a sample of its rays, timing each candidate with the
-              library's own kernel timer (BGPU_TIMED);
-tune_render() picks the block size of the bounce kernels, the shading
-              kernel's occupancy variant, the path-pool size and (when asked)
-              the shading-queue grouping for a render job, timing whole
-              renders.
+tune_trace()    picks the BVH width, the traversal mode (lane refill or one
+                ray per thread), the block size and the LDS stack depth of the
+                ray-batch kernels for a scene and a sample of its rays, timing
+                each candidate with the library's own kernel timer
+                (BGPU_TIMED);
+tune_launches() the same over knobs that need no new scene, for launches the
+                caller already holds (e.g. resident device batches);
+tune_render()   picks the block size of the bounce kernels, the shading
+                kernels' occupancy variants, the LDS stack depth, the
+                path-pool size and (when asked) the shading-queue grouping
+                for a render job, timing whole renders.
 
-Both leave the winning configuration set on the context and return it with
+All three leave the winning configuration set on the context and return it with
 the table of timings.  Results never depend on the knobs (the tests check
 bit-exact intersections and equal images for every setting), so the choice
 is purely about speed.
@@ -26,13 +29,42 @@ def _best(table):
     return min(table, key=lambda row: row["ms"])
 
 
-def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(1, 0), repeats=3,
-               coherent=False):
-    """Times closest + any hit over `rays` ((n, 8) float32, host) for every
-    (bvh.width, trace.persistent, trace.block); returns (best, table).  A
-    scene is built per width (the width is fixed at scene creation)."""
+def tune_launches(ctx, launches, space, repeats=3):
+    """Times `launches` (callables taking timed=True and returning the kernel
+    time in ms) for every combination of the knob values in `space`
+    ({key: values}); a candidate's score is the sum over launches of the
+    median of `repeats` timings.  Leaves the best combination set and returns
+    (best, table)."""
+    keys = list(space)
+    keep = {k: ctx.config_get(k) for k in keys}
     table = []
-    keep = {k: ctx.config_get(k) for k in ("bvh.width", "trace.persistent", "trace.block")}
+    try:
+        for combo in itertools.product(*(space[k] for k in keys)):
+            for k, v in zip(keys, combo):
+                ctx.config_set(k, v)
+            for fn in launches:  # warm-up (and the grid-size query of a new depth)
+                fn(timed=True)
+            ms = sum(float(np.median([fn(timed=True) for _ in range(repeats)])) for fn in launches)
+            row = dict(zip(keys, combo))
+            row["ms"] = round(ms, 4)
+            table.append(row)
+    finally:
+        for k, v in keep.items():
+            ctx.config_set(k, v)
+    best = _best(table)
+    for k in keys:
+        ctx.config_set(k, best[k])
+    return best, table
+
+
+def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(1, 0), stacks=(8, 12, 16),
+               repeats=3, coherent=False):
+    """Times closest + any hit over `rays` ((n, 8) float32, host) for every
+    (bvh.width, trace.persistent, trace.block, trace.stack); returns (best,
+    table).  A scene is built per width (the width is fixed at scene
+    creation)."""
+    table = []
+    keep = {k: ctx.config_get(k) for k in ("bvh.width", "trace.persistent", "trace.block", "trace.stack")}
     flags = abi.TIMED | (abi.COHERENT if coherent else 0)
     try:
         for w in widths:
@@ -41,16 +73,17 @@ def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(
             try:
                 built = gs.bvh_info()["width"]
                 dev = _HostRays(ctx, rays)
-                for mode, block in itertools.product(modes, blocks):
+                for mode, block, stack in itertools.product(modes, blocks, stacks):
                     ctx.config_set("trace.persistent", mode)
                     ctx.config_set("trace.block", block)
+                    ctx.config_set("trace.stack", stack)
                     dev.closest(gs, flags)  # warm-up (and the grid-size query)
                     dev.any(gs, flags)
                     ms = []
                     for _ in range(repeats):
                         ms.append(dev.closest(gs, flags) + dev.any(gs, flags))
                     table.append({"bvh.width": built, "trace.persistent": mode, "trace.block": block,
-                                  "ms": float(np.median(ms))})
+                                  "trace.stack": stack, "ms": float(np.median(ms))})
                 dev.close()
             finally:
                 gs.close()
@@ -58,33 +91,33 @@ def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(
         for k, v in keep.items():
             ctx.config_set(k, v)
     best = _best(table)
-    for k in ("bvh.width", "trace.persistent", "trace.block"):
+    for k in ("bvh.width", "trace.persistent", "trace.block", "trace.stack"):
         ctx.config_set(k, best[k])
     return best, table
 
 
-def tune_render(ctx, gscene, spec, blocks=(64, 128, 256), shade_waves=(2, 0),
-                pools=(1 << 19, 1 << 20, 1 << 21), sorts=(2,), repeats=2):
+def tune_render(ctx, gscene, spec, blocks=(64, 128, 256), shade_waves=(2, 0), diffuse_waves=(2,),
+                stacks=(12,), pools=(1 << 19, 1 << 20, 1 << 21), sorts=(2,), repeats=2):
     """Times the plain render of `spec` on `gscene` for every (pt.block,
-    pt.shade_waves, pt.pool, pt.sort); returns (best, table)."""
+    pt.shade_waves, pt.diffuse_waves, pt.stack, pt.pool, pt.sort); returns
+    (best, table)."""
     from .runtime import GpuRenderer
     table = []
-    keys = ("pt.block", "pt.shade_waves", "pt.pool", "pt.sort")
+    keys = ("pt.block", "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.pool", "pt.sort")
     keep = {k: ctx.config_get(k) for k in keys}
     try:
-        for block, waves, pool, sort in itertools.product(blocks, shade_waves, pools, sorts):
-            ctx.config_set("pt.block", block)
-            ctx.config_set("pt.shade_waves", waves)
-            ctx.config_set("pt.pool", pool)
-            ctx.config_set("pt.sort", sort)
+        for combo in itertools.product(blocks, shade_waves, diffuse_waves, stacks, pools, sorts):
+            for k, v in zip(keys, combo):
+                ctx.config_set(k, v)
             GpuRenderer(spec).render_plain(gscene)  # engine build + warm-up
             ms = []
             for _ in range(repeats):
                 t0 = time.perf_counter()
                 GpuRenderer(spec).render_plain(gscene)
                 ms.append((time.perf_counter() - t0) * 1e3)
-            table.append({"pt.block": block, "pt.shade_waves": waves, "pt.pool": pool, "pt.sort": sort,
-                          "ms": float(np.median(ms))})
+            row = dict(zip(keys, combo))
+            row["ms"] = float(np.median(ms))
+            table.append(row)
     finally:
         for k, v in keep.items():
             ctx.config_set(k, v)

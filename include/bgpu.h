@@ -263,6 +263,8 @@ typedef struct {
   uint64_t prims_tested;     /* BGPU_COUNT queries: primitive tests */
   uint64_t branches_dropped; /* direct-lighting specular branches lost to a full
                                 branch stack; 0 unless an internal bound is broken */
+  uint64_t shape_nodes_visited; /* BGPU_COUNT: the shape tree's part of nodes_visited */
+  uint64_t shapes_tested;       /* BGPU_COUNT: the shape tests among prims_tested */
 } bgpu_stats;
 
 /* ---- entry points ------------------------------------------------------ */
@@ -356,7 +358,10 @@ int bgpu_profile_get(bgpu_ctx* ctx, bgpu_profile* out);
  * "bvh.device" (1, the default: scenes created afterwards with more than
  * 1024 triangles build their triangle tree on the GPU, the same tree as the
  * host builder's; 0: on the host), "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
- * ray when theirs is done; 0: one ray per thread), "pt.block"
+ * ray when theirs is done; 0: one ray per thread), "trace.stack" and
+ * "pt.stack" (8..24, default 12: traversal stack entries per lane kept in
+ * LDS by the intersect kernels and by the bounce loop; deeper entries go to
+ * private memory), "pt.block"
  * (64/128/256, bounce kernels), "pt.persistent" (1: the bounce loop's
  * traversal runs as persistent passes that refill lanes with new rays;
  * 0, the default: one ray per thread),
@@ -366,7 +371,9 @@ int bgpu_profile_get(bgpu_ctx* ctx, bgpu_profile* out);
  * "pt.profile" (1: time the bounce loop per phase, bgpu_profile_get; 0, the
  * default: replay the captured graph),
  * "pt.shade_waves" (0: no occupancy hint for the shading kernel, 2: at least
- * two waves per SIMD), "pt.sort" (1: the shading queue is grouped by
+ * two waves per SIMD), "pt.diffuse_waves" (the same for the shading kernel
+ * compiled for diffuse-only materials: 0, 2, the default, or 4 waves per
+ * SIMD), "pt.sort" (1: the shading queue is grouped by
  * material kind each bounce, so that a wave runs one BSDF code path; 0: path
  * order; 2, the default: grouped while a scene whose materials mix BSDF
  * kinds has jobs in flight).  Changing a pt.* knob is refused with BGPU_E_STATE

@@ -212,6 +212,44 @@ def test_bvh_widths_bit_exact(gpu_ctx, width, persistent):
         gpu_ctx.config_set("trace.persistent", prev[1])
 
 
+def test_stack_depth_same_results(gpu_ctx):
+    """trace.stack only moves the boundary between the LDS and the private
+    part of the traversal stack: every depth gives the default's results bit
+    for bit, on a binary tree deep enough that depth 8 spills, through both
+    kernels (lane refill and one ray per thread)."""
+    import ctypes as C
+    keys = ("bvh.width", "trace.persistent", "trace.stack")
+    keep = {k: gpu_ctx.config_get(k) for k in keys}
+    try:
+        gpu_ctx.config_set("bvh.width", 2)
+        flat = scenes.icosphere_scene(levels=5, num_spheres=2000, seed=5, res=64)
+        gs = gpu_ctx.scene(flat)
+        try:
+            assert gs.bvh_info()["stack_bound"] > 8
+            rays = scenes.incoherent_rays(*scenes.scene_bounds(flat), 30000, seed=12)
+            for persistent in (1, 0):
+                gpu_ctx.config_set("trace.persistent", persistent)
+                gpu_ctx.config_set("trace.stack", 12)
+                base, occ = gs.intersect(rays), gs.intersects(rays)
+                assert (base[3] >= 0).sum() > 3000
+                for depth in (8, 16, 24):
+                    gpu_ctx.config_set("trace.stack", depth)
+                    assert gpu_ctx.config_get("trace.stack") == depth
+                    got = gs.intersect(rays)
+                    for a, b in zip(base, got):
+                        assert np.array_equal(a.view(np.int32), b.view(np.int32)), (persistent, depth)
+                    assert np.array_equal(occ, gs.intersects(rays)), (persistent, depth)
+        finally:
+            gs.close()
+        for bad in (7, 25):
+            with pytest.raises(abi.BgpuError) as e:
+                gpu_ctx.config_set("trace.stack", bad)
+            assert e.value.code == abi.E_INVALID
+    finally:
+        for k, v in keep.items():
+            gpu_ctx.config_set(k, v)
+
+
 def test_coherent_hint_same_results(gpu_ctx):
     """BGPU_COHERENT only chooses how lanes take rays: the results equal the
     default launch bit for bit."""
@@ -242,15 +280,16 @@ def test_autotuner_picks_a_valid_config(gpu_ctx):
     """The tuner times every candidate, leaves the fastest set on the
     context, and the tuned context still answers bit-exactly."""
     from bling_amd import tune
-    keep = {k: gpu_ctx.config_get(k) for k in ("bvh.width", "trace.persistent", "trace.block")}
+    keys = ("bvh.width", "trace.persistent", "trace.block", "trace.stack")
+    keep = {k: gpu_ctx.config_get(k) for k in keys}
     try:
         flat = scenes.small_scene(num_tris=3000, num_spheres=300, seed=17)
         rays = scenes.incoherent_rays(*scenes.scene_bounds(flat), 20000, seed=7)
         best, table = tune.tune_trace(gpu_ctx, flat, rays, widths=(2, 4), blocks=(64, 128), modes=(1, 0),
-                                      repeats=1)
-        assert len(table) == 8 and all(row["ms"] > 0 for row in table)
+                                      stacks=(8, 16), repeats=1)
+        assert len(table) == 16 and all(row["ms"] > 0 for row in table)
         assert best == min(table, key=lambda r: r["ms"])
-        for k in ("bvh.width", "trace.persistent", "trace.block"):
+        for k in keys:
             assert gpu_ctx.config_get(k) == best[k]
         gs = gpu_ctx.scene(flat)
         g = gs.intersect(rays)
