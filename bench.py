@@ -131,8 +131,9 @@ def main():
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         ctx.config_set(k, int(v))
-    for k in ("bvh.width", "bvh.device", "trace.block", "trace.persistent", "trace.stack", "pt.block", "pt.pool",
-              "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.persistent", "pt.sort"):
+    for k in ("bvh.width", "bvh.device", "trace.block", "trace.persistent", "trace.stack", "trace.refill",
+              "trace.walk", "pt.block", "pt.pool",
+              "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.walk", "pt.persistent", "pt.sort"):
         knobs[k] = ctx.config_get(k)
     out = {
         "metric": METRIC,
@@ -268,7 +269,8 @@ def intersect_leg(ctx, args):
     tuning = None
     if not args.no_tune:
         from bling_amd import tune
-        space = {"trace.block": (64, 128, 256), "trace.stack": (8, 12, 16)}
+        space = {"trace.block": (128, 256), "trace.stack": (8, 12, 16), "trace.refill": (16, 32, 48),
+                 "trace.walk": (0, 8, 16, 24, 32)}
         best, table = tune.tune_launches(ctx, [fn for _, fn in launches], space, repeats=3)
         tuning = {"space": {k: list(v) for k, v in space.items()}, "best": best, "table": table,
                   "score": "sum over the launches of the median of 3 BGPU_TIMED kernel times (ms)"}
@@ -332,7 +334,8 @@ def intersect_leg(ctx, args):
         "per_launch": per_launch,
         "shape_hit_fraction_incoherent": round(shape_frac, 4),
         "roofline": roofline,
-        "knobs": {k: ctx.config_get(k) for k in ("trace.block", "trace.stack", "trace.persistent")},
+        "knobs": {k: ctx.config_get(k) for k in ("trace.block", "trace.stack", "trace.persistent", "trace.refill",
+                                                 "trace.walk")},
         "tuning": tuning,
     }
     if not args.no_cpu and not args.only:

@@ -37,13 +37,33 @@ class Shape(C.Structure):
                 ("w2o", F16), ("emission", F16)]
 
 
-TEX_NONE, TEX_CHECKER, TEX_GRAPHPAPER = 0, 1, 2
-MAP_UV, MAP_PLANAR = 0, 1
+TEX_NONE, TEX_CHECKER, TEX_GRAPHPAPER, TEX_NODES = 0, 1, 2, 3
+MAP_UV, MAP_PLANAR, MAP_IDENTITY = 0, 1, 2
+# texture node operations (bgpu_tex_node.op)
+TN_SCONST, TN_SIMAGE, TN_CELL, TN_FBM, TN_PERLIN, TN_CRYSTAL, TN_SCALE = 1, 2, 3, 4, 5, 6, 7
+TN_CONST, TN_IMAGE, TN_BLEND, TN_CHECKER, TN_GRADIENT, TN_GRAPHPAPER, TN_GSTEP = 16, 17, 18, 19, 20, 21, 22
 
 
 class Texture(C.Structure):
+    # `params` shares its storage with the int32 `roots` of BGPU_TEX_NODES
+    # (a C union); texture_roots() views it as integers
     _fields_ = [("kind", C.c_int32), ("mapping", C.c_int32), ("slots", C.c_int32), ("pad", C.c_int32),
                 ("params", C.c_float * 12), ("alt", (C.c_float * 16) * 2)]
+
+
+def texture_roots(tex):
+    """The roots[12] view of a Texture's params (BGPU_TEX_NODES)."""
+    return (C.c_int32 * 12).from_buffer(tex, Texture.params.offset)
+
+
+class TexNode(C.Structure):
+    _fields_ = [("op", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("c", C.c_int32), ("map", C.c_int32),
+                ("pad", C.c_int32 * 3), ("q", C.c_float * 4), ("m", C.c_float * 16), ("spec", C.c_float * 16)]
+
+
+class Image(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32), ("pad", C.c_int32),
+                ("texels", FP)]
 
 
 class Material(C.Structure):
@@ -79,7 +99,9 @@ class SceneDesc(C.Structure):
                 ("num_shapes", C.c_uint32), ("shapes", C.POINTER(Shape)),
                 ("num_materials", C.c_uint32), ("materials", C.POINTER(Material)),
                 ("num_lights", C.c_uint32), ("lights", C.POINTER(Light)),
-                ("camera", Camera), ("film", Film)]
+                ("camera", Camera), ("film", Film),
+                ("num_tex_nodes", C.c_uint32), ("tex_nodes", C.POINTER(TexNode)),
+                ("num_images", C.c_uint32), ("images", C.POINTER(Image))]
 
 
 class Rays(C.Structure):

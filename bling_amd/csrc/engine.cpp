@@ -994,6 +994,14 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
     ctx->trace_cfg.stack = (int)value;
     return BGPU_OK;
   }
+  if (k == "trace.refill" || k == "trace.walk") {
+    const bool refill = k == "trace.refill";
+    if (value < (refill ? 1 : 0) || value > 64)
+      return fail(BGPU_E_INVALID, refill ? "trace.refill must be in [1, 64]" : "trace.walk must be in [0, 64]");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (refill ? ctx->trace_cfg.refill : ctx->trace_cfg.walk) = (int)value;
+    return BGPU_OK;
+  }
   if (k == "bvh.width") {
     if (value != 2 && value != 4 && value != 8) return fail(BGPU_E_INVALID, "bvh.width must be 2, 4 or 8");
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1024,6 +1032,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
       return fail(BGPU_E_INVALID, "pt.stack must be in [" + std::to_string(kMinStackLds) + ", " +
                                       std::to_string(kMaxStackLds) + "]");
     next.stack = (int)value;
+  } else if (k == "pt.walk") {
+    if (value < 0 || value > 64) return fail(BGPU_E_INVALID, "pt.walk must be in [0, 64]");
+    next.walk = (int)value;
   } else if (k == "pt.diffuse_waves") {
     if (value != 0 && value != 2 && value != 4) return fail(BGPU_E_INVALID, "pt.diffuse_waves must be 0, 2 or 4");
     next.diffuse_waves = (int)value;
@@ -1073,7 +1084,10 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "bvh.device") *value = ctx->bvh_device;
   else if (k == "trace.persistent") *value = ctx->trace_cfg.persistent;
   else if (k == "trace.stack") *value = ctx->trace_cfg.stack;
+  else if (k == "trace.refill") *value = ctx->trace_cfg.refill;
+  else if (k == "trace.walk") *value = ctx->trace_cfg.walk;
   else if (k == "pt.stack") *value = ctx->pt_cfg.stack;
+  else if (k == "pt.walk") *value = ctx->pt_cfg.walk;
   else if (k == "pt.block") *value = ctx->pt_cfg.block;
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;

@@ -70,9 +70,11 @@ struct DevHits {
 
 // Launch-shape knobs chosen by the autotuner.
 struct TraceConfig {
-  int block = 128;
+  int block = 256;
   int persistent = 1;  // lanes refill with new rays (1) or one ray per thread (0)
   int stack = 12;      // traversal stack entries per lane kept in LDS
+  int refill = 32;     // lane refill: idle lanes that trigger it (1..64)
+  int walk = 16;       // walk inner nodes while at least this many lanes do (0: until all are at leaves)
 };
 // The range of TraceConfig::stack (and of the bounce loop's pt.stack); deeper
 // entries spill to private memory.

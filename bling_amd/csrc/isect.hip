@@ -43,14 +43,14 @@ __device__ __forceinline__ void add_counts(unsigned long long* counters, const t
 
 template <int BLOCK, bool STATS, int W>
 __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, int n, DevHits out,
-                                                   unsigned long long* counters, int nshort) {
+                                                   unsigned long long* counters, int nshort, int walk) {
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
   trav::Best best;
   trav::Counts cn;
-  trav::closest_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort), best, cn);
+  trav::closest_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort, walk), best, cn);
   if (best.prim >= 0) {
     out.t[i] = best.t;
     out.b1[i] = best.b1;
@@ -67,53 +67,57 @@ __global__ __launch_bounds__(BLOCK) void k_closest(DevScene sc, DevRays rays, in
 
 template <int BLOCK, bool STATS, int W>
 __global__ __launch_bounds__(BLOCK) void k_any(DevScene sc, DevRays rays, int n, uint8_t* occ,
-                                               unsigned long long* counters, int nshort) {
+                                               unsigned long long* counters, int nshort, int walk) {
   const int tid = threadIdx.x;
   const int i = blockIdx.x * BLOCK + tid;
   if (i >= n) return;
   DRay r = load_ray(rays, i);
   trav::Counts cn;
-  bool found = trav::any_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort), cn);
+  bool found = trav::any_hit_w<W, STATS>(sc, r, trav::lane_slot<BLOCK>(nshort, walk), cn);
   occ[i] = found ? 1 : 0;
   if (STATS) add_counts(counters, cn);
 }
 
 template <int BLOCK, int W>
 hipError_t run_closest_w(const DevScene& sc, const DevRays& r, int n, const DevHits& h, hipStream_t s,
-                         unsigned long long* counters, int nshort) {
+                         unsigned long long* counters, int nshort, int walk) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
   const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
   if (counters)
-    hipLaunchKernelGGL((k_closest<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort);
+    hipLaunchKernelGGL((k_closest<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort,
+                       walk);
   else
-    hipLaunchKernelGGL((k_closest<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort);
+    hipLaunchKernelGGL((k_closest<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, h, counters, nshort,
+                       walk);
   return hipGetLastError();
 }
 template <int BLOCK>
 hipError_t run_closest(const DevScene& sc, const DevRays& r, int n, const DevHits& h, hipStream_t s,
-                       unsigned long long* counters, int nshort) {
-  if (sc.width == 8) return run_closest_w<BLOCK, 8>(sc, r, n, h, s, counters, nshort);
-  if (sc.width == 4) return run_closest_w<BLOCK, 4>(sc, r, n, h, s, counters, nshort);
-  return run_closest_w<BLOCK, 2>(sc, r, n, h, s, counters, nshort);
+                       unsigned long long* counters, int nshort, int walk) {
+  if (sc.width == 8) return run_closest_w<BLOCK, 8>(sc, r, n, h, s, counters, nshort, walk);
+  if (sc.width == 4) return run_closest_w<BLOCK, 4>(sc, r, n, h, s, counters, nshort, walk);
+  return run_closest_w<BLOCK, 2>(sc, r, n, h, s, counters, nshort, walk);
 }
 
 template <int BLOCK, int W>
 hipError_t run_any_w(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hipStream_t s,
-                     unsigned long long* counters, int nshort) {
+                     unsigned long long* counters, int nshort, int walk) {
   dim3 grid((n + BLOCK - 1) / BLOCK);
   const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
   if (counters)
-    hipLaunchKernelGGL((k_any<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort);
+    hipLaunchKernelGGL((k_any<BLOCK, true, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort,
+                       walk);
   else
-    hipLaunchKernelGGL((k_any<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort);
+    hipLaunchKernelGGL((k_any<BLOCK, false, W>), grid, dim3(BLOCK), lds, s, sc, r, n, occ, counters, nshort,
+                       walk);
   return hipGetLastError();
 }
 template <int BLOCK>
 hipError_t run_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ, hipStream_t s,
-                   unsigned long long* counters, int nshort) {
-  if (sc.width == 8) return run_any_w<BLOCK, 8>(sc, r, n, occ, s, counters, nshort);
-  if (sc.width == 4) return run_any_w<BLOCK, 4>(sc, r, n, occ, s, counters, nshort);
-  return run_any_w<BLOCK, 2>(sc, r, n, occ, s, counters, nshort);
+                   unsigned long long* counters, int nshort, int walk) {
+  if (sc.width == 8) return run_any_w<BLOCK, 8>(sc, r, n, occ, s, counters, nshort, walk);
+  if (sc.width == 4) return run_any_w<BLOCK, 4>(sc, r, n, occ, s, counters, nshort, walk);
+  return run_any_w<BLOCK, 2>(sc, r, n, occ, s, counters, nshort, walk);
 }
 
 // ---------------------------------------------------------------- persistent
@@ -161,14 +165,17 @@ struct BatchSrc {
 template <int BLOCK, int W, bool ANY, bool FAT, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_persist(DevScene sc, DevRays rays, int n, DevHits out,
                                                    uint8_t* occ, unsigned int* work,
-                                                   unsigned long long* counters, int nshort) {
+                                                   unsigned long long* counters, int nshort,
+                                                   trav::PersistTune tune) {
   BatchSrc<ANY, FAT> src{sc, rays, out, occ};
-  trav::persist_trace<W, ANY, FAT, STATS>(src, n, work, trav::lane_slot<BLOCK>(nshort), counters);
+  trav::persist_trace<W, ANY, FAT, STATS>(src, n, work, trav::lane_slot<BLOCK>(nshort, tune.walk_min), counters,
+                                          tune);
 }
 
 template <int BLOCK, int W, bool ANY, bool STATS>
 hipError_t run_persist(const DevScene& sc, const DevRays& r, int n, const DevHits& h, uint8_t* occ,
-                       unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort) {
+                       unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort,
+                       trav::PersistTune tune) {
   // one counter per pass, zeroed together
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, work, 2);
   static int g_tri[trav::kMaxShort + 1] = {}, g_shape[trav::kMaxShort + 1] = {};
@@ -176,24 +183,25 @@ hipError_t run_persist(const DevScene& sc, const DevRays& r, int n, const DevHit
   const int gs = trav::persistent_grid_cached(g_shape, k_persist<BLOCK, W, ANY, true, STATS>, BLOCK, nshort);
   const size_t lds = trav::stack_lds_bytes(nshort, BLOCK);
   hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, false, STATS>), dim3(gt), dim3(BLOCK), lds, s, sc, r, n, h,
-                     occ, work, counters, nshort);
+                     occ, work, counters, nshort, tune);
   if (sc.shape_root >= 0)
     hipLaunchKernelGGL((k_persist<BLOCK, W, ANY, true, STATS>), dim3(gs), dim3(BLOCK), lds, s, sc, r, n, h,
-                       occ, work + 1, counters, nshort);
+                       occ, work + 1, counters, nshort, tune);
   return hipGetLastError();
 }
 
 template <int BLOCK, bool ANY>
 hipError_t run_persist_any_w(const DevScene& sc, const DevRays& r, int n, const DevHits& h, uint8_t* occ,
-                             unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort) {
+                             unsigned int* work, hipStream_t s, unsigned long long* counters, int nshort,
+                             trav::PersistTune tune) {
   if (counters) {
-    if (sc.width == 8) return run_persist<BLOCK, 8, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
-    if (sc.width == 4) return run_persist<BLOCK, 4, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
-    return run_persist<BLOCK, 2, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort);
+    if (sc.width == 8) return run_persist<BLOCK, 8, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort, tune);
+    if (sc.width == 4) return run_persist<BLOCK, 4, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort, tune);
+    return run_persist<BLOCK, 2, ANY, true>(sc, r, n, h, occ, work, s, counters, nshort, tune);
   }
-  if (sc.width == 8) return run_persist<BLOCK, 8, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
-  if (sc.width == 4) return run_persist<BLOCK, 4, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
-  return run_persist<BLOCK, 2, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort);
+  if (sc.width == 8) return run_persist<BLOCK, 8, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort, tune);
+  if (sc.width == 4) return run_persist<BLOCK, 4, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort, tune);
+  return run_persist<BLOCK, 2, ANY, false>(sc, r, n, h, occ, work, s, counters, nshort, tune);
 }
 
 }  // namespace
@@ -210,17 +218,21 @@ hipError_t launch_closest(const DevScene& sc, const DevRays& r, int n, const Dev
   if (n <= 0) return hipSuccess;
   const int ns = cfg.stack;
   if (ns < trav::kMinShort || ns > trav::kMaxShort) return hipErrorInvalidValue;
+  if (cfg.refill < 1 || cfg.refill > 64 || cfg.walk < 0 || cfg.walk > 64) return hipErrorInvalidValue;
+  trav::PersistTune tune;
+  tune.refill_min = cfg.refill;
+  tune.walk_min = cfg.walk;
   if (cfg.persistent && work) {
     switch (cfg.block) {
-      case 64: return run_persist_any_w<64, false>(sc, r, n, h, nullptr, work, s, counters, ns);
-      case 256: return run_persist_any_w<256, false>(sc, r, n, h, nullptr, work, s, counters, ns);
-      default: return run_persist_any_w<128, false>(sc, r, n, h, nullptr, work, s, counters, ns);
+      case 64: return run_persist_any_w<64, false>(sc, r, n, h, nullptr, work, s, counters, ns, tune);
+      case 256: return run_persist_any_w<256, false>(sc, r, n, h, nullptr, work, s, counters, ns, tune);
+      default: return run_persist_any_w<128, false>(sc, r, n, h, nullptr, work, s, counters, ns, tune);
     }
   }
   switch (cfg.block) {
-    case 64: return run_closest<64>(sc, r, n, h, s, counters, ns);
-    case 256: return run_closest<256>(sc, r, n, h, s, counters, ns);
-    default: return run_closest<128>(sc, r, n, h, s, counters, ns);
+    case 64: return run_closest<64>(sc, r, n, h, s, counters, ns, cfg.walk);
+    case 256: return run_closest<256>(sc, r, n, h, s, counters, ns, cfg.walk);
+    default: return run_closest<128>(sc, r, n, h, s, counters, ns, cfg.walk);
   }
 }
 
@@ -230,18 +242,22 @@ hipError_t launch_any(const DevScene& sc, const DevRays& r, int n, uint8_t* occ,
   if (n <= 0) return hipSuccess;
   const int ns = cfg.stack;
   if (ns < trav::kMinShort || ns > trav::kMaxShort) return hipErrorInvalidValue;
+  if (cfg.refill < 1 || cfg.refill > 64 || cfg.walk < 0 || cfg.walk > 64) return hipErrorInvalidValue;
+  trav::PersistTune tune;
+  tune.refill_min = cfg.refill;
+  tune.walk_min = cfg.walk;
   if (cfg.persistent && work) {
     const DevHits none{nullptr, nullptr, nullptr, nullptr};
     switch (cfg.block) {
-      case 64: return run_persist_any_w<64, true>(sc, r, n, none, occ, work, s, counters, ns);
-      case 256: return run_persist_any_w<256, true>(sc, r, n, none, occ, work, s, counters, ns);
-      default: return run_persist_any_w<128, true>(sc, r, n, none, occ, work, s, counters, ns);
+      case 64: return run_persist_any_w<64, true>(sc, r, n, none, occ, work, s, counters, ns, tune);
+      case 256: return run_persist_any_w<256, true>(sc, r, n, none, occ, work, s, counters, ns, tune);
+      default: return run_persist_any_w<128, true>(sc, r, n, none, occ, work, s, counters, ns, tune);
     }
   }
   switch (cfg.block) {
-    case 64: return run_any<64>(sc, r, n, occ, s, counters, ns);
-    case 256: return run_any<256>(sc, r, n, occ, s, counters, ns);
-    default: return run_any<128>(sc, r, n, occ, s, counters, ns);
+    case 64: return run_any<64>(sc, r, n, occ, s, counters, ns, cfg.walk);
+    case 256: return run_any<256>(sc, r, n, occ, s, counters, ns, cfg.walk);
+    default: return run_any<128>(sc, r, n, occ, s, counters, ns, cfg.walk);
   }
 }
 

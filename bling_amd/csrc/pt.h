@@ -162,6 +162,7 @@ struct PtConfig {
   int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2 waves per SIMD)
   int diffuse_waves = 2;  // the same for its diffuse-only variant (0, 2 or 4)
   int stack = 12;         // traversal stack entries per lane kept in LDS (kMinStackLds..kMaxStackLds)
+  int walk = 8;           // traversal: walk inner nodes while at least this many lanes do (0: all)
   int persistent = 0;   // traversal passes with lane refill (1) or one ray per thread (0)
   int sort = 2;         // shading queue grouped by material kind: 1 always, 0 never, 2 while a
                         // resident scene has more than one BxDF makeup
@@ -184,6 +185,7 @@ struct PtKernelArgs {
   int32_t direct;      // direct-lighting jobs may be resident: run their shading kernel
   int32_t shade_mode;  // SHADE_MODE_*: which shading variants the iteration runs
   int32_t nshort;      // traversal stack entries per lane in LDS (PtConfig::stack)
+  int32_t walk;        // traversal walk threshold (PtConfig::walk)
 };
 
 // Shading variants of an iteration: one kernel for every BxDF kind; the

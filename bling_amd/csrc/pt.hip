@@ -96,6 +96,19 @@ __device__ __forceinline__ void xyz_accum(float* a, int Pn, int i, const float* 
   a[2 * Pn + i] += src[2 * Pn + i];
 }
 
+// A ray stored in the pool as 7 (tmax given) or 8 SoA words.
+__device__ __forceinline__ DRay load_pool_ray(const float* w, int Pn, int i, float tmax) {
+  DRay r;
+  r.o = mk3(w[i], w[Pn + i], w[2 * Pn + i]);
+  r.d = mk3(w[3 * Pn + i], w[4 * Pn + i], w[5 * Pn + i]);
+  r.tmin = w[6 * Pn + i];
+  r.tmax = tmax;
+  return r;
+}
+__device__ __forceinline__ DRay load_pool_ray(const float* w, int Pn, int i) {
+  return load_pool_ray(w, Pn, i, w[7 * Pn + i]);
+}
+
 __device__ __forceinline__ unsigned int wave_sum(unsigned int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -326,7 +339,7 @@ __device__ __forceinline__ bool pt_any(const DevScene& sc, const DRay& r, const 
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_act;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;  // whole block idle
@@ -334,15 +347,13 @@ __global__ __launch_bounds__(BLOCK) void k_extend(PtKernelArgs a) {
   const int Pn = P.P;
   bool valid = (unsigned)q < n;
   int i = valid ? (int)P.q_act[q] : 0;
+  // the ray is loaded before the scene lookup, so that the two latencies
+  // overlap (a loop over scenes would keep the loads behind it)
+  const DRay r = load_pool_ray(P.ray, Pn, i, INFINITY);
   int scene = valid ? a.jobs[P.job[i]].scene : 0;
   for_uniform_scene(valid, scene, [&](int scene_u) {
     {
       const DevScene sc = a.scenes[scene_u].geo;
-      DRay r;
-      r.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
-      r.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
-      r.tmin = P.ray[6 * Pn + i];
-      r.tmax = INFINITY;
       trav::Best best;
       pt_closest<W>(sc, r, slot, best);
       P.hit[i] = best.t;
@@ -562,11 +573,9 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
     {
       const DevPtScene& S = a.scenes[scene_u];
       const DevJob& J = a.jobs[jslot];
-      DRay ray;
-      ray.o = mk3(P.ray[i], P.ray[Pn + i], P.ray[2 * Pn + i]);
-      ray.d = mk3(P.ray[3 * Pn + i], P.ray[4 * Pn + i], P.ray[5 * Pn + i]);
-      ray.tmin = P.ray[6 * Pn + i];
-      ray.tmax = INFINITY;
+      // loaded here rather than before the scene loop: values live across
+      // the loop would stay in registers through the whole shading body
+      const DRay ray = load_pool_ray(P.ray, Pn, i, INFINITY);
       int prim = P.hprim[i];
       int ds = P.dspec[i];
       int depth = ds & 0xffff;
@@ -793,7 +802,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_shadow;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
@@ -801,14 +810,10 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
   const int Pn = P.P;
   bool valid = (unsigned)q < n;
   int i = valid ? (int)P.q_shadow[q] : 0;
+  const DRay r = load_pool_ray(P.sray, Pn, i);  // before the scene lookup (k_extend)
   for_uniform_scene(valid, valid ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
     {
       const DevScene sc = a.scenes[scene_u].geo;
-      DRay r;
-      r.o = mk3(P.sray[i], P.sray[Pn + i], P.sray[2 * Pn + i]);
-      r.d = mk3(P.sray[3 * Pn + i], P.sray[4 * Pn + i], P.sray[5 * Pn + i]);
-      r.tmin = P.sray[6 * Pn + i];
-      r.tmax = P.sray[7 * Pn + i];
       if (!pt_any<W>(sc, r, slot)) {  // occluded (Scene.hs:45-47)
         xyz_accum(P.rad, Pn, i, P.pend_s);
       }
@@ -818,7 +823,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(PtKernelArgs a) {
 
 template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int n = a.cnt->n_mis;
   if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
@@ -826,16 +831,14 @@ __global__ __launch_bounds__(BLOCK) void k_mis(PtKernelArgs a) {
   const int Pn = P.P;
   bool valid = (unsigned)q < n;
   int i = valid ? (int)P.q_mis[q] : 0;
+  // before the scene lookup (k_extend); tmax is bound by the sampled light's
+  // hit (the root box sees tmax = inf)
+  const DRay r = load_pool_ray(P.mray, Pn, i);
+  const int ln = P.mlight[i];
   for_uniform_scene(valid, valid ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
     {
       const DevPtScene& S = a.scenes[scene_u];
       const DevScene sc = S.geo;
-      DRay r;
-      r.o = mk3(P.mray[i], P.mray[Pn + i], P.mray[2 * Pn + i]);
-      r.d = mk3(P.mray[3 * Pn + i], P.mray[4 * Pn + i], P.mray[5 * Pn + i]);
-      r.tmin = P.mray[6 * Pn + i];
-      r.tmax = P.mray[7 * Pn + i];  // bound by the sampled light's hit (the root box sees tmax = inf)
-      int ln = P.mlight[i];
       const DLight& L = S.lights[ln];
       trav::Best best;
       best.prim = -1;
@@ -966,21 +969,24 @@ struct MisSrc {  // closest hit of the MIS ray into hit / hprim (resolved by k_m
 
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_extend_p(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   ExtendSrc<FAT> src{a};
-  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_act, nullptr, slot, nullptr);
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_act, nullptr, slot, nullptr,
+                                            trav::PersistTune{trav::kRefillMin, a.walk});
 }
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_shadow_p(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   ShadowSrc<FAT> src{a};
-  trav::persist_trace<W, true, FAT, false>(src, (int)a.cnt->n_shadow, nullptr, slot, nullptr);
+  trav::persist_trace<W, true, FAT, false>(src, (int)a.cnt->n_shadow, nullptr, slot, nullptr,
+                                            trav::PersistTune{trav::kRefillMin, a.walk});
 }
 template <int BLOCK, int W, bool FAT>
 __global__ __launch_bounds__(BLOCK) void k_mis_p(PtKernelArgs a) {
-  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort);
+  const trav::LdsSlot slot = trav::lane_slot<BLOCK>(a.nshort, a.walk);
   MisSrc<FAT> src{a};
-  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_mis, nullptr, slot, nullptr);
+  trav::persist_trace<W, false, FAT, false>(src, (int)a.cnt->n_mis, nullptr, slot, nullptr,
+                                            trav::PersistTune{trav::kRefillMin, a.walk});
 }
 
 // The MIS ray reaches the sampled light: the hit primitive is that AreaLight
@@ -1306,8 +1312,10 @@ static hipError_t enqueue_iter_w(const PtKernelArgs& a, int width, const PtConfi
 hipError_t pt_enqueue_iteration(const PtKernelArgs& args, const PtConfig& cfg, int width, hipStream_t s,
                                 hipEvent_t* marks) {
   if (cfg.stack < trav::kMinShort || cfg.stack > trav::kMaxShort) return hipErrorInvalidValue;
+  if (cfg.walk < 0 || cfg.walk > 64) return hipErrorInvalidValue;
   PtKernelArgs a = args;
   a.nshort = cfg.stack;
+  a.walk = cfg.walk;
   switch (cfg.block) {
     case 64: return enqueue_iter_w<64>(a, width, cfg, s, marks);
     case 256: return enqueue_iter_w<256>(a, width, cfg, s, marks);

@@ -11,6 +11,9 @@
 //   materials bgpu_material[num_materials]
 //   lights    bgpu_light[num_lights] (radiance pointers stored as 0)
 //   one radiance map per light, f32[16 w h]
+//   tex_nodes bgpu_tex_node[num_tex_nodes]
+//   images    bgpu_image[num_images] (texel pointers stored as 0)
+//   the texels of each image, f32[channels w h]
 // The header records the record sizes; a file written with other record
 // layouts is refused rather than misread.
 #include <algorithm>
@@ -26,7 +29,7 @@ namespace bgpu {
 namespace {
 
 constexpr char kMagic[8] = {'B', 'G', 'P', 'U', 'S', 'C', 'N', '1'};
-constexpr uint32_t kVersion = 2;  // 2: materials carry three spectra and four scalars
+constexpr uint32_t kVersion = 3;  // 2: three spectra and four scalars per material; 3: texture nodes, images
 constexpr uint32_t kByteOrder = 0x01020304u;
 constexpr uint32_t kHasNormals = 1, kHasUvs = 2, kHasTriMesh = 4;
 constexpr uint64_t kAlign = 64;
@@ -41,6 +44,7 @@ struct FileHeader {
   uint64_t file_bytes;
   bgpu_camera camera;
   bgpu_film film;
+  uint32_t num_tex_nodes, num_images, size_tex_node, size_image;
 };
 static_assert(sizeof(FileHeader) <= 512, "header must fit its 512 bytes");
 constexpr uint64_t kHeaderBytes = 512;
@@ -52,10 +56,23 @@ uint64_t up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 struct Sections {
   uint64_t vertices, normals, uvs, indices, tri_mesh, meshes, shapes, materials, lights;
   std::vector<uint64_t> maps;
+  uint64_t tex_nodes, images;
+  std::vector<uint64_t> texels;
   uint64_t end;
 };
 
-bool layout(const FileHeader& h, const std::vector<bgpu_light>& lights, Sections* s) {
+// Texels of an image record, or 0 when its size is out of range.
+uint64_t image_floats(const bgpu_image& im) {
+  if (im.width <= 0 || im.height <= 0 || (im.channels != 1 && im.channels != 16)) return 0;
+  if ((uint64_t)im.width * (uint64_t)im.height > (1ull << 28)) return 0;
+  return (uint64_t)im.channels * im.width * im.height;
+}
+
+// Offsets of every section; light maps and texel blocks only once their
+// records are known (`lights`, `images` non-null).  Returns false when a map
+// or an image is out of range.
+bool layout(const FileHeader& h, const std::vector<bgpu_light>* lights, const std::vector<bgpu_image>* images,
+            Sections* s) {
   uint64_t o = kHeaderBytes;
   auto take = [&](uint64_t bytes) {
     uint64_t at = o;
@@ -73,9 +90,20 @@ bool layout(const FileHeader& h, const std::vector<bgpu_light>& lights, Sections
   s->materials = take(sizeof(bgpu_material) * (uint64_t)h.num_materials);
   s->lights = take(sizeof(bgpu_light) * (uint64_t)h.num_lights);
   s->maps.clear();
-  for (const bgpu_light& l : lights) {
+  s->texels.clear();
+  s->tex_nodes = s->images = s->end = 0;
+  if (!lights) return true;
+  for (const bgpu_light& l : *lights) {
     if (l.width <= 0 || l.height <= 0 || (uint64_t)l.width * (uint64_t)l.height > (1ull << 28)) return false;
     s->maps.push_back(take(64ull * l.width * l.height));
+  }
+  s->tex_nodes = take(sizeof(bgpu_tex_node) * (uint64_t)h.num_tex_nodes);
+  s->images = take(sizeof(bgpu_image) * (uint64_t)h.num_images);
+  if (!images) return true;
+  for (const bgpu_image& im : *images) {
+    const uint64_t n = image_floats(im);
+    if (!n) return false;
+    s->texels.push_back(take(4 * n));
   }
   s->end = o;
   return true;
@@ -93,6 +121,7 @@ struct OwnedDesc {
   bgpu_scene_desc desc;  // first member: its address is the block's
   std::vector<unsigned char> bytes;
   std::vector<bgpu_light> lights;
+  std::vector<bgpu_image> images;
 };
 static_assert(std::is_standard_layout<OwnedDesc>::value, "desc must sit at the block's address");
 
@@ -107,8 +136,11 @@ extern "C" int bgpu_scene_write(const bgpu_scene_desc* d, const char* path) {
     return fail(BGPU_E_INVALID, "bgpu_scene_write: too many vertices or triangles");
   if ((d->num_vertices && !d->vertices) || (d->num_triangles && !d->indices) ||
       (d->num_meshes && !d->meshes) || (d->num_shapes && !d->shapes) ||
-      (d->num_materials && !d->materials) || (d->num_lights && !d->lights))
+      (d->num_materials && !d->materials) || (d->num_lights && !d->lights) ||
+      (d->num_tex_nodes && !d->tex_nodes) || (d->num_images && !d->images))
     return fail(BGPU_E_INVALID, "bgpu_scene_write: a counted array is NULL");
+  if (d->num_tex_nodes > kMaxCount || d->num_images > (1u << 16))
+    return fail(BGPU_E_INVALID, "bgpu_scene_write: too many texture nodes or images");
   FileHeader h;
   std::memset(&h, 0, sizeof(h));
   std::memcpy(h.magic, kMagic, 8);
@@ -129,11 +161,19 @@ extern "C" int bgpu_scene_write(const bgpu_scene_desc* d, const char* path) {
   h.size_film = sizeof(bgpu_film);
   h.camera = d->camera;
   h.film = d->film;
+  h.num_tex_nodes = d->num_tex_nodes;
+  h.num_images = d->num_images;
+  h.size_tex_node = sizeof(bgpu_tex_node);
+  h.size_image = sizeof(bgpu_image);
   std::vector<bgpu_light> lights(d->lights, d->lights + d->num_lights);
   for (const bgpu_light& l : lights)
     if (!l.radiance) return fail(BGPU_E_INVALID, "bgpu_scene_write: light without a radiance map");
+  std::vector<bgpu_image> images(d->images, d->images + d->num_images);
+  for (const bgpu_image& im : images)
+    if (!im.texels) return fail(BGPU_E_INVALID, "bgpu_scene_write: image without texels");
   Sections s;
-  if (!layout(h, lights, &s)) return fail(BGPU_E_INVALID, "bgpu_scene_write: light map size out of range");
+  if (!layout(h, &lights, &images, &s))
+    return fail(BGPU_E_INVALID, "bgpu_scene_write: light map or image size out of range");
   h.file_bytes = s.end;
 
   File f(std::fopen(path, "wb"));
@@ -167,6 +207,11 @@ extern "C" int bgpu_scene_write(const bgpu_scene_desc* d, const char* path) {
   put(s.lights, stored.data(), sizeof(bgpu_light) * stored.size());
   for (size_t i = 0; i < lights.size(); ++i)
     put(s.maps[i], lights[i].radiance, 64ull * lights[i].width * lights[i].height);
+  put(s.tex_nodes, d->tex_nodes, sizeof(bgpu_tex_node) * (uint64_t)d->num_tex_nodes);
+  std::vector<bgpu_image> stored_images = images;
+  for (bgpu_image& im : stored_images) im.texels = nullptr;
+  put(s.images, stored_images.data(), sizeof(bgpu_image) * stored_images.size());
+  for (size_t i = 0; i < images.size(); ++i) put(s.texels[i], images[i].texels, 4 * image_floats(images[i]));
   put(s.end, nullptr, 0);
   if (!ok || std::fflush(f.get()) != 0) return fail(BGPU_E_INVALID, std::string("bgpu_scene_write: write failed: ") + path);
   return BGPU_OK;
@@ -197,20 +242,28 @@ extern "C" int bgpu_scene_read(const char* path, bgpu_scene_desc** out) {
     return fail(BGPU_E_INVALID, "bgpu_scene_read: unsupported version or byte order");
   if (h.size_mesh != sizeof(bgpu_mesh) || h.size_shape != sizeof(bgpu_shape) ||
       h.size_material != sizeof(bgpu_material) || h.size_light != sizeof(bgpu_light) ||
-      h.size_camera != sizeof(bgpu_camera) || h.size_film != sizeof(bgpu_film))
+      h.size_camera != sizeof(bgpu_camera) || h.size_film != sizeof(bgpu_film) ||
+      h.size_tex_node != sizeof(bgpu_tex_node) || h.size_image != sizeof(bgpu_image))
     return fail(BGPU_E_INVALID, "bgpu_scene_read: record layouts differ from this library's");
   if (h.num_vertices > kMaxCount || h.num_triangles > kMaxCount || h.num_meshes > kMaxCount ||
       h.num_shapes > kMaxCount || h.num_materials > kMaxCount || h.num_lights > (1u << 16) ||
+      h.num_tex_nodes > kMaxCount || h.num_images > (1u << 16) ||
       (h.flags & ~(kHasNormals | kHasUvs | kHasTriMesh)))
     return fail(BGPU_E_INVALID, "bgpu_scene_read: corrupt header");
   if (h.file_bytes != (uint64_t)size) return fail(BGPU_E_INVALID, "bgpu_scene_read: file size differs from its header");
-  // Light records first: the map sections' sizes depend on them.
+  // Light records first, then image records: the sections after each depend
+  // on them.
   Sections s;
-  if (!layout(h, {}, &s) || s.lights + sizeof(bgpu_light) * (uint64_t)h.num_lights > (uint64_t)size)
+  if (!layout(h, nullptr, nullptr, &s) || s.lights + sizeof(bgpu_light) * (uint64_t)h.num_lights > (uint64_t)size)
     return fail(BGPU_E_INVALID, "bgpu_scene_read: file shorter than its sections");
   od->lights.resize(h.num_lights);
   if (h.num_lights) std::memcpy(od->lights.data(), od->bytes.data() + s.lights, sizeof(bgpu_light) * h.num_lights);
-  if (!layout(h, od->lights, &s)) return fail(BGPU_E_INVALID, "bgpu_scene_read: light map size out of range");
+  if (!layout(h, &od->lights, nullptr, &s)) return fail(BGPU_E_INVALID, "bgpu_scene_read: light map size out of range");
+  if (s.images + sizeof(bgpu_image) * (uint64_t)h.num_images > (uint64_t)size)
+    return fail(BGPU_E_INVALID, "bgpu_scene_read: file shorter than its sections");
+  od->images.resize(h.num_images);
+  if (h.num_images) std::memcpy(od->images.data(), od->bytes.data() + s.images, sizeof(bgpu_image) * h.num_images);
+  if (!layout(h, &od->lights, &od->images, &s)) return fail(BGPU_E_INVALID, "bgpu_scene_read: image size out of range");
   if (s.end != (uint64_t)size) return fail(BGPU_E_INVALID, "bgpu_scene_read: file size differs from its sections");
   unsigned char* b = od->bytes.data();
   bgpu_scene_desc& d = od->desc;
@@ -233,6 +286,11 @@ extern "C" int bgpu_scene_read(const char* path, bgpu_scene_desc** out) {
   d.lights = od->lights.data();
   d.camera = h.camera;
   d.film = h.film;
+  d.num_tex_nodes = h.num_tex_nodes;
+  d.tex_nodes = reinterpret_cast<const bgpu_tex_node*>(b + s.tex_nodes);
+  for (size_t i = 0; i < od->images.size(); ++i) od->images[i].texels = reinterpret_cast<const float*>(b + s.texels[i]);
+  d.num_images = h.num_images;
+  d.images = od->images.data();
   *out = &od.release()->desc;
   return BGPU_OK;
 }

@@ -34,18 +34,22 @@ inline size_t stack_lds_bytes(int nshort, int block) {
   return (size_t)lds_per_lane(nshort) * (size_t)block * sizeof(int);
 }
 
-// Where a lane's LDS stack entries are: entry d at lds[d * stride], d < nshort.
+// Where a lane's LDS stack entries are (entry d at lds[d * stride],
+// d < nshort), and the launch's walk threshold: a wave walks inner nodes
+// until every lane is at a leaf or done, or (walk > 0) until fewer than
+// `walk` lanes still walk while another waits at a leaf.
 struct LdsSlot {
   int* lds;
   int stride;
   int nshort;
+  int walk;
 };
 // This lane's slot in the block's dynamic LDS (stack_lds_bytes(nshort, BLOCK)
 // bytes, given at launch).
 template <int BLOCK>
-__device__ __forceinline__ LdsSlot lane_slot(int nshort) {
+__device__ __forceinline__ LdsSlot lane_slot(int nshort, int walk) {
   extern __shared__ int bgpu_stack_lds[];
-  return LdsSlot{bgpu_stack_lds + threadIdx.x, BLOCK, nshort};
+  return LdsSlot{bgpu_stack_lds + threadIdx.x, BLOCK, nshort, walk};
 }
 
 // The counters live in registers; only the spill arrays, which are indexed
@@ -55,6 +59,7 @@ struct Stack {
   int* lds;    // lds_per_lane(nshort) entries per lane, `stride` apart
   int stride;
   int nshort;  // entries in LDS (uniform over the launch)
+  int walk;    // walk threshold of the launch (LdsSlot)
   int sp;
   int* spill_n;    // kSpill entries
   float* spill_t;  // kSpill entries when kCull
@@ -62,6 +67,7 @@ struct Stack {
     lds = s.lds;
     stride = s.stride;
     nshort = s.nshort;
+    walk = s.walk;
     sp = 0;
     spill_n = sn;
     spill_t = stt;
@@ -194,14 +200,33 @@ __device__ __forceinline__ void sort_children(float (&k)[W], int (&l)[W]) {
 
 // Traverses one tree.  CLOSEST: shrinks best.t; ANY: returns at the first hit.
 // EXACT (with ANY): shapes are tested with `intersect` rather than
-// `intersects`, so that "any hit" is exactly "a closest hit exists".
+// `intersects`, so that "any hit" is exactly "a closest hit exists".  Each
+// lane visits the same nodes in the same order whatever the other lanes do;
+// the wave walks inner nodes together, then tests the leaves it reached
+// together (st.walk sets when the walk stops early, see LdsSlot).
 template <int W, bool ANY, bool FAT, bool STATS, bool EXACT = false>
 __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRay& r, f3 inv,
                                          f3 oi, Stack& st, Best& best, Counts& cn) {
-  int cur = root;
+  int cur = root;  // inner node (>= 0, not kNoChild), leaf (< 0) or kNoChild (done)
   st.sp = 0;
+  // the next stack entry whose subtree may still hold a hit at or before
+  // best.t (an equal t can still win the tie on the primitive id), or kNoChild
+  auto next = [&]() -> int {
+    while (st.sp > 0) {
+      int n;
+      float t;
+      st.pop(&n, &t);
+      if (ANY || !kCull || t <= best.t) return n;
+    }
+    return kNoChild;
+  };
   for (;;) {
-    if (cur >= 0) {
+    const int n_live = st.walk > 0 ? __popcll(__ballot(true)) : 0;
+    while (cur >= 0 && cur != kNoChild) {
+      if (st.walk > 0) {
+        const int n_walk = __popcll(__ballot(true));
+        if (n_walk < st.walk && n_walk < n_live) break;
+      }
       const float4* nd = sc.nodes + (size_t)cur * (2 * W);
       float v[8 * W];
 #pragma unroll
@@ -235,9 +260,12 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
         for (int c = W - 1; c >= 1; --c)
           if (c < nhit) st.push(lk[c], key[c]);
         cur = lk[0];
-        continue;
+      } else {
+        cur = next();
       }
-    } else {
+    }
+    if (cur == kNoChild) break;
+    if (cur < 0) {  // a leaf
       int enc = ~cur;
       int first = enc >> 4, cnt = enc & 15;
       for (int k = first; k < first + cnt; ++k) {
@@ -274,21 +302,9 @@ __device__ __forceinline__ bool traverse(const DevScene& sc, int root, const DRa
           best.b2 = b2;
         }
       }
+      cur = next();
+      if (cur == kNoChild) break;
     }
-    // next entry whose subtree may still hold a hit at or before best.t (an
-    // equal t can still win the tie on the primitive id)
-    bool found = false;
-    while (st.sp > 0) {
-      int n;
-      float t;
-      st.pop(&n, &t);
-      if (ANY || !kCull || t <= best.t) {
-        cur = n;
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
   }
   return false;
 }

@@ -2,12 +2,13 @@
 // the bounce-loop kernels.  One pass walks one tree (triangles: FAT false;
 // shapes: FAT true) for work items 0..n-1.  Lanes take a new item as soon as
 // theirs is done, so a wave does not idle on its slowest ray, and refill
-// once kRefillMin lanes are idle (new items then arrive in groups of
+// once `refill_min` lanes are idle (new items then arrive in groups of
 // neighbours).  Items come either from slices of kSlice taken from a global
 // counter (`work` non-null: waves balance each other), or from one static
 // contiguous range per wave (`work` null: no atomics, for short queues).  A
 // step first runs inner nodes until every lane of the wave is at a leaf or
-// done, then tests the leaves together.
+// done (or, with `walk_min` > 0, until fewer than walk_min lanes still walk
+// while others wait at leaves), then tests the leaves together.
 //
 // The item source `Src` supplies, per item k:
 //   bool begin(int k, DRay& r, LaneScene& ls, Best& best)
@@ -33,6 +34,12 @@ namespace trav {
 #endif
 constexpr unsigned int kSlice = BGPU_SLICE;
 constexpr int kRefillMin = BGPU_REFILL_MIN;
+
+// Launch-time scheduling of a persistent pass (results do not depend on it).
+struct PersistTune {
+  int refill_min = kRefillMin;  // idle lanes that trigger a refill (1..64)
+  int walk_min = 0;             // 0: walk until every lane is at a leaf
+};
 // A lane that reaches a leaf while other lanes of its wave still walk inner
 // nodes postpones the leaf and walks on (speculative traversal, Aila and
 // Laine 2009), so that the leaf step runs with more lanes active.
@@ -49,7 +56,7 @@ struct LaneScene {
 
 template <int W, bool ANY, bool FAT, bool STATS, class Src>
 __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* work, const LdsSlot& slot,
-                                              unsigned long long* counters) {
+                                              unsigned long long* counters, PersistTune tune = {}) {
   const int lane = __lane_id();
   int spill_n[kSpill];
   float spill_t[kCull ? kSpill : 1];
@@ -75,7 +82,7 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
   for (;;) {
     // refill idle lanes from the wave's slice, taking a new slice when empty
     const int n_idle = __popcll(__ballot(item < 0));
-    if (n_idle > 0 && (n_idle >= kRefillMin || n_idle == 64) && (!drained || s_next < s_end)) {
+    if (n_idle > 0 && (n_idle >= tune.refill_min || n_idle == 64) && (!drained || s_next < s_end)) {
       if (!drained && s_next >= s_end) {
         unsigned int base = 0;
         if (lane == 0) base = atomicAdd(work, kSlice);
@@ -116,8 +123,16 @@ __device__ __forceinline__ void persist_trace(Src& src, int n, unsigned int* wor
         cur = kNoChild;
       }
     }
-    // inner nodes until every lane is at a leaf or done
+    // inner nodes until every lane is at a leaf or done; with walk_min, the
+    // walk also stops once fewer than walk_min lanes still walk while at
+    // least one other lane waits at a leaf (the decision is wave-uniform, and
+    // a waiting lane guarantees progress)
+    const int n_live = tune.walk_min > 0 ? __popcll(__ballot(item >= 0)) : 0;
     while (item >= 0 && cur >= 0 && cur != kNoChild) {
+      if (tune.walk_min > 0) {
+        const int n_walk = __popcll(__ballot(true));
+        if (n_walk < tune.walk_min && n_walk < n_live) break;
+      }
       const float4* nd = ls.nodes + (size_t)cur * (2 * W);
       float v[8 * W];
 #pragma unroll

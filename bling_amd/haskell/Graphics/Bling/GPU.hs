@@ -12,7 +12,7 @@
 module Graphics.Bling.GPU (
    -- * Flat scene description
    GpuSceneBuilder, emptyGpuScene, gpuAddMesh, gpuAddShape, gpuAddMaterial, gpuAddInfiniteLight,
-   gpuSetCamera, gpuSetFilm, gpuNextLightId, withSceneDesc,
+   gpuSetCamera, gpuSetFilm, gpuNextLightId, gpuAddTexNode, gpuAddImage, withSceneDesc,
 
    -- * The renderer
    GpuJob(..), defaultGpuJob, GpuRenderer(..), mkGpuRenderer, GpuError(..),
@@ -67,13 +67,24 @@ data GpuSceneBuilder = GSB
    , gsbCamera    :: Camera
    , gsbFilm      :: Film
    , gsbFilter    :: Maybe Filter -- ^ the reference's filter value, for the reported images
+   , gsbTexNodes  :: [TexNode]    -- ^ texture nodes, newest first; node id = position from the end
+   , gsbImages    :: [(Int32, Int32, Int32, VS.Vector Float)]
+     -- ^ image textures (width, height, channels, texels), newest first
    }
 
 emptyGpuScene :: GpuSceneBuilder
-emptyGpuScene = GSB [] [] [] [] 0 cam film Nothing where
+emptyGpuScene = GSB [] [] [] [] 0 cam film Nothing [] [] where
    cam = Camera cameraPerspective 0 ident ident 0 1 0 0
    film = Film 256 256 filterBox 0 [0, 0, 0, 0]
    ident = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
+
+-- | Records a texture node (its children recorded before it); returns its id.
+gpuAddTexNode :: TexNode -> GpuSceneBuilder -> (Int32, GpuSceneBuilder)
+gpuAddTexNode n b = (fromIntegral (length (gsbTexNodes b)), b { gsbTexNodes = n : gsbTexNodes b })
+
+-- | Records an image texture's texels; returns its id.
+gpuAddImage :: (Int32, Int32, Int32, VS.Vector Float) -> GpuSceneBuilder -> (Int32, GpuSceneBuilder)
+gpuAddImage i b = (fromIntegral (length (gsbImages b)), b { gsbImages = i : gsbImages b })
 
 -- | Records a material; returns its id (the index the descriptor gives it).
 gpuAddMaterial :: Material -> GpuSceneBuilder -> (Int32, GpuSceneBuilder)
@@ -129,7 +140,11 @@ withSceneDesc b act = do
     withArray (gsbShapes b) $ \pss ->
     withArray (reverse (gsbMaterials b)) $ \pmat ->
     withMaps (map snd lights) $ \mapPtrs ->
-    withArray [l { lgRadiance = mp } | ((l, _), mp) <- zip lights mapPtrs] $ \pls -> do
+    withArray [l { lgRadiance = mp } | ((l, _), mp) <- zip lights mapPtrs] $ \pls ->
+    withArray (reverse (gsbTexNodes b)) $ \pnodes ->
+    withMaps (map (\(_, _, _, t) -> t) (reverse (gsbImages b))) $ \texPtrs ->
+    withArray [Image w h c 0 t | ((w, h, c, _), t) <- zip (reverse (gsbImages b)) texPtrs] $ \pimgs -> do
+      let imgs = gsbImages b
       let d = SceneDesc
                 { sdNumVertices = fromIntegral (VS.length verts `quot` 3)
                 , sdNumTriangles = fromIntegral (VS.length idx `quot` 3)
@@ -142,7 +157,9 @@ withSceneDesc b act = do
                 , sdNumShapes = fromIntegral (length (gsbShapes b)), sdShapes = pss
                 , sdNumMaterials = fromIntegral (length (gsbMaterials b)), sdMaterials = pmat
                 , sdNumLights = fromIntegral (length lights), sdLights = pls
-                , sdCamera = gsbCamera b, sdFilm = gsbFilm b }
+                , sdCamera = gsbCamera b, sdFilm = gsbFilm b
+                , sdNumTexNodes = fromIntegral (length (gsbTexNodes b)), sdTexNodes = pnodes
+                , sdNumImages = fromIntegral (length imgs), sdImages = pimgs }
       with d act
    where
       withMaps [] k = k []

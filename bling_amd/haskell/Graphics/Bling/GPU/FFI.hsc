@@ -143,6 +143,47 @@ instance Storable Texture where
     mapM_ (\(k, s) -> pokeFloats 16 (#{ptr bgpu_texture, alt} p `plusPtr` (64 * k)) s)
           (zip [0, 1] (take 2 (txAlt t ++ repeat [])))
 
+-- | bgpu_tex_node: one node of a texture (Texture.hs:59-414) in a scene's
+-- node list; children precede their parents.
+data TexNode = TexNode
+  { tnOp, tnA, tnB, tnC, tnMap :: !Int32, tnPad :: [Int32]
+  , tnQ :: [Float], tnM :: [Float], tnSpec :: [Float] }
+
+instance Storable TexNode where
+  sizeOf _ = #{size bgpu_tex_node}
+  alignment _ = #{alignment bgpu_tex_node}
+  peek p = TexNode <$> #{peek bgpu_tex_node, op} p <*> #{peek bgpu_tex_node, a} p
+    <*> #{peek bgpu_tex_node, b} p <*> #{peek bgpu_tex_node, c} p <*> #{peek bgpu_tex_node, map} p
+    <*> peekArray 3 (#{ptr bgpu_tex_node, pad} p)
+    <*> peekFloats 4 (#{ptr bgpu_tex_node, q} p) <*> peekFloats 16 (#{ptr bgpu_tex_node, m} p)
+    <*> peekFloats 16 (#{ptr bgpu_tex_node, spec} p)
+  poke p n = do
+    #{poke bgpu_tex_node, op} p (tnOp n)
+    #{poke bgpu_tex_node, a} p (tnA n)
+    #{poke bgpu_tex_node, b} p (tnB n)
+    #{poke bgpu_tex_node, c} p (tnC n)
+    #{poke bgpu_tex_node, map} p (tnMap n)
+    pokeArray (#{ptr bgpu_tex_node, pad} p) (take 3 (tnPad n ++ repeat 0))
+    pokeFloats 4 (#{ptr bgpu_tex_node, q} p) (tnQ n)
+    pokeFloats 16 (#{ptr bgpu_tex_node, m} p) (tnM n)
+    pokeFloats 16 (#{ptr bgpu_tex_node, spec} p) (tnSpec n)
+
+-- | bgpu_image: the texels of an image texture (16 bands per RGB pixel, or
+-- one grey value), row-major from the top row.
+data Image = Image { imWidth, imHeight, imChannels, imPad :: !Int32, imTexels :: Ptr CFloat }
+
+instance Storable Image where
+  sizeOf _ = #{size bgpu_image}
+  alignment _ = #{alignment bgpu_image}
+  peek p = Image <$> #{peek bgpu_image, width} p <*> #{peek bgpu_image, height} p
+    <*> #{peek bgpu_image, channels} p <*> #{peek bgpu_image, pad} p <*> #{peek bgpu_image, texels} p
+  poke p i = do
+    #{poke bgpu_image, width} p (imWidth i)
+    #{poke bgpu_image, height} p (imHeight i)
+    #{poke bgpu_image, channels} p (imChannels i)
+    #{poke bgpu_image, pad} p (imPad i)
+    #{poke bgpu_image, texels} p (imTexels i)
+
 -- | bgpu_material: Material.hs:32-127 with constant or selector textures.
 data Material = Material
   { mtType :: !Int32, mtPad :: !Int32, mtScalar :: [Float], mtSpec :: [[Float]], mtTex :: Texture }
@@ -244,7 +285,9 @@ data SceneDesc = SceneDesc
   , sdNumShapes :: !Word32, sdShapes :: Ptr Shape
   , sdNumMaterials :: !Word32, sdMaterials :: Ptr Material
   , sdNumLights :: !Word32, sdLights :: Ptr Light
-  , sdCamera :: Camera, sdFilm :: Film }
+  , sdCamera :: Camera, sdFilm :: Film
+  , sdNumTexNodes :: !Word32, sdTexNodes :: Ptr TexNode
+  , sdNumImages :: !Word32, sdImages :: Ptr Image }
 
 instance Storable SceneDesc where
   sizeOf _ = #{size bgpu_scene_desc}
@@ -258,6 +301,8 @@ instance Storable SceneDesc where
     <*> #{peek bgpu_scene_desc, num_materials} p <*> #{peek bgpu_scene_desc, materials} p
     <*> #{peek bgpu_scene_desc, num_lights} p <*> #{peek bgpu_scene_desc, lights} p
     <*> #{peek bgpu_scene_desc, camera} p <*> #{peek bgpu_scene_desc, film} p
+    <*> #{peek bgpu_scene_desc, num_tex_nodes} p <*> #{peek bgpu_scene_desc, tex_nodes} p
+    <*> #{peek bgpu_scene_desc, num_images} p <*> #{peek bgpu_scene_desc, images} p
   poke p d = do
     #{poke bgpu_scene_desc, num_vertices} p (sdNumVertices d)
     #{poke bgpu_scene_desc, num_triangles} p (sdNumTriangles d)
@@ -276,6 +321,10 @@ instance Storable SceneDesc where
     #{poke bgpu_scene_desc, lights} p (sdLights d)
     #{poke bgpu_scene_desc, camera} p (sdCamera d)
     #{poke bgpu_scene_desc, film} p (sdFilm d)
+    #{poke bgpu_scene_desc, num_tex_nodes} p (sdNumTexNodes d)
+    #{poke bgpu_scene_desc, tex_nodes} p (sdTexNodes d)
+    #{poke bgpu_scene_desc, num_images} p (sdNumImages d)
+    #{poke bgpu_scene_desc, images} p (sdImages d)
 
 -- | bgpu_rays: a structure-of-arrays ray batch (Math.hs:375-380).
 data Rays = Rays

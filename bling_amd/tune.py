@@ -58,13 +58,14 @@ def tune_launches(ctx, launches, space, repeats=3):
 
 
 def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(1, 0), stacks=(8, 12, 16),
-               repeats=3, coherent=False):
+               walks=(0, 16), repeats=3, coherent=False):
     """Times closest + any hit over `rays` ((n, 8) float32, host) for every
-    (bvh.width, trace.persistent, trace.block, trace.stack); returns (best,
-    table).  A scene is built per width (the width is fixed at scene
-    creation)."""
+    (bvh.width, trace.persistent, trace.block, trace.stack, trace.walk);
+    returns (best, table).  A scene is built per width (the width is fixed at
+    scene creation)."""
     table = []
-    keep = {k: ctx.config_get(k) for k in ("bvh.width", "trace.persistent", "trace.block", "trace.stack")}
+    keys = ("bvh.width", "trace.persistent", "trace.block", "trace.stack", "trace.walk")
+    keep = {k: ctx.config_get(k) for k in keys}
     flags = abi.TIMED | (abi.COHERENT if coherent else 0)
     try:
         for w in widths:
@@ -73,17 +74,18 @@ def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(
             try:
                 built = gs.bvh_info()["width"]
                 dev = _HostRays(ctx, rays)
-                for mode, block, stack in itertools.product(modes, blocks, stacks):
+                for mode, block, stack, walk in itertools.product(modes, blocks, stacks, walks):
                     ctx.config_set("trace.persistent", mode)
                     ctx.config_set("trace.block", block)
                     ctx.config_set("trace.stack", stack)
+                    ctx.config_set("trace.walk", walk)
                     dev.closest(gs, flags)  # warm-up (and the grid-size query)
                     dev.any(gs, flags)
                     ms = []
                     for _ in range(repeats):
                         ms.append(dev.closest(gs, flags) + dev.any(gs, flags))
                     table.append({"bvh.width": built, "trace.persistent": mode, "trace.block": block,
-                                  "trace.stack": stack, "ms": float(np.median(ms))})
+                                  "trace.stack": stack, "trace.walk": walk, "ms": float(np.median(ms))})
                 dev.close()
             finally:
                 gs.close()
@@ -91,22 +93,22 @@ def tune_trace(ctx, flat, rays, widths=(2, 4, 8), blocks=(64, 128, 256), modes=(
         for k, v in keep.items():
             ctx.config_set(k, v)
     best = _best(table)
-    for k in ("bvh.width", "trace.persistent", "trace.block", "trace.stack"):
+    for k in keys:
         ctx.config_set(k, best[k])
     return best, table
 
 
 def tune_render(ctx, gscene, spec, blocks=(64, 128, 256), shade_waves=(2, 0), diffuse_waves=(2,),
-                stacks=(12,), pools=(1 << 19, 1 << 20, 1 << 21), sorts=(2,), repeats=2):
+                stacks=(12,), walks=(8,), pools=(1 << 19, 1 << 20, 1 << 21), sorts=(2,), repeats=2):
     """Times the plain render of `spec` on `gscene` for every (pt.block,
-    pt.shade_waves, pt.diffuse_waves, pt.stack, pt.pool, pt.sort); returns
-    (best, table)."""
+    pt.shade_waves, pt.diffuse_waves, pt.stack, pt.walk, pt.pool, pt.sort);
+    returns (best, table)."""
     from .runtime import GpuRenderer
     table = []
-    keys = ("pt.block", "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.pool", "pt.sort")
+    keys = ("pt.block", "pt.shade_waves", "pt.diffuse_waves", "pt.stack", "pt.walk", "pt.pool", "pt.sort")
     keep = {k: ctx.config_get(k) for k in keys}
     try:
-        for combo in itertools.product(blocks, shade_waves, diffuse_waves, stacks, pools, sorts):
+        for combo in itertools.product(blocks, shade_waves, diffuse_waves, stacks, walks, pools, sorts):
             for k, v in zip(keys, combo):
                 ctx.config_set(k, v)
             GpuRenderer(spec).render_plain(gscene)  # engine build + warm-up

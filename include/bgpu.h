@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BGPU_ABI_VERSION 1
+#define BGPU_ABI_VERSION 2
 #define BGPU_NUM_BANDS 16 /* Spectrum.hs:34-36 */
 
 /* ---- error codes ------------------------------------------------------ */
@@ -105,16 +105,67 @@ enum {
  *               integer), tex1 elsewhere.  params: lw, then the mapping:
  *               UV      su sv ou ov            (Texture.hs:166-170)
  *               PLANAR  vu.xyz vv.xyz ou ov    (Texture.hs:172-179) */
-enum { BGPU_TEX_NONE = 0, BGPU_TEX_CHECKER = 1, BGPU_TEX_GRAPHPAPER = 2 };
-enum { BGPU_MAP_UV = 0, BGPU_MAP_PLANAR = 1 };
+enum { BGPU_TEX_NONE = 0, BGPU_TEX_CHECKER = 1, BGPU_TEX_GRAPHPAPER = 2, BGPU_TEX_NODES = 3 };
+enum { BGPU_MAP_UV = 0, BGPU_MAP_PLANAR = 1, BGPU_MAP_IDENTITY = 2 };
+/* kind BGPU_TEX_NODES: any of the material's inputs is a texture of the
+ * scene's node list (bgpu_tex_node): bit k of `slots` (k < 3) makes spec[k]
+ * the spectrum texture rooted at roots[k], bit 4 + k (k < 4) makes scalar[k]
+ * the scalar texture rooted at roots[4 + k]; the others stay constant. */
 typedef struct {
   int32_t kind;
   int32_t mapping;
   int32_t slots;
   int32_t pad;
-  float params[12];
+  union {
+    float params[12];
+    int32_t roots[12];
+  };
   float alt[2][16];
 } bgpu_texture;
+
+/* A texture as a node list (Texture.hs:59-414, the textures the material
+ * parser builds, MaterialParser.hs:106-226): every node's children come
+ * before it.  Scalar textures:
+ *   SCONST     constant q[0]
+ *   SIMAGE     readImageScalarMap: grey image `a` (channels 1) over the 2d mapping
+ *   CELL       cellNoise: distance a (0 euclidian, 1 squared euclidian,
+ *              2 manhattan, 3 chebyshev) over the 3d mapping
+ *   FBM        texMap3dToTexture (fbm octaves=a omega=q[0]) over the 3d mapping
+ *   PERLIN     noiseTexture over the 3d mapping
+ *   CRYSTAL    quasiCrystal octaves=a over the 2d mapping
+ *   SCALE      scaleTexture q[0] q[1] (child a): q[0] + q[1] * a
+ * Spectrum textures:
+ *   CONST      constant spec
+ *   IMAGE      readImageTextureMap: image `a` (channels 16) over the 2d mapping
+ *   BLEND      spectrumBlend tex1=a tex2=b f=c
+ *   CHECKER    checkerBoard scale=q[0..2] tex1=a tex2=b
+ *   GRADIENT   gradient (mkGradient steps) f=a: steps are the GSTEP nodes
+ *              b .. b+c-1 (position q[0], colour spec), ascending by position
+ *   GRAPHPAPER graphPaper lw=q[0] over the 2d mapping, tex1=a tex2=b
+ * Mappings (`map`, `m`): 2d UV m = su sv ou ov; 2d PLANAR m = vu.xyz vv.xyz
+ * ou ov; 3d IDENTITY m = the world-to-texture matrix, 4x4 row-major. */
+enum {
+  BGPU_TN_SCONST = 1, BGPU_TN_SIMAGE = 2, BGPU_TN_CELL = 3, BGPU_TN_FBM = 4, BGPU_TN_PERLIN = 5,
+  BGPU_TN_CRYSTAL = 6, BGPU_TN_SCALE = 7,
+  BGPU_TN_CONST = 16, BGPU_TN_IMAGE = 17, BGPU_TN_BLEND = 18, BGPU_TN_CHECKER = 19, BGPU_TN_GRADIENT = 20,
+  BGPU_TN_GRAPHPAPER = 21, BGPU_TN_GSTEP = 22
+};
+typedef struct {
+  int32_t op;
+  int32_t a, b, c;
+  int32_t map;
+  int32_t pad[3];
+  float q[4];
+  float m[16];
+  float spec[16];
+} bgpu_tex_node;
+/* An image of a texture: width x height texels of `channels` floats (16:
+ * pixelSpectrum of each RGB pixel, Texture.hs:84-86; 1: grey / 255),
+ * row-major from the top row (JuicyPixels' pixelAt). */
+typedef struct {
+  int32_t width, height, channels, pad;
+  const float* texels;
+} bgpu_image;
 
 typedef struct {
   int32_t type;
@@ -193,6 +244,10 @@ typedef struct {
   const bgpu_light* lights;
   bgpu_camera camera;
   bgpu_film film;
+  uint32_t num_tex_nodes;          /* texture node list of BGPU_TEX_NODES materials */
+  const bgpu_tex_node* tex_nodes;
+  uint32_t num_images;
+  const bgpu_image* images;
 } bgpu_scene_desc;
 
 /* ---- ray batches ------------------------------------------------------ */
@@ -354,14 +409,20 @@ int bgpu_profile_get(bgpu_ctx* ctx, bgpu_profile* out);
 /* Tuning knobs (no counterpart in the reference, whose CPU integrator has
  * none): "bvh.width" (2/4/8 children per BVH node of scenes created
  * afterwards; a scene whose worst-case traversal stack would not fit falls
- * back to a narrower width), "trace.block" (64/128/256, intersect kernels),
+ * back to a narrower width), "trace.block" (64/128/256, default 256, intersect kernels),
  * "bvh.device" (1, the default: scenes created afterwards with more than
  * 1024 triangles build their triangle tree on the GPU, the same tree as the
  * host builder's; 0: on the host), "trace.persistent" (1: intersect kernels keep lanes busy by taking a new
  * ray when theirs is done; 0: one ray per thread), "trace.stack" and
  * "pt.stack" (8..24, default 12: traversal stack entries per lane kept in
  * LDS by the intersect kernels and by the bounce loop; deeper entries go to
- * private memory), "pt.block"
+ * private memory), "trace.refill" (1..64, default 32: idle lanes that make
+ * a wave of the lane-refill kernels take new rays), "trace.walk" (0: a wave
+ * walks inner nodes until every lane is at a leaf; n, default 16: until
+ * fewer than n lanes still walk while another waits at a leaf; both
+ * traversal kernels), "pt.walk" (the same for the bounce loop's
+ * traversals, default 8),
+ * "pt.block"
  * (64/128/256, bounce kernels), "pt.persistent" (1: the bounce loop's
  * traversal runs as persistent passes that refill lanes with new rays;
  * 0, the default: one ray per thread),

@@ -39,7 +39,7 @@ STRUCTS = {
     "bgpu_light": abi.Light, "bgpu_camera": abi.Camera, "bgpu_film": abi.Film,
     "bgpu_scene_desc": abi.SceneDesc, "bgpu_rays": abi.Rays, "bgpu_hits": abi.Hits,
     "bgpu_job_desc": abi.JobDesc, "bgpu_stats": abi.Stats, "bgpu_bvh_info": abi.BvhInfo,
-    "bgpu_profile": abi.Profile,
+    "bgpu_profile": abi.Profile, "bgpu_tex_node": abi.TexNode, "bgpu_image": abi.Image,
 }
 
 
@@ -65,7 +65,7 @@ def test_ctypes_layout_matches_c():
 
 def test_null_arguments_fail_with_invalid():
     lib = abi.load()
-    assert lib.bgpu_version() == 1
+    assert lib.bgpu_version() == 2
     h = C.c_void_p()
     assert lib.bgpu_scene_create(None, None, C.byref(h)) == abi.E_INVALID
     assert b"NULL" in lib.bgpu_last_error()

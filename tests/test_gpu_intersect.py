@@ -216,9 +216,10 @@ def test_stack_depth_same_results(gpu_ctx):
     """trace.stack only moves the boundary between the LDS and the private
     part of the traversal stack: every depth gives the default's results bit
     for bit, on a binary tree deep enough that depth 8 spills, through both
-    kernels (lane refill and one ray per thread)."""
+    kernels (lane refill and one ray per thread).  trace.refill and
+    trace.walk only schedule the traversal: same results too."""
     import ctypes as C
-    keys = ("bvh.width", "trace.persistent", "trace.stack")
+    keys = ("bvh.width", "trace.persistent", "trace.stack", "trace.refill", "trace.walk")
     keep = {k: gpu_ctx.config_get(k) for k in keys}
     try:
         gpu_ctx.config_set("bvh.width", 2)
@@ -239,11 +240,24 @@ def test_stack_depth_same_results(gpu_ctx):
                     for a, b in zip(base, got):
                         assert np.array_equal(a.view(np.int32), b.view(np.int32)), (persistent, depth)
                     assert np.array_equal(occ, gs.intersects(rays)), (persistent, depth)
+            # the scheduling knobs: same results (refill: lane-refill kernel
+            # only; walk: both kernels)
+            gpu_ctx.config_set("trace.stack", 12)
+            for persistent, refill, walk in ((1, 1, 0), (1, 64, 0), (1, 32, 8), (1, 16, 32), (1, 32, 64), (1, 1, 1),
+                                             (0, 32, 8), (0, 32, 16), (0, 32, 64), (0, 32, 1)):
+                gpu_ctx.config_set("trace.persistent", persistent)
+                gpu_ctx.config_set("trace.refill", refill)
+                gpu_ctx.config_set("trace.walk", walk)
+                got = gs.intersect(rays)
+                for a, b in zip(base, got):
+                    assert np.array_equal(a.view(np.int32), b.view(np.int32)), (persistent, refill, walk)
+                assert np.array_equal(occ, gs.intersects(rays)), (persistent, refill, walk)
         finally:
             gs.close()
-        for bad in (7, 25):
+        for key, bad in (("trace.stack", 7), ("trace.stack", 25), ("trace.refill", 0), ("trace.refill", 65),
+                         ("trace.walk", -1), ("trace.walk", 65)):
             with pytest.raises(abi.BgpuError) as e:
-                gpu_ctx.config_set("trace.stack", bad)
+                gpu_ctx.config_set(key, bad)
             assert e.value.code == abi.E_INVALID
     finally:
         for k, v in keep.items():
@@ -280,14 +294,14 @@ def test_autotuner_picks_a_valid_config(gpu_ctx):
     """The tuner times every candidate, leaves the fastest set on the
     context, and the tuned context still answers bit-exactly."""
     from bling_amd import tune
-    keys = ("bvh.width", "trace.persistent", "trace.block", "trace.stack")
+    keys = ("bvh.width", "trace.persistent", "trace.block", "trace.stack", "trace.walk")
     keep = {k: gpu_ctx.config_get(k) for k in keys}
     try:
         flat = scenes.small_scene(num_tris=3000, num_spheres=300, seed=17)
         rays = scenes.incoherent_rays(*scenes.scene_bounds(flat), 20000, seed=7)
         best, table = tune.tune_trace(gpu_ctx, flat, rays, widths=(2, 4), blocks=(64, 128), modes=(1, 0),
-                                      stacks=(8, 16), repeats=1)
-        assert len(table) == 16 and all(row["ms"] > 0 for row in table)
+                                      stacks=(8, 16), walks=(0, 16), repeats=1)
+        assert len(table) == 32 and all(row["ms"] > 0 for row in table)
         assert best == min(table, key=lambda r: r["ms"])
         for k in keys:
             assert gpu_ctx.config_get(k) == best[k]

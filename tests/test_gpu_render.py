@@ -215,11 +215,12 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=4)
     base, _ = render_gpu(gpu_ctx, flat, spec)
     defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves", "pt.persistent",
-                                                   "pt.sort", "pt.diffuse_waves", "pt.stack")}
+                                                   "pt.sort", "pt.diffuse_waves", "pt.stack", "pt.walk")}
     try:
         for key, val in (("pt.shade_waves", 0), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096),
                          ("pt.persistent", 0), ("pt.persistent", 1), ("pt.sort", 0), ("pt.sort", 1),
-                         ("pt.diffuse_waves", 0), ("pt.diffuse_waves", 4), ("pt.stack", 8), ("pt.stack", 24)):
+                         ("pt.diffuse_waves", 0), ("pt.diffuse_waves", 4), ("pt.stack", 8), ("pt.stack", 24),
+                         ("pt.walk", 16), ("pt.walk", 1)):
             gpu_ctx.config_set(key, val)
             assert gpu_ctx.config_get(key) == val
             img, _ = render_gpu(gpu_ctx, flat, spec)

@@ -22,7 +22,7 @@ STRUCTS = {
     "bgpu_mesh": abi.Mesh, "bgpu_light": abi.Light, "bgpu_camera": abi.Camera, "bgpu_film": abi.Film,
     "bgpu_scene_desc": abi.SceneDesc, "bgpu_rays": abi.Rays, "bgpu_hits": abi.Hits,
     "bgpu_job_desc": abi.JobDesc, "bgpu_stats": abi.Stats, "bgpu_bvh_info": abi.BvhInfo,
-    "bgpu_profile": abi.Profile,
+    "bgpu_profile": abi.Profile, "bgpu_tex_node": abi.TexNode, "bgpu_image": abi.Image,
 }
 
 
