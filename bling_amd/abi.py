@@ -149,7 +149,7 @@ EXPORTS = [
     "bgpu_job_cancel", "bgpu_job_release", "bgpu_job_latency_ms", "bgpu_stats_get",
     "bgpu_stats_reset", "bgpu_config_set", "bgpu_config_get", "bgpu_scene_bvh_info",
     "bgpu_scene_write", "bgpu_scene_read", "bgpu_scene_desc_free", "bgpu_scene_load",
-    "bgpu_mem_info", "bgpu_profile_get", "bgpu_intersect_closest_noalloc",
+    "bgpu_mem_info", "bgpu_profile_get", "bgpu_intersect_closest_noalloc", "bgpu_scene_texture_eval",
 ]
 
 _lib = None
@@ -180,6 +180,7 @@ def load(path=LIB_PATH):
     lib.bgpu_intersect_closest.argtypes = [vp, C.POINTER(Rays), C.c_size_t, C.POINTER(Hits), C.c_int]
     lib.bgpu_intersect_any.argtypes = [vp, C.POINTER(Rays), C.c_size_t, vp, C.c_int]
     lib.bgpu_intersect_closest_noalloc.argtypes = [vp, C.POINTER(Rays), C.c_size_t, C.POINTER(Hits), C.c_int]
+    lib.bgpu_scene_texture_eval.argtypes = [vp, C.c_int, C.c_size_t, vp, vp, vp]
     lib.bgpu_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_double)]
     lib.bgpu_render.argtypes = [vp, C.POINTER(JobDesc), vp, vp]
     lib.bgpu_job_submit.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(vp)]

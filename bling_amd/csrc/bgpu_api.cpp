@@ -161,6 +161,34 @@ int bgpu_scene_bvh_info(const bgpu_scene* scene, bgpu_bvh_info* out) {
   return BGPU_OK;
 }
 
+int bgpu_scene_texture_eval(bgpu_scene* scene, int node, size_t n, const float* xyz, const float* uv, float* out) {
+  if (!scene || (n && (!xyz || !uv || !out))) return fail(BGPU_E_INVALID, "bgpu_scene_texture_eval: NULL argument");
+  const PtScene& pt = scene->pt;
+  if (node < 0 || (size_t)node >= pt.tex_op_host.size())
+    return fail(BGPU_E_INVALID, "bgpu_scene_texture_eval: no such texture node");
+  const int op = pt.tex_op_host[node];
+  const bool spectrum = op >= BGPU_TN_CONST && op <= BGPU_TN_GRAPHPAPER;
+  if (!spectrum && !(op >= BGPU_TN_SCONST && op <= BGPU_TN_SCALE))
+    return fail(BGPU_E_INVALID, "bgpu_scene_texture_eval: the node is not a texture (a gradient step)");
+  const int lo = pt.tex_lo_host[node];
+  if (node - lo + 1 > 64) return fail(BGPU_E_INVALID, "bgpu_scene_texture_eval: texture of more than 64 nodes");
+  if (n > (size_t)0x7fffffff) return fail(BGPU_E_INVALID, "bgpu_scene_texture_eval: too many points");
+  if (n == 0) return BGPU_OK;
+  Ctx* c = scene->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  BGPU_HIP(hipSetDevice(c->device));
+  const size_t nout = n * (spectrum ? 16 : 1);
+  DevBuf dx, du, dout;
+  if (int rc = dx.upload(c, xyz, 12 * n, c->stream)) return rc;
+  if (int rc = du.upload(c, uv, 8 * n, c->stream)) return rc;
+  if (int rc = dout.alloc(c, 4 * nout)) return rc;
+  BGPU_HIP(pt_texture_eval(pt.host, lo, node, spectrum, (int)n, dx.as<float>(), du.as<float>(), dout.as<float>(),
+                           c->stream));
+  BGPU_HIP(hipMemcpyAsync(out, dout.ptr, 4 * nout, hipMemcpyDeviceToHost, c->stream));
+  BGPU_HIP(hipStreamSynchronize(c->stream));
+  return BGPU_OK;
+}
+
 int bgpu_scene_bytes(const bgpu_scene* scene, size_t* bytes) {
   if (!scene || !bytes) return fail(BGPU_E_INVALID, "bgpu_scene_bytes: NULL argument");
   *bytes = scene->device_bytes();

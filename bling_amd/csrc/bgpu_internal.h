@@ -93,7 +93,7 @@ struct Scene {
   // shading data for the path tracer
   PtScene pt{};
   DevBuf tri_idx, verts, normals, uvs, tri_mat, tri_flags, shape_x, materials, lights,
-      light_maps, light_dist, filter_tbl;
+      light_maps, light_dist, filter_tbl, tex_nodes, tex_lo, images, texels, mat_src;
   bgpu_film film{};
   bgpu_camera camera{};
   std::atomic<int> refs{1};  // the user's handle plus engines referencing it

@@ -96,6 +96,7 @@ struct Engine {
   PtConfig cfg;
   int width = 0;  // BVH width the captured traversal kernels assume (0: any)
   DevBuf pool_buf, jobs_buf, ring_buf, scenes_buf, cnt_buf, admit_buf, prog_buf;
+  DevBuf pmat_buf;  // per-slot BxDF records of node-textured hits (allocated on first need)
   PtKernelArgs args{};
   AdmitReq* h_admit = nullptr;
   PtCounters* h_cnt = nullptr;
@@ -141,6 +142,7 @@ struct Engine {
     cnt_buf.release();
     admit_buf.release();
     prog_buf.release();
+    pmat_buf.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -221,11 +223,14 @@ struct Engine {
 
   // Captures one bounce iteration whose traversal kernels are specialised
   // for BVH width w (0: any width), shading in class-grouped order if `sort`
-  // with the variants of `shade_mode` (SHADE_MODE_SPLIT needs `sort`).
-  int capture(int w, bool sort, bool direct, int shade_mode) {
+  // with the variants of `shade_mode` (SHADE_MODE_SPLIT needs `sort`), with
+  // the texture kernel if `textured` (pmat_buf allocated).
+  int capture(int w, bool sort, bool direct, int shade_mode, bool textured = false) {
     args.sort_shade = sort ? 1 : 0;
     args.direct = direct ? 1 : 0;
     args.shade_mode = shade_mode;
+    args.textured = textured ? 1 : 0;
+    args.pool.pmat = pmat_buf.ptr ? pmat_buf.as<DMaterial>() : nullptr;
     if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
     if (graph) BGPU_HIP(hipGraphDestroy(graph));
     gexec = nullptr;
@@ -338,10 +343,19 @@ struct Engine {
     bool want_direct = j->desc.integrator == BGPU_INTEGRATOR_DIRECT;
     for (HostJob* o : slots)
       if (o && o->desc.integrator == BGPU_INTEGRATOR_DIRECT) want_direct = true;
-    if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0) ||
-        want_mode != args.shade_mode) {
+    // the texture kernel runs while a resident scene has node textures; its
+    // per-slot records are allocated on the first such job
+    bool want_tex = j->scene->pt.node_tex;
+    for (Scene* s : scenes)
+      if (s && s->pt.node_tex) want_tex = true;
+    if (want_tex && !pmat_buf.ptr) {
       BGPU_HIP(hipStreamSynchronize(stream));
-      if (int rc = capture(want_w, want_sort, want_direct, want_mode)) return rc;
+      if (int rc = pmat_buf.alloc(ctx, sizeof(DMaterial) * (size_t)P)) return rc;
+    }
+    if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0) ||
+        want_mode != args.shade_mode || want_tex != (args.textured != 0)) {
+      BGPU_HIP(hipStreamSynchronize(stream));
+      if (int rc = capture(want_w, want_sort, want_direct, want_mode, want_tex)) return rc;
     }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;

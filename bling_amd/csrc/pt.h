@@ -58,6 +58,25 @@ struct DevPtScene {
   const float* maps;
   const float* dist;
   int32_t ntri, nlights, nmats, pad;
+  // node textures (BGPU_TEX_NODES materials): the node list, each node's
+  // range start (the lowest node of its subtree), the images as (width,
+  // height, channels, offset into texels), and the inputs of each such
+  // material (indexed by its DMaterial::alt)
+  const bgpu_tex_node* tex_nodes;
+  const int32_t* tex_lo;
+  const int4* images;
+  const float* texels;
+  const struct DMatSrc* mat_src;
+};
+
+// The inputs of a material whose inputs are textures: its constant inputs,
+// which slots are textured (bgpu_texture::slots) and their roots and range
+// starts.
+struct DMatSrc {
+  int32_t type, slots;
+  int32_t roots[8], lo[8];
+  float scalar[4];
+  float spec[3][16];
 };
 
 // Camera and film of a job (ProjectiveCamera, Camera.hs:24-31; the film's
@@ -154,6 +173,9 @@ struct PathPool {
   uint32_t* q_mis;
   uint32_t* q_sh;   // q_act grouped by shading class (when sorting)
   uint32_t* q_cls;  // shading class of q_act[q]
+  // per slot, the BxDF records of a hit on a material whose inputs are
+  // textures, evaluated at the hit (allocated while such a scene is resident)
+  DMaterial* pmat;
 };
 
 struct PtConfig {
@@ -186,6 +208,7 @@ struct PtKernelArgs {
   int32_t shade_mode;  // SHADE_MODE_*: which shading variants the iteration runs
   int32_t nshort;      // traversal stack entries per lane in LDS (PtConfig::stack)
   int32_t walk;        // traversal walk threshold (PtConfig::walk)
+  int32_t textured;    // a resident scene has node-textured materials: run the texture kernel
 };
 
 // Shading variants of an iteration: one kernel for every BxDF kind; the
@@ -200,6 +223,9 @@ struct PtScene {
   DevPtScene host{};                      // filled by build_pt_data, pointers are device
   int shade_kinds = 0;                    // distinct BxDF makeups of the materials in use
   bool diffuse_only = false;              // every material in use has Lambert / Oren-Nayar components only
+  bool node_tex = false;                  // a material's inputs are textures of the node list
+  std::vector<int32_t> tex_lo_host;       // per texture node, the lowest node of its subtree
+  std::vector<int32_t> tex_op_host;       // per texture node, its operation
 };
 
 // Fills a job view from a camera and film description.
@@ -214,6 +240,11 @@ hipError_t pt_enqueue_iteration(const PtKernelArgs& a, const PtConfig& cfg, int 
                                 hipEvent_t* marks = nullptr);
 // Enqueues the admission of jobs listed in `admit` (count n) on `s`.
 hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s);
+// Evaluates the node texture [lo, root] of scene `sc` at n points (xyz, and
+// the surface parameters uv): 16 bands per point for a spectrum texture,
+// one value for a scalar one.  Device pointers.
+hipError_t pt_texture_eval(const DevPtScene& sc, int lo, int root, bool spectrum, int n, const float* xyz,
+                           const float* uv, float* out, hipStream_t s);
 hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s);
 
 }  // namespace bgpu

@@ -15,6 +15,8 @@
 // Kernel grids cover the pool; queue lengths are read on the device, so the
 // sequence replays from a captured hipGraph without host round trips.
 #include "pt_device.h"
+#include "pt_material.h"
+#include "pt_texture.h"
 #include "trav_persist.h"
 
 namespace bgpu {
@@ -533,6 +535,54 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
   }
 }
 
+// Node textures (Texture.hs:59-414): for each active path whose hit's
+// material takes inputs from the scene's texture node list, evaluates them
+// at the shading DG (`tex dgs`) and writes the material's BxDF records, built
+// from them as from constants (Material.hs:32-127), to the path's pmat
+// entry, which make_bsdf takes for that material.  Runs only while such a
+// scene is resident.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_texture(PtKernelArgs a) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  const unsigned int n = a.cnt->n_act;
+  if ((unsigned)(blockIdx.x * BLOCK) >= n) return;
+  const PathPool& P = a.pool;
+  const int Pn = P.P;
+  const bool valid = (unsigned)q < n;
+  const int i = valid ? (int)P.q_act[q] : 0;
+  const int prim = P.hprim[i];
+  const bool hit = valid && prim >= 0;
+  for_uniform_scene(hit, hit ? a.jobs[P.job[i]].scene : 0, [&](int scene_u) {
+    const DevPtScene& S = a.scenes[scene_u];
+    if (!S.mat_src) return;  // the scene has no node textures
+    const int mat = prim < S.ntri ? S.tri_info[prim].x : S.geo.shapes[prim - S.ntri].material;
+    if (mat < 0 || mat >= S.nmats || S.mats[mat].tex_kind != BGPU_TEX_NODES) return;
+    const DMatSrc& src = S.mat_src[S.mats[mat].alt];
+    const DRay ray = load_pool_ray(P.ray, Pn, i, INFINITY);
+    DG dgg, dgs;
+    float eps;
+    int m2, light;
+    hit_dg(S, ray, P.hit[i], P.hit[Pn + i], P.hit[2 * Pn + i], prim, &dgg, &dgs, &eps, &m2, &light);
+    const tx::Tex T{S.tex_nodes, S.images, S.texels};
+    const tx::TexPt t{dgs.p, dgs.u, dgs.v};
+    MatIn in;
+    in.type = src.type;
+    for (int k = 0; k < 4; ++k)
+      in.scalar[k] = ((src.slots >> (4 + k)) & 1) ? tx::eval_scalar(T, src.lo[4 + k], src.roots[4 + k], t)
+                                                  : src.scalar[k];
+    for (int k = 0; k < 3; ++k) {
+      if ((src.slots >> k) & 1) {
+        tx::eval_spectrum(T, src.lo[k], src.roots[k], t, in.spec[k]);
+      } else {
+        for (int b = 0; b < 16; ++b) in.spec[k][b] = src.spec[k][b];
+      }
+    }
+    DMaterial dm;
+    build_material(in, dm);
+    P.pmat[i] = dm;
+  });
+}
+
 // Queue ranges of the shading variants: the whole queue, or (grouped queue)
 // its diffuse classes [0, n_diff) and the rest [n_diff, n_act).
 enum { SHADE_ALL = 0, SHADE_DIFFUSE_PART = 1, SHADE_OTHER_PART = 2 };
@@ -608,7 +658,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
         float eps;
         int mat, light;
         hit_dg(S, ray, th, hb1, hb2, prim, &dgg, &dgs, &eps, &mat, &light);
-        Bsdf bsdf = make_bsdf(S, mat, dgg, dgs);
+        Bsdf bsdf = make_bsdf(S, mat, dgg, dgs, P.pmat ? P.pmat + i : nullptr);
         uint32_t d = (uint32_t)depth, un = (uint32_t)n;
         float lNumU = sm.get1(un, 1 + 4 * d);
         float ld1, ld2, bd1, bd2;
@@ -723,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
       float eps;
       int mat, light;
       hit_dg(S, ray, th, hb1, hb2, prim, &dgg, &dgs, &eps, &mat, &light);
-      Bsdf bsdf = make_bsdf(S, mat, dgg, dgs);
+      Bsdf bsdf = make_bsdf(S, mat, dgg, dgs, P.pmat ? P.pmat + i : nullptr);
       const uint32_t k = 2u * (uint32_t)depth, un = (uint32_t)n;
       const float uln = sm.get1(un, k), ubc = sm.get1(un, k + 1);
       float uld1, uld2, ubd1, ubd2;
@@ -1199,6 +1249,32 @@ hipError_t pt_enqueue_init(const PtKernelArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+template <bool SPECTRUM>
+__global__ void k_tex_eval(DevPtScene sc, int lo, int root, int n, const float* xyz, const float* uv, float* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const tx::Tex T{sc.tex_nodes, sc.images, sc.texels};
+  const tx::TexPt t{mk3(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]), uv[2 * k], uv[2 * k + 1]};
+  if (SPECTRUM) {
+    float v[16];
+    tx::eval_spectrum(T, lo, root, t, v);
+    for (int b = 0; b < 16; ++b) out[16 * k + b] = v[b];
+  } else {
+    out[k] = tx::eval_scalar(T, lo, root, t);
+  }
+}
+}  // namespace
+
+hipError_t pt_texture_eval(const DevPtScene& sc, int lo, int root, bool spectrum, int n, const float* xyz,
+                           const float* uv, float* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g((n + 255) / 256), b(256);
+  if (spectrum) hipLaunchKernelGGL(k_tex_eval<true>, g, b, 0, s, sc, lo, root, n, xyz, uv, out);
+  else hipLaunchKernelGGL(k_tex_eval<false>, g, b, 0, s, sc, lo, root, n, xyz, uv, out);
+  return hipGetLastError();
+}
+
 hipError_t pt_enqueue_admit(const PtKernelArgs& a, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, s, a, n);
   return hipGetLastError();
@@ -1272,6 +1348,7 @@ static hipError_t enqueue_iter(const PtKernelArgs& a, const PtConfig& cfg, bool 
       hipLaunchKernelGGL((k_extend_p<BLOCK, W, false>), dim3(e), b, lds, s, a);
       hipLaunchKernelGGL((k_extend_p<BLOCK, W, true>), dim3(e), b, lds, s, a);
       mark(marks, PH_EXTEND + 1, s);
+      if (a.textured) hipLaunchKernelGGL((k_texture<BLOCK>), g, b, 0, s, a);
       launch_shade<BLOCK>(a, cfg, g, b, s, marks);
       hipLaunchKernelGGL((k_shadow_p<BLOCK, W, false>), dim3(sh), b, lds, s, a);
       hipLaunchKernelGGL((k_shadow_p<BLOCK, W, true>), dim3(sh), b, lds, s, a);
@@ -1287,6 +1364,7 @@ static hipError_t enqueue_iter(const PtKernelArgs& a, const PtConfig& cfg, bool 
   }
   hipLaunchKernelGGL((k_extend<BLOCK, W>), g, b, lds, s, a);
   mark(marks, PH_EXTEND + 1, s);
+  if (a.textured) hipLaunchKernelGGL((k_texture<BLOCK>), g, b, 0, s, a);
   launch_shade<BLOCK>(a, cfg, g, b, s, marks);
   hipLaunchKernelGGL((k_shadow<BLOCK, W>), g, b, lds, s, a);
   mark(marks, PH_SHADOW + 1, s);

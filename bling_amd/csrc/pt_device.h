@@ -737,10 +737,14 @@ __device__ __forceinline__ bool tex_selects_alt(const DMaterial& m, const DG& dg
   return xf < lo || zf < lo || xf > hi || zf > hi;
 }
 
-__device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, const DG& dgg, const DG& dgs) {
+// `pm`: the path's BxDF records evaluated by the texture kernel, taken for a
+// material whose inputs are node textures.
+__device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, const DG& dgg, const DG& dgs,
+                                          const DMaterial* pm) {
   Bsdf b;
   b.m = (material >= 0 && material < S.nmats) ? &S.mats[material] : nullptr;
-  if (b.m && b.m->tex_kind != BGPU_TEX_NONE && tex_selects_alt(*b.m, dgs)) b.m = &S.mats[b.m->alt];
+  if (b.m && b.m->tex_kind == BGPU_TEX_NODES) b.m = pm;
+  else if (b.m && b.m->tex_kind != BGPU_TEX_NONE && tex_selects_alt(*b.m, dgs)) b.m = &S.mats[b.m->alt];
   f3 nn = dgs.n, sn = norm3(dgs.dpdu);
   b.cs = Frame{sn, cross3(nn, sn), nn};
   b.p = dgs.p;

@@ -10,6 +10,8 @@
 #include <cstring>
 
 #include "bgpu_internal.h"
+#include "pt_material.h"
+#include "pt_texture.h"
 #include "spectral_tables.h"
 
 namespace bgpu {
@@ -17,8 +19,6 @@ namespace bgpu {
 namespace {
 
 inline float hmaxf(float x, float y) { return (x <= y) ? y : x; }
-inline float hminf(float x, float y) { return (x <= y) ? x : y; }
-inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 float sY(const float* s) {  // Spectrum.hs:371-373
   float a = 0.0f;
@@ -96,12 +96,6 @@ const float* tex_px(const bgpu_light& l, float u, float v) {  // Texture.hs:94-9
   return l.radiance + 16 * ((size_t)py * l.width + px);
 }
 
-DBxdf bx_zero() {
-  DBxdf b;
-  std::memset(&b, 0, sizeof(b));
-  return b;
-}
-
 }  // namespace
 
 int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
@@ -135,142 +129,113 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
 
   // materials (Material.hs:32-96)
   std::vector<DMaterial> mats(std::max<uint32_t>(d->num_materials, 1));
-  auto blinn_e = [](float rough) {
-    float e = 1.0f / rough;
-    return (e > 10000.0f || std::isnan(e)) ? 10000.0f : e;
-  };
   // one material from its type and scalars, with spectra spec[0], spec[1]
+  // (the constant ones, or a selector texture's tex2 alternates)
   auto build = [&](const bgpu_material& m, const float (*spec)[16]) {
+    MatIn in;
+    in.type = m.type;
+    for (int k = 0; k < 4; ++k) in.scalar[k] = m.scalar[k];
+    for (int k = 0; k < 16; ++k) {
+      in.spec[0][k] = spec[0][k];
+      in.spec[1][k] = spec[1][k];
+      in.spec[2][k] = m.spec[2][k];
+    }
     DMaterial o;
     std::memset(&o, 0, sizeof(o));
-    switch (m.type) {
-      case BGPU_MAT_MATTE: {
-        DBxdf b = bx_zero();
-        std::memcpy(b.r, spec[0], 64);
-        float sig = m.scalar[0];
-        if (sig == 0.0f) {
-          b.kind = K_LAMBERT;
-        } else {
-          b.kind = K_OREN;
-          float s = clampf(sig, 0.0f, 1.0f);
-          float s2 = s * s;
-          b.A = 1.0f - (s2 / (2.0f * (s2 + 0.33f)));
-          b.B = 0.45f * s2 / (s2 + 0.09f);
-        }
-        b.type = BX_R | BX_D;
-        o.c[o.n++] = b;
-        break;
-      }
-      case BGPU_MAT_PLASTIC: {
-        DBxdf dd = bx_zero();
-        dd.kind = K_LAMBERT;
-        dd.type = BX_R | BX_D;
-        std::memcpy(dd.r, spec[0], 64);
-        DBxdf g = bx_zero();
-        g.kind = K_MF;
-        g.type = BX_R | BX_G;
-        std::memcpy(g.r, spec[1], 64);
-        g.e = blinn_e(m.scalar[0]);
-        g.fr = FR_DIEL;
-        g.ei = 1.0f;
-        g.et = 1.5f;
-        o.c[o.n++] = dd;
-        o.c[o.n++] = g;
-        break;
-      }
-      case BGPU_MAT_METAL: {
-        DBxdf g = bx_zero();
-        g.kind = K_MF;
-        g.type = BX_R | BX_G;
-        for (int k = 0; k < 16; ++k) g.r[k] = 1.0f;
-        g.e = blinn_e(m.scalar[0]);
-        g.fr = FR_COND;
-        std::memcpy(g.eta, spec[0], 64);
-        std::memcpy(g.k, spec[1], 64);
-        o.c[o.n++] = g;
-        break;
-      }
-      case BGPU_MAT_GLASS: {
-        DBxdf r = bx_zero();
-        r.kind = K_SPECR;
-        r.type = BX_R | BX_S;
-        r.fr = FR_DIEL;
-        r.ei = 1.0f;
-        r.et = m.scalar[0];
-        DBxdf t = bx_zero();
-        t.kind = K_SPECT;
-        t.type = BX_T | BX_S;
-        t.ei = 1.0f;
-        t.et = m.scalar[0];
-        for (int k = 0; k < 16; ++k) {  // sClamp' (Spectrum.hs:459-461)
-          r.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
-          t.r[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k]));
-        }
-        o.c[o.n++] = r;
-        o.c[o.n++] = t;
-        break;
-      }
-      case BGPU_MAT_TRANSMATTE: {  // translucentMatte (Material.hs:43-53)
-        const float sig = m.scalar[0];
-        auto lobe = [&](const float* s, bool flip) {
-          DBxdf b = bx_zero();
-          std::memcpy(b.r, s, 64);
-          if (sig == 0.0f) {
-            b.kind = K_LAMBERT;
-          } else {
-            b.kind = K_OREN;
-            float c = clampf(sig, 0.0f, 1.0f);
-            float c2 = c * c;
-            b.A = 1.0f - (c2 / (2.0f * (c2 + 0.33f)));
-            b.B = 0.45f * c2 / (c2 + 0.09f);
-          }
-          b.type = flip ? (BX_T | BX_D) : (BX_R | BX_D);  // bxdfTypeFlip [Reflection, Transmission]
-          b.flip = flip ? 1 : 0;
-          return b;
-        };
-        float r[16], t[16];
-        for (int k = 0; k < 16; ++k) {
-          r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
-          t[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k])) * (1.0f - r[k]);
-        }
-        o.c[o.n++] = lobe(r, false);
-        o.c[o.n++] = lobe(t, true);
-        break;
-      }
-      case BGPU_MAT_SUBSTRATE: {  // mkSubstrate (Material.hs:111-127)
-        DBxdf b = bx_zero();
-        b.kind = K_FB;
-        b.type = BX_R | BX_G;
-        for (int k = 0; k < 16; ++k) {  // sClamp 0 1
-          b.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
-          b.eta[k] = hmaxf(0.0f, hminf(1.0f, spec[1][k]));
-          b.k[k] = hmaxf(0.0f, hminf(1.0f, m.spec[2][k]));
-        }
-        b.e = blinn_e(hmaxf(0.0f, m.scalar[0]));  // mkAnisotropic (1 / u) (1 / v), u = max 0 urough
-        b.ei = blinn_e(hmaxf(0.0f, m.scalar[1]));
-        b.et = m.scalar[2];
-        o.c[o.n++] = b;
-        break;
-      }
-      case BGPU_MAT_MIRROR: {
-        DBxdf r = bx_zero();
-        r.kind = K_SPECR;
-        r.type = BX_R | BX_S;
-        r.fr = FR_NOOP;
-        for (int k = 0; k < 16; ++k) r.r[k] = hmaxf(0.0f, hminf(1.0f, spec[0][k]));
-        o.c[o.n++] = r;
-        break;
-      }
-      default:
-        break;
-    }
+    build_material(in, o);
     return o;
   };
+  // the texture node list: every child precedes its node and has the kind
+  // its operation takes; lo[k] is the lowest node of k's subtree, and a
+  // texture is evaluated over [lo, root], at most tx::kMaxTexRange nodes
+  const uint32_t nn = d->num_tex_nodes;
+  if (nn && !d->tex_nodes) return fail(BGPU_E_INVALID, "tex_nodes is NULL");
+  if (d->num_images && !d->images) return fail(BGPU_E_INVALID, "images is NULL");
+  std::vector<int32_t> lo(nn);
+  std::vector<int4> images(d->num_images);
+  std::vector<float> texels;
+  for (uint32_t k = 0; k < d->num_images; ++k) {
+    const bgpu_image& im = d->images[k];
+    if (im.width <= 0 || im.height <= 0 || (im.channels != 1 && im.channels != 16) || !im.texels ||
+        (uint64_t)im.width * (uint64_t)im.height > (1ull << 26))
+      return fail(BGPU_E_INVALID, "image " + std::to_string(k) + ": bad size, channels or texels");
+    if (texels.size() + (size_t)im.channels * im.width * im.height > 0x7fffffffull)
+      return fail(BGPU_E_INVALID, "images: more than 2^31 texels in all");
+    images[k] = int4{im.width, im.height, im.channels, (int32_t)texels.size()};
+    texels.insert(texels.end(), im.texels, im.texels + (size_t)im.channels * im.width * im.height);
+  }
+  auto is_scalar = [](int op) { return op >= BGPU_TN_SCONST && op <= BGPU_TN_SCALE; };
+  auto is_spectrum = [](int op) { return op >= BGPU_TN_CONST && op <= BGPU_TN_GRAPHPAPER; };
+  for (uint32_t k = 0; k < nn; ++k) {
+    const bgpu_tex_node& t = d->tex_nodes[k];
+    const int i = (int)k;
+    int low = i;
+    auto child = [&](int c, bool scalar) {
+      if (c < 0 || c >= i) return false;
+      if (!(scalar ? is_scalar(d->tex_nodes[c].op) : is_spectrum(d->tex_nodes[c].op))) return false;
+      low = std::min(low, lo[c]);
+      return true;
+    };
+    auto map2 = [&]() { return t.map == BGPU_MAP_UV || t.map == BGPU_MAP_PLANAR; };
+    auto img = [&](int ch) { return t.a >= 0 && (uint32_t)t.a < d->num_images && d->images[t.a].channels == ch; };
+    bool ok = true;
+    switch (t.op) {
+      case BGPU_TN_SCONST: case BGPU_TN_CONST: case BGPU_TN_GSTEP: break;
+      case BGPU_TN_SIMAGE: ok = img(1) && map2(); break;
+      case BGPU_TN_IMAGE: ok = img(16) && map2(); break;
+      case BGPU_TN_CELL: ok = t.a >= 0 && t.a <= 3 && t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_FBM: ok = t.a >= 0 && t.a <= 64 && t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_PERLIN: ok = t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_CRYSTAL: ok = t.a >= 1 && t.a <= 1024 && map2(); break;
+      case BGPU_TN_SCALE: ok = child(t.a, true); break;
+      case BGPU_TN_BLEND: ok = child(t.a, false) && child(t.b, false) && child(t.c, true); break;
+      case BGPU_TN_CHECKER: ok = child(t.a, false) && child(t.b, false); break;
+      case BGPU_TN_GRAPHPAPER: ok = child(t.a, false) && child(t.b, false) && map2(); break;
+      case BGPU_TN_GRADIENT:  // steps: GSTEP nodes b .. b + c - 1, ascending by position
+        ok = child(t.a, true) && t.c >= 1 && t.b >= 0 && t.b + t.c <= i;
+        for (int j = 0; ok && j < t.c; ++j) {
+          ok = d->tex_nodes[t.b + j].op == BGPU_TN_GSTEP &&
+               (j == 0 || d->tex_nodes[t.b + j - 1].q[0] <= d->tex_nodes[t.b + j].q[0]);
+          low = std::min(low, t.b + j);
+        }
+        break;
+      default: ok = false;
+    }
+    if (!ok) return fail(BGPU_E_INVALID, "texture node " + std::to_string(k) + ": bad operation or operands");
+    lo[k] = low;
+  }
+  std::vector<DMatSrc> srcs;
   for (uint32_t i = 0; i < d->num_materials; ++i) {
     const bgpu_material& m = d->materials[i];
     mats[i] = build(m, m.spec);
     const bgpu_texture& tx = m.tex;
     if (tx.kind == BGPU_TEX_NONE) continue;
+    if (tx.kind == BGPU_TEX_NODES) {  // inputs evaluated per hit by the texture kernel
+      if (tx.slots <= 0 || (tx.slots & ~0xf7) != 0)
+        return fail(BGPU_E_INVALID, "material " + std::to_string(i) + ": bad textured slots");
+      DMatSrc src;
+      std::memset(&src, 0, sizeof(src));
+      src.type = m.type;
+      src.slots = tx.slots;
+      std::memcpy(src.scalar, m.scalar, sizeof(src.scalar));
+      std::memcpy(src.spec, m.spec, sizeof(src.spec));
+      for (int k = 0; k < 8; ++k) {
+        if (!((tx.slots >> k) & 1)) continue;
+        const int r = tx.roots[k];
+        if (r < 0 || (uint32_t)r >= nn ||
+            !(k < 3 ? is_spectrum(d->tex_nodes[r].op) : is_scalar(d->tex_nodes[r].op)) ||
+            r - lo[r] + 1 > tx::kMaxTexRange)
+          return fail(BGPU_E_INVALID, "material " + std::to_string(i) + ": bad texture root, or a texture of more "
+                                      "than " + std::to_string(tx::kMaxTexRange) + " nodes");
+        src.roots[k] = r;
+        src.lo[k] = lo[r];
+      }
+      mats[i].tex_kind = BGPU_TEX_NODES;
+      mats[i].alt = (int32_t)srcs.size();
+      srcs.push_back(src);
+      pt.node_tex = true;
+      continue;
+    }
     if ((tx.kind != BGPU_TEX_CHECKER && tx.kind != BGPU_TEX_GRAPHPAPER) ||
         (tx.mapping != BGPU_MAP_UV && tx.mapping != BGPU_MAP_PLANAR) || tx.slots < 1 || tx.slots > 3)
       return fail(BGPU_E_INVALID, "material texture: unknown kind, mapping or slots");
@@ -304,7 +269,8 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     for (uint32_t s = 0; s < ns; ++s)
       if (d->shapes[s].material >= 0 && (size_t)d->shapes[s].material < used.size()) used[d->shapes[s].material] = 1;
     for (size_t i = 0; i < d->num_materials; ++i)  // a texture may select the alternate entry
-      if (used[i] && mats[i].tex_kind != BGPU_TEX_NONE) used[mats[i].alt] = 1;
+      if (used[i] && (mats[i].tex_kind == BGPU_TEX_CHECKER || mats[i].tex_kind == BGPU_TEX_GRAPHPAPER))
+        used[mats[i].alt] = 1;
     std::vector<int> sigs;
     bool diffuse = true;
     for (size_t i = 0; i < mats.size(); ++i) {
@@ -385,6 +351,24 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   H.lights = sc->lights.as<DLight>();
   H.maps = sc->light_maps.as<float>();
   H.dist = sc->light_dist.as<float>();
+  pt.tex_lo_host = lo;
+  pt.tex_op_host.resize(nn);
+  for (uint32_t k = 0; k < nn; ++k) pt.tex_op_host[k] = d->tex_nodes[k].op;
+  if (nn) {
+    if (int rc = sc->tex_nodes.upload(ctx, d->tex_nodes, sizeof(bgpu_tex_node) * nn, st)) return rc;
+    if (int rc = sc->tex_lo.upload(ctx, lo.data(), 4 * lo.size(), st)) return rc;
+    if (!images.empty())
+      if (int rc = sc->images.upload(ctx, images.data(), sizeof(int4) * images.size(), st)) return rc;
+    if (!texels.empty())
+      if (int rc = sc->texels.upload(ctx, texels.data(), 4 * texels.size(), st)) return rc;
+    if (!srcs.empty())
+      if (int rc = sc->mat_src.upload(ctx, srcs.data(), sizeof(DMatSrc) * srcs.size(), st)) return rc;
+    H.tex_nodes = sc->tex_nodes.as<bgpu_tex_node>();
+    H.tex_lo = sc->tex_lo.as<int32_t>();
+    H.images = images.empty() ? nullptr : sc->images.as<int4>();
+    H.texels = texels.empty() ? nullptr : sc->texels.as<float>();
+    H.mat_src = srcs.empty() ? nullptr : sc->mat_src.as<DMatSrc>();
+  }
   H.ntri = (int32_t)nt;
   H.nlights = (int32_t)(d->num_lights + std::count_if(pt.shape_light_host.begin(),
                                                       pt.shape_light_host.end(),

@@ -469,6 +469,8 @@ foreign import ccall safe "bgpu_intersect_closest" c_intersectClosest
   :: Ptr Scene -> Ptr Rays -> CSize -> Ptr Hits -> CInt -> IO CInt
 foreign import ccall safe "bgpu_intersect_closest_noalloc" c_intersectClosestNoalloc
   :: Ptr Scene -> Ptr Rays -> CSize -> Ptr Hits -> CInt -> IO CInt
+foreign import ccall safe "bgpu_scene_texture_eval" c_sceneTextureEval
+  :: Ptr Scene -> CInt -> CSize -> Ptr CFloat -> Ptr CFloat -> Ptr CFloat -> IO CInt
 foreign import ccall safe "bgpu_intersect_any" c_intersectAny
   :: Ptr Scene -> Ptr Rays -> CSize -> Ptr Word8 -> CInt -> IO CInt
 foreign import ccall unsafe "bgpu_last_kernel_ms" c_lastKernelMs :: Ptr Ctx -> Ptr CDouble -> IO CInt

@@ -126,6 +126,19 @@ class GpuScene:
         _ck(self.ctx.lib.bgpu_scene_bytes(self.h, C.byref(v)))
         return v.value
 
+    def texture_eval(self, node, xyz, uv):
+        """Texture `node` of the scene's node list evaluated on the device at
+        points xyz (n, 3) with surface parameters uv (n, 2): (n, 16) for a
+        spectrum texture, (n,) for a scalar one (bgpu_scene_texture_eval)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        uv = np.ascontiguousarray(uv, np.float32)
+        n = len(xyz)
+        op = self.flat.tex_nodes[node].op if 0 <= node < self.flat.desc.num_tex_nodes else 0
+        out = np.zeros((n, 16) if op >= abi.TN_CONST else n, np.float32)
+        _ck(self.ctx.lib.bgpu_scene_texture_eval(self.h, node, n, xyz.ctypes.data, uv.ctypes.data,
+                                                  out.ctypes.data))
+        return out
+
     # --- host arrays ---------------------------------------------------
     def intersect(self, rays, coherent=False):
         """rays: (n, 8) float32 [ox oy oz dx dy dz tmin tmax]; returns

@@ -14,7 +14,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import abi
+from . import abi, textures
 from .spectral_tables import TABLES
 from .transform import Transform
 
@@ -125,6 +125,8 @@ class Material:
         self.tex_kind, self.tex_map, self.tex_slots = abi.TEX_NONE, abi.MAP_UV, 0
         self.tex_params = [0.0] * 12
         self.alt = [const(0), const(0)]
+        # inputs taken from node textures (TEX_NODES): slot -> textures.Tex
+        self.node_spec, self.node_scalar = {}, {}
 
     def textured(self, slot, kind, params, alt, mapping=abi.MAP_UV):
         """This material with spectrum slot `slot` (0 or 1) textured: `kind`
@@ -136,6 +138,22 @@ class Material:
         m.tex_params = [float(F(x)) for x in params] + [0.0] * (12 - len(params))
         m.alt = [self.spec0.copy(), self.spec1.copy()]
         m.alt[slot] = np.asarray(alt, F)
+        return m
+
+    def textured_inputs(self, spec=None, scalar=None):
+        """This material with inputs taken from textures (textures.Tex,
+        evaluated per hit): `spec` maps spectrum slots (0..2) and `scalar`
+        scalar slots (0..3) to textures; the other inputs stay constant."""
+        spec, scalar = dict(spec or {}), dict(scalar or {})
+        if not spec and not scalar:
+            return self
+        if any(k not in (0, 1, 2) or t.scalar for k, t in spec.items()) or \
+                any(k not in (0, 1, 2, 3) or not t.scalar for k, t in scalar.items()):
+            raise ValueError("spectrum slots 0..2 take spectrum textures, scalar slots 0..3 scalar ones")
+        m = Material(self.kind, self.spec0, self.spec1, self.s0, self.s1, self.spec2, self.s2)
+        m.tex_kind = abi.TEX_NODES
+        m.tex_slots = sum(1 << k for k in spec) + sum(1 << (4 + k) for k in scalar)
+        m.node_spec, m.node_scalar = spec, scalar
         return m
 
     @staticmethod
@@ -350,6 +368,7 @@ class FlatScene:
         self.indices = np.ascontiguousarray(self.indices)
         mats = b.materials or [Material.matte(const(0.5))]
         self.materials = (abi.Material * len(mats))()
+        tex_nodes, tex_images = [], []
         for i, m in enumerate(mats):
             self.materials[i].type = m.kind
             self.materials[i].scalar[0] = m.s0
@@ -366,6 +385,14 @@ class FlatScene:
             for k in range(16):
                 t.alt[0][k] = m.alt[0][k]
                 t.alt[1][k] = m.alt[1][k]
+            if m.tex_kind == abi.TEX_NODES:  # roots into the scene's node list
+                roots = abi.texture_roots(t)
+                for k in range(12):
+                    roots[k] = -1
+                for k, tex in m.node_spec.items():
+                    roots[k] = textures.emit(tex, tex_nodes, tex_images)
+                for k, tex in m.node_scalar.items():
+                    roots[4 + k] = textures.emit(tex, tex_nodes, tex_images)
         meshes = b.meshes or [(0, 0)]
         self.meshes = (abi.Mesh * len(meshes))()
         for i, (mat, fl) in enumerate(meshes):
@@ -395,6 +422,17 @@ class FlatScene:
         d.materials = C.cast(self.materials, C.POINTER(abi.Material))
         d.num_lights = len(b.lights)
         d.lights = C.cast(self.lights, C.POINTER(abi.Light))
+        # the texture node list and its images (texels kept alive here)
+        self.tex_nodes = (abi.TexNode * max(1, len(tex_nodes)))(*tex_nodes)
+        self.image_texels = [np.ascontiguousarray(t.reshape(-1), F) for _, t in tex_images]
+        self.images = (abi.Image * max(1, len(tex_images)))()
+        for i, ((ch, t), flat) in enumerate(zip(tex_images, self.image_texels)):
+            self.images[i].width, self.images[i].height, self.images[i].channels = t.shape[1], t.shape[0], ch
+            self.images[i].texels = flat.ctypes.data_as(abi.FP)
+        d.num_tex_nodes = len(tex_nodes)
+        d.tex_nodes = C.cast(self.tex_nodes, C.POINTER(abi.TexNode))
+        d.num_images = len(tex_images)
+        d.images = C.cast(self.images, C.POINTER(abi.Image))
         if b.camera is not None:
             d.camera = b.camera
         fw, fh, ft, fp = b.film

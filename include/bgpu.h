@@ -366,6 +366,15 @@ int bgpu_intersect_any(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
  * threads bypassing the context's lock) race. */
 int bgpu_intersect_closest_noalloc(bgpu_scene* scene, const bgpu_rays* rays, size_t n,
                                    bgpu_hits* hits, int flags);
+/* Evaluates texture `node` of the scene's node list (bgpu_tex_node) on the
+ * device at n points: xyz (3n) is the hit point and uv (2n) the surface
+ * parameters a texture reads (dgP, dgU, dgV); out gets 16 bands per point
+ * for a spectrum texture, one value for a scalar one.  Host arrays.  No
+ * counterpart in the reference (whose textures are closures, Texture.hs:59-
+ * 414): it exposes the per-hit evaluation that shading uses, for tests and
+ * tools. */
+int bgpu_scene_texture_eval(bgpu_scene* scene, int node, size_t n, const float* xyz, const float* uv,
+                            float* out);
 /* Duration in ms of the last BGPU_TIMED intersect launch on this context. */
 int bgpu_last_kernel_ms(bgpu_ctx* ctx, double* ms);
 

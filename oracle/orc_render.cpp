@@ -924,6 +924,21 @@ static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
   b.ng = dgg.n;
   if (material < 0 || material >= (int)c.s->materials.size()) return b;
   const bgpu_material& m = c.s->materials[material];
+  if (m.tex.kind == BGPU_TEX_NODES) {
+    // textured inputs evaluated at the shading DG (`tex dgs`), the others
+    // constant; the material is then built from them as from constants
+    bgpu_material mt = m;
+    const tex::TexDG tdg{dgs.p, dgs.u, dgs.v};
+    for (int k = 0; k < 3; ++k)
+      if ((m.tex.slots >> k) & 1) {
+        const tex::Spec16 v = tex::spectrum(c.s->textures, m.tex.roots[k], tdg);
+        for (int j = 0; j < 16; ++j) mt.spec[k][j] = v[j];
+      }
+    for (int k = 0; k < 4; ++k)
+      if ((m.tex.slots >> (4 + k)) & 1) mt.scalar[k] = tex::scalar(c.s->textures, m.tex.roots[4 + k], tdg);
+    addComponents(mt, specOf(mt.spec[0]), specOf(mt.spec[1]), b);
+    return b;
+  }
   const float* sp0 = m.spec[0];
   const float* sp1 = m.spec[1];
   if (m.tex.kind != BGPU_TEX_NONE && texSelectsTex2(m.tex, dgs)) {
@@ -1463,7 +1478,31 @@ void orc_film_splat(const bgpu_film* film, int n, const float* sx, const float* 
 }
 
 // Selects a planted mistake (the MUT_* list); 0 restores the restatement.
-void orc_set_mutant(int m) { orc::g_mutant = m; }
+void orc_set_mutant(int m) {
+  orc::g_mutant = m;
+  orc::tex::texMutant() = m;
+}
+
+// Texture `node` of the scene's node list at n points (xyz, uv): 16 bands
+// per point for a spectrum texture, one value for a scalar one.  Returns
+// 0, or -1 for a node that is not a texture.
+int orc_tex_eval(void* scene, int node, int n, const float* xyz, const float* uv, float* out) {
+  const orc::Scene* s = static_cast<const orc::Scene*>(scene);
+  if (node < 0 || node >= (int)s->textures.nodes.size()) return -1;
+  const int op = s->textures.nodes[node].op;
+  const bool spectrum = orc::tex::isSpectrumOp(op);
+  if (!spectrum && !orc::tex::isScalarOp(op)) return -1;
+  for (int k = 0; k < n; ++k) {
+    const orc::tex::TexDG dg{orc::mk(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]), uv[2 * k], uv[2 * k + 1]};
+    if (spectrum) {
+      const orc::tex::Spec16 v = orc::tex::spectrum(s->textures, node, dg);
+      for (int b = 0; b < 16; ++b) out[16 * k + b] = v[b];
+    } else {
+      out[k] = orc::tex::scalar(s->textures, node, dg);
+    }
+  }
+  return 0;
+}
 
 // frDielectric etai etat cosi (kind 0) or frConductor eta k cosi (kind 1) as
 // 16 bands (Fresnel.hs:21-70).

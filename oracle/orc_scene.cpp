@@ -303,6 +303,31 @@ Scene* buildScene(const bgpu_scene_desc* d, const char** err) {
   }
   s->camera = d->camera;
   s->film = d->film;
+  // texture nodes: every child precedes its node, references are in range
+  if (d->num_tex_nodes && !d->tex_nodes) { *err = "tex_nodes is NULL"; delete s; return nullptr; }
+  s->textures.nodes.assign(d->tex_nodes, d->tex_nodes + d->num_tex_nodes);
+  for (uint32_t i = 0; i < d->num_images; ++i) {
+    const bgpu_image& im = d->images[i];
+    if (im.width <= 0 || im.height <= 0 || (im.channels != 1 && im.channels != 16) || !im.texels) {
+      *err = "bad image"; delete s; return nullptr;
+    }
+    tex::Textures::Image o;
+    o.w = im.width;
+    o.h = im.height;
+    o.ch = im.channels;
+    o.texels.assign(im.texels, im.texels + (size_t)im.channels * im.width * im.height);
+    s->textures.images.push_back(std::move(o));
+  }
+  if (const char* e = tex::validate(s->textures)) { *err = e; delete s; return nullptr; }
+  for (const bgpu_material& m : s->materials)
+    if (m.tex.kind == BGPU_TEX_NODES)
+      for (int k = 0; k < 8; ++k) {
+        if (!((m.tex.slots >> k) & 1)) continue;
+        const int r = m.tex.roots[k];
+        const bool ok = k != 3 && r >= 0 && r < (int)d->num_tex_nodes &&
+                        (k < 3 ? tex::isSpectrumOp(d->tex_nodes[r].op) : tex::isScalarOp(d->tex_nodes[r].op));
+        if (!ok || (m.tex.slots >> 8) != 0) { *err = "bad texture root"; delete s; return nullptr; }
+      }
   // mkKdTree: bounds = foldl' mappend mempty (map worldBounds ps)  (KdTree.hs:111)
   s->bounds = emptyAABB();
   for (int id = 0; id < s->nprims(); ++id) s->bounds = extend(s->bounds, s->primBounds(id));

@@ -13,6 +13,7 @@
 
 #include "../include/bgpu.h"
 #include "orc_geom.h"
+#include "orc_texture.h"
 
 namespace orc {
 
@@ -48,6 +49,7 @@ struct Scene {
   std::vector<std::vector<float>> light_maps;
   bgpu_camera camera;
   bgpu_film film;
+  tex::Textures textures;  // the texture node list and its images
 
   int nprims() const { return ntri + (int)shapes.size(); }
   V3 v(int i) const { return verts[i]; }

@@ -11,7 +11,8 @@
 #include <cstdint>
 #include <vector>
 
-#include "bgpu.h"
+#include "../bling_amd/csrc/noise_table.h"
+#include "../include/bgpu.h"
 #include "orc_math.h"
 
 namespace orc {
@@ -35,13 +36,30 @@ struct Textures {
 };
 
 // The noise permutation, noisePerms (Texture.hs:400-414), twice.
-extern const int kNoisePerms[512];
+constexpr const int* kNoisePerms = bgpu::kNoisePermTable;
+
+// A planted mistake in the texture functions (the oracle's mutant switch,
+// test-only): 0 none; kMutPerlinWeight noiseWeight taken as t; kMutCellOwn
+// only the point's own cell searched; kMutGradSwap the gradient's weights
+// swapped; kMutCrystalWrap no mirroring of odd integer parts; kMutBlendSwap
+// the blend's weights swapped; kMutImageFlip the image's rows not flipped.
+enum { kMutPerlinWeight = 14, kMutCellOwn = 15, kMutGradSwap = 16, kMutCrystalWrap = 17, kMutBlendSwap = 18,
+       kMutImageFlip = 19 };
+inline int& texMutant() {
+  static int m = 0;
+  return m;
+}
+
+// floor to an integer; 0 for values no 64-bit integer holds (NaN included:
+// the reference's `floor` of such a Float is unspecified)
+inline long long floorLL(float x) { return std::fabs(x) < 4.6e18f ? (long long)std::floor(x) : 0ll; }
 
 // lerp (Math.hs:108-110)
 inline float lerpT(float t, float a, float b) { return (1.0f - t) * a + t * b; }
 
 // noiseWeight (Texture.hs:388-391): 6 t^4 t - 15 t^4 + 10 t^3
 inline float noiseWeight(float t) {
+  if (texMutant() == kMutPerlinWeight) return t;
   float t3 = t * t * t;
   float t4 = t3 * t;
   return 6.0f * t4 * t - 15.0f * t4 + 10.0f * t3;
@@ -59,7 +77,7 @@ inline float noiseGrad(int x, int y, int z, float dx, float dy, float dz) {
 
 // perlin3d (Texture.hs:358-386)
 inline float perlin3d(float x, float y, float z) {
-  long long fx = (long long)std::floor(x), fy = (long long)std::floor(y), fz = (long long)std::floor(z);
+  long long fx = floorLL(x), fy = floorLL(y), fz = floorLL(z);
   float dx = x - (float)fx, dy = y - (float)fy, dz = z - (float)fz;
   int ix = (int)(fx & 255), iy = (int)(fy & 255), iz = (int)(fz & 255);
   float w000 = noiseGrad(ix, iy, iz, dx, dy, dz);
@@ -120,12 +138,13 @@ inline float cellDist(int kind, V3 a, V3 b) {  // euclidianDist .. chebyshevDist
   }
 }
 inline float cellNoise(int kind, V3 p) {
-  long long ox = (long long)std::floor(p.x), oy = (long long)std::floor(p.y), oz = (long long)std::floor(p.z);
+  long long ox = floorLL(p.x), oy = floorLL(p.y), oz = floorLL(p.z);
   bool first = true;
   float best = 0.0f;
-  for (long long x = ox - 1; x <= ox + 1; ++x)
-    for (long long y = oy - 1; y <= oy + 1; ++y)
-      for (long long z = oz - 1; z <= oz + 1; ++z) {
+  const long long r = texMutant() == kMutCellOwn ? 0 : 1;
+  for (long long x = ox - r; x <= ox + r; ++x)
+    for (long long y = oy - r; y <= oy + r; ++y)
+      for (long long z = oz - r; z <= oz + r; ++z) {
         long long u = lcg(cellHash(x, y, z));
         int n = cellCount(u);
         for (int k = 0; k < n; ++k) {
@@ -156,13 +175,13 @@ inline float quasiCrystal(int octaves, float x, float y) {
   }
   // wrap: properFraction, then odd integer parts mirror
   float tr = std::trunc(s);
-  long long k = (long long)tr;
+  long long k = floorLL(tr);
   float v = s - tr;
   if (v < 0.0f) {
     k = k - 1;
     v = 1.0f + v;
   }
-  return (k & 1) != 0 ? 1.0f - v : v;
+  return ((k & 1) != 0 && texMutant() != kMutCrystalWrap) ? 1.0f - v : v;
 }
 
 // The 2d mappings (Texture.hs:162-179) and the 3d identity mapping
@@ -186,9 +205,48 @@ inline V3 map3d(const bgpu_tex_node& n, const TexDG& dg) {
 // getPixel / getPixelScalar (Texture.hs:89-104): the texel index at (u, v).
 inline long long imageTexel(const Textures::Image& im, float u, float v) {
   auto modp = [](long long a, long long b) { long long r = a % b; return r < 0 ? r + b : r; };
-  long long px = modp((long long)std::floor(u * (float)im.w), im.w);
-  long long py = modp((long long)std::floor((-v) * (float)im.h), im.h);
+  long long px = modp(floorLL(u * (float)im.w), im.w);
+  long long py = modp(floorLL((texMutant() == kMutImageFlip ? v : -v) * (float)im.h), im.h);
   return py * im.w + px;
+}
+
+// Whether each node's children precede it and have the kind its operation
+// takes (scalar or spectrum); nullptr if so, else the reason.
+inline bool isScalarOp(int op) { return op >= BGPU_TN_SCONST && op <= BGPU_TN_SCALE; }
+inline bool isSpectrumOp(int op) { return op >= BGPU_TN_CONST && op <= BGPU_TN_GRAPHPAPER; }
+inline const char* validate(const Textures& T) {
+  const int n = (int)T.nodes.size();
+  for (int i = 0; i < n; ++i) {
+    const bgpu_tex_node& t = T.nodes[i];
+    auto sc = [&](int c) { return c >= 0 && c < i && isScalarOp(T.nodes[c].op); };
+    auto sp = [&](int c) { return c >= 0 && c < i && isSpectrumOp(T.nodes[c].op); };
+    auto map2 = [&]() { return t.map == BGPU_MAP_UV || t.map == BGPU_MAP_PLANAR; };
+    auto img = [&](int ch) { return t.a >= 0 && t.a < (int)T.images.size() && T.images[t.a].ch == ch; };
+    bool ok = true;
+    switch (t.op) {
+      case BGPU_TN_SCONST: case BGPU_TN_CONST: case BGPU_TN_GSTEP: break;
+      case BGPU_TN_SIMAGE: ok = img(1) && map2(); break;
+      case BGPU_TN_IMAGE: ok = img(16) && map2(); break;
+      case BGPU_TN_CELL: ok = t.a >= 0 && t.a <= 3 && t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_FBM: ok = t.a >= 0 && t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_PERLIN: ok = t.map == BGPU_MAP_IDENTITY; break;
+      case BGPU_TN_CRYSTAL: ok = t.a >= 1 && map2(); break;
+      case BGPU_TN_SCALE: ok = sc(t.a); break;
+      case BGPU_TN_BLEND: ok = sp(t.a) && sp(t.b) && sc(t.c); break;
+      case BGPU_TN_CHECKER: ok = sp(t.a) && sp(t.b); break;
+      case BGPU_TN_GRAPHPAPER: ok = sp(t.a) && sp(t.b) && map2(); break;
+      case BGPU_TN_GRADIENT:
+        ok = sc(t.a) && t.c >= 1 && t.b >= 0 && t.b + t.c <= i;
+        for (int k = 0; ok && k < t.c; ++k) {
+          ok = T.nodes[t.b + k].op == BGPU_TN_GSTEP;
+          if (ok && k > 0) ok = T.nodes[t.b + k - 1].q[0] <= T.nodes[t.b + k].q[0];
+        }
+        break;
+      default: ok = false;
+    }
+    if (!ok) return "bad texture node";
+  }
+  return nullptr;
 }
 
 float scalar(const Textures& T, int node, const TexDG& dg);
@@ -213,12 +271,12 @@ inline Spec16 spectrum(const Textures& T, int node, const TexDG& dg) {
       float x = scalar(T, n.c, dg);
       if (x <= 0.0f) return v1;
       if (x >= 1.0f) return v2;
-      for (int b = 0; b < 16; ++b) out[b] = v1[b] * (1.0f - x) + v2[b] * x;
+      const float x1 = texMutant() == kMutBlendSwap ? 1.0f - x : x;
+      for (int b = 0; b < 16; ++b) out[b] = v1[b] * (1.0f - x1) + v2[b] * x1;
       return out;
     }
     case BGPU_TN_CHECKER: {  // checkerBoard (Texture.hs:209-219)
-      long long s = (long long)std::floor(dg.p.x * n.q[0]) + (long long)std::floor(dg.p.y * n.q[1]) +
-                    (long long)std::floor(dg.p.z * n.q[2]);
+      long long s = floorLL(dg.p.x * n.q[0]) + floorLL(dg.p.y * n.q[1]) + floorLL(dg.p.z * n.q[2]);
       return ((s % 2) + 2) % 2 == 0 ? spectrum(T, n.a, dg) : spectrum(T, n.b, dg);
     }
     case BGPU_TN_GRADIENT: {  // gradient (Texture.hs:236-247)
@@ -246,6 +304,7 @@ inline Spec16 spectrum(const Textures& T, int node, const TexDG& dg) {
       if (idx <= 0) return out;  // f is NaN: the reference's fromJust fails; black here
       float p0 = T.nodes[first + idx - 1].q[0], p1 = T.nodes[first + idx].q[0];
       float w = (f - p0) / (p1 - p0);
+      if (texMutant() == kMutGradSwap) w = 1.0f - w;
       Spec16 c0 = col(idx - 1), c1 = col(idx);
       for (int b = 0; b < 16; ++b) out[b] = c0[b] * (1.0f - w) + c1[b] * w;
       return out;

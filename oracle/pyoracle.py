@@ -56,6 +56,21 @@ class OracleScene:
             load().orc_scene_destroy(self.h)
             self.h = None
 
+    def tex_eval(self, node, xyz, uv):
+        """Texture `node` of the scene's node list at points xyz (n, 3) with
+        surface parameters uv (n, 2): (n, 16) for a spectrum texture, (n,)
+        for a scalar one."""
+        lib = load()
+        lib.orc_tex_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        uv = np.ascontiguousarray(uv, np.float32)
+        n = len(xyz)
+        op = self.flat.tex_nodes[node].op if node < self.flat.desc.num_tex_nodes else 0
+        out = np.zeros((n, 16) if op >= 16 else n, np.float32)
+        if lib.orc_tex_eval(self.h, node, n, xyz.ctypes.data, uv.ctypes.data, out.ctypes.data) != 0:
+            raise ValueError("not a texture node: %d" % node)
+        return out
+
     def __del__(self):
         try:
             self.close()
@@ -228,6 +243,7 @@ class mutant:
     the oracle for the duration of a block."""
     FR_DIEL_ETA, FR_COND_K, BLINN_NORM, MF_G, TRANS_ETA2, TRANS_DIR, OREN_SWAP = 1, 2, 3, 4, 5, 6, 7
     ENV_SIN, ENV_ROW, PATH_PC, PATH_T, POWER, FILTER_CLIP = 8, 9, 10, 11, 12, 13
+    PERLIN_WEIGHT, CELL_OWN, GRAD_SWAP, CRYSTAL_WRAP, BLEND_SWAP, IMAGE_FLIP = 14, 15, 16, 17, 18, 19
 
     def __init__(self, m):
         self.m = m

@@ -6,7 +6,8 @@ import numpy as np
 
 from bling_amd import abi, scenes
 from bling_amd import transform as T
-from bling_amd.scene import Material, SceneBuilder, const
+from bling_amd.scene import Material, SceneBuilder, const, rgb_illum, rgb_refl
+from bling_amd.textures import Map2d, Map3d, Tex, image_spectra
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SHAPES = {"sphere": abi.SHAPE_SPHERE, "quad": abi.SHAPE_QUAD, "disk": abi.SHAPE_DISK,
@@ -218,4 +219,58 @@ def two_mirrors_under_sky(r=0.9, sky=2.0, res=4, fov=1.0):
     c2w = T.look_at((0.0, 0.0, 0.0), (0.0, 0.0, 5.0), (0, 1, 0))
     sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
     sb.set_film(res, res)
+    return sb.build()
+
+
+def checker_image(h=16, w=24, seed=3):
+    """An 8-bit RGB test image: coloured blocks (for image textures)."""
+    rng = np.random.default_rng(seed)
+    blocks = rng.integers(0, 256, (4, 6, 3))
+    return np.repeat(np.repeat(blocks, h // 4, axis=0), w // 6, axis=1).astype(np.uint8)
+
+
+def node_texture_scene(res=48):
+    """Every node texture on a lit floor (Texture.hs:59-414): a quad floor
+    whose matte kd blends two colours by a quasi-crystal on a planar mapping
+    and whose sigma is a scaled fbm; a plastic sphere coloured by a gradient
+    of Euclidean cell noise; a matte box with an RGB image over the box's
+    own uv; a mesh quad with graph paper over a Perlin blend (uv mapping); a
+    substrate sphere with a checker kd, a grey-image ka and a textured
+    urough; under a uniform sky and a quad light."""
+    sb = SceneBuilder()
+    floor = Material.matte(rgb_refl(0.5, 0.5, 0.5)).textured_inputs(
+        spec={0: Tex.blend(Tex.spectrum(rgb_refl(0.8, 0.8, 0.75)), Tex.spectrum(rgb_refl(0.2, 0.25, 0.3)),
+                           Tex.crystal(5, Map2d.planar((1.0, 0, 0), (0, 0, 1.0), -3.0, 0.0)))},
+        scalar={0: Tex.scale(0.25, 0.25, Tex.fbm(4, 0.5, Map3d.identity(T.scale((2.0, 2.0, 2.0)))))})
+    fm = sb.material(floor)
+    sb.shape(abi.SHAPE_QUAD, (4.0, 4.0), o2w=T.rotate_x(90), material=fm)
+    cellm = sb.material(Material.plastic(const(0), rgb_refl(0.6, 0.6, 0.6), 0.05).textured_inputs(
+        spec={0: Tex.gradient(Tex.cell("euclidian", Map3d.identity(T.scale((3.0, 3.0, 3.0)))),
+                              [(0.0, rgb_refl(0.1, 0.1, 0.1)), (0.6, rgb_refl(0.2, 0.7, 0.7)),
+                               (1.0, rgb_refl(0.9, 0.6, 0.2))])}))
+    sb.shape(abi.SHAPE_SPHERE, (0.8,), o2w=T.translate((-1.2, 0.8, 0.3)), material=cellm)
+    img = sb.material(Material.matte(const(0.5)).textured_inputs(
+        spec={0: Tex.image(image_spectra(checker_image()), Map2d.uv(1.0, 1.0, 0.0, 0.0))}))
+    sb.shape(abi.SHAPE_BOX, (-0.5, 0.0, -0.5, 0.5, 1.0, 0.5), o2w=T.rotate_y(30) @ T.translate((1.3, 0.0, 0.4)),
+             material=img)
+    paper = sb.material(Material.matte(const(0.5)).textured_inputs(
+        spec={0: Tex.graph_paper(0.08, Map2d.uv(8.0, 8.0, 0.0, 0.0),
+                                 Tex.blend(Tex.spectrum(rgb_refl(0.7, 0.7, 0.2)), Tex.spectrum(rgb_refl(0.2, 0.3, 0.7)),
+                                           Tex.scale(0.5, 0.5, Tex.perlin(Map3d.identity(T.scale((4.0, 4.0, 4.0)))))),
+                                 Tex.spectrum(rgb_refl(0.05, 0.05, 0.05)))}))
+    sb.mesh(np.array([[-2.5, 0.01, 1.2], [-0.5, 0.01, 1.2], [-0.5, 0.01, 2.4], [-2.5, 0.01, 2.4]], np.float32),
+            np.array([[0, 1, 2], [0, 2, 3]], np.int32), material=paper,
+            uvs=np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32))
+    grey = np.clip(np.arange(12 * 12).reshape(12, 12) * 2, 0, 255).astype(np.uint8)
+    sub = sb.material(Material.substrate(const(0.5), rgb_refl(0.1, 0.1, 0.1), const(0.0), 0.05, 0.05, 0.3)
+                      .textured_inputs(spec={0: Tex.checker((4.0, 4.0, 4.0), Tex.spectrum(rgb_refl(0.8, 0.2, 0.2)),
+                                                            Tex.spectrum(rgb_refl(0.9, 0.9, 0.9))),
+                                             2: Tex.spectrum(rgb_refl(0.3, 0.1, 0.0))},
+                                       scalar={0: Tex.scale(0.02, 0.2, Tex.image_grey(grey, Map2d.uv(1.0, 1.0, 0.0, 0.0)))}))
+    sb.shape(abi.SHAPE_SPHERE, (0.6,), o2w=T.translate((0.2, 0.6, -1.0)), material=sub)
+    lt = T.rotate_x(-90) @ T.translate((0.0, 4.0, 0.0))
+    sb.shape(abi.SHAPE_QUAD, (1.0, 1.0), o2w=lt, material=fm, emission=rgb_illum(8.0, 8.0, 8.0))
+    sb.infinite_light(np.tile(const(0.4), (2, 4, 1)))
+    c2w = T.look_at((0.0, 3.5, -5.0), (0.0, 0.5, 0.3), (0, 1, 0))
+    sb.look(c2w, 50.0, res, res)
     return sb.build()

@@ -35,6 +35,7 @@ SCENES = {
     "cornell": lambda: scenes.cornell(32),
     "materials_env": lambda: cases.materials_scene(res=24, env=True),
     "textured_uvs": lambda: cases.textured_scene(res=24),
+    "node_textures": lambda: cases.node_texture_scene(res=16),
     "divergent_small": lambda: scenes.divergent_scene(res=32, hm=16, n_tori=2, torus_tris=200, n_spheres=10,
                                                       sky=(16, 8)),
 }
@@ -66,10 +67,20 @@ def test_round_trip_is_byte_exact(tmp_path, name):
             _records(e.materials, e.num_materials, abi.Material)
         assert _light_bytes(d) == _light_bytes(e)
         assert f.num_prims == flat.num_prims
+        assert (d.num_tex_nodes, d.num_images) == (e.num_tex_nodes, e.num_images)
+        assert _records(d.tex_nodes, d.num_tex_nodes, abi.TexNode) == \
+            _records(e.tex_nodes, e.num_tex_nodes, abi.TexNode)
+        for i in range(d.num_images):
+            a, b = d.images[i], e.images[i]
+            assert (a.width, a.height, a.channels) == (b.width, b.height, b.channels)
+            n = a.width * a.height * a.channels
+            assert np.ctypeslib.as_array(a.texels, (n,)).tobytes() == np.ctypeslib.as_array(b.texels, (n,)).tobytes()
 
 
-def test_oracle_renders_the_read_back_scene_identically(tmp_path):
-    flat = cases.materials_scene(res=16, env=True)
+@pytest.mark.parametrize("name", ["materials_env", "node_textures"])
+def test_oracle_renders_the_read_back_scene_identically(tmp_path, name):
+    flat = {"materials_env": lambda: cases.materials_scene(res=16, env=True),
+            "node_textures": lambda: cases.node_texture_scene(res=16)}[name]()
     path = tmp_path / "m.bgs"
     flat.save(path)
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=1,
