@@ -13,8 +13,13 @@ The grammar follows the reference's Parsec parser:
   IO/CameraParser.hs          perspective camera (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
-                              transMatte, blackbody with constant textures, checker and
-                              graphPaper over constants, blends of constants
+                              transMatte, substrate, blackbody; every texture
+                              (constant, image, cellNoise, fbm, perlin,
+                              crystal, scale, blend, checker, gradient,
+                              graphPaper; uv, planar and identity mappings):
+                              checker / graphPaper over constants become the
+                              material's selector texture, anything else a
+                              node texture (textures.Tex) evaluated per hit
   IO/PrimitiveParser.hs       mesh (polygon faces triangulated as fans,
                               TriangleMesh.hs:23-29) and shape primitives
   IO/LightParser.hs           infinite lights with a constant map; emission
@@ -32,16 +37,16 @@ keeping its own primitive order; explicit lights are prepended the same way
 Two kinds of error:
   BlingSyntaxError  the reference's parser rejects the text as well;
   Unsupported       valid `.bling` that the GPU path does not render
-                    (textures other than constants, heightMap, waveFront,
-                    fractals, Bezier patches, point and directional
-                    lights, sun/sky and image maps, the environment camera,
-                    bumpMap, substrate, shinyMetal, transMatte).
+                    (waveFront, fractals, Bezier patches, point and
+                    directional lights, sun/sky and image light maps, the
+                    environment camera, bumpMap, shinyMetal).
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
 """
 import ctypes
 import ctypes.util
+import os
 import re
 from fractions import Fraction
 
@@ -51,6 +56,7 @@ from . import abi
 from . import noise
 from . import transform as T
 from .scene import Material, SceneBuilder, rgb_illum, rgb_refl, spd
+from .textures import CELL_DISTANCES, Map2d, Map3d, Tex, image_spectra
 
 F = np.float32
 BANDS = 16
@@ -183,9 +189,10 @@ def heightmap_mesh(elev, ns, nt):
 
 
 class _Parser:
-    def __init__(self, text):
+    def __init__(self, text, base="."):
         self.toks = _tokens(text)
         self.i = 0
+        self.base = base  # basePath: the scene file's folder (RenderJob.hs:31-34)
         # start state (RenderJob.hs:19-29)
         self.res = (640, 480)
         self.renderer = ("sampler", ("stratified", 2, 2), ("path", 7, 3))
@@ -347,87 +354,201 @@ class _Parser:
         self.close()
 
     # -- textures and materials (MaterialParser.hs)
+    def qstring(self):
+        """pQString (ParserCore.hs:87-101): the text between the quotes, a
+        backslash pair kept as written."""
+        return self.take("str")[1:-1]
+
     def spectrum_texture(self, name):
-        """A spectrum texture (MaterialParser.hs:207-241): a constant spectrum,
-        or a selector ("checker" | "graphpaper", mapping, params, tex1, tex2)
-        over two constants; a blend of constants by a constant is folded."""
+        """pSpectrumTexture (MaterialParser.hs:189-228) as a textures.Tex
+        tree; a blend of constants by a constant folds to a constant."""
         self.named(name)
         self.open()
         t = self.word()
         if t == "constant":
-            v = self.spectrum()
+            v = Tex.spectrum(self.spectrum())
         elif t == "blend":
             t1 = self.spectrum_texture("tex1")
             t2 = self.spectrum_texture("tex2")
             f = self.scalar_texture("f")
-            if not (isinstance(t1, np.ndarray) and isinstance(t2, np.ndarray)):
-                raise Unsupported("blend of non-constant textures")
-            if f <= F(0):  # spectrumBlend (Texture.hs:135-145)
-                v = t1
-            elif f >= F(1):
-                v = t2
+            if t1.op == abi.TN_CONST and t2.op == abi.TN_CONST and f.op == abi.TN_SCONST:
+                a, b, w = t1.spec, t2.spec, F(f.q[0])
+                if w <= F(0):  # spectrumBlend (Texture.hs:135-145)
+                    v = t1
+                elif w >= F(1):
+                    v = t2
+                else:
+                    v = Tex.spectrum(np.array([F(F(x * F(F(1) - w)) + F(y * w)) for x, y in zip(a, b)], F))
             else:
-                v = np.array([F(F(a * F(F(1) - f)) + F(b * f)) for a, b in zip(t1, t2)], F)
+                v = Tex.blend(t1, t2, f)
+        elif t == "image":
+            self.open()
+            v = self.image_texture(grey=False)
+            self.close()
         elif t == "checker":
             scale = self.vec()
-            t1 = self.spectrum_texture("tex1")
-            t2 = self.spectrum_texture("tex2")
-            v = ("checker", abi.MAP_UV, list(scale), t1, t2)
+            v = Tex.checker(scale, self.spectrum_texture("tex1"), self.spectrum_texture("tex2"))
+        elif t == "gradient":
+            f = self.scalar_texture("f")
+            self.named("steps")
+            self.open()
+            steps = []
+            if self.peek()[0] == "num":  # sepBy: zero or more (pos, spectrum)
+                steps.append((self.flt(), self.spectrum()))
+                while self.peek()[1] == ",":
+                    self.take("punct", ",")
+                    steps.append((self.flt(), self.spectrum()))
+            self.close()
+            if not steps:
+                self.fail("empty list given to mkGradient")
+            v = Tex.gradient(f, steps)
         elif t == "graphPaper":
             lw = self.flt()
-            mapping, mp = self.mapping_2d("map")
-            t1 = self.spectrum_texture("tex1")
-            t2 = self.spectrum_texture("tex2")
-            v = ("graphpaper", mapping, [lw] + mp, t1, t2)
-        elif t in ("image", "gradient"):
-            raise Unsupported("spectrum texture %r" % t)
+            mapping = self.mapping_2d("map")
+            v = Tex.graph_paper(lw, mapping, self.spectrum_texture("tex1"), self.spectrum_texture("tex2"))
         else:
             self.fail("unknown texture type " + t)
-        if isinstance(v, tuple) and not (isinstance(v[3], np.ndarray) and isinstance(v[4], np.ndarray)):
-            raise Unsupported("%s texture over non-constant textures" % v[0])
         self.close()
         return v
 
     def mapping_2d(self, name):
-        """pTextureMapping2d (MaterialParser.hs:159-178): (mapping, params)."""
+        """pTextureMapping2d (MaterialParser.hs:159-178)."""
         self.named(name)
         self.open()
         t = self.word()
         if t == "uv":
-            mapping, params = abi.MAP_UV, [self.flt(), self.flt(), self.flt(), self.flt()]
+            m = Map2d.uv(self.flt(), self.flt(), self.flt(), self.flt())
         elif t == "planar":
             vu, vv = self.vec(), self.vec()
-            mapping, params = abi.MAP_PLANAR, list(vu) + list(vv) + [self.flt(), self.flt()]
+            m = Map2d.planar(vu, vv, self.flt(), self.flt())
         else:
             self.fail("unknown 2d mapping " + t)
         self.close()
-        return mapping, params
+        return m
 
-    def scalar_texture(self, name):
+    def mapping_3d(self, name):
+        """pTextureMapping3d (MaterialParser.hs:180-187): `identity` and the
+        world-to-texture transform block."""
         self.named(name)
         self.open()
         t = self.word()
-        if t in ("image", "cellNoise", "fbm", "perlin", "crystal", "scale"):
-            raise Unsupported("scalar texture %r (only constant textures are on the GPU path)" % t)
-        if t != "constant":
+        if t != "identity":
+            self.fail("unknown 3d mapping " + t)
+        m = Map3d.identity(self.transform_block())
+        self.close()
+        return m
+
+    def image_texture(self, grey):
+        """pImageTexture / pImageScalar (MaterialParser.hs:106-113, 189-196):
+        `file "name" map {...}`, the file relative to the scene file's
+        folder, decoded as readImageTextureMap / readImageScalarMap
+        (Texture.hs:106-124) accept it: RGB, RGBA (alpha dropped) or YCbCr
+        (converted to RGB) 8-bit images for spectra, 8-bit grey images for
+        scalars."""
+        self.named("file")
+        name = self.qstring()
+        path = os.path.join(self.base, name)
+        try:
+            from PIL import Image
+        except ImportError:
+            raise Unsupported("image textures need PIL to decode %r" % name)
+        try:
+            with Image.open(path) as im:
+                im.load()
+                mode, bits = im.mode, getattr(im, "bits", 8)
+                if grey:
+                    if mode != "L":
+                        self.fail("unsupported image type %s for a scalar image texture (%r)" % (mode, name))
+                    px = np.asarray(im, np.uint8)
+                else:
+                    if mode == "P":
+                        im = im.convert("RGBA" if "transparency" in im.info else "RGB")
+                        mode = im.mode
+                    if mode not in ("RGB", "RGBA", "YCbCr") or bits != 8:
+                        self.fail("unsupported image type %s for an image texture (%r)" % (mode, name))
+                    px = np.asarray(im.convert("RGB") if mode == "YCbCr" else im, np.uint8)[..., :3]
+        except OSError as e:
+            self.fail("cannot read image %r: %s" % (name, e))
+        mapping = self.mapping_2d("map")
+        return Tex.image_grey(px, mapping) if grey else Tex.image(image_spectra(px), mapping)
+
+    def scalar_texture(self, name):
+        """pScalarTexture (MaterialParser.hs:115-157) as a textures.Tex tree;
+        scale of a constant folds to a constant (a + s * v in binary32)."""
+        self.named(name)
+        self.open()
+        t = self.word()
+        if t == "constant":
+            v = Tex.constant(self.flt())
+        elif t == "image":
+            self.open()
+            v = self.image_texture(grey=True)
+            self.close()
+        elif t == "cellNoise":
+            d = self.word()
+            if d not in CELL_DISTANCES:
+                self.fail("unknown distance function " + d)
+            v = Tex.cell(d, self.mapping_3d("map"))
+        elif t == "fbm":
+            octaves = self.named_int("octaves")
+            omega = self.named_flt("omega")
+            v = Tex.fbm(octaves, omega, self.mapping_3d("map"))
+        elif t == "perlin":
+            v = Tex.perlin(self.mapping_3d("map"))
+        elif t == "crystal":
+            octaves = self.named_int("octaves")
+            v = Tex.crystal(octaves, self.mapping_2d("map"))
+        elif t == "scale":
+            a = self.flt()
+            sc = self.flt()
+            inner = self.scalar_texture("tex")
+            if inner.op == abi.TN_SCONST:  # scaleTexture (Texture.hs:184-185)
+                v = Tex.constant(F(F(a) + F(F(sc) * F(inner.q[0]))))
+            else:
+                v = Tex.scale(a, sc, inner)
+        else:
             self.fail("unknown texture type " + t)
-        v = self.flt()
         self.close()
         return v
 
     @staticmethod
-    def textured(make, slots):
-        """The material `make(spectra)` with selector textures in its
-        spectrum slots: every textured slot must use the same selector."""
-        tex = [v for v in slots if isinstance(v, tuple)]
-        m = make([v[3] if isinstance(v, tuple) else v for v in slots])
+    def _selector(t):
+        """A checker or graph paper over two constants, as the material's
+        selector texture (kind, mapping, params, tex1, tex2); None otherwise."""
+        if t.op not in (abi.TN_CHECKER, abi.TN_GRAPHPAPER) or \
+                any(k.op != abi.TN_CONST for k in t.kids):
+            return None
+        if t.op == abi.TN_CHECKER:
+            return ("checker", abi.MAP_UV, list(t.q[:3]), t.kids[0].spec, t.kids[1].spec)
+        return ("graphpaper", t.mapping.kind, [t.q[0]] + list(t.mapping.params), t.kids[0].spec, t.kids[1].spec)
+
+    @classmethod
+    def make_material(cls, make, spectra, scalars):
+        """The material `make(spectra, scalars)` with its inputs from
+        textures.  Constants stay constants; when every textured input is a
+        checker or graph paper over constants in spectrum slot 0 or 1 and all
+        of them use the same selector, the material takes the selector
+        texture (bgpu_texture); any other texture makes the material's
+        textured inputs node textures, evaluated per hit."""
+        zero = np.zeros(BANDS, F)
+        sel = [cls._selector(t) for t in spectra]
+        simple = all(t.op == abi.TN_SCONST for t in scalars) and \
+            all(t.op == abi.TN_CONST or (sel[k] is not None and k < 2) for k, t in enumerate(spectra))
+        if simple:
+            tex = [v for v in sel if v is not None]
+            if tex and any(v[:3] != tex[0][:3] for v in tex[1:]):
+                simple = False
+        if not simple:
+            m = make([t.spec if t.op == abi.TN_CONST else zero for t in spectra],
+                     [t.q[0] if t.op == abi.TN_SCONST else 0.0 for t in scalars])
+            return m.textured_inputs(spec={k: t for k, t in enumerate(spectra) if t.op != abi.TN_CONST},
+                                     scalar={k: t for k, t in enumerate(scalars) if t.op != abi.TN_SCONST})
+        m = make([v[3] if v is not None else t.spec for v, t in zip(sel, spectra)], [t.q[0] for t in scalars])
         if not tex:
             return m
         kind, mapping, params = tex[0][:3]
-        if any(v[:3] != (kind, mapping, params) for v in tex[1:]):
-            raise Unsupported("material with differently textured parameters")
-        bits = sum(1 << k for k, v in enumerate(slots) if isinstance(v, tuple))
-        alt = [v[4] if isinstance(v, tuple) else v for v in slots] + [None]
+        bits = sum(1 << k for k, v in enumerate(sel) if v is not None)
+        alt = [v[4] if v is not None else t.spec for v, t in zip(sel, spectra)]
         code = abi.TEX_CHECKER if kind == "checker" else abi.TEX_GRAPHPAPER
         out = m.textured(0, code, params, alt[0], mapping) if bits & 1 else m.textured(1, code, params, alt[1], mapping)
         if bits == 3:
@@ -436,46 +557,35 @@ class _Parser:
         return out
 
     def material_body(self):
+        """pMaterial (MaterialParser.hs:26-104): each material's spectrum
+        and scalar inputs in the slots of bgpu_material."""
         t = self.word()
         if t == "blackbody":
             return Material.black()
+        st, sc = self.spectrum_texture, self.scalar_texture
         if t == "matte":
-            kd = self.spectrum_texture("kd")
-            sigma = self.scalar_texture("sigma")
-            return self.textured(lambda s: Material.matte(s[0], sigma), [kd])
+            kd = st("kd")
+            return self.make_material(lambda s, x: Material.matte(s[0], x[0]), [kd], [sc("sigma")])
         if t == "plastic":
-            kd = self.spectrum_texture("kd")
-            ks = self.spectrum_texture("ks")
-            rough = self.scalar_texture("rough")
-            return self.textured(lambda s: Material.plastic(s[0], s[1], rough), [kd, ks])
+            kd, ks = st("kd"), st("ks")
+            return self.make_material(lambda s, x: Material.plastic(s[0], s[1], x[0]), [kd, ks], [sc("rough")])
         if t == "metal":
-            eta = self.spectrum_texture("eta")
-            k = self.spectrum_texture("k")
-            rough = self.scalar_texture("rough")
-            return self.textured(lambda s: Material.metal(s[0], s[1], rough), [eta, k])
+            eta, k = st("eta"), st("k")
+            return self.make_material(lambda s, x: Material.metal(s[0], s[1], x[0]), [eta, k], [sc("rough")])
         if t == "glass":
-            ior = self.scalar_texture("ior")
-            kr = self.spectrum_texture("kr")
-            kt = self.spectrum_texture("kt")
-            return self.textured(lambda s: Material.glass(ior, s[0], s[1]), [kr, kt])
+            ior = sc("ior")
+            kr, kt = st("kr"), st("kt")
+            return self.make_material(lambda s, x: Material.glass(x[0], s[0], s[1]), [kr, kt], [ior])
         if t == "mirror":
-            kr = self.spectrum_texture("kr")
-            return self.textured(lambda s: Material.mirror(s[0]), [kr])
+            return self.make_material(lambda s, x: Material.mirror(s[0]), [st("kr")], [])
         if t == "transMatte":  # pMatteTranslucent (MaterialParser.hs:92-97)
-            kr = self.spectrum_texture("kr")
-            kt = self.spectrum_texture("kt")
-            ks = self.scalar_texture("ks")
-            return self.textured(lambda s: Material.trans_matte(s[0], s[1], ks), [kr, kt])
+            kr, kt = st("kr"), st("kt")
+            return self.make_material(lambda s, x: Material.trans_matte(s[0], s[1], x[0]), [kr, kt], [sc("ks")])
         if t == "substrate":  # pSubstrateMaterial (MaterialParser.hs:50-58)
-            kd = self.spectrum_texture("kd")
-            ks = self.spectrum_texture("ks")
-            ka = self.spectrum_texture("ka")
-            urough = self.scalar_texture("urough")
-            vrough = self.scalar_texture("vrough")
-            depth = self.scalar_texture("depth")
-            if isinstance(ka, tuple):
-                raise Unsupported("substrate with a textured ka (selectors cover kd and ks)")
-            return self.textured(lambda s: Material.substrate(s[0], s[1], ka, urough, vrough, depth), [kd, ks])
+            kd, ks, ka = st("kd"), st("ks"), st("ka")
+            ur, vr, depth = sc("urough"), sc("vrough"), sc("depth")
+            return self.make_material(lambda s, x: Material.substrate(s[0], s[1], s[2], x[0], x[1], x[2]),
+                                 [kd, ks, ka], [ur, vr, depth])
         if t in ("bumpMap", "shinyMetal"):
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)
@@ -761,13 +871,14 @@ class BlingScene:
                        rr_start_depth=7, passes=passes, seed=seed)
 
 
-def parse(text):
-    """Parses `.bling` text; returns a BlingScene."""
-    p = _Parser(text)
+def parse(text, base="."):
+    """Parses `.bling` text; returns a BlingScene.  Files the text names
+    (image textures) are read relative to `base`."""
+    p = _Parser(text, base)
     p.run()
     return BlingScene(p)
 
 
 def load(path):
     with open(path) as f:
-        return parse(f.read())
+        return parse(f.read(), os.path.dirname(os.path.abspath(path)))

@@ -55,12 +55,32 @@ matMirror = #{const BGPU_MAT_MIRROR}
 matTransMatte = #{const BGPU_MAT_TRANSMATTE}
 matSubstrate = #{const BGPU_MAT_SUBSTRATE}
 
-texNone, texChecker, texGraphPaper, mapUV, mapPlanar :: Int32
+texNone, texChecker, texGraphPaper, texNodes, mapUV, mapPlanar, mapIdentity :: Int32
 texNone = #{const BGPU_TEX_NONE}
 texChecker = #{const BGPU_TEX_CHECKER}
 texGraphPaper = #{const BGPU_TEX_GRAPHPAPER}
+texNodes = #{const BGPU_TEX_NODES}
 mapUV = #{const BGPU_MAP_UV}
 mapPlanar = #{const BGPU_MAP_PLANAR}
+mapIdentity = #{const BGPU_MAP_IDENTITY}
+
+tnSConst, tnSImage, tnCell, tnFbm, tnPerlin, tnCrystal, tnScale :: Int32
+tnSConst = #{const BGPU_TN_SCONST}
+tnSImage = #{const BGPU_TN_SIMAGE}
+tnCell = #{const BGPU_TN_CELL}
+tnFbm = #{const BGPU_TN_FBM}
+tnPerlin = #{const BGPU_TN_PERLIN}
+tnCrystal = #{const BGPU_TN_CRYSTAL}
+tnScale = #{const BGPU_TN_SCALE}
+
+tnConst, tnImage, tnBlend, tnChecker, tnGradient, tnGraphPaper, tnGStep :: Int32
+tnConst = #{const BGPU_TN_CONST}
+tnImage = #{const BGPU_TN_IMAGE}
+tnBlend = #{const BGPU_TN_BLEND}
+tnChecker = #{const BGPU_TN_CHECKER}
+tnGradient = #{const BGPU_TN_GRADIENT}
+tnGraphPaper = #{const BGPU_TN_GRAPHPAPER}
+tnGStep = #{const BGPU_TN_GSTEP}
 
 meshNormals, meshUVs, lightInfinite, cameraPerspective :: Int32
 meshNormals = #{const BGPU_MESH_NORMALS}

@@ -11,7 +11,10 @@
 --     @"gpu" -> pGpuRenderer (gpuScene s)@;
 --   * one call next to each constructor the parser already calls:
 --     'captureMaterial' beside the material parsers
---     (IO/MaterialParser.hs:30-104), 'captureMesh' beside mkTriangleMesh and
+--     (IO/MaterialParser.hs:30-104), 'captureTexture' beside the texture
+--     parsers whose result is not a constant (IO/MaterialParser.hs:106-228)
+--     with the material then taking 'nodeTexture' of the roots,
+--     'captureMesh' beside mkTriangleMesh and
 --     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureShape' beside
 --     mkGeom, 'captureInfinite' beside mkInfiniteAreaLight
 --     (IO/LightParser.hs:29-33), 'captureView' where the camera and filter
@@ -24,14 +27,17 @@
 -- accessors a maintainer exports (Spectrum.hs:46, Transform.hs:123-127).
 module Graphics.Bling.IO.GpuParser (
    newGpuScene, pGpuRenderer, GpuMaterial(..), captureMaterial, captureMesh, captureShape,
-   captureInfinite, captureView
+   captureInfinite, captureView, GpuTex(..), GpuMap2d(..), captureTexture, nodeTexture
    ) where
 
 import Control.Monad.IO.Class (liftIO)
 import Data.IORef
+import Data.List (sortBy)
+import Data.Ord (comparing)
 import qualified Data.Vector as V
 import qualified Data.Vector.Storable as VS
 import Foreign (nullPtr)
+import GHC.Float (castWord32ToFloat)
 import Text.ParserCombinators.Parsec
 
 import Graphics.Bling.Filter (Filter)
@@ -71,8 +77,9 @@ pGpuRenderer ref = (flip namedBlock) "sampled" $ do
    return $ mkAnyRenderer $ mkGpuRenderer ref job { gjNu = nu, gjNv = nv }
 
 -- | The materials on the GPU path, with the values their parsers read
--- (Material.hs:32-127); textured slots are refused by the caller unless
--- they are constants or a checker / graphPaper selector over constants.
+-- (Material.hs:32-127).  A textured input carries any value in its slot
+-- here; the material's Texture ('nodeTexture') names the texture that
+-- replaces it per hit.
 data GpuMaterial
    = GMatte Spectrum Float                 -- ^ kd, sigma
    | GPlastic Spectrum Spectrum Float      -- ^ kd, ks, roughness
@@ -101,6 +108,77 @@ captureMaterial ref gm tex = liftIO $ atomicModifyIORef' ref (\b -> let (i, b') 
          GBlack -> rec matBlack [] []
       rec t sc sp = Material t 0 (take 4 (sc ++ repeat 0)) (take 3 (map spectrumBands sp ++ repeat (replicate 16 0)))
                              (maybe noTexture id tex)
+
+-- | The 2d texture mappings (Texture.hs:162-179) with their parameters.
+data GpuMap2d = GUV Float Float Float Float                    -- ^ su sv ou ov
+              | GPlanar (Float, Float, Float) (Float, Float, Float) Float Float  -- ^ vu vv ou ov
+
+-- | A texture as the material parser builds it (IO/MaterialParser.hs:106-228),
+-- with the values it reads: a 3d mapping is identityMapping3d's
+-- world-to-texture Transform; images are the decoded texels (row-major from
+-- the top: 16 bands of pixelSpectrum per texel for spectra, x / 255 for
+-- grey), with their width and height.
+data GpuTex
+   = GSConst Float
+   | GSImage Int Int (VS.Vector Float) GpuMap2d
+   | GCell Int Transform                     -- ^ 0 euclidian, 1 euclidian2, 2 manhattan, 3 chebyshev
+   | GFbm Int Float Transform                -- ^ octaves, omega
+   | GPerlin Transform
+   | GCrystal Int GpuMap2d                   -- ^ octaves
+   | GScale Float Float GpuTex
+   | GConst Spectrum
+   | GImage Int Int (VS.Vector Float) GpuMap2d
+   | GBlend GpuTex GpuTex GpuTex             -- ^ tex1 tex2 f
+   | GChecker (Float, Float, Float) GpuTex GpuTex
+   | GGradient GpuTex [(Float, Spectrum)]    -- ^ f, steps (mkGradient sorts them)
+   | GGraphPaper Float GpuMap2d GpuTex GpuTex
+
+-- | Records a texture in the scene's node list, children first; returns
+-- the root's node id.
+captureTexture :: IORef GpuSceneBuilder -> GpuTex -> JobParser Int32
+captureTexture ref t = liftIO $ atomicModifyIORef' ref (\b -> let (i, b') = emit t b in (b', i)) where
+   node op = TexNode op 0 0 0 0 [0, 0, 0] [0, 0, 0, 0] (replicate 16 0) (replicate 16 0)
+   add n b = gpuAddTexNode n b
+   q4 xs = take 4 (xs ++ repeat 0)
+   m2 n (GUV su sv ou ov) = n { tnMap = mapUV, tnM = take 16 ([su, sv, ou, ov] ++ repeat 0) }
+   m2 n (GPlanar (a, b', c) (d, e, f) ou ov) = n { tnMap = mapPlanar, tnM = take 16 ([a, b', c, d, e, f, ou, ov] ++ repeat 0) }
+   m3 n w2t = n { tnMap = mapIdentity, tnM = concat (transformRows w2t) }
+   image w h ch px b = gpuAddImage (fromIntegral w, fromIntegral h, ch, px) b
+   emit tex b = case tex of
+      GSConst v -> add (node tnSConst) { tnQ = q4 [v] } b
+      GSImage w h px m -> let (k, b1) = image w h 1 px b in add (m2 (node tnSImage) m) { tnA = k } b1
+      GCell d w2t -> add (m3 (node tnCell) w2t) { tnA = fromIntegral d } b
+      GFbm o w w2t -> add (m3 (node tnFbm) w2t) { tnA = fromIntegral o, tnQ = q4 [w] } b
+      GPerlin w2t -> add (m3 (node tnPerlin) w2t) b
+      GCrystal o m -> add (m2 (node tnCrystal) m) { tnA = fromIntegral o } b
+      GScale a s t' -> let (k, b1) = emit t' b in add (node tnScale) { tnA = k, tnQ = q4 [a, s] } b1
+      GConst s -> add (node tnConst) { tnSpec = spectrumBands s } b
+      GImage w h px m -> let (k, b1) = image w h 16 px b in add (m2 (node tnImage) m) { tnA = k } b1
+      GBlend t1 t2 f ->
+         let (k1, b1) = emit t1 b; (k2, b2) = emit t2 b1; (k3, b3) = emit f b2
+         in add (node tnBlend) { tnA = k1, tnB = k2, tnC = k3 } b3
+      GChecker (sx, sy, sz) t1 t2 ->
+         let (k1, b1) = emit t1 b; (k2, b2) = emit t2 b1
+         in add (node tnChecker) { tnA = k1, tnB = k2, tnQ = q4 [sx, sy, sz] } b2
+      GGradient f steps ->
+         let (k, b1) = emit f b
+             sorted = sortBy (comparing fst) steps   -- mkGradient's sortBy (stable)
+             first = fromIntegral (length (gsbTexNodes b1))
+             b2 = foldl (\acc (pos, c) -> snd (add (node tnGStep) { tnQ = q4 [pos], tnSpec = spectrumBands c } acc))
+                        b1 sorted
+         in add (node tnGradient) { tnA = k, tnB = first, tnC = fromIntegral (length steps) } b2
+      GGraphPaper lw m t1 t2 ->
+         let (k1, b1) = emit t1 b; (k2, b2) = emit t2 b1
+         in add (m2 (node tnGraphPaper) m) { tnA = k1, tnB = k2, tnQ = q4 [lw] } b2
+
+-- | The material Texture whose spectrum slots k (0..2) and scalar slots k
+-- (0..3) take the textures rooted at the given node ids (bgpu_texture of
+-- kind BGPU_TEX_NODES: the roots share the params' storage).
+nodeTexture :: [(Int, Int32)] -> [(Int, Int32)] -> Texture
+nodeTexture spec scalar = Texture texNodes 0 slots 0 (map root [0 .. 11]) [replicate 16 0, replicate 16 0] where
+   slots = sum [2 ^ k | (k, _) <- spec] + sum [2 ^ (4 + k) | (k, _) <- scalar]
+   ids = [(k, r) | (k, r) <- spec] ++ [(4 + k, r) | (k, r) <- scalar]
+   root k = castWord32ToFloat (fromIntegral (maybe (-1) id (lookup k ids)))
 
 -- | mkTriangleMesh o2w mat p i n uv (Primitive/TriangleMesh.hs:39-59): the
 -- vertices and normals go to world space with the same transPoint /

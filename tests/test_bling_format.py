@@ -23,6 +23,7 @@ from bling_amd import abi, scenes
 from bling_amd import bling_format as B
 from bling_amd import transform as T
 from bling_amd.scene import rgb_illum, rgb_refl, spd
+from bling_amd.textures import Map2d, Map3d, Tex
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_EXAMPLES = "/root/reference/examples"
@@ -153,13 +154,13 @@ def test_errors():
         B.parse("imageSize 10")
     with pytest.raises(B.Unsupported):
         B.parse("prim { mandelbulb order 8 epsilon 0.001 iterations 20 }")
-    with pytest.raises(B.Unsupported):
-        B.parse("material { matte kd { checker 1 1 1 tex1 { checker 2 2 2 tex1 { constant rgbR 1 1 1 } "
-                "tex2 { constant rgbR 0 0 0 } } tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
-    with pytest.raises(B.Unsupported):  # two slots under different selectors
-        B.parse("material { plastic kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } "
-                "tex2 { constant rgbR 0 0 0 } } ks { checker 2 1 1 tex1 { constant rgbR 1 1 1 } "
-                "tex2 { constant rgbR 0 0 0 } } rough { constant 0.1 } }")
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("material { matte kd { constant rgbR 1 1 1 } sigma { cellNoise taxicab map { identity { } } } }")
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("material { matte kd { gradient f { constant 0 } steps { } } sigma { constant 0 } }")
+    with pytest.raises(B.BlingSyntaxError):  # graphPaper without its mapping (an older grammar)
+        B.parse("material { matte kd { graphPaper 0.1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
     with pytest.raises(B.Unsupported):
         B.parse("camera { environment }")
     with pytest.raises(ValueError, match="out of bounds"):
@@ -216,6 +217,7 @@ EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit li
     "glass-torus.bling": (0, 4, 0, (640, 360)),
     "cornell-box-underwater.bling": (79210, 3, 0, (720, 720)),
     "trans-matte.bling": (28, 2, 0, (480, 480)),
+    "substrate.bling": (0, 5, 0, (1280, 720)),
 }
 
 
@@ -233,7 +235,7 @@ def test_reference_examples():
             seen[name] = "syntax"
     for name, want in EXPECT.items():
         assert seen[name] == want, (name, seen[name])
-    assert sum(v == "unsupported" for v in seen.values()) >= 10
+    assert sum(v == "unsupported" for v in seen.values()) >= 8
     # cornell-box.bling: camera from its lookAt, film from its first filter
     b = B.load(os.path.join(REF_EXAMPLES, "cornell-box.bling"))
     c2w, _ = T.perspective_camera(T.look_at((278, 273, -800), (278, 273, 0), (0, 1, 0)) @ T.Transform(),
@@ -251,6 +253,13 @@ def test_reference_examples():
     paper = [t for t in tex if t.kind == abi.TEX_GRAPHPAPER]
     assert len(paper) == 1 and paper[0].mapping == abi.MAP_UV
     assert list(paper[0].params)[:5] == [F(0.05), 15, 15, 0, 0]
+    # substrate.bling: one sphere's depth is `scale 0.03 0.03 (fbm 3 0.5)` over identity { scale 1 1 1 }
+    b = B.load(os.path.join(REF_EXAMPLES, "substrate.bling"))
+    ops = [b.flat.tex_nodes[i].op for i in range(b.flat.desc.num_tex_nodes)]
+    assert ops == [abi.TN_FBM, abi.TN_SCALE]
+    m = [b.flat.materials[i] for i in range(b.flat.desc.num_materials) if b.flat.materials[i].tex.kind == abi.TEX_NODES]
+    assert len(m) == 1 and m[0].type == abi.MAT_SUBSTRATE
+    assert abi.texture_roots(m[0].tex)[4 + 2] == 1
 
 
 def test_noise_lattice_and_gradients():
@@ -309,3 +318,119 @@ def test_heightmap_mesh_closed_forms():
     f2 = b.flat
     assert f2.num_triangles == 5 * 3 * 2 and f2.normals is not None and f2.uvs is not None
     assert (f2.vertices[:, 1] == 0).all() and np.allclose(f2.normals, [0, -1, 0])
+
+
+TEX_TEXT = {  # .bling texture text -> the same texture built directly
+    "cell": ("cellNoise manhattan map { identity { scale 2 2 2 } }",
+             lambda: Tex.cell("manhattan", Map3d.identity(T.scale((2.0, 2.0, 2.0))))),
+    "fbm": ("fbm octaves 4 omega 0.6 map { identity { translate 1 0 0 rotateY 30 } }",
+            lambda: Tex.fbm(4, 0.6, Map3d.identity(T.translate((1.0, 0.0, 0.0)) @ T.rotate_y(30.0)))),
+    "perlin": ("perlin map { identity { scale 3 3 3 } }",
+               lambda: Tex.perlin(Map3d.identity(T.scale((3.0, 3.0, 3.0))))),
+    "crystal": ("crystal octaves 5 map { planar 1 0 0 0 0 1 0.5 0.25 }",
+                lambda: Tex.crystal(5, Map2d.planar((1.0, 0, 0), (0, 0, 1.0), 0.5, 0.25))),
+    "scale": ("scale 0.1 0.3 tex { perlin map { identity { } } }",
+              lambda: Tex.scale(0.1, 0.3, Tex.perlin(Map3d.identity()))),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TEX_TEXT))
+def test_scalar_textures_parse_to_their_trees(name):
+    """Each scalar texture (MaterialParser.hs:115-157), as a matte's sigma,
+    evaluates like the same tree built directly, at seeded points."""
+    text, make = TEX_TEXT[name]
+    b = B.parse("material { matte kd { constant rgbR 0.5 0.5 0.5 } sigma { %s } } "
+                "prim { shape { sphere radius 1 } }" % text)
+    m = b.flat.materials[0]
+    assert m.tex.kind == abi.TEX_NODES
+    root = abi.texture_roots(m.tex)[4]
+    want = _direct(make())
+    xyz = np.random.default_rng(1).uniform(-3, 3, (256, 3)).astype(F)
+    uv = np.random.default_rng(2).uniform(-1, 1, (256, 2)).astype(F)
+    assert np.array_equal(_eval(b.flat, root, xyz, uv), _eval(want[0], want[1], xyz, uv))
+
+
+def test_spectrum_textures_parse_to_their_trees():
+    """blend / checker / gradient / graphPaper over non-constant textures
+    (MaterialParser.hs:189-228) become node textures: gradient steps are
+    sorted by position, a constant blend or scale folds."""
+    text = ("gradient f { cellNoise euclidian map { identity { scale 2 2 2 } } } "
+            "steps { 0.9 rgbR 0.9 0.1 0.1, 0.1 rgbR 0.1 0.1 0.9,0.5 rgbR 0.2 0.8 0.2 }")
+    b = B.parse("material { plastic kd { %s } ks { checker 1 2 3 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { blend tex1 { constant rgbR 0 0 0 } tex2 { constant rgbR 1 0 0 } "
+                "f { perlin map { identity { } } } } } rough { scale 0.1 0.5 tex { constant 0.2 } } } "
+                "prim { shape { sphere radius 1 } }" % text)
+    m = b.flat.materials[0]
+    assert m.tex.kind == abi.TEX_NODES and m.scalar[0] == F(F(0.1) + F(F(0.5) * F(0.2)))
+    roots = abi.texture_roots(m.tex)
+    grad = b.flat.tex_nodes[roots[0]]
+    assert grad.op == abi.TN_GRADIENT and grad.c == 3
+    assert [b.flat.tex_nodes[grad.b + k].q[0] for k in range(3)] == [F(0.1), F(0.5), F(0.9)]
+    assert b.flat.tex_nodes[roots[1]].op == abi.TN_CHECKER and roots[4] < 0
+    steps = [(0.9, rgb_refl(0.9, 0.1, 0.1)), (0.1, rgb_refl(0.1, 0.1, 0.9)), (0.5, rgb_refl(0.2, 0.8, 0.2))]
+    want = _direct(Tex.gradient(Tex.cell("euclidian", Map3d.identity(T.scale((2.0, 2.0, 2.0)))), steps), spec=True)
+    xyz = np.random.default_rng(3).uniform(-3, 3, (256, 3)).astype(F)
+    uv = np.zeros((256, 2), F)
+    assert np.array_equal(_eval(b.flat, roots[0], xyz, uv), _eval(want[0], want[1], xyz, uv))
+    # two spectrum slots under different selectors: node textures too
+    b = B.parse("material { plastic kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } ks { checker 2 1 1 tex1 { constant rgbR 1 1 1 } "
+                "tex2 { constant rgbR 0 0 0 } } rough { constant 0.1 } } prim { shape { sphere radius 1 } }")
+    assert b.flat.materials[0].tex.kind == abi.TEX_NODES
+
+
+def _direct(t, spec=None):
+    from bling_amd.scene import Material, SceneBuilder, const
+    sb = SceneBuilder()
+    m = Material.matte(const(0.5))
+    sb.material(m.textured_inputs(spec={0: t}) if not t.scalar else m.textured_inputs(scalar={0: t}))
+    sb.shape(abi.SHAPE_SPHERE, (1.0,))
+    flat = sb.build()
+    return flat, abi.texture_roots(flat.materials[0].tex)[0 if not t.scalar else 4]
+
+
+def _eval(flat, root, xyz, uv):
+    from oracle import pyoracle
+    o = pyoracle.OracleScene(flat)
+    try:
+        return o.tex_eval(root, xyz, uv)
+    finally:
+        o.close()
+
+
+def test_image_textures(tmp_path):
+    """`image { file "x" map {...} }` (MaterialParser.hs:106-113, 189-196):
+    the file is read next to the scene file; RGB and RGBA (alpha dropped)
+    8-bit images give pixelSpectrum texels, 8-bit grey images x / 255; an
+    image of another kind, or a missing file, is an error."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(4)
+    rgb = rng.integers(0, 256, (5, 7, 3)).astype(np.uint8)
+    rgba = np.concatenate([rgb, rng.integers(0, 256, (5, 7, 1)).astype(np.uint8)], 2)
+    grey = rng.integers(0, 256, (3, 4)).astype(np.uint8)
+    Image.fromarray(rgb, "RGB").save(tmp_path / "c.png")
+    Image.fromarray(rgba, "RGBA").save(tmp_path / "ca.png")
+    Image.fromarray(grey, "L").save(tmp_path / "g.png")
+    (tmp_path / "sub").mkdir()
+    scene = ('material { matte kd { image { file "%s" map { uv 1 1 0 0 } } } '
+             'sigma { image { file "g.png" map { planar 1 0 0 0 1 0 0 0 } } } } prim { shape { sphere radius 1 } }')
+    from bling_amd.textures import image_spectra
+    for name in ("c.png", "ca.png"):
+        path = tmp_path / "sub" / "s.bling"
+        path.write_text((scene % ("../" + name)).replace('"g.png"', '"../g.png"'))
+        b = B.load(str(path))
+        d = b.flat.desc
+        assert d.num_images == 2
+        im = d.images[0]
+        assert (im.width, im.height, im.channels) == (7, 5, 16)
+        tex = np.ctypeslib.as_array(im.texels, (5 * 7 * 16,)).reshape(5, 7, 16)
+        assert np.array_equal(tex, image_spectra(rgb))
+        g = d.images[1]
+        assert (g.width, g.height, g.channels) == (4, 3, 1)
+        assert np.array_equal(np.ctypeslib.as_array(g.texels, (12,)).reshape(3, 4), grey.astype(F) / F(255))
+    with pytest.raises(B.BlingSyntaxError, match="unsupported image type"):
+        B.parse(scene % "g.png", base=str(tmp_path))       # a grey image as a spectrum texture
+    with pytest.raises(B.BlingSyntaxError, match="unsupported image type"):
+        B.parse((scene % "c.png").replace('"g.png"', '"c.png"'), base=str(tmp_path))
+    with pytest.raises(B.BlingSyntaxError, match="cannot read"):
+        B.parse(scene % "missing.png", base=str(tmp_path))
