@@ -134,102 +134,137 @@ struct DShapeX {  // shading-side data, indexed like DShape
   float o2w[16];
 };
 
+// The quad's intersect (Shape.hs:157-171): the z = 0 plane within
+// |x| <= sx, |y| <= sy.
+__device__ __forceinline__ bool quad_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
+  float sx = s.p[0], sy = s.p[1];
+  if (fabsf(ray.d.z) < 1e-7f) return false;
+  float t = -ray.o.z / ray.d.z;
+  if (t < ray.tmin || t > ray.tmax) return false;
+  f3 p = ray_at(ray, t);
+  if (fabsf(p.x) > sx || fabsf(p.y) > sy) return false;
+  *to = t;
+  *axis = 0;
+  return true;
+}
+
+// The other shapes' intersect (Shape.hs:86-229) in object space; each
+// returns t, and the box its slab axis.
+__device__ __forceinline__ bool box_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
+  float nearT = -INFINITY, farT = INFINITY;
+  int dd = 0;
+#pragma unroll
+  for (int dim = 0; dim < 3; ++dim) {
+    if (nearT > farT) return false;
+    float dInv = 1.0f / comp3(ray.d, dim);
+    float oc = comp3(ray.o, dim);
+    float t1p = (s.p[3 + dim] - oc) * dInv;
+    float t2p = (s.p[dim] - oc) * dInv;
+    float t1 = t1p, t2 = t2p;
+    if (t1p > t2p) {
+      t1 = t2p;
+      t2 = t1p;
+    }
+    int ndd = (nearT < t1) ? dim : dd;
+    nearT = hmax(nearT, t1);
+    farT = hmin(farT, t2);
+    dd = ndd;
+  }
+  if (nearT > farT) return false;
+  float t0 = hmin(nearT, farT);
+  float t1 = hmax(nearT, farT);
+  if (t0 > ray.tmax || t0 < ray.tmin) return false;
+  float t = (t0 < ray.tmin) ? t1 : t0;
+  if (t > ray.tmax) return false;
+  *to = t;
+  *axis = dd;
+  return true;
+}
+__device__ __forceinline__ bool cyl_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
+  float r = s.p[0], zmin = s.p[1], zmax = s.p[2], phimax = s.p[3];
+  float a = ray.d.x * ray.d.x + ray.d.y * ray.d.y;
+  float b = 2.0f * (ray.d.x * ray.o.x + ray.d.y * ray.o.y);
+  float c = ray.o.x * ray.o.x + ray.o.y * ray.o.y - r * r;
+  float t0, t1;
+  if (!solve_quadric(a, b, c, &t0, &t1)) return false;
+  if (t0 > ray.tmax) return false;
+  if (t1 < ray.tmin) return false;
+  f3 h0 = ray_at(ray, t0);
+  if (t0 > ray.tmin && h0.z > zmin && h0.z < zmax && atan2p(h0.y, h0.x) <= phimax) {
+    *to = t0;
+    *axis = 0;
+    return true;
+  }
+  f3 h1 = ray_at(ray, t1);
+  if (t1 <= ray.tmax && h1.z > zmin && h1.z < zmax && atan2p(h1.y, h1.x) <= phimax) {
+    *to = t1;
+    *axis = 0;
+    return true;
+  }
+  return false;
+}
+__device__ __forceinline__ bool disk_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
+  float h = s.p[0], rad = s.p[1], irad = s.p[2], phimax = s.p[3];
+  if (fabsf(ray.d.z) < 1e-7f) return false;
+  float t = (h - ray.o.z) / ray.d.z;
+  if (t < ray.tmin || t > ray.tmax) return false;
+  f3 p = ray_at(ray, t);
+  float d2 = p.x * p.x + p.y * p.y;
+  if (d2 > rad * rad || d2 < irad * irad) return false;
+  if (atan2p(p.y, p.x) > phimax) return false;
+  *to = t;
+  *axis = 0;
+  return true;
+}
+__device__ __forceinline__ bool sphere_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
+  float r = s.p[0];
+  float a = sqlen3(ray.d);
+  float b = 2.0f * dot3(ray.o, ray.d);
+  float c = sqlen3(ray.o) - r * r;
+  float t1, t2;
+  if (!solve_quadric(a, b, c, &t1, &t2)) return false;
+  if (t1 > ray.tmax) return false;
+  if (t2 < ray.tmin) return false;
+  float t = (t1 < ray.tmin) ? t2 : t1;
+  if (t > ray.tmax) return false;
+  *to = t;
+  *axis = 0;
+  return true;
+}
+
 // Shape.intersect in object space; returns t and the slab axis for boxes.
 __device__ __forceinline__ bool shape_intersect(const DShape& s, const DRay& ray, float* to, int* axis) {
   switch (s.type) {
-    case SH_BOX: {
-      float nearT = -INFINITY, farT = INFINITY;
-      int dd = 0;
-#pragma unroll
-      for (int dim = 0; dim < 3; ++dim) {
-        if (nearT > farT) return false;
-        float dInv = 1.0f / comp3(ray.d, dim);
-        float oc = comp3(ray.o, dim);
-        float t1p = (s.p[3 + dim] - oc) * dInv;
-        float t2p = (s.p[dim] - oc) * dInv;
-        float t1 = t1p, t2 = t2p;
-        if (t1p > t2p) {
-          t1 = t2p;
-          t2 = t1p;
-        }
-        int ndd = (nearT < t1) ? dim : dd;
-        nearT = hmax(nearT, t1);
-        farT = hmin(farT, t2);
-        dd = ndd;
-      }
-      if (nearT > farT) return false;
-      float t0 = hmin(nearT, farT);
-      float t1 = hmax(nearT, farT);
-      if (t0 > ray.tmax || t0 < ray.tmin) return false;
-      float t = (t0 < ray.tmin) ? t1 : t0;
-      if (t > ray.tmax) return false;
-      *to = t;
-      *axis = dd;
-      return true;
-    }
-    case SH_CYL: {
-      float r = s.p[0], zmin = s.p[1], zmax = s.p[2], phimax = s.p[3];
-      float a = ray.d.x * ray.d.x + ray.d.y * ray.d.y;
-      float b = 2.0f * (ray.d.x * ray.o.x + ray.d.y * ray.o.y);
-      float c = ray.o.x * ray.o.x + ray.o.y * ray.o.y - r * r;
-      float t0, t1;
-      if (!solve_quadric(a, b, c, &t0, &t1)) return false;
-      if (t0 > ray.tmax) return false;
-      if (t1 < ray.tmin) return false;
-      f3 h0 = ray_at(ray, t0);
-      if (t0 > ray.tmin && h0.z > zmin && h0.z < zmax && atan2p(h0.y, h0.x) <= phimax) {
-        *to = t0;
-        *axis = 0;
-        return true;
-      }
-      f3 h1 = ray_at(ray, t1);
-      if (t1 <= ray.tmax && h1.z > zmin && h1.z < zmax && atan2p(h1.y, h1.x) <= phimax) {
-        *to = t1;
-        *axis = 0;
-        return true;
-      }
-      return false;
-    }
-    case SH_DISK: {
-      float h = s.p[0], rad = s.p[1], irad = s.p[2], phimax = s.p[3];
-      if (fabsf(ray.d.z) < 1e-7f) return false;
-      float t = (h - ray.o.z) / ray.d.z;
-      if (t < ray.tmin || t > ray.tmax) return false;
-      f3 p = ray_at(ray, t);
-      float d2 = p.x * p.x + p.y * p.y;
-      if (d2 > rad * rad || d2 < irad * irad) return false;
-      if (atan2p(p.y, p.x) > phimax) return false;
-      *to = t;
-      *axis = 0;
-      return true;
-    }
-    case SH_QUAD: {
-      float sx = s.p[0], sy = s.p[1];
-      if (fabsf(ray.d.z) < 1e-7f) return false;
-      float t = -ray.o.z / ray.d.z;
-      if (t < ray.tmin || t > ray.tmax) return false;
-      f3 p = ray_at(ray, t);
-      if (fabsf(p.x) > sx || fabsf(p.y) > sy) return false;
-      *to = t;
-      *axis = 0;
-      return true;
-    }
-    default: {  // sphere
-      float r = s.p[0];
-      float a = sqlen3(ray.d);
-      float b = 2.0f * dot3(ray.o, ray.d);
-      float c = sqlen3(ray.o) - r * r;
-      float t1, t2;
-      if (!solve_quadric(a, b, c, &t1, &t2)) return false;
-      if (t1 > ray.tmax) return false;
-      if (t2 < ray.tmin) return false;
-      float t = (t1 < ray.tmin) ? t2 : t1;
-      if (t > ray.tmax) return false;
-      *to = t;
-      *axis = 0;
-      return true;
-    }
+    case SH_BOX:
+      return box_intersect(s, ray, to, axis);
+    case SH_CYL:
+      return cyl_intersect(s, ray, to, axis);
+    case SH_DISK:
+      return disk_intersect(s, ray, to, axis);
+    case SH_QUAD:
+      return quad_intersect(s, ray, to, axis);
+    default:
+      return sphere_intersect(s, ray, to, axis);
   }
+}
+
+// Shape kinds as a bit set (bit SH_x), for code compiled for a subset.
+constexpr int kAllShapes = (1 << SH_BOX) | (1 << SH_CYL) | (1 << SH_DISK) | (1 << SH_QUAD) | (1 << SH_SPHERE);
+template <int SK>
+__device__ __forceinline__ constexpr bool one_shape_kind() {
+  return SK != 0 && (SK & (SK - 1)) == 0;
+}
+// shape_intersect for shapes whose kind is in SK (the caller guarantees it):
+// only those kinds' code is compiled, and no dispatch for a single kind.
+template <int SK>
+__device__ __forceinline__ bool shape_intersect_k(const DShape& s, const DRay& ray, float* to, int* axis) {
+  constexpr bool one = one_shape_kind<SK>();
+  if ((SK >> SH_QUAD & 1) && (one || s.type == SH_QUAD)) return quad_intersect(s, ray, to, axis);
+  if ((SK >> SH_SPHERE & 1) && (one || s.type == SH_SPHERE)) return sphere_intersect(s, ray, to, axis);
+  if ((SK >> SH_BOX & 1) && (one || s.type == SH_BOX)) return box_intersect(s, ray, to, axis);
+  if ((SK >> SH_DISK & 1) && (one || s.type == SH_DISK)) return disk_intersect(s, ray, to, axis);
+  if ((SK >> SH_CYL & 1) && (one || s.type == SH_CYL)) return cyl_intersect(s, ray, to, axis);
+  return false;
 }
 
 // Shape.intersects (Shape.hs:231-284).

@@ -225,11 +225,12 @@ struct Engine {
   // for BVH width w (0: any width), shading in class-grouped order if `sort`
   // with the variants of `shade_mode` (SHADE_MODE_SPLIT needs `sort`), with
   // the texture kernel if `textured` (pmat_buf allocated).
-  int capture(int w, bool sort, bool direct, int shade_mode, bool textured = false) {
+  int capture(int w, bool sort, bool direct, int shade_mode, bool textured = false, int light_mode = LIGHTS_ALL) {
     args.sort_shade = sort ? 1 : 0;
     args.direct = direct ? 1 : 0;
     args.shade_mode = shade_mode;
     args.textured = textured ? 1 : 0;
+    args.light_mode = light_mode;
     args.pool.pmat = pmat_buf.ptr ? pmat_buf.as<DMaterial>() : nullptr;
     if (gexec) BGPU_HIP(hipGraphExecDestroy(gexec));
     if (graph) BGPU_HIP(hipGraphDestroy(graph));
@@ -339,6 +340,14 @@ struct Engine {
     for (Scene* s : scenes)
       if (s && !s->pt.diffuse_only) all_diffuse = false;
     const int want_mode = all_diffuse ? SHADE_MODE_DIFFUSE : (want_sort ? SHADE_MODE_SPLIT : SHADE_MODE_ALL);
+    // the diffuse variant's light kinds: those of every resident scene
+    int kinds = j->scene->pt.light_kinds;
+    for (Scene* s : scenes)
+      if (s) kinds |= s->pt.light_kinds;
+    const int want_lights = cfg.light_variants == 0      ? LIGHTS_ALL
+                            : (kinds & ~kQuadLights) == 0 ? LIGHTS_QUAD
+                            : (kinds & ~kQSILights) == 0  ? LIGHTS_QSI
+                                                          : LIGHTS_ALL;
     // the direct-lighting shading kernel runs while such a job is resident
     bool want_direct = j->desc.integrator == BGPU_INTEGRATOR_DIRECT;
     for (HostJob* o : slots)
@@ -353,9 +362,9 @@ struct Engine {
       if (int rc = pmat_buf.alloc(ctx, sizeof(DMaterial) * (size_t)P)) return rc;
     }
     if (want_w != width || want_sort != (args.sort_shade != 0) || want_direct != (args.direct != 0) ||
-        want_mode != args.shade_mode || want_tex != (args.textured != 0)) {
+        want_mode != args.shade_mode || want_tex != (args.textured != 0) || want_lights != args.light_mode) {
       BGPU_HIP(hipStreamSynchronize(stream));
-      if (int rc = capture(want_w, want_sort, want_direct, want_mode, want_tex)) return rc;
+      if (int rc = capture(want_w, want_sort, want_direct, want_mode, want_tex, want_lights)) return rc;
     }
     int sidx = scene_index(j->scene);
     if (sidx < 0) return sidx;
@@ -1052,6 +1061,9 @@ int bgpu_config_set(bgpu_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "pt.diffuse_waves") {
     if (value != 0 && value != 2 && value != 4) return fail(BGPU_E_INVALID, "pt.diffuse_waves must be 0, 2 or 4");
     next.diffuse_waves = (int)value;
+  } else if (k == "pt.light_variants") {
+    if (value != 0 && value != 1) return fail(BGPU_E_INVALID, "pt.light_variants must be 0 or 1");
+    next.light_variants = (int)value;
   } else if (k == "pt.sort") {
     if (value < 0 || value > 2) return fail(BGPU_E_INVALID, "pt.sort must be 0, 1 or 2");
     next.sort = (int)value;
@@ -1106,6 +1118,7 @@ int bgpu_config_get(bgpu_ctx* ctx, const char* key, int64_t* value) {
   else if (k == "pt.pool") *value = ctx->pt_cfg.pool;
   else if (k == "pt.shade_waves") *value = ctx->pt_cfg.shade_waves;
   else if (k == "pt.diffuse_waves") *value = ctx->pt_cfg.diffuse_waves;
+  else if (k == "pt.light_variants") *value = ctx->pt_cfg.light_variants;
   else if (k == "pt.persistent") *value = ctx->pt_cfg.persistent;
   else if (k == "pt.sort") *value = ctx->pt_cfg.sort;
   else if (k == "pt.dl_depth") *value = ctx->pt_cfg.dl_depth;

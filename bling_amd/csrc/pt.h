@@ -183,6 +183,7 @@ struct PtConfig {
   int pool = 1 << 20;
   int shade_waves = 2;  // occupancy hint of the shading kernel (0 or 2 waves per SIMD)
   int diffuse_waves = 2;  // the same for its diffuse-only variant (0, 2 or 4)
+  int light_variants = 1;  // the diffuse variant's light-kind variants (LIGHTS_*) on (1) or off (0)
   int stack = 12;         // traversal stack entries per lane kept in LDS (kMinStackLds..kMaxStackLds)
   int walk = 8;           // traversal: walk inner nodes while at least this many lanes do (0: all)
   int persistent = 0;   // traversal passes with lane refill (1) or one ray per thread (0)
@@ -209,7 +210,19 @@ struct PtKernelArgs {
   int32_t nshort;      // traversal stack entries per lane in LDS (PtConfig::stack)
   int32_t walk;        // traversal walk threshold (PtConfig::walk)
   int32_t textured;    // a resident scene has node-textured materials: run the texture kernel
+  int32_t light_mode;  // LIGHTS_*: the light kinds the diffuse shading variant compiles in
 };
+
+// Light kinds of the diffuse shading variant: every kind; or, when every
+// resident scene's lights are of these kinds, quads only, or quads, spheres
+// and infinite lights (the other shapes' light code and its registers left
+// out).
+enum { LIGHTS_ALL = 0, LIGHTS_QUAD = 1, LIGHTS_QSI = 2 };
+// Bits of a light-kind set: bit k < 5 an area light on bgpu shape type k,
+// bit kLightInf an infinite light.
+constexpr int kLightInf = 5;
+constexpr int kQuadLights = 1 << BGPU_SHAPE_QUAD;
+constexpr int kQSILights = (1 << BGPU_SHAPE_QUAD) | (1 << BGPU_SHAPE_SPHERE) | (1 << kLightInf);
 
 // Shading variants of an iteration: one kernel for every BxDF kind; the
 // diffuse variant alone (every resident scene's materials are Lambert /
@@ -224,6 +237,7 @@ struct PtScene {
   int shade_kinds = 0;                    // distinct BxDF makeups of the materials in use
   bool diffuse_only = false;              // every material in use has Lambert / Oren-Nayar components only
   bool node_tex = false;                  // a material's inputs are textures of the node list
+  int light_kinds = 0;                    // bit k: an area light on shape type k; bit 5: an infinite light
   std::vector<int32_t> tex_lo_host;       // per texture node, the lowest node of its subtree
   std::vector<int32_t> tex_op_host;       // per texture node, its operation
 };

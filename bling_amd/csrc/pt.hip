@@ -446,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void k_class_scatter(PtKernelArgs a) {
 // (:61-82): the light sample's shadow ray and the BSDF sample's MIS ray are
 // queued with the contributions they carry if unoccluded / if they reach the
 // light, scaled by the path throughput thr_at(band).
-template <int KM, class Thr>
+template <int KM, int LK, class Thr>
 __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const PathPool& P, int Pn, int i,
                                                  const Bsdf& bsdf, f3 wo, f3 p, float eps, float lNumU, float ld1,
                                                  float ld2, float bComp, float bd1, float bd2, Thr thr_at,
@@ -461,7 +461,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
       lscale = (float)lc;
     }
     {  // sampleLightMis (Scene.hs:61-69)
-      LSample ls = sample_light(S, ln, p, eps, ld1, ld2);
+      LSample ls = sample_light<LK>(S, ln, p, eps, ld1, ld2);
       if (ls.pdf != 0.0f && ls.li) {
         SpecVal f = eval_bsdf<KM>(bsdf, wo, ls.wi);
         float w = power_heuristic(ls.pdf, bsdf_pdf<KM>(bsdf, wo, ls.wi));
@@ -490,11 +490,12 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
       int tb;
       float bpdf = sample_bsdf<KM>(bsdf, wo, bComp, bd1, bd2, fb, &wib, &tb);
       if (bpdf != 0.0f) {
-        float lpdf = light_pdf(S, ln, p, wib);
+        float lpdf = light_pdf<LK>(S, ln, p, wib);
         float w = power_heuristic(bpdf, lpdf);
         // the emission a hit on the sampled light returns is known here:
         // the area light's radiance, or the infinite light's along wib
-        const float* e = S.lights[ln].type == LT_INF ? light_le(S, ln, wib) : S.lights[ln].L;
+        const float* e = has_inf_light<LK>() && S.lights[ln].type == LT_INF ? light_le<LK>(S, ln, wib)
+                                                                            : S.lights[ln].L;
         Xyz c;
         bool fBlack = true;
 #pragma unroll 4
@@ -509,7 +510,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
         // since nothing beyond the light can be the closest hit.
         float mt = INFINITY;
         bool misses = false;
-        if (S.lights[ln].type == LT_AREA) {
+        if (has_area_light<LK>() && S.lights[ln].type == LT_AREA) {
           const DShape& lsh = S.geo.shapes[S.lights[ln].shape];
           DRay ro;
           ro.o = xform_point(lsh.w2o, p);
@@ -518,7 +519,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
           ro.tmax = INFINITY;
           float tl;
           int ax;
-          if (shape_intersect(lsh, ro, &tl, &ax)) mt = nextafterf(tl, INFINITY);
+          if (shape_intersect_k<light_shapes<LK>()>(lsh, ro, &tl, &ax)) mt = nextafterf(tl, INFINITY);
           else misses = true;
         }
         if (!fBlack && !misses) {
@@ -605,8 +606,9 @@ __device__ __forceinline__ void shade_range(const PtKernelArgs& a, unsigned int*
 
 // Path shading (Path.hs:41-87).  KM: the BxDF kinds compiled in (kAllKinds,
 // or kDiffuseKinds for the variant that shades only diffuse materials with
-// far fewer registers); PART: the queue range it shades.
-template <int BLOCK, int MINW, int KM, int PART>
+// far fewer registers); LK: the light kinds compiled in (kAllLights, or the
+// quad-only or infinite-only sets); PART: the queue range it shades.
+template <int BLOCK, int MINW, int KM, int PART, int LK = kAllLights>
 __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
   unsigned int lo, hi;
   shade_range<PART>(a, &lo, &hi);
@@ -636,7 +638,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
       if (prim < 0) {  // Path.hs:44-47
         if (spec) {
           for (int li = 0; li < S.nlights; ++li) {
-            const float* le = light_le(S, li, ray.d);
+            const float* le = light_le<LK>(S, li, ray.d);
             if (le) {
               Xyz c;
 #pragma unroll
@@ -675,8 +677,8 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
           for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
           xyz_accum(P.rad, Pn, i, c);
         }
-        sample_one_light<KM>(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at, want_shadow,
-                         want_mis);
+        sample_one_light<KM, LK>(S, P, Pn, i, bsdf, wo, p, eps, lNumU, ld1, ld2, bComp, bd1, bd2, thr_at,
+                                 want_shadow, want_mis);
         // Russian roulette (Path.hs:68-72)
         float pc = 1.0f;
         if (depth > J.rr) {
@@ -787,8 +789,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
         for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
         xyz_accum(P.rad, Pn, i, c);
       }
-      sample_one_light<kAllKinds>(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2, thr_at, want_shadow,
-                       want_mis);
+      sample_one_light<kAllKinds, kAllLights>(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2,
+                                              thr_at, want_shadow, want_mis);
       if (depth + 1 < J.md) {  // cont (DirectLighting.hs:45-59)
         SpecVal fr, ft;
         f3 wr, wt;
@@ -1286,9 +1288,22 @@ constexpr int kWideBlock = 1024;
 
 // The shading kernel for the diffuse-only kind set (PART: all of the queue,
 // or its diffuse classes), compiled for 1, 2 or 4 waves per SIMD (a register
-// budget of 512 / waves VGPRs; what does not fit spills to scratch).
+// budget of 512 / waves VGPRs; what does not fit spills to scratch).  At the
+// default block and 2 waves it also comes in the light-kind variants of
+// a.light_mode: quad lights only (the Cornell box), or quad, sphere and
+// infinite lights.
 template <int BLOCK, int PART>
 static void launch_diffuse(const PtKernelArgs& a, int waves, dim3 g, dim3 b, hipStream_t s) {
+  if constexpr (BLOCK == 128) {
+    if (waves == 2 && a.light_mode == LIGHTS_QUAD) {
+      hipLaunchKernelGGL((k_shade<BLOCK, 2, kDiffuseKinds, PART, kQuadLights>), g, b, 0, s, a);
+      return;
+    }
+    if (waves == 2 && a.light_mode == LIGHTS_QSI) {
+      hipLaunchKernelGGL((k_shade<BLOCK, 2, kDiffuseKinds, PART, kQSILights>), g, b, 0, s, a);
+      return;
+    }
+  }
   if (waves == 4) hipLaunchKernelGGL((k_shade<BLOCK, 4, kDiffuseKinds, PART>), g, b, 0, s, a);
   else if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2, kDiffuseKinds, PART>), g, b, 0, s, a);
   else hipLaunchKernelGGL((k_shade<BLOCK, 1, kDiffuseKinds, PART>), g, b, 0, s, a);

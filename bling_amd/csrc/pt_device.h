@@ -175,45 +175,69 @@ __device__ __forceinline__ DG mk_dg1(f3 p, f3 n) {
   return DG{p, n, f.s, f.t, 0.0f, 0.0f};
 }
 
-// Object-space DG of a shape hit (Shape.hs:86-229); u/v as the quad and
-// the sphere set them, 0 for the shapes built with mkDg'.
+// The quad's DG at p (Shape.hs:157-171).
+__device__ __forceinline__ DG quad_dg(const DShape& s, f3 p) {
+  DG d = mk_dg(p, mk3(s.p[0], 0.0f, 0.0f), mk3(0.0f, s.p[1], 0.0f));
+  d.u = (s.p[0] + p.x) / (2.0f * s.p[0]);
+  d.v = (s.p[1] + p.y) / (2.0f * s.p[1]);
+  return d;
+}
+
+// The other shapes' DG at p (Shape.hs:86-229); u/v as the sphere sets
+// them, 0 for the shapes built with mkDg'.
+__device__ __forceinline__ DG box_dg(const DShape& s, f3 p, int axis) {
+  float half = (s.p[axis] + s.p[3 + axis]) / 2.0f;
+  float dir = comp3(p, axis) > half ? 1.0f : -1.0f;
+  return mk_dg1(p, norm3(set_comp(axis, dir, mk3(0, 0, 0))));
+}
+__device__ __forceinline__ DG cyl_dg(const DShape& s, f3 p) {
+  float phimax = s.p[3];
+  f3 dpdu = mk3(-(phimax * p.y), phimax * p.x, 0.0f);
+  f3 dpdv = mk3(0.0f, 0.0f, s.p[2] - s.p[1]);
+  return mk_dg1(p, norm3(cross3(dpdu, dpdv)));
+}
+__device__ __forceinline__ DG disk_dg(f3 p) { return mk_dg1(p, mk3(0.0f, 0.0f, -1.0f)); }
+__device__ __forceinline__ DG sphere_dg(const DShape& s, f3 p) {
+  float r = s.p[0];
+  const float thetaMin = kPi, thetaMax = 0.0f, phiMax = kTwoPi;
+  float theta = acosf(clampf_(p.z / r, -1.0f, 1.0f));
+  float zr = __builtin_sqrtf(p.x * p.x + p.y * p.y);
+  float iz = 1.0f / zr;
+  float cosphi = p.x * iz, sinphi = p.y * iz;
+  f3 dpdu = mk3(-(phiMax * p.y), phiMax * p.x, 0.0f);
+  f3 dpdv = scl3(mk3(p.z * cosphi, p.z * sinphi, -(r * sinf(theta))), thetaMax - thetaMin);
+  DG d = mk_dg(p, dpdu, dpdv);
+  d.u = atan2p(p.y, p.x) / phiMax;  // Shape.hs:189-192
+  d.v = (theta - thetaMin) / (thetaMax - thetaMin);
+  return d;
+}
+
+// Object-space DG of a shape hit (Shape.hs:86-229).
 __device__ __forceinline__ DG shape_dg(const DShape& s, const DRay& ray, float t, int axis) {
   f3 p = ray_at(ray, t);
   switch (s.type) {
-    case SH_BOX: {
-      float half = (s.p[axis] + s.p[3 + axis]) / 2.0f;
-      float dir = comp3(p, axis) > half ? 1.0f : -1.0f;
-      return mk_dg1(p, norm3(set_comp(axis, dir, mk3(0, 0, 0))));
-    }
-    case SH_CYL: {
-      float phimax = s.p[3];
-      f3 dpdu = mk3(-(phimax * p.y), phimax * p.x, 0.0f);
-      f3 dpdv = mk3(0.0f, 0.0f, s.p[2] - s.p[1]);
-      return mk_dg1(p, norm3(cross3(dpdu, dpdv)));
-    }
+    case SH_BOX:
+      return box_dg(s, p, axis);
+    case SH_CYL:
+      return cyl_dg(s, p);
     case SH_DISK:
-      return mk_dg1(p, mk3(0.0f, 0.0f, -1.0f));
-    case SH_QUAD: {  // Shape.hs:157-171
-      DG d = mk_dg(p, mk3(s.p[0], 0.0f, 0.0f), mk3(0.0f, s.p[1], 0.0f));
-      d.u = (s.p[0] + p.x) / (2.0f * s.p[0]);
-      d.v = (s.p[1] + p.y) / (2.0f * s.p[1]);
-      return d;
-    }
-    default: {
-      float r = s.p[0];
-      const float thetaMin = kPi, thetaMax = 0.0f, phiMax = kTwoPi;
-      float theta = acosf(clampf_(p.z / r, -1.0f, 1.0f));
-      float zr = __builtin_sqrtf(p.x * p.x + p.y * p.y);
-      float iz = 1.0f / zr;
-      float cosphi = p.x * iz, sinphi = p.y * iz;
-      f3 dpdu = mk3(-(phiMax * p.y), phiMax * p.x, 0.0f);
-      f3 dpdv = scl3(mk3(p.z * cosphi, p.z * sinphi, -(r * sinf(theta))), thetaMax - thetaMin);
-      DG d = mk_dg(p, dpdu, dpdv);
-      d.u = atan2p(p.y, p.x) / phiMax;  // Shape.hs:189-192
-      d.v = (theta - thetaMin) / (thetaMax - thetaMin);
-      return d;
-    }
+      return disk_dg(p);
+    case SH_QUAD:
+      return quad_dg(s, p);
+    default:
+      return sphere_dg(s, p);
   }
+}
+// shape_dg for shapes whose kind is in SK (see shape_intersect_k).
+template <int SK>
+__device__ __forceinline__ DG shape_dg_k(const DShape& s, const DRay& ray, float t, int axis) {
+  constexpr bool one = one_shape_kind<SK>();
+  f3 p = ray_at(ray, t);
+  if ((SK >> SH_QUAD & 1) && (one || s.type == SH_QUAD)) return quad_dg(s, p);
+  if ((SK >> SH_SPHERE & 1) && (one || s.type == SH_SPHERE)) return sphere_dg(s, p);
+  if ((SK >> SH_BOX & 1) && (one || s.type == SH_BOX)) return box_dg(s, p, axis);
+  if ((SK >> SH_DISK & 1) && (one || s.type == SH_DISK)) return disk_dg(p);
+  return cyl_dg(s, p);
 }
 
 // Geometric (dgg) and shading (dgs) DG at a hit, world space.
@@ -903,19 +927,39 @@ __device__ __forceinline__ float sample_bsdf_flt(const Bsdf& b, f3 woW, int flag
 }
 
 // -------------------------------------------------------------- lights
+// Light kinds a shading variant compiles (bit SH_x: area lights on shape x;
+// bit kLightInf: infinite lights).  The routines below take the set as a
+// template argument: the variant for scenes whose lights are all quads
+// carries none of the other shapes' sampling, pdf and intersection code and
+// their registers.  The caller guarantees that every light it passes is of a
+// kind in the set.
+constexpr int kAllLights = kAllShapes | (1 << kLightInf);
+template <int LK>
+__device__ __forceinline__ constexpr int light_shapes() {
+  return LK & kAllShapes;
+}
+template <int LK>
+__device__ __forceinline__ constexpr bool has_area_light() {
+  return light_shapes<LK>() != 0;
+}
+template <int LK>
+__device__ __forceinline__ constexpr bool has_inf_light() {
+  return ((LK >> kLightInf) & 1) != 0;
+}
+template <int SK = kAllShapes>
 __device__ __forceinline__ float shape_area(const DShape& s) {  // Shape.hs:314-328
-  switch (s.type) {
-    case SH_BOX: {
-      float h = s.p[3] - s.p[0], w = s.p[4] - s.p[1], l = s.p[5] - s.p[2];
-      return 2.0f * (h * w + h * l + w * l);
-    }
-    case SH_CYL: return 2.0f * kPi * s.p[0] * (s.p[2] - s.p[1]);
-    case SH_DISK: return kPi * (s.p[1] * s.p[1] - s.p[2] * s.p[2]);
-    case SH_QUAD: return 4.0f * s.p[0] * s.p[1];
-    default: return s.p[0] * s.p[0] * 4.0f * kPi;
+  constexpr bool one = one_shape_kind<SK>();
+  if ((SK >> SH_QUAD & 1) && (one || s.type == SH_QUAD)) return 4.0f * s.p[0] * s.p[1];
+  if ((SK >> SH_SPHERE & 1) && (one || s.type == SH_SPHERE)) return s.p[0] * s.p[0] * 4.0f * kPi;
+  if ((SK >> SH_BOX & 1) && (one || s.type == SH_BOX)) {
+    float h = s.p[3] - s.p[0], w = s.p[4] - s.p[1], l = s.p[5] - s.p[2];
+    return 2.0f * (h * w + h * l + w * l);
   }
+  if ((SK >> SH_DISK & 1) && (one || s.type == SH_DISK)) return kPi * (s.p[1] * s.p[1] - s.p[2] * s.p[2]);
+  return 2.0f * kPi * s.p[0] * (s.p[2] - s.p[1]);
 }
 __device__ __forceinline__ float cone_pdf(float c) { return c >= 1.0f ? 0.0f : 1.0f / (kTwoPi * (1.0f - c)); }
+template <int LK = kAllLights>
 __device__ __forceinline__ float general_pdf(const DShape& s, f3 p, f3 wi) {  // Shape.hs:346-350
   DRay r;
   r.o = p;
@@ -924,18 +968,22 @@ __device__ __forceinline__ float general_pdf(const DShape& s, f3 p, f3 wi) {  //
   r.tmax = INFINITY;
   float t;
   int ax;
-  if (!shape_intersect(s, r, &t, &ax)) return 0.0f;
-  DG d = shape_dg(s, r, t, ax);
-  float pd = sqlen3(sub3(p, ray_at(r, t))) / (fabsf(dot3(d.n, neg3(wi))) * shape_area(s));
+  constexpr int SK = light_shapes<LK>();
+  if (!shape_intersect_k<SK>(s, r, &t, &ax)) return 0.0f;
+  DG d = shape_dg_k<SK>(s, r, t, ax);
+  float pd = sqlen3(sub3(p, ray_at(r, t))) / (fabsf(dot3(d.n, neg3(wi))) * shape_area<SK>(s));
   return isinf(pd) ? 0.0f : pd;
 }
+template <int LK = kAllLights>
 __device__ __forceinline__ float shape_pdf(const DShape& s, f3 p, f3 wi) {  // Shape.hs:333-344
-  if (s.type == SH_SPHERE && !(sqlen3(p) - s.p[0] * s.p[0] < 1e-4f)) {
+  constexpr int SK = light_shapes<LK>();
+  constexpr bool one = one_shape_kind<SK>();
+  if ((SK >> SH_SPHERE & 1) && (one || s.type == SH_SPHERE) && !(sqlen3(p) - s.p[0] * s.p[0] < 1e-4f)) {
     float r = s.p[0];
     float st2 = r * r / sqlen3(p);
     return cone_pdf(__builtin_sqrtf(hmax(0.0f, 1.0f - st2)));
   }
-  return general_pdf(s, p, wi);
+  return general_pdf<LK>(s, p, wi);
 }
 __device__ __forceinline__ void remap_rand(int segs, float u, int* seg, float* u2) {
   float fs = (float)segs;
@@ -944,65 +992,69 @@ __device__ __forceinline__ void remap_rand(int segs, float u, int* seg, float* u
   *seg = sg;
   *u2 = (u - (float)sg / fs) * fs;
 }
-__device__ __forceinline__ void sample_shape(const DShape& s, f3 p, float u1, float u2, f3* ps, f3* ns) {
-  switch (s.type) {
-    case SH_SPHERE: {
-      float r = s.p[0];
-      if (!(sqlen3(p) - r * r < 1e-4f)) {  // Shape.hs:364-375
-        f3 dn = norm3(neg3(p));
-        Frame cs = coord_system(dn);
-        float ctm = __builtin_sqrtf(hmax(0.0f, 1.0f - (r * r) / sqlen3(p)));
-        float ct = lerpf_(u1, ctm, 1.0f);
-        float st = __builtin_sqrtf(1.0f - ct * ct);
-        float phi = u2 * kTwoPi;
-        f3 d = add3(add3(scl3(cs.s, cosf(phi) * st), scl3(cs.t, sinf(phi) * st)), scl3(cs.n, ct));
-        DRay ray;
-        ray.o = p;
-        ray.d = d;
-        ray.tmin = 0.0f;
-        ray.tmax = INFINITY;
-        float t;
-        int ax;
-        f3 q = shape_intersect(s, ray, &t, &ax) ? ray_at(ray, t) : scl3(dn, r);
-        *ps = q;
-        *ns = norm3(q);
-        return;
-      }
-      f3 q = uniform_sphere(u1, u2);
-      *ps = scl3(q, r);
-      *ns = q;
-      return;
-    }
-    case SH_BOX: {
-      int axis, nf;
-      float a, c;
-      remap_rand(3, u1, &axis, &a);
-      remap_rand(2, u2, &nf, &c);
-      *ns = set_comp(axis, (float)nf * 2.0f - 1.0f, mk3(0, 0, 0));
-      int oa0 = (axis + 1) % 3, oa1 = (axis + 2) % 3;
-      f3 pmin = mk3(s.p[0], s.p[1], s.p[2]), pmax = mk3(s.p[3], s.p[4], s.p[5]);
-      f3 base = nf == 0 ? pmin : pmax;
-      base = set_comp(oa1, lerpf_(c, comp3(pmin, oa1), comp3(pmax, oa1)), base);
-      *ps = set_comp(oa0, lerpf_(a, comp3(pmin, oa0), comp3(pmax, oa0)), base);
-      return;
-    }
-    case SH_CYL: {
-      float z = lerpf_(u1, s.p[1], s.p[2]), phi = lerpf_(u2, 0.0f, kTwoPi);
-      *ps = mk3(s.p[0] * cosf(phi), s.p[0] * sinf(phi), z);
-      *ns = norm3(mk3(ps->x, ps->y, 0.0f));
-      return;
-    }
-    case SH_DISK: {
-      float r = lerpf_(u1, s.p[2], s.p[1]), phi = lerpf_(u2, 0.0f, s.p[3]);
-      *ps = mk3(r * cosf(phi), r * sinf(phi), s.p[0]);
-      *ns = mk3(0.0f, 0.0f, -1.0f);
-      return;
-    }
-    default:
-      *ps = mk3(lerpf_(u1, -s.p[0], s.p[0]), lerpf_(u2, -s.p[1], s.p[1]), 0.0f);
-      *ns = mk3(0.0f, 0.0f, -1.0f);
-      return;
+// The quad's sampleShape (Shape.hs:404-407).
+__device__ __forceinline__ void quad_sample(const DShape& s, float u1, float u2, f3* ps, f3* ns) {
+  *ps = mk3(lerpf_(u1, -s.p[0], s.p[0]), lerpf_(u2, -s.p[1], s.p[1]), 0.0f);
+  *ns = mk3(0.0f, 0.0f, -1.0f);
+}
+// The other shapes' sampleShape (Shape.hs:352-410).
+__device__ __forceinline__ void sphere_sample(const DShape& s, f3 p, float u1, float u2, f3* ps, f3* ns) {
+  float r = s.p[0];
+  if (!(sqlen3(p) - r * r < 1e-4f)) {  // Shape.hs:364-375
+    f3 dn = norm3(neg3(p));
+    Frame cs = coord_system(dn);
+    float ctm = __builtin_sqrtf(hmax(0.0f, 1.0f - (r * r) / sqlen3(p)));
+    float ct = lerpf_(u1, ctm, 1.0f);
+    float st = __builtin_sqrtf(1.0f - ct * ct);
+    float phi = u2 * kTwoPi;
+    f3 d = add3(add3(scl3(cs.s, cosf(phi) * st), scl3(cs.t, sinf(phi) * st)), scl3(cs.n, ct));
+    DRay ray;
+    ray.o = p;
+    ray.d = d;
+    ray.tmin = 0.0f;
+    ray.tmax = INFINITY;
+    float t;
+    int ax;
+    f3 q = sphere_intersect(s, ray, &t, &ax) ? ray_at(ray, t) : scl3(dn, r);
+    *ps = q;
+    *ns = norm3(q);
+    return;
   }
+  f3 q = uniform_sphere(u1, u2);
+  *ps = scl3(q, r);
+  *ns = q;
+}
+__device__ __forceinline__ void box_sample(const DShape& s, float u1, float u2, f3* ps, f3* ns) {
+  int axis, nf;
+  float a, c;
+  remap_rand(3, u1, &axis, &a);
+  remap_rand(2, u2, &nf, &c);
+  *ns = set_comp(axis, (float)nf * 2.0f - 1.0f, mk3(0, 0, 0));
+  int oa0 = (axis + 1) % 3, oa1 = (axis + 2) % 3;
+  f3 pmin = mk3(s.p[0], s.p[1], s.p[2]), pmax = mk3(s.p[3], s.p[4], s.p[5]);
+  f3 base = nf == 0 ? pmin : pmax;
+  base = set_comp(oa1, lerpf_(c, comp3(pmin, oa1), comp3(pmax, oa1)), base);
+  *ps = set_comp(oa0, lerpf_(a, comp3(pmin, oa0), comp3(pmax, oa0)), base);
+}
+__device__ __forceinline__ void cyl_sample(const DShape& s, float u1, float u2, f3* ps, f3* ns) {
+  float z = lerpf_(u1, s.p[1], s.p[2]), phi = lerpf_(u2, 0.0f, kTwoPi);
+  *ps = mk3(s.p[0] * cosf(phi), s.p[0] * sinf(phi), z);
+  *ns = norm3(mk3(ps->x, ps->y, 0.0f));
+}
+__device__ __forceinline__ void disk_sample(const DShape& s, float u1, float u2, f3* ps, f3* ns) {
+  float r = lerpf_(u1, s.p[2], s.p[1]), phi = lerpf_(u2, 0.0f, s.p[3]);
+  *ps = mk3(r * cosf(phi), r * sinf(phi), s.p[0]);
+  *ns = mk3(0.0f, 0.0f, -1.0f);
+}
+template <int LK = kAllLights>
+__device__ __forceinline__ void sample_shape(const DShape& s, f3 p, float u1, float u2, f3* ps, f3* ns) {
+  constexpr int SK = light_shapes<LK>();
+  constexpr bool one = one_shape_kind<SK>();
+  if ((SK >> SH_QUAD & 1) && (one || s.type == SH_QUAD)) quad_sample(s, u1, u2, ps, ns);
+  else if ((SK >> SH_SPHERE & 1) && (one || s.type == SH_SPHERE)) sphere_sample(s, p, u1, u2, ps, ns);
+  else if ((SK >> SH_BOX & 1) && (one || s.type == SH_BOX)) box_sample(s, u1, u2, ps, ns);
+  else if ((SK >> SH_DISK & 1) && (one || s.type == SH_DISK)) disk_sample(s, u1, u2, ps, ns);
+  else cyl_sample(s, u1, u2, ps, ns);
 }
 
 // Dist1D arrays at `d`: n, func[n], cdf[n+1], fi ; Dist2D: nu, nv, marginal, then nv conditionals.
@@ -1050,6 +1102,7 @@ struct LSample {
   DRay ray;
   float pdf;
 };
+template <int LK = kAllLights>
 __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 pW, float e, float u1,
                                                 float u2) {
   const DLight& l = S.lights[li];
@@ -1061,7 +1114,7 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
   ls.ray.tmin = 0.0f;
   ls.ray.tmax = 1.0f;
   ls.pdf = 0.0f;
-  if (l.type == LT_INF) {  // Light.hs:130-141
+  if (has_inf_light<LK>() && l.type == LT_INF) {  // Light.hs:130-141
     const float* d = S.dist + l.dist_off;
     float v, pdf1, u, pdf0;
     int off, o2;
@@ -1083,24 +1136,26 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
     ls.pdf = mapPdf / (2.0f * kPi * kPi * sint);
     return ls;
   }
+  if constexpr (!has_area_light<LK>()) return ls;
   const DShape& s = S.geo.shapes[l.shape];  // Light.hs:152-160
   const float* l2w = S.shapex[l.shape].o2w;
   f3 p = xform_point(s.w2o, pW);
   f3 ps, ns;
-  sample_shape(s, p, u1, u2, &ps, &ns);
+  sample_shape<LK>(s, p, u1, u2, &ps, &ns);
   f3 wi = norm3(sub3(ps, p));
   ls.li = dot3(ns, wi) < 0.0f ? l.L : nullptr;
   ls.wi = xform_vector(l2w, wi);
-  ls.pdf = shape_pdf(s, p, wi);
+  ls.pdf = shape_pdf<LK>(s, p, wi);
   ls.ray.o = xform_point(l2w, p);
   ls.ray.d = xform_vector(l2w, wi);
   ls.ray.tmin = e;
   ls.ray.tmax = __builtin_sqrtf(sqlen3(sub3(ps, p))) - e;
   return ls;
 }
+template <int LK = kAllLights>
 __device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3 wi) {  // Light.hs:215-229
   const DLight& l = S.lights[li];
-  if (l.type == LT_INF) {
+  if (has_inf_light<LK>() && l.type == LT_INF) {
     f3 v = xform_vector(l.w2l, wi);
     float theta = acosf(hmax(-1.0f, hmin(1.0f, v.z)));
     float pp = atan2f(v.y, v.x);
@@ -1121,11 +1176,14 @@ __device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3
     float pd = (c[1 + iu] * marg[1 + iv]) / (cfi * mfi);
     return pd / (2.0f * kPi * kPi * sint);
   }
+  if constexpr (!has_area_light<LK>()) return 0.0f;
   const DShape& s = S.geo.shapes[l.shape];
-  return shape_pdf(s, xform_point(s.w2o, p), xform_vector(s.w2o, wi));
+  return shape_pdf<LK>(s, xform_point(s.w2o, p), xform_vector(s.w2o, wi));
 }
 // le light ray (Light.hs:98-106): nullptr for area lights
+template <int LK = kAllLights>
 __device__ __forceinline__ const float* light_le(const DevPtScene& S, int li, f3 dir) {
+  if constexpr (!has_inf_light<LK>()) return nullptr;
   const DLight& l = S.lights[li];
   if (l.type != LT_INF) return nullptr;
   f3 wh = norm3(xform_vector(l.w2l, dir));

@@ -334,6 +334,9 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     pt.shape_light_host[s] = (int32_t)lights.size();
     lights.push_back(o);
   }
+  pt.light_kinds = 0;
+  for (const DLight& l : lights)
+    pt.light_kinds |= l.type == LT_INF ? (1 << kLightInf) : (1 << d->shapes[l.shape].type);
   if (lights.empty()) lights.push_back(DLight{});
   if (maps.empty()) maps.push_back(0.0f);
   if (dist.empty()) dist.push_back(0.0f);

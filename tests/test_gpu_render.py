@@ -215,12 +215,13 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=4)
     base, _ = render_gpu(gpu_ctx, flat, spec)
     defaults = {k: gpu_ctx.config_get(k) for k in ("pt.block", "pt.pool", "pt.shade_waves", "pt.persistent",
-                                                   "pt.sort", "pt.diffuse_waves", "pt.stack", "pt.walk")}
+                                                   "pt.sort", "pt.diffuse_waves", "pt.stack", "pt.walk",
+                                                   "pt.light_variants")}
     try:
         for key, val in (("pt.shade_waves", 0), ("pt.block", 64), ("pt.block", 256), ("pt.pool", 4096),
                          ("pt.persistent", 0), ("pt.persistent", 1), ("pt.sort", 0), ("pt.sort", 1),
                          ("pt.diffuse_waves", 0), ("pt.diffuse_waves", 4), ("pt.stack", 8), ("pt.stack", 24),
-                         ("pt.walk", 16), ("pt.walk", 1)):
+                         ("pt.walk", 16), ("pt.walk", 1), ("pt.light_variants", 0)):
             gpu_ctx.config_set(key, val)
             assert gpu_ctx.config_get(key) == val
             img, _ = render_gpu(gpu_ctx, flat, spec)
@@ -248,6 +249,44 @@ def test_tuning_knobs_keep_results(gpu_ctx):
     finally:
         for k, v in defaults.items():
             gpu_ctx.config_set(k, v)
+
+
+def test_light_kind_variants_keep_results(gpu_ctx):
+    """The diffuse shading kernel's light-kind variants (quad lights only;
+    quad, sphere and infinite lights) give the all-kinds kernel's image:
+    the Cornell box (one quad light), a lit room (a quad and a sphere
+    light), a room under a sky map with a quad light, and one lit by a box
+    (every kind)."""
+    from bling_amd.scene import Material, SceneBuilder, rgb_illum, rgb_refl
+    from bling_amd import transform as T
+
+    def room(extra):
+        sb = SceneBuilder()
+        scenes._lit_room(sb, 2.0)
+        m = sb.material(Material.matte(rgb_refl(0.5, 0.6, 0.7)))
+        sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((0.0, 0.5, 0.0)), material=m)
+        extra(sb, m)
+        sb.look(T.look_at((0.0, 2.0, -6.0), (0.0, 0.5, 0.0), (0, 1, 0)), 50.0, 32, 32)
+        return sb.build()
+
+    flats = {"quad": scenes.cornell(32),
+             "quad+sphere": room(lambda sb, m: None),
+             "sky": room(lambda sb, m: sb.infinite_light(np.tile(rgb_illum(0.3, 0.3, 0.3), (4, 8, 1)))),
+             "box": room(lambda sb, m: sb.shape(abi.SHAPE_BOX, (-0.2, 0.0, -0.2, 0.2, 0.4, 0.2),
+                                                  o2w=T.translate((1.0, 0.0, 0.5)), material=m,
+                                                  emission=rgb_illum(2.0, 2.0, 2.0)))}
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=8)
+    try:
+        for name, flat in flats.items():
+            gpu_ctx.config_set("pt.light_variants", 1)
+            a, _ = render_gpu(gpu_ctx, flat, spec)
+            gpu_ctx.config_set("pt.light_variants", 0)
+            b, _ = render_gpu(gpu_ctx, flat, spec)
+            assert np.array_equal(a[..., 0], b[..., 0]), name
+            np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5, err_msg=name)
+            assert a[..., 2].sum() > 0, name
+    finally:
+        gpu_ctx.config_set("pt.light_variants", 1)
 
 
 def test_divergent_materials_matches_oracle(gpu_ctx):

@@ -204,6 +204,8 @@ MUTATIONS = [
     (mutant.GRAD_SWAP, "gradient_blend"),
     (mutant.BLEND_SWAP, "gradient_blend"),
     (mutant.IMAGE_FLIP, "image"),
+    (mutant.FBM_FREQ, "fbm"),
+    (mutant.CHECKER_TRUNC, "selectors_scale"),
 ]
 
 
