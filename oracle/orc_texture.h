@@ -42,9 +42,11 @@ constexpr const int* kNoisePerms = bgpu::kNoisePermTable;
 // test-only): 0 none; kMutPerlinWeight noiseWeight taken as t; kMutCellOwn
 // only the point's own cell searched; kMutGradSwap the gradient's weights
 // swapped; kMutCrystalWrap no mirroring of odd integer parts; kMutBlendSwap
-// the blend's weights swapped; kMutImageFlip the image's rows not flipped.
+// the blend's weights swapped; kMutImageFlip the image's rows not flipped;
+// kMutFbmFreq octave frequencies 2^k instead of 1.99^k; kMutCheckerTrunc the
+// checker's cells truncated toward zero instead of floored.
 enum { kMutPerlinWeight = 14, kMutCellOwn = 15, kMutGradSwap = 16, kMutCrystalWrap = 17, kMutBlendSwap = 18,
-       kMutImageFlip = 19 };
+       kMutImageFlip = 19, kMutFbmFreq = 20, kMutCheckerTrunc = 21 };
 inline int& texMutant() {
   static int m = 0;
   return m;
@@ -101,7 +103,7 @@ inline float fbm(int octaves, float omega, V3 p) {
   float s = 0.0f, l = 1.0f, o = 1.0f;
   for (int k = 0; k < octaves; ++k) {
     s = s + o * perlin3d(p.x * l, p.y * l, p.z * l);
-    l = 1.99f * l;
+    l = (texMutant() == kMutFbmFreq ? 2.0f : 1.99f) * l;
     o = omega * o;
   }
   return s;
@@ -276,7 +278,10 @@ inline Spec16 spectrum(const Textures& T, int node, const TexDG& dg) {
       return out;
     }
     case BGPU_TN_CHECKER: {  // checkerBoard (Texture.hs:209-219)
-      long long s = floorLL(dg.p.x * n.q[0]) + floorLL(dg.p.y * n.q[1]) + floorLL(dg.p.z * n.q[2]);
+      long long s = texMutant() == kMutCheckerTrunc
+                            ? (long long)std::trunc(dg.p.x * n.q[0]) + (long long)std::trunc(dg.p.y * n.q[1]) +
+                                  (long long)std::trunc(dg.p.z * n.q[2])
+                            : floorLL(dg.p.x * n.q[0]) + floorLL(dg.p.y * n.q[1]) + floorLL(dg.p.z * n.q[2]);
       return ((s % 2) + 2) % 2 == 0 ? spectrum(T, n.a, dg) : spectrum(T, n.b, dg);
     }
     case BGPU_TN_GRADIENT: {  // gradient (Texture.hs:236-247)
