@@ -13,7 +13,7 @@ The grammar follows the reference's Parsec parser:
   IO/CameraParser.hs          perspective camera (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
-                              transMatte, substrate, blackbody; every texture
+                              transMatte, substrate, blackbody, bumpMap; every texture
                               (constant, image, cellNoise, fbm, perlin,
                               crystal, scale, blend, checker, gradient,
                               graphPaper; uv, planar and identity mappings):
@@ -39,7 +39,7 @@ Two kinds of error:
   Unsupported       valid `.bling` that the GPU path does not render
                     (waveFront, fractals, Bezier patches, point and
                     directional lights, sun/sky and image light maps, the
-                    environment camera, bumpMap, shinyMetal).
+                    environment camera, shinyMetal).
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -523,16 +523,16 @@ class _Parser:
         return ("graphpaper", t.mapping.kind, [t.q[0]] + list(t.mapping.params), t.kids[0].spec, t.kids[1].spec)
 
     @classmethod
-    def make_material(cls, make, spectra, scalars):
+    def make_material(cls, make, spectra, scalars, bump=None):
         """The material `make(spectra, scalars)` with its inputs from
         textures.  Constants stay constants; when every textured input is a
         checker or graph paper over constants in spectrum slot 0 or 1 and all
         of them use the same selector, the material takes the selector
-        texture (bgpu_texture); any other texture makes the material's
-        textured inputs node textures, evaluated per hit."""
+        texture (bgpu_texture); any other texture, or a bump map, makes the
+        material's textured inputs node textures, evaluated per hit."""
         zero = np.zeros(BANDS, F)
         sel = [cls._selector(t) for t in spectra]
-        simple = all(t.op == abi.TN_SCONST for t in scalars) and \
+        simple = bump is None and all(t.op == abi.TN_SCONST for t in scalars) and \
             all(t.op == abi.TN_CONST or (sel[k] is not None and k < 2) for k, t in enumerate(spectra))
         if simple:
             tex = [v for v in sel if v is not None]
@@ -542,7 +542,8 @@ class _Parser:
             m = make([t.spec if t.op == abi.TN_CONST else zero for t in spectra],
                      [t.q[0] if t.op == abi.TN_SCONST else 0.0 for t in scalars])
             return m.textured_inputs(spec={k: t for k, t in enumerate(spectra) if t.op != abi.TN_CONST},
-                                     scalar={k: t for k, t in enumerate(scalars) if t.op != abi.TN_SCONST})
+                                     scalar={k: t for k, t in enumerate(scalars) if t.op != abi.TN_SCONST},
+                                     bump=bump)
         m = make([v[3] if v is not None else t.spec for v, t in zip(sel, spectra)], [t.q[0] for t in scalars])
         if not tex:
             return m
@@ -556,37 +557,46 @@ class _Parser:
             out.alt = [np.asarray(alt[0], F), np.asarray(alt[1], F)]
         return out
 
-    def material_body(self):
-        """pMaterial (MaterialParser.hs:26-104): each material's spectrum
-        and scalar inputs in the slots of bgpu_material."""
+    def material_body(self, bump=None):
+        """pMaterial' (MaterialParser.hs:30-104): each material's spectrum
+        and scalar inputs in the slots of bgpu_material; `bumpMap bump {d}
+        <material>` bump-maps the material it wraps (pBumpMap, :44-48)."""
         t = self.word()
         if t == "blackbody":
             return Material.black()
+        if t == "bumpMap":
+            d = self.scalar_texture("bump")
+            if bump is not None:
+                raise Unsupported("a bump map over a bump-mapped material")
+            return self.material_body(bump=d)
         st, sc = self.spectrum_texture, self.scalar_texture
+
+        def mk(make, spectra, scalars):
+            return self.make_material(make, spectra, scalars, bump)
         if t == "matte":
             kd = st("kd")
-            return self.make_material(lambda s, x: Material.matte(s[0], x[0]), [kd], [sc("sigma")])
+            return mk(lambda s, x: Material.matte(s[0], x[0]), [kd], [sc("sigma")])
         if t == "plastic":
             kd, ks = st("kd"), st("ks")
-            return self.make_material(lambda s, x: Material.plastic(s[0], s[1], x[0]), [kd, ks], [sc("rough")])
+            return mk(lambda s, x: Material.plastic(s[0], s[1], x[0]), [kd, ks], [sc("rough")])
         if t == "metal":
             eta, k = st("eta"), st("k")
-            return self.make_material(lambda s, x: Material.metal(s[0], s[1], x[0]), [eta, k], [sc("rough")])
+            return mk(lambda s, x: Material.metal(s[0], s[1], x[0]), [eta, k], [sc("rough")])
         if t == "glass":
             ior = sc("ior")
             kr, kt = st("kr"), st("kt")
-            return self.make_material(lambda s, x: Material.glass(x[0], s[0], s[1]), [kr, kt], [ior])
+            return mk(lambda s, x: Material.glass(x[0], s[0], s[1]), [kr, kt], [ior])
         if t == "mirror":
-            return self.make_material(lambda s, x: Material.mirror(s[0]), [st("kr")], [])
+            return mk(lambda s, x: Material.mirror(s[0]), [st("kr")], [])
         if t == "transMatte":  # pMatteTranslucent (MaterialParser.hs:92-97)
             kr, kt = st("kr"), st("kt")
-            return self.make_material(lambda s, x: Material.trans_matte(s[0], s[1], x[0]), [kr, kt], [sc("ks")])
+            return mk(lambda s, x: Material.trans_matte(s[0], s[1], x[0]), [kr, kt], [sc("ks")])
         if t == "substrate":  # pSubstrateMaterial (MaterialParser.hs:50-58)
             kd, ks, ka = st("kd"), st("ks"), st("ka")
             ur, vr, depth = sc("urough"), sc("vrough"), sc("depth")
-            return self.make_material(lambda s, x: Material.substrate(s[0], s[1], s[2], x[0], x[1], x[2]),
-                                 [kd, ks, ka], [ur, vr, depth])
-        if t in ("bumpMap", "shinyMetal"):
+            return mk(lambda s, x: Material.substrate(s[0], s[1], s[2], x[0], x[1], x[2]),
+                      [kd, ks, ka], [ur, vr, depth])
+        if t == "shinyMetal":
             raise Unsupported("material %r" % t)
         self.fail("unknown material type " + t)
 

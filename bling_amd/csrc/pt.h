@@ -71,10 +71,10 @@ struct DevPtScene {
 
 // The inputs of a material whose inputs are textures: its constant inputs,
 // which slots are textured (bgpu_texture::slots) and their roots and range
-// starts.
+// starts (index 8: the bump displacement).
 struct DMatSrc {
   int32_t type, slots;
-  int32_t roots[8], lo[8];
+  int32_t roots[9], lo[9];
   float scalar[4];
   float spec[3][16];
 };

@@ -272,9 +272,9 @@ __global__ __launch_bounds__(BLOCK) void k_generate(PtKernelArgs a) {
         int ix = J.view.ext_x0 + pix % J.view.ext_w, iy = J.view.ext_y0 + pix / J.view.ext_w;
         Sampler sm{pixel_key(J.seed, (uint32_t)pass, ix, iy), (uint32_t)J.nu, (uint32_t)J.nv,
                    (uint32_t)J.N, 4u * (uint32_t)J.sd, 3u * (uint32_t)J.sd};
-        float ox, oy, lu, lv;
+        float ox, oy, lu = 0.0f, lv = 0.0f;
         sm.camera((uint32_t)n, &ox, &oy);
-        sm.lens((uint32_t)n, &lu, &lv);
+        if (J.view.lens_radius > 0.0f) sm.lens((uint32_t)n, &lu, &lv);  // a pinhole camera does not read it
         float sx = (float)ix + ox, sy = (float)iy + oy;
         DRay r = fire_ray(J.view, sx, sy, lu, lv);
         const int Pn = P.P;
@@ -580,6 +580,27 @@ __global__ __launch_bounds__(BLOCK) void k_texture(PtKernelArgs a) {
     }
     DMaterial dm;
     build_material(in, dm);
+    if ((src.slots >> 8) & 1) {  // bump d dgg dgs (Reflection.hs:347-383)
+      const float du = 0.01f, dv = 0.01f;
+      const float uDisp =
+          tx::eval_scalar(T, src.lo[8], src.roots[8], tx::TexPt{add3(dgs.p, scl3(dgs.dpdu, du)), dgs.u + du, dgs.v});
+      const float vDisp =
+          tx::eval_scalar(T, src.lo[8], src.roots[8], tx::TexPt{add3(dgs.p, scl3(dgs.dpdv, dv)), dgs.u, dgs.v + dv});
+      const float disp = tx::eval_scalar(T, src.lo[8], src.roots[8], t);
+      const float vscale = (vDisp - disp) / dv;
+      const f3 dpdv = add3(dgs.dpdv, scl3(dgs.n, vscale));
+      const float uscale = (uDisp - disp) / du;
+      const f3 dpdu = add3(dgs.dpdu, scl3(dgs.n, uscale));
+      f3 nb = norm3(cross3(dpdu, dpdv));
+      if (dot3(nb, dgg.n) < 0.0f) nb = mk3(0.0f - nb.x, 0.0f - nb.y, 0.0f - nb.z);  // faceForward, Num's 0 - v
+      dm.tex_map = 1;  // make_bsdf takes the bumped normal and dpdu from tex[0..5]
+      dm.tex[0] = nb.x;
+      dm.tex[1] = nb.y;
+      dm.tex[2] = nb.z;
+      dm.tex[3] = dpdu.x;
+      dm.tex[4] = dpdu.y;
+      dm.tex[5] = dpdu.z;
+    }
     P.pmat[i] = dm;
   });
 }

@@ -770,6 +770,10 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevPtScene& S, int material, con
   if (b.m && b.m->tex_kind == BGPU_TEX_NODES) b.m = pm;
   else if (b.m && b.m->tex_kind != BGPU_TEX_NONE && tex_selects_alt(*b.m, dgs)) b.m = &S.mats[b.m->alt];
   f3 nn = dgs.n, sn = norm3(dgs.dpdu);
+  if (b.m && b.m == pm && pm->tex_map == 1) {  // bumpMapped (Reflection.hs:344-345): the bumped shading DG
+    nn = mk3(pm->tex[0], pm->tex[1], pm->tex[2]);
+    sn = norm3(mk3(pm->tex[3], pm->tex[4], pm->tex[5]));
+  }
   b.cs = Frame{sn, cross3(nn, sn), nn};
   b.p = dgs.p;
   b.ng = dgg.n;

@@ -211,7 +211,7 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     const bgpu_texture& tx = m.tex;
     if (tx.kind == BGPU_TEX_NONE) continue;
     if (tx.kind == BGPU_TEX_NODES) {  // inputs evaluated per hit by the texture kernel
-      if (tx.slots <= 0 || (tx.slots & ~0xf7) != 0)
+      if (tx.slots <= 0 || (tx.slots & ~0x1f7) != 0)
         return fail(BGPU_E_INVALID, "material " + std::to_string(i) + ": bad textured slots");
       DMatSrc src;
       std::memset(&src, 0, sizeof(src));
@@ -219,7 +219,7 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
       src.slots = tx.slots;
       std::memcpy(src.scalar, m.scalar, sizeof(src.scalar));
       std::memcpy(src.spec, m.spec, sizeof(src.spec));
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < 9; ++k) {  // spectra 0..2, scalars 4..7, the bump displacement 8
         if (!((tx.slots >> k) & 1)) continue;
         const int r = tx.roots[k];
         if (r < 0 || (uint32_t)r >= nn ||

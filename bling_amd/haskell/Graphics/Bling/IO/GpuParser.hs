@@ -172,12 +172,15 @@ captureTexture ref t = liftIO $ atomicModifyIORef' ref (\b -> let (i, b') = emit
          in add (m2 (node tnGraphPaper) m) { tnA = k1, tnB = k2, tnQ = q4 [lw] } b2
 
 -- | The material Texture whose spectrum slots k (0..2) and scalar slots k
--- (0..3) take the textures rooted at the given node ids (bgpu_texture of
--- kind BGPU_TEX_NODES: the roots share the params' storage).
-nodeTexture :: [(Int, Int32)] -> [(Int, Int32)] -> Texture
-nodeTexture spec scalar = Texture texNodes 0 slots 0 (map root [0 .. 11]) [replicate 16 0, replicate 16 0] where
-   slots = sum [2 ^ k | (k, _) <- spec] + sum [2 ^ (4 + k) | (k, _) <- scalar]
-   ids = [(k, r) | (k, r) <- spec] ++ [(4 + k, r) | (k, r) <- scalar]
+-- (0..3) take the textures rooted at the given node ids, bump-mapped by the
+-- scalar texture rooted at the last argument's id when there is one
+-- (bumpMapped, Reflection.hs:344-383; the case "bumpMap" of pMaterial',
+-- IO/MaterialParser.hs:44-48).  bgpu_texture of kind BGPU_TEX_NODES: the
+-- roots share the params' storage.
+nodeTexture :: [(Int, Int32)] -> [(Int, Int32)] -> Maybe Int32 -> Texture
+nodeTexture spec scalar bump = Texture texNodes 0 slots 0 (map root [0 .. 11]) [replicate 16 0, replicate 16 0] where
+   slots = sum [2 ^ k | (k, _) <- spec] + sum [2 ^ (4 + k) | (k, _) <- scalar] + maybe 0 (const 256) bump
+   ids = [(k, r) | (k, r) <- spec] ++ [(4 + k, r) | (k, r) <- scalar] ++ maybe [] (\r -> [(8, r)]) bump
    root k = castWord32ToFloat (fromIntegral (maybe (-1) id (lookup k ids)))
 
 -- | mkTriangleMesh o2w mat p i n uv (Primitive/TriangleMesh.hs:39-59): the

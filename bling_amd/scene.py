@@ -126,7 +126,7 @@ class Material:
         self.tex_params = [0.0] * 12
         self.alt = [const(0), const(0)]
         # inputs taken from node textures (TEX_NODES): slot -> textures.Tex
-        self.node_spec, self.node_scalar = {}, {}
+        self.node_spec, self.node_scalar, self.node_bump = {}, {}, None
 
     def textured(self, slot, kind, params, alt, mapping=abi.MAP_UV):
         """This material with spectrum slot `slot` (0 or 1) textured: `kind`
@@ -140,20 +140,24 @@ class Material:
         m.alt[slot] = np.asarray(alt, F)
         return m
 
-    def textured_inputs(self, spec=None, scalar=None):
+    def textured_inputs(self, spec=None, scalar=None, bump=None):
         """This material with inputs taken from textures (textures.Tex,
         evaluated per hit): `spec` maps spectrum slots (0..2) and `scalar`
-        scalar slots (0..3) to textures; the other inputs stay constant."""
+        scalar slots (0..3) to textures; the other inputs stay constant.
+        `bump`: a scalar texture the material is bump-mapped by
+        (bumpMapped, Reflection.hs:344-383)."""
         spec, scalar = dict(spec or {}), dict(scalar or {})
-        if not spec and not scalar:
+        if not spec and not scalar and bump is None:
             return self
         if any(k not in (0, 1, 2) or t.scalar for k, t in spec.items()) or \
                 any(k not in (0, 1, 2, 3) or not t.scalar for k, t in scalar.items()):
             raise ValueError("spectrum slots 0..2 take spectrum textures, scalar slots 0..3 scalar ones")
+        if bump is not None and not bump.scalar:
+            raise ValueError("a bump map takes a scalar texture")
         m = Material(self.kind, self.spec0, self.spec1, self.s0, self.s1, self.spec2, self.s2)
         m.tex_kind = abi.TEX_NODES
-        m.tex_slots = sum(1 << k for k in spec) + sum(1 << (4 + k) for k in scalar)
-        m.node_spec, m.node_scalar = spec, scalar
+        m.tex_slots = sum(1 << k for k in spec) + sum(1 << (4 + k) for k in scalar) + (256 if bump is not None else 0)
+        m.node_spec, m.node_scalar, m.node_bump = spec, scalar, bump
         return m
 
     @staticmethod
@@ -393,6 +397,8 @@ class FlatScene:
                     roots[k] = textures.emit(tex, tex_nodes, tex_images)
                 for k, tex in m.node_scalar.items():
                     roots[4 + k] = textures.emit(tex, tex_nodes, tex_images)
+                if m.node_bump is not None:
+                    roots[8] = textures.emit(m.node_bump, tex_nodes, tex_images)
         meshes = b.meshes or [(0, 0)]
         self.meshes = (abi.Mesh * len(meshes))()
         for i, (mat, fl) in enumerate(meshes):

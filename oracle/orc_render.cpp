@@ -58,7 +58,10 @@ enum {
   MUT_PATH_PC = 10,        // path throughput: 1 / pc dropped
   MUT_PATH_T = 11,         // path throughput: t' = f (the old t dropped)
   MUT_POWER = 12,          // balance heuristic instead of the power heuristic
-  MUT_FILTER_CLIP = 13     // film splats not clipped at the tile origin
+  MUT_FILTER_CLIP = 13,    // film splats not clipped at the tile origin
+  // 14..21: the texture mistakes of orc_texture.h
+  MUT_BUMP_FACE = 22,      // bump: the normal not faced toward the geometric one
+  MUT_BUMP_SHIFT = 23      // bump: the u-shifted point moved along dpdv
 };
 static int g_mutant = MUT_NONE;
 
@@ -914,15 +917,45 @@ static bool texSelectsTex2(const bgpu_texture& t, const DG& dg) {
 
 static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1, Bsdf& b);
 
-static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs) {
+// bump d dgg dgs (Reflection.hs:347-383): the shading DG displaced along its
+// normal by the scalar texture rooted at `root`, sampled at dgs and at dgs
+// shifted by 0.01 in u and in v; the new normal faces the geometric one.
+static DG bumpDG(const Ctx& c, int root, const DG& dgg, const DG& dgs) {
+  const float du = 0.01f, dv = 0.01f;
+  const tex::TexDG t0{dgs.p, dgs.u, dgs.v};
+  const tex::TexDG tu{dgs.p + scale(g_mutant == MUT_BUMP_SHIFT ? dgs.dpdv : dgs.dpdu, du), dgs.u + du, dgs.v};
+  const tex::TexDG tv{dgs.p + scale(dgs.dpdv, dv), dgs.u, dgs.v + dv};
+  const float uDisp = tex::scalar(c.s->textures, root, tu);
+  const float vDisp = tex::scalar(c.s->textures, root, tv);
+  const float disp = tex::scalar(c.s->textures, root, t0);
+  const float vscale = (vDisp - disp) / dv;
+  const V3 dpdv = dgs.dpdv + scale(dgs.n, vscale);
+  const float uscale = (uDisp - disp) / du;
+  const V3 dpdu = dgs.dpdu + scale(dgs.n, uscale);
+  V3 nn = normalize(cross(dpdu, dpdv));
+  if (dot(nn, dgg.n) < 0.0f && g_mutant != MUT_BUMP_FACE)
+    nn = mk(0.0f - nn.x, 0.0f - nn.y, 0.0f - nn.z);  // faceForward, Num's 0 - v
+  DG r = dgs;
+  r.n = nn;
+  r.dpdu = dpdu;
+  r.dpdv = dpdv;
+  return r;
+}
+
+static Bsdf makeBsdf(const Ctx& c, int material, const DG& dgg, const DG& dgs_in) {
   Bsdf b;
   b.n = 0;
+  const bool valid = material >= 0 && material < (int)c.s->materials.size();
+  // bumpMapped d mat dgg dgs = mat dgg (bump d dgg dgs) (Reflection.hs:344-345)
+  const bool bumped = valid && c.s->materials[material].tex.kind == BGPU_TEX_NODES &&
+                      ((c.s->materials[material].tex.slots >> 8) & 1);
+  const DG dgs = bumped ? bumpDG(c, c.s->materials[material].tex.roots[8], dgg, dgs_in) : dgs_in;
   // mkBsdf (Reflection.hs:209-218)
   V3 nn = dgs.n, sn = normalize(dgs.dpdu);
   b.cs = Frame{sn, cross(nn, sn), nn};
   b.p = dgs.p;
   b.ng = dgg.n;
-  if (material < 0 || material >= (int)c.s->materials.size()) return b;
+  if (!valid) return b;
   const bgpu_material& m = c.s->materials[material];
   if (m.tex.kind == BGPU_TEX_NODES) {
     // textured inputs evaluated at the shading DG (`tex dgs`), the others
@@ -1500,6 +1533,37 @@ int orc_tex_eval(void* scene, int node, int n, const float* xyz, const float* uv
     } else {
       out[k] = orc::tex::scalar(s->textures, node, dg);
     }
+  }
+  return 0;
+}
+
+// The bumped shading DG of material `material` (bumpMapped, its bump texture
+// at roots[8]) for a shading DG g = p, n, dpdu, dpdv, u, v (14 floats) and a
+// geometric normal ng: out = n, dpdu, dpdv (9 floats).  Returns 0, or -1 when
+// the material is not bump-mapped.
+int orc_bump(void* scene, int material, const float* g, const float* ng, float* out) {
+  using namespace orc;
+  const Scene* s = static_cast<const Scene*>(scene);
+  if (material < 0 || material >= (int)s->materials.size()) return -1;
+  const bgpu_material& m = s->materials[material];
+  if (m.tex.kind != BGPU_TEX_NODES || !((m.tex.slots >> 8) & 1)) return -1;
+  Ctx c;
+  c.s = s;
+  DG dgs{};
+  dgs.p = mk(g[0], g[1], g[2]);
+  dgs.n = mk(g[3], g[4], g[5]);
+  dgs.dpdu = mk(g[6], g[7], g[8]);
+  dgs.dpdv = mk(g[9], g[10], g[11]);
+  dgs.u = g[12];
+  dgs.v = g[13];
+  DG dgg = dgs;
+  dgg.n = mk(ng[0], ng[1], ng[2]);
+  const DG r = bumpDG(c, m.tex.roots[8], dgg, dgs);
+  const V3 v[3] = {r.n, r.dpdu, r.dpdv};
+  for (int k = 0; k < 3; ++k) {
+    out[3 * k] = v[k].x;
+    out[3 * k + 1] = v[k].y;
+    out[3 * k + 2] = v[k].z;
   }
   return 0;
 }

@@ -321,12 +321,12 @@ Scene* buildScene(const bgpu_scene_desc* d, const char** err) {
   if (const char* e = tex::validate(s->textures)) { *err = e; delete s; return nullptr; }
   for (const bgpu_material& m : s->materials)
     if (m.tex.kind == BGPU_TEX_NODES)
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < 9; ++k) {  // spectra 0..2, scalars 4..7, the bump displacement 8
         if (!((m.tex.slots >> k) & 1)) continue;
         const int r = m.tex.roots[k];
         const bool ok = k != 3 && r >= 0 && r < (int)d->num_tex_nodes &&
                         (k < 3 ? tex::isSpectrumOp(d->tex_nodes[r].op) : tex::isScalarOp(d->tex_nodes[r].op));
-        if (!ok || (m.tex.slots >> 8) != 0) { *err = "bad texture root"; delete s; return nullptr; }
+        if (!ok || (m.tex.slots >> 9) != 0) { *err = "bad texture root"; delete s; return nullptr; }
       }
   // mkKdTree: bounds = foldl' mappend mempty (map worldBounds ps)  (KdTree.hs:111)
   s->bounds = emptyAABB();

@@ -71,6 +71,19 @@ class OracleScene:
             raise ValueError("not a texture node: %d" % node)
         return out
 
+    def bump(self, material, g, ng):
+        """bump d dgg dgs for bump-mapped `material` at the shading DG g = (p,
+        n, dpdu, dpdv, u, v) (14 floats) with geometric normal ng: the bumped
+        (n, dpdu, dpdv) as a (3, 3) array."""
+        lib = load()
+        lib.orc_bump.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        g = np.ascontiguousarray(g, np.float32).reshape(14)
+        ng = np.ascontiguousarray(ng, np.float32).reshape(3)
+        out = np.zeros(9, np.float32)
+        if lib.orc_bump(self.h, material, g.ctypes.data, ng.ctypes.data, out.ctypes.data) != 0:
+            raise ValueError("material %d is not bump-mapped" % material)
+        return out.reshape(3, 3)
+
     def __del__(self):
         try:
             self.close()
@@ -244,7 +257,7 @@ class mutant:
     FR_DIEL_ETA, FR_COND_K, BLINN_NORM, MF_G, TRANS_ETA2, TRANS_DIR, OREN_SWAP = 1, 2, 3, 4, 5, 6, 7
     ENV_SIN, ENV_ROW, PATH_PC, PATH_T, POWER, FILTER_CLIP = 8, 9, 10, 11, 12, 13
     PERLIN_WEIGHT, CELL_OWN, GRAD_SWAP, CRYSTAL_WRAP, BLEND_SWAP, IMAGE_FLIP = 14, 15, 16, 17, 18, 19
-    FBM_FREQ, CHECKER_TRUNC = 20, 21
+    FBM_FREQ, CHECKER_TRUNC, BUMP_FACE, BUMP_SHIFT = 20, 21, 22, 23
 
     def __init__(self, m):
         self.m = m

@@ -274,3 +274,32 @@ def node_texture_scene(res=48):
     c2w = T.look_at((0.0, 3.5, -5.0), (0.0, 0.5, 0.3), (0, 1, 0))
     sb.look(c2w, 50.0, res, res)
     return sb.build()
+
+
+def bump_scene(res=48):
+    """Bump-mapped materials (bumpMapped, Reflection.hs:344-383): a plastic
+    sphere under Euclidean cell noise, a matte floor quad under fBm, a mesh
+    with uvs under a uv-mapped quasi-crystal and a checker-textured matte
+    under Perlin noise, lit by a quad light and a uniform sky."""
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.6)).textured_inputs(
+        bump=Tex.scale(0.0, 0.05, Tex.fbm(4, 0.5, Map3d.identity(T.scale((3.0, 3.0, 3.0)))))))
+    sb.shape(abi.SHAPE_QUAD, (4.0, 4.0), o2w=T.rotate_x(90), material=floor)
+    cell = sb.material(Material.plastic(rgb_refl(0.2, 0.5, 0.7), rgb_refl(0.5, 0.5, 0.5), 0.05).textured_inputs(
+        bump=Tex.scale(0.0, 0.2, Tex.cell("euclidian", Map3d.identity(T.scale((2.0, 2.0, 2.0)))))))
+    sb.shape(abi.SHAPE_SPHERE, (0.8,), o2w=T.translate((-1.0, 0.8, 0.2)), material=cell)
+    cry = sb.material(Material.matte(rgb_refl(0.8, 0.5, 0.3)).textured_inputs(
+        bump=Tex.scale(0.0, 0.03, Tex.crystal(5, Map2d.uv(6.0, 6.0, 0.0, 0.0)))))
+    sb.mesh(np.array([[0.2, 0.0, -0.5], [1.8, 0.0, -0.5], [1.8, 1.5, 0.2], [0.2, 1.5, 0.2]], np.float32),
+            np.array([[0, 1, 2], [0, 2, 3]], np.int32), material=cry,
+            uvs=np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32))
+    chk = sb.material(Material.matte(const(0.5)).textured_inputs(
+        spec={0: Tex.checker((4.0, 4.0, 4.0), Tex.spectrum(rgb_refl(0.9, 0.9, 0.2)),
+                             Tex.spectrum(rgb_refl(0.1, 0.1, 0.6)))},
+        bump=Tex.scale(0.0, 0.1, Tex.perlin(Map3d.identity(T.scale((5.0, 5.0, 5.0)))))))
+    sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((0.6, 0.5, 1.2)), material=chk)
+    lt = T.rotate_x(-90) @ T.translate((0.0, 4.0, 0.0))
+    sb.shape(abi.SHAPE_QUAD, (1.0, 1.0), o2w=lt, material=floor, emission=rgb_illum(8.0, 8.0, 8.0))
+    sb.infinite_light(np.tile(const(0.3), (2, 4, 1)))
+    sb.look(T.look_at((0.0, 3.0, -5.0), (0.0, 0.5, 0.3), (0, 1, 0)), 50.0, res, res)
+    return sb.build()

@@ -434,3 +434,26 @@ def test_image_textures(tmp_path):
         B.parse((scene % "c.png").replace('"g.png"', '"c.png"'), base=str(tmp_path))
     with pytest.raises(B.BlingSyntaxError, match="cannot read"):
         B.parse(scene % "missing.png", base=str(tmp_path))
+
+
+def test_bump_map_materials():
+    """`bumpMap bump {d} <material>` (MaterialParser.hs:44-48) bump-maps the
+    material it wraps: the displacement's root in slot 8, the wrapped
+    material's textures (a checker over constants included) as node
+    textures; a bump map over a bump-mapped material is refused."""
+    b = B.parse("material { bumpMap bump { scale 0.2 0.2 tex { cellNoise euclidian map { identity { scale 2 2 2 } } } } "
+                "plastic kd { checker 1 1 1 tex1 { constant rgbR 1 1 1 } tex2 { constant rgbR 0 0 0 } } "
+                "ks { constant rgbR 0.5 0.5 0.5 } rough { constant 0.1 } } prim { shape { sphere radius 1 } }")
+    m = b.flat.materials[0]
+    roots = abi.texture_roots(m.tex)
+    assert m.tex.kind == abi.TEX_NODES and m.tex.slots == (1 | 256) and m.type == abi.MAT_PLASTIC
+    assert b.flat.tex_nodes[roots[0]].op == abi.TN_CHECKER and b.flat.tex_nodes[roots[8]].op == abi.TN_SCALE
+    assert m.scalar[0] == F(0.1)
+    b = B.parse("material { bumpMap bump { constant 0.5 } matte kd { constant rgbR 0.5 0.5 0.5 } "
+                "sigma { constant 0 } } prim { shape { sphere radius 1 } }")
+    assert b.flat.materials[0].tex.slots == 256
+    with pytest.raises(B.Unsupported):
+        B.parse("material { bumpMap bump { constant 0.5 } bumpMap bump { constant 0.1 } matte "
+                "kd { constant rgbR 0.5 0.5 0.5 } sigma { constant 0 } } prim { shape { sphere radius 1 } }")
+    assert B.parse("material { bumpMap bump { constant 0.5 } blackbody } "
+                   "prim { shape { sphere radius 1 } }").flat.materials[0].type == abi.MAT_BLACK

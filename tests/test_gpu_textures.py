@@ -9,7 +9,8 @@ Texture.hs:59-414.
 - a render of a scene whose materials take node textures (every op, 2d and
   3d mappings, images, textured scalars) tracks the oracle with the same
   seed (equal sample weights, median relative pixel difference < 1e-4) and
-  is within 3 sigma of it with another seed;
+  is within 3 sigma of it with another seed; so does a scene of bump-mapped
+  materials (bumpMapped, Reflection.hs:344-383);
 - the served path (continuous batching, textured and untextured jobs in
   flight together) gives the plain path's image;
 - evaluating a node that is not a texture (out of range, a gradient step)
@@ -149,6 +150,26 @@ def test_node_texture_scene_matches_oracle(gpu_ctx):
     frac, zmean, zmax = compare(g, gsq, o2, osq2)
     assert frac < 0.015, (frac, zmean, zmax)
     assert abs(zmean) < 0.08, zmean
+
+
+def test_bump_mapped_scene_matches_oracle(gpu_ctx):
+    """Bump maps over cell noise, fBm, a quasi-crystal (uv mapping) and
+    Perlin noise on spheres, a quad and a uv mesh, one material also
+    node-textured: the same seed tracks the oracle, and the bumps change
+    the image."""
+    flat = cases.bump_scene(res=48)
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=3, rr_start_depth=3, passes=2,
+                   seed=44, variance=True)
+    g, gsq = _render(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    assert np.array_equal(g[..., 0], o[..., 0])
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
 
 
 def test_node_texture_direct_lighting_matches_oracle(gpu_ctx):

@@ -11,7 +11,11 @@ mathematics and Texture.hs's definitions fix.
   one wave is (cos x + 1) / 2; even in (x, y);
 - gradient, blend, checker, graphPaper, scale: their definitions at points
   that fix them; image: texel (px, py) at its centre, wrapped, rows from the
-  top (v negated).
+  top (v negated);
+- bump mapping: a constant displacement leaves the shading frame alone; a
+  displacement (cos x + 1) / 2 tilts dpdu by its finite difference along the
+  normal (over u, and over p for a planar mapping); the normal faces the
+  geometric one.
 Each pin fails when a plausible mistake is planted in the oracle.
 The cell hash and its Poisson table are the reference's constants; beyond
 the properties above they are parity unpinned.
@@ -187,9 +191,43 @@ def pin_mappings():
     assert np.array_equal(evaluate(tex, xs, np.zeros((4, 2), F)), spectra[0])
 
 
+def bump_frame(disp, g, ng):
+    """The oracle's bumped (n, dpdu, dpdv) for a matte bump-mapped by `disp`."""
+    sb = SceneBuilder()
+    sb.material(Material.matte(const(0.5)).textured_inputs(bump=disp))
+    sb.mesh(np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0]], F), np.array([[0, 1, 2]], np.int32))
+    o = pyoracle.OracleScene(sb.build())
+    try:
+        return o.bump(0, np.asarray(g, F), np.asarray(ng, F))
+    finally:
+        o.close()
+
+
+def pin_bump():
+    """bump d dgg dgs (Reflection.hs:347-383) on a plane z = 0 with dpdu =
+    (2, 0, 0), dpdv = (0, 3, 0): a constant displacement leaves the frame
+    alone; (cos x + 1) / 2 over x = u (uv mapping) or x = p.x (planar
+    mapping, where the u-shifted point moves by 0.01 dpdu = 0.02 in x) tilts
+    dpdu by the finite difference along n; the normal faces ng."""
+    n, dpdu, dpdv = (0, 0, 1), (2, 0, 0), (0, 3, 0)
+    for u in (0.3, 1.1, 2.5):
+        g = [0.4, 0.7, 0.0, *n, *dpdu, *dpdv, u, 0.2]
+        r = bump_frame(Tex.constant(0.3), g, n)
+        assert np.array_equal(r, np.array([n, dpdu, dpdv], F))
+        for mapping, x, h in ((Map2d.uv(1, 1, 0, 0), u, 0.01), (Map2d.planar((1, 0, 0), (0, 1, 0), 0, 0), 0.4, 0.02)):
+            r = bump_frame(Tex.crystal(1, mapping), g, n)
+            us = ((np.cos(x + h) + 1) / 2 - (np.cos(x) + 1) / 2) / 0.01
+            assert np.allclose(r[1], [2, 0, us], atol=2e-4), (r[1], us)
+            assert np.array_equal(r[2], np.array(dpdv, F))  # nothing varies along v
+            want = np.array([-3 * us, 0, 6]) / np.hypot(3 * us, 6)
+            assert np.allclose(r[0], want, atol=2e-4), (r[0], want)
+            flipped = bump_frame(Tex.crystal(1, mapping), g, (0, 0, -1))
+            assert np.allclose(flipped[0], -want, atol=2e-4)
+
+
 PINS = {"perlin": pin_perlin, "fbm": pin_fbm, "cell": pin_cell, "crystal": pin_crystal,
         "gradient_blend": pin_gradient_blend, "selectors_scale": pin_selectors_scale, "image": pin_image,
-        "mappings": pin_mappings}
+        "mappings": pin_mappings, "bump": pin_bump}
 
 
 @pytest.mark.parametrize("name", sorted(PINS))
@@ -206,6 +244,8 @@ MUTATIONS = [
     (mutant.IMAGE_FLIP, "image"),
     (mutant.FBM_FREQ, "fbm"),
     (mutant.CHECKER_TRUNC, "selectors_scale"),
+    (mutant.BUMP_FACE, "bump"),
+    (mutant.BUMP_SHIFT, "bump"),
 ]
 
 
