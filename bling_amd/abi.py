@@ -16,7 +16,8 @@ OK, E_INVALID, E_NOMEM, E_HIP, E_BUDGET, E_STATE, E_CANCELLED, E_NODEV, E_TIMEOU
     0, -1, -2, -3, -4, -5, -6, -7, -8)
 
 SHAPE_BOX, SHAPE_CYLINDER, SHAPE_DISK, SHAPE_QUAD, SHAPE_SPHERE = range(5)
-MAT_BLACK, MAT_MATTE, MAT_PLASTIC, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_TRANSMATTE, MAT_SUBSTRATE = range(8)
+MAT_BLACK, MAT_MATTE, MAT_PLASTIC, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_TRANSMATTE, MAT_SUBSTRATE, \
+    MAT_SHINYMETAL = range(9)
 MESH_NORMALS, MESH_UVS = 1, 2
 LIGHT_INFINITE = 1
 CAMERA_PERSPECTIVE, CAMERA_ENVIRONMENT = 0, 1

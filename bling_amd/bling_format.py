@@ -13,7 +13,8 @@ The grammar follows the reference's Parsec parser:
   IO/CameraParser.hs          perspective camera (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
-                              transMatte, substrate, blackbody, bumpMap; every texture
+                              transMatte, substrate, shinyMetal, blackbody,
+                              bumpMap; every texture
                               (constant, image, cellNoise, fbm, perlin,
                               crystal, scale, blend, checker, gradient,
                               graphPaper; uv, planar and identity mappings):
@@ -39,7 +40,7 @@ Two kinds of error:
   Unsupported       valid `.bling` that the GPU path does not render
                     (waveFront, fractals, Bezier patches, point and
                     directional lights, sun/sky and image light maps, the
-                    environment camera, shinyMetal).
+                    environment camera).
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -596,8 +597,9 @@ class _Parser:
             ur, vr, depth = sc("urough"), sc("vrough"), sc("depth")
             return mk(lambda s, x: Material.substrate(s[0], s[1], s[2], x[0], x[1], x[2]),
                       [kd, ks, ka], [ur, vr, depth])
-        if t == "shinyMetal":
-            raise Unsupported("material %r" % t)
+        if t == "shinyMetal":  # pShinyMetal (MaterialParser.hs:74-79)
+            kr, ks = st("kr"), st("ks")
+            return mk(lambda s, x: Material.shiny_metal(s[0], s[1], x[0]), [kr, ks], [sc("rough")])
         self.fail("unknown material type " + t)
 
     def scalar_map_2d(self):

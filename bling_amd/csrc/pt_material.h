@@ -30,6 +30,15 @@ __host__ __device__ inline float blinn_e(float rough) {
   float e = 1.0f / rough;
   return (e > 10000.0f || e != e) ? 10000.0f : e;
 }
+// frApproxEta r = (1 + r') / (1 - r'), r' = sqrt (sClamp 0 0.999 r), and
+// frApproxK r = sqrt (refl / (1 - refl)) * 2, refl = sClamp 0 0.999 r, per
+// band (Fresnel.hs:72-78)
+__host__ __device__ inline void approx_eta_k(float r, float* eta, float* k) {
+  const float refl = hmax(0.0f, hmin(0.999f, r));
+  const float rs = sqrtf(refl);
+  *eta = (1.0f + rs) / (1.0f - rs);
+  *k = sqrtf(refl / (1.0f - refl)) * 2.0f;
+}
 // Oren-Nayar's A and B of sigma (Diffuse.hs:29-40); kind Lambert at sigma 0
 __host__ __device__ inline void diffuse_lobe(DBxdf& b, float sig) {
   if (sig == 0.0f) {
@@ -134,6 +143,23 @@ __host__ __device__ inline void build_material(const MatIn& m, DMaterial& o) {
       b.e = blinn_e(hmax(0.0f, m.scalar[0]));  // mkAnisotropic (1 / u) (1 / v), u = max 0 urough
       b.ei = blinn_e(hmax(0.0f, m.scalar[1]));
       b.et = m.scalar[2];
+      break;
+    }
+    case BGPU_MAT_SHINYMETAL: {  // mkShinyMetal (Material.hs:98-109)
+      DBxdf& g = o.c[o.n++];
+      g.kind = K_MF;
+      g.type = BX_R | BX_G;
+      g.e = blinn_e(m.scalar[0]);
+      g.fr = FR_COND;
+      DBxdf& r = o.c[o.n++];
+      r.kind = K_SPECR;
+      r.type = BX_R | BX_S;
+      r.fr = FR_COND;
+      for (int k = 0; k < 16; ++k) {
+        g.r[k] = r.r[k] = 1.0f;
+        approx_eta_k(m.spec[1][k], &g.eta[k], &g.k[k]);
+        approx_eta_k(m.spec[0][k], &r.eta[k], &r.k[k]);
+      }
       break;
     }
     case BGPU_MAT_MIRROR: {

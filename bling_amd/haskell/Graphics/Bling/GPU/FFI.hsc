@@ -45,7 +45,7 @@ shapeDisk = #{const BGPU_SHAPE_DISK}
 shapeQuad = #{const BGPU_SHAPE_QUAD}
 shapeSphere = #{const BGPU_SHAPE_SPHERE}
 
-matBlack, matMatte, matPlastic, matMetal, matGlass, matMirror, matTransMatte, matSubstrate :: Int32
+matBlack, matMatte, matPlastic, matMetal, matGlass, matMirror, matTransMatte, matSubstrate, matShinyMetal :: Int32
 matBlack = #{const BGPU_MAT_BLACK}
 matMatte = #{const BGPU_MAT_MATTE}
 matPlastic = #{const BGPU_MAT_PLASTIC}
@@ -54,6 +54,7 @@ matGlass = #{const BGPU_MAT_GLASS}
 matMirror = #{const BGPU_MAT_MIRROR}
 matTransMatte = #{const BGPU_MAT_TRANSMATTE}
 matSubstrate = #{const BGPU_MAT_SUBSTRATE}
+matShinyMetal = #{const BGPU_MAT_SHINYMETAL}
 
 texNone, texChecker, texGraphPaper, texNodes, mapUV, mapPlanar, mapIdentity :: Int32
 texNone = #{const BGPU_TEX_NONE}

@@ -88,6 +88,7 @@ data GpuMaterial
    | GMirror Spectrum                      -- ^ kr
    | GTransMatte Spectrum Spectrum Float   -- ^ kr, kt, sigma
    | GSubstrate Spectrum Spectrum Spectrum Float Float Float  -- ^ kd, ks, ka, urough, vrough, depth
+   | GShinyMetal Spectrum Spectrum Float   -- ^ kr, ks, roughness
    | GBlack
 
 noTexture :: Texture
@@ -105,6 +106,7 @@ captureMaterial ref gm tex = liftIO $ atomicModifyIORef' ref (\b -> let (i, b') 
          GMirror kr -> rec matMirror [] [kr]
          GTransMatte kr kt s -> rec matTransMatte [s] [kr, kt]
          GSubstrate kd ks ka u v d -> rec matSubstrate [u, v, d] [kd, ks, ka]
+         GShinyMetal kr ks r -> rec matShinyMetal [r] [kr, ks]
          GBlack -> rec matBlack [] []
       rec t sc sp = Material t 0 (take 4 (sc ++ repeat 0)) (take 3 (map spectrumBands sp ++ repeat (replicate 16 0)))
                              (maybe noTexture id tex)

@@ -193,6 +193,13 @@ class Material:
         return Material(abi.MAT_SUBSTRATE, kd, ks, urough, vrough, ka, depth)
 
     @staticmethod
+    def shiny_metal(kr, ks, rough):
+        """mkShinyMetal kr ks rough (Material.hs:98-109): a Blinn microfacet
+        and a specular reflection with conductor Fresnel terms approximated
+        from ks and kr (frApproxEta, frApproxK, Fresnel.hs:72-78)."""
+        return Material(abi.MAT_SHINYMETAL, kr, ks, rough)
+
+    @staticmethod
     def black():
         return Material(abi.MAT_BLACK)
 

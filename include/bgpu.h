@@ -78,7 +78,7 @@ typedef struct {
   float emission[16]; /* spectral radiance when emissive */
 } bgpu_shape;
 
-/* Materials, Material.hs:32-96 (constant textures only). */
+/* Materials, Material.hs:32-127 (inputs constant, or textures: bgpu_texture). */
 enum {
   BGPU_MAT_BLACK = 0,   /* blackBodyMaterial: no BxDF (Reflection.hs:337-338) */
   BGPU_MAT_MATTE = 1,   /* spec[0]=kd, scalar[0]=sigma (Lambertian if 0, else Oren-Nayar) */
@@ -89,10 +89,14 @@ enum {
   BGPU_MAT_TRANSMATTE = 6, /* translucentMatte (Material.hs:43-53): spec[0]=kr, spec[1]=kt (both clamped
                               to [0,1], kt then scaled by 1 - kr), scalar[0]=sigma; a diffuse reflection and
                               the same lobe turned into transmission (brdfToBtdf, Reflection.hs:188-195) */
-  BGPU_MAT_SUBSTRATE = 7   /* mkSubstrate (Material.hs:111-127): spec[0]=kd, spec[1]=ks, spec[2]=ka (each
+  BGPU_MAT_SUBSTRATE = 7,  /* mkSubstrate (Material.hs:111-127): spec[0]=kd, spec[1]=ks, spec[2]=ka (each
                               clamped to [0,1]), scalar[0]=urough, scalar[1]=vrough, scalar[2]=depth; one
                               FresnelBlend BRDF over the anisotropic distribution with exponents
                               1/max(0, urough) and 1/max(0, vrough) (Microfacet.hs:56-104,136-196) */
+  BGPU_MAT_SHINYMETAL = 8  /* mkShinyMetal (Material.hs:98-109): spec[0]=kr, spec[1]=ks, scalar[0]=roughness;
+                              a Blinn microfacet with frConductor (frApproxEta ks) (frApproxK ks) and a
+                              specular reflection with frConductor (frApproxEta kr) (frApproxK kr), both
+                              white (Fresnel.hs:72-78) */
 };
 /* A texture that selects, per hit, between two constant spectra
  * (Texture.hs:191-219): spec[k] of the material is tex1's value and

@@ -61,7 +61,8 @@ enum {
   MUT_FILTER_CLIP = 13,    // film splats not clipped at the tile origin
   // 14..21: the texture mistakes of orc_texture.h
   MUT_BUMP_FACE = 22,      // bump: the normal not faced toward the geometric one
-  MUT_BUMP_SHIFT = 23      // bump: the u-shifted point moved along dpdv
+  MUT_BUMP_SHIFT = 23,     // bump: the u-shifted point moved along dpdv
+  MUT_SHINY_SQRT = 24      // frApproxEta: r' = r, the square root dropped
 };
 static int g_mutant = MUT_NONE;
 
@@ -1089,6 +1090,44 @@ static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1
         x.flip = side;
         b.c[b.n++] = x;
       }
+      break;
+    }
+    case BGPU_MAT_SHINYMETAL: {  // mkShinyMetal (Material.hs:98-109): [diff, spec]
+      // frApproxEta / frApproxK (Fresnel.hs:72-78), band by band
+      auto approxEta = [](const Spec& r) {
+        Spec o;
+        for (int i = 0; i < 16; ++i) {
+          float c = hmax(0.0f, hmin(0.999f, r[i]));
+          float rp = g_mutant == MUT_SHINY_SQRT ? c : std::sqrt(c);
+          o[i] = (1.0f + rp) / (1.0f - rp);
+        }
+        return o;
+      };
+      auto approxK = [](const Spec& r) {
+        Spec o;
+        for (int i = 0; i < 16; ++i) {
+          float refl = hmax(0.0f, hmin(0.999f, r[i]));
+          o[i] = std::sqrt(refl / (1.0f - refl)) * 2.0f;
+        }
+        return o;
+      };
+      Bxdf g{};
+      g.kind = K_MF;
+      g.type = BX_R | BX_G;
+      g.r = sconst(1.0f);
+      g.e = blinnE(m.scalar[0]);
+      g.fr = FR_COND;
+      g.eta = approxEta(s1);
+      g.k = approxK(s1);
+      b.c[b.n++] = g;
+      Bxdf r{};
+      r.kind = K_SPECR;
+      r.type = BX_R | BX_S;
+      r.r = sconst(1.0f);
+      r.fr = FR_COND;
+      r.eta = approxEta(s0);
+      r.k = approxK(s0);
+      b.c[b.n++] = r;
       break;
     }
     case BGPU_MAT_MIRROR: {  // Material.hs:68-74

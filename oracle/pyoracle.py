@@ -257,7 +257,7 @@ class mutant:
     FR_DIEL_ETA, FR_COND_K, BLINN_NORM, MF_G, TRANS_ETA2, TRANS_DIR, OREN_SWAP = 1, 2, 3, 4, 5, 6, 7
     ENV_SIN, ENV_ROW, PATH_PC, PATH_T, POWER, FILTER_CLIP = 8, 9, 10, 11, 12, 13
     PERLIN_WEIGHT, CELL_OWN, GRAD_SWAP, CRYSTAL_WRAP, BLEND_SWAP, IMAGE_FLIP = 14, 15, 16, 17, 18, 19
-    FBM_FREQ, CHECKER_TRUNC, BUMP_FACE, BUMP_SHIFT = 20, 21, 22, 23
+    FBM_FREQ, CHECKER_TRUNC, BUMP_FACE, BUMP_SHIFT, SHINY_SQRT = 20, 21, 22, 23, 24
 
     def __init__(self, m):
         self.m = m

@@ -457,3 +457,12 @@ def test_bump_map_materials():
                 "kd { constant rgbR 0.5 0.5 0.5 } sigma { constant 0 } } prim { shape { sphere radius 1 } }")
     assert B.parse("material { bumpMap bump { constant 0.5 } blackbody } "
                    "prim { shape { sphere radius 1 } }").flat.materials[0].type == abi.MAT_BLACK
+
+
+def test_shiny_metal_material():
+    """`shinyMetal kr {..} ks {..} rough {..}` (MaterialParser.hs:74-79)."""
+    b = B.parse("material { shinyMetal kr { constant rgbR 0.9 0.6 0.3 } ks { constant rgbR 0.5 0.5 0.5 } "
+                "rough { constant 0.05 } } prim { shape { sphere radius 1 } }")
+    m = b.flat.materials[0]
+    assert (m.type, m.scalar[0]) == (abi.MAT_SHINYMETAL, F(0.05))
+    assert list(m.spec[0]) == list(rgb_refl(0.9, 0.6, 0.3)) and list(m.spec[1]) == list(rgb_refl(0.5, 0.5, 0.5))

@@ -251,6 +251,40 @@ def test_tuning_knobs_keep_results(gpu_ctx):
             gpu_ctx.config_set(k, v)
 
 
+def test_shiny_metal_matches_oracle(gpu_ctx):
+    """shinyMetal spheres (mkShinyMetal, Material.hs:98-109: a microfacet and
+    a specular reflection, conductor terms approximated from ks and kr) on
+    a matte floor under a quad light and a sky: the same seed tracks the
+    oracle."""
+    from bling_amd.scene import Material, SceneBuilder, rgb_illum, rgb_refl
+    from bling_amd import transform as T
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.6)))
+    sb.shape(abi.SHAPE_QUAD, (4.0, 4.0), o2w=T.rotate_x(90), material=floor)
+    for x, (kr, ks, rough) in zip((-1.2, 0.0, 1.2), ((rgb_refl(0.9, 0.6, 0.3), rgb_refl(0.8, 0.8, 0.8), 0.02),
+                                                     (rgb_refl(0.3, 0.3, 0.3), rgb_refl(0.9, 0.7, 0.2), 0.3),
+                                                     (rgb_refl(0.95, 0.95, 0.95), rgb_refl(0.1, 0.1, 0.1), 0.0))):
+        m = sb.material(Material.shiny_metal(kr, ks, rough))
+        sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((x, 0.5, 0.0)), material=m)
+    sb.shape(abi.SHAPE_QUAD, (1.0, 1.0), o2w=T.rotate_x(-90) @ T.translate((0.0, 3.0, 0.0)), material=floor,
+             emission=rgb_illum(8.0, 8.0, 8.0))
+    sb.infinite_light(np.tile(rgb_illum(0.4, 0.4, 0.5), (2, 4, 1)))
+    sb.look(T.look_at((0.0, 1.5, -4.0), (0.0, 0.5, 0.0), (0, 1, 0)), 50.0, 48, 48)
+    flat = sb.build()
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, rr_start_depth=3, passes=2, seed=45,
+                   variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+
+
 def test_light_kind_variants_keep_results(gpu_ctx):
     """The diffuse shading kernel's light-kind variants (quad lights only;
     quad, sphere and infinite lights) give the all-kinds kernel's image:

@@ -153,6 +153,38 @@ def pin_torrance_sparrow_reciprocal():
     np.testing.assert_allclose(fa[big], fb[big], rtol=1e-4)
 
 
+def pin_shiny_metal():
+    """mkShinyMetal kr ks rough (Material.hs:98-109) is [microfacet, specRefl]
+    with conductor terms of frApproxEta / frApproxK (Fresnel.hs:72-78):
+    eta = (1 + sqrt r') / (1 - sqrt r'), k = 2 sqrt(r' / (1 - r')), r'
+    = r clamped to [0, 0.999].  With wo the normal the specular component
+    returns the textbook conductor reflectance ((eta-1)^2 + k^2) /
+    ((eta+1)^2 + k^2) of kr's eta and k, and the microfacet component
+    that reflectance of ks's times (e + 2) / 8pi.  Each approximation alone
+    reproduces r (k = 0: ((eta-1)/(eta+1))^2 = r'; eta = 1: k^2/(4 + k^2) =
+    r'), the check that the two formulas are the reflectance's inverses."""
+    rough = 0.05
+    e = 1.0 / np.float32(rough)
+    n1 = np.float32([[0, 0, 1]])
+    for kr, ks in ((0.3, 0.7), (0.95, 0.05), (1.2, 0.5)):
+        m = cases.material_abi(Material.shiny_metal(const(kr), const(ks), rough))
+        wi, f, pdf = pyoracle.bxdf_sample(m, n1, np.float32([[0.5, 0.5]]), comp=1)
+        assert np.allclose(wi, [[0, 0, 1]]) and pdf[0] == 1.0
+
+        def refl(r):
+            r = min(max(r, 0.0), 0.999)
+            eta, k = (1 + np.sqrt(r)) / (1 - np.sqrt(r)), 2 * np.sqrt(r / (1 - r))
+            assert abs(((eta - 1) / (eta + 1)) ** 2 - r) < 1e-12 and abs(k * k / (4 + k * k) - r) < 1e-12
+            return ((eta - 1) ** 2 + k * k) / ((eta + 1) ** 2 + k * k)
+        np.testing.assert_allclose(f[0], refl(kr), rtol=2e-5)
+        fm, _ = pyoracle.bxdf_eval(m, n1, n1, comp=0)
+        np.testing.assert_allclose(fm[0], refl(ks) * (e + 2) / (8 * np.pi), rtol=2e-5)
+
+
+def test_shiny_metal_closed_forms():
+    pin_shiny_metal()
+
+
 def test_blinn_distribution_is_normalised():
     pin_blinn_normalised()
 
@@ -537,6 +569,7 @@ MUTATIONS = [
     (mutant.PATH_T, lambda: pin_two_mirrors(rr_start=7)),
     (mutant.POWER, lambda: pin_two_mirrors(rr_start=7)),
     (mutant.FILTER_CLIP, pin_film_tiles),
+    (mutant.SHINY_SQRT, pin_shiny_metal),
 ]
 
 
