@@ -23,7 +23,8 @@ The grammar follows the reference's Parsec parser:
                               node texture (textures.Tex) evaluated per hit
   IO/PrimitiveParser.hs       mesh (polygon faces triangulated as fans,
                               TriangleMesh.hs:23-29) and shape primitives
-  IO/LightParser.hs           infinite lights with a constant map; emission
+  IO/LightParser.hs           infinite lights with a constant map, point and
+                              directional lights; emission
   IO/RendererParser.hs,       renderer blocks (every kind is read; only the
   IO/IntegratorParser.hs      sampler renderer with a stratified sampler and
                               the path or directLighting integrator maps
@@ -38,9 +39,8 @@ keeping its own primitive order; explicit lights are prepended the same way
 Two kinds of error:
   BlingSyntaxError  the reference's parser rejects the text as well;
   Unsupported       valid `.bling` that the GPU path does not render
-                    (waveFront, fractals, Bezier patches, point and
-                    directional lights, sun/sky and image light maps, the
-                    environment camera).
+                    (waveFront, fractals, Bezier patches, sun/sky and image
+                    light maps, the environment camera).
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -714,9 +714,15 @@ class _Parser:
                 self.fail("unknown map type " + m)
             s = self.spectrum()
             self.close()
-            self.lights.insert(0, (np.asarray(s, F).reshape(1, 1, BANDS), w2l))
-        elif t in ("point", "directional"):
-            raise Unsupported("%s light" % t)
+            self.lights.insert(0, ("infinite", np.asarray(s, F).reshape(1, 1, BANDS), w2l))
+        elif t == "point":  # pPointLight (LightParser.hs:35-39)
+            self.named("intensity")
+            r = self.spectrum()
+            self.lights.insert(0, ("point", r, self.named_vec("position")))
+        elif t == "directional":  # pDirectionalLight (LightParser.hs:41-45)
+            self.named("intensity")
+            r = self.spectrum()
+            self.lights.insert(0, ("directional", r, self.named_vec("normal")))
         else:
             self.fail("unknown light type " + t)
         self.close()
@@ -850,8 +856,13 @@ class BlingScene:
                     _, kind, params, xf, m, emit, sid = rec
                     sb.shape(kind, params, o2w=xf, material=mat(m), emission=emit)
                     sb.shapes[-1].light_id = sid
-        for radiance, w2l in p.lights:
-            sb.infinite_light(radiance, w2l)
+        for kind, a, b in p.lights:
+            if kind == "infinite":
+                sb.infinite_light(a, b)
+            elif kind == "point":
+                sb.point_light(a, b)
+            else:
+                sb.directional_light(a, b)
         _, c2w, r2c, lr, fd, fov = p.camera
         sb.perspective_camera(c2w, r2c, lens_radius=lr, focal_distance=fd)
         filt, fparams = p.filter

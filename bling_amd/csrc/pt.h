@@ -37,11 +37,12 @@ struct DMaterial {
   DBxdf c[2];
 };
 
-enum { LT_AREA = 0, LT_INF = 1 };
+enum { LT_AREA = 0, LT_INF = 1, LT_POINT = 2, LT_DIR = 3 };
 struct DLight {
   int32_t type, shape, id, w, h, map_off, dist_off, pad;
-  float L[16];
+  float L[16];  // area: radiance; point, directional: intensity
   float w2l[16], l2w[16];
+  float vec[4];  // point: position; directional: the unit direction toward the light
 };
 
 // Everything the bounce kernels need about one scene (device resident).
@@ -219,8 +220,11 @@ struct PtKernelArgs {
 // out).
 enum { LIGHTS_ALL = 0, LIGHTS_QUAD = 1, LIGHTS_QSI = 2 };
 // Bits of a light-kind set: bit k < 5 an area light on bgpu shape type k,
-// bit kLightInf an infinite light.
+// bit kLightInf an infinite light, kLightPoint a point light, kLightDir a
+// directional light.
 constexpr int kLightInf = 5;
+constexpr int kLightPoint = 6;
+constexpr int kLightDir = 7;
 constexpr int kQuadLights = 1 << BGPU_SHAPE_QUAD;
 constexpr int kQSILights = (1 << BGPU_SHAPE_QUAD) | (1 << BGPU_SHAPE_SPHERE) | (1 << kLightInf);
 

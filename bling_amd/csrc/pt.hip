@@ -461,16 +461,18 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
       lscale = (float)lc;
     }
     {  // sampleLightMis (Scene.hs:61-69)
-      LSample ls = sample_light<LK>(S, ln, p, eps, ld1, ld2);
+      LSample ls = sample_light<LK>(S, ln, p, eps, bsdf.cs.n, ld1, ld2);  // n: bsdfShadingNormal (Path.hs:61)
       if (ls.pdf != 0.0f && ls.li) {
         SpecVal f = eval_bsdf<KM>(bsdf, wo, ls.wi);
-        float w = power_heuristic(ls.pdf, bsdf_pdf<KM>(bsdf, wo, ls.wi));
-        float k = w / ls.pdf;
+        // a delta light's sample is not weighted (Scene.hs:65)
+        float k = has_delta_light<LK>() && ls.delta ? 1.0f / ls.pdf
+                                                    : power_heuristic(ls.pdf, bsdf_pdf<KM>(bsdf, wo, ls.wi)) / ls.pdf;
         Xyz c;
         bool liBlack = true, fBlack = true;
 #pragma unroll 4
         for (int b = 0; b < 16; ++b) {
           float fv = spec_at<KM>(f, b), li = ls.li[b];
+          if constexpr (has_delta_light<LK>()) li = li * ls.scale;  // sScale r s (1 for other lights: exact)
           liBlack = liBlack && li == 0.0f;
           fBlack = fBlack && fv == 0.0f;
           c.add(b, thr_at(b) * (((fv * li) * k) * lscale));
@@ -484,7 +486,9 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
         }
       }
     }
-    {  // sampleBsdfMis (Scene.hs:71-82)
+    // sampleBsdfMis (Scene.hs:71-82): black for a delta light, whose le and
+    // intLe are black and which no ray can hit
+    if (!(has_delta_light<LK>() && (S.lights[ln].type == LT_POINT || S.lights[ln].type == LT_DIR))) {
       SpecVal fb;
       f3 wib;
       int tb;

@@ -937,7 +937,7 @@ __device__ __forceinline__ float sample_bsdf_flt(const Bsdf& b, f3 woW, int flag
 // carries none of the other shapes' sampling, pdf and intersection code and
 // their registers.  The caller guarantees that every light it passes is of a
 // kind in the set.
-constexpr int kAllLights = kAllShapes | (1 << kLightInf);
+constexpr int kAllLights = kAllShapes | (1 << kLightInf) | (1 << kLightPoint) | (1 << kLightDir);
 template <int LK>
 __device__ __forceinline__ constexpr int light_shapes() {
   return LK & kAllShapes;
@@ -949,6 +949,10 @@ __device__ __forceinline__ constexpr bool has_area_light() {
 template <int LK>
 __device__ __forceinline__ constexpr bool has_inf_light() {
   return ((LK >> kLightInf) & 1) != 0;
+}
+template <int LK>
+__device__ __forceinline__ constexpr bool has_delta_light() {
+  return ((LK >> kLightPoint) & 3) != 0;
 }
 template <int SK = kAllShapes>
 __device__ __forceinline__ float shape_area(const DShape& s) {  // Shape.hs:314-328
@@ -1101,13 +1105,15 @@ __device__ __forceinline__ const float* tex_px(const DevPtScene& S, const DLight
 }
 
 struct LSample {
-  const float* li;  // radiance bands in HBM, nullptr when black
+  const float* li;  // radiance bands in HBM (times `scale`), nullptr when black
+  float scale;      // 1, or a delta light's falloff / cosine factor
+  bool delta;       // a point or directional light (Light.hs:142-150)
   f3 wi;
   DRay ray;
   float pdf;
 };
 template <int LK = kAllLights>
-__device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 pW, float e, float u1,
+__device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 pW, float e, f3 n, float u1,
                                                 float u2) {
   const DLight& l = S.lights[li];
   LSample ls;
@@ -1118,6 +1124,36 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
   ls.ray.tmin = 0.0f;
   ls.ray.tmax = 1.0f;
   ls.pdf = 0.0f;
+  ls.scale = 1.0f;
+  ls.delta = false;
+  if constexpr (has_delta_light<LK>()) {
+    if (l.type == LT_POINT) {  // Light.hs:146-150: r / |pos - p|^2 toward pos, the ray along pos - p
+      const f3 d = sub3(mk3(l.vec[0], l.vec[1], l.vec[2]), pW);
+      ls.li = l.L;
+      ls.scale = 1.0f / sqlen3(d);
+      ls.wi = norm3(d);
+      ls.ray.o = pW;
+      ls.ray.d = d;
+      ls.ray.tmin = e;
+      ls.ray.tmax = INFINITY;
+      ls.pdf = 1.0f;
+      ls.delta = true;
+      return ls;
+    }
+    if (l.type == LT_DIR) {  // Light.hs:142-144: r |n . d| from d; n is the shading normal
+      const f3 d = mk3(l.vec[0], l.vec[1], l.vec[2]);
+      ls.li = l.L;
+      ls.scale = fabsf(dot3(n, d));
+      ls.wi = d;
+      ls.ray.o = pW;
+      ls.ray.d = d;
+      ls.ray.tmin = e;
+      ls.ray.tmax = INFINITY;
+      ls.pdf = 1.0f;
+      ls.delta = true;
+      return ls;
+    }
+  }
   if (has_inf_light<LK>() && l.type == LT_INF) {  // Light.hs:130-141
     const float* d = S.dist + l.dist_off;
     float v, pdf1, u, pdf0;

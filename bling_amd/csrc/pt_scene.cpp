@@ -289,6 +289,32 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   std::vector<float> maps, dist;
   for (uint32_t i = 0; i < d->num_lights; ++i) {
     const bgpu_light& l = d->lights[i];
+    if (l.type == BGPU_LIGHT_POINT || l.type == BGPU_LIGHT_DIRECTIONAL) {  // mkPointLight, mkDirectional
+      if (!l.radiance) return fail(BGPU_E_INVALID, "point or directional light without an intensity");
+      DLight o;
+      std::memset(&o, 0, sizeof(o));
+      o.type = l.type == BGPU_LIGHT_POINT ? LT_POINT : LT_DIR;
+      o.shape = -1;
+      o.id = -1;
+      std::memcpy(o.L, l.radiance, 64);
+      float x = l.vec[0], y = l.vec[1], z = l.vec[2];
+      if (o.type == LT_DIR) {  // mkDirectional s n = Directional s (normalize n) (Light.hs:53-54)
+        const float sq = (x * x + y * y) + z * z;
+        if (sq != 0.0f) {
+          const float il = 1.0f / std::sqrt(sq);
+          x *= il;
+          y *= il;
+          z *= il;
+        } else {
+          x = 0.0f, y = 1.0f, z = 0.0f;
+        }
+      }
+      o.vec[0] = x;
+      o.vec[1] = y;
+      o.vec[2] = z;
+      lights.push_back(o);
+      continue;
+    }
     if (l.type != BGPU_LIGHT_INFINITE || l.width <= 0 || l.height <= 0 || !l.radiance)
       return fail(BGPU_E_INVALID, "unsupported or empty explicit light");
     DLight o;
@@ -336,7 +362,10 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   }
   pt.light_kinds = 0;
   for (const DLight& l : lights)
-    pt.light_kinds |= l.type == LT_INF ? (1 << kLightInf) : (1 << d->shapes[l.shape].type);
+    pt.light_kinds |= l.type == LT_INF     ? (1 << kLightInf)
+                      : l.type == LT_POINT ? (1 << kLightPoint)
+                      : l.type == LT_DIR   ? (1 << kLightDir)
+                                           : (1 << d->shapes[l.shape].type);
   if (lights.empty()) lights.push_back(DLight{});
   if (maps.empty()) maps.push_back(0.0f);
   if (dist.empty()) dist.push_back(0.0f);

@@ -29,7 +29,8 @@ namespace bgpu {
 namespace {
 
 constexpr char kMagic[8] = {'B', 'G', 'P', 'U', 'S', 'C', 'N', '1'};
-constexpr uint32_t kVersion = 3;  // 2: three spectra and four scalars per material; 3: texture nodes, images
+constexpr uint32_t kVersion = 4;  // 2: three spectra and four scalars per material; 3: texture nodes, images;
+                                  // 4: point and directional lights (bgpu_light::vec)
 constexpr uint32_t kByteOrder = 0x01020304u;
 constexpr uint32_t kHasNormals = 1, kHasUvs = 2, kHasTriMesh = 4;
 constexpr uint64_t kAlign = 64;

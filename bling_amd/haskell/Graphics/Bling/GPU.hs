@@ -107,6 +107,7 @@ gpuAddShape :: Shape -> GpuSceneBuilder -> GpuSceneBuilder
 gpuAddShape s b = b { gsbShapes = s : gsbShapes b, gsbNextId = gsbNextId b + 1 }
 
 -- | Records an infinite light and its width x height x 16 map.
+-- (point and directional lights too: their intensity is the one-texel map)
 gpuAddInfiniteLight :: Light -> VS.Vector Float -> GpuSceneBuilder -> GpuSceneBuilder
 gpuAddInfiniteLight l m b = b { gsbLights = (l, m) : gsbLights b }
 

@@ -83,10 +83,12 @@ tnGradient = #{const BGPU_TN_GRADIENT}
 tnGraphPaper = #{const BGPU_TN_GRAPHPAPER}
 tnGStep = #{const BGPU_TN_GSTEP}
 
-meshNormals, meshUVs, lightInfinite, cameraPerspective :: Int32
+meshNormals, meshUVs, lightInfinite, lightPoint, lightDirectional, cameraPerspective :: Int32
 meshNormals = #{const BGPU_MESH_NORMALS}
 meshUVs = #{const BGPU_MESH_UVS}
 lightInfinite = #{const BGPU_LIGHT_INFINITE}
+lightPoint = #{const BGPU_LIGHT_POINT}
+lightDirectional = #{const BGPU_LIGHT_DIRECTIONAL}
 cameraPerspective = #{const BGPU_CAMERA_PERSPECTIVE}
 
 filterBox, filterGauss, filterMitchell, filterSinc, filterTriangle :: Int32
@@ -238,7 +240,7 @@ instance Storable Mesh where
 -- until bgpu_scene_create returns.
 data Light = Light
   { lgType :: !Int32, lgWidth :: !Int32, lgHeight :: !Int32, lgPad :: !Int32
-  , lgRadiance :: Ptr CFloat, lgW2l :: [Float], lgL2w :: [Float] }
+  , lgRadiance :: Ptr CFloat, lgW2l :: [Float], lgL2w :: [Float], lgVec :: [Float] }
 
 instance Storable Light where
   sizeOf _ = #{size bgpu_light}
@@ -246,6 +248,7 @@ instance Storable Light where
   peek p = Light <$> #{peek bgpu_light, type} p <*> #{peek bgpu_light, width} p
     <*> #{peek bgpu_light, height} p <*> #{peek bgpu_light, pad} p <*> #{peek bgpu_light, radiance} p
     <*> peekFloats 16 (#{ptr bgpu_light, w2l} p) <*> peekFloats 16 (#{ptr bgpu_light, l2w} p)
+    <*> peekFloats 4 (#{ptr bgpu_light, vec} p)
   poke p l = do
     #{poke bgpu_light, type} p (lgType l)
     #{poke bgpu_light, width} p (lgWidth l)
@@ -254,6 +257,7 @@ instance Storable Light where
     #{poke bgpu_light, radiance} p (lgRadiance l)
     pokeFloats 16 (#{ptr bgpu_light, w2l} p) (lgW2l l)
     pokeFloats 16 (#{ptr bgpu_light, l2w} p) (lgL2w l)
+    pokeFloats 4 (#{ptr bgpu_light, vec} p) (lgVec l)
 
 -- | bgpu_camera: a ProjectiveCamera after mkProjective (Camera.hs:108-147).
 data Camera = Camera

@@ -16,7 +16,8 @@
 --     with the material then taking 'nodeTexture' of the roots,
 --     'captureMesh' beside mkTriangleMesh and
 --     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureShape' beside
---     mkGeom, 'captureInfinite' beside mkInfiniteAreaLight
+--     mkGeom, 'captureInfinite', 'capturePoint' and 'captureDirectional' beside
+--     mkInfiniteAreaLight, mkPointLight and mkDirectional
 --     (IO/LightParser.hs:29-33), 'captureView' where the camera and filter
 --     are set (IO/CameraParser.hs:30-38, IO/ParserCore.hs).
 --
@@ -27,7 +28,8 @@
 -- accessors a maintainer exports (Spectrum.hs:46, Transform.hs:123-127).
 module Graphics.Bling.IO.GpuParser (
    newGpuScene, pGpuRenderer, GpuMaterial(..), captureMaterial, captureMesh, captureShape,
-   captureInfinite, captureView, GpuTex(..), GpuMap2d(..), captureTexture, nodeTexture
+   captureInfinite, capturePoint, captureDirectional, captureView, GpuTex(..), GpuMap2d(..), captureTexture,
+   nodeTexture
    ) where
 
 import Control.Monad.IO.Class (liftIO)
@@ -218,7 +220,22 @@ captureInfinite :: IORef GpuSceneBuilder -> Transform -> Int -> Int -> [Spectrum
 captureInfinite ref w2l w h texels = liftIO $ modifyIORef' ref (gpuAddInfiniteLight l m) where
    (w2lRows, l2wRows) = transformRows w2l
    l = Light lightInfinite (fromIntegral w) (fromIntegral h) 0 nullPtr (concat w2lRows) (concat l2wRows)
+             [0, 0, 0, 0]
    m = VS.fromList (concatMap spectrumBands texels)
+
+-- | mkPointLight intensity position and mkDirectional intensity normal
+-- (Light.hs:53-61), beside pPointLight / pDirectionalLight
+-- (IO/LightParser.hs:35-47): the intensity as a one-texel map, the vector
+-- as given.
+capturePoint, captureDirectional :: IORef GpuSceneBuilder -> Spectrum -> Vector -> JobParser ()
+capturePoint ref r p = captureDelta ref lightPoint r p
+captureDirectional ref r n = captureDelta ref lightDirectional r n
+
+captureDelta :: IORef GpuSceneBuilder -> Int32 -> Spectrum -> Vector -> JobParser ()
+captureDelta ref ty r v = liftIO $ modifyIORef' ref (gpuAddInfiniteLight l m) where
+   ident = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
+   l = Light ty 1 1 0 nullPtr ident ident [vx v, vy v, vz v, 0]
+   m = VS.fromList (spectrumBands r)
 
 -- | The camera after mkProjective (Camera.hs:108-147) and the film
 -- (imageSize and filter, Filter.hs:12-96).

@@ -298,6 +298,28 @@ class SceneBuilder:
             l.l2w[k] = w2l.inv[k]
         self.lights.append((l, m))
 
+    def point_light(self, intensity, position):
+        """mkPointLight intensity position (Light.hs:57-61): a delta light
+        whose radiance falls off with the squared distance."""
+        self._delta_light(abi.LIGHT_POINT, intensity, position)
+
+    def directional_light(self, intensity, normal):
+        """mkDirectional intensity normal (Light.hs:53-54): a delta light
+        from direction `normal` (normalized by the library)."""
+        self._delta_light(abi.LIGHT_DIRECTIONAL, intensity, normal)
+
+    def _delta_light(self, kind, intensity, vec):
+        l = abi.Light()
+        l.type = kind
+        l.width = l.height = 1
+        ident = Transform()
+        for k in range(16):
+            l.w2l[k] = ident.m[k]
+            l.l2w[k] = ident.inv[k]
+        for k in range(3):
+            l.vec[k] = float(F(vec[k]))
+        self.lights.append((l, np.ascontiguousarray(np.asarray(intensity, F).reshape(1, 16))))
+
     def perspective_camera(self, c2w, r2c, lens_radius=0.0, focal_distance=1.0):
         cam = abi.Camera()
         cam.type = abi.CAMERA_PERSPECTIVE

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BGPU_ABI_VERSION 2
+#define BGPU_ABI_VERSION 3
 #define BGPU_NUM_BANDS 16 /* Spectrum.hs:34-36 */
 
 /* ---- error codes ------------------------------------------------------ */
@@ -188,9 +188,17 @@ typedef struct {
   int32_t flags;
 } bgpu_mesh;
 
-/* Explicit (non-geometric) lights.  Only the infinite area light with a
- * discrete spectral map is on the path (Light.hs:72-82,104-106,130-141,220-225). */
-enum { BGPU_LIGHT_INFINITE = 1 };
+/* Explicit (non-geometric) lights (Light.hs:30-229):
+ *   INFINITE     mkInfiniteAreaLight with a discrete spectral map of width x
+ *                height texels and its world-to-light transform;
+ *   POINT        mkPointLight: `radiance` is the 16-band intensity (width =
+ *                height = 1), `vec` the position;
+ *   DIRECTIONAL  mkDirectional: `radiance` is the 16-band intensity (width =
+ *                height = 1), `vec` the normal as given (the library
+ *                normalizes it as mkDirectional does).
+ * Point and directional lights are delta lights: sampled with pdf 1 and
+ * never hit by rays (Scene.hs:61-82). */
+enum { BGPU_LIGHT_INFINITE = 1, BGPU_LIGHT_POINT = 2, BGPU_LIGHT_DIRECTIONAL = 3 };
 typedef struct {
   int32_t type;
   int32_t width, height; /* texSize of the map */
@@ -198,6 +206,7 @@ typedef struct {
   const float* radiance; /* width*height*16 floats, pixel (x,y) at ((y*width)+x)*16 */
   float w2l[16];         /* world-to-light matrix, row-major */
   float l2w[16];         /* its inverse */
+  float vec[4];          /* point: position; directional: normal (xyz, w unused) */
 } bgpu_light;
 
 enum { BGPU_CAMERA_PERSPECTIVE = 0, BGPU_CAMERA_ENVIRONMENT = 1 };

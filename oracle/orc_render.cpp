@@ -62,7 +62,9 @@ enum {
   // 14..21: the texture mistakes of orc_texture.h
   MUT_BUMP_FACE = 22,      // bump: the normal not faced toward the geometric one
   MUT_BUMP_SHIFT = 23,     // bump: the u-shifted point moved along dpdv
-  MUT_SHINY_SQRT = 24      // frApproxEta: r' = r, the square root dropped
+  MUT_SHINY_SQRT = 24,     // frApproxEta: r' = r, the square root dropped
+  MUT_POINT_FALLOFF = 25,  // point light: the 1 / |pos - p|^2 falloff dropped
+  MUT_DIR_COS = 26         // directional light: the |n . d| factor dropped
 };
 static int g_mutant = MUT_NONE;
 
@@ -383,10 +385,11 @@ struct Dist2D {
 
 // ------------------------------------------------------------------ lights
 struct LightRec {
-  int type;  // 0 area (shape index), 1 infinite
+  int type;  // 0 area (shape index), 1 infinite, 2 point, 3 directional
   int shape;
   int id;
-  Spec L;
+  Spec L;  // area: radiance; point, directional: intensity
+  V3 vec;  // point: position; directional: normalize n
   // infinite
   int w, h;
   const float* map;
@@ -421,6 +424,7 @@ struct LightSample {
   V3 wi;
   Ray ray;
   float pdf;
+  bool delta = false;
 };
 
 static float shapeArea(const Shape& s) {  // Shape.hs:314-328
@@ -529,8 +533,25 @@ struct Ctx {
   std::atomic<uint64_t> rays{0};
 };
 
-static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, float u1, float u2) {
+static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, V3 n, float u1, float u2) {
   LightSample ls;
+  if (l.type == 2) {  // PointLight (Light.hs:146-150)
+    const V3 d = l.vec - pW;
+    ls.li = g_mutant == MUT_POINT_FALLOFF ? l.L : sscale(l.L, 1.0f / sqLen(d));
+    ls.wi = normalize(d);
+    ls.ray = Ray{pW, d, e, kInf};  // along pos - p, not normalised, to infinity
+    ls.pdf = 1.0f;
+    ls.delta = true;
+    return ls;
+  }
+  if (l.type == 3) {  // Directional (Light.hs:142-144)
+    ls.li = g_mutant == MUT_DIR_COS ? l.L : sscale(l.L, std::fabs(dot(n, l.vec)));
+    ls.wi = l.vec;
+    ls.ray = Ray{pW, l.vec, e, kInf};
+    ls.pdf = 1.0f;
+    ls.delta = true;
+    return ls;
+  }
   if (l.type == 1) {  // Light.hs:130-141
     float uv[2], mapPdf;
     float pdf1, pdf0;
@@ -566,6 +587,7 @@ static LightSample sampleLight(const Ctx& c, const LightRec& l, V3 pW, float e, 
 }
 
 static float lightPdf(const Ctx& c, const LightRec& l, V3 p, V3 wi) {  // Light.hs:215-229
+  if (l.type == 2 || l.type == 3) return 0.0f;  // point, directional: no direction is sampled by chance
   if (l.type == 1) {
     V3 v = transVector(l.w2l, wi);
     float theta = std::acos(hmax(-1.0f, hmin(1.0f, v.z)));  // sphericalTheta (Math.hs:159-161)
@@ -1227,13 +1249,17 @@ static Spec estimateDirect(Ctx& c, int li, V3 p, float eps, V3 n, V3 wo, const B
   const LightRec& l = c.lights[li];
   Spec ls = sconst(0);
   {  // sampleLightMis (Scene.hs:61-69)
-    LightSample s = sampleLight(c, l, p, eps, ulDir1, ulDir2);
+    LightSample s = sampleLight(c, l, p, eps, n, ulDir1, ulDir2);
     Spec f = evalBsdf(bsdf, wo, s.wi);
     if (!(s.pdf == 0.0f || isBlack(s.li) || isBlack(f))) {
       c.rays++;
       if (!sceneOccluded(*c.s, s.ray, false)) {
-        float w = powerHeuristic(s.pdf, bsdfPdf(bsdf, wo, s.wi));
-        ls = sscale(f * s.li, w / s.pdf);
+        if (s.delta) {
+          ls = sscale(f * s.li, 1.0f / s.pdf);
+        } else {
+          float w = powerHeuristic(s.pdf, bsdfPdf(bsdf, wo, s.wi));
+          ls = sscale(f * s.li, w / s.pdf);
+        }
       }
     }
   }
@@ -1257,7 +1283,6 @@ static Spec estimateDirect(Ctx& c, int li, V3 p, float eps, V3 n, V3 wo, const B
       }
     }
   }
-  (void)n;
   return ls + bs;
 }
 
@@ -1648,7 +1673,7 @@ void orc_envlight(int w, int h, const float* map, const float* w2l, const float*
   for (int i = 0; i < n; ++i) {
     Spec le;
     if (op == 0) {
-      LightSample ls = sampleLight(c, l, mk(0, 0, 0), 0.0f, u[2 * i], u[2 * i + 1]);
+      LightSample ls = sampleLight(c, l, mk(0, 0, 0), 0.0f, mk(0, 0, 1), u[2 * i], u[2 * i + 1]);
       wi[3 * i] = ls.wi.x, wi[3 * i + 1] = ls.wi.y, wi[3 * i + 2] = ls.wi.z;
       pdf[i] = ls.pdf;
       le = ls.li;
@@ -1727,6 +1752,14 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
     l.type = 1;
     l.shape = -1;
     l.id = -1;
+    if (s.lights[i].type == BGPU_LIGHT_POINT || s.lights[i].type == BGPU_LIGHT_DIRECTIONAL) {  // Light.hs:53-61
+      l.type = s.lights[i].type == BGPU_LIGHT_POINT ? 2 : 3;
+      l.L = specOf(s.light_maps[i].data());
+      V3 v = mk(s.lights[i].vec[0], s.lights[i].vec[1], s.lights[i].vec[2]);
+      l.vec = l.type == 3 ? normalize(v) : v;  // mkDirectional s n = Directional s (normalize n)
+      c.lights.push_back(l);
+      continue;
+    }
     l.w = s.lights[i].width;
     l.h = s.lights[i].height;
     l.map = s.light_maps[i].data();
@@ -1734,7 +1767,8 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
     std::memcpy(l.l2w.m, s.lights[i].l2w, sizeof(l.l2w.m));
     c.lights.push_back(l);
   }
-  for (auto& l : c.lights) l.dist = buildDist(l);
+  for (auto& l : c.lights)
+    if (l.type == 1) l.dist = buildDist(l);
   c.shapeLight.assign(s.shapes.size(), -1);
   for (size_t i = 0; i < s.shapes.size(); ++i) {
     if (!s.shapes[i].emissive) continue;

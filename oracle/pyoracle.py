@@ -258,6 +258,7 @@ class mutant:
     ENV_SIN, ENV_ROW, PATH_PC, PATH_T, POWER, FILTER_CLIP = 8, 9, 10, 11, 12, 13
     PERLIN_WEIGHT, CELL_OWN, GRAD_SWAP, CRYSTAL_WRAP, BLEND_SWAP, IMAGE_FLIP = 14, 15, 16, 17, 18, 19
     FBM_FREQ, CHECKER_TRUNC, BUMP_FACE, BUMP_SHIFT, SHINY_SQRT = 20, 21, 22, 23, 24
+    POINT_FALLOFF, DIR_COS = 25, 26
 
     def __init__(self, m):
         self.m = m

@@ -303,3 +303,23 @@ def bump_scene(res=48):
     sb.infinite_light(np.tile(const(0.3), (2, 4, 1)))
     sb.look(T.look_at((0.0, 3.0, -5.0), (0.0, 0.5, 0.3), (0, 1, 0)), 50.0, res, res)
     return sb.build()
+
+
+def delta_light_over_plane(kind, rho=0.5, I=3.0, D=2.0, theta=60.0, res=4, fov=0.1):
+    """A Lambertian plane z = 0 lit by one delta light, seen near the origin
+    by a narrow camera: a point light of intensity I at height D (Lo = rho I
+    / (pi D^2) below it), or a directional light of intensity I from
+    `theta` degrees off the normal (Lo = rho I cos^2(theta) / pi: the light's
+    |n . d| (Light.hs:142-144) and the BSDF's cosine)."""
+    sb = SceneBuilder()
+    m = sb.material(Material.matte(const(rho)))
+    sb.shape(abi.SHAPE_QUAD, (1000.0, 1000.0), material=m)
+    if kind == "point":
+        sb.point_light(const(I), (0.0, 0.0, D))
+    else:
+        t = np.radians(theta)
+        sb.directional_light(const(I), (2.0 * np.sin(t), 0.0, 2.0 * np.cos(t)))  # normalized by the library
+    c2w = T.look_at((3.0, 0.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1))
+    sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
+    sb.set_film(res, res)
+    return sb.build()

@@ -466,3 +466,18 @@ def test_shiny_metal_material():
     m = b.flat.materials[0]
     assert (m.type, m.scalar[0]) == (abi.MAT_SHINYMETAL, F(0.05))
     assert list(m.spec[0]) == list(rgb_refl(0.9, 0.6, 0.3)) and list(m.spec[1]) == list(rgb_refl(0.5, 0.5, 0.5))
+
+
+def test_point_and_directional_lights():
+    """`light { point intensity .. position .. }` and `light { directional
+    intensity .. normal .. }` (LightParser.hs:35-45) become explicit delta
+    lights, in the reference's light order (prepended, Scene.hs:37-43)."""
+    b = B.parse("light { point intensity rgbI 1 2 3 position 0 4 0 } "
+                "light { directional intensity rgbI 2 2 2 normal 0 2 1 } prim { shape { sphere radius 1 } }")
+    d = b.flat.desc
+    assert d.num_lights == 2
+    first, second = d.lights[0], d.lights[1]
+    assert (first.type, second.type) == (abi.LIGHT_DIRECTIONAL, abi.LIGHT_POINT)
+    assert list(first.vec)[:3] == [0.0, 2.0, 1.0] and list(second.vec)[:3] == [0.0, 4.0, 0.0]
+    got = np.ctypeslib.as_array(second.radiance, (16,))
+    assert np.array_equal(got, rgb_illum(1, 2, 3))

@@ -285,6 +285,60 @@ def test_shiny_metal_matches_oracle(gpu_ctx):
     assert frac < 0.01, (frac, zmean, zmax)
 
 
+def test_delta_lights_closed_forms_and_oracle(gpu_ctx):
+    """Point and directional lights (Light.hs:53-61,142-150): the closed
+    forms of a Lambertian plane below each, and a scene that mixes them with
+    an area light and a sky over matte, plastic and glass spheres, against
+    the oracle at the same seed."""
+    from tests import cases
+    from bling_amd.scene import Material, SceneBuilder, const, rgb_illum, rgb_refl
+    from bling_amd import transform as T
+    spec1 = JobSpec(spp_nu=2, spp_nv=2, max_depth=1, sample_depth=1, seed=5, variance=True)
+    rho, I, D = 0.5, 3.0, 2.0
+    g, gsq = render_gpu(gpu_ctx, cases.delta_light_over_plane("point", rho=rho, I=I, D=D), spec1)
+    m, _ = mean_se(g, gsq)
+    np.testing.assert_allclose(m[..., 1], rho * I / (np.pi * D * D), rtol=1e-4)
+    g, gsq = render_gpu(gpu_ctx, cases.delta_light_over_plane("directional", rho=rho, I=I, theta=60.0), spec1)
+    m, _ = mean_se(g, gsq)
+    np.testing.assert_allclose(m[..., 1], rho * I * 0.25 / np.pi, rtol=1e-4)
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.6)))
+    sb.shape(abi.SHAPE_QUAD, (4.0, 4.0), o2w=T.rotate_x(90), material=floor)
+    mats = [Material.matte(rgb_refl(0.7, 0.3, 0.2), 0.3), Material.plastic(rgb_refl(0.2, 0.4, 0.7),
+            rgb_refl(0.5, 0.5, 0.5), 0.05), Material.glass(1.5, const(1.0), const(1.0))]
+    for x, m in zip((-1.2, 0.0, 1.2), mats):
+        sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((x, 0.5, 0.0)), material=sb.material(m))
+    sb.shape(abi.SHAPE_QUAD, (0.5, 0.5), o2w=T.rotate_x(-90) @ T.translate((1.5, 3.0, 0.0)), material=floor,
+             emission=rgb_illum(6.0, 6.0, 6.0))
+    sb.point_light(rgb_illum(4.0, 3.0, 2.0), (-1.0, 2.5, -1.0))
+    sb.directional_light(rgb_illum(1.0, 1.0, 1.2), (0.3, 1.0, -0.4))
+    sb.infinite_light(np.tile(rgb_illum(0.2, 0.2, 0.3), (2, 4, 1)))
+    sb.look(T.look_at((0.0, 1.5, -4.0), (0.0, 0.5, 0.0), (0, 1, 0)), 50.0, 48, 48)
+    flat = sb.build()
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, rr_start_depth=3, passes=2, seed=46,
+                   variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all()
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    for integrator in ("direct",):
+        sd = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=1, seed=47,
+                     variance=True, integrator=integrator)
+        g, gsq = render_gpu(gpu_ctx, flat, sd)
+        o, osq, _ = pyoracle.render(flat, sd)
+        assert np.array_equal(g[..., 0], o[..., 0])
+        mg, _ = mean_se(g, gsq)
+        mo, _ = mean_se(o, osq)
+        rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+        assert np.median(rel) < 1e-4, np.median(rel)
+
+
 def test_light_kind_variants_keep_results(gpu_ctx):
     """The diffuse shading kernel's light-kind variants (quad lights only;
     quad, sphere and infinite lights) give the all-kinds kernel's image:
@@ -308,7 +362,8 @@ def test_light_kind_variants_keep_results(gpu_ctx):
              "sky": room(lambda sb, m: sb.infinite_light(np.tile(rgb_illum(0.3, 0.3, 0.3), (4, 8, 1)))),
              "box": room(lambda sb, m: sb.shape(abi.SHAPE_BOX, (-0.2, 0.0, -0.2, 0.2, 0.4, 0.2),
                                                   o2w=T.translate((1.0, 0.0, 0.5)), material=m,
-                                                  emission=rgb_illum(2.0, 2.0, 2.0)))}
+                                                  emission=rgb_illum(2.0, 2.0, 2.0))),
+             "point": room(lambda sb, m: sb.point_light(rgb_illum(3.0, 3.0, 3.0), (0.5, 2.0, -0.5)))}
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=8)
     try:
         for name, flat in flats.items():

@@ -181,6 +181,36 @@ def pin_shiny_metal():
         np.testing.assert_allclose(fm[0], refl(ks) * (e + 2) / (8 * np.pi), rtol=2e-5)
 
 
+def _delta_y(flat):
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=1, sample_depth=1, passes=1, seed=5, variance=True)
+    img, sq, _ = pyoracle.render(flat, spec, threads=1)
+    m, _ = mean_se(img, sq)
+    return m[..., 1]
+
+
+def pin_point_light():
+    """mkPointLight (Light.hs:146-150): below a point light of intensity I
+    at height D a Lambertian plane returns rho I / (pi D^2)."""
+    rho, I, D = 0.5, 3.0, 2.0
+    y = _delta_y(cases.delta_light_over_plane("point", rho=rho, I=I, D=D))
+    np.testing.assert_allclose(y, rho * I / (np.pi * D * D), rtol=1e-4)
+
+
+def pin_directional_light():
+    """mkDirectional (Light.hs:53-54,142-144): a directional light of
+    intensity I from theta off the normal gives rho I cos^2(theta) / pi (its
+    |n . d| and the BSDF's cosine), whatever the length of the normal given."""
+    rho, I = 0.5, 3.0
+    for theta in (0.0, 60.0):
+        y = _delta_y(cases.delta_light_over_plane("directional", rho=rho, I=I, theta=theta))
+        np.testing.assert_allclose(y, rho * I * np.cos(np.radians(theta)) ** 2 / np.pi, rtol=1e-4, atol=1e-7)
+
+
+def test_delta_lights_closed_forms():
+    pin_point_light()
+    pin_directional_light()
+
+
 def test_shiny_metal_closed_forms():
     pin_shiny_metal()
 
@@ -570,6 +600,8 @@ MUTATIONS = [
     (mutant.POWER, lambda: pin_two_mirrors(rr_start=7)),
     (mutant.FILTER_CLIP, pin_film_tiles),
     (mutant.SHINY_SQRT, pin_shiny_metal),
+    (mutant.POINT_FALLOFF, pin_point_light),
+    (mutant.DIR_COS, pin_directional_light),
 ]
 
 
