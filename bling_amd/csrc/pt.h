@@ -41,8 +41,9 @@ enum { LT_AREA = 0, LT_INF = 1, LT_POINT = 2, LT_DIR = 3 };
 struct DLight {
   int32_t type, shape, id, w, h, map_off, dist_off, pad;
   float L[16];  // area: radiance; point, directional: intensity
-  float w2l[16], l2w[16];
+  float w2l[16], l2w[16];  // infinite: its transform; area: l2w is the shape's o2w
   float vec[4];  // point: position; directional: the unit direction toward the light
+  DShape sh;     // area: a copy of the emitting shape's record, read with the light (one load less)
 };
 
 // Everything the bounce kernels need about one scene (device resident).

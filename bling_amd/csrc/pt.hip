@@ -515,7 +515,7 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
         float mt = INFINITY;
         bool misses = false;
         if (has_area_light<LK>() && S.lights[ln].type == LT_AREA) {
-          const DShape& lsh = S.geo.shapes[S.lights[ln].shape];
+          const DShape& lsh = S.lights[ln].sh;
           DRay ro;
           ro.o = xform_point(lsh.w2o, p);
           ro.d = xform_vector(lsh.w2o, wib);

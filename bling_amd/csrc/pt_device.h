@@ -1177,8 +1177,8 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
     return ls;
   }
   if constexpr (!has_area_light<LK>()) return ls;
-  const DShape& s = S.geo.shapes[l.shape];  // Light.hs:152-160
-  const float* l2w = S.shapex[l.shape].o2w;
+  const DShape& s = l.sh;  // Light.hs:152-160
+  const float* l2w = l.l2w;
   f3 p = xform_point(s.w2o, pW);
   f3 ps, ns;
   sample_shape<LK>(s, p, u1, u2, &ps, &ns);
@@ -1217,7 +1217,7 @@ __device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3
     return pd / (2.0f * kPi * kPi * sint);
   }
   if constexpr (!has_area_light<LK>()) return 0.0f;
-  const DShape& s = S.geo.shapes[l.shape];
+  const DShape& s = l.sh;
   return shape_pdf<LK>(s, xform_point(s.w2o, p), xform_vector(s.w2o, wi));
 }
 // le light ray (Light.hs:98-106): nullptr for area lights

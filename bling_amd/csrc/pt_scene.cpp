@@ -357,6 +357,12 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     o.shape = (int32_t)s;
     o.id = d->shapes[s].light_id;
     std::memcpy(o.L, d->shapes[s].emission, 64);
+    o.sh.type = d->shapes[s].type;  // the shape's DShape record, as the scene's shape table holds it
+    o.sh.material = d->shapes[s].material;
+    o.sh.light = (int32_t)lights.size();
+    std::memcpy(o.sh.p, d->shapes[s].params, sizeof(o.sh.p));
+    std::memcpy(o.sh.w2o, d->shapes[s].w2o, sizeof(o.sh.w2o));
+    std::memcpy(o.l2w, d->shapes[s].o2w, sizeof(o.l2w));
     pt.shape_light_host[s] = (int32_t)lights.size();
     lights.push_back(o);
   }

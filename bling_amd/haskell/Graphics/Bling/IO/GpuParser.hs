@@ -18,7 +18,7 @@
 --     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureShape' beside
 --     mkGeom, 'captureInfinite', 'capturePoint' and 'captureDirectional' beside
 --     mkInfiniteAreaLight, mkPointLight and mkDirectional
---     (IO/LightParser.hs:29-33), 'captureView' where the camera and filter
+--     (IO/LightParser.hs:29-45), 'captureView' where the camera and filter
 --     are set (IO/CameraParser.hs:30-38, IO/ParserCore.hs).
 --
 -- The capture functions take the values those constructors receive, so
