@@ -10,7 +10,7 @@ The grammar follows the reference's Parsec parser:
                               (rgbR, rgbI, spd, temp, `%rrggbb` colours), blocks
   IO/TransformParser.hs       transform / newTransform blocks: rotateX/Y/Z,
                               scale, translate, matrix, lookAt
-  IO/CameraParser.hs          perspective camera (built at parse time with
+  IO/CameraParser.hs          perspective and environment cameras (built at parse time with
                               the image size then in effect)
   IO/MaterialParser.hs        matte, plastic, metal, glass, mirror,
                               transMatte, substrate, shinyMetal, blackbody,
@@ -40,7 +40,7 @@ Two kinds of error:
   BlingSyntaxError  the reference's parser rejects the text as well;
   Unsupported       valid `.bling` that the GPU path does not render
                     (waveFront, fractals, Bezier patches, sun/sky and image
-                    light maps, the environment camera).
+                    light maps).
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -348,8 +348,8 @@ class _Parser:
             lr = self.named_flt("lensRadius")
             fd = self.named_flt("focalDistance")
             self.camera = self._perspective(self.transform, lr, fd, fov, self.res[0], self.res[1])
-        elif t == "environment":
-            self.camera = ("environment",)
+        elif t == "environment":  # pEnvironmentCam (CameraParser.hs:40-44)
+            self.camera = ("environment", self.transform, self.res)
         else:
             self.fail("unknown camera type " + t)
         self.close()
@@ -836,8 +836,6 @@ class BlingScene:
     block describes."""
 
     def __init__(self, p):
-        if p.camera[0] != "perspective":
-            raise Unsupported("environment camera")
         sb = SceneBuilder()
         mat_index = {}
 
@@ -863,11 +861,15 @@ class BlingScene:
                 sb.point_light(a, b)
             else:
                 sb.directional_light(a, b)
-        _, c2w, r2c, lr, fd, fov = p.camera
-        sb.perspective_camera(c2w, r2c, lens_radius=lr, focal_distance=fd)
         filt, fparams = p.filter
+        if p.camera[0] == "environment":
+            _, c2w, (sx, sy) = p.camera
+            sb.environment_camera(c2w, sx, sy)
+        else:
+            _, c2w, r2c, lr, fd, fov = p.camera
+            sb.perspective_camera(c2w, r2c, lens_radius=lr, focal_distance=fd)
+            sb.lens = (c2w, float(fov), float(lr), float(fd), int(filt), tuple(float(x) for x in fparams))
         sb.set_film(p.res[0], p.res[1], filt, fparams)
-        sb.lens = (c2w, float(fov), float(lr), float(fd), int(filt), tuple(float(x) for x in fparams))
         self.flat = sb.build()
         self.width, self.height = p.res
         self.renderer = p.renderer

@@ -87,6 +87,7 @@ struct DevView {
   float c2w[16], r2c[16];
   float lens_radius, focal_distance;
   int32_t cam_type, pad;
+  float env_sx, env_sy;  // the environment camera's raster size (Camera.hs:70-76)
   int32_t film_w, film_h, ext_x0, ext_y0, ext_w, ext_h;
   float fw, fh;
   float filter[256];

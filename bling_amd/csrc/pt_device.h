@@ -1235,6 +1235,15 @@ __device__ __forceinline__ const float* light_le(const DevPtScene& S, int li, f3
 
 // ------------------------------------------------------------- camera
 __device__ __forceinline__ DRay fire_ray(const DevView& V, float ix, float iy, float lu, float lv) {
+  if (V.cam_type == BGPU_CAMERA_ENVIRONMENT) {  // Camera.hs:70-76: the sphere of directions over the raster
+    const float t = (kPi * iy) / V.env_sy, p = (kTwoPi * ix) / V.env_sx;
+    DRay w;
+    w.o = xform_point(V.c2w, mk3(0.0f, 0.0f, 0.0f));
+    w.d = xform_vector(V.c2w, mk3(sinf(t) * cosf(p), cosf(t), sinf(t) * sinf(p)));
+    w.tmin = 0.0f;
+    w.tmax = INFINITY;
+    return w;
+  }
   f3 pc = xform_point(V.r2c, mk3(ix, iy, 0.0f));  // Camera.hs:49-68
   DRay r;
   r.o = mk3(0.0f, 0.0f, 0.0f);

@@ -416,7 +416,10 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
 }
 
 int make_view(const bgpu_camera& cam, const bgpu_film& f, DevView* v) {
-  if (cam.type != BGPU_CAMERA_PERSPECTIVE) return fail(BGPU_E_INVALID, "unsupported camera type");
+  if (cam.type != BGPU_CAMERA_PERSPECTIVE && cam.type != BGPU_CAMERA_ENVIRONMENT)
+    return fail(BGPU_E_INVALID, "unsupported camera type");
+  if (cam.type == BGPU_CAMERA_ENVIRONMENT && !(cam.xres > 0.0f && cam.yres > 0.0f))
+    return fail(BGPU_E_INVALID, "environment camera: raster size must be positive");
   if (f.width <= 0 || f.height <= 0 || f.width > 65536 || f.height > 65536)
     return fail(BGPU_E_INVALID, "film size must be in [1, 65536]");
   if (f.filter < BGPU_FILTER_BOX || f.filter > BGPU_FILTER_TRIANGLE)
@@ -427,6 +430,8 @@ int make_view(const bgpu_camera& cam, const bgpu_film& f, DevView* v) {
   std::memcpy(v->r2c, cam.r2c, 64);
   v->lens_radius = cam.lens_radius;
   v->focal_distance = cam.focal_distance;
+  v->env_sx = cam.xres;
+  v->env_sy = cam.yres;
   v->film_w = f.width;
   v->film_h = f.height;
   if (f.filter == BGPU_FILTER_BOX) {

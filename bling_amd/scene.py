@@ -331,6 +331,13 @@ class SceneBuilder:
         self.camera = cam
         self.c2w, self.r2c = c2w, r2c
 
+    def environment_camera(self, c2w, width, height):
+        """mkEnvironmentCamera c2w sx sy (Camera.hs:70-76,149-155): every
+        direction over the raster, at the image size in effect."""
+        self.camera = View.environment(c2w, width, height).camera
+        self.c2w, self.r2c = c2w, None
+        self.lens = ("environment", c2w)
+
     def set_film(self, width, height, filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
         self.film = (int(width), int(height), int(filt), tuple(fparams))
 
@@ -370,6 +377,19 @@ class View:
             cam.r2c[k] = r2c.m[k]
         cam.lens_radius = F(lens_radius)
         cam.focal_distance = F(focal_distance)
+        film = abi.Film()
+        film.width, film.height, film.filter = int(width), int(height), int(filt)
+        for k in range(4):
+            film.fparams[k] = fparams[k]
+        return View(cam, film)
+
+    @staticmethod
+    def environment(c2w, width, height, filt=abi.FILTER_BOX, fparams=(0.0, 0.0, 0.0, 0.0)):
+        cam = abi.Camera()
+        cam.type = abi.CAMERA_ENVIRONMENT
+        for k in range(16):
+            cam.c2w[k] = c2w.m[k]
+        cam.xres, cam.yres = F(width), F(height)
         film = abi.Film()
         film.width, film.height, film.filter = int(width), int(height), int(filt)
         for k in range(4):
@@ -493,6 +513,9 @@ class FlatScene:
         RenderJob's camera follows its imageSize, IO/CameraParser.hs:30-38)."""
         if self.lens is None:
             raise ValueError("scene was not built with SceneBuilder.look")
+        if self.lens[0] == "environment":
+            ft, fp = self.film[2], self.film[3]
+            return View.environment(self.lens[1], width, height, ft, fp)
         c2w, fov, lr, fd, ft, fp = self.lens
         return View.perspective(c2w, fov, width, height, lr, fd, ft, fp)
 

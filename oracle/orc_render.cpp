@@ -1498,10 +1498,15 @@ static void addSample(const Ctx& c, float* wxyz, float* sq, int W, int H, float 
   }
 }
 
-static Ray fireRay(const bgpu_camera& cam, float ix, float iy, float lu, float lv) {  // Camera.hs:49-68
+static Ray fireRay(const bgpu_camera& cam, float ix, float iy, float lu, float lv) {  // Camera.hs:49-76
   M4 r2c, c2w;
   std::memcpy(r2c.m, cam.r2c, sizeof(r2c.m));
   std::memcpy(c2w.m, cam.c2w, sizeof(c2w.m));
+  if (cam.type == BGPU_CAMERA_ENVIRONMENT) {  // fireRay (Environment c2w sx sy)
+    const float t = kPi * iy / cam.yres, p = 2.0f * kPi * ix / cam.xres;
+    const V3 d = mk(std::sin(t) * std::cos(p), std::cos(t), std::sin(t) * std::sin(p));
+    return transRay(c2w, Ray{mk(0, 0, 0), d, 0.0f, kInf});
+  }
   V3 pCamera = transPoint(r2c, mk(ix, iy, 0.0f));
   Ray ray{mk(0, 0, 0), normalize(pCamera), 0.0f, kInf};
   if (cam.lens_radius > 0.0f) {

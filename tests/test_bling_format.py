@@ -161,8 +161,11 @@ def test_errors():
     with pytest.raises(B.BlingSyntaxError):  # graphPaper without its mapping (an older grammar)
         B.parse("material { matte kd { graphPaper 0.1 tex1 { constant rgbR 1 1 1 } "
                 "tex2 { constant rgbR 0 0 0 } } sigma { constant 0 } }")
-    with pytest.raises(B.Unsupported):
-        B.parse("camera { environment }")
+    b = B.parse("imageSize 32 16 transform { translate 0 1 0 } camera { environment } imageSize 64 32 "
+                "prim { shape { sphere radius 1 } }")  # the raster size in effect when the camera is parsed
+    cam = b.flat.desc.camera
+    assert (cam.type, cam.xres, cam.yres) == (abi.CAMERA_ENVIRONMENT, 32.0, 16.0)
+    assert (b.flat.desc.film.width, b.flat.desc.film.height) == (64, 32)
     with pytest.raises(ValueError, match="out of bounds"):
         B.parse("prim { mesh vertexCount 3 faceCount 1 v 0 0 0 v 1 0 0 v 0 1 0 f 0 1 5 }")
     b = B.parse("renderer { sppm photonCount 100 maxDepth 5 radius 1 } prim { shape { sphere radius 1 } }")

@@ -339,6 +339,36 @@ def test_delta_lights_closed_forms_and_oracle(gpu_ctx):
         assert np.median(rel) < 1e-4, np.median(rel)
 
 
+def test_environment_camera_matches_oracle(gpu_ctx):
+    """The environment camera (Camera.hs:70-76) inside a lit room with a
+    sky: the same seed tracks the oracle (the device's sinf/cosf may move a
+    ray by an ulp, so pixels are compared by their median difference)."""
+    from bling_amd.scene import Material, SceneBuilder, rgb_illum, rgb_refl
+    from bling_amd import transform as T
+    sb = SceneBuilder()
+    scenes._lit_room(sb, 2.0)
+    m = sb.material(Material.plastic(rgb_refl(0.3, 0.5, 0.7), rgb_refl(0.5, 0.5, 0.5), 0.05))
+    sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((0.8, 0.5, 0.6)), material=m)
+    sb.infinite_light(np.tile(rgb_illum(0.3, 0.3, 0.4), (2, 4, 1)))
+    sb.environment_camera(T.translate((0.0, 1.0, 0.0)), 64, 32)
+    sb.set_film(64, 32)
+    flat = sb.build()
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=5, sample_depth=3, rr_start_depth=3, passes=2, seed=48,
+                   variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all() and g[..., 2].sum() > 0
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    v = flat.view(32, 16)  # a job's view at another size keeps the camera kind
+    assert v.camera.type == abi.CAMERA_ENVIRONMENT and (v.camera.xres, v.camera.yres) == (32.0, 16.0)
+
+
 def test_light_kind_variants_keep_results(gpu_ctx):
     """The diffuse shading kernel's light-kind variants (quad lights only;
     quad, sphere and infinite lights) give the all-kinds kernel's image:

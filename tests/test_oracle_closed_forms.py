@@ -211,6 +211,31 @@ def test_delta_lights_closed_forms():
     pin_directional_light()
 
 
+def test_environment_camera_sees_the_sky_map():
+    """fireRay (Environment c2w sx sy) (Camera.hs:70-76): pixel (x, y) looks
+    along (sin t cos p, cos t, sin t sin p), t = pi y / sy, p = 2 pi x / sx:
+    its top rows look along +y and its bottom rows along -y.  Under a sky
+    map whose two halves differ, turned so that they meet at the world's
+    y = 0 plane, the image's top row is one half at its exact XYZ and the
+    bottom row the other, whatever the column."""
+    from bling_amd import abi
+    from bling_amd.scene import SceneBuilder, spectrum_to_xyz
+    from bling_amd import transform as T
+    sb = SceneBuilder()
+    sky = np.zeros((2, 4, 16), np.float32)
+    sky[0] = const(2.0)   # texel rows from the top: theta < pi/2 is the light's +z half
+    sky[1] = const(0.5)
+    sb.infinite_light(sky, w2l=T.rotate_x(-90))  # world +y to the light's +z
+    sb.environment_camera(T.Transform(), 8, 4)
+    sb.set_film(8, 4)
+    flat = sb.build()
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=1, sample_depth=1, passes=1, seed=3, variance=True)
+    img, sq, _ = pyoracle.render(flat, spec, threads=1)
+    m, _ = mean_se(img, sq)
+    np.testing.assert_allclose(m[0, :, 1], spectrum_to_xyz(const(2.0))[1], rtol=1e-5)
+    np.testing.assert_allclose(m[3, :, 1], spectrum_to_xyz(const(0.5))[1], rtol=1e-5)
+
+
 def test_shiny_metal_closed_forms():
     pin_shiny_metal()
 
