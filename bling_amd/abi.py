@@ -20,6 +20,7 @@ MAT_BLACK, MAT_MATTE, MAT_PLASTIC, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_TRANSMA
     MAT_SHINYMETAL = range(9)
 MESH_NORMALS, MESH_UVS = 1, 2
 LIGHT_INFINITE, LIGHT_POINT, LIGHT_DIRECTIONAL = 1, 2, 3
+LIGHTMAP_TEXELS, LIGHTMAP_SUNSKY = 0, 1
 CAMERA_PERSPECTIVE, CAMERA_ENVIRONMENT = 0, 1
 FILTER_BOX, FILTER_GAUSS, FILTER_MITCHELL, FILTER_SINC, FILTER_TRIANGLE = range(5)
 DEVICE_PTRS, TIMED, COUNT, COHERENT = 1, 2, 4, 8
@@ -78,7 +79,8 @@ class Mesh(C.Structure):
 
 class Light(C.Structure):
     _fields_ = [("type", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
-                ("pad", C.c_int32), ("radiance", FP), ("w2l", F16), ("l2w", F16), ("vec", C.c_float * 4)]
+                ("map", C.c_int32), ("radiance", FP), ("w2l", F16), ("l2w", F16), ("vec", C.c_float * 4),
+                ("sky", C.c_float * 8)]
 
 
 class Camera(C.Structure):

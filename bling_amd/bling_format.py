@@ -39,8 +39,8 @@ keeping its own primitive order; explicit lights are prepended the same way
 Two kinds of error:
   BlingSyntaxError  the reference's parser rejects the text as well;
   Unsupported       valid `.bling` that the GPU path does not render
-                    (waveFront, fractals, Bezier patches, sun/sky and image
-                    light maps).
+                    (waveFront, fractals, Bezier patches), and light map
+                    files other than Radiance RGBE.
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -56,7 +56,7 @@ import numpy as np
 from . import abi
 from . import noise
 from . import transform as T
-from .scene import Material, SceneBuilder, rgb_illum, rgb_refl, spd
+from .scene import Material, SceneBuilder, rgb_illum, rgb_illum_array, rgb_refl, spd
 from .textures import CELL_DISTANCES, Map2d, Map3d, Tex, image_spectra
 
 F = np.float32
@@ -115,6 +115,83 @@ def f32(s):
         if best is None or key < best[0]:
             best = (key, cand)
     return F(best[1])
+
+
+def read_hdr(path):
+    """A Radiance RGBE (.hdr) image as float RGB rows, (h, w, 3) binary32,
+    for `file` light maps: readTexture (IO/Bitmap.hs:13-19) takes what
+    JuicyPixels' readImage decodes to ImageRGBF, which for these files is
+    the RGBE decode.  JuicyPixels is not part of the reference checkout
+    (a Stackage dependency, stack.yaml:18); this follows the published
+    format: a text header ending in an empty line, the resolution line
+    `-Y h +X w` (rows top to bottom), then scanlines flat, in the old
+    run-length form or the newer per-channel run-length form, and a pixel
+    (r, g, b, e) decodes to (r, g, b) * 2^(e - 136), or 0 when e = 0."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if not (data.startswith(b"#?RADIANCE") or data.startswith(b"#?RGBE")):
+        raise ValueError("not a Radiance RGBE file")
+    pos = data.find(b"\n\n")
+    if pos < 0:
+        raise ValueError("RGBE header without an end")
+    head = data[:pos].split(b"\n")
+    for line in head:
+        if line.startswith(b"FORMAT=") and line.strip() != b"FORMAT=32-bit_rle_rgbe":
+            raise Unsupported("light map format %r" % line.decode("latin-1"))
+    end = data.find(b"\n", pos + 2)
+    res = data[pos + 2:end].split()
+    if len(res) != 4 or res[0] != b"-Y" or res[2] != b"+X":
+        raise Unsupported("RGBE orientation %r (only -Y h +X w)" % data[pos + 2:end].decode("latin-1"))
+    h, w = int(res[1]), int(res[3])
+    if h <= 0 or w <= 0:
+        raise ValueError("RGBE image of size %d x %d" % (w, h))
+    buf = np.frombuffer(data, np.uint8)
+    at = end + 1
+    out = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        row = out[y]
+        if (8 <= w < 0x8000 and at + 4 <= len(buf) and buf[at] == 2 and buf[at + 1] == 2
+                and (int(buf[at + 2]) << 8 | int(buf[at + 3])) == w and buf[at + 2] < 128):
+            at += 4
+            for c in range(4):  # per-channel runs
+                x = 0
+                while x < w:
+                    n = int(buf[at])
+                    if n > 128:
+                        n -= 128
+                        if x + n > w:
+                            raise ValueError("RGBE run past the scanline")
+                        row[x:x + n, c] = buf[at + 1]
+                        at += 2
+                    else:
+                        if n == 0 or x + n > w:
+                            raise ValueError("bad RGBE literal run")
+                        row[x:x + n, c] = buf[at + 1:at + 1 + n]
+                        at += 1 + n
+                    x += n
+            continue
+        x, shift = 0, 0
+        while x < w:  # flat pixels, (1, 1, 1, n) repeating the previous one n << shift times
+            px = buf[at:at + 4]
+            if len(px) < 4:
+                raise ValueError("RGBE data ends early")
+            at += 4
+            if px[0] == 1 and px[1] == 1 and px[2] == 1:
+                if x == 0:
+                    raise ValueError("RGBE repeat at a scanline's start")
+                n = int(px[3]) << shift
+                if x + n > w:
+                    raise ValueError("RGBE repeat past the scanline")
+                row[x:x + n] = row[x - 1]
+                x += n
+                shift += 8
+            else:
+                row[x] = px
+                x += 1
+                shift = 0
+    e = out[..., 3].astype(np.int32)
+    f = np.where(e == 0, F(0), np.ldexp(F(1), e - 136).astype(F)).astype(F)
+    return (out[..., :3].astype(F) * f[..., None]).astype(F)
 
 
 def _powi5(x):
@@ -708,13 +785,29 @@ class _Parser:
             self.named("l")
             self.open()
             m = self.word()
-            if m in ("file", "sunSky"):
-                raise Unsupported("infinite light map %r (only constant maps are on the GPU path)" % m)
-            if m != "constant":
+            if m == "constant":  # constSpectrumMap2d: a 1 x 1 map (Texture.hs:131-132)
+                s = self.spectrum()
+                self.lights.insert(0, ("infinite", np.asarray(s, F).reshape(1, 1, BANDS), w2l))
+            elif m == "file":  # readTexture: rgbToSpectrumIllum per pixel (IO/Bitmap.hs:13-28)
+                name = self.qstring()
+                path = os.path.join(self.base, name)
+                try:
+                    rgb = read_hdr(path)
+                except OSError as e:
+                    self.fail("cannot read light map %r: %s" % (name, e))
+                except ValueError as e:
+                    if isinstance(e, Unsupported):
+                        raise
+                    self.fail("can't convert image format to texture (%r: %s)" % (name, e))
+                self.lights.insert(0, ("infinite", rgb_illum_array(rgb), w2l))
+            elif m == "sunSky":  # mkSunSkyLight east sunDir turbidity (IO/MaterialParser.hs:266-270)
+                east = self.named_vec("east")
+                sun = self.named_vec("sunDir")
+                turb = self.named_flt("turbidity")
+                self.lights.insert(0, ("sunsky", (east, sun, turb), w2l))
+            else:
                 self.fail("unknown map type " + m)
-            s = self.spectrum()
             self.close()
-            self.lights.insert(0, ("infinite", np.asarray(s, F).reshape(1, 1, BANDS), w2l))
         elif t == "point":  # pPointLight (LightParser.hs:35-39)
             self.named("intensity")
             r = self.spectrum()
@@ -857,6 +950,8 @@ class BlingScene:
         for kind, a, b in p.lights:
             if kind == "infinite":
                 sb.infinite_light(a, b)
+            elif kind == "sunsky":
+                sb.sun_sky_light(a[0], a[1], a[2], b)
             elif kind == "point":
                 sb.point_light(a, b)
             else:

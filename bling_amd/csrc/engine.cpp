@@ -344,7 +344,9 @@ struct Engine {
     int kinds = j->scene->pt.light_kinds;
     for (Scene* s : scenes)
       if (s) kinds |= s->pt.light_kinds;
-    const int want_lights = cfg.light_variants == 0      ? LIGHTS_ALL
+    // (a sun/sky map needs the kernels that evaluate it, whatever the knob)
+    const int want_lights = (kinds >> kLightSky & 1)      ? LIGHTS_SKY
+                            : cfg.light_variants == 0     ? LIGHTS_ALL
                             : (kinds & ~kQuadLights) == 0 ? LIGHTS_QUAD
                             : (kinds & ~kQSILights) == 0  ? LIGHTS_QSI
                                                           : LIGHTS_ALL;

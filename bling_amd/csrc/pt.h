@@ -39,7 +39,8 @@ struct DMaterial {
 
 enum { LT_AREA = 0, LT_INF = 1, LT_POINT = 2, LT_DIR = 3 };
 struct DLight {
-  int32_t type, shape, id, w, h, map_off, dist_off, pad;
+  int32_t type, shape, id, w, h, map_off, dist_off;
+  int32_t sky;  // infinite: 1 for a sun/sky map (a kSkyFloats block at map_off), 0 for texels
   float L[16];  // area: radiance; point, directional: intensity
   float w2l[16], l2w[16];  // infinite: its transform; area: l2w is the shape's o2w
   float vec[4];  // point: position; directional: the unit direction toward the light
@@ -220,13 +221,14 @@ struct PtKernelArgs {
 // resident scene's lights are of these kinds, quads only, or quads, spheres
 // and infinite lights (the other shapes' light code and its registers left
 // out).
-enum { LIGHTS_ALL = 0, LIGHTS_QUAD = 1, LIGHTS_QSI = 2 };
+enum { LIGHTS_ALL = 0, LIGHTS_QUAD = 1, LIGHTS_QSI = 2, LIGHTS_SKY = 3 };
 // Bits of a light-kind set: bit k < 5 an area light on bgpu shape type k,
 // bit kLightInf an infinite light, kLightPoint a point light, kLightDir a
 // directional light.
 constexpr int kLightInf = 5;
 constexpr int kLightPoint = 6;
 constexpr int kLightDir = 7;
+constexpr int kLightSky = 8;  // an infinite light with a sun/sky map (also sets kLightInf)
 constexpr int kQuadLights = 1 << BGPU_SHAPE_QUAD;
 constexpr int kQSILights = (1 << BGPU_SHAPE_QUAD) | (1 << BGPU_SHAPE_SPHERE) | (1 << kLightInf);
 

@@ -462,16 +462,17 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
     }
     {  // sampleLightMis (Scene.hs:61-69)
       LSample ls = sample_light<LK>(S, ln, p, eps, bsdf.cs.n, ld1, ld2);  // n: bsdfShadingNormal (Path.hs:61)
-      if (ls.pdf != 0.0f && ls.li) {
+      if (ls.pdf != 0.0f && (!lrad_black<LK>(ls.li) || (has_sky_light<LK>() && ls.sky))) {
         SpecVal f = eval_bsdf<KM>(bsdf, wo, ls.wi);
         // a delta light's sample is not weighted (Scene.hs:65)
         float k = has_delta_light<LK>() && ls.delta ? 1.0f / ls.pdf
                                                     : power_heuristic(ls.pdf, bsdf_pdf<KM>(bsdf, wo, ls.wi)) / ls.pdf;
+        const LRad lr = sample_radiance<LK>(S, ln, ls);
         Xyz c;
         bool liBlack = true, fBlack = true;
 #pragma unroll 4
         for (int b = 0; b < 16; ++b) {
-          float fv = spec_at<KM>(f, b), li = ls.li[b];
+          float fv = spec_at<KM>(f, b), li = lrad_at<LK>(lr, b);
           if constexpr (has_delta_light<LK>()) li = li * ls.scale;  // sScale r s (1 for other lights: exact)
           liBlack = liBlack && li == 0.0f;
           fBlack = fBlack && fv == 0.0f;
@@ -498,15 +499,15 @@ __device__ __forceinline__ void sample_one_light(const DevPtScene& S, const Path
         float w = power_heuristic(bpdf, lpdf);
         // the emission a hit on the sampled light returns is known here:
         // the area light's radiance, or the infinite light's along wib
-        const float* e = has_inf_light<LK>() && S.lights[ln].type == LT_INF ? light_le<LK>(S, ln, wib)
-                                                                            : S.lights[ln].L;
+        const LRad e = has_inf_light<LK>() && S.lights[ln].type == LT_INF ? light_le<LK>(S, ln, wib)
+                                                                           : lrad(S.lights[ln].L);
         Xyz c;
         bool fBlack = true;
 #pragma unroll 4
         for (int b = 0; b < 16; ++b) {
           float fv = spec_at<KM>(fb, b);
           fBlack = fBlack && fv == 0.0f;
-          c.add(b, (thr_at(b) * ((fv * w) * lscale)) * e[b]);
+          c.add(b, (thr_at(b) * ((fv * w) * lscale)) * lrad_at<LK>(e, b));
         }
         // An area light is reached only if the ray hits its shape: the shape
         // test with tmax = inf (the traversal's own arithmetic, monotone in
@@ -663,11 +664,11 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
       if (prim < 0) {  // Path.hs:44-47
         if (spec) {
           for (int li = 0; li < S.nlights; ++li) {
-            const float* le = light_le<LK>(S, li, ray.d);
-            if (le) {
+            const LRad le = light_le<LK>(S, li, ray.d);
+            if (!lrad_black<LK>(le)) {
               Xyz c;
 #pragma unroll
-              for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * le[b]);
+              for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * lrad_at<LK>(le, b));
               xyz_accum(P.rad, Pn, i, c);
             }
           }
@@ -764,7 +765,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_shade(PtKernelArgs a) {
 // transmission (both sampled with uComp 0.5, uDir (0.5, 0.5)) while depth + 1
 // < maxDepth, each branch weighted by its sampled f.  A path that escapes
 // (black) or has no branch left pops the next pending one, or ends.
-template <int BLOCK>
+template <int BLOCK, int LK = kAllLights>
 __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   const unsigned int nact = a.cnt->n_act;
@@ -814,7 +815,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_direct(PtKernelArgs a) {
         for (int b = 0; b < 16; ++b) c.add(b, thr_at(b) * L[b]);
         xyz_accum(P.rad, Pn, i, c);
       }
-      sample_one_light<kAllKinds, kAllLights>(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2,
+      sample_one_light<kAllKinds, LK>(S, P, Pn, i, bsdf, wo, p, eps, uln, uld1, uld2, ubc, ubd1, ubd2,
                                               thr_at, want_shadow, want_mis);
       if (depth + 1 < J.md) {  // cont (DirectLighting.hs:45-59)
         SpecVal fr, ft;
@@ -1348,6 +1349,13 @@ static void launch_shade(const PtKernelArgs& a, const PtConfig& cfg, dim3 g, dim
     hipLaunchKernelGGL(k_class_scatter<kSortBlock>, gs, bs, 0, s, a);
   }
   mark(marks, PH_CLASSIFY + 1, s);
+  if (a.light_mode == LIGHTS_SKY) {  // a sun/sky map is resident: every class in the kernel that evaluates it
+    if (waves == 2) hipLaunchKernelGGL((k_shade<BLOCK, 2, kAllKinds, SHADE_ALL, kAllLightsSky>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_shade<BLOCK, 1, kAllKinds, SHADE_ALL, kAllLightsSky>), g, b, 0, s, a);
+    if (a.direct) hipLaunchKernelGGL((k_shade_direct<BLOCK, kAllLightsSky>), g, b, 0, s, a);
+    mark(marks, PH_SHADE + 1, s);
+    return;
+  }
   if (a.shade_mode == SHADE_MODE_DIFFUSE) {
     launch_diffuse<BLOCK, SHADE_ALL>(a, cfg.diffuse_waves, g, b, s);
   } else if (a.shade_mode == SHADE_MODE_SPLIT) {

@@ -8,6 +8,7 @@
 #pragma once
 #include "../../include/bgpu.h"
 #include "pt.h"
+#include "pt_sky.h"
 #include "spectral_tables.h"
 
 namespace bgpu {
@@ -936,8 +937,11 @@ __device__ __forceinline__ float sample_bsdf_flt(const Bsdf& b, f3 woW, int flag
 // template argument: the variant for scenes whose lights are all quads
 // carries none of the other shapes' sampling, pdf and intersection code and
 // their registers.  The caller guarantees that every light it passes is of a
-// kind in the set.
+// kind in the set.  kAllLights leaves out sun/sky maps: their evaluation
+// runs only in the kernels compiled with kAllLightsSky, while a scene holding
+// one is resident.
 constexpr int kAllLights = kAllShapes | (1 << kLightInf) | (1 << kLightPoint) | (1 << kLightDir);
+constexpr int kAllLightsSky = kAllLights | (1 << kLightSky);
 template <int LK>
 __device__ __forceinline__ constexpr int light_shapes() {
   return LK & kAllShapes;
@@ -953,6 +957,52 @@ __device__ __forceinline__ constexpr bool has_inf_light() {
 template <int LK>
 __device__ __forceinline__ constexpr bool has_delta_light() {
   return ((LK >> kLightPoint) & 3) != 0;
+}
+template <int LK>
+__device__ __forceinline__ constexpr bool has_sky_light() {
+  return ((LK >> kLightSky) & 1) != 0;
+}
+
+// A light's radiance along one direction: the 16 bands at `p` (an area
+// light's radiance, a delta light's intensity, a map texel, or the sun's
+// spectrum where a sun/sky map sees the sun), plus for a sun/sky map the
+// sky's colour as a combination of bases (SkyColor); black when both are
+// absent.  Light kinds without a sun/sky map use `p` alone.
+struct LRad {
+  const float* p;
+  const float* sky;  // the sun/sky block, nullptr without a sky term
+  SkyColor c;
+};
+__device__ __forceinline__ LRad lrad(const float* p) {
+  LRad r;
+  r.p = p;
+  r.sky = nullptr;
+  return r;
+}
+template <int LK>
+__device__ __forceinline__ bool lrad_black(const LRad& r) {
+  return !r.p && (!has_sky_light<LK>() || !r.sky);
+}
+template <int LK>
+__device__ __forceinline__ float lrad_at(const LRad& r, int b) {
+  if constexpr (!has_sky_light<LK>()) return r.p[b];
+  if (!r.sky) return r.p ? r.p[b] : 0.0f;
+  return sky_band(r.sky, r.c, b);
+}
+// sky_color as a call: the shading kernels run at their register limit, and
+// the model's arithmetic inlined into them would make the allocator spill on
+// every path; as a call, only sun/sky evaluations save and restore registers.
+__device__ __noinline__ SkyColor sky_color_call(const float* par, float u, float v) {
+  return sky_color(par, u, v);
+}
+// The sun/sky map of light l at (u, v): sky term and sun as sky_color gives them.
+__device__ __forceinline__ LRad sky_rad(const DevPtScene& S, const DLight& l, float u, float v) {
+  const float* blk = S.maps + l.map_off;
+  LRad r;
+  r.c = sky_color_call(blk + SKY_PAR, u, v);
+  r.sky = (r.c.sky || r.c.sun) ? blk : nullptr;
+  r.p = nullptr;
+  return r;
 }
 template <int SK = kAllShapes>
 __device__ __forceinline__ float shape_area(const DShape& s) {  // Shape.hs:314-328
@@ -1094,20 +1144,27 @@ __device__ __forceinline__ const float* d2_cond(const float* d, int v) {
   const float* first = marg + 3 + 2 * nv;
   return first + (size_t)v * (3 + 2 * nu);
 }
+// A texel map's value at (u, v), as rgbfToTexMap looks it up (IO/Bitmap.hs:21-28)
 __device__ __forceinline__ const float* tex_px(const DevPtScene& S, const DLight& l, float u, float v) {
-  int px = (int)floorf(u * (float)l.w) % l.w;  // mod' (Texture.hs:89-92)
-  px = px < 0 ? px + l.w : px;
-  int py = (int)floorf((-v) * (float)l.h) % l.h;
-  py = py < 0 ? py + l.h : py;
-  px = px < 0 ? 0 : (px >= l.w ? l.w - 1 : px);  // non-finite coordinates
-  py = py < 0 ? 0 : (py >= l.h ? l.h - 1 : py);
+  int px = (int)floorf((1.0f - u) * (float)l.w);
+  int py = (int)floorf((1.0f - v) * (float)l.h);
+  px = px < 0 ? 0 : (px > l.w - 1 ? l.w - 1 : px);  // max 0 $ min (w - 1), also for non-finite input
+  py = py < 0 ? 0 : (py > l.h - 1 ? l.h - 1 : py);
   return S.maps + l.map_off + 16 * ((size_t)py * l.w + px);
+}
+// texMapEval of an infinite light's map at (u, v)
+template <int LK>
+__device__ __forceinline__ LRad map_eval(const DevPtScene& S, const DLight& l, float u, float v) {
+  if (has_sky_light<LK>() && l.sky) return sky_rad(S, l, u, v);
+  return lrad(tex_px(S, l, u, v));
 }
 
 struct LSample {
-  const float* li;  // radiance bands in HBM (times `scale`), nullptr when black
+  LRad li;          // radiance (times `scale`); for a sun/sky map see `sky`
   float scale;      // 1, or a delta light's falloff / cosine factor
   bool delta;       // a point or directional light (Light.hs:142-150)
+  bool sky;         // a sun/sky map's sample: li is left black and the map is
+  float su, sv;     // evaluated at (su, sv) by sample_radiance where it is used
   f3 wi;
   DRay ray;
   float pdf;
@@ -1117,7 +1174,8 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
                                                 float u2) {
   const DLight& l = S.lights[li];
   LSample ls;
-  ls.li = nullptr;
+  ls.li = lrad(nullptr);
+  ls.sky = false;
   ls.wi = mk3(0, 1, 0);
   ls.ray.o = mk3(0, 0, 0);
   ls.ray.d = mk3(0, 1, 0);
@@ -1129,7 +1187,7 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
   if constexpr (has_delta_light<LK>()) {
     if (l.type == LT_POINT) {  // Light.hs:146-150: r / |pos - p|^2 toward pos, the ray along pos - p
       const f3 d = sub3(mk3(l.vec[0], l.vec[1], l.vec[2]), pW);
-      ls.li = l.L;
+      ls.li = lrad(l.L);
       ls.scale = 1.0f / sqlen3(d);
       ls.wi = norm3(d);
       ls.ray.o = pW;
@@ -1142,7 +1200,7 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
     }
     if (l.type == LT_DIR) {  // Light.hs:142-144: r |n . d| from d; n is the shading normal
       const f3 d = mk3(l.vec[0], l.vec[1], l.vec[2]);
-      ls.li = l.L;
+      ls.li = lrad(l.L);
       ls.scale = fabsf(dot3(n, d));
       ls.wi = d;
       ls.ray.o = pW;
@@ -1167,7 +1225,13 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
     if (sint == 0.0f) return ls;
     f3 dl = mk3(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
     f3 wi = xform_vector(l.l2w, dl);
-    ls.li = tex_px(S, l, u, v);
+    if (has_sky_light<LK>() && l.sky) {  // evaluated late: its terms would stay live across the BSDF
+      ls.sky = true;
+      ls.su = u;
+      ls.sv = v;
+    } else {
+      ls.li = lrad(tex_px(S, l, u, v));
+    }
     ls.wi = wi;
     ls.ray.o = pW;
     ls.ray.d = wi;
@@ -1183,7 +1247,7 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
   f3 ps, ns;
   sample_shape<LK>(s, p, u1, u2, &ps, &ns);
   f3 wi = norm3(sub3(ps, p));
-  ls.li = dot3(ns, wi) < 0.0f ? l.L : nullptr;
+  ls.li = lrad(dot3(ns, wi) < 0.0f ? l.L : nullptr);
   ls.wi = xform_vector(l2w, wi);
   ls.pdf = shape_pdf<LK>(s, p, wi);
   ls.ray.o = xform_point(l2w, p);
@@ -1191,6 +1255,12 @@ __device__ __forceinline__ LSample sample_light(const DevPtScene& S, int li, f3 
   ls.ray.tmin = e;
   ls.ray.tmax = __builtin_sqrtf(sqlen3(sub3(ps, p))) - e;
   return ls;
+}
+// A light sample's radiance, the sun/sky map's evaluated here.
+template <int LK>
+__device__ __forceinline__ LRad sample_radiance(const DevPtScene& S, int li, const LSample& ls) {
+  if (has_sky_light<LK>() && ls.sky) return sky_rad(S, S.lights[li], ls.su, ls.sv);
+  return ls.li;
 }
 template <int LK = kAllLights>
 __device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3 wi) {  // Light.hs:215-229
@@ -1220,17 +1290,17 @@ __device__ __forceinline__ float light_pdf(const DevPtScene& S, int li, f3 p, f3
   const DShape& s = l.sh;
   return shape_pdf<LK>(s, xform_point(s.w2o, p), xform_vector(s.w2o, wi));
 }
-// le light ray (Light.hs:98-106): nullptr for area lights
+// le light ray (Light.hs:98-106): black for area lights
 template <int LK = kAllLights>
-__device__ __forceinline__ const float* light_le(const DevPtScene& S, int li, f3 dir) {
-  if constexpr (!has_inf_light<LK>()) return nullptr;
+__device__ __forceinline__ LRad light_le(const DevPtScene& S, int li, f3 dir) {
+  if constexpr (!has_inf_light<LK>()) return lrad(nullptr);
   const DLight& l = S.lights[li];
-  if (l.type != LT_INF) return nullptr;
+  if (l.type != LT_INF) return lrad(nullptr);
   f3 wh = norm3(xform_vector(l.w2l, dir));
   float theta = acosf(hmax(-1.0f, hmin(1.0f, wh.z)));
   float pp = atan2f(wh.y, wh.x);
   float phi = pp < 0.0f ? pp + 2.0f * kPi : pp;
-  return tex_px(S, l, phi / (2.0f * kPi), theta / kPi);
+  return map_eval<LK>(S, l, phi / (2.0f * kPi), theta / kPi);
 }
 
 // ------------------------------------------------------------- camera

@@ -3,7 +3,8 @@
 //   lights     -> explicit lights, then geometric (emissive shape) lights in
 //                 shape order, the list of mkScene (Scene.hs:37-43)
 //   env maps   -> the Dist2D of mkInfiniteAreaLight (Light.hs:76-77,
-//                 Montecarlo.hs:34-104), built in binary32 in the same order
+//                 Montecarlo.hs:34-104), built in binary32 in the same order;
+//                 a sun/sky map's parameter block (SunSky.hs)
 //   film       -> 16x16 filter table (Image.hs:48-61) and sample extent (Image.hs:162-168)
 #include <algorithm>
 #include <cmath>
@@ -11,6 +12,7 @@
 
 #include "bgpu_internal.h"
 #include "pt_material.h"
+#include "pt_sky.h"
 #include "pt_texture.h"
 #include "spectral_tables.h"
 
@@ -88,12 +90,123 @@ float dist1d_fi(const std::vector<float>& f) {
   return acc;
 }
 
-const float* tex_px(const bgpu_light& l, float u, float v) {  // Texture.hs:94-99
-  int px = (int)std::floor(u * (float)l.width) % l.width;
-  px = px < 0 ? px + l.width : px;
-  int py = (int)std::floor((-v) * (float)l.height) % l.height;
-  py = py < 0 ? py + l.height : py;
+// A texel map's value at (u, v): rgbfToTexMap (IO/Bitmap.hs:21-28)
+const float* tex_px(const bgpu_light& l, float u, float v) {
+  int px = (int)std::floor((1.0f - u) * (float)l.width);
+  int py = (int)std::floor((1.0f - v) * (float)l.height);
+  px = px < 0 ? 0 : (px > l.width - 1 ? l.width - 1 : px);
+  py = py < 0 ? 0 : (py > l.height - 1 ? l.height - 1 : py);
   return l.radiance + 16 * ((size_t)py * l.width + px);
+}
+
+// ---------------------------------------------------------------- sun/sky
+struct V3 {
+  float x, y, z;
+};
+V3 v_norm(V3 v) {  // normalize (Math.hs:349-353)
+  const float sq = (v.x * v.x + v.y * v.y) + v.z * v.z;
+  if (sq == 0.0f) return {0.0f, 1.0f, 0.0f};
+  const float il = 1.0f / std::sqrt(sq);
+  return {v.x * il, v.y * il, v.z * il};
+}
+V3 v_cross(V3 u, V3 v) { return {u.y * v.z - u.z * v.y, -(u.x * v.z - u.z * v.x), u.x * v.y - u.y * v.x}; }
+float v_dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// evalSpd of a regular SPD (Spectrum.hs:271-280)
+float spd_regular(const float* a, int n, float l0, float l1, float l) {
+  if (l <= l0) return a[0];
+  if (l >= l1) return a[n - 1];
+  const float d1 = 1.0f / ((l1 - l0) / (float)(n - 1));
+  const float x = (l - l0) * d1;
+  const int b0 = (int)std::floor(x);
+  const int b1 = std::min(b0 + 1, n - 1);
+  const float dx = x - (float)b0;
+  return (1.0f - dx) * a[b0] + dx * a[b1];
+}
+// evalSpd of an irregular SPD, its binary search included (Spectrum.hs:258-269)
+float spd_irregular(const float* ls, const float* vs, int n, float l) {
+  if (l <= ls[0]) return vs[0];
+  if (l >= ls[n - 1]) return vs[n - 1];
+  int lo = 0, hi = n - 1;
+  for (;;) {
+    const int mid = (lo + hi) / 2;
+    if (lo == mid) break;
+    if (ls[mid] == l) {
+      lo = mid;
+      break;
+    }
+    if (ls[mid] < l) lo = mid;
+    else hi = mid;
+  }
+  const float t = (l - ls[lo]) / (ls[lo + 1] - ls[lo]);
+  return (1.0f - t) * vs[lo] + t * vs[lo + 1];  // lerp (Math.hs:108-110)
+}
+
+// The attenuated solar spectrum's SPD at lambda l (sunSpectrum', SunSky.hs:119-155)
+float sun_spd(float l, float m, float turb) {
+  const float tR = std::exp(-((m * 0.008735f) * std::pow(l / 1000.0f, -4.08f)));
+  const float beta = 0.04608365822050f * turb - 0.04586025928522f;
+  const float tA = std::exp(-((m * beta) * std::pow(l / 1000.0f, -1.3f)));
+  const float tO = std::exp(-((m * spd_irregular(BGPU_SKY_KO_L, BGPU_SKY_KO_A, 64, l)) * 0.35f));
+  const float kg = spd_irregular(BGPU_SKY_KG_L, BGPU_SKY_KG_A, 4, l);
+  const float tG = std::exp(-(((1.41f * kg) * m) / std::pow(1.0f + (118.93f * kg) * m, 0.45f)));
+  const float kwa = spd_irregular(BGPU_SKY_KWA_L, BGPU_SKY_KWA_A, 13, l);
+  const float tWA = std::exp(-((((0.2385f * kwa) * 2.0f) * m) / std::pow(1.0f + ((20.07f * kwa) * 2.0f) * m, 0.45f)));
+  return ((((spd_regular(BGPU_SKY_SOL, 38, 380.0f, 750.0f, l) * tR) * tA) * tO) * tG) * tWA;
+}
+
+// The block sky_color reads (pt_sky.h) for mkSunSkyLight east sunDir turbidity:
+// initSky (SunSky.hs:49-74), sunSpectrum' (:115-155), sunThetaMax2 (:42-47).
+void sky_init(const float* east, const float* sun, float t, float* blk) {
+  std::memset(blk, 0, sizeof(float) * kSkyFloats);
+  const float* bases[7] = {BGPU_RGB_ILLUM_WHITE, BGPU_RGB_ILLUM_CYAN, BGPU_RGB_ILLUM_MAGENTA, BGPU_RGB_ILLUM_YELLOW,
+                           BGPU_RGB_ILLUM_RED,   BGPU_RGB_ILLUM_GREEN, BGPU_RGB_ILLUM_BLUE};
+  for (int k = 0; k < 7; ++k) std::memcpy(blk + SKY_BASES + 16 * k, bases[k], 64);
+  // basis = coordinateSystem' (normalize up) (normalize east), up = (0, 1, 0) (Math.hs:427-432)
+  const V3 w = v_norm(v_norm({0.0f, 1.0f, 0.0f}));
+  const V3 u = v_norm(v_cross(v_norm({east[0], east[1], east[2]}), w));
+  const V3 v = v_cross(w, u);
+  auto to_local = [&](V3 d) { return V3{v_dot(d, u), v_dot(d, v), v_dot(d, w)}; };  // worldToLocal
+  const V3 sdw = {sun[0], sun[1], sun[2]};
+  const V3 sd = v_norm(to_local(v_norm(sdw)));
+  const V3 disk = v_norm(to_local(sdw));
+  const float z = sd.z;
+  const float st = std::acos(z < -1.0f ? -1.0f : (z > 1.0f ? 1.0f : z));
+  const float st2 = st * st, st3 = (st * st) * st, t2 = t * t;
+  const float chi = (4.0f / 9.0f - t / 120.0f) * (kPiF - 2.0f * st);
+  float* par = blk + SKY_PAR;
+  par[SKYP_SUNDIR] = sd.x, par[SKYP_SUNDIR + 1] = sd.y, par[SKYP_SUNDIR + 2] = sd.z;
+  par[SKYP_SUNTHETA] = st;
+  const float py[5] = {0.17872f * t - 1.46303f, -0.35540f * t + 0.42749f, -0.02266f * t + 5.32505f,
+                       0.12064f * t - 2.57705f, -0.06696f * t + 0.37027f};
+  const float px[5] = {-0.01925f * t - 0.25922f, -0.06651f * t + 0.00081f, -0.00041f * t + 0.21247f,
+                       -0.06409f * t - 0.89887f, -0.00325f * t + 0.04517f};
+  const float pyc[5] = {-0.01669f * t - 0.26078f, -0.09495f * t + 0.00921f, -0.00792f * t + 0.21023f,
+                        -0.04405f * t - 1.65369f, -0.01092f * t + 0.05291f};
+  std::memcpy(par + SKYP_PX, px, sizeof(px));
+  std::memcpy(par + SKYP_PY, pyc, sizeof(pyc));
+  std::memcpy(par + SKYP_PYY, py, sizeof(py));
+  par[SKYP_ZYY] = ((((4.04530f * t - 4.97100f) * std::tan(chi)) - 0.2155f * t) + 2.4192f) * 1000.0f;
+  par[SKYP_ZX] = ((((0.00165f * st3 - 0.00374f * st2) + 0.00208f * st) * t2) +
+                  ((((-0.02902f * st3 + 0.06377f * st2) - 0.03202f * st) + 0.00394f) * t)) +
+                 (((0.11693f * st3 - 0.21196f * st2) + 0.06052f * st) + 0.25885f);
+  par[SKYP_ZY] = ((((0.00275f * st3 - 0.00610f * st2) + 0.00316f * st) * t2) +
+                  ((((-0.04212f * st3 + 0.08970f * st2) - 0.04153f * st) + 0.00515f) * t)) +
+                 (((0.15346f * st3 - 0.26756f * st2) + 0.06669f * st) + 0.26688f);
+  const float sint2 = 6.955e5f / 1.496e8f;
+  par[SKYP_STM] = std::sqrt(hmaxf(0.0f, 1.0f - sint2));
+  par[SKYP_DISK] = disk.x, par[SKYP_DISK + 1] = disk.y, par[SKYP_DISK + 2] = disk.z;
+  std::memcpy(par + SKYP_S, BGPU_S0_XYZ, 12);
+  std::memcpy(par + SKYP_S + 3, BGPU_S1_XYZ, 12);
+  std::memcpy(par + SKYP_S + 6, BGPU_S2_XYZ, 12);
+  if (!(sd.z < 0.0f)) {  // below the horizon the sun is black
+    const float m = 1.0f / (std::cos(st) + 0.000940f * std::pow(1.6386f - st, -1.253f));
+    for (int i = 0; i < 16; ++i) {  // fromSpd of an SPD function: the band ends' mean (Spectrum.hs:316,329-335)
+      const float a = (float)i / 16.0f, b = (float)(i + 1) / 16.0f;
+      const float l0 = (1.0f - a) * 400.0f + a * 700.0f, l1 = (1.0f - b) * 400.0f + b * 700.0f;
+      blk[SKY_SUN + i] = (sun_spd(l0, m, t) + sun_spd(l1, m, t)) * 0.5f;
+    }
+  }
 }
 
 }  // namespace
@@ -315,22 +428,32 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
       lights.push_back(o);
       continue;
     }
-    if (l.type != BGPU_LIGHT_INFINITE || l.width <= 0 || l.height <= 0 || !l.radiance)
+    const bool sky = l.type == BGPU_LIGHT_INFINITE && l.map == BGPU_LIGHTMAP_SUNSKY;
+    if (l.type != BGPU_LIGHT_INFINITE || (l.map != BGPU_LIGHTMAP_TEXELS && !sky) ||
+        (!sky && (l.width <= 0 || l.height <= 0 || !l.radiance)))
       return fail(BGPU_E_INVALID, "unsupported or empty explicit light");
+    if (sky && !(l.sky[6] == l.sky[6]))
+      return fail(BGPU_E_INVALID, "sun/sky map: turbidity is not a number");
     DLight o;
     std::memset(&o, 0, sizeof(o));
     o.type = LT_INF;
     o.shape = -1;
     o.id = -1;
-    o.w = l.width;
-    o.h = l.height;
+    o.sky = sky ? 1 : 0;
+    o.w = sky ? kSkyMapW : l.width;
+    o.h = sky ? kSkyMapH : l.height;
     o.map_off = (int32_t)maps.size();
-    maps.insert(maps.end(), l.radiance, l.radiance + 16ull * l.width * l.height);
+    if (sky) {
+      maps.resize(maps.size() + kSkyFloats);
+      sky_init(l.sky, l.sky + 3, l.sky[6], maps.data() + o.map_off);
+    } else {
+      maps.insert(maps.end(), l.radiance, l.radiance + 16ull * l.width * l.height);
+    }
     std::memcpy(o.w2l, l.w2l, 64);
     std::memcpy(o.l2w, l.l2w, 64);
     // mkDist2D (texSize rmap) (sY . eval), eval (x, y) at Cartesian (x/sx, y/sy)
     o.dist_off = (int32_t)dist.size();
-    int nu = l.width, nvv = l.height;
+    int nu = o.w, nvv = o.h;
     float b;
     std::memcpy(&b, &nu, 4);
     dist.push_back(b);
@@ -339,9 +462,20 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
     float fsx = (float)nu, fsy = (float)nvv;
     std::vector<std::vector<float>> rows(nvv);
     std::vector<float> fis(nvv);
+    const float* blk = maps.data() + o.map_off;
     for (int y = 0; y < nvv; ++y) {
       rows[y].resize(nu);
-      for (int x = 0; x < nu; ++x) rows[y][x] = sY(tex_px(l, (float)x / fsx, (float)y / fsy));
+      for (int x = 0; x < nu; ++x) {
+        const float u = (float)x / fsx, v = (float)y / fsy;
+        if (sky) {
+          const SkyColor c = sky_color(blk + SKY_PAR, u, v);
+          float s16[16];
+          for (int k = 0; k < 16; ++k) s16[k] = sky_band(blk, c, k);
+          rows[y][x] = sY(s16);
+        } else {
+          rows[y][x] = sY(tex_px(l, u, v));
+        }
+      }
       fis[y] = dist1d_fi(rows[y]);
     }
     push_dist1d(dist, fis);
@@ -368,7 +502,7 @@ int build_pt_data(Ctx* ctx, const bgpu_scene_desc* d, Scene* sc) {
   }
   pt.light_kinds = 0;
   for (const DLight& l : lights)
-    pt.light_kinds |= l.type == LT_INF     ? (1 << kLightInf)
+    pt.light_kinds |= l.type == LT_INF     ? (1 << kLightInf) | (l.sky ? 1 << kLightSky : 0)
                       : l.type == LT_POINT ? (1 << kLightPoint)
                       : l.type == LT_DIR   ? (1 << kLightDir)
                                            : (1 << d->shapes[l.shape].type);

@@ -10,7 +10,7 @@
 //   shapes    bgpu_shape[num_shapes]
 //   materials bgpu_material[num_materials]
 //   lights    bgpu_light[num_lights] (radiance pointers stored as 0)
-//   one radiance map per light, f32[16 w h]
+//   one radiance map per light, f32[16 w h] (none for a sun/sky map)
 //   tex_nodes bgpu_tex_node[num_tex_nodes]
 //   images    bgpu_image[num_images] (texel pointers stored as 0)
 //   the texels of each image, f32[channels w h]
@@ -29,8 +29,9 @@ namespace bgpu {
 namespace {
 
 constexpr char kMagic[8] = {'B', 'G', 'P', 'U', 'S', 'C', 'N', '1'};
-constexpr uint32_t kVersion = 4;  // 2: three spectra and four scalars per material; 3: texture nodes, images;
-                                  // 4: point and directional lights (bgpu_light::vec)
+constexpr uint32_t kVersion = 5;  // 2: three spectra and four scalars per material; 3: texture nodes, images;
+                                  // 4: point and directional lights (bgpu_light::vec);
+                                  // 5: sun/sky maps (bgpu_light::map, ::sky)
 constexpr uint32_t kByteOrder = 0x01020304u;
 constexpr uint32_t kHasNormals = 1, kHasUvs = 2, kHasTriMesh = 4;
 constexpr uint64_t kAlign = 64;
@@ -61,6 +62,9 @@ struct Sections {
   std::vector<uint64_t> texels;
   uint64_t end;
 };
+
+// A light evaluated from its parameters stores no radiance map.
+bool has_map(const bgpu_light& l) { return !(l.type == BGPU_LIGHT_INFINITE && l.map == BGPU_LIGHTMAP_SUNSKY); }
 
 // Texels of an image record, or 0 when its size is out of range.
 uint64_t image_floats(const bgpu_image& im) {
@@ -95,6 +99,10 @@ bool layout(const FileHeader& h, const std::vector<bgpu_light>* lights, const st
   s->tex_nodes = s->images = s->end = 0;
   if (!lights) return true;
   for (const bgpu_light& l : *lights) {
+    if (!has_map(l)) {
+      s->maps.push_back(0);
+      continue;
+    }
     if (l.width <= 0 || l.height <= 0 || (uint64_t)l.width * (uint64_t)l.height > (1ull << 28)) return false;
     s->maps.push_back(take(64ull * l.width * l.height));
   }
@@ -168,7 +176,7 @@ extern "C" int bgpu_scene_write(const bgpu_scene_desc* d, const char* path) {
   h.size_image = sizeof(bgpu_image);
   std::vector<bgpu_light> lights(d->lights, d->lights + d->num_lights);
   for (const bgpu_light& l : lights)
-    if (!l.radiance) return fail(BGPU_E_INVALID, "bgpu_scene_write: light without a radiance map");
+    if (has_map(l) && !l.radiance) return fail(BGPU_E_INVALID, "bgpu_scene_write: light without a radiance map");
   std::vector<bgpu_image> images(d->images, d->images + d->num_images);
   for (const bgpu_image& im : images)
     if (!im.texels) return fail(BGPU_E_INVALID, "bgpu_scene_write: image without texels");
@@ -207,7 +215,7 @@ extern "C" int bgpu_scene_write(const bgpu_scene_desc* d, const char* path) {
   for (bgpu_light& l : stored) l.radiance = nullptr;
   put(s.lights, stored.data(), sizeof(bgpu_light) * stored.size());
   for (size_t i = 0; i < lights.size(); ++i)
-    put(s.maps[i], lights[i].radiance, 64ull * lights[i].width * lights[i].height);
+    if (has_map(lights[i])) put(s.maps[i], lights[i].radiance, 64ull * lights[i].width * lights[i].height);
   put(s.tex_nodes, d->tex_nodes, sizeof(bgpu_tex_node) * (uint64_t)d->num_tex_nodes);
   std::vector<bgpu_image> stored_images = images;
   for (bgpu_image& im : stored_images) im.texels = nullptr;
@@ -282,7 +290,8 @@ extern "C" int bgpu_scene_read(const char* path, bgpu_scene_desc** out) {
   d.shapes = reinterpret_cast<const bgpu_shape*>(b + s.shapes);
   d.num_materials = h.num_materials;
   d.materials = reinterpret_cast<const bgpu_material*>(b + s.materials);
-  for (size_t i = 0; i < od->lights.size(); ++i) od->lights[i].radiance = reinterpret_cast<const float*>(b + s.maps[i]);
+  for (size_t i = 0; i < od->lights.size(); ++i)
+    od->lights[i].radiance = has_map(od->lights[i]) ? reinterpret_cast<const float*>(b + s.maps[i]) : nullptr;
   d.num_lights = h.num_lights;
   d.lights = od->lights.data();
   d.camera = h.camera;

@@ -84,6 +84,9 @@ tnGraphPaper = #{const BGPU_TN_GRAPHPAPER}
 tnGStep = #{const BGPU_TN_GSTEP}
 
 meshNormals, meshUVs, lightInfinite, lightPoint, lightDirectional, cameraPerspective :: Int32
+lightmapTexels, lightmapSunSky :: Int32
+lightmapTexels = #{const BGPU_LIGHTMAP_TEXELS}
+lightmapSunSky = #{const BGPU_LIGHTMAP_SUNSKY}
 meshNormals = #{const BGPU_MESH_NORMALS}
 meshUVs = #{const BGPU_MESH_UVS}
 lightInfinite = #{const BGPU_LIGHT_INFINITE}
@@ -235,29 +238,31 @@ instance Storable Mesh where
   peek p = Mesh <$> #{peek bgpu_mesh, material} p <*> #{peek bgpu_mesh, flags} p
   poke p m = #{poke bgpu_mesh, material} p (msMaterial m) >> #{poke bgpu_mesh, flags} p (msFlags m)
 
--- | bgpu_light: an infinite area light with a discrete spectral map
--- (mkInfiniteAreaLight, Light.hs:72-82); the map stays owned by the caller
--- until bgpu_scene_create returns.
+-- | bgpu_light: an infinite area light (mkInfiniteAreaLight, Light.hs:72-82)
+-- with a texel map or a sun/sky map's parameters (lgMap), or a point or
+-- directional light; a texel map stays owned by the caller until
+-- bgpu_scene_create returns.
 data Light = Light
-  { lgType :: !Int32, lgWidth :: !Int32, lgHeight :: !Int32, lgPad :: !Int32
-  , lgRadiance :: Ptr CFloat, lgW2l :: [Float], lgL2w :: [Float], lgVec :: [Float] }
+  { lgType :: !Int32, lgWidth :: !Int32, lgHeight :: !Int32, lgMap :: !Int32
+  , lgRadiance :: Ptr CFloat, lgW2l :: [Float], lgL2w :: [Float], lgVec :: [Float], lgSky :: [Float] }
 
 instance Storable Light where
   sizeOf _ = #{size bgpu_light}
   alignment _ = #{alignment bgpu_light}
   peek p = Light <$> #{peek bgpu_light, type} p <*> #{peek bgpu_light, width} p
-    <*> #{peek bgpu_light, height} p <*> #{peek bgpu_light, pad} p <*> #{peek bgpu_light, radiance} p
+    <*> #{peek bgpu_light, height} p <*> #{peek bgpu_light, map} p <*> #{peek bgpu_light, radiance} p
     <*> peekFloats 16 (#{ptr bgpu_light, w2l} p) <*> peekFloats 16 (#{ptr bgpu_light, l2w} p)
-    <*> peekFloats 4 (#{ptr bgpu_light, vec} p)
+    <*> peekFloats 4 (#{ptr bgpu_light, vec} p) <*> peekFloats 8 (#{ptr bgpu_light, sky} p)
   poke p l = do
     #{poke bgpu_light, type} p (lgType l)
     #{poke bgpu_light, width} p (lgWidth l)
     #{poke bgpu_light, height} p (lgHeight l)
-    #{poke bgpu_light, pad} p (lgPad l)
+    #{poke bgpu_light, map} p (lgMap l)
     #{poke bgpu_light, radiance} p (lgRadiance l)
     pokeFloats 16 (#{ptr bgpu_light, w2l} p) (lgW2l l)
     pokeFloats 16 (#{ptr bgpu_light, l2w} p) (lgL2w l)
     pokeFloats 4 (#{ptr bgpu_light, vec} p) (lgVec l)
+    pokeFloats 8 (#{ptr bgpu_light, sky} p) (lgSky l)
 
 -- | bgpu_camera: a ProjectiveCamera after mkProjective (Camera.hs:108-147).
 data Camera = Camera

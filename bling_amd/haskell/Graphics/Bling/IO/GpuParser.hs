@@ -16,8 +16,9 @@
 --     with the material then taking 'nodeTexture' of the roots,
 --     'captureMesh' beside mkTriangleMesh and
 --     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureShape' beside
---     mkGeom, 'captureInfinite', 'capturePoint' and 'captureDirectional' beside
---     mkInfiniteAreaLight, mkPointLight and mkDirectional
+--     mkGeom, 'captureInfinite' (or 'captureSunSky' for a @sunSky@ map,
+--     IO/MaterialParser.hs:266-270), 'capturePoint' and 'captureDirectional'
+--     beside mkInfiniteAreaLight, mkPointLight and mkDirectional
 --     (IO/LightParser.hs:29-45), 'captureView' where the camera and filter
 --     are set (IO/CameraParser.hs:30-38, IO/ParserCore.hs).
 --
@@ -28,7 +29,7 @@
 -- accessors a maintainer exports (Spectrum.hs:46, Transform.hs:123-127).
 module Graphics.Bling.IO.GpuParser (
    newGpuScene, pGpuRenderer, GpuMaterial(..), captureMaterial, captureMesh, captureShape,
-   captureInfinite, capturePoint, captureDirectional, captureView, GpuTex(..), GpuMap2d(..), captureTexture,
+   captureInfinite, captureSunSky, capturePoint, captureDirectional, captureView, GpuTex(..), GpuMap2d(..), captureTexture,
    nodeTexture
    ) where
 
@@ -215,13 +216,23 @@ captureShape ref o2w mat em s = liftIO $ modifyIORef' ref $ \b ->
    in gpuAddShape sh b
 
 -- | mkInfiniteAreaLight rmap t (Light.hs:72-82) for a map of w x h texels,
--- each 16 bands, row-major from the top (the texel lookup of Texture.hs:94-101).
+-- each 16 bands, row-major, looked up as rgbfToTexMap does (IO/Bitmap.hs:21-28):
+-- texel (x, y) is texMapEval rmap at (1 - (x + 0.5) / w, 1 - (y + 0.5) / h),
+-- the decoded file's pixel (x, y); a @constant@ map is one texel.
 captureInfinite :: IORef GpuSceneBuilder -> Transform -> Int -> Int -> [Spectrum] -> JobParser ()
 captureInfinite ref w2l w h texels = liftIO $ modifyIORef' ref (gpuAddInfiniteLight l m) where
    (w2lRows, l2wRows) = transformRows w2l
-   l = Light lightInfinite (fromIntegral w) (fromIntegral h) 0 nullPtr (concat w2lRows) (concat l2wRows)
-             [0, 0, 0, 0]
+   l = Light lightInfinite (fromIntegral w) (fromIntegral h) lightmapTexels nullPtr (concat w2lRows)
+             (concat l2wRows) [0, 0, 0, 0] (replicate 8 0)
    m = VS.fromList (concatMap spectrumBands texels)
+
+-- | mkInfiniteAreaLight (mkSunSkyLight east sunDir turbidity) t (SunSky.hs:12-24):
+-- the library evaluates the sky and the sun from the three parameters.
+captureSunSky :: IORef GpuSceneBuilder -> Transform -> Vector -> Vector -> Float -> JobParser ()
+captureSunSky ref w2l east sd turb = liftIO $ modifyIORef' ref (gpuAddInfiniteLight l VS.empty) where
+   (w2lRows, l2wRows) = transformRows w2l
+   l = Light lightInfinite 640 480 lightmapSunSky nullPtr (concat w2lRows) (concat l2wRows) [0, 0, 0, 0]
+             [vx east, vy east, vz east, vx sd, vy sd, vz sd, turb, 0]
 
 -- | mkPointLight intensity position and mkDirectional intensity normal
 -- (Light.hs:53-61), beside pPointLight / pDirectionalLight
@@ -234,7 +245,7 @@ captureDirectional ref r n = captureDelta ref lightDirectional r n
 captureDelta :: IORef GpuSceneBuilder -> Int32 -> Spectrum -> Vector -> JobParser ()
 captureDelta ref ty r v = liftIO $ modifyIORef' ref (gpuAddInfiniteLight l m) where
    ident = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
-   l = Light ty 1 1 0 nullPtr ident ident [vx v, vy v, vz v, 0]
+   l = Light ty 1 1 lightmapTexels nullPtr ident ident [vx v, vy v, vz v, 0] (replicate 8 0)
    m = VS.fromList (spectrumBands r)
 
 -- | The camera after mkProjective (Camera.hs:108-147) and the film

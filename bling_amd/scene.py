@@ -54,6 +54,32 @@ def _rgb_to_spectrum(kind, r, g, b):
     return ((wb * b).astype(F) + rest.astype(F)).astype(F)
 
 
+def rgb_illum_array(rgb):
+    """rgbToSpectrumIllum of every (r, g, b) row of an (..., 3) float32
+    array: (..., 16), the same binary32 operations as _rgb_to_spectrum."""
+    a = np.asarray(rgb, F)
+    r, g, b = a[..., 0:1], a[..., 1:2], a[..., 2:3]
+    t = lambda n: _T["rgb_illum_%s" % n]
+    c1 = (r <= g) & (r <= b)
+    c2 = ~c1 & (g <= r) & (g <= b)
+    c3 = ~c1 & ~c2
+    out = np.zeros(a.shape[:-1] + (BANDS,), F)
+
+    def put(mask, w, cw, b1, k1, b2, k2):
+        m = mask[..., 0]
+        if m.any():
+            rest = (t(b1) * k1[m]).astype(F) + (t(b2) * k2[m]).astype(F)
+            out[m] = (t("white") * cw[m]).astype(F) + rest.astype(F)
+
+    put(c1 & (g <= b), None, r, "cyan", g - r, "blue", b - g)
+    put(c1 & ~(g <= b), None, r, "cyan", b - r, "green", g - b)
+    put(c2 & (r <= b), None, g, "magenta", r - g, "blue", b - r)
+    put(c2 & ~(r <= b), None, g, "magenta", b - g, "red", r - b)
+    put(c3 & (r <= b), None, b, "yellow", r - b, "green", g - r)
+    put(c3 & ~(r <= b), None, b, "yellow", g - b, "red", r - g)
+    return out
+
+
 def rgb_refl(r, g, b):
     return _rgb_to_spectrum("refl", r, g, b)
 
@@ -286,7 +312,10 @@ class SceneBuilder:
 
     def infinite_light(self, radiance_map, w2l=None):
         """mkInfiniteAreaLight rmap t (Light.hs:72-82); radiance_map is
-        (height, width, 16) spectral pixels."""
+        (height, width, 16) spectral pixels, looked up as rgbfToTexMap does
+        (IO/Bitmap.hs:21-28): pixel (x, y) covers u in ((w-1-x)/w, (w-x)/w],
+        v likewise, so row 0 holds the largest v (theta near pi) and column 0
+        the largest u (phi near 2 pi)."""
         w2l = w2l or Transform()
         m = np.ascontiguousarray(np.asarray(radiance_map, F).reshape(-1, 16))
         h, w = np.asarray(radiance_map).shape[:2]
@@ -297,6 +326,22 @@ class SceneBuilder:
             l.w2l[k] = w2l.m[k]
             l.l2w[k] = w2l.inv[k]
         self.lights.append((l, m))
+
+    def sun_sky_light(self, east, sun_dir, turbidity, w2l=None):
+        """mkInfiniteAreaLight (mkSunSkyLight east sunDir turbidity) t
+        (SunSky.hs:12-24): Perez' sky and the attenuated sun, evaluated per
+        direction by the library."""
+        w2l = w2l or Transform()
+        l = abi.Light()
+        l.type = abi.LIGHT_INFINITE
+        l.map = abi.LIGHTMAP_SUNSKY
+        l.width, l.height = 640, 480
+        for k in range(16):
+            l.w2l[k] = w2l.m[k]
+            l.l2w[k] = w2l.inv[k]
+        for k, v in enumerate(list(east) + list(sun_dir) + [turbidity]):
+            l.sky[k] = float(v)
+        self.lights.append((l, None))
 
     def point_light(self, intensity, position):
         """mkPointLight intensity position (Light.hs:57-61): a delta light
@@ -459,7 +504,7 @@ class FlatScene:
         self.light_maps = [m for _, m in b.lights]
         self.lights = (abi.Light * max(1, len(b.lights)))()
         for i, (l, m) in enumerate(b.lights):
-            l.radiance = m.ctypes.data_as(abi.FP)
+            l.radiance = m.ctypes.data_as(abi.FP) if m is not None else None
             self.lights[i] = l
         d = abi.SceneDesc()
         d.num_vertices = nv

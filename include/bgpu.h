@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BGPU_ABI_VERSION 3
+#define BGPU_ABI_VERSION 4
 #define BGPU_NUM_BANDS 16 /* Spectrum.hs:34-36 */
 
 /* ---- error codes ------------------------------------------------------ */
@@ -189,8 +189,18 @@ typedef struct {
 } bgpu_mesh;
 
 /* Explicit (non-geometric) lights (Light.hs:30-229):
- *   INFINITE     mkInfiniteAreaLight with a discrete spectral map of width x
- *                height texels and its world-to-light transform;
+ *   INFINITE     mkInfiniteAreaLight (Light.hs:71-82) with its world-to-light
+ *                transform and one of the maps pDiscSpectrumMap2d reads
+ *                (IO/MaterialParser.hs:250-272), chosen by `map`:
+ *                  BGPU_LIGHTMAP_TEXELS  a spectral map of width x height
+ *                    texels (`radiance`), looked up as rgbfToTexMap does
+ *                    (IO/Bitmap.hs:21-28): texel (clamp (floor ((1 - u) w)),
+ *                    clamp (floor ((1 - v) h))).  A `constant` map is 1 x 1,
+ *                    a `file` map the decoded image's rgbToSpectrumIllum.
+ *                  BGPU_LIGHTMAP_SUNSKY  mkSunSkyLight east sunDir turbidity
+ *                    (SunSky.hs:12-24), evaluated per direction: sky[0..2]
+ *                    east, sky[3..5] sunDir, sky[6] turbidity; `radiance`,
+ *                    width and height are unused (the map is 640 x 480);
  *   POINT        mkPointLight: `radiance` is the 16-band intensity (width =
  *                height = 1), `vec` the position;
  *   DIRECTIONAL  mkDirectional: `radiance` is the 16-band intensity (width =
@@ -199,14 +209,16 @@ typedef struct {
  * Point and directional lights are delta lights: sampled with pdf 1 and
  * never hit by rays (Scene.hs:61-82). */
 enum { BGPU_LIGHT_INFINITE = 1, BGPU_LIGHT_POINT = 2, BGPU_LIGHT_DIRECTIONAL = 3 };
+enum { BGPU_LIGHTMAP_TEXELS = 0, BGPU_LIGHTMAP_SUNSKY = 1 };
 typedef struct {
   int32_t type;
-  int32_t width, height; /* texSize of the map */
-  int32_t pad;
+  int32_t width, height; /* texSize of a texel map */
+  int32_t map;           /* infinite lights: BGPU_LIGHTMAP_* */
   const float* radiance; /* width*height*16 floats, pixel (x,y) at ((y*width)+x)*16 */
   float w2l[16];         /* world-to-light matrix, row-major */
   float l2w[16];         /* its inverse */
   float vec[4];          /* point: position; directional: normal (xyz, w unused) */
+  float sky[8];          /* sun/sky map: east xyz, sunDir xyz, turbidity, unused */
 } bgpu_light;
 
 enum { BGPU_CAMERA_PERSPECTIVE = 0, BGPU_CAMERA_ENVIRONMENT = 1 };

@@ -64,7 +64,11 @@ enum {
   MUT_BUMP_SHIFT = 23,     // bump: the u-shifted point moved along dpdv
   MUT_SHINY_SQRT = 24,     // frApproxEta: r' = r, the square root dropped
   MUT_POINT_FALLOFF = 25,  // point light: the 1 / |pos - p|^2 falloff dropped
-  MUT_DIR_COS = 26         // directional light: the |n . d| factor dropped
+  MUT_DIR_COS = 26,        // directional light: the |n . d| factor dropped
+  MUT_SKY_UP = 27,         // sun/sky: the sky above +z instead of -z (dz = dir.z)
+  MUT_SKY_DISK = 28,       // sun/sky: the sun disk tested against sunD, z not flipped
+  MUT_SKY_SUN_BELOW = 29,  // sun/sky: the sun's spectrum kept with the sun below the horizon
+  MUT_LIGHT_TEXEL = 30     // texel light maps looked up as getPixel (Texture.hs:94-99) instead of rgbfToTexMap
 };
 static int g_mutant = MUT_NONE;
 
@@ -384,6 +388,16 @@ struct Dist2D {
 };
 
 // ------------------------------------------------------------------ lights
+// The Perez sun/sky model's data (SkyData and sunSpectrum', SunSky.hs:29-155)
+struct SkyData {
+  V3 sunDir;       // normalize (worldToLocal basis (normalize sdw))
+  float sunTheta;
+  float px[5], py[5], pY[5];
+  float zx, zy, zY;
+  Spec sunR;
+  V3 diskDir;      // normalize (worldToLocal basis sdw), what sunSpectrum tests against
+};
+
 struct LightRec {
   int type;  // 0 area (shape index), 1 infinite, 2 point, 3 directional
   int shape;
@@ -392,15 +406,155 @@ struct LightRec {
   V3 vec;  // point: position; directional: normalize n
   // infinite
   int w, h;
-  const float* map;
+  const float* map;  // texel maps
+  bool sky;          // a sun/sky map (mkSunSkyLight)
+  SkyData sd;
   M4 w2l, l2w;
   Dist2D dist;
 };
 
-static Spec texEval(const LightRec& l, float u, float v) {  // getPixel (Texture.hs:94-99)
-  auto md = [](int a, int b) { int r = a % b; return r < 0 ? r + b : r; };
-  int px = md((int)std::floor(u * (float)l.w), l.w);
-  int py = md((int)std::floor((-v) * (float)l.h), l.h);
+// rgbToSpectrum over the illuminant bases (Spectrum.hs:143-159)
+static Spec rgbIllum(float r, float g, float b) {
+  const Spec wb = specOf(BGPU_RGB_ILLUM_WHITE), cb = specOf(BGPU_RGB_ILLUM_CYAN),
+             mb = specOf(BGPU_RGB_ILLUM_MAGENTA), yb = specOf(BGPU_RGB_ILLUM_YELLOW),
+             rb = specOf(BGPU_RGB_ILLUM_RED), gb = specOf(BGPU_RGB_ILLUM_GREEN), bb = specOf(BGPU_RGB_ILLUM_BLUE);
+  if (r <= g && r <= b)
+    return sscale(wb, r) + (g <= b ? sscale(cb, g - r) + sscale(bb, b - g) : sscale(cb, b - r) + sscale(gb, g - b));
+  if (g <= r && g <= b)
+    return sscale(wb, g) + (r <= b ? sscale(mb, r - g) + sscale(bb, b - r) : sscale(mb, b - g) + sscale(rb, r - b));
+  return sscale(wb, b) + (r <= b ? sscale(yb, r - b) + sscale(gb, g - r) : sscale(yb, g - b) + sscale(rb, r - g));
+}
+
+static float evalRegular(const float* a, int n, float l0, float l1, float l) {  // Spectrum.hs:271-280
+  if (l <= l0) return a[0];
+  if (l >= l1) return a[n - 1];
+  float d1 = 1.0f / ((l1 - l0) / (float)(n - 1));
+  float x = (l - l0) * d1;
+  int b0 = (int)std::floor(x);
+  int b1 = b0 + 1 < n - 1 ? b0 + 1 : n - 1;
+  float dx = x - (float)b0;
+  return (1.0f - dx) * a[b0] + dx * a[b1];
+}
+
+static float evalIrregular(const float* ls, const float* vs, int n, float l) {  // Spectrum.hs:258-269
+  if (l <= ls[0]) return vs[0];
+  if (l >= ls[n - 1]) return vs[n - 1];
+  // fi lo hi: binary search for the interval's index
+  int lo = 0, hi = n - 1, i;
+  for (;;) {
+    int mid = (lo + hi) / 2;
+    if (lo == mid) { i = lo; break; }
+    if (ls[mid] == l) { i = mid; break; }
+    if (ls[mid] < l) lo = mid;
+    else hi = mid;
+  }
+  float t = (l - ls[i]) / (ls[i + 1] - ls[i]);
+  return (1.0f - t) * vs[i] + t * vs[i + 1];
+}
+
+static SkyData initSky(V3 east, V3 sdw, float t) {  // mkSunSkyLight / initSky (SunSky.hs:12-74)
+  SkyData k;
+  V3 w = normalize(normalize(mk(0, 1, 0)));  // coordinateSystem' (normalize up) (normalize east)
+  V3 u = normalize(cross(normalize(east), w));
+  V3 v = cross(w, u);
+  auto toLocal = [&](V3 d) { return mk(dot(d, u), dot(d, v), dot(d, w)); };
+  k.sunDir = normalize(toLocal(normalize(sdw)));
+  k.diskDir = normalize(toLocal(sdw));
+  float st = std::acos(clampf(k.sunDir.z, -1.0f, 1.0f));
+  k.sunTheta = st;
+  float st2 = st * st, st3 = st * st * st, t2 = t * t;
+  float chi = (4.0f / 9.0f - t / 120.0f) * (kPi - 2.0f * st);
+  const float pY[5] = {0.17872f * t - 1.46303f, -0.35540f * t + 0.42749f, -0.02266f * t + 5.32505f,
+                       0.12064f * t - 2.57705f, -0.06696f * t + 0.37027f};
+  const float px[5] = {-0.01925f * t - 0.25922f, -0.06651f * t + 0.00081f, -0.00041f * t + 0.21247f,
+                       -0.06409f * t - 0.89887f, -0.00325f * t + 0.04517f};
+  const float py[5] = {-0.01669f * t - 0.26078f, -0.09495f * t + 0.00921f, -0.00792f * t + 0.21023f,
+                       -0.04405f * t - 1.65369f, -0.01092f * t + 0.05291f};
+  for (int i = 0; i < 5; ++i) k.pY[i] = pY[i], k.px[i] = px[i], k.py[i] = py[i];
+  k.zY = ((4.04530f * t - 4.97100f) * std::tan(chi) - 0.2155f * t + 2.4192f) * 1000.0f;
+  k.zx = (0.00165f * st3 - 0.00374f * st2 + 0.00208f * st) * t2 +
+         (-0.02902f * st3 + 0.06377f * st2 - 0.03202f * st + 0.00394f) * t +
+         (0.11693f * st3 - 0.21196f * st2 + 0.06052f * st + 0.25885f);
+  k.zy = (0.00275f * st3 - 0.00610f * st2 + 0.00316f * st) * t2 +
+         (-0.04212f * st3 + 0.08970f * st2 - 0.04153f * st + 0.00515f) * t +
+         (0.15346f * st3 - 0.26756f * st2 + 0.06669f * st + 0.26688f);
+  // sunSpectrum' (SunSky.hs:115-155): black with the sun below the horizon
+  k.sunR = sconst(0.0f);
+  if (!(k.sunDir.z < 0.0f) || g_mutant == MUT_SKY_SUN_BELOW) {
+    float m = 1.0f / (std::cos(st) + 0.000940f * std::pow(1.6386f - st, -1.253f));  // relative optical mass
+    auto sf = [&](float l) {
+      float tR = std::exp(-m * 0.008735f * std::pow(l / 1000.0f, -4.08f));
+      float beta = 0.04608365822050f * t - 0.04586025928522f;
+      float tA = std::exp(-m * beta * std::pow(l / 1000.0f, -1.3f));
+      float tO = std::exp(-m * evalIrregular(BGPU_SKY_KO_L, BGPU_SKY_KO_A, 64, l) * 0.35f);
+      float kg = evalIrregular(BGPU_SKY_KG_L, BGPU_SKY_KG_A, 4, l);
+      float tG = std::exp(-1.41f * kg * m / std::pow(1.0f + 118.93f * kg * m, 0.45f));
+      float kwa = evalIrregular(BGPU_SKY_KWA_L, BGPU_SKY_KWA_A, 13, l);
+      float tWA = std::exp(-0.2385f * kwa * 2.0f * m / std::pow(1.0f + 20.07f * kwa * 2.0f * m, 0.45f));
+      return evalRegular(BGPU_SKY_SOL, 38, 380.0f, 750.0f, l) * tR * tA * tO * tG * tWA;
+    };
+    for (int i = 0; i < 16; ++i) {  // fromSpd: (f l0 + f l1) / 2 per band (Spectrum.hs:316,329-335)
+      float a = (float)i / 16.0f, b = (float)(i + 1) / 16.0f;
+      float l0 = (1.0f - a) * 400.0f + a * 700.0f, l1 = (1.0f - b) * 400.0f + b * 700.0f;
+      k.sunR[i] = (sf(l0) + sf(l1)) * 0.5f;
+    }
+  }
+  return k;
+}
+
+static float perez(const float* p, float sunT, float t, float g, float lvz) {  // SunSky.hs:99-104
+  float num = (1.0f + p[0] * std::exp(p[1] / std::cos(t))) * (1.0f + p[2] * std::exp(p[3] * g)) +
+              p[4] * std::cos(g) * std::cos(g);
+  float den = (1.0f + p[0] * std::exp(p[1])) * (1.0f + p[2] * std::exp(p[3] * sunT)) +
+              p[4] * std::cos(sunT) * std::cos(sunT);
+  return lvz * num / den;
+}
+
+static Spec skyEval(const SkyData& k, float u, float v) {  // mkSunSkyLight's eval (SunSky.hs:17-20)
+  float phi = u * 2.0f * kPi, th = v * kPi;  // cartToSph
+  V3 dir = mk(std::sin(th) * std::cos(phi), std::sin(th) * std::sin(phi), std::cos(th));  // sphToDir
+  Spec sky = sconst(0.0f);
+  float dz = g_mutant == MUT_SKY_UP ? dir.z : -dir.z;  // skySpectrum (SunSky.hs:80-96)
+  if (!(dz < 1e-4f)) {
+    float theta = std::acos(dz);
+    float gamma = std::acos(clampf(dot(dir, k.sunDir), -1.0f, 1.0f));
+    float x = perez(k.px, k.sunTheta, theta, gamma, k.zx);
+    float y = perez(k.py, k.sunTheta, theta, gamma, k.zy);
+    float yy = perez(k.pY, k.sunTheta, theta, gamma, k.zY) * 1e-4f;
+    // chromaticityToXYZ (Spectrum.hs:229-251)
+    float den = 0.0241f + 0.2562f * x - 0.7341f * y;
+    float m1 = (-1.3515f - 1.7703f * x + 5.9114f * y) / den;
+    float m2 = (0.03f - 31.4424f * x + 30.0717f * y) / den;
+    float cx = BGPU_S0_XYZ[0] + m1 * BGPU_S1_XYZ[0] + m2 * BGPU_S2_XYZ[0];
+    float cy = BGPU_S0_XYZ[1] + m1 * BGPU_S1_XYZ[1] + m2 * BGPU_S2_XYZ[1];
+    float cz = BGPU_S0_XYZ[2] + m1 * BGPU_S1_XYZ[2] + m2 * BGPU_S2_XYZ[2];
+    float X = cx * yy / cy, Z = cz * yy / cy;
+    // xyzToSpectrum = rgbToSpectrumIllum . xyzToRgb (Spectrum.hs:162-168,357-358)
+    float r = 3.240479f * X - 1.537150f * yy - 0.498535f * Z;
+    float g = -0.969256f * X + 1.875991f * yy + 0.041556f * Z;
+    float b = 0.055648f * X - 0.204043f * yy + 1.057311f * Z;
+    sky = rgbIllum(r, g, b);
+  }
+  // sunSpectrum (SunSky.hs:106-113)
+  V3 sd = g_mutant == MUT_SKY_DISK ? k.diskDir : mk(k.diskDir.x, k.diskDir.y, -k.diskDir.z);
+  float stm = std::sqrt(hmax(0.0f, 1.0f - 6.955e5f / 1.496e8f));  // sunThetaMax2 (SunSky.hs:42-47)
+  return sky + (dot(sd, dir) > stm ? k.sunR : sconst(0.0f));
+}
+
+// texMapEval of an infinite light's map at (u, v): a sun/sky map, or a texel
+// map looked up as rgbfToTexMap does (IO/Bitmap.hs:21-28)
+static Spec texEval(const LightRec& l, float u, float v) {
+  if (l.sky) return skyEval(l.sd, u, v);
+  if (g_mutant == MUT_LIGHT_TEXEL) {
+    auto md = [](int a, int b) { int r = a % b; return r < 0 ? r + b : r; };
+    int px = md((int)std::floor(u * (float)l.w), l.w);
+    int py = md((int)std::floor((-v) * (float)l.h), l.h);
+    return specOf(l.map + 16 * ((size_t)py * l.w + px));
+  }
+  int px = (int)std::floor((1.0f - u) * (float)l.w);
+  int py = (int)std::floor((1.0f - v) * (float)l.h);
+  px = px < 0 ? 0 : (px > l.w - 1 ? l.w - 1 : px);
+  py = py < 0 ? 0 : (py > l.h - 1 ? l.h - 1 : py);
   return specOf(l.map + 16 * ((size_t)py * l.w + px));
 }
 
@@ -1660,16 +1814,9 @@ void orc_blinn(float e, int n, const float* wh, float* d, float* pdf) {
 // Light.hs:72-82,130-141,215-229): op 0 samples n directions with (u1, u2) =
 // u[2 i..] into wi (3 n floats), their pdf and radiance li (16 n); op 1
 // evaluates the pdf and le at the given wi.  w2l and l2w are row-major 4x4.
-void orc_envlight(int w, int h, const float* map, const float* w2l, const float* l2w, int op, int n,
-                  const float* u, float* wi, float* pdf, float* li) {
+static void envlightOps(orc::LightRec& l, const float* w2l, const float* l2w, int op, int n, const float* u,
+                        float* wi, float* pdf, float* li) {
   using namespace orc;
-  LightRec l{};
-  l.type = 1;
-  l.shape = -1;
-  l.id = -1;
-  l.w = w;
-  l.h = h;
-  l.map = map;
   std::memcpy(l.w2l.m, w2l, sizeof(l.w2l.m));
   std::memcpy(l.l2w.m, l2w, sizeof(l.l2w.m));
   l.dist = buildDist(l);
@@ -1689,6 +1836,50 @@ void orc_envlight(int w, int h, const float* map, const float* w2l, const float*
     }
     for (int b = 0; b < 16; ++b) li[16 * i + b] = le[b];
   }
+}
+
+void orc_envlight(int w, int h, const float* map, const float* w2l, const float* l2w, int op, int n,
+                  const float* u, float* wi, float* pdf, float* li) {
+  using namespace orc;
+  LightRec l{};
+  l.type = 1;
+  l.shape = -1;
+  l.id = -1;
+  l.w = w;
+  l.h = h;
+  l.map = map;
+  envlightOps(l, w2l, l2w, op, n, u, wi, pdf, li);
+}
+
+// The same for an infinite light with the sun/sky map mkSunSkyLight east
+// sunDir turbidity, sky = [east xyz, sunDir xyz, turbidity].
+void orc_skylight(const float* sky, const float* w2l, const float* l2w, int op, int n, const float* u, float* wi,
+                  float* pdf, float* li) {
+  using namespace orc;
+  LightRec l{};
+  l.type = 1;
+  l.shape = -1;
+  l.id = -1;
+  l.sky = true;
+  l.sd = initSky(mk(sky[0], sky[1], sky[2]), mk(sky[3], sky[4], sky[5]), sky[6]);
+  l.w = 640;
+  l.h = 480;
+  envlightOps(l, w2l, l2w, op, n, u, wi, pdf, li);
+}
+
+// The sun/sky map itself: its spectrum at n points uv (2 n floats) into
+// out (16 n), and the model's data: sunR (16), sunDir (3), sunTheta, the
+// zenith x, y and Y, into data (23 floats).
+void orc_sky_eval(const float* sky, int n, const float* uv, float* out, float* data) {
+  using namespace orc;
+  SkyData k = initSky(mk(sky[0], sky[1], sky[2]), mk(sky[3], sky[4], sky[5]), sky[6]);
+  for (int i = 0; i < n; ++i) {
+    Spec s = skyEval(k, uv[2 * i], uv[2 * i + 1]);
+    for (int b = 0; b < 16; ++b) out[16 * i + b] = s[b];
+  }
+  for (int b = 0; b < 16; ++b) data[b] = k.sunR[b];
+  data[16] = k.sunDir.x, data[17] = k.sunDir.y, data[18] = k.sunDir.z;
+  data[19] = k.sunTheta, data[20] = k.zx, data[21] = k.zy, data[22] = k.zY;
 }
 
 // Component `comp` of material m in its local shading frame, for n
@@ -1765,9 +1956,18 @@ int orc_render(void* sp, const bgpu_job_desc* job, float* wxyz, float* sq, int n
       c.lights.push_back(l);
       continue;
     }
-    l.w = s.lights[i].width;
-    l.h = s.lights[i].height;
-    l.map = s.light_maps[i].data();
+    l.sky = s.lights[i].map == BGPU_LIGHTMAP_SUNSKY;
+    if (l.sky) {  // mkSunSkyLight east sunDir turbidity: 640 x 480 (SunSky.hs:17)
+      const float* k = s.lights[i].sky;
+      l.sd = initSky(mk(k[0], k[1], k[2]), mk(k[3], k[4], k[5]), k[6]);
+      l.w = 640;
+      l.h = 480;
+      l.map = nullptr;
+    } else {
+      l.w = s.lights[i].width;
+      l.h = s.lights[i].height;
+      l.map = s.light_maps[i].data();
+    }
     std::memcpy(l.w2l.m, s.lights[i].w2l, sizeof(l.w2l.m));
     std::memcpy(l.l2w.m, s.lights[i].l2w, sizeof(l.l2w.m));
     c.lights.push_back(l);

@@ -251,6 +251,47 @@ def envlight(radiance_map, u=None, wi=None, w2l=None):
     return (wi, pdf, li) if u is not None else (pdf, li)
 
 
+def skylight(sky, u=None, wi=None, w2l=None):
+    """An infinite light with the sun/sky map of sky = (east xyz, sunDir xyz,
+    turbidity): with u (n, 2) its samples (wi, pdf, li); with wi (n, 3) its
+    (pdf, le) there."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_skylight.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
+    k = np.ascontiguousarray(sky, np.float32)
+    eye = np.eye(4, dtype=np.float32)
+    w2l = eye if w2l is None else np.ascontiguousarray(w2l, np.float32)
+    l2w = np.ascontiguousarray(np.linalg.inv(w2l.astype(np.float64)).astype(np.float32))
+    if u is not None:
+        u = np.ascontiguousarray(u, np.float32)
+        n = len(u)
+        wi = np.zeros((n, 3), np.float32)
+    else:
+        wi = np.ascontiguousarray(wi, np.float32)
+        n = len(wi)
+    pdf = np.zeros(n, np.float32)
+    li = np.zeros((n, 16), np.float32)
+    lib.orc_skylight(k.ctypes.data, w2l.ctypes.data, l2w.ctypes.data, 0 if u is not None else 1, n,
+                     None if u is None else u.ctypes.data, wi.ctypes.data, pdf.ctypes.data, li.ctypes.data)
+    return (wi, pdf, li) if u is not None else (pdf, li)
+
+
+def sky_eval(sky, uv):
+    """The sun/sky map of sky = (east xyz, sunDir xyz, turbidity) at the map
+    coordinates uv (n, 2): (spectra (n, 16), data) with data a dict of the
+    model's sunR, sunDir, sunTheta and zenith x, y, Y."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_sky_eval.argtypes = [vp, C.c_int, vp, vp, vp]
+    k = np.ascontiguousarray(sky, np.float32)
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 16), np.float32)
+    data = np.zeros(23, np.float32)
+    lib.orc_sky_eval(k.ctypes.data, len(uv), uv.ctypes.data, out.ctypes.data, data.ctypes.data)
+    return out, dict(sunR=data[:16].copy(), sunDir=data[16:19].copy(), sunTheta=data[19], zx=data[20],
+                     zy=data[21], zY=data[22])
+
+
 class mutant:
     """Context manager planting mistake `m` (orc_render.cpp's MUT_* list) in
     the oracle for the duration of a block."""
@@ -259,6 +300,7 @@ class mutant:
     PERLIN_WEIGHT, CELL_OWN, GRAD_SWAP, CRYSTAL_WRAP, BLEND_SWAP, IMAGE_FLIP = 14, 15, 16, 17, 18, 19
     FBM_FREQ, CHECKER_TRUNC, BUMP_FACE, BUMP_SHIFT, SHINY_SQRT = 20, 21, 22, 23, 24
     POINT_FALLOFF, DIR_COS = 25, 26
+    SKY_UP, SKY_DISK, SKY_SUN_BELOW, LIGHT_TEXEL = 27, 28, 29, 30
 
     def __init__(self, m):
         self.m = m

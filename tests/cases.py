@@ -323,3 +323,24 @@ def delta_light_over_plane(kind, rho=0.5, I=3.0, D=2.0, theta=60.0, res=4, fov=0
     sb.perspective_camera(*T.perspective_camera(c2w, 0.0, 1.0, fov, res, res))
     sb.set_film(res, res)
     return sb.build()
+
+
+def sun_sky_scene(env_camera=False, res=48):
+    """Matte, plastic and glass spheres on a floor under a sun/sky light map
+    (the light of race.bling, infinite { rotateX -90 } with sunSky east 1 0 0,
+    the sun raised and turned toward the camera), seen by a perspective
+    camera or, from above the floor, by the environment camera (2:1 raster)."""
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.6, 0.6, 0.6)))
+    sb.shape(abi.SHAPE_QUAD, (6.0, 6.0), o2w=T.rotate_x(90), material=floor)
+    mats = [Material.matte(rgb_refl(0.7, 0.3, 0.2), 0.3), Material.plastic(rgb_refl(0.2, 0.4, 0.7),
+            rgb_refl(0.5, 0.5, 0.5), 0.05), Material.glass(1.5, const(1.0), const(1.0))]
+    for x, m in zip((-1.2, 0.0, 1.2), mats):
+        sb.shape(abi.SHAPE_SPHERE, (0.5,), o2w=T.translate((x, 0.5, 0.0)), material=sb.material(m))
+    sb.sun_sky_light((1.0, 0.0, 0.0), (0.4, 0.5, 1.0), 5.0, T.rotate_x(-90))
+    if env_camera:
+        sb.environment_camera(T.translate((0.0, 1.0, -2.0)), 2 * res, res)
+        sb.set_film(2 * res, res)
+    else:
+        sb.look(T.look_at((0.0, 1.2, -4.0), (0.0, 0.6, 0.0), (0, 1, 0)), 60.0, res, res)
+    return sb.build()

@@ -65,7 +65,7 @@ def test_ctypes_layout_matches_c():
 
 def test_null_arguments_fail_with_invalid():
     lib = abi.load()
-    assert lib.bgpu_version() == 3
+    assert lib.bgpu_version() == 4
     h = C.c_void_p()
     assert lib.bgpu_scene_create(None, None, C.byref(h)) == abi.E_INVALID
     assert b"NULL" in lib.bgpu_last_error()

@@ -221,6 +221,10 @@ EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit li
     "cornell-box-underwater.bling": (79210, 3, 0, (720, 720)),
     "trans-matte.bling": (28, 2, 0, (480, 480)),
     "substrate.bling": (0, 5, 0, (1280, 720)),
+    "heightmap.bling": (1682, 1, 0, (640, 360)),
+    "bumpmap.bling": (2, 1, 1, (640, 360)),      # sun/sky light maps
+    "cellnoise.bling": (0, 9, 1, (640, 360)),
+    "race.bling": (0, 14, 1, (480, 320)),
 }
 
 
@@ -238,7 +242,12 @@ def test_reference_examples():
             seen[name] = "syntax"
     for name, want in EXPECT.items():
         assert seen[name] == want, (name, seen[name])
-    assert sum(v == "unsupported" for v in seen.values()) >= 8
+    assert sum(v == "unsupported" for v in seen.values()) >= 3  # waveFront, julia, mandelbulb
+    # race.bling: light { infinite { rotateX -90 } l { sunSky east 1 0 0 sunDir 1 0.2 0 turbidity 5 } }
+    lt = B.load(os.path.join(REF_EXAMPLES, "race.bling")).flat.desc.lights[0]
+    assert (lt.type, lt.map) == (abi.LIGHT_INFINITE, abi.LIGHTMAP_SUNSKY)
+    assert list(lt.sky)[:7] == [1, 0, 0, 1, F(0.2), 0, 5]
+    assert list(lt.w2l) == T.rotate_x(-90).m
     # cornell-box.bling: camera from its lookAt, film from its first filter
     b = B.load(os.path.join(REF_EXAMPLES, "cornell-box.bling"))
     c2w, _ = T.perspective_camera(T.look_at((278, 273, -800), (278, 273, 0), (0, 1, 0)) @ T.Transform(),
@@ -484,3 +493,75 @@ def test_point_and_directional_lights():
     assert list(first.vec)[:3] == [0.0, 2.0, 1.0] and list(second.vec)[:3] == [0.0, 4.0, 0.0]
     got = np.ctypeslib.as_array(second.radiance, (16,))
     assert np.array_equal(got, rgb_illum(1, 2, 3))
+
+
+def _rgbe_file(path, px, rle):
+    """Writes (h, w, 4) RGBE bytes as a Radiance file, flat or in the
+    per-channel run-length form."""
+    h, w = px.shape[:2]
+    out = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (h, w))
+    for y in range(h):
+        if not rle:
+            out += px[y].tobytes()
+            continue
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            ch, x = px[y, :, c], 0
+            while x < w:  # a run where values repeat, literals otherwise
+                n = 1
+                while x + n < w and n < 127 and ch[x + n] == ch[x]:
+                    n += 1
+                if n >= 3:
+                    out += bytes([128 + n, ch[x]])
+                else:
+                    n = min(w - x, 128)
+                    k = 1
+                    while k < n and not (x + k + 2 < w and ch[x + k] == ch[x + k + 1] == ch[x + k + 2]):
+                        k += 1
+                    out += bytes([k]) + ch[x:x + k].tobytes()
+                    n = k
+                x += n
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+def test_hdr_light_maps(tmp_path):
+    """`l { file "x.hdr" }` (IO/MaterialParser.hs:257-264, IO/Bitmap.hs:13-28):
+    RGBE pixels decode to (r, g, b) 2^(e - 136) (0 when e = 0) and each
+    becomes rgbToSpectrumIllum of its colour; flat and run-length files give
+    the same map; non-RGBE files are refused like the reference's
+    \"can't convert image format to texture\"."""
+    rng = np.random.default_rng(5)
+    h, w = 5, 12
+    px = rng.integers(0, 256, size=(h, w, 4), dtype=np.uint8)
+    px[:, :, 3] = rng.integers(120, 140, size=(h, w))
+    px[1, 2:9] = px[1, 2]  # runs for the run-length form
+    px[3, 4, 3] = 0        # e = 0: black whatever the mantissas
+    want = px[..., :3].astype(np.float64) * np.exp2(px[..., 3:].astype(np.float64) - 136)
+    want[px[..., 3] == 0] = 0
+    for rle in (False, True):
+        _rgbe_file(str(tmp_path / "sky.hdr"), px, rle)
+        rgb = B.read_hdr(str(tmp_path / "sky.hdr"))
+        assert rgb.dtype == np.float32 and np.array_equal(rgb, want.astype(np.float32))
+        b = B.parse("light { infinite { rotateX -90 } l { file \"sky.hdr\" } }", base=str(tmp_path))
+        lt = b.flat.desc.lights[0]
+        assert (lt.type, lt.map, lt.width, lt.height) == (abi.LIGHT_INFINITE, abi.LIGHTMAP_TEXELS, w, h)
+        m = b.flat.light_maps[0].reshape(h, w, 16)
+        for (y, x) in ((0, 0), (1, 5), (3, 4), (4, 11)):
+            assert np.array_equal(m[y, x], rgb_illum(*rgb[y, x]))
+    (tmp_path / "flat.png").write_bytes(b"\x89PNG\r\n\x1a\n")
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("light { infinite { } l { file \"flat.png\" } }", base=str(tmp_path))
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("light { infinite { } l { file \"missing.hdr\" } }", base=str(tmp_path))
+
+
+def test_sun_sky_light_map():
+    """`l { sunSky east .. sunDir .. turbidity .. }` (IO/MaterialParser.hs:266-270)."""
+    b = B.parse("light { infinite { rotateX -90 } l { sunSky east 0 0 1 sunDir 0.5 1 1 turbidity 5 } }")
+    lt = b.flat.desc.lights[0]
+    assert (lt.type, lt.map) == (abi.LIGHT_INFINITE, abi.LIGHTMAP_SUNSKY)
+    assert list(lt.sky)[:7] == [0, 0, 1, 0.5, 1, 1, 5]
+    assert not lt.radiance
+    with pytest.raises(B.BlingSyntaxError):
+        B.parse("light { infinite { } l { sunSky east 0 0 1 turbidity 5 } }")

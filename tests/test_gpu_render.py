@@ -369,6 +369,83 @@ def test_environment_camera_matches_oracle(gpu_ctx):
     assert v.camera.type == abi.CAMERA_ENVIRONMENT and (v.camera.xres, v.camera.yres) == (32.0, 16.0)
 
 
+@pytest.mark.parametrize("env_camera", [False, True], ids=["perspective", "environment"])
+def test_sun_sky_matches_oracle(gpu_ctx, env_camera):
+    """An infinite light with a sun/sky map (SunSky.hs), evaluated per
+    direction on the device, over matte, plastic and glass spheres: the same
+    seed tracks the oracle.  The environment camera sees the whole map, the
+    sun's disk included, directly."""
+    from tests import cases
+    flat = cases.sun_sky_scene(env_camera)
+    spec = JobSpec(spp_nu=4, spp_nv=4, max_depth=6, sample_depth=3, rr_start_depth=3, passes=2, seed=51,
+                   variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.isfinite(g).all() and g[..., 2].sum() > 0
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    frac, zmean, zmax = compare(g, gsq, o, osq)
+    assert frac < 0.01, (frac, zmean, zmax)
+    if env_camera:  # the sun's disk is in view: some pixels hold sunR
+        assert mg[..., 1].max() > 5 * np.median(mg[..., 1])
+
+
+def test_sun_sky_direct_lighting_and_knob(gpu_ctx):
+    """The direct-lighting integrator under a sun/sky map tracks the oracle,
+    and with the light-kind variants switched off the engine still picks the
+    kernels that evaluate the map (the same image as with them on, up to the
+    order of the film's sums)."""
+    from tests import cases
+    flat = cases.sun_sky_scene(False, res=32)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, passes=2, seed=53, variance=True,
+                   integrator="direct")
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=5, sample_depth=3, rr_start_depth=3, seed=54, variance=True)
+    a, _ = render_gpu(gpu_ctx, flat, spec)
+    try:
+        gpu_ctx.config_set("pt.light_variants", 0)
+        b, _ = render_gpu(gpu_ctx, flat, spec)
+    finally:
+        gpu_ctx.config_set("pt.light_variants", 1)
+    # the film's per-pixel sums take samples in either order: equal weights, close sums
+    assert np.array_equal(a[..., 0], b[..., 0])
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-5)
+
+
+def test_texel_light_map_orientation_matches_oracle(gpu_ctx):
+    """A texel light map of distinct texels seen through the environment
+    camera: the device's rgbfToTexMap lookup (IO/Bitmap.hs:21-28) and
+    Dist2D pick the oracle's texels."""
+    from bling_amd.scene import Material, SceneBuilder, rgb_illum, rgb_refl
+    from bling_amd import transform as T
+    rng = np.random.default_rng(12)
+    sb = SceneBuilder()
+    floor = sb.material(Material.matte(rgb_refl(0.5, 0.5, 0.5)))
+    sb.shape(abi.SHAPE_SPHERE, (0.4,), o2w=T.translate((0.0, -0.6, 1.0)), material=floor)
+    texels = np.stack([[rgb_illum(*rng.uniform(0.05, 2.0, 3)) for _ in range(12)] for _ in range(6)])
+    sb.infinite_light(texels, T.rotate_y(30) @ T.rotate_x(-90))
+    sb.environment_camera(T.Transform(), 72, 36)
+    sb.set_film(72, 36)
+    flat = sb.build()
+    spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=3, sample_depth=3, seed=52, variance=True)
+    g, gsq = render_gpu(gpu_ctx, flat, spec)
+    o, osq, _ = pyoracle.render(flat, spec)
+    assert np.array_equal(g[..., 0], o[..., 0])
+    mg, _ = mean_se(g, gsq)
+    mo, _ = mean_se(o, osq)
+    rel = np.abs(mg - mo) / np.maximum(np.abs(mo), 1e-3)
+    assert np.median(rel) < 1e-4, np.median(rel)
+
+
 def test_light_kind_variants_keep_results(gpu_ctx):
     """The diffuse shading kernel's light-kind variants (quad lights only;
     quad, sphere and infinite lights) give the all-kinds kernel's image:

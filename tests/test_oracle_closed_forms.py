@@ -409,7 +409,11 @@ def pin_env_pdf_normalised():
 def pin_env_samples_follow_the_map():
     """Cell (iu, iv) of the sampled (phi / 2pi, theta / pi) holds the mass of
     the map's luminance at the cell's corner (mkDist2D evaluates the map at
-    (iu / w, iv / h); the map lookup takes row floor(-v h) mod h)."""
+    (iu / w, iv / h)); a light map is looked up as rgbfToTexMap does
+    (IO/Bitmap.hs:21-28): texel (min (w - 1) (floor ((1 - u) w)), the same
+    in v), so the corner (iu, iv) reads texel (min(w - iu, w - 1),
+    min(h - iv, h - 1)), and the radiance along a direction is the texel at
+    its own (u, v)."""
     m, vals = _env_map()
     h, w = vals.shape
     rng = np.random.default_rng(42)
@@ -423,7 +427,8 @@ def pin_env_samples_follow_the_map():
     iu = np.minimum((phi / (2 * np.pi) * w).astype(int), w - 1)
     iv = np.minimum((theta / np.pi * h).astype(int), h - 1)
     obs = np.bincount(iv * w + iu, minlength=h * w).astype(np.float64)
-    mass = vals[(h - np.arange(h)) % h, :].astype(np.float64).reshape(-1)
+    rows, cols = np.minimum(h - np.arange(h), h - 1), np.minimum(w - np.arange(w), w - 1)
+    mass = vals[rows][:, cols].astype(np.float64).reshape(-1)
     exp = mass / mass.sum() * ok.sum()
     chi2 = np.sum((obs - exp) ** 2 / exp)
     dof = h * w - 1
@@ -432,6 +437,9 @@ def pin_env_samples_follow_the_map():
     pdf2, le = pyoracle.envlight(m, wi=wi[ok])
     assert np.mean(np.isclose(pdf2, pdf[ok], rtol=1e-3)) > 0.995
     assert np.mean(np.all(le == li[ok], axis=1)) > 0.99
+    tx = np.clip(np.floor((1 - phi / (2 * np.pi)) * w), 0, w - 1).astype(int)
+    ty = np.clip(np.floor((1 - theta / np.pi) * h), 0, h - 1).astype(int)
+    assert np.mean(li[ok][:, 0] == vals[ty, tx]) > 0.99
 
 
 def test_env_light_pdf_and_samples():
@@ -620,6 +628,7 @@ MUTATIONS = [
     (mutant.OREN_SWAP, pin_oren_nayar),
     (mutant.ENV_SIN, pin_env_pdf_normalised),
     (mutant.ENV_ROW, pin_env_samples_follow_the_map),
+    (mutant.LIGHT_TEXEL, pin_env_samples_follow_the_map),
     (mutant.PATH_PC, lambda: pin_two_mirrors(rr_start=0)),
     (mutant.PATH_T, lambda: pin_two_mirrors(rr_start=7)),
     (mutant.POWER, lambda: pin_two_mirrors(rr_start=7)),

@@ -25,7 +25,11 @@ def _light_bytes(desc):
     for i in range(desc.num_lights):
         l = abi.Light()
         C.memmove(C.byref(l), C.byref(desc.lights[i]), C.sizeof(abi.Light))
-        m = np.ctypeslib.as_array(C.cast(l.radiance, C.POINTER(C.c_float)), (l.width * l.height * 16,)).copy()
+        if l.radiance:
+            m = np.ctypeslib.as_array(C.cast(l.radiance, C.POINTER(C.c_float)), (l.width * l.height * 16,)).copy()
+        else:  # a sun/sky map: parameters only
+            assert l.map == abi.LIGHTMAP_SUNSKY
+            m = np.zeros(0, np.float32)
         l.radiance = None
         out.append((bytes(l), m.tobytes()))
     return out
@@ -36,6 +40,7 @@ SCENES = {
     "materials_env": lambda: cases.materials_scene(res=24, env=True),
     "textured_uvs": lambda: cases.textured_scene(res=24),
     "node_textures": lambda: cases.node_texture_scene(res=16),
+    "sun_sky": lambda: cases.sun_sky_scene(res=16),
     "divergent_small": lambda: scenes.divergent_scene(res=32, hm=16, n_tori=2, torus_tris=200, n_spheres=10,
                                                       sky=(16, 8)),
 }
@@ -77,10 +82,11 @@ def test_round_trip_is_byte_exact(tmp_path, name):
             assert np.ctypeslib.as_array(a.texels, (n,)).tobytes() == np.ctypeslib.as_array(b.texels, (n,)).tobytes()
 
 
-@pytest.mark.parametrize("name", ["materials_env", "node_textures"])
+@pytest.mark.parametrize("name", ["materials_env", "node_textures", "sun_sky"])
 def test_oracle_renders_the_read_back_scene_identically(tmp_path, name):
     flat = {"materials_env": lambda: cases.materials_scene(res=16, env=True),
-            "node_textures": lambda: cases.node_texture_scene(res=16)}[name]()
+            "node_textures": lambda: cases.node_texture_scene(res=16),
+            "sun_sky": lambda: cases.sun_sky_scene(res=16)}[name]()
     path = tmp_path / "m.bgs"
     flat.save(path)
     spec = JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, rr_start_depth=3, passes=1,
