@@ -22,9 +22,11 @@ The grammar follows the reference's Parsec parser:
                               material's selector texture, anything else a
                               node texture (textures.Tex) evaluated per hit
   IO/PrimitiveParser.hs       mesh (polygon faces triangulated as fans,
-                              TriangleMesh.hs:23-29) and shape primitives
-  IO/LightParser.hs           infinite lights with a constant map, point and
-                              directional lights; emission
+                              TriangleMesh.hs:23-29), heightMap, waveFront
+                              (IO/WaveFront.hs) and shape primitives
+  IO/LightParser.hs           infinite lights with a constant, file (RGBE)
+                              or sunSky map, point and directional lights;
+                              emission
   IO/RendererParser.hs,       renderer blocks (every kind is read; only the
   IO/IntegratorParser.hs      sampler renderer with a stratified sampler and
                               the path or directLighting integrator maps
@@ -39,8 +41,8 @@ keeping its own primitive order; explicit lights are prepended the same way
 Two kinds of error:
   BlingSyntaxError  the reference's parser rejects the text as well;
   Unsupported       valid `.bling` that the GPU path does not render
-                    (waveFront, fractals, Bezier patches), and light map
-                    files other than Radiance RGBE.
+                    (fractals, Bezier patches), and light map files other
+                    than Radiance RGBE.
 
 Numbers are read like GHC's `read` (the exact decimal rounded once to
 binary32), so the scene carries the values the Haskell front end computes.
@@ -192,6 +194,94 @@ def read_hdr(path):
     e = out[..., 3].astype(np.int32)
     f = np.where(e == 0, F(0), np.ldexp(F(1), e - 136).astype(F)).astype(F)
     return (out[..., :3].astype(F) * f[..., None]).astype(F)
+
+
+_OBJ_NUM = re.compile(rb"[+-]?[0-9]+(?:\.[0-9]+)?")
+_OBJ_INT = re.compile(rb"[0-9]+")
+
+
+def read_wavefront(data):
+    """waveFrontParser (IO/WaveFront.hs:118-211) over the bytes of an .obj
+    file: (points (n, 3), normals (m, 3) normalized, uvs (k, 2), corners
+    (c, 3) of 0-based (point, uv, normal) indices, -1 where absent, faces
+    triangulated as fans, and [(material name, first corner)] in file
+    order).  Lines are `v x y z [w]`, `vn x y z`, `vt u [v [w]]` (v
+    defaults to 1), `f a[/[t][/n]] ...`, `usemtl name`; anything else is
+    skipped.  Numbers are decimals without exponents (bytestring-lexing's
+    readDecimal), indices positive; spaces only between fields, lines end
+    in a newline.  A malformed v/vn/vt/f line is an error, as it is for the
+    reference's parser."""
+    pts, nrm, uvs, corners, mtls = [], [], [], [], []
+
+    def nums(fields, lo, hi, line):
+        if not (lo <= len(fields) <= hi) or not all(_OBJ_NUM.fullmatch(x) for x in fields):
+            raise ValueError("bad line %r" % line[:60])
+        return [f32(x.decode()) for x in fields]
+
+    lines = data.split(b"\n")
+    if lines and lines[-1] == b"":
+        lines.pop()
+    elif lines:  # the last line lacks its newline: v, vn, vt and f need one
+        last = lines[-1]
+        if last[:1] in (b"v", b"f") and not last.startswith(b"vp"):
+            raise ValueError("line without a newline: %r" % last[:60])
+    for line in lines:
+        if line.startswith(b"vn"):
+            x, y, z = nums([t for t in line[2:].split(b" ") if t] if line[2:3] == b" " else [b"?"], 3, 3, line)
+            sq = F(F(F(x * x) + F(y * y)) + F(z * z))  # normalize (Math.hs:349-353)
+            if sq != 0:
+                il = F(F(1) / F(np.sqrt(sq)))
+                nrm.append((F(x * il), F(y * il), F(z * il)))
+            else:
+                nrm.append((F(0), F(1), F(0)))
+        elif line.startswith(b"vt"):
+            f = nums([t for t in line[2:].split(b" ") if t] if line[2:3] == b" " else [b"?"], 1, 3, line)
+            uvs.append((f[0], f[1] if len(f) > 1 else F(1)))
+        elif line.startswith(b"usemtl"):
+            if line[6:7] != b" ":
+                raise ValueError("bad line %r" % line[:60])
+            mtls.append((line[6:].lstrip(b" ").decode("latin-1"), len(corners)))
+        elif line.startswith(b"v"):
+            if line[1:2] != b" ":
+                continue  # `vp` and the like: skipped
+            f = nums([t for t in line[1:].split(b" ") if t], 3, 4, line)
+            pts.append(tuple(f[:3]))
+        elif line.startswith(b"f"):
+            if line[1:2] != b" ":
+                raise ValueError("bad line %r" % line[:60])
+            face = []
+            for tok in [t for t in line[1:].split(b" ") if t]:
+                parts = tok.split(b"/")
+                if len(parts) > 3 or not _OBJ_INT.fullmatch(parts[0]) or \
+                        any(p and not _OBJ_INT.fullmatch(p) for p in parts[1:]) or (len(parts) == 3 and not parts[2]):
+                    raise ValueError("bad face %r" % line[:60])
+                t_ = int(parts[1]) if len(parts) > 1 and parts[1] else 0
+                n_ = int(parts[2]) if len(parts) > 2 else 0
+                face.append((int(parts[0]) - 1, t_ - 1, n_ - 1))  # pred
+            if not face:
+                raise ValueError("bad face %r" % line[:60])
+            for k in range(1, len(face) - 1):  # triangulate (Primitive/TriangleMesh.hs:23-29)
+                corners += [face[0], face[k], face[k + 1]]
+    return (np.array(pts, F).reshape(-1, 3), np.array(nrm, F).reshape(-1, 3), np.array(uvs, F).reshape(-1, 2),
+            np.array(corners, np.int64).reshape(-1, 3), mtls)
+
+
+def wavefront_triangles(corners, mtls):
+    """The triangles parseWaveFront makes (IO/WaveFront.hs:101-116): the
+    material intervals start at ("default", 0) and at each usemtl's corner
+    count, and end at the next start, the last one at the number of
+    TRIANGLES (cnt = corners / 3, mixing units as the reference does); each
+    interval makes a triangle at every third corner index from its start
+    while below start + length.  Returns [(material name, first corner)]."""
+    cnt = len(corners) // 3
+    starts = [("default", 0)] + list(mtls)
+    ends = [c for _, c in mtls] + [cnt]
+    out = []
+    for (name, st), en in zip(starts, ends):
+        ln = en - st
+        if ln > 0:
+            out += [(name, i) for i in range(st, st + ln, 3)]
+    return out
 
 
 def _powi5(x):
@@ -739,12 +829,77 @@ class _Parser:
             sid = self.curr_id
             self.curr_id += 1
             stmt = [("shape", kind, params, self.transform, self.material, self.emit, sid)]
-        elif t in ("waveFront", "julia", "mandelbulb", "bezier"):
+        elif t == "waveFront":  # PrimitiveParser.hs:69-72, IO/WaveFront.hs
+            stmt = self.wavefront()
+        elif t in ("julia", "mandelbulb", "bezier"):
             raise Unsupported("primitive %r" % t)
         else:
             self.fail("unknown primitive type " + t)
         self.close()
         self.prims.append(stmt)
+
+    def wavefront(self):
+        """`waveFront "file.obj" materials { "name" { <material> } ... }`:
+        the file relative to the scene's folder (readFileBS), the named
+        materials with the current one as the default (pNamedMaterialMap,
+        PrimitiveParser.hs:76-84).  Points go through the current transform;
+        normals and uvs are used as the file gives them (the triangles shade
+        with an identity transform, Primitive.hs:65), and a triangle takes
+        its third corner's uv and normal index from its second corner
+        (wfTriUVs / wfTriNormals read corner i + 1 twice, WaveFront.hs:52-79)."""
+        name = self.qstring()
+        self.named("materials")
+        self.open()
+        mats = {}
+        while self.peek()[0] == "str":
+            mn = self.qstring()
+            self.open()
+            mats[mn] = self.material_body()
+            self.close()
+        self.close()
+        path = os.path.join(self.base, name)
+        try:
+            with open(path, "rb") as f:
+                data = f.read()
+        except OSError as e:
+            self.fail("cannot read %r: %s" % (name, e))
+        try:
+            pts, nrm, uvs, corners, mtls = read_wavefront(data)
+        except ValueError as e:
+            self.fail("%s: %s" % (name, e))
+        tris = wavefront_triangles(corners, mtls)
+        if not tris:
+            return []
+        xf = self.transform
+        wp = xf.points(pts) if len(pts) else pts
+        runs = []  # (material, has uv, has normals) runs of consecutive triangles
+        for mname, c in tris:
+            if c + 2 >= len(corners):
+                self.fail("%s: a material interval runs past the faces" % name)
+            (p0, t0, n0), (p1, t1, n1), (p2, _, _) = corners[c], corners[c + 1], corners[c + 2]
+            ti = (t0, t1, t1)
+            ni = (n0, n1, n1)
+            if max(p0, p1, p2) >= len(pts):
+                self.fail("%s: point index out of range" % name)
+            has_uv = min(ti) >= 0
+            has_n = not (max(ni) < 0)
+            if has_n and (min(ni) < 0 or max(ni) >= len(nrm)):
+                self.fail("%s: normal index out of range" % name)
+            if has_uv and max(ti) >= len(uvs):
+                self.fail("%s: uv index out of range" % name)
+            m = mats.get(mname, self.material)
+            key = (id(m), has_uv, has_n)
+            if not runs or runs[-1][0] != key:
+                runs.append((key, m, has_uv, has_n, []))
+            runs[-1][4].append(((p0, p1, p2), ti, ni))
+        out = []
+        for _, m, has_uv, has_n, ts in runs:
+            v = np.concatenate([wp[list(p)] for p, _, _ in ts])
+            faces = np.arange(3 * len(ts), dtype=np.int64).reshape(-1, 3)
+            uv = np.concatenate([uvs[list(t)] for _, t, _ in ts]) if has_uv else None
+            nn = np.concatenate([nrm[list(n)] for _, _, n in ts]) if has_n else None
+            out.append(("objmesh", v, faces, m, nn, uv))
+        return out
 
     def shape_block(self):
         self.open()
@@ -943,6 +1098,9 @@ class BlingScene:
                     _, verts, faces, xf, m, norms, uvs = rec
                     if len(faces):
                         sb.mesh(verts, faces, o2w=xf, material=mat(m), normals=norms, uvs=uvs)
+                elif rec[0] == "objmesh":  # points already in world space, normals as the file gives them
+                    _, verts, faces, m, norms, uvs = rec
+                    sb.mesh(verts, faces, material=mat(m), normals=norms, uvs=uvs, world=True)
                 else:
                     _, kind, params, xf, m, emit, sid = rec
                     sb.shape(kind, params, o2w=xf, material=mat(m), emission=emit)

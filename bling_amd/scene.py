@@ -254,8 +254,13 @@ class SceneBuilder:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def mesh(self, verts, faces, o2w=None, material=0, normals=None, uvs=None):
-        """mkTriangleMesh o2w mat p i n uv: faces are index triples (0-based)."""
+    def mesh(self, verts, faces, o2w=None, material=0, normals=None, uvs=None, world=False):
+        """mkTriangleMesh o2w mat p i n uv: faces are index triples (0-based).
+        world=True: points and normals are already what the triangles use
+        (a Wavefront file's transformed points and untransformed normals),
+        and no transform is applied."""
+        if world and o2w is not None:
+            raise ValueError("a world-space mesh takes no transform")
         o2w = o2w or Transform()
         v = np.asarray(verts, F).reshape(-1, 3)
         i = np.asarray(faces, np.int64).reshape(-1, 3)
@@ -264,11 +269,12 @@ class SceneBuilder:
         if (i >= len(v)).any():
             raise ValueError("mkTriangleMesh: contains out of bounds indices")
         flags = 0
-        wv = o2w.points(v)
+        wv = v.copy() if world else o2w.points(v)
         self.verts.append(wv)
         if normals is not None:
             flags |= abi.MESH_NORMALS
-            self.norms.append(o2w.normals(np.asarray(normals, F).reshape(-1, 3)))
+            n = np.asarray(normals, F).reshape(-1, 3)
+            self.norms.append(n.copy() if world else o2w.normals(n))
         else:
             self.norms.append(None)
         if uvs is not None:

@@ -225,6 +225,7 @@ EXPECT = {  # the examples the GPU path renders: (triangles, shapes, explicit li
     "bumpmap.bling": (2, 1, 1, (640, 360)),      # sun/sky light maps
     "cellnoise.bling": (0, 9, 1, (640, 360)),
     "race.bling": (0, 14, 1, (480, 320)),
+    "ducky.bling": (13456, 1, 1, (640, 360)),    # a Wavefront mesh (see test_wavefront_meshes)
 }
 
 
@@ -242,7 +243,7 @@ def test_reference_examples():
             seen[name] = "syntax"
     for name, want in EXPECT.items():
         assert seen[name] == want, (name, seen[name])
-    assert sum(v == "unsupported" for v in seen.values()) >= 3  # waveFront, julia, mandelbulb
+    assert sum(v == "unsupported" for v in seen.values()) >= 2  # julia, mandelbulb
     # race.bling: light { infinite { rotateX -90 } l { sunSky east 1 0 0 sunDir 1 0.2 0 turbidity 5 } }
     lt = B.load(os.path.join(REF_EXAMPLES, "race.bling")).flat.desc.lights[0]
     assert (lt.type, lt.map) == (abi.LIGHT_INFINITE, abi.LIGHTMAP_SUNSKY)
@@ -565,3 +566,64 @@ def test_sun_sky_light_map():
     assert not lt.radiance
     with pytest.raises(B.BlingSyntaxError):
         B.parse("light { infinite { } l { sunSky east 0 0 1 turbidity 5 } }")
+
+
+OBJ = b"""# two quads, a triangle; materials by usemtl
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v  0 0 1 1
+vt 0.25 0.5
+vt 0.75
+vn 0 0 2
+vn 1 0 0
+f 1/1/1 2/2/1 3/1/2 4/2/2
+usemtl red
+f 1//2 2//1 5//2
+f 2 3 5
+usemtl blue
+f 3/2 4/1 5/2
+f 1 2 3
+"""
+
+
+def test_wavefront_meshes(tmp_path):
+    """`waveFront "x.obj" materials {..}` (PrimitiveParser.hs:69-72,
+    IO/WaveFront.hs): fans of the faces; the material intervals in corner
+    units except the last, which ends at the triangle count; the third
+    corner's uv and normal taken from the second; points through the
+    transform, normals normalized and not transformed."""
+    pts, nrm, uvs, corners, mtls = B.read_wavefront(OBJ)
+    assert pts.shape == (5, 3) and list(uvs[1]) == [0.75, 1.0]
+    assert np.array_equal(nrm, np.array([[0, 0, 1], [1, 0, 0]], np.float32))
+    assert len(corners) == 3 * 6 and mtls == [("red", 6), ("blue", 12)]
+    assert list(corners[3]) == [0, 0, 0] and list(corners[5]) == [3, 1, 1]  # fan (0, 2, 3) of the quad
+    # intervals: default [0, 6) -> corners 0, 3; red [6, 12) -> 6, 9; blue [12, 6) (cnt = 6) -> none
+    assert B.wavefront_triangles(corners, mtls) == [("default", 0), ("default", 3), ("red", 6), ("red", 9)]
+    (tmp_path / "m.obj").write_bytes(OBJ)
+    text = ("transform { translate 0 0 5 }\n"
+            "material { matte kd { constant rgbR 0.5 0.5 0.5 } sigma { constant 0 } }\n"
+            "prim { waveFront \"m.obj\" materials { \"red\" { matte kd { constant rgbR 0.9 0.1 0.1 } "
+            "sigma { constant 0 } } } }")
+    b = B.parse(text, base=str(tmp_path))
+    fl = b.flat
+    assert fl.num_triangles == 4 and fl.desc.num_meshes == 3
+    v = fl.vertices.reshape(-1, 3, 3)
+    assert np.array_equal(v[0], np.array([[0, 0, 5], [1, 0, 5], [1, 1, 5]], np.float32))
+    # mesh 0: both default triangles with uvs and normals; the third corner repeats the second's
+    assert list(fl.tri_mesh) == [0, 0, 1, 2]
+    n = fl.normals.reshape(-1, 3, 3)
+    assert np.array_equal(n[0], np.array([[0, 0, 1], [0, 0, 1], [0, 0, 1]], np.float32))
+    assert np.array_equal(n[1], np.array([[0, 0, 1], [1, 0, 0], [1, 0, 0]], np.float32))
+    assert np.array_equal(n[2], np.array([[1, 0, 0], [0, 0, 1], [0, 0, 1]], np.float32))
+    u = fl.uvs.reshape(-1, 3, 2)
+    assert np.array_equal(u[0], np.array([[0.25, 0.5], [0.75, 1.0], [0.75, 1.0]], np.float32))
+    m = [fl.meshes[k] for k in range(fl.desc.num_meshes)]
+    assert (m[0].flags, m[1].flags, m[2].flags) == (abi.MESH_NORMALS | abi.MESH_UVS, abi.MESH_NORMALS, 0)
+    red = fl.materials[m[1].material]
+    assert red.spec[0][0] == rgb_refl(0.9, 0.1, 0.1)[0] and m[2].material == m[1].material != m[0].material
+    for bad in (b"v 1 2\n", b"v 1 2 3", b"f -1 2 3\n", b"vn 1e-3 0 1\n"):
+        (tmp_path / "bad.obj").write_bytes(bad)
+        with pytest.raises(B.BlingSyntaxError):
+            B.parse("prim { waveFront \"bad.obj\" materials { } }", base=str(tmp_path))
