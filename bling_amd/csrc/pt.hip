@@ -585,7 +585,7 @@ __global__ __launch_bounds__(BLOCK) void k_texture(PtKernelArgs a) {
     }
     DMaterial dm;
     build_material(in, dm);
-    if ((src.slots >> 8) & 1) {  // bump d dgg dgs (Reflection.hs:347-383)
+    if ((src.slots >> 8) & 1) {  // bump d dgg dgs (Reflection.hs:347-378)
       const float du = 0.01f, dv = 0.01f;
       const float uDisp =
           tx::eval_scalar(T, src.lo[8], src.roots[8], tx::TexPt{add3(dgs.p, scl3(dgs.dpdu, du)), dgs.u + du, dgs.v});

@@ -59,7 +59,7 @@ newGpuScene = newIORef emptyGpuScene
 -- | @gpu sampled { sampler { stratified nu nv } integrator { path maxDepth md
 -- sampleDepth sd } }@ or with @integrator { directLighting maxDepth md }@:
 -- the syntax of the sampler renderer (IO/RendererParser.hs:56-73,
--- IO/IntegratorParser.hs:15-50); the GPU path samples stratified only.
+-- IO/IntegratorParser.hs:15-47); the GPU path samples stratified only.
 pGpuRenderer :: IORef GpuSceneBuilder -> JobParser AnyRenderer
 pGpuRenderer ref = (flip namedBlock) "sampled" $ do
    (nu, nv) <- (flip namedBlock) "sampler" $ pString >>= \t -> case t of
@@ -179,7 +179,7 @@ captureTexture ref t = liftIO $ atomicModifyIORef' ref (\b -> let (i, b') = emit
 -- | The material Texture whose spectrum slots k (0..2) and scalar slots k
 -- (0..3) take the textures rooted at the given node ids, bump-mapped by the
 -- scalar texture rooted at the last argument's id when there is one
--- (bumpMapped, Reflection.hs:344-383; the case "bumpMap" of pMaterial',
+-- (bumpMapped, Reflection.hs:344-378; the case "bumpMap" of pMaterial',
 -- IO/MaterialParser.hs:44-48).  bgpu_texture of kind BGPU_TEX_NODES: the
 -- roots share the params' storage.
 nodeTexture :: [(Int, Int32)] -> [(Int, Int32)] -> Maybe Int32 -> Texture

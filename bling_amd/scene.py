@@ -171,7 +171,7 @@ class Material:
         evaluated per hit): `spec` maps spectrum slots (0..2) and `scalar`
         scalar slots (0..3) to textures; the other inputs stay constant.
         `bump`: a scalar texture the material is bump-mapped by
-        (bumpMapped, Reflection.hs:344-383)."""
+        (bumpMapped, Reflection.hs:344-378)."""
         spec, scalar = dict(spec or {}), dict(scalar or {})
         if not spec and not scalar and bump is None:
             return self

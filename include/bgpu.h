@@ -116,7 +116,7 @@ enum { BGPU_MAP_UV = 0, BGPU_MAP_PLANAR = 1, BGPU_MAP_IDENTITY = 2 };
  * the spectrum texture rooted at roots[k], bit 4 + k (k < 4) makes scalar[k]
  * the scalar texture rooted at roots[4 + k]; the others stay constant.  Bit
  * 8 bump-maps the material by the scalar texture rooted at roots[8]
- * (bumpMapped, Reflection.hs:344-383). */
+ * (bumpMapped, Reflection.hs:344-378). */
 typedef struct {
   int32_t kind;
   int32_t mapping;

@@ -1094,7 +1094,7 @@ static bool texSelectsTex2(const bgpu_texture& t, const DG& dg) {
 
 static void addComponents(const bgpu_material& m, const Spec& s0, const Spec& s1, Bsdf& b);
 
-// bump d dgg dgs (Reflection.hs:347-383): the shading DG displaced along its
+// bump d dgg dgs (Reflection.hs:347-378): the shading DG displaced along its
 // normal by the scalar texture rooted at `root`, sampled at dgs and at dgs
 // shifted by 0.01 in u and in v; the new normal faces the geometric one.
 static DG bumpDG(const Ctx& c, int root, const DG& dgg, const DG& dgs) {

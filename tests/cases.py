@@ -277,7 +277,7 @@ def node_texture_scene(res=48):
 
 
 def bump_scene(res=48):
-    """Bump-mapped materials (bumpMapped, Reflection.hs:344-383): a plastic
+    """Bump-mapped materials (bumpMapped, Reflection.hs:344-378): a plastic
     sphere under Euclidean cell noise, a matte floor quad under fBm, a mesh
     with uvs under a uv-mapped quasi-crystal and a checker-textured matte
     under Perlin noise, lit by a quad light and a uniform sky."""

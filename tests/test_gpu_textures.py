@@ -10,7 +10,7 @@ Texture.hs:59-414.
   3d mappings, images, textured scalars) tracks the oracle with the same
   seed (equal sample weights, median relative pixel difference < 1e-4) and
   is within 3 sigma of it with another seed; so does a scene of bump-mapped
-  materials (bumpMapped, Reflection.hs:344-383);
+  materials (bumpMapped, Reflection.hs:344-378);
 - the served path (continuous batching, textured and untextured jobs in
   flight together) gives the plain path's image;
 - evaluating a node that is not a texture (out of range, a gradient step)

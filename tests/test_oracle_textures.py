@@ -204,7 +204,7 @@ def bump_frame(disp, g, ng):
 
 
 def pin_bump():
-    """bump d dgg dgs (Reflection.hs:347-383) on a plane z = 0 with dpdu =
+    """bump d dgg dgs (Reflection.hs:347-378) on a plane z = 0 with dpdu =
     (2, 0, 0), dpdv = (0, 3, 0): a constant displacement leaves the frame
     alone; (cos x + 1) / 2 over x = u (uv mapping) or x = p.x (planar
     mapping, where the u-shifted point moves by 0.01 dpdu = 0.02 in x) tilts
