@@ -15,7 +15,8 @@
 --     parsers whose result is not a constant (IO/MaterialParser.hs:106-228)
 --     with the material then taking 'nodeTexture' of the roots,
 --     'captureMesh' beside mkTriangleMesh and
---     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureShape' beside
+--     mkHeightMap (IO/PrimitiveParser.hs:28-139), 'captureTriangle' beside
+--     mkWFTri (IO/WaveFront.hs:101-116), 'captureShape' beside
 --     mkGeom, 'captureInfinite' (or 'captureSunSky' for a @sunSky@ map,
 --     IO/MaterialParser.hs:266-270), 'capturePoint' and 'captureDirectional'
 --     beside mkInfiniteAreaLight, mkPointLight and mkDirectional
@@ -28,7 +29,7 @@
 -- behind its export lists: 'spectrumBands' and 'transformRows' are the two
 -- accessors a maintainer exports (Spectrum.hs:46, Transform.hs:123-127).
 module Graphics.Bling.IO.GpuParser (
-   newGpuScene, pGpuRenderer, GpuMaterial(..), captureMaterial, captureMesh, captureShape,
+   newGpuScene, pGpuRenderer, GpuMaterial(..), captureMaterial, captureMesh, captureTriangle, captureShape,
    captureInfinite, captureSunSky, capturePoint, captureDirectional, captureView, GpuTex(..), GpuMap2d(..), captureTexture,
    nodeTexture
    ) where
@@ -200,6 +201,22 @@ captureMesh ref o2w mat ps is ns uvs = liftIO $ modifyIORef' ref (gpuAddMesh mat
    iv = VS.fromList $ map fromIntegral (V.toList is)
    nv = fmap (\n -> VS.fromList $ concatMap (xyz . transNormal o2w) (V.toList n)) ns
    uvv = fmap (\u -> VS.fromList $ concatMap (\(a, b) -> [a, b]) (V.toList u)) uvs
+
+-- | One triangle of parseWaveFront (IO/WaveFront.hs:101-116), beside
+-- mkWFTri: its points as wfTriVerts gives them (already transformed), the
+-- uvs of wfTriUVs when all three uv indices are set, and wfTriNormals as
+-- given.  Those functions' own reading of the corners (the third corner's
+-- uv and normal taken from the second) thus carries over unchanged; the
+-- triangle shades with an identity transform (Primitive.hs:65), so nothing
+-- here is transformed.
+captureTriangle :: IORef GpuSceneBuilder -> Int32 -> (Point, Point, Point)
+                -> Maybe (Float, Float, Float, Float, Float, Float) -> Maybe (Normal, Normal, Normal) -> JobParser ()
+captureTriangle ref mat (p1, p2, p3) uvs ns = liftIO $ modifyIORef' ref (gpuAddMesh mat pv iv nv uvv) where
+   xyz v = [vx v, vy v, vz v]
+   pv = VS.fromList $ concatMap xyz [p1, p2, p3]
+   iv = VS.fromList [0, 1, 2]
+   nv = fmap (\(n1, n2, n3) -> VS.fromList $ concatMap xyz [n1, n2, n3]) ns
+   uvv = fmap (\(u1, v1, u2, v2, u3, v3) -> VS.fromList [u1, v1, u2, v2, u3, v3]) uvs
 
 -- | mkGeom o2w _ mat emission shape gid (Primitive/Geometry.hs:14-37).
 captureShape :: IORef GpuSceneBuilder -> Transform -> Int32 -> Maybe Spectrum -> S.Shape -> JobParser ()
