@@ -56,10 +56,16 @@ def test_cornell_independent_seeds_3sigma(gpu_ctx):
     assert abs(zmean) < 0.05, zmean
 
 
-def test_served_jobs_match_plain(gpu_ctx):
+@pytest.mark.parametrize("mix", ["cornell", "with_sky"])
+def test_served_jobs_match_plain(gpu_ctx, mix):
     """Each job's result is the same alone (plain path) and in the mix
-    (served path, several jobs and scenes in flight)."""
+    (served path, several jobs and scenes in flight).  With a sun/sky scene
+    resident, every job in flight runs the kernels that evaluate the sky,
+    the Cornell boxes' included."""
+    from tests import cases
     flats = [scenes.cornell(48), scenes.cornell(32, light_radiance=8.0)]
+    if mix == "with_sky":
+        flats.append(cases.sun_sky_scene(False, res=32))
     specs = [JobSpec(spp_nu=2, spp_nv=2, max_depth=4, sample_depth=3, passes=2, seed=s, variance=True)
              for s in (1, 2, 3)]
     gss = [gpu_ctx.scene(f) for f in flats]
