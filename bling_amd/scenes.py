@@ -323,12 +323,15 @@ def torus(major, minor, nu, nv):
 
 def sky_gradient(w=512, h=256):
     """A w x h spectral sky map: a bluish zenith over a pale horizon and a
-    dark ground, rows from the +z pole (v = 0) down."""
+    dark ground.  A light map's row y covers v in ((h-1-y)/h, (h-y)/h]
+    (rgbfToTexMap, IO/Bitmap.hs:21-28), so the rows run from the -z pole
+    (v = 1, ground) at row 0 to the +z pole (v = 0, zenith) at row h - 1."""
     zen, hor, gnd = rgb_illum(0.3, 0.5, 1.0), rgb_illum(1.0, 0.95, 0.85), rgb_illum(0.15, 0.12, 0.1)
     zen, hor, gnd = np.asarray(zen, F), np.asarray(hor, F), np.asarray(gnd, F)
     v = (np.arange(h, dtype=np.float64) + 0.5) / h
     rows = np.where((v < 0.5)[:, None], zen[None] + (hor - zen)[None] * (v / 0.5)[:, None] ** 0.7,
                     gnd[None] + (hor - gnd)[None] * np.exp(-(v - 0.5) * 12.0)[:, None])
+    rows = rows[::-1]  # v = (h - 1 - y + 0.5) / h at row y
     return np.ascontiguousarray(np.broadcast_to(rows[:, None, :], (h, w, 16)).astype(F))
 
 
