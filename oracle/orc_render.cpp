@@ -11,6 +11,8 @@
 //   Texture.hs:191-219 (checkerBoard, graphPaper selectors)
 //   Material.hs:111-127 (mkSubstrate)        Reflection/Microfacet.hs:56-104,136-196
 //                                            (FresnelBlend, the anisotropic distribution)
+//   SunSky.hs:12-158 (the sun/sky map)       Spectrum.hs:143-159,229-251 (rgbToSpectrum,
+//                                            chromaticityToXYZ)
 //
 // Pins (tests/test_oracle_pins.py):
 //   path integrator, NEE + MIS   rho L under a uniform sky; rho Le (R/D)^2 below a
@@ -27,9 +29,17 @@
 //                                pdf and sampler in closed form; sampled moments
 //   FresnelBlend                 sample = eval / pdf at its own pdf; the coating's
 //                                absorption is Beer-Lambert
+// Pins (tests/test_oracle_sky.py):
+//   sun/sky light map (SunSky.hs)   the zenith with the sun on the horizon; the
+//                                black lower half; the sun disk's centre and
+//                                bound; sunR black below the horizon and falling
+//                                with turbidity and the sun's height
+//   texel light maps             rgbfToTexMap's lookup (IO/Bitmap.hs:21-28), in
+//                                tests/test_oracle_closed_forms.py
 // Parity unpinned: the bits of glibc sinf/cosf/acosf/atan2f/powf/expf as GHC
-// calls them (they move sampled directions, compared statistically), and
-// FresnelBlend's diffuse and specular factors (the reference's own forms).
+// calls them (they move sampled directions, compared statistically),
+// FresnelBlend's diffuse and specular factors (the reference's own forms),
+// and sunR's per-band values beyond its orderings.
 #include <array>
 #include <atomic>
 #include <cmath>
